@@ -1,0 +1,181 @@
+/*
+ * C ABI of the MI355X (gfx950) screened Poisson reconstruction library
+ * (libmpr_hip.so). Plain pointers and sizes only; no exceptions cross this boundary.
+ *
+ * Two groups of entry points:
+ *
+ * 1. Drop-in kernel ABI. The same symbol names, argument lists, argument meaning and
+ *    return values as the reference's ctypes table (python/mojo_poissonrecon/_lib.py:20-32),
+ *    operating on caller-owned HOST buffers (C-contiguous float64/int64, grids (n,n,n)
+ *    row-major [z][y][x]). Each call copies its inputs to the GPU, runs the HIP kernels and
+ *    copies the results back before returning. The reference passes buffers as int64
+ *    addresses; these take the same addresses typed as pointers. Like the reference they
+ *    cannot raise: on a device error they print to stderr, record mpr_last_error() and
+ *    return -1 (integer results) or NaN (float results).
+ *
+ * 2. Device-resident pipeline ("jobs"). A job owns every device buffer of one
+ *    reconstruction at a fixed depth and point capacity; points are uploaded once and the
+ *    whole solve (normalise, splat, assemble, multigrid, iso value, extraction) runs on the
+ *    job's stream without host round trips. These return 0 on success and a negative
+ *    status on failure, with the message in mpr_last_error().
+ */
+#ifndef MPR_HIP_H
+#define MPR_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- 1. drop-in kernel ABI ------------------------------------------------------------ */
+
+/* Replaces mpr_parallel_init (src/capi.mojo:36-38; _lib.py:21,85): selects and initialises
+ * the GPU; returns 0, or -1 if no usable device is present. */
+int64_t mpr_parallel_init(void);
+
+/* Replaces mpr_octree_codes_f64 (src/capi.mojo:62-99; _lib.py:22): codes (P, depth+1). */
+void mpr_octree_codes_f64(const double *points, int64_t point_count, int64_t depth,
+                          int64_t *codes);
+
+/* Replaces mpr_splat_f64 (src/capi.mojo:102-224; _lib.py:23): accumulates into the five
+ * caller-zeroed grids; returns the number of accepted samples. */
+int64_t mpr_splat_f64(const double *points, const double *normals, int64_t point_count,
+                      int64_t n, int64_t confidence, double *density, double *vx, double *vy,
+                      double *vz, double *screen, double samples_per_node);
+
+/* Replaces mpr_assemble_f64 (src/capi.mojo:227-279; _lib.py:24). */
+void mpr_assemble_f64(const double *vx, const double *vy, const double *vz,
+                      const double *screen, double *rhs, int64_t n, double point_weight,
+                      double target_value);
+
+/* Replaces mpr_relax_f64 (src/capi.mojo:309-410; _lib.py:25): in place. */
+void mpr_relax_f64(double *solution, const double *rhs, const double *screen, int64_t n,
+                   double point_weight, int64_t iterations);
+
+/* Replaces mpr_residual_f64 (src/capi.mojo:473-553; _lib.py:26): returns ||r||_2. */
+double mpr_residual_f64(const double *solution, const double *rhs, const double *screen,
+                        double *residual, int64_t n, double point_weight);
+
+/* Replaces mpr_restrict_f64 (src/capi.mojo:556-673; _lib.py:27). */
+void mpr_restrict_f64(const double *fine, double *coarse, int64_t fine_n);
+
+/* Replaces mpr_prolong_add_f64 (src/capi.mojo:676-742; _lib.py:28): fine += P coarse. */
+void mpr_prolong_add_f64(const double *coarse, double *fine, int64_t fine_n);
+
+/* Replaces mpr_iso_value_f64 (src/capi.mojo:745-795; _lib.py:29). */
+double mpr_iso_value_f64(const double *field, const double *points, const double *normals,
+                         int64_t point_count, int64_t n, int64_t confidence);
+
+/* Replaces mpr_dual_vertices_f64 (src/capi.mojo:839-1041; _lib.py:30). capacity <= 0:
+ * count mode, writes ceil((n-1)/4) partial counts into cell_map. */
+int64_t mpr_dual_vertices_f64(const double *field, const double *density, int64_t n, double iso,
+                              int64_t *cell_map, double *vertices, double *vertex_density,
+                              int64_t capacity);
+
+/* Replaces mpr_dual_faces_f64 (src/capi.mojo:1075-1129; _lib.py:31). */
+int64_t mpr_dual_faces_f64(const double *field, int64_t n, double iso, const int64_t *cell_map,
+                           int64_t *faces, int64_t capacity);
+
+/* ---- status ---------------------------------------------------------------------------- */
+
+/* Message of the last failure on the calling thread ("" if none). No reference
+ * counterpart: the reference's native side cannot fail (MOJO_NOTES.md:41-42). */
+const char *mpr_last_error(void);
+
+/* Device name, total HBM bytes and compute-unit count of the current GPU. */
+int mpr_device_info(char *name, int64_t name_len, int64_t *total_bytes, int64_t *cu_count);
+
+/* ---- 2. device-resident pipeline --------------------------------------------------------
+ * The stages replace core.py:196-306 (solve_screened_poisson), core.py:309-374
+ * (extract_isosurface) and core.py:388-426 (reconstruct) as one device pass. */
+
+enum {
+  MPR_PRECISION_FP64 = 0,  /* every stage in float64, as the reference */
+  MPR_PRECISION_MIXED = 1  /* float32 V-cycles inside a float64 defect-correction loop */
+};
+
+enum {
+  MPR_STAGE_NORMALIZE = 1,  /* points are world space: bbox normalise on device (core.py:377-385) */
+  MPR_STAGE_SOLVE = 2,      /* splat, assemble, multigrid, residual, iso value */
+  MPR_STAGE_EXTRACT = 4,    /* dual-grid mesh of the solution at the iso value */
+  MPR_STAGE_GRIDS = 8,      /* also keep the natural-layout rhs for mpr_job_fetch_grids */
+  MPR_STAGE_WORLD = 16      /* write mesh vertices in world space (core.py:425) */
+};
+
+typedef struct mpr_params {
+  double point_weight;     /* alpha, default 4.0 */
+  double samples_per_node; /* default 1.5 */
+  double scale;            /* bbox scale for MPR_STAGE_NORMALIZE, default 1.1 */
+  int32_t cycles;          /* V-cycles, default 4 */
+  int32_t confidence;      /* 1: weight samples by normal length */
+  int32_t pre_smooth;      /* default 3 */
+  int32_t post_smooth;     /* default 3 */
+  int32_t stages;          /* MPR_STAGE_* bits */
+  int32_t reserved;
+} mpr_params;
+
+typedef struct mpr_result {
+  int64_t accepted;       /* splat-accepted samples */
+  int64_t vertex_count;
+  int64_t face_count;
+  double iso_value;
+  double residual_norm;   /* float64 ||b - A x|| on the finest grid */
+  double origin[3];       /* normalisation transform (MPR_STAGE_NORMALIZE) */
+  double width;
+} mpr_result;
+
+typedef struct mpr_job mpr_job;
+
+/* Device bytes a job of this shape holds (used for admission by HBM footprint). */
+int64_t mpr_job_footprint(int32_t depth, int64_t max_points, int32_t precision);
+int mpr_job_create(int32_t depth, int64_t max_points, int32_t precision, mpr_job **out);
+int mpr_job_destroy(mpr_job *job);
+/* Copies P oriented points (host, (P,3) float64 each) to the job; asynchronous w.r.t. host
+ * only in that it is ordered on the job's stream. */
+int mpr_job_set_points(mpr_job *job, const double *points, const double *normals, int64_t P);
+/* Enqueues the requested stages on the job's stream and returns without waiting. */
+int mpr_job_run(mpr_job *job, const mpr_params *params);
+/* Waits for the job, resolves extraction capacity overflow (re-emitting into larger
+ * buffers) and returns the scalar results. */
+int mpr_job_wait(mpr_job *job, mpr_result *result);
+/* Non-blocking completion test: 1 done, 0 running, <0 error. */
+int mpr_job_query(mpr_job *job);
+int mpr_job_fetch_mesh(mpr_job *job, double *vertices, double *vertex_density, int64_t *faces);
+/* Any pointer may be NULL. rhs requires MPR_STAGE_GRIDS on the run. */
+int mpr_job_fetch_grids(mpr_job *job, double *field, double *density, double *screen,
+                        double *rhs);
+/* Breadth-first octree of the job's (unit-cube) points (core.py:86-127); synchronises and
+ * returns the node count; then fetch the six arrays (point_leaf has P entries). */
+int mpr_job_build_octree(mpr_job *job, int32_t depth, int64_t *node_count);
+int mpr_job_fetch_octree(mpr_job *job, int64_t *depth, int64_t *morton, int64_t *parent,
+                         int64_t *child_start, int64_t *child_count, int64_t *point_leaf);
+/* Records the job's run (for fixed params and point count) into a HIP graph and replays
+ * it; mpr_job_launch replays without re-recording. */
+int mpr_job_capture(mpr_job *job, const mpr_params *params);
+int mpr_job_launch(mpr_job *job);
+/* Device pointers of the job's input buffers (for callers whose points are already in HBM). */
+int mpr_job_device_points(mpr_job *job, double **points, double **normals);
+int mpr_job_set_count(mpr_job *job, int64_t P);
+
+/* Kernel timing on the job's own stream with HIP events: launches kernel `kernel` `reps`
+ * times on the job's finest level (after a run) and returns the mean milliseconds per
+ * launch and the algorithmic HBM bytes one launch moves. Kernels: 0 FP64 red-black colour
+ * sweep, 1 FP32 colour sweep (mixed jobs), 2 FP64 residual, 3 FP64 restriction,
+ * 4 FP64 prolongation, 5 FP32 residual, 6 FP64->FP32 defect residual. No reference
+ * counterpart (measurement hook). */
+int mpr_job_time_kernel(mpr_job *job, int32_t kernel, int32_t reps, double *ms_per_launch,
+                        double *bytes_per_launch);
+
+/* Extraction of a caller grid (core.py:309-374): field and density host (n,n,n) float64
+ * (density may be NULL); returns counts; then fetch and destroy. */
+typedef struct mpr_mesh mpr_mesh;
+int mpr_extract(const double *field, const double *density, int64_t n, double iso,
+                mpr_mesh **out, int64_t *vertex_count, int64_t *face_count);
+int mpr_mesh_fetch(mpr_mesh *mesh, double *vertices, double *vertex_density, int64_t *faces);
+int mpr_mesh_destroy(mpr_mesh *mesh);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

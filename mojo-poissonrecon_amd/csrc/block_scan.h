@@ -1,0 +1,40 @@
+// Workgroup-level helpers for order-preserving compaction on wave64 hardware.
+#pragma once
+
+#include "mpr_common.h"
+
+namespace mpr {
+
+// Exclusive prefix of `flag` over the 256 threads of a block in thread order, using one
+// ballot per wavefront and the 4 wave totals in LDS. `total` receives the block's count.
+// Every thread of the block must call it (it synchronises twice).
+__device__ __forceinline__ uint32_t block_prefix256(bool flag, uint32_t* lds4, uint32_t& total) {
+  const uint64_t bal = __ballot(flag);
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t in_wave = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+  if (lane == 0) lds4[wave] = (uint32_t)__popcll(bal);
+  __syncthreads();
+  uint32_t base = 0, tot = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < 4; ++w) {
+    const uint32_t c = lds4[w];
+    base += w < wave ? c : 0u;
+    tot += c;
+  }
+  __syncthreads();
+  total = tot;
+  return base + in_wave;
+}
+
+// Sum of a per-thread count over a 256-thread block (result valid in every thread).
+__device__ __forceinline__ uint32_t block_count256(uint32_t v, uint32_t* lds4) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  if (lane == 0) lds4[wave] = v;
+  __syncthreads();
+  const uint32_t tot = lds4[0] + lds4[1] + lds4[2] + lds4[3];
+  __syncthreads();
+  return tot;
+}
+
+}  // namespace mpr

@@ -1,0 +1,364 @@
+// Dual-grid (surface-nets) iso-surface extraction for gfx950 (capi.mojo:798-1129).
+//
+// The reference assigns vertex indices in raster order with a serial loop and emits faces
+// in three serial edge passes. Here both become order-preserving stream compactions:
+// a count kernel (one 4096-item tile per workgroup, wave64 ballot + popcount), a one-
+// workgroup exclusive scan of the tile counts, and an emit kernel that re-derives each
+// flag and its rank inside the tile with ballots, so indices and face order are exactly
+// the reference's.
+#include <algorithm>
+
+#include "block_scan.h"
+#include "mpr_internal.h"
+
+namespace mpr {
+
+namespace {
+
+constexpr int kTileItems = 16;
+constexpr uint32_t kTile = kThreads * kTileItems;
+
+__device__ __forceinline__ bool cell_active(const double* f, int x, int y, int z, int n,
+                                            double iso) {
+  int below = 0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+    below += f[nat_index(x + (c & 1), y + ((c >> 1) & 1), z + ((c >> 2) & 1), n)] < iso;
+  return below != 0 && below != 8;
+}
+
+__device__ __forceinline__ void cell_decode(uint32_t i, uint32_t cells, const FastDiv& dc, int& x,
+                                            int& y, int& z) {
+  const uint32_t q = dc.div(i);
+  x = (int)(i - q * cells);
+  const uint32_t zz = dc.div(q);
+  y = (int)(q - zz * cells);
+  z = (int)zz;
+}
+
+__global__ __launch_bounds__(kThreads) void k_cells_count(const double* f, int n, const double* isop,
+                                                          FastDiv dc, uint32_t* counts) {
+  __shared__ uint32_t lds4[4];
+  const double iso = *isop;
+  const uint32_t cells = (uint32_t)(n - 1), total = cells * cells * cells;
+  uint32_t c = 0;
+  for (int r = 0; r < kTileItems; ++r) {
+    const uint32_t i = blockIdx.x * kTile + r * kThreads + threadIdx.x;
+    if (i < total) {
+      int x, y, z;
+      cell_decode(i, cells, dc, x, y, z);
+      c += cell_active(f, x, y, z, n, iso);
+    }
+  }
+  c = block_count256(c, lds4);
+  if (threadIdx.x == 0) counts[blockIdx.x] = c;
+}
+
+// count mode of the compatibility ABI: one partial count per 4-plane z chunk
+__global__ __launch_bounds__(kThreads) void k_cells_chunk_counts(const double* f, int n,
+                                                                 const double* isop, FastDiv dc,
+                                                                 unsigned long long* chunks) {
+  const double iso = *isop;
+  const uint32_t cells = (uint32_t)(n - 1);
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= cells * cells * cells) return;
+  int x, y, z;
+  cell_decode(i, cells, dc, x, y, z);
+  if (cell_active(f, x, y, z, n, iso)) atomicAdd(&chunks[z >> 2], 1ull);
+}
+
+// vertex of an active cell: mean of the edge roots in x-, y-, z-axis order (capi.mojo:958-1038)
+__device__ __forceinline__ void dual_vertex(const double* f, const double* dens, int x, int y,
+                                            int z, int n, double iso, double out[3],
+                                            double& vdens) {
+  double v[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+    v[c] = f[nat_index(x + (c & 1), y + ((c >> 1) & 1), z + ((c >> 2) & 1), n)];
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  int roots = 0;
+#pragma unroll
+  for (int axis = 0; axis < 3; ++axis)
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        int c0, c1;
+        double x0, y0, z0, x1, y1, z1;
+        if (axis == 0) {
+          c0 = 2 * a + 4 * b; c1 = c0 + 1;
+          x0 = 0.0; y0 = (double)a; z0 = (double)b; x1 = 1.0; y1 = y0; z1 = z0;
+        } else if (axis == 1) {
+          c0 = a + 4 * b; c1 = c0 + 2;
+          x0 = (double)a; y0 = 0.0; z0 = (double)b; x1 = x0; y1 = 1.0; z1 = z0;
+        } else {
+          c0 = a + 2 * b; c1 = c0 + 4;
+          x0 = (double)a; y0 = (double)b; z0 = 0.0; x1 = x0; y1 = y0; z1 = 1.0;
+        }
+        const double v0 = v[c0], v1 = v[c1];
+        if ((v0 < iso) == (v1 < iso)) continue;
+        double t = (iso - v0) / (v1 - v0);
+        t = t < 0.0 ? 0.0 : t;
+        t = t > 1.0 ? 1.0 : t;
+        sx = sx + x0 + (x1 - x0) * t;
+        sy = sy + y0 + (y1 - y0) * t;
+        sz = sz + z0 + (z1 - z0) * t;
+        roots += 1;
+      }
+  const double inv = 1.0 / (double)roots;
+  const double cells = (double)(n - 1);
+  out[0] = ((double)x + sx * inv) / cells;
+  out[1] = ((double)y + sy * inv) / cells;
+  out[2] = ((double)z + sz * inv) / cells;
+  double ds = 0.0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+    ds += dens[nat_index(x + (c & 1), y + ((c >> 1) & 1), z + ((c >> 2) & 1), n)];
+  vdens = ds * 0.125;
+}
+
+// Fill mode: active cells get consecutive raster-order indices. Cells whose index reaches
+// `capacity` keep the mark 0 and emit nothing (capi.mojo:905-963). If `xform` is non-null
+// ([origin x, y, z, width]) vertices are written as v * width + origin (core.py:425).
+template <class CM>
+__global__ __launch_bounds__(kThreads) void k_cells_emit(const double* f, const double* dens,
+                                                         int n, const double* isop,
+                                                         FastDiv dc,
+                                                         const uint32_t* tile_offsets,
+                                                         CM* cell_map, double* verts,
+                                                         double* vdens, uint64_t capacity,
+                                                         const double* xform) {
+  __shared__ uint32_t lds4[4];
+  const double iso = *isop;
+  const uint32_t cells = (uint32_t)(n - 1), total = cells * cells * cells;
+  uint32_t running = tile_offsets[blockIdx.x];
+  for (int r = 0; r < kTileItems; ++r) {
+    const uint32_t i = blockIdx.x * kTile + r * kThreads + threadIdx.x;
+    int x = 0, y = 0, z = 0;
+    bool act = false;
+    if (i < total) {
+      cell_decode(i, cells, dc, x, y, z);
+      act = cell_active(f, x, y, z, n, iso);
+    }
+    uint32_t cnt;
+    const uint32_t pre = block_prefix256(act, lds4, cnt);
+    if (i < total) {
+      if (act) {
+        const uint64_t idx = (uint64_t)running + pre;
+        if (idx < capacity) {
+          double v[3], d;
+          dual_vertex(f, dens, x, y, z, n, iso, v, d);
+          if (xform) {
+            for (int a = 0; a < 3; ++a) v[a] = v[a] * xform[3] + xform[a];
+          }
+          verts[3 * idx] = v[0];
+          verts[3 * idx + 1] = v[1];
+          verts[3 * idx + 2] = v[2];
+          vdens[idx] = d;
+          cell_map[i] = (CM)idx;
+        } else {
+          cell_map[i] = (CM)0;
+        }
+      } else {
+        cell_map[i] = (CM)-1;
+      }
+    }
+    running += cnt;
+  }
+}
+
+struct EdgeGeom {
+  uint32_t cells, per_pass;  // per_pass = (cells-1)^2 * cells
+  FastDiv d_pass, d_cells, d_cm1;
+};
+
+// Decodes edge item e of the three passes (capi.mojo:1090-1128) into the edge's lower node
+// and the four incident cells (a, b, c, d) in the reference's quad order.
+__device__ __forceinline__ void edge_decode(uint32_t e, const EdgeGeom& G, int& axis, int& x,
+                                            int& y, int& z, int q[4][3]) {
+  const uint32_t cells = G.cells, cm1 = cells - 1;
+  axis = (int)G.d_pass.div(e);
+  const uint32_t j = e - (uint32_t)axis * G.per_pass;
+  if (axis == 0) {
+    const uint32_t r = G.d_cells.div(j);
+    x = (int)(j - r * cells);
+    const uint32_t zz = G.d_cm1.div(r);
+    y = 1 + (int)(r - zz * cm1);
+    z = 1 + (int)zz;
+    const int t[4][3] = {{x, y - 1, z - 1}, {x, y, z - 1}, {x, y, z}, {x, y - 1, z}};
+    for (int k = 0; k < 4; ++k) for (int a = 0; a < 3; ++a) q[k][a] = t[k][a];
+  } else if (axis == 1) {
+    const uint32_t r = G.d_cm1.div(j);
+    x = 1 + (int)(j - r * cm1);
+    const uint32_t zz = G.d_cells.div(r);
+    y = (int)(r - zz * cells);
+    z = 1 + (int)zz;
+    const int t[4][3] = {{x - 1, y, z - 1}, {x - 1, y, z}, {x, y, z}, {x, y, z - 1}};
+    for (int k = 0; k < 4; ++k) for (int a = 0; a < 3; ++a) q[k][a] = t[k][a];
+  } else {
+    const uint32_t r = G.d_cm1.div(j);
+    x = 1 + (int)(j - r * cm1);
+    const uint32_t zz = G.d_cm1.div(r);
+    y = 1 + (int)(r - zz * cm1);
+    z = (int)zz;
+    const int t[4][3] = {{x - 1, y - 1, z}, {x, y - 1, z}, {x, y, z}, {x - 1, y, z}};
+    for (int k = 0; k < 4; ++k) for (int a = 0; a < 3; ++a) q[k][a] = t[k][a];
+  }
+}
+
+template <class CM>
+__device__ __forceinline__ bool edge_quad(const double* f, int n, double iso, const CM* cm,
+                                          const EdgeGeom& G, uint32_t e, int64_t idx[4],
+                                          bool& reverse) {
+  int axis, x, y, z, q[4][3];
+  edge_decode(e, G, axis, x, y, z, q);
+  const double v0 = f[nat_index(x, y, z, n)];
+  const double v1 = f[nat_index(x + (axis == 0), y + (axis == 1), z + (axis == 2), n)];
+  reverse = v0 < iso;
+  if ((v0 < iso) == (v1 < iso)) return false;
+  const uint32_t cells = G.cells;
+  bool ok = true;
+  for (int k = 0; k < 4; ++k) {
+    idx[k] = (int64_t)cm[((uint64_t)q[k][2] * cells + q[k][1]) * cells + q[k][0]];
+    ok = ok && idx[k] >= 0;
+  }
+  return ok;
+}
+
+template <class CM>
+__global__ __launch_bounds__(kThreads) void k_edges_count(const double* f, int n,
+                                                          const double* isop, const CM* cm,
+                                                          EdgeGeom G, uint32_t* counts) {
+  __shared__ uint32_t lds4[4];
+  const double iso = *isop;
+  const uint32_t total = 3u * G.per_pass;
+  uint32_t c = 0;
+  for (int r = 0; r < kTileItems; ++r) {
+    const uint32_t e = blockIdx.x * kTile + r * kThreads + threadIdx.x;
+    if (e < total) {
+      int64_t idx[4];
+      bool rev;
+      c += edge_quad(f, n, iso, cm, G, e, idx, rev);
+    }
+  }
+  c = block_count256(c, lds4);
+  if (threadIdx.x == 0) counts[blockIdx.x] = c;
+}
+
+// capi.mojo:1044-1072: quad (a,b,c,d) -> [a,b,c],[a,c,d], or [a,d,c],[a,c,b] when v0 < iso
+template <class CM>
+__global__ __launch_bounds__(kThreads) void k_edges_emit(const double* f, int n,
+                                                         const double* isop, const CM* cm,
+                                                         EdgeGeom G,
+                                                         const uint32_t* tile_offsets,
+                                                         int64_t* faces, uint64_t capacity) {
+  __shared__ uint32_t lds4[4];
+  const double iso = *isop;
+  const uint32_t total = 3u * G.per_pass;
+  uint32_t running = tile_offsets[blockIdx.x];
+  for (int r = 0; r < kTileItems; ++r) {
+    const uint32_t e = blockIdx.x * kTile + r * kThreads + threadIdx.x;
+    int64_t q[4] = {0, 0, 0, 0};
+    bool rev = false, on = false;
+    if (e < total) on = edge_quad(f, n, iso, cm, G, e, q, rev);
+    uint32_t cnt;
+    const uint32_t pre = block_prefix256(on, lds4, cnt);
+    if (on) {
+      const uint64_t fc = 2ull * ((uint64_t)running + pre);
+      if (fc + 2 <= capacity) {
+        int64_t* o = faces + 3 * fc;
+        if (rev) {
+          o[0] = q[0]; o[1] = q[3]; o[2] = q[2]; o[3] = q[0]; o[4] = q[2]; o[5] = q[1];
+        } else {
+          o[0] = q[0]; o[1] = q[1]; o[2] = q[2]; o[3] = q[0]; o[4] = q[2]; o[5] = q[3];
+        }
+      }
+    }
+    running += cnt;
+  }
+}
+
+EdgeGeom make_edge_geom(int n) {
+  EdgeGeom G;
+  G.cells = (uint32_t)(n - 1);
+  G.per_pass = (G.cells - 1) * (G.cells - 1) * G.cells;
+  G.d_pass = FastDiv(std::max<uint32_t>(1u, G.per_pass));
+  G.d_cells = FastDiv(std::max<uint32_t>(1u, G.cells));
+  G.d_cm1 = FastDiv(std::max<uint32_t>(1u, G.cells - 1));
+  return G;
+}
+
+}  // namespace
+
+uint32_t cell_tiles(int n) {
+  const uint64_t cells = (uint64_t)(n - 1);
+  return (uint32_t)std::max<uint64_t>(1, (cells * cells * cells + kTile - 1) / kTile);
+}
+
+uint32_t edge_tiles(int n) {
+  const uint64_t cells = (uint64_t)(n - 1);
+  const uint64_t e = cells ? 3ull * (cells - 1) * (cells - 1) * cells : 0;
+  return (uint32_t)std::max<uint64_t>(1, (e + kTile - 1) / kTile);
+}
+
+void launch_cells_count(const double* f, int n, const double* iso, uint32_t* counts,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_cells_count, dim3(cell_tiles(n)), dim3(kThreads), 0, s, f, n, iso,
+                     FastDiv((uint32_t)(n - 1)), counts);
+  MPR_LAUNCH_CHECK();
+}
+
+void launch_cells_chunk_counts(const double* f, int n, const double* iso, int64_t* chunks,
+                               hipStream_t s) {
+  const uint64_t cells = (uint64_t)(n - 1);
+  const uint64_t total = cells * cells * cells;
+  if (!total) return;
+  hipLaunchKernelGGL(k_cells_chunk_counts, dim3(blocks_for(total, kThreads)), dim3(kThreads), 0,
+                     s, f, n, iso, FastDiv((uint32_t)cells), (unsigned long long*)chunks);
+  MPR_LAUNCH_CHECK();
+}
+
+template <class CM>
+void launch_cells_emit(const double* f, const double* dens, int n, const double* iso,
+                       const uint32_t* tile_offsets, CM* cell_map, double* verts, double* vdens,
+                       uint64_t capacity, const double* xform, hipStream_t s) {
+  hipLaunchKernelGGL(k_cells_emit<CM>, dim3(cell_tiles(n)), dim3(kThreads), 0, s, f, dens, n, iso,
+                     FastDiv((uint32_t)(n - 1)), tile_offsets, cell_map, verts, vdens, capacity,
+                     xform);
+  MPR_LAUNCH_CHECK();
+}
+
+template <class CM>
+void launch_edges_count(const double* f, int n, const double* iso, const CM* cm,
+                        uint32_t* counts,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_edges_count<CM>, dim3(edge_tiles(n)), dim3(kThreads), 0, s, f, n, iso, cm,
+                     make_edge_geom(n), counts);
+  MPR_LAUNCH_CHECK();
+}
+
+template <class CM>
+void launch_edges_emit(const double* f, int n, const double* iso, const CM* cm,
+                       const uint32_t* tile_offsets, int64_t* faces, uint64_t capacity,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_edges_emit<CM>, dim3(edge_tiles(n)), dim3(kThreads), 0, s, f, n, iso, cm,
+                     make_edge_geom(n), tile_offsets, faces, capacity);
+  MPR_LAUNCH_CHECK();
+}
+
+template void launch_cells_emit<int32_t>(const double*, const double*, int, const double*, const uint32_t*,
+                                         int32_t*, double*, double*, uint64_t, const double*,
+                                         hipStream_t);
+template void launch_cells_emit<int64_t>(const double*, const double*, int, const double*, const uint32_t*,
+                                         int64_t*, double*, double*, uint64_t, const double*,
+                                         hipStream_t);
+template void launch_edges_count<int32_t>(const double*, int, const double*, const int32_t*, uint32_t*,
+                                          hipStream_t);
+template void launch_edges_count<int64_t>(const double*, int, const double*, const int64_t*, uint32_t*,
+                                          hipStream_t);
+template void launch_edges_emit<int32_t>(const double*, int, const double*, const int32_t*,
+                                         const uint32_t*, int64_t*, uint64_t, hipStream_t);
+template void launch_edges_emit<int64_t>(const double*, int, const double*, const int64_t*,
+                                         const uint32_t*, int64_t*, uint64_t, hipStream_t);
+
+}  // namespace mpr

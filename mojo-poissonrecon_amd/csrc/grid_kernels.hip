@@ -1,0 +1,446 @@
+// Multigrid grid kernels for gfx950: red-black smoothing, residual (+ squared norm),
+// full-weighting restriction, trilinear prolongation, layout conversions and RHS assembly.
+//
+// All of them are HBM-bound streaming stencils (well under 1 flop per byte), so they are
+// written for bandwidth: one node per lane, flattened indices so every wavefront touches
+// contiguous 64-word runs, the colour-split layout so a colour sweep never drags the
+// other colour's words through the cache, and an XCD-aware block order (blocks are
+// dispatched round-robin over the 8 XCDs, so block b is remapped to give each XCD one
+// contiguous slab of planes and keep the z-1/z+1 neighbour planes in that XCD's L2).
+#include <algorithm>
+
+#include "mpr_internal.h"
+#include "stencil.h"
+
+namespace mpr {
+
+namespace {
+
+__device__ __forceinline__ uint32_t logical_block() {
+  const uint32_t per = gridDim.x >> 3;  // launch grids are padded to a multiple of 8
+  return (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+}
+
+inline unsigned xcd_grid(uint64_t work, unsigned threads) {
+  unsigned nb = blocks_for(work, threads);
+  return (nb + 7u) & ~7u;
+}
+
+template <class T>
+__global__ __launch_bounds__(kThreads) void k_fill(T* p, T v, uint64_t count) {
+  uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * kThreads;
+  for (; i < count; i += stride) p[i] = v;
+}
+
+template <class T>
+__global__ __launch_bounds__(kThreads) void k_relax_color(T* X, const T* R, const T* S, Geom g,
+                                                          int c, T pw, T h2) {
+  uint32_t t = logical_block() * kThreads + threadIdx.x;
+  if (t >= g.C) return;
+  int x, y, z;
+  if (!split_decode(g, c, t, x, y, z)) return;
+  relax_node(X, R, S, g, (uint32_t)c * g.C + t, x, y, z, pw, h2);
+}
+
+__device__ __forceinline__ double block_sum(double v, double* lds) {
+  // fixed-order reduction: wavefront butterfly, then the 4 wave totals in order
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) lds[wave] = v;
+  __syncthreads();
+  double total = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kThreads / 64; ++w) total += lds[w];
+  return total;
+}
+
+template <class T, class TR>
+__global__ __launch_bounds__(kThreads) void k_residual(const T* X, const T* R, const T* S,
+                                                       TR* out, Geom g, T pw, T h2,
+                                                       double* partials) {
+  __shared__ double lds[kThreads / 64];
+  const uint32_t lb = logical_block();
+  const uint32_t t = lb * kThreads + threadIdx.x;
+  double sq = 0.0;
+  if (t < 2u * g.C) {
+    const int c = t >= g.C;
+    const uint32_t tc = t - (uint32_t)c * g.C;
+    int x, y, z;
+    if (split_decode(g, c, tc, x, y, z)) {
+      T r = residual_node(X, R, S, g, t, x, y, z, pw, h2);
+      out[t] = (TR)r;
+      sq = (double)r * (double)r;
+    } else {
+      out[t] = TR(0);
+    }
+  }
+  if (partials) {
+    double total = block_sum(sq, lds);
+    if (threadIdx.x == 0 && lb * kThreads < 2u * g.C) partials[lb] = total;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_norm_finish(const double* partials, int count,
+                                                      double* out) {
+  __shared__ double lds[1024];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < count; i += 1024) v += partials[i];
+  lds[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) lds[threadIdx.x] += lds[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sqrt(lds[0]);
+}
+
+template <class T>
+__global__ __launch_bounds__(kThreads) void k_restrict(const T* F, Geom gf, T* Cc, Geom gc,
+                                                       T scale) {
+  const uint32_t t = logical_block() * kThreads + threadIdx.x;
+  if (t >= 2u * gc.C) return;
+  const int c = t >= gc.C;
+  int X, Y, Z;
+  if (!split_decode(gc, c, t - (uint32_t)c * gc.C, X, Y, Z)) return;
+  Cc[t] = restrict_node(F, gf, X, Y, Z, scale);
+}
+
+template <class T>
+__global__ __launch_bounds__(kThreads) void k_prolong_add(const T* Cc, Geom gc, T* F, Geom gf) {
+  const uint32_t t = logical_block() * kThreads + threadIdx.x;
+  if (t >= 2u * gf.C) return;
+  const int c = t >= gf.C;
+  int x, y, z;
+  if (!split_decode(gf, c, t - (uint32_t)c * gf.C, x, y, z)) return;
+  F[t] += prolong_node(Cc, gc, x, y, z);
+}
+
+template <class TI, class TO>
+__global__ __launch_bounds__(kThreads) void k_nat_to_split(const TI* nat, TO* split, Geom g) {
+  const uint32_t t = logical_block() * kThreads + threadIdx.x;
+  if (t >= 2u * g.C) return;
+  const int c = t >= g.C;
+  int x, y, z;
+  if (split_decode(g, c, t - (uint32_t)c * g.C, x, y, z))
+    split[t] = (TO)nat[nat_index(x, y, z, g.n)];
+  else
+    split[t] = TO(0);
+}
+
+template <class T>
+__global__ __launch_bounds__(kThreads) void k_split_to_nat(const T* split, double* nat, Geom g,
+                                                           FastDiv div_n) {
+  const uint32_t t = logical_block() * kThreads + threadIdx.x;
+  const uint32_t n = (uint32_t)g.n;
+  if (t >= n * n * n) return;
+  uint32_t zy = div_n.div(t);
+  uint32_t x = t - zy * n;
+  uint32_t z = div_n.div(zy);
+  uint32_t y = zy - z * n;
+  nat[t] = (double)split[split_index(g, (int)x, (int)y, (int)z)];
+}
+
+template <class TI, class TO>
+__global__ __launch_bounds__(kThreads) void k_convert(const TI* in, TO* out, uint64_t count) {
+  uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * kThreads;
+  for (; i < count; i += stride) out[i] = (TO)in[i];
+}
+
+__global__ __launch_bounds__(kThreads) void k_add_correction(double* X, const float* E,
+                                                             uint64_t count) {
+  uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * kThreads;
+  for (; i < count; i += stride) X[i] += (double)E[i];
+}
+
+__global__ __launch_bounds__(kThreads) void k_assemble(const double* vx, const double* vy,
+                                                       const double* vz, const double* scr,
+                                                       double* rhs_nat, double* rhs_split,
+                                                       double* scr_split, double* density,
+                                                       double spn, Geom g, FastDiv div_n,
+                                                       double pw, double target) {
+  const uint32_t t = logical_block() * kThreads + threadIdx.x;
+  const int n = g.n;
+  if (t >= (uint32_t)n * n * n) return;
+  uint32_t zy = div_n.div(t);
+  const int x = (int)(t - zy * (uint32_t)n);
+  const int z = (int)div_n.div(zy);
+  const int y = (int)(zy - (uint32_t)z * (uint32_t)n);
+  const double h_inv = (double)(n - 1);
+  const double h_inv2 = h_inv * h_inv;
+  const int64_t sy = n, sz = (int64_t)n * n;
+  double div = 0.0;
+  if (x == 0) div += (vx[t + 1] - vx[t]) * h_inv;
+  else if (x == n - 1) div += (vx[t] - vx[t - 1]) * h_inv;
+  else div += (vx[t + 1] - vx[t - 1]) * (0.5 * h_inv);
+  if (y == 0) div += (vy[t + sy] - vy[t]) * h_inv;
+  else if (y == n - 1) div += (vy[t] - vy[t - sy]) * h_inv;
+  else div += (vy[t + sy] - vy[t - sy]) * (0.5 * h_inv);
+  if (z == 0) div += (vz[t + sz] - vz[t]) * h_inv;
+  else if (z == n - 1) div += (vz[t] - vz[t - sz]) * h_inv;
+  else div += (vz[t + sz] - vz[t - sz]) * (0.5 * h_inv);
+  const double s = scr[t];
+  const double screening = pw * s * h_inv2;
+  const double r = -div + screening * target;
+  if (rhs_nat) rhs_nat[t] = r;
+  if (rhs_split) {
+    const uint32_t si = split_index(g, x, y, z);
+    rhs_split[si] = r;
+    scr_split[si] = s;
+  }
+  if (density) density[t] = density[t] / spn;
+}
+
+template <class T>
+__device__ void block_relax(LevelPtrs<T>& L, T pw, int iterations) {
+  const Geom& g = L.g;
+  const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
+  for (int it = 0; it < iterations; ++it)
+    for (int c = 0; c < 2; ++c) {
+      for (uint32_t t = threadIdx.x; t < g.C; t += blockDim.x) {
+        int x, y, z;
+        if (split_decode(g, c, t, x, y, z))
+          relax_node(L.x, L.rhs, L.screen, g, (uint32_t)c * g.C + t, x, y, z, pw, h2);
+      }
+      __syncthreads();
+    }
+}
+
+template <class T>
+struct CoarseArgs {
+  LevelPtrs<T> lv[kMaxCoarseLevels];
+  int count;
+  int zero_top;
+  T pw;
+  int pre, post;
+};
+
+// One workgroup runs the whole V-cycle below (and including) the top level it is given
+// (core.py:141-193 unrolled into a down sweep and an up sweep), with all levels in LDS.
+template <class T>
+__global__ __launch_bounds__(1024) void k_coarse_vcycle(CoarseArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* sm = reinterpret_cast<T*>(smem_raw);
+  LevelPtrs<T> L[kMaxCoarseLevels];
+  size_t off = 0;
+  for (int l = 0; l < a.count; ++l) {
+    const uint32_t m = 2u * a.lv[l].g.C;
+    L[l].g = a.lv[l].g;
+    L[l].x = sm + off; off += m;
+    L[l].rhs = sm + off; off += m;
+    L[l].screen = sm + off; off += m;
+    L[l].res = sm + off; off += m;
+  }
+  const uint32_t m0 = 2u * a.lv[0].g.C;
+  for (uint32_t i = threadIdx.x; i < m0; i += blockDim.x) {
+    L[0].x[i] = a.zero_top ? T(0) : a.lv[0].x[i];
+    L[0].rhs[i] = a.lv[0].rhs[i];
+  }
+  for (int l = 0; l < a.count; ++l) {
+    const uint32_t m = 2u * a.lv[l].g.C;
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) L[l].screen[i] = a.lv[l].screen[i];
+  }
+  __syncthreads();
+  const T pw = a.pw;
+  for (int l = 0; l + 1 < a.count; ++l) {
+    block_relax(L[l], pw, a.pre);
+    const Geom& g = L[l].g;
+    const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
+    for (uint32_t t = threadIdx.x; t < 2u * g.C; t += blockDim.x) {
+      const int c = t >= g.C;
+      int x, y, z;
+      if (split_decode(g, c, t - (uint32_t)c * g.C, x, y, z))
+        L[l].res[t] = residual_node(L[l].x, L[l].rhs, L[l].screen, g, t, x, y, z, pw, h2);
+    }
+    __syncthreads();
+    const Geom& gc = L[l + 1].g;
+    for (uint32_t t = threadIdx.x; t < 2u * gc.C; t += blockDim.x) {
+      const int c = t >= gc.C;
+      int X, Y, Z;
+      if (split_decode(gc, c, t - (uint32_t)c * gc.C, X, Y, Z))
+        L[l + 1].rhs[t] = restrict_node(L[l].res, g, X, Y, Z, T(1));
+      L[l + 1].x[t] = T(0);
+    }
+    __syncthreads();
+  }
+  block_relax(L[a.count - 1], pw, 40);  // core.py:152-156
+  for (int l = a.count - 2; l >= 0; --l) {
+    const Geom& g = L[l].g;
+    for (uint32_t t = threadIdx.x; t < 2u * g.C; t += blockDim.x) {
+      const int c = t >= g.C;
+      int x, y, z;
+      if (split_decode(g, c, t - (uint32_t)c * g.C, x, y, z))
+        L[l].x[t] += prolong_node(L[l + 1].x, L[l + 1].g, x, y, z);
+    }
+    __syncthreads();
+    block_relax(L[l], pw, a.post);
+  }
+  for (uint32_t i = threadIdx.x; i < m0; i += blockDim.x) a.lv[0].x[i] = L[0].x[i];
+}
+
+}  // namespace
+
+template <class T>
+void launch_fill(T* p, T value, uint64_t count, hipStream_t s) {
+  if (!count) return;
+  unsigned nb = (unsigned)std::min<uint64_t>(blocks_for(count, kThreads), 8192);
+  hipLaunchKernelGGL(k_fill<T>, dim3(nb), dim3(kThreads), 0, s, p, value, count);
+  MPR_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_relax_color(T* X, const T* R, const T* S, const Geom& g, double pw, int c,
+                        hipStream_t s) {
+  const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
+  const unsigned nb = xcd_grid(g.C, kThreads);
+  hipLaunchKernelGGL(k_relax_color<T>, dim3(nb), dim3(kThreads), 0, s, X, R, S, g, c, (T)pw, h2);
+  MPR_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_relax(T* X, const T* R, const T* S, const Geom& g, double pw, int iterations,
+                  hipStream_t s) {
+  for (int it = 0; it < iterations; ++it)
+    for (int c = 0; c < 2; ++c) launch_relax_color<T>(X, R, S, g, pw, c, s);
+}
+
+template <class T, class TR>
+int launch_residual(const T* X, const T* R, const T* S, TR* out, const Geom& g, double pw,
+                    double* partials, hipStream_t s) {
+  const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
+  const unsigned nb = xcd_grid(2ull * g.C, kThreads);
+  hipLaunchKernelGGL((k_residual<T, TR>), dim3(nb), dim3(kThreads), 0, s, X, R, S, out, g,
+                     (T)pw, h2, partials);
+  MPR_LAUNCH_CHECK();
+  return (int)blocks_for(2ull * g.C, kThreads);
+}
+
+void launch_norm_finish(const double* partials, int count, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_norm_finish, dim3(1), dim3(1024), 0, s, partials, count, out);
+  MPR_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_restrict(const T* F, const Geom& gf, T* Cc, const Geom& gc, double scale,
+                     hipStream_t s) {
+  const unsigned nb = xcd_grid(2ull * gc.C, kThreads);
+  hipLaunchKernelGGL(k_restrict<T>, dim3(nb), dim3(kThreads), 0, s, F, gf, Cc, gc, (T)scale);
+  MPR_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_prolong_add(const T* Cc, const Geom& gc, T* F, const Geom& gf, hipStream_t s) {
+  const unsigned nb = xcd_grid(2ull * gf.C, kThreads);
+  hipLaunchKernelGGL(k_prolong_add<T>, dim3(nb), dim3(kThreads), 0, s, Cc, gc, F, gf);
+  MPR_LAUNCH_CHECK();
+}
+
+template <class TI, class TO>
+void launch_nat_to_split(const TI* nat, TO* split, const Geom& g, hipStream_t s) {
+  const unsigned nb = xcd_grid(2ull * g.C, kThreads);
+  hipLaunchKernelGGL((k_nat_to_split<TI, TO>), dim3(nb), dim3(kThreads), 0, s, nat, split, g);
+  MPR_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_split_to_nat(const T* split, double* nat, const Geom& g, hipStream_t s) {
+  const uint64_t m = (uint64_t)g.n * g.n * g.n;
+  const unsigned nb = xcd_grid(m, kThreads);
+  hipLaunchKernelGGL(k_split_to_nat<T>, dim3(nb), dim3(kThreads), 0, s, split, nat, g,
+                     FastDiv((uint32_t)g.n));
+  MPR_LAUNCH_CHECK();
+}
+
+template <class TI, class TO>
+void launch_convert(const TI* in, TO* out, uint64_t count, hipStream_t s) {
+  if (!count) return;
+  unsigned nb = (unsigned)std::min<uint64_t>(blocks_for(count, kThreads), 8192);
+  hipLaunchKernelGGL((k_convert<TI, TO>), dim3(nb), dim3(kThreads), 0, s, in, out, count);
+  MPR_LAUNCH_CHECK();
+}
+
+void launch_add_correction(double* X, const float* E, uint64_t count, hipStream_t s) {
+  unsigned nb = (unsigned)std::min<uint64_t>(blocks_for(count, kThreads), 8192);
+  hipLaunchKernelGGL(k_add_correction, dim3(nb), dim3(kThreads), 0, s, X, E, count);
+  MPR_LAUNCH_CHECK();
+}
+
+void launch_assemble(const double* vx, const double* vy, const double* vz,
+                     const double* screen_nat, double* rhs_nat, double* rhs_split,
+                     double* screen_split, double* density, double spn, const Geom& g,
+                     double pw, double target, hipStream_t s) {
+  const uint64_t m = (uint64_t)g.n * g.n * g.n;
+  const unsigned nb = xcd_grid(m, kThreads);
+  hipLaunchKernelGGL(k_assemble, dim3(nb), dim3(kThreads), 0, s, vx, vy, vz, screen_nat,
+                     rhs_nat, rhs_split, screen_split, density, spn, g, FastDiv((uint32_t)g.n),
+                     pw, target);
+  MPR_LAUNCH_CHECK();
+}
+
+template <class T>
+size_t coarse_lds_bytes(const Geom* geoms, int count) {
+  size_t words = 0;
+  for (int l = 0; l < count; ++l) words += 4ull * 2ull * geoms[l].C;
+  return words * sizeof(T);
+}
+
+template <class T>
+bool launch_coarse_vcycle(const LevelPtrs<T>* levels, int count, bool zero_top, double pw,
+                          int pre, int post, hipStream_t s) {
+  if (count < 1 || count > kMaxCoarseLevels) return false;
+  Geom geoms[kMaxCoarseLevels];
+  for (int l = 0; l < count; ++l) geoms[l] = levels[l].g;
+  const size_t bytes = coarse_lds_bytes<T>(geoms, count);
+  if (bytes > 160 * 1024) return false;
+  static bool attr_set = false;
+  if (!attr_set) {
+    MPR_HIP(hipFuncSetAttribute((const void*)k_coarse_vcycle<T>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    attr_set = true;
+  }
+  CoarseArgs<T> a;
+  for (int l = 0; l < count; ++l) a.lv[l] = levels[l];
+  a.count = count;
+  a.zero_top = zero_top ? 1 : 0;
+  a.pw = (T)pw;
+  a.pre = pre;
+  a.post = post;
+  hipLaunchKernelGGL(k_coarse_vcycle<T>, dim3(1), dim3(1024), bytes, s, a);
+  MPR_LAUNCH_CHECK();
+  return true;
+}
+
+#define MPR_INST(T)                                                                          \
+  template void launch_fill<T>(T*, T, uint64_t, hipStream_t);                                \
+  template void launch_relax<T>(T*, const T*, const T*, const Geom&, double, int, hipStream_t); \
+  template void launch_relax_color<T>(T*, const T*, const T*, const Geom&, double, int,         \
+                                      hipStream_t);                                            \
+  template void launch_restrict<T>(const T*, const Geom&, T*, const Geom&, double, hipStream_t); \
+  template void launch_prolong_add<T>(const T*, const Geom&, T*, const Geom&, hipStream_t);  \
+  template void launch_split_to_nat<T>(const T*, double*, const Geom&, hipStream_t);         \
+  template bool launch_coarse_vcycle<T>(const LevelPtrs<T>*, int, bool, double, int, int,    \
+                                        hipStream_t);                                        \
+  template size_t coarse_lds_bytes<T>(const Geom*, int);
+MPR_INST(double)
+MPR_INST(float)
+#undef MPR_INST
+template void launch_fill<int32_t>(int32_t*, int32_t, uint64_t, hipStream_t);
+template void launch_fill<int64_t>(int64_t*, int64_t, uint64_t, hipStream_t);
+template void launch_fill<unsigned long long>(unsigned long long*, unsigned long long, uint64_t,
+                                              hipStream_t);
+template int launch_residual<double, double>(const double*, const double*, const double*,
+                                             double*, const Geom&, double, double*, hipStream_t);
+template int launch_residual<double, float>(const double*, const double*, const double*, float*,
+                                            const Geom&, double, double*, hipStream_t);
+template int launch_residual<float, float>(const float*, const float*, const float*, float*,
+                                           const Geom&, double, double*, hipStream_t);
+template void launch_nat_to_split<double, double>(const double*, double*, const Geom&,
+                                                  hipStream_t);
+template void launch_nat_to_split<double, float>(const double*, float*, const Geom&,
+                                                 hipStream_t);
+template void launch_convert<double, float>(const double*, float*, uint64_t, hipStream_t);
+template void launch_convert<float, double>(const float*, double*, uint64_t, hipStream_t);
+
+}  // namespace mpr

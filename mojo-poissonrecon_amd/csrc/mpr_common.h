@@ -1,0 +1,111 @@
+// Shared definitions for the MI355X (gfx950) screened Poisson library.
+//
+// Grid layouts in HBM
+// -------------------
+// * natural: (n,n,n) row-major [z][y][x], index (z*n + y)*n + x — the layout of every grid
+//   that crosses the API (capi.mojo:41-48).
+// * colour-split: the solver's layout. Node (x,y,z) has colour c = (x+y+z)&1. Each colour
+//   owns a block of C = n*n*H words, H = (n+1)/2, holding row (z,y) of that colour as
+//   H consecutive words, node x at slot x>>1 (rows whose first node of colour c is at
+//   x = 1 have one unused pad slot at the end). A red-black sweep of colour c then reads
+//   and writes colour c contiguously and reads colour 1-c contiguously: no half-used
+//   cache lines, so a full Gauss-Seidel iteration moves 4 words per node.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#define MPR_HD __host__ __device__ __forceinline__
+
+namespace mpr {
+
+// ---------------------------------------------------------------------------------------
+// error plumbing: no C++ exception crosses the C ABI; every entry point returns a status
+// and records a message retrievable through mpr_last_error().
+void set_error(const std::string& msg);
+const char* last_error();
+
+struct HipError {
+  std::string msg;
+};
+
+#define MPR_HIP(call)                                                                  \
+  do {                                                                                 \
+    hipError_t _e = (call);                                                            \
+    if (_e != hipSuccess)                                                              \
+      throw ::mpr::HipError{std::string(#call) + ": " + hipGetErrorString(_e) + " (" + \
+                            __FILE__ + ":" + std::to_string(__LINE__) + ")"};          \
+  } while (0)
+
+#define MPR_LAUNCH_CHECK() MPR_HIP(hipGetLastError())
+
+// ---------------------------------------------------------------------------------------
+// unsigned 32-bit division by a run-time invariant divisor (multiply-high + shift,
+// Granlund & Montgomery 1994), used to decode flattened thread indices.
+struct FastDiv {
+  uint32_t d, m, s;
+  MPR_HD FastDiv() : d(1), m(1), s(0) {}
+  explicit FastDiv(uint32_t div) : d(div) {
+    uint32_t l = 0;
+    while ((1ull << l) < div) ++l;
+    s = l;
+    m = (uint32_t)((((1ull << 32) * ((1ull << l) - div)) / div) + 1);
+  }
+  MPR_HD uint32_t div(uint32_t t) const {
+#ifdef __HIP_DEVICE_COMPILE__
+    uint32_t hi = __umulhi(t, m);
+#else
+    uint32_t hi = (uint32_t)(((uint64_t)t * m) >> 32);
+#endif
+    return (uint32_t)(((uint64_t)hi + t) >> s);
+  }
+};
+
+// geometry of one grid level in the colour-split layout
+struct Geom {
+  int32_t n;     // nodes per side
+  int32_t H;     // slots per colour row, (n+1)/2
+  uint32_t C;    // words per colour block, n*n*H
+  uint32_t nH;   // n*H
+  FastDiv div_nH, div_H, div_C;
+};
+
+inline Geom make_geom(int32_t n) {
+  Geom g;
+  g.n = n;
+  g.H = (n + 1) / 2;
+  g.nH = (uint32_t)n * (uint32_t)g.H;
+  g.C = g.nH * (uint32_t)n;
+  g.div_nH = FastDiv(g.nH);
+  g.div_H = FastDiv((uint32_t)g.H);
+  g.div_C = FastDiv(g.C);
+  return g;
+}
+
+MPR_HD int64_t nat_index(int x, int y, int z, int n) { return ((int64_t)z * n + y) * n + x; }
+
+MPR_HD uint32_t split_index(const Geom& g, int x, int y, int z) {
+  uint32_t c = (uint32_t)((x + y + z) & 1);
+  return c * g.C + ((uint32_t)z * (uint32_t)g.n + (uint32_t)y) * (uint32_t)g.H + (uint32_t)(x >> 1);
+}
+
+// Decode an index inside colour block c into (x,y,z); returns false for a pad slot.
+MPR_HD bool split_decode(const Geom& g, int c, uint32_t t, int& x, int& y, int& z) {
+  uint32_t zz = g.div_nH.div(t);
+  uint32_t r = t - zz * g.nH;
+  uint32_t yy = g.div_H.div(r);
+  uint32_t k = r - yy * (uint32_t)g.H;
+  int p = (c ^ (int)yy ^ (int)zz) & 1;
+  x = 2 * (int)k + p;
+  y = (int)yy;
+  z = (int)zz;
+  return x < g.n;
+}
+
+inline unsigned blocks_for(uint64_t work, unsigned threads) {
+  return (unsigned)((work + threads - 1) / threads);
+}
+
+}  // namespace mpr

@@ -1,0 +1,113 @@
+// Internal launch interface between the kernel files and the host-side driver.
+// Every launcher enqueues on the given stream and never synchronises.
+#pragma once
+
+#include "mpr_common.h"
+
+namespace mpr {
+
+constexpr int kThreads = 256;
+constexpr int kMaxCoarseLevels = 8;
+
+template <class T>
+struct LevelPtrs {
+  T* x;
+  T* rhs;
+  T* screen;
+  T* res;
+  Geom g;
+};
+
+// ---- grid kernels (grid_kernels.hip) ----------------------------------------------------
+template <class T> void launch_fill(T* p, T value, uint64_t count, hipStream_t s);
+template <class T>
+void launch_relax(T* X, const T* R, const T* S, const Geom& g, double pw, int iterations,
+                  hipStream_t s);
+// one red-black half sweep (colour c)
+template <class T>
+void launch_relax_color(T* X, const T* R, const T* S, const Geom& g, double pw, int c,
+                        hipStream_t s);
+// r = rhs - A x written as TR into `out` (split layout). When `partials` is non-null each
+// block also writes the sum of squares of its residuals there; returns the block count.
+template <class T, class TR>
+int launch_residual(const T* X, const T* R, const T* S, TR* out, const Geom& g, double pw,
+                    double* partials, hipStream_t s);
+// out = sqrt(sum partials) with a fixed summation order
+void launch_norm_finish(const double* partials, int count, double* out, hipStream_t s);
+template <class T>
+void launch_restrict(const T* F, const Geom& gf, T* Cc, const Geom& gc, double scale,
+                     hipStream_t s);
+template <class T>
+void launch_prolong_add(const T* Cc, const Geom& gc, T* F, const Geom& gf, hipStream_t s);
+template <class TI, class TO>
+void launch_nat_to_split(const TI* nat, TO* split, const Geom& g, hipStream_t s);
+template <class T> void launch_split_to_nat(const T* split, double* nat, const Geom& g, hipStream_t s);
+template <class TI, class TO> void launch_convert(const TI* in, TO* out, uint64_t count, hipStream_t s);
+void launch_add_correction(double* X, const float* E, uint64_t count, hipStream_t s);
+// rhs = -div(v) + alpha * screen * h^-2 * target (capi.mojo:227-279), written in the natural
+// layout (rhs_nat, optional) and the split layout (rhs_split); also copies the screening
+// weights into the split layout and, if density is non-null, divides it by spn in place
+// (core.py:255).
+void launch_assemble(const double* vx, const double* vy, const double* vz,
+                     const double* screen_nat, double* rhs_nat, double* rhs_split,
+                     double* screen_split, double* density, double spn, const Geom& g,
+                     double pw, double target, hipStream_t s);
+// Runs the V-cycle of levels[0..count) (top level first, last level n <= 3) inside one
+// workgroup with every level resident in LDS. Returns false if the levels do not fit.
+template <class T>
+bool launch_coarse_vcycle(const LevelPtrs<T>* levels, int count, bool zero_top, double pw,
+                          int pre, int post, hipStream_t s);
+template <class T> size_t coarse_lds_bytes(const Geom* geoms, int count);
+
+// ---- point kernels (point_kernels.hip) --------------------------------------------------
+int bbox_partial_count(uint32_t P);
+// core.py:377-385 on the device: params = [origin x, y, z, width], unit = (p - origin)/width
+void launch_normalize(const double* pts, uint32_t P, double scale, double* part, double* params,
+                      double* unit, hipStream_t s);
+void launch_octree_codes(const double* pts, uint32_t P, int depth, int64_t* codes, hipStream_t s);
+void launch_scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t count, uint32_t* total,
+                           hipStream_t s);
+size_t radix_scratch_words(uint32_t n);
+void launch_radix_sort(uint64_t* keys, uint32_t* vals, uint64_t* k_tmp, uint32_t* v_tmp,
+                       uint32_t n, int bits, uint32_t* hist, uint32_t* offs, hipStream_t s);
+void launch_leaf_keys(const double* pts, uint32_t P, int depth, uint64_t* keys, uint32_t* perm,
+                      hipStream_t s);
+uint32_t compaction_tiles(uint64_t n);
+void launch_level_counts(const uint64_t* keys, uint32_t n, int d, int depth, uint32_t* counts,
+                         uint32_t* offsets_out, uint32_t* level_total, hipStream_t s);
+void launch_level_offsets(const uint32_t* level_count, int levels, int64_t* offsets,
+                          hipStream_t s);
+void launch_level_emit(const uint64_t* keys, uint32_t n, int d, int depth, const int64_t* offsets,
+                       const uint32_t* tile_offsets, const uint32_t* uidx_prev, uint32_t* uidx_cur,
+                       int64_t* morton, int64_t* depth_out, int64_t* parent, int64_t* child_start,
+                       int64_t* child_count, hipStream_t s);
+void launch_point_leaf(const uint32_t* perm, const uint32_t* uidx, uint32_t n,
+                       const int64_t* offsets, int depth, int64_t* point_leaf, hipStream_t s);
+// capi.mojo:102-224; density/screen/vx/vy/vz accumulate (callers zero them)
+void launch_splat(const double* pts, const double* nrm, uint32_t P, int n, int confidence,
+                  double* density, double* vx, double* vy, double* vz, double* screen,
+                  double spn, unsigned long long* accepted, hipStream_t s);
+int iso_partial_count(uint32_t P);
+void launch_iso(const double* field, const double* pts, const double* nrm, uint32_t P, int n,
+                int confidence, double* part, double* iso, hipStream_t s);
+
+// ---- extraction kernels (extract_kernels.hip) --------------------------------------------
+uint32_t cell_tiles(int n);
+uint32_t edge_tiles(int n);
+// iso is a device pointer: the pipeline computes it on the GPU
+void launch_cells_count(const double* f, int n, const double* iso, uint32_t* counts, hipStream_t s);
+void launch_cells_chunk_counts(const double* f, int n, const double* iso, int64_t* chunks,
+                               hipStream_t s);
+template <class CM>
+void launch_cells_emit(const double* f, const double* dens, int n, const double* iso,
+                       const uint32_t* tile_offsets, CM* cell_map, double* verts, double* vdens,
+                       uint64_t capacity, const double* xform, hipStream_t s);
+template <class CM>
+void launch_edges_count(const double* f, int n, const double* iso, const CM* cm, uint32_t* counts,
+                        hipStream_t s);
+template <class CM>
+void launch_edges_emit(const double* f, int n, const double* iso, const CM* cm,
+                       const uint32_t* tile_offsets, int64_t* faces, uint64_t capacity,
+                       hipStream_t s);
+
+}  // namespace mpr

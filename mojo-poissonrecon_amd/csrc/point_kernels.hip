@@ -1,0 +1,588 @@
+// Point-stage kernels for gfx950: unit-cube normalisation, Morton codes, a stable LSB
+// radix sort, the breadth-first octree arrays, the two-pass trilinear splat and the
+// iso-value reduction. All are byte-bound scatters/gathers; the splat accumulates with
+// the native FP64 atomic add of the MI355X L2 (global_atomic_add_f64).
+#include <algorithm>
+
+#include "block_scan.h"
+#include "mpr_internal.h"
+
+namespace mpr {
+
+namespace {
+
+// capi.mojo:51-59: scaled Euclidean norm
+__device__ __forceinline__ double norm3(double x, double y, double z) {
+  double ax = fabs(x), ay = fabs(y), az = fabs(z);
+  double m = ay > az ? ay : az;
+  double scale = ax > m ? ax : m;
+  if (scale == 0.0) return 0.0;
+  double sx = x / scale, sy = y / scale, sz = z / scale;
+  return scale * sqrt(sx * sx + sy * sy + sz * sz);
+}
+
+__device__ __forceinline__ double block_sum_d(double v, double* lds4) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) lds4[wave] = v;
+  __syncthreads();
+  double tot = ((lds4[0] + lds4[1]) + lds4[2]) + lds4[3];
+  __syncthreads();
+  return tot;
+}
+
+// ---- normalisation (core.py:377-385) -------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_bbox_partials(const double* pts, uint32_t P,
+                                                            double* part) {
+  __shared__ double lds[6][kThreads];
+  double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < P; i += gridDim.x * kThreads)
+    for (int a = 0; a < 3; ++a) {
+      double v = pts[3ull * i + a];
+      mn[a] = v < mn[a] ? v : mn[a];
+      mx[a] = v > mx[a] ? v : mx[a];
+    }
+  for (int a = 0; a < 3; ++a) {
+    lds[a][threadIdx.x] = mn[a];
+    lds[3 + a][threadIdx.x] = mx[a];
+  }
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int a = 0; a < 3; ++a) {
+        double o = lds[a][threadIdx.x + w];
+        if (o < lds[a][threadIdx.x]) lds[a][threadIdx.x] = o;
+        o = lds[3 + a][threadIdx.x + w];
+        if (o > lds[3 + a][threadIdx.x]) lds[3 + a][threadIdx.x] = o;
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = lds[threadIdx.x][0];
+}
+
+// params: [origin x, y, z, width]
+__global__ void k_bbox_finish(const double* part, int nparts, double scale, double* params) {
+  if (threadIdx.x != 0) return;
+  double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int b = 0; b < nparts; ++b)
+    for (int a = 0; a < 3; ++a) {
+      double lo = part[6 * b + a], hi = part[6 * b + 3 + a];
+      mn[a] = lo < mn[a] ? lo : mn[a];
+      mx[a] = hi > mx[a] ? hi : mx[a];
+    }
+  double ext = mx[0] - mn[0];
+  for (int a = 1; a < 3; ++a) ext = (mx[a] - mn[a]) > ext ? (mx[a] - mn[a]) : ext;
+  double width = ext * scale;
+  if (!isfinite(width) || width <= 0.0) width = 1.0;
+  for (int a = 0; a < 3; ++a) {
+    double center = 0.5 * (mn[a] + mx[a]);
+    params[a] = center - 0.5 * width;
+  }
+  params[3] = width;
+}
+
+__global__ __launch_bounds__(kThreads) void k_normalize(const double* pts, uint32_t P,
+                                                        const double* params, double* unit) {
+  const uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= 3ull * P) return;
+  const int a = (int)(i % 3);
+  unit[i] = (pts[i] - params[a]) / params[3];
+}
+
+// ---- Morton codes (capi.mojo:62-99) ---------------------------------------------------
+__device__ __forceinline__ int64_t level_child(double v, int64_t scale) {
+  int64_t q = (int64_t)(v * (double)scale);
+  if (q < 0) q = 0;
+  if (q >= scale) q = scale - 1;
+  return q & 1;
+}
+
+__global__ __launch_bounds__(kThreads) void k_octree_codes(const double* pts, uint32_t P,
+                                                           int depth, int64_t* codes) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= P) return;
+  const double x = pts[3ull * i], y = pts[3ull * i + 1], z = pts[3ull * i + 2];
+  int64_t code = 0;
+  int64_t* row = codes + (uint64_t)i * (depth + 1);
+  row[0] = 0;
+  for (int d = 0; d < depth; ++d) {
+    const int64_t scale = (int64_t)1 << (d + 1);
+    const int64_t child = level_child(x, scale) | (level_child(y, scale) << 1) |
+                          (level_child(z, scale) << 2);
+    code = (code << 3) | child;
+    row[d + 1] = code;
+  }
+}
+
+// leaf (level-depth) code and the identity permutation, the sort's input
+__global__ __launch_bounds__(kThreads) void k_leaf_keys(const double* pts, uint32_t P, int depth,
+                                                        uint64_t* keys, uint32_t* perm) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= P) return;
+  const double x = pts[3ull * i], y = pts[3ull * i + 1], z = pts[3ull * i + 2];
+  int64_t code = 0;
+  for (int d = 0; d < depth; ++d) {
+    const int64_t scale = (int64_t)1 << (d + 1);
+    code = (code << 3) | level_child(x, scale) | (level_child(y, scale) << 1) |
+           (level_child(z, scale) << 2);
+  }
+  keys[i] = (uint64_t)code;
+  perm[i] = i;
+}
+
+// ---- stable LSB radix sort, 8-bit digits ---------------------------------------------
+constexpr int kRadixItems = 8;                        // items per thread per tile
+constexpr int kRadixTile = kThreads * kRadixItems;    // 2048 keys per block
+
+__global__ __launch_bounds__(kThreads) void k_radix_hist(const uint64_t* keys, uint32_t n,
+                                                         int shift, uint32_t* hist,
+                                                         uint32_t nblocks) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t base = blockIdx.x * kRadixTile;
+  for (int r = 0; r < kRadixItems; ++r) {
+    const uint32_t i = base + r * kThreads + threadIdx.x;
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];  // digit-major
+}
+
+__global__ __launch_bounds__(kThreads) void k_radix_scatter(const uint64_t* kin,
+                                                            const uint32_t* vin, uint64_t* kout,
+                                                            uint32_t* vout, uint32_t n, int shift,
+                                                            const uint32_t* offsets,
+                                                            uint32_t nblocks) {
+  __shared__ uint32_t wave_cnt[4][256];
+  __shared__ uint32_t running[256];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  running[threadIdx.x] = offsets[threadIdx.x * nblocks + blockIdx.x];
+  const uint32_t base = blockIdx.x * kRadixTile;
+  for (int r = 0; r < kRadixItems; ++r) {
+    for (int w = 0; w < 4; ++w) wave_cnt[w][threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t i = base + r * kThreads + threadIdx.x;
+    const bool valid = i < n;
+    uint64_t key = 0;
+    uint32_t val = 0, digit = 0;
+    if (valid) {
+      key = kin[i];
+      val = vin[i];
+      digit = (uint32_t)(key >> shift) & 255u;
+    }
+    uint64_t match = __ballot(valid);
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (digit >> b) & 1u;
+      const uint64_t bal = __ballot(bit);
+      match &= bit ? bal : ~bal;
+    }
+    const uint32_t rank = (uint32_t)__popcll(match & ((1ull << lane) - 1ull));
+    if (valid && rank == 0) wave_cnt[wave][digit] = (uint32_t)__popcll(match);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = running[digit] + rank;
+      for (uint32_t w = 0; w < wave; ++w) pos += wave_cnt[w][digit];
+      kout[pos] = key;
+      vout[pos] = val;
+    }
+    __syncthreads();
+    running[threadIdx.x] += wave_cnt[0][threadIdx.x] + wave_cnt[1][threadIdx.x] +
+                            wave_cnt[2][threadIdx.x] + wave_cnt[3][threadIdx.x];
+    __syncthreads();
+  }
+}
+
+// ---- octree levels (core.py:99-127) -----------------------------------------------------
+constexpr int kTileItems = 16;
+constexpr uint32_t kTile = kThreads * kTileItems;  // 4096 items per block for compaction
+
+__device__ __forceinline__ bool level_head(const uint64_t* keys, uint32_t i, int sh) {
+  return i == 0 || (keys[i] >> sh) != (keys[i - 1] >> sh);
+}
+
+__global__ __launch_bounds__(kThreads) void k_level_count(const uint64_t* keys, uint32_t n,
+                                                          int sh, uint32_t* counts) {
+  __shared__ uint32_t lds4[4];
+  uint32_t c = 0;
+  for (int r = 0; r < kTileItems; ++r) {
+    const uint32_t i = blockIdx.x * kTile + r * kThreads + threadIdx.x;
+    if (i < n) c += level_head(keys, i, sh);
+  }
+  c = block_count256(c, lds4);
+  if (threadIdx.x == 0) counts[blockIdx.x] = c;
+}
+
+// level offsets: offsets[d] = sum of level sizes before d; offsets[levels] = node count
+__global__ void k_level_offsets(const uint32_t* level_count, int levels, int64_t* offsets) {
+  if (threadIdx.x != 0) return;
+  int64_t acc = 0;
+  for (int d = 0; d < levels; ++d) {
+    offsets[d] = acc;
+    acc += level_count[d];
+  }
+  offsets[levels] = acc;
+}
+
+// Writes the level-d nodes (first occurrence of each level-d code in sorted order):
+// morton, depth, parent, first-child and child count of the parent; records for every
+// sorted point the level-d local node index (uidx_cur) used by level d+1.
+__global__ __launch_bounds__(kThreads) void k_level_emit(
+    const uint64_t* keys, uint32_t n, int d, int depth, const int64_t* offsets,
+    const uint32_t* tile_offsets, const uint32_t* uidx_prev, uint32_t* uidx_cur,
+    int64_t* morton, int64_t* depth_out, int64_t* parent, int64_t* child_start,
+    unsigned long long* child_count) {
+  __shared__ uint32_t lds4[4];
+  const int sh = 3 * (depth - d);
+  const int64_t off_d = offsets[d];
+  const int64_t off_p = d > 0 ? offsets[d - 1] : 0;
+  uint32_t running = tile_offsets[blockIdx.x];
+  for (int r = 0; r < kTileItems; ++r) {
+    const uint32_t i = blockIdx.x * kTile + r * kThreads + threadIdx.x;
+    const bool valid = i < n;
+    const bool head = valid && level_head(keys, i, sh);
+    uint32_t total;
+    const uint32_t pre = block_prefix256(head, lds4, total);
+    if (valid) {
+      // local index of this point's level-d node = number of heads up to and including i, -1
+      const uint32_t u = running + pre + (head ? 1u : 0u) - 1u;
+      if (uidx_cur) uidx_cur[i] = u;
+      if (head) {
+        const int64_t node = off_d + u;
+        morton[node] = (int64_t)(keys[i] >> sh);
+        depth_out[node] = d;
+        if (d > 0) {
+          const int64_t p = off_p + uidx_prev[i];
+          parent[node] = p;
+          if (level_head(keys, i, sh + 3)) child_start[p] = node;
+          atomicAdd(&child_count[p], 1ull);
+        } else {
+          parent[node] = -1;
+        }
+      }
+    }
+    running += total;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_point_leaf(const uint32_t* perm,
+                                                         const uint32_t* uidx, uint32_t n,
+                                                         const int64_t* offsets, int depth,
+                                                         int64_t* point_leaf) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) point_leaf[perm[i]] = offsets[depth] + uidx[i];
+}
+
+// ---- device-wide exclusive scan of tile counts (one workgroup) -------------------------
+__global__ __launch_bounds__(1024) void k_scan_exclusive(const uint32_t* in, uint32_t* out,
+                                                         uint32_t count, uint32_t* total) {
+  __shared__ uint32_t part[1024];
+  const uint32_t per = (count + 1023u) / 1024u;
+  const uint32_t b = threadIdx.x * per, e = min(count, b + per);
+  uint32_t s = 0;
+  for (uint32_t i = b; i < e; ++i) s += in[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  // Hillis-Steele inclusive scan of the 1024 chunk sums
+  for (uint32_t w = 1; w < 1024; w <<= 1) {
+    uint32_t v = threadIdx.x >= w ? part[threadIdx.x - w] : 0u;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
+  for (uint32_t i = b; i < e; ++i) {
+    const uint32_t v = in[i];
+    out[i] = run;
+    run += v;
+  }
+  if (threadIdx.x == 1023 && total) *total = part[1023];
+}
+
+// ---- splat (capi.mojo:102-224) ---------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_splat_pass1(const double* pts, const double* nrm,
+                                                          uint32_t P, int n, int confidence,
+                                                          double* density, double* screen,
+                                                          unsigned long long* accepted) {
+  __shared__ uint32_t lds4[4];
+  const uint32_t p = blockIdx.x * kThreads + threadIdx.x;
+  uint32_t ok = 0;
+  if (p < P) {
+    const double length = norm3(nrm[3ull * p], nrm[3ull * p + 1], nrm[3ull * p + 2]);
+    if (length > 0.0) {
+      ok = 1;
+      const int res = n - 1;
+      const double sw = confidence ? length : 1.0;
+      double gx = pts[3ull * p] * (double)res, gy = pts[3ull * p + 1] * (double)res,
+             gz = pts[3ull * p + 2] * (double)res;
+      int64_t ix = (int64_t)gx, iy = (int64_t)gy, iz = (int64_t)gz;
+      if (ix < 0) ix = 0;
+      if (iy < 0) iy = 0;
+      if (iz < 0) iz = 0;
+      if (ix >= res) { ix = res - 1; gx = (double)res; }
+      if (iy >= res) { iy = res - 1; gy = (double)res; }
+      if (iz >= res) { iz = res - 1; gz = (double)res; }
+      const double fx = gx - (double)ix, fy = gy - (double)iy, fz = gz - (double)iz;
+      for (int dz = 0; dz < 2; ++dz) {
+        const double wz = dz ? fz : 1.0 - fz;
+        for (int dy = 0; dy < 2; ++dy) {
+          const double wy = dy ? fy : 1.0 - fy;
+          for (int dx = 0; dx < 2; ++dx) {
+            const double wx = dx ? fx : 1.0 - fx;
+            const double w = wx * wy * wz * sw;
+            const int64_t idx = nat_index((int)ix + dx, (int)iy + dy, (int)iz + dz, n);
+            unsafeAtomicAdd(&density[idx], 2.0 * w);
+            unsafeAtomicAdd(&screen[idx], w);
+          }
+        }
+      }
+    }
+  }
+  const uint32_t c = block_count256(ok, lds4);
+  if (threadIdx.x == 0 && c) atomicAdd(accepted, (unsigned long long)c);
+}
+
+__global__ __launch_bounds__(kThreads) void k_splat_pass2(const double* pts, const double* nrm,
+                                                          uint32_t P, int n, int confidence,
+                                                          const double* density, double* vx,
+                                                          double* vy, double* vz, double spn) {
+  const uint32_t p = blockIdx.x * kThreads + threadIdx.x;
+  if (p >= P) return;
+  double nx = nrm[3ull * p], ny = nrm[3ull * p + 1], nz = nrm[3ull * p + 2];
+  const double length = norm3(nx, ny, nz);
+  if (!(length > 0.0)) return;
+  const double sw = confidence ? length : 1.0;
+  nx /= length;
+  ny /= length;
+  nz /= length;
+  const int64_t res = n - 1;
+  const double gx = pts[3ull * p] * (double)res, gy = pts[3ull * p + 1] * (double)res,
+               gz = pts[3ull * p + 2] * (double)res;
+  int64_t ix = (int64_t)gx, iy = (int64_t)gy, iz = (int64_t)gz;
+  ix = ix < 0 ? 0 : ix; ix = ix > res - 1 ? res - 1 : ix;
+  iy = iy < 0 ? 0 : iy; iy = iy > res - 1 ? res - 1 : iy;
+  iz = iz < 0 ? 0 : iz; iz = iz > res - 1 ? res - 1 : iz;
+  double fx = gx - (double)ix, fy = gy - (double)iy, fz = gz - (double)iz;
+  fx = fx < 0.0 ? 0.0 : fx; fx = fx > 1.0 ? 1.0 : fx;
+  fy = fy < 0.0 ? 0.0 : fy; fy = fy > 1.0 ? 1.0 : fy;
+  fz = fz < 0.0 ? 0.0 : fz; fz = fz > 1.0 ? 1.0 : fz;
+  double local = 0.0;
+  for (int dz = 0; dz < 2; ++dz) {
+    const double wz = dz ? fz : 1.0 - fz;
+    for (int dy = 0; dy < 2; ++dy) {
+      const double wy = dy ? fy : 1.0 - fy;
+      for (int dx = 0; dx < 2; ++dx) {
+        const double wx = dx ? fx : 1.0 - fx;
+        local += density[nat_index((int)ix + dx, (int)iy + dy, (int)iz + dz, n)] * wx * wy * wz;
+      }
+    }
+  }
+  const double q = local / spn;
+  const double factor = q > 1.0 ? q : 1.0;
+  for (int dz = 0; dz < 2; ++dz) {
+    const double wz = dz ? fz : 1.0 - fz;
+    for (int dy = 0; dy < 2; ++dy) {
+      const double wy = dy ? fy : 1.0 - fy;
+      for (int dx = 0; dx < 2; ++dx) {
+        const double wx = dx ? fx : 1.0 - fx;
+        const double w = wx * wy * wz * sw;
+        const int64_t idx = nat_index((int)ix + dx, (int)iy + dy, (int)iz + dz, n);
+        const double vw = w * (double)res / factor;
+        unsafeAtomicAdd(&vx[idx], -(nx * vw));
+        unsafeAtomicAdd(&vy[idx], -(ny * vw));
+        unsafeAtomicAdd(&vz[idx], -(nz * vw));
+      }
+    }
+  }
+}
+
+// ---- iso value (capi.mojo:745-795) -----------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_iso_partials(const double* field,
+                                                           const double* pts, const double* nrm,
+                                                           uint32_t P, int n, int confidence,
+                                                           double* part) {
+  __shared__ double lds4[4];
+  const uint32_t p = blockIdx.x * kThreads + threadIdx.x;
+  double vw = 0.0, wt = 0.0;
+  if (p < P) {
+    const double weight =
+        confidence ? norm3(nrm[3ull * p], nrm[3ull * p + 1], nrm[3ull * p + 2]) : 1.0;
+    const int64_t res = n - 1;
+    const double gx = pts[3ull * p] * (double)res, gy = pts[3ull * p + 1] * (double)res,
+                 gz = pts[3ull * p + 2] * (double)res;
+    int64_t ix = (int64_t)gx, iy = (int64_t)gy, iz = (int64_t)gz;
+    ix = ix < 0 ? 0 : ix; ix = ix > res - 1 ? res - 1 : ix;
+    iy = iy < 0 ? 0 : iy; iy = iy > res - 1 ? res - 1 : iy;
+    iz = iz < 0 ? 0 : iz; iz = iz > res - 1 ? res - 1 : iz;
+    double fx = gx - (double)ix, fy = gy - (double)iy, fz = gz - (double)iz;
+    fx = fx < 0.0 ? 0.0 : fx; fx = fx > 1.0 ? 1.0 : fx;
+    fy = fy < 0.0 ? 0.0 : fy; fy = fy > 1.0 ? 1.0 : fy;
+    fz = fz < 0.0 ? 0.0 : fz; fz = fz > 1.0 ? 1.0 : fz;
+    double value = 0.0;
+    for (int dz = 0; dz < 2; ++dz) {
+      const double wz = dz ? fz : 1.0 - fz;
+      for (int dy = 0; dy < 2; ++dy) {
+        const double wy = dy ? fy : 1.0 - fy;
+        for (int dx = 0; dx < 2; ++dx) {
+          const double wx = dx ? fx : 1.0 - fx;
+          value += field[nat_index((int)ix + dx, (int)iy + dy, (int)iz + dz, n)] * wx * wy * wz;
+        }
+      }
+    }
+    vw = value * weight;
+    wt = weight;
+  }
+  const double a = block_sum_d(vw, lds4);
+  const double b = block_sum_d(wt, lds4);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = a;
+    part[2 * blockIdx.x + 1] = b;
+  }
+}
+
+__global__ void k_iso_finish(const double* part, int nparts, double* iso) {
+  if (threadIdx.x != 0) return;
+  double a = 0.0, b = 0.0;
+  for (int i = 0; i < nparts; ++i) {
+    a += part[2 * i];
+    b += part[2 * i + 1];
+  }
+  iso[0] = b > 0.0 ? a / b : 0.0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+int bbox_partial_count(uint32_t P) {
+  return (int)std::min<uint32_t>(std::max<uint32_t>(1u, blocks_for(P, kThreads)), 1024u);
+}
+
+void launch_normalize(const double* pts, uint32_t P, double scale, double* part, double* params,
+                      double* unit, hipStream_t s) {
+  const int nb = bbox_partial_count(P);
+  hipLaunchKernelGGL(k_bbox_partials, dim3(nb), dim3(kThreads), 0, s, pts, P, part);
+  MPR_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_bbox_finish, dim3(1), dim3(64), 0, s, part, nb, scale, params);
+  MPR_LAUNCH_CHECK();
+  if (P) {
+    hipLaunchKernelGGL(k_normalize, dim3(blocks_for(3ull * P, kThreads)), dim3(kThreads), 0, s,
+                       pts, P, params, unit);
+    MPR_LAUNCH_CHECK();
+  }
+}
+
+void launch_octree_codes(const double* pts, uint32_t P, int depth, int64_t* codes,
+                         hipStream_t s) {
+  if (!P) return;
+  hipLaunchKernelGGL(k_octree_codes, dim3(blocks_for(P, kThreads)), dim3(kThreads), 0, s, pts, P,
+                     depth, codes);
+  MPR_LAUNCH_CHECK();
+}
+
+void launch_scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t count, uint32_t* total,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, in, out, count, total);
+  MPR_LAUNCH_CHECK();
+}
+
+size_t radix_scratch_words(uint32_t n) {
+  const uint32_t nb = std::max<uint32_t>(1u, (n + kRadixTile - 1) / kRadixTile);
+  return 256ull * nb;
+}
+
+// Sorts (keys, vals) by the low `bits` key bits, stably. The result ends in (keys, vals);
+// (k_tmp, v_tmp) are ping-pong buffers; hist and offs hold radix_scratch_words(n) words.
+void launch_radix_sort(uint64_t* keys, uint32_t* vals, uint64_t* k_tmp, uint32_t* v_tmp,
+                       uint32_t n, int bits, uint32_t* hist, uint32_t* offs, hipStream_t s) {
+  if (n <= 1) return;
+  const uint32_t nb = (n + kRadixTile - 1) / kRadixTile;
+  const int passes = (bits + 7) / 8;
+  uint64_t *ka = keys, *kb = k_tmp;
+  uint32_t *va = vals, *vb = v_tmp;
+  for (int p = 0; p < passes; ++p) {
+    const int shift = 8 * p;
+    hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kThreads), 0, s, ka, n, shift, hist, nb);
+    MPR_LAUNCH_CHECK();
+    launch_scan_exclusive(hist, offs, 256u * nb, nullptr, s);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kThreads), 0, s, ka, va, kb, vb, n, shift,
+                       offs, nb);
+    MPR_LAUNCH_CHECK();
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  if (ka != keys) {
+    MPR_HIP(hipMemcpyAsync(keys, ka, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, s));
+    MPR_HIP(hipMemcpyAsync(vals, va, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
+  }
+}
+
+void launch_leaf_keys(const double* pts, uint32_t P, int depth, uint64_t* keys, uint32_t* perm,
+                      hipStream_t s) {
+  if (!P) return;
+  hipLaunchKernelGGL(k_leaf_keys, dim3(blocks_for(P, kThreads)), dim3(kThreads), 0, s, pts, P,
+                     depth, keys, perm);
+  MPR_LAUNCH_CHECK();
+}
+
+uint32_t compaction_tiles(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, (n + kTile - 1) / kTile); }
+
+void launch_level_counts(const uint64_t* keys, uint32_t n, int d, int depth, uint32_t* counts,
+                         uint32_t* offsets_out, uint32_t* level_total, hipStream_t s) {
+  const uint32_t nb = compaction_tiles(n);
+  hipLaunchKernelGGL(k_level_count, dim3(nb), dim3(kThreads), 0, s, keys, n, 3 * (depth - d),
+                     counts);
+  MPR_LAUNCH_CHECK();
+  launch_scan_exclusive(counts, offsets_out, nb, level_total, s);
+}
+
+void launch_level_offsets(const uint32_t* level_count, int levels, int64_t* offsets,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_level_offsets, dim3(1), dim3(64), 0, s, level_count, levels, offsets);
+  MPR_LAUNCH_CHECK();
+}
+
+void launch_level_emit(const uint64_t* keys, uint32_t n, int d, int depth, const int64_t* offsets,
+                       const uint32_t* tile_offsets, const uint32_t* uidx_prev, uint32_t* uidx_cur,
+                       int64_t* morton, int64_t* depth_out, int64_t* parent, int64_t* child_start,
+                       int64_t* child_count, hipStream_t s) {
+  const uint32_t nb = compaction_tiles(n);
+  hipLaunchKernelGGL(k_level_emit, dim3(nb), dim3(kThreads), 0, s, keys, n, d, depth, offsets,
+                     tile_offsets, uidx_prev, uidx_cur, morton, depth_out, parent, child_start,
+                     (unsigned long long*)child_count);
+  MPR_LAUNCH_CHECK();
+}
+
+void launch_point_leaf(const uint32_t* perm, const uint32_t* uidx, uint32_t n,
+                       const int64_t* offsets, int depth, int64_t* point_leaf, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_point_leaf, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s, perm, uidx,
+                     n, offsets, depth, point_leaf);
+  MPR_LAUNCH_CHECK();
+}
+
+void launch_splat(const double* pts, const double* nrm, uint32_t P, int n, int confidence,
+                  double* density, double* vx, double* vy, double* vz, double* screen,
+                  double spn, unsigned long long* accepted, hipStream_t s) {
+  if (!P) return;
+  const unsigned nb = blocks_for(P, kThreads);
+  hipLaunchKernelGGL(k_splat_pass1, dim3(nb), dim3(kThreads), 0, s, pts, nrm, P, n, confidence,
+                     density, screen, accepted);
+  MPR_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_splat_pass2, dim3(nb), dim3(kThreads), 0, s, pts, nrm, P, n, confidence,
+                     density, vx, vy, vz, spn);
+  MPR_LAUNCH_CHECK();
+}
+
+int iso_partial_count(uint32_t P) { return (int)std::max<uint32_t>(1u, blocks_for(P, kThreads)); }
+
+void launch_iso(const double* field, const double* pts, const double* nrm, uint32_t P, int n,
+                int confidence, double* part, double* iso, hipStream_t s) {
+  const int nb = iso_partial_count(P);
+  hipLaunchKernelGGL(k_iso_partials, dim3(nb), dim3(kThreads), 0, s, field, pts, nrm, P, n,
+                     confidence, part);
+  MPR_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_iso_finish, dim3(1), dim3(64), 0, s, part, nb, iso);
+  MPR_LAUNCH_CHECK();
+}
+
+}  // namespace mpr

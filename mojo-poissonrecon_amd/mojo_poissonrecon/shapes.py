@@ -1,0 +1,110 @@
+"""Seeded synthetic oriented point clouds for the benchmark and parity workloads.
+
+* sphere: z ~ U(-1, 1), theta ~ U(0, 2 pi), normals = positions (README.md:64-71 and
+  bench/bench.py:34-41 of the reference);
+* torus: area-uniform samples (rejection on (R + r cos v) / (R + r)), R = 0.6, r = 0.25;
+* superquadric: signed-power parametric samples, normals from the implicit gradient.
+
+Noise: isotropic Gaussian position noise with sigma given as a fraction of the bbox
+diagonal; normal noise rotates each unit normal by an angle ~ N(0, deg) about a uniformly
+random axis perpendicular to it; normal lengths are drawn from ``length_range``.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+def sphere(count: int, seed: int = 0) -> tuple[np.ndarray, np.ndarray]:
+    rng = np.random.default_rng(seed)
+    z = rng.uniform(-1.0, 1.0, count)
+    theta = rng.uniform(0.0, 2.0 * np.pi, count)
+    radius = np.sqrt(1.0 - z * z)
+    p = np.ascontiguousarray(np.column_stack((radius * np.cos(theta), radius * np.sin(theta), z)))
+    return p, p.copy()
+
+
+def torus(count: int, seed: int = 0, R: float = 0.6, r: float = 0.25):
+    rng = np.random.default_rng(seed)
+    us, vs = [], []
+    need = count
+    while need > 0:
+        u = rng.uniform(0, 2 * np.pi, 2 * need + 16)
+        v = rng.uniform(0, 2 * np.pi, 2 * need + 16)
+        keep = rng.uniform(0, 1, u.size) < (R + r * np.cos(v)) / (R + r)
+        us.append(u[keep][:need])
+        vs.append(v[keep][:need])
+        need -= int(min(keep.sum(), need))
+    u, v = np.concatenate(us)[:count], np.concatenate(vs)[:count]
+    p = np.column_stack(((R + r * np.cos(v)) * np.cos(u), (R + r * np.cos(v)) * np.sin(u),
+                         r * np.sin(v)))
+    nrm = np.column_stack((np.cos(v) * np.cos(u), np.cos(v) * np.sin(u), np.sin(v)))
+    return np.ascontiguousarray(p), np.ascontiguousarray(nrm)
+
+
+def _spow(x, e):
+    return np.sign(x) * np.abs(x) ** e
+
+
+def superquadric(count: int, seed: int = 0, e1: float = 0.3, e2: float = 0.3,
+                 half=(0.8, 0.6, 0.4)):
+    rng = np.random.default_rng(seed)
+    eta = rng.uniform(-np.pi / 2, np.pi / 2, count)
+    omega = rng.uniform(-np.pi, np.pi, count)
+    a, b, c = half
+    ce, se, cw, sw = np.cos(eta), np.sin(eta), np.cos(omega), np.sin(omega)
+    p = np.column_stack((a * _spow(ce, e1) * _spow(cw, e2), b * _spow(ce, e1) * _spow(sw, e2),
+                         c * _spow(se, e1)))
+    # gradient of F = ((x/a)^(2/e2) + (y/b)^(2/e2))^(e2/e1) + (z/c)^(2/e1)
+    nrm = np.column_stack((_spow(ce, 2 - e1) * _spow(cw, 2 - e2) / a,
+                           _spow(ce, 2 - e1) * _spow(sw, 2 - e2) / b,
+                           _spow(se, 2 - e1) / c))
+    length = np.linalg.norm(nrm, axis=1)
+    bad = ~(length > 0)
+    nrm[bad] = p[bad]
+    nrm /= np.linalg.norm(nrm, axis=1)[:, None]
+    return np.ascontiguousarray(p), np.ascontiguousarray(nrm)
+
+
+def perturb(points, normals, seed: int = 0, sigma: float = 0.0, degrees: float = 0.0,
+            length_range=(1.0, 1.0)):
+    """Position noise (sigma x bbox diagonal), angular normal noise and normal lengths."""
+    rng = np.random.default_rng(seed + 7919)
+    p = np.array(points, dtype=np.float64)
+    n = np.array(normals, dtype=np.float64)
+    n /= np.linalg.norm(n, axis=1)[:, None]
+    if sigma > 0:
+        diag = float(np.linalg.norm(p.max(axis=0) - p.min(axis=0)))
+        p += rng.normal(0.0, sigma * diag, p.shape)
+    if degrees > 0:
+        t = rng.normal(size=n.shape)
+        t -= np.einsum("ij,ij->i", t, n)[:, None] * n
+        t /= np.linalg.norm(t, axis=1)[:, None]
+        ang = np.deg2rad(rng.normal(0.0, degrees, len(n)))
+        n = n * np.cos(ang)[:, None] + t * np.sin(ang)[:, None]
+        n /= np.linalg.norm(n, axis=1)[:, None]
+    lo, hi = length_range
+    if lo != 1.0 or hi != 1.0:
+        n *= rng.uniform(lo, hi, len(n))[:, None]
+    return np.ascontiguousarray(p), np.ascontiguousarray(n)
+
+
+def torus_workload(count: int = 200_000, seed: int = 0):
+    """Single FP64 depth-8 workload: 200k torus points, sigma 0.002 diag, 5 deg normal noise,
+    normal lengths U(0.5, 1.5), used with confidence weighting."""
+    p, n = torus(count, seed)
+    return perturb(p, n, seed, sigma=0.002, degrees=5.0, length_range=(0.5, 1.5))
+
+
+def superquadric_workload(count: int = 2_000_000, seed: int = 0):
+    """Mixed-precision depth-9 workload: 2M superquadric points, sigma 0.001, 3 deg."""
+    p, n = superquadric(count, seed)
+    return perturb(p, n, seed, sigma=0.001, degrees=3.0)
+
+
+def random_rotation(rng) -> np.ndarray:
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    q *= np.sign(np.diag(r))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q

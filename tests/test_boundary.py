@@ -1,0 +1,145 @@
+"""CPU checks of the drop-in boundary: the C-ABI library loads and exports every symbol its
+header declares, the buffer conversion rules and validation messages match the reference,
+and the host-side mesh preprocessing matches the reference's fixtures. No GPU calls."""
+
+from __future__ import annotations
+
+import os
+import re
+
+import numpy as np
+import pytest
+
+import mojo_poissonrecon as mpr
+import oracle
+from mojo_poissonrecon import _lib
+from mojo_poissonrecon._lib import addr, f64, i64
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "mpr_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(mpr_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    handle = _lib.load_library()
+    names = header_symbols()
+    assert len(names) >= 30
+    for name in names:
+        assert hasattr(handle, name), name
+    # the reference's kernel table is a subset (python/mojo_poissonrecon/_lib.py:20-32)
+    for name in _lib._SIGNATURES:
+        assert name in names
+
+
+def test_reference_kernel_signatures_unchanged():
+    I, F = _lib.I, _lib.F
+    assert _lib._SIGNATURES["mpr_splat_f64"] == ([I] * 10 + [F], I)
+    assert _lib._SIGNATURES["mpr_residual_f64"] == ([I, I, I, I, I, F], F)
+    assert _lib._SIGNATURES["mpr_dual_vertices_f64"] == ([I, I, I, F, I, I, I, I], I)
+
+
+def test_job_footprint_is_computed_without_a_device():
+    fp8 = mpr.Job.footprint(8, 200_000)
+    fp8m = mpr.Job.footprint(8, 200_000, _lib.PRECISION_MIXED)
+    n = 257
+    assert fp8 > 7 * 8 * n ** 3  # at least the seven natural-layout float64 grids
+    assert fp8m > fp8
+    assert mpr.Job.footprint(11, 10) == -1
+
+
+def test_ffi_conversions_reject_narrowing_and_bad_buffers():
+    # reference tests/test_kernels.py:91-103
+    with pytest.raises(TypeError, match="complex"):
+        f64(np.array([1 + 2j]))
+    with pytest.raises(TypeError, match="real numbers"):
+        f64(["1.0"])
+    with pytest.raises(ValueError, match="exactly"):
+        f64(np.array([2**53 + 1], dtype=np.int64))
+    with pytest.raises(ValueError, match="finite integers"):
+        i64(np.array([1.5]))
+    with pytest.raises(TypeError, match="dtype"):
+        addr(np.ones(3, dtype=np.float32))
+    with pytest.raises(ValueError, match="C-contiguous"):
+        addr(np.ones((3, 3), dtype=np.float64)[:, ::2])
+
+
+def test_public_validation_rejects_unsafe_values():
+    # reference tests/test_kernels.py:198-208 (raised before any device work)
+    points = np.array([[0.25, 0.25, 0.25]])
+    normals = np.array([[1.0, 0.0, 0.0]])
+    with pytest.raises(TypeError, match="depth"):
+        mpr.solve_screened_poisson(points, normals, depth=2.5)
+    with pytest.raises(ValueError, match="finite"):
+        mpr.solve_screened_poisson(points, normals, point_weight=np.nan)
+    with pytest.raises(ValueError, match="density must be finite"):
+        mpr.extract_isosurface(np.zeros((2, 2, 2)), density=np.full((2, 2, 2), np.nan))
+    with pytest.raises(ValueError, match="greater than 1"):
+        mpr.reconstruct(points, normals, scale=1.0)
+    with pytest.raises(ValueError, match="depth must be between"):
+        mpr.solve_screened_poisson(points, normals, depth=11)
+    with pytest.raises(ValueError, match="precision"):
+        mpr.solve_screened_poisson(points, normals, precision="fp16")
+    with pytest.raises(ValueError, match="inside"):
+        mpr.solve_screened_poisson(points + 1.0, normals)
+    with pytest.raises(ValueError, match="cubic"):
+        mpr.extract_isosurface(np.zeros((2, 3, 2)))
+    with pytest.raises(ValueError, match="octree points"):
+        mpr.build_octree(points - 1.0)
+
+
+EMPTY = (np.empty((0, 3)), np.empty((0, 3), dtype=np.int64), 0)
+SINGLE_TRIANGLE = (np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0]]),
+                   np.array([[0, 1, 2]]), 3)
+DUPLICATE_VERTICES = (np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0],
+                                [0.0, 0.0, 0.0]]), np.array([[0, 1, 2], [3, 1, 2]]), 4)
+ZERO_AREA = (np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [2.0, 0.0, 0.0]]),
+             np.array([[0, 1, 2]]), 0)
+NON_MANIFOLD = (np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.5, 1.0, 0.0], [0.5, -1.0, 0.0],
+                          [0.5, 0.0, 1.0]]), np.array([[0, 1, 2], [1, 0, 3], [0, 1, 4]]), 5)
+UNREFERENCED = (np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [9.0, 9.0, 9.0]]),
+                np.array([[0, 1, 2]]), 3)
+
+
+@pytest.mark.parametrize("vertices,faces,expected",
+                         [EMPTY, SINGLE_TRIANGLE, DUPLICATE_VERTICES, ZERO_AREA, NON_MANIFOLD,
+                          UNREFERENCED])
+def test_degenerate_mesh_preprocessing(vertices, faces, expected):
+    # reference tests/test_reconstruction.py:86-144
+    points, normals = mpr.oriented_points_from_mesh(vertices, faces)
+    assert len(points) == expected
+    assert points.shape == normals.shape
+    assert np.isfinite(normals).all()
+    if expected:
+        assert np.allclose(np.linalg.norm(normals, axis=1), 1.0)
+    want_p, want_n = oracle.oriented_points_from_mesh(vertices, faces)
+    assert np.array_equal(points, want_p) and np.array_equal(normals, want_n)
+
+
+def test_mesh_normals_are_area_weighted():
+    # reference tests/test_reconstruction.py:171-179
+    vertices = np.array([[0, 0, 0], [2, 0, 0], [0, 2, 0], [0, 0, 1]], dtype=np.float64)
+    faces = np.array([[0, 1, 2], [0, 3, 1]], dtype=np.int64)
+    points, normals = mpr.oriented_points_from_mesh(vertices, faces)
+    origin = np.flatnonzero(np.all(points == vertices[0], axis=1))[0]
+    assert np.allclose(normals[origin], [0.0, 1 / np.sqrt(5), 2 / np.sqrt(5)])
+
+
+def test_vectorised_mesh_normals_equal_the_per_face_loop_bitwise():
+    rng = np.random.default_rng(5)
+    vertices = rng.normal(size=(300, 3))
+    faces = rng.integers(0, 300, size=(1000, 3))
+    faces[:10, 1] = faces[:10, 0]  # degenerate faces
+    p, n = mpr.oriented_points_from_mesh(vertices, faces)
+    wp, wn = oracle.oriented_points_from_mesh(vertices, faces)
+    assert np.array_equal(p, wp) and np.array_equal(n, wn)
+
+
+def test_host_normalisation_matches_reference_restatement():
+    p = np.random.default_rng(1).normal(size=(1000, 3)) * [3.0, 1.0, 0.5] + 7.0
+    a = mpr.core._normalize(p, 1.1)
+    b = oracle.normalize(p, 1.1)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
