@@ -194,15 +194,15 @@ __global__ __launch_bounds__(kThreads) void k_assemble(const double* vx, const d
 }
 
 template <class T>
-__device__ void block_relax(LevelPtrs<T>& L, T pw, int iterations) {
-  const Geom& g = L.g;
+__device__ __forceinline__ void block_relax(T* x, const T* rhs, const T* scr, const Geom& g, T pw,
+                                            int iterations) {
   const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
   for (int it = 0; it < iterations; ++it)
     for (int c = 0; c < 2; ++c) {
       for (uint32_t t = threadIdx.x; t < g.C; t += blockDim.x) {
-        int x, y, z;
-        if (split_decode(g, c, t, x, y, z))
-          relax_node(L.x, L.rhs, L.screen, g, (uint32_t)c * g.C + t, x, y, z, pw, h2);
+        int xx, yy, zz;
+        if (split_decode(g, c, t, xx, yy, zz))
+          relax_node(x, rhs, scr, g, (uint32_t)c * g.C + t, xx, yy, zz, pw, h2);
       }
       __syncthreads();
     }
@@ -210,74 +210,86 @@ __device__ void block_relax(LevelPtrs<T>& L, T pw, int iterations) {
 
 template <class T>
 struct CoarseArgs {
-  LevelPtrs<T> lv[kMaxCoarseLevels];
+  LevelPtrs<T> lv[kMaxCoarseLevels];  // global buffers (only the top x/rhs and the screens
+                                      // are read, only the top x is written back)
+  uint32_t off[kMaxCoarseLevels];     // LDS word offset of each level's x|rhs|screen|res
   int count;
   int zero_top;
   T pw;
   int pre, post;
 };
 
-// One workgroup runs the whole V-cycle below (and including) the top level it is given
-// (core.py:141-193 unrolled into a down sweep and an up sweep), with all levels in LDS.
+// Level L of the single-workgroup V-cycle (core.py:141-193). Levels are template
+// parameters so every geometry and LDS offset is a compile-time-indexed kernel argument
+// (scalar registers) rather than a private array.
+template <class T, int L>
+__device__ void coarse_level(const CoarseArgs<T>& a, T* sm) {
+  if constexpr (L < kMaxCoarseLevels) {
+    const Geom g = a.lv[L].g;
+    const uint32_t m = 2u * g.C;
+    T* x = sm + a.off[L];
+    T* rhs = x + m;
+    T* scr = rhs + m;
+    T* res = scr + m;
+    if (L + 1 >= a.count) {  // coarsest: 40 iterations (core.py:152-156)
+      block_relax(x, rhs, scr, g, a.pw, 40);
+      return;
+    }
+    block_relax(x, rhs, scr, g, a.pw, a.pre);
+    const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
+    for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) {
+      const int c = t >= g.C;
+      int xx, yy, zz;
+      if (split_decode(g, c, t - (uint32_t)c * g.C, xx, yy, zz))
+        res[t] = residual_node(x, rhs, scr, g, t, xx, yy, zz, a.pw, h2);
+    }
+    __syncthreads();
+    const Geom gc = a.lv[L + 1].g;
+    const uint32_t mc = 2u * gc.C;
+    T* xc = sm + a.off[L + 1];
+    T* rhsc = xc + mc;
+    for (uint32_t t = threadIdx.x; t < mc; t += blockDim.x) {
+      const int c = t >= gc.C;
+      int X, Y, Z;
+      if (split_decode(gc, c, t - (uint32_t)c * gc.C, X, Y, Z))
+        rhsc[t] = restrict_node(res, g, X, Y, Z, T(1));
+      xc[t] = T(0);
+    }
+    __syncthreads();
+    coarse_level<T, L + 1>(a, sm);
+    for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) {
+      const int c = t >= g.C;
+      int xx, yy, zz;
+      if (split_decode(g, c, t - (uint32_t)c * g.C, xx, yy, zz))
+        x[t] += prolong_node(xc, gc, xx, yy, zz);
+    }
+    __syncthreads();
+    block_relax(x, rhs, scr, g, a.pw, a.post);
+  }
+}
+
+// One workgroup runs the whole V-cycle below (and including) the top level it is given,
+// with all levels resident in LDS.
 template <class T>
 __global__ __launch_bounds__(1024) void k_coarse_vcycle(CoarseArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* sm = reinterpret_cast<T*>(smem_raw);
-  LevelPtrs<T> L[kMaxCoarseLevels];
-  size_t off = 0;
-  for (int l = 0; l < a.count; ++l) {
-    const uint32_t m = 2u * a.lv[l].g.C;
-    L[l].g = a.lv[l].g;
-    L[l].x = sm + off; off += m;
-    L[l].rhs = sm + off; off += m;
-    L[l].screen = sm + off; off += m;
-    L[l].res = sm + off; off += m;
-  }
   const uint32_t m0 = 2u * a.lv[0].g.C;
+  T* x0 = sm + a.off[0];
   for (uint32_t i = threadIdx.x; i < m0; i += blockDim.x) {
-    L[0].x[i] = a.zero_top ? T(0) : a.lv[0].x[i];
-    L[0].rhs[i] = a.lv[0].rhs[i];
+    x0[i] = a.zero_top ? T(0) : a.lv[0].x[i];
+    x0[m0 + i] = a.lv[0].rhs[i];
   }
   for (int l = 0; l < a.count; ++l) {
     const uint32_t m = 2u * a.lv[l].g.C;
-    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) L[l].screen[i] = a.lv[l].screen[i];
+    T* scr = sm + a.off[l] + 2 * m;
+    const T* src = a.lv[l].screen;
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) scr[i] = src[i];
   }
   __syncthreads();
-  const T pw = a.pw;
-  for (int l = 0; l + 1 < a.count; ++l) {
-    block_relax(L[l], pw, a.pre);
-    const Geom& g = L[l].g;
-    const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
-    for (uint32_t t = threadIdx.x; t < 2u * g.C; t += blockDim.x) {
-      const int c = t >= g.C;
-      int x, y, z;
-      if (split_decode(g, c, t - (uint32_t)c * g.C, x, y, z))
-        L[l].res[t] = residual_node(L[l].x, L[l].rhs, L[l].screen, g, t, x, y, z, pw, h2);
-    }
-    __syncthreads();
-    const Geom& gc = L[l + 1].g;
-    for (uint32_t t = threadIdx.x; t < 2u * gc.C; t += blockDim.x) {
-      const int c = t >= gc.C;
-      int X, Y, Z;
-      if (split_decode(gc, c, t - (uint32_t)c * gc.C, X, Y, Z))
-        L[l + 1].rhs[t] = restrict_node(L[l].res, g, X, Y, Z, T(1));
-      L[l + 1].x[t] = T(0);
-    }
-    __syncthreads();
-  }
-  block_relax(L[a.count - 1], pw, 40);  // core.py:152-156
-  for (int l = a.count - 2; l >= 0; --l) {
-    const Geom& g = L[l].g;
-    for (uint32_t t = threadIdx.x; t < 2u * g.C; t += blockDim.x) {
-      const int c = t >= g.C;
-      int x, y, z;
-      if (split_decode(g, c, t - (uint32_t)c * g.C, x, y, z))
-        L[l].x[t] += prolong_node(L[l + 1].x, L[l + 1].g, x, y, z);
-    }
-    __syncthreads();
-    block_relax(L[l], pw, a.post);
-  }
-  for (uint32_t i = threadIdx.x; i < m0; i += blockDim.x) a.lv[0].x[i] = L[0].x[i];
+  coarse_level<T, 0>(a, sm);
+  T* dst = a.lv[0].x;
+  for (uint32_t i = threadIdx.x; i < m0; i += blockDim.x) dst[i] = x0[i];
 }
 
 }  // namespace
@@ -401,7 +413,16 @@ bool launch_coarse_vcycle(const LevelPtrs<T>* levels, int count, bool zero_top, 
     attr_set = true;
   }
   CoarseArgs<T> a;
-  for (int l = 0; l < count; ++l) a.lv[l] = levels[l];
+  uint32_t off = 0;
+  for (int l = 0; l < count; ++l) {
+    a.lv[l] = levels[l];
+    a.off[l] = off;
+    off += 4u * 2u * levels[l].g.C;
+  }
+  for (int l = count; l < kMaxCoarseLevels; ++l) {
+    a.lv[l] = levels[count - 1];
+    a.off[l] = 0;
+  }
   a.count = count;
   a.zero_top = zero_top ? 1 : 0;
   a.pw = (T)pw;

@@ -61,15 +61,36 @@ __global__ __launch_bounds__(kThreads) void k_bbox_partials(const double* pts, u
 }
 
 // params: [origin x, y, z, width]
-__global__ void k_bbox_finish(const double* part, int nparts, double scale, double* params) {
-  if (threadIdx.x != 0) return;
+__global__ __launch_bounds__(kThreads) void k_bbox_finish(const double* part, int nparts,
+                                                          double scale, double* params) {
+  __shared__ double lds[6][kThreads];
   double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int b = 0; b < nparts; ++b)
+  for (int b = threadIdx.x; b < nparts; b += kThreads)
     for (int a = 0; a < 3; ++a) {
       double lo = part[6 * b + a], hi = part[6 * b + 3 + a];
       mn[a] = lo < mn[a] ? lo : mn[a];
       mx[a] = hi > mx[a] ? hi : mx[a];
     }
+  for (int a = 0; a < 3; ++a) {
+    lds[a][threadIdx.x] = mn[a];
+    lds[3 + a][threadIdx.x] = mx[a];
+  }
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int a = 0; a < 3; ++a) {
+        double o = lds[a][threadIdx.x + w];
+        if (o < lds[a][threadIdx.x]) lds[a][threadIdx.x] = o;
+        o = lds[3 + a][threadIdx.x + w];
+        if (o > lds[3 + a][threadIdx.x]) lds[3 + a][threadIdx.x] = o;
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  for (int a = 0; a < 3; ++a) {
+    mn[a] = lds[a][0];
+    mx[a] = lds[3 + a][0];
+  }
   double ext = mx[0] - mn[0];
   for (int a = 1; a < 3; ++a) ext = (mx[a] - mn[a]) > ext ? (mx[a] - mn[a]) : ext;
   double width = ext * scale;
@@ -440,14 +461,26 @@ __global__ __launch_bounds__(kThreads) void k_iso_partials(const double* field,
   }
 }
 
-__global__ void k_iso_finish(const double* part, int nparts, double* iso) {
-  if (threadIdx.x != 0) return;
+// fixed summation order: thread t sums partials t, t+256, ...; then a fixed tree
+__global__ __launch_bounds__(kThreads) void k_iso_finish(const double* part, int nparts,
+                                                         double* iso) {
+  __shared__ double la[kThreads], lb[kThreads];
   double a = 0.0, b = 0.0;
-  for (int i = 0; i < nparts; ++i) {
+  for (int i = threadIdx.x; i < nparts; i += kThreads) {
     a += part[2 * i];
     b += part[2 * i + 1];
   }
-  iso[0] = b > 0.0 ? a / b : 0.0;
+  la[threadIdx.x] = a;
+  lb[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      la[threadIdx.x] += la[threadIdx.x + w];
+      lb[threadIdx.x] += lb[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) iso[0] = lb[0] > 0.0 ? la[0] / lb[0] : 0.0;
 }
 
 }  // namespace
@@ -462,7 +495,7 @@ void launch_normalize(const double* pts, uint32_t P, double scale, double* part,
   const int nb = bbox_partial_count(P);
   hipLaunchKernelGGL(k_bbox_partials, dim3(nb), dim3(kThreads), 0, s, pts, P, part);
   MPR_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_bbox_finish, dim3(1), dim3(64), 0, s, part, nb, scale, params);
+  hipLaunchKernelGGL(k_bbox_finish, dim3(1), dim3(kThreads), 0, s, part, nb, scale, params);
   MPR_LAUNCH_CHECK();
   if (P) {
     hipLaunchKernelGGL(k_normalize, dim3(blocks_for(3ull * P, kThreads)), dim3(kThreads), 0, s,
@@ -581,7 +614,7 @@ void launch_iso(const double* field, const double* pts, const double* nrm, uint3
   hipLaunchKernelGGL(k_iso_partials, dim3(nb), dim3(kThreads), 0, s, field, pts, nrm, P, n,
                      confidence, part);
   MPR_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_iso_finish, dim3(1), dim3(64), 0, s, part, nb, iso);
+  hipLaunchKernelGGL(k_iso_finish, dim3(1), dim3(kThreads), 0, s, part, nb, iso);
   MPR_LAUNCH_CHECK();
 }
 

@@ -13,18 +13,31 @@ namespace mpr {
 // Neumann 7-point neighbour sum in the order x-1, x+1, y-1, y+1, z-1, z+1 and the number
 // of in-grid neighbours (capi.mojo:282-306). All neighbours of (x,y,z) have the other
 // colour, so with the split layout they live in colour block 1-c.
+//
+// All six loads are issued unconditionally from in-grid addresses (a missing neighbour
+// reads the node's own slot instead) and only the additions are guarded, so a lane keeps
+// six loads in flight instead of waiting on each in turn; the guarded sum is the same
+// sequence of roundings as skipping the term.
 template <class T>
 MPR_HD T neighbour_sum(const T* X, const Geom& g, int x, int y, int z, int& deg) {
   const int n = g.n;
+  const bool bx0 = x > 0, bx1 = x + 1 < n, by0 = y > 0, by1 = y + 1 < n, bz0 = z > 0,
+             bz1 = z + 1 < n;
+  const uint32_t self = split_index(g, x, y, z);
+  const T v0 = X[bx0 ? split_index(g, x - 1, y, z) : self];
+  const T v1 = X[bx1 ? split_index(g, x + 1, y, z) : self];
+  const T v2 = X[by0 ? split_index(g, x, y - 1, z) : self];
+  const T v3 = X[by1 ? split_index(g, x, y + 1, z) : self];
+  const T v4 = X[bz0 ? split_index(g, x, y, z - 1) : self];
+  const T v5 = X[bz1 ? split_index(g, x, y, z + 1) : self];
   T s = T(0);
-  int d = 0;
-  if (x > 0) { s += X[split_index(g, x - 1, y, z)]; ++d; }
-  if (x + 1 < n) { s += X[split_index(g, x + 1, y, z)]; ++d; }
-  if (y > 0) { s += X[split_index(g, x, y - 1, z)]; ++d; }
-  if (y + 1 < n) { s += X[split_index(g, x, y + 1, z)]; ++d; }
-  if (z > 0) { s += X[split_index(g, x, y, z - 1)]; ++d; }
-  if (z + 1 < n) { s += X[split_index(g, x, y, z + 1)]; ++d; }
-  deg = d;
+  if (bx0) s += v0;
+  if (bx1) s += v1;
+  if (by0) s += v2;
+  if (by1) s += v3;
+  if (bz0) s += v4;
+  if (bz1) s += v5;
+  deg = (int)bx0 + (int)bx1 + (int)by0 + (int)by1 + (int)bz0 + (int)bz1;
   return s;
 }
 
@@ -56,25 +69,50 @@ MPR_HD T residual_node(const T* X, const T* R, const T* S, const Geom& g, uint32
 // centre and 1 at the sides per axis, terms in (oz, oy, ox) order, divided by the sum of
 // the in-grid weights, then multiplied by `scale` (1 for residuals, 4 for the screening
 // pyramid of core.py:267-269; both scalings are exact).
+//
+// As in neighbour_sum, the 27 loads use clamped (in-grid) coordinates and only the
+// accumulation is guarded, so all loads can be in flight at once.
 template <class T>
 MPR_HD T restrict_node(const T* F, const Geom& gf, int X, int Y, int Z, T scale) {
   const int nf = gf.n;
-  T sum = T(0), wsum = T(0);
+  T v[27];
+#pragma unroll
   for (int oz = -1; oz <= 1; ++oz) {
     int zz = 2 * Z + oz;
-    if (zz < 0 || zz >= nf) continue;
-    T wz = oz == 0 ? T(2) : T(1);
+    zz = zz < 0 ? 0 : (zz >= nf ? nf - 1 : zz);
+#pragma unroll
     for (int oy = -1; oy <= 1; ++oy) {
       int yy = 2 * Y + oy;
-      if (yy < 0 || yy >= nf) continue;
-      T wy = oy == 0 ? T(2) : T(1);
+      yy = yy < 0 ? 0 : (yy >= nf ? nf - 1 : yy);
+#pragma unroll
       for (int ox = -1; ox <= 1; ++ox) {
         int xx = 2 * X + ox;
-        if (xx < 0 || xx >= nf) continue;
-        T wx = ox == 0 ? T(2) : T(1);
-        T w = wx * wy * wz;
-        sum += F[split_index(gf, xx, yy, zz)] * w;
-        wsum += w;
+        xx = xx < 0 ? 0 : (xx >= nf ? nf - 1 : xx);
+        v[(oz + 1) * 9 + (oy + 1) * 3 + (ox + 1)] = F[split_index(gf, xx, yy, zz)];
+      }
+    }
+  }
+  T sum = T(0), wsum = T(0);
+#pragma unroll
+  for (int oz = -1; oz <= 1; ++oz) {
+    const int zz = 2 * Z + oz;
+    const bool inz = zz >= 0 && zz < nf;
+    const T wz = oz == 0 ? T(2) : T(1);
+#pragma unroll
+    for (int oy = -1; oy <= 1; ++oy) {
+      const int yy = 2 * Y + oy;
+      const bool iny = inz && yy >= 0 && yy < nf;
+      const T wy = oy == 0 ? T(2) : T(1);
+#pragma unroll
+      for (int ox = -1; ox <= 1; ++ox) {
+        const int xx = 2 * X + ox;
+        const bool in = iny && xx >= 0 && xx < nf;
+        const T wx = ox == 0 ? T(2) : T(1);
+        const T w = wx * wy * wz;
+        if (in) {
+          sum += v[(oz + 1) * 9 + (oy + 1) * 3 + (ox + 1)] * w;
+          wsum += w;
+        }
       }
     }
   }
