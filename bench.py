@@ -2,23 +2,30 @@
 """Reconstructions per second and p50/p99 latency of depth-8 screened Poisson
 reconstruction on one MI355X, FP32-mixed against FP64.
 
-Workload (one "step" = one full reconstruction): 200,000 oriented points on a torus
-(R = 0.6, r = 0.25), Gaussian position noise 0.002 x bbox diagonal, 5 degree normal noise,
-normal lengths U(0.5, 1.5) with confidence weighting; depth 8 (257^3 grid), reference
-defaults (point weight 4, 4 V-cycles of 3/3 smoothing, 40 coarsest iterations, 1.5 samples
-per node, scale 1.1). The points are resident in HBM before the timed region; each step
-normalises them, splats, assembles, solves, computes the iso value and extracts the mesh
-on the device and returns the counts to the host (the mesh stays in HBM).
+Headline workload (one "step" = one full reconstruction): 200,000 oriented points on a
+torus (R = 0.6, r = 0.25), Gaussian position noise 0.002 x bbox diagonal, 5 degree normal
+noise, normal lengths U(0.5, 1.5) with confidence weighting; depth 8 (257^3 grid),
+reference defaults (point weight 4, 4 V-cycles of 3/3 smoothing, 40 coarsest iterations,
+1.5 samples per node, scale 1.1). Points are resident in HBM before the timed region;
+each reconstruction normalises them, splats, assembles, solves, computes the iso value and
+extracts the mesh on the device and returns its counts to the host (the mesh stays in HBM).
 
-Prints one JSON line. `value` is the mixed-precision recon/s; the FP64 figures, the
-Hausdorff bound between the two meshes, the dominant kernel's roofline and the CPU
-oracle baseline are carried beside it.
+Modes, each timed over --steps iterations after --warmup:
+  fp64        the plain FP64 solve, one request at a time (job run + wait)
+  mixed       FP32 V-cycles inside FP64 defect correction, one request at a time
+  mixed_graph the same recorded once as a HIP graph and replayed
+  mixed_batch --batch depth-8 reconstructions (different seeds) as one batched graph launch
+`value` is the mixed_batch recon/s. Also reported: the 64-instance batch of uneven depths
+and sizes (depths U{5..8}, points log-U[2k, 200k], sphere/torus/superquadric) with its
+per-instance error against FP64, a served request stream, the dominant kernel's roofline
+and the CPU oracle baseline. Prints one JSON line.
 """
 
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -32,31 +39,34 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 
 
+def log(*args):
+    print("[bench]", *args, file=sys.stderr, flush=True)
+
+
 def hausdorff_bound(va: np.ndarray, vb: np.ndarray) -> float:
     """Symmetric max over vertices of the distance to the nearest vertex of the other mesh
     (an upper bound of the vertex-to-surface Hausdorff distance)."""
     from scipy.spatial import cKDTree
 
     if not len(va) or not len(vb):
-        return float("inf")
-    d1 = cKDTree(vb).query(va)[0].max()
-    d2 = cKDTree(va).query(vb)[0].max()
-    return float(max(d1, d2))
+        return 0.0 if len(va) == len(vb) else float("inf")
+    return float(max(cKDTree(vb).query(va)[0].max(), cKDTree(va).query(vb)[0].max()))
 
 
-def timed_steps(job, params, steps: int, warmup: int):
+def timed(step, steps: int, warmup: int):
     for _ in range(warmup):
-        job.run(params)
-        job.wait()
+        step()
     lat = []
     t0 = time.perf_counter()
     for _ in range(steps):
         s = time.perf_counter()
-        job.run(params)
-        job.wait()
+        step()
         lat.append(time.perf_counter() - s)
-    total = time.perf_counter() - t0
-    return total, np.array(lat)
+    return time.perf_counter() - t0, np.array(lat)
+
+
+def pct(lat, q):
+    return round(float(np.percentile(lat, q)) * 1e3, 4)
 
 
 def cpu_baseline(points, normals, depth: int, budget_s: float = 20.0) -> dict:
@@ -78,6 +88,97 @@ def cpu_baseline(points, normals, depth: int, budget_s: float = 20.0) -> dict:
     }
 
 
+def uneven_batch(count: int, seed: int):
+    """64-instance workload: depths U{5..8}, points log-uniform [2k, 200k], shapes cycle
+    sphere -> torus -> superquadric, position noise U[0, 0.003] x diagonal."""
+    from mojo_poissonrecon import shapes
+
+    rng = np.random.default_rng(seed)
+    makers = (shapes.sphere, shapes.torus, shapes.superquadric)
+    clouds, depths = [], []
+    for i in range(count):
+        npts = int(round(math.exp(rng.uniform(math.log(2e3), math.log(2e5)))))
+        p, n = makers[i % 3](npts, seed=seed * 1000 + i)
+        p, n = shapes.perturb(p, n, seed=seed * 1000 + i, sigma=float(rng.uniform(0, 0.003)))
+        clouds.append((p, n))
+        depths.append(int(rng.integers(5, 9)))
+    return clouds, depths
+
+
+def run_batch_mode(clouds, depths, precision, reps):
+    """Records the batched run as one graph, replays it; returns (seconds per launch, meshes)."""
+    from mojo_poissonrecon.batch import Batch
+    from mojo_poissonrecon.core import Job, mesh_from_job, prepare_reconstruct
+
+    reqs = [prepare_reconstruct(p, n, depth=d, precision=precision) for (p, n), d in
+            zip(clouds, depths)]
+    jobs = [Job(r.depth, len(r.points), r.precision) for r in reqs]
+    for j, r in zip(jobs, reqs):
+        j.set_points(r.points, r.normals)
+    b = Batch(jobs)
+    b.capture([r.params for r in reqs])
+    b.launch()
+    b.wait()
+    t0 = time.perf_counter()
+    lat = []
+    for _ in range(reps):
+        s = time.perf_counter()
+        b.launch()
+        results = b.wait()
+        lat.append(time.perf_counter() - s)
+    total = time.perf_counter() - t0
+    meshes = [mesh_from_job(j, res, len(r.points)) for j, res, r in zip(jobs, results, reqs)]
+    b.close()
+    for j in jobs:
+        j.close()
+    return total / reps, np.array(lat), meshes
+
+
+def serve_stream(requests: int, qps: float, seed: int):
+    """Poisson arrivals at `qps`; each request a cloud of 5k-500k points (log-uniform) on a
+    random shape, randomly rotated and scaled into [-1,1]^3, depth from {6,7,8,9} weighted
+    4:3:2:1. A pool of pre-generated clouds is rotated per request to bound host time."""
+    from mojo_poissonrecon import shapes
+    from mojo_poissonrecon.serve import ReconstructionServer
+
+    rng = np.random.default_rng(seed)
+    makers = (shapes.sphere, shapes.torus, shapes.superquadric)
+    pool = []
+    for i in range(24):
+        npts = int(round(math.exp(rng.uniform(math.log(5e3), math.log(5e5)))))
+        p, n = makers[i % 3](npts, seed=7000 + i)
+        pool.append(shapes.perturb(p, n, seed=7000 + i, sigma=0.001, degrees=2.0))
+    gaps = rng.exponential(1.0 / qps, requests)
+    arrivals = np.cumsum(gaps)
+    depth_choice = rng.choice([6, 7, 8, 9], size=requests, p=[0.4, 0.3, 0.2, 0.1])
+    picks = rng.integers(0, len(pool), requests)
+    server = ReconstructionServer(max_inflight=16, max_batch=8, precision="mixed")
+    done = []
+    start = time.perf_counter()
+    nxt = 0
+    while len(done) < requests:
+        now = time.perf_counter() - start
+        while nxt < requests and arrivals[nxt] <= now:
+            p, n = pool[picks[nxt]]
+            rot = shapes.random_rotation(rng)
+            s = float(rng.uniform(0.5, 1.0))
+            pts = p @ rot.T
+            pts = pts / np.abs(pts).max() * s
+            server.submit(pts, n @ rot.T, depth=int(depth_choice[nxt]), arrival=start + arrivals[nxt])
+            nxt += 1
+        done += server.step()
+    wall = time.perf_counter() - start
+    server.close()
+    lat = np.array([r.latency for r in done])
+    return {
+        "requests": requests, "qps_offered": qps, "recon_per_s": round(requests / wall, 2),
+        "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "batches": server.stats.batches,
+        "peak_inflight": server.stats.peak_inflight,
+        "peak_hbm_gb": round(server.stats.peak_bytes / 1e9, 2),
+        "depth_mix": "6:7:8:9 = 4:3:2:1", "points": "5k-500k log-uniform",
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,7 +186,12 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--points", type=int, default=200_000)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--instances", type=int, default=64)
+    ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-qps", type=float, default=100.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the 64-batch and serving runs")
     ap.add_argument("--kernel-reps", type=int, default=40)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -103,49 +209,92 @@ def main() -> int:
     points, normals = shapes.torus_workload(args.points, seed=0)
     stages = (_lib.STAGE_NORMALIZE | _lib.STAGE_SOLVE | _lib.STAGE_EXTRACT | _lib.STAGE_WORLD)
     params = Job.params(stages, confidence=True)
-    results = {}
+    modes = {}
     meshes = {}
     jobs = {}
     for name, prec in (("fp64", _lib.PRECISION_FP64), ("mixed", _lib.PRECISION_MIXED)):
         job = Job(args.depth, len(points), prec)
         job.set_points(points, normals)
-        total, lat = timed_steps(job, params, args.steps, args.warmup)
+
+        def step(job=job):
+            job.run(params)
+            job.wait()
+
+        total, lat = timed(step, args.steps, args.warmup)
         r = job.wait()
         meshes[name] = job.fetch_mesh(r)
-        results[name] = dict(total=total, lat=lat, iso=r.iso_value, res=r.residual_norm,
-                             nv=int(r.vertex_count), nf=int(r.face_count))
+        modes[name] = dict(value=args.steps / total, p50=pct(lat, 50), p99=pct(lat, 99),
+                           nv=int(r.vertex_count), iso=r.iso_value, res=r.residual_norm)
         jobs[name] = job
-        print(f"[bench] {name}: {args.steps / total:.2f} recon/s, p50 "
-              f"{np.percentile(lat, 50) * 1e3:.2f} ms, {r.vertex_count} vertices", file=sys.stderr,
-              flush=True)
+        log(f"{name}: {modes[name]['value']:.2f} recon/s, p50 {modes[name]['p50']} ms")
+    # graph replay of the mixed single request
+    jm = jobs["mixed"]
+    jm.capture(params)
+
+    def gstep():
+        jm.launch()
+        jm.wait()
+
+    total, lat = timed(gstep, args.steps, args.warmup)
+    modes["mixed_graph"] = dict(value=args.steps / total, p50=pct(lat, 50), p99=pct(lat, 99))
+    log(f"mixed_graph: {modes['mixed_graph']['value']:.2f} recon/s")
+    # batched: --batch depth-8 reconstructions of the workload family in one graph launch
+    bclouds = [shapes.torus_workload(args.points, seed=s) for s in range(args.batch)]
+    per_launch, blat, bmeshes = run_batch_mode(bclouds, [args.depth] * args.batch, "mixed",
+                                               max(2, args.steps // 2))
+    modes["mixed_batch"] = dict(value=args.batch / per_launch, p50=pct(blat, 50),
+                                p99=pct(blat, 99))
+    log(f"mixed_batch x{args.batch}: {modes['mixed_batch']['value']:.2f} recon/s")
 
     diag = float(np.linalg.norm(points.max(axis=0) - points.min(axis=0)))
     hd = hausdorff_bound(meshes["mixed"][0], meshes["fp64"][0]) / diag
 
     # dominant kernels, timed on the jobs' own streams with HIP events
     kernels = {}
-    for name, kid, job in (("rb_sweep_f64", 0, jobs["fp64"]), ("rb_sweep_f32", 1, jobs["mixed"]),
-                           ("residual_f64", 2, jobs["fp64"]), ("restrict_f64", 3, jobs["fp64"]),
-                           ("prolong_f64", 4, jobs["fp64"])):
+    for kname, kid, job in (("rb_sweep_f64", 0, jobs["fp64"]), ("rb_sweep_f32", 1, jobs["mixed"]),
+                            ("residual_f64", 2, jobs["fp64"]), ("restrict_f64", 3, jobs["fp64"]),
+                            ("prolong_f64", 4, jobs["fp64"]), ("residual_f32", 5, jobs["mixed"]),
+                            ("defect_f64_f32", 6, jobs["mixed"])):
         ms, nbytes = _lib.ctypes.c_double(), _lib.ctypes.c_double()
         _lib.check(_lib.lib().mpr_job_time_kernel(job.handle, kid, args.kernel_reps,
                                                    _lib.ctypes.byref(ms),
                                                    _lib.ctypes.byref(nbytes)),
                    "mpr_job_time_kernel")
-        kernels[name] = dict(ms=ms.value, bytes=nbytes.value,
-                             gbs=nbytes.value / (ms.value * 1e-3) / 1e9)
+        kernels[kname] = dict(ms=ms.value, bytes=nbytes.value,
+                              gbs=nbytes.value / (ms.value * 1e-3) / 1e9)
+    for job in jobs.values():
+        job.close()
     dom = kernels["rb_sweep_f32"]
 
-    mixed, fp64 = results["mixed"], results["fp64"]
-    value = args.steps / mixed["total"]
+    extras = {}
+    if not args.no_extras:
+        clouds, depths = uneven_batch(args.instances, seed=1)
+        t32, _, m32 = run_batch_mode(clouds, depths, "mixed", 3)
+        t64, _, m64 = run_batch_mode(clouds, depths, "fp64", 3)
+        errs = []
+        for (p, _), a, b in zip(clouds, m32, m64):
+            d = float(np.linalg.norm(p.max(axis=0) - p.min(axis=0)))
+            errs.append(hausdorff_bound(a.vertices, b.vertices) / d)
+        extras["batch64"] = {
+            "instances": args.instances, "depths": "U{5,6,7,8}", "points": "2k-200k log-uniform",
+            "mixed_recon_per_s": round(args.instances / t32, 2),
+            "fp64_recon_per_s": round(args.instances / t64, 2),
+            "max_hausdorff_mixed_vs_fp64_over_diag": max(errs),
+            "launch_ms": round(t32 * 1e3, 3),
+        }
+        log("batch64:", extras["batch64"])
+        extras["serve"] = serve_stream(args.serve_requests, args.serve_qps, seed=2)
+        log("serve:", extras["serve"])
+
+    mixed_b, fp64 = modes["mixed_batch"], modes["fp64"]
     line = {
         "metric": "recon/s + p50/p99 latency, depth 8, FP32-mixed vs FP64 at fixed Hausdorff",
-        "value": round(value, 3),
+        "value": round(mixed_b["value"], 3),
         "unit": "recon/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(mixed["total"] / args.steps * 1e3, 4),
+        "ms_per_step": round(1e3 / mixed_b["value"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -154,22 +303,22 @@ def main() -> int:
         "config": {
             "workload": f"torus {len(points)} points, depth {args.depth} (257^3), sigma 0.002 diag, "
                         "5 deg normal noise, normal lengths U(0.5,1.5), confidence on",
+            "mode": f"mixed precision, {args.batch} reconstructions per batched graph launch",
             "depth": args.depth,
             "points": len(points),
             "parallelism": "single GPU",
         },
-        "p50_ms": round(float(np.percentile(mixed["lat"], 50)) * 1e3, 4),
-        "p99_ms": round(float(np.percentile(mixed["lat"], 99)) * 1e3, 4),
-        "fp64": {
-            "value": round(args.steps / fp64["total"], 3),
-            "p50_ms": round(float(np.percentile(fp64["lat"], 50)) * 1e3, 4),
-            "p99_ms": round(float(np.percentile(fp64["lat"], 99)) * 1e3, 4),
-        },
-        "speedup_mixed_vs_fp64": round(fp64["total"] / mixed["total"], 3),
+        "p50_ms": mixed_b["p50"],
+        "p99_ms": mixed_b["p99"],
+        "modes": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv)
+                      for kk, vv in v.items() if kk in ("value", "p50", "p99")}
+                  for k, v in modes.items()},
+        "speedup_vs_plain_fp64": round(mixed_b["value"] / fp64["value"], 3),
         "hausdorff_mixed_vs_fp64_over_diag": hd,
-        "mesh": {"vertices": mixed["nv"], "faces": mixed["nf"], "fp64_vertices": fp64["nv"]},
+        "mesh": {"vertices": modes["mixed"]["nv"], "fp64_vertices": fp64["nv"]},
         "roofline": {
-            "kernel": "rb_sweep_f32 (k_relax_color<float>, finest level, one colour)",
+            "kernel": "rb_sweep_f32 (k_relax_color<float> / kb_relax_color<float>, finest level, "
+                      "one colour)",
             "bound": "hbm",
             "achieved": round(dom["gbs"], 1),
             "peak": HBM_PEAK_GBS,
@@ -180,11 +329,10 @@ def main() -> int:
         "kernels": {k: {"us": round(v["ms"] * 1e3, 2), "GBps": round(v["gbs"], 1)}
                     for k, v in kernels.items()},
     }
+    line.update(extras)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(points, normals, args.depth)
     print(json.dumps(line), flush=True)
-    for job in jobs.values():
-        job.close()
     return 0
 
 

@@ -158,6 +158,26 @@ int mpr_job_launch(mpr_job *job);
 int mpr_job_device_points(mpr_job *job, double **points, double **normals);
 int mpr_job_set_count(mpr_job *job, int64_t P);
 
+/* ---- batches: many jobs, one launch per V-cycle step ----------------------------------
+ * No reference counterpart (the reference solves one request per call). The jobs keep
+ * their own buffers; the batch owns a stream and device pointer tables so that every
+ * V-cycle step is one launch over all jobs that have a grid of that size (jobs may have
+ * different depths). Each job's arithmetic is exactly its solo run's. All jobs of a batch
+ * share a precision; per-job params may differ except cycles, pre/post smoothing and
+ * point_weight, which must be equal. While a batch runs, its jobs must not run alone. */
+typedef struct mpr_batch mpr_batch;
+int mpr_batch_create(mpr_job *const *jobs, int32_t count, mpr_batch **out);
+int mpr_batch_destroy(mpr_batch *batch);
+/* params: one per job */
+int mpr_batch_run(mpr_batch *batch, const mpr_params *params);
+/* records the whole batched run (setup, V-cycles, extraction, read-back of the scalars)
+ * as one HIP graph; mpr_batch_launch replays it in a single launch */
+int mpr_batch_capture(mpr_batch *batch, const mpr_params *params);
+int mpr_batch_launch(mpr_batch *batch);
+int mpr_batch_query(mpr_batch *batch);
+/* waits, resolves extraction overflow per job; results: one per job (may be NULL) */
+int mpr_batch_wait(mpr_batch *batch, mpr_result *results);
+
 /* Kernel timing on the job's own stream with HIP events: launches kernel `kernel` `reps`
  * times on the job's finest level (after a run) and returns the mean milliseconds per
  * launch and the algorithmic HBM bytes one launch moves. Kernels: 0 FP64 red-black colour

@@ -1,0 +1,123 @@
+// Batched multigrid: device pointer tables and the host-side V-cycle driver that runs the
+// V-cycles of many instances (of any depths) with one launch per step and size class.
+#pragma once
+
+#include <vector>
+
+#include "mpr_internal.h"
+
+namespace mpr {
+
+constexpr int kMaxCoarseClass = 4;                // coarse solve handles n <= 17
+constexpr int kCoarseThreads = 1024;              // launch bound of the coarse kernel
+constexpr size_t kCoarseLdsBytes = 160 * 1024;    // LDS per workgroup on gfx950
+
+// one instance's buffers for one V-cycle step at a size class and the next-coarser class
+template <class T>
+struct StepPtrs {
+  T* x;
+  const T* rhs;
+  const T* screen;
+  T* res;
+  T* xc;    // next-coarser x
+  T* rhsc;  // next-coarser rhs
+};
+
+// top-level buffers of one instance for the FP64 defect-correction loop
+struct TopPtrs {
+  double* x64;
+  const double* rhs64;
+  const double* screen64;
+  float* r32;
+  const float* e32;
+};
+
+// one instance of the LDS-resident coarse V-cycle: its top coarse class, whether that
+// level starts from zero, its top x/rhs and the screening weights of every coarse class
+template <class T>
+struct CoarseEntry {
+  int top;
+  int zero_top;
+  T* x;
+  const T* rhs;
+  const T* screen[kMaxCoarseClass + 1];
+};
+
+template <class T>
+struct CoarseArgsB {
+  Geom geom[kMaxCoarseClass + 1];
+  uint32_t off[kMaxCoarseClass + 1];  // LDS word offset of each class's x|rhs|screen|res
+  T pw;
+  int pre, post;
+};
+
+template <class T>
+void launch_b_relax(const StepPtrs<T>* tab, uint32_t count, const Geom& g, double pw,
+                    int iterations, hipStream_t s);
+template <class T>
+void launch_b_residual(const StepPtrs<T>* tab, uint32_t count, const Geom& g, double pw,
+                       hipStream_t s);
+template <class T>
+void launch_b_restrict(const StepPtrs<T>* tab, uint32_t count, const Geom& gf, const Geom& gc,
+                       bool zero_coarse, hipStream_t s);
+template <class T>
+void launch_b_prolong(const StepPtrs<T>* tab, uint32_t count, const Geom& gc, const Geom& gf,
+                      hipStream_t s);
+template <class T>
+void launch_b_fill_x(const StepPtrs<T>* tab, uint32_t count, const Geom& g, hipStream_t s);
+void launch_b_defect(const TopPtrs* tab, uint32_t count, const Geom& g, double pw, hipStream_t s);
+void launch_b_correct(const TopPtrs* tab, uint32_t count, const Geom& g, hipStream_t s);
+template <class T>
+void launch_b_coarse(const CoarseEntry<T>* tab, uint32_t count, int top_class, double pw, int pre,
+                     int post, hipStream_t s);
+// largest coarse class whose LDS-resident hierarchy fits one workgroup
+template <class T> int coarse_b_top_class();
+
+// Host description of one instance: levels[k] is size class depth - k (levels[0] is the
+// finest grid).
+template <class T>
+struct InstanceLevels {
+  int depth;
+  std::vector<LevelPtrs<T>> levels;
+};
+
+// Device tables + schedule for the V-cycles of a fixed set of instances.
+template <class T>
+class BatchedVcycle {
+ public:
+  // Builds and uploads the tables (synchronously) into `table_mem` (device memory of at
+  // least table_bytes(instances) bytes).
+  void build(const std::vector<InstanceLevels<T>>& inst, void* table_mem);
+  static size_t table_bytes(const std::vector<InstanceLevels<T>>& inst);
+  // One V-cycle of every instance (core.py:141-193). zero_tops: every instance's top x
+  // starts at zero (the correction equation of the defect-correction loop).
+  void run(double pw, int pre, int post, bool zero_tops, hipStream_t s) const;
+  int smax = 0, sc = 0;
+
+ private:
+  std::vector<Geom> geom_;                  // by size class
+  std::vector<StepPtrs<T>*> step_;          // by class: instances with depth >= class
+  std::vector<uint32_t> step_count_;
+  std::vector<StepPtrs<T>*> top_;           // by class: instances whose finest class it is
+  std::vector<uint32_t> top_count_;
+  CoarseEntry<T>* coarse_ = nullptr;        // all instances
+  CoarseEntry<T>* coarse_zero_ = nullptr;   // same, every top zeroed
+  uint32_t coarse_count_ = 0;
+  int coarse_class_ = 0;
+};
+
+// FP64 defect-correction tables: instances grouped by finest class.
+class DefectTables {
+ public:
+  void build(const std::vector<int>& depths, const std::vector<TopPtrs>& tops, void* table_mem);
+  static size_t table_bytes(size_t count) { return sizeof(TopPtrs) * count + 64 * 16; }
+  void defect(double pw, hipStream_t s) const;
+  void correct(hipStream_t s) const;
+
+ private:
+  std::vector<Geom> geom_;
+  std::vector<TopPtrs*> tab_;
+  std::vector<uint32_t> count_;
+};
+
+}  // namespace mpr

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "mpr_hip.h"
+#include "mpr_batch.h"
 #include "mpr_internal.h"
 
 namespace mpr {
@@ -72,41 +73,6 @@ struct HostScalars {
   double params[4];  // origin xyz, width
 };
 
-template <class T>
-int coarse_start(const std::vector<LevelPtrs<T>>& L) {
-  const int nl = (int)L.size();
-  int from = nl - 1;
-  std::vector<Geom> g(nl);
-  for (int l = 0; l < nl; ++l) g[l] = L[l].g;
-  while (from > 0 && nl - (from - 1) <= kMaxCoarseLevels &&
-         coarse_lds_bytes<T>(&g[from - 1], nl - (from - 1)) <= 160 * 1024)
-    --from;
-  return from;
-}
-
-// core.py:141-193 for levels [top, L.size()); multi-block kernels above `cfrom`, one
-// LDS-resident workgroup from `cfrom` down. `zero_top`: the top level's x starts at 0.
-template <class T>
-void vcycle(std::vector<LevelPtrs<T>>& L, int top, int cfrom, bool zero_top, double pw, int pre,
-            int post, hipStream_t s) {
-  int l = top;
-  if (zero_top && top < cfrom) launch_fill<T>(L[top].x, T(0), 2ull * L[top].g.C, s);
-  for (; l < cfrom; ++l) {
-    launch_relax<T>(L[l].x, L[l].rhs, L[l].screen, L[l].g, pw, pre, s);
-    launch_residual<T, T>(L[l].x, L[l].rhs, L[l].screen, L[l].res, L[l].g, pw, nullptr, s);
-    launch_restrict<T>(L[l].res, L[l].g, L[l + 1].rhs, L[l + 1].g, 1.0, s);
-    // solution[l+1].fill(0) (core.py:174); the coarse kernel zeroes its own top level
-    if (l + 1 < cfrom) launch_fill<T>(L[l + 1].x, T(0), 2ull * L[l + 1].g.C, s);
-  }
-  const bool zero_c = l > top || zero_top;
-  if (!launch_coarse_vcycle<T>(&L[l], (int)L.size() - l, zero_c, pw, pre, post, s))
-    throw HipError{"coarse levels do not fit in LDS"};
-  for (l = l - 1; l >= top; --l) {
-    launch_prolong_add<T>(L[l + 1].x, L[l + 1].g, L[l].x, L[l].g, s);
-    launch_relax<T>(L[l].x, L[l].rhs, L[l].screen, L[l].g, pw, post, s);
-  }
-}
-
 }  // namespace
 
 struct Job {
@@ -122,7 +88,10 @@ struct Job {
          *rhs = nullptr, *field = nullptr;
   std::vector<LevelPtrs<double>> L64;
   std::vector<LevelPtrs<float>> L32;
-  int cfrom64 = 0, cfrom32 = 0;
+  // this job's own V-cycle schedule (a batch of one) and defect-correction tables
+  BatchedVcycle<double> vc64;
+  BatchedVcycle<float> vc32;
+  DefectTables dtab;
   // scalars and partial sums
   unsigned long long* d_accepted = nullptr;
   double *d_iso = nullptr, *d_resnorm = nullptr, *d_params = nullptr;
@@ -151,10 +120,149 @@ struct Job {
           *o_ccount = nullptr, *o_leaf = nullptr;
 
   void plan(DevArena& a);
+  InstanceLevels<double> levels64() const { return {depth, L64}; }
+  InstanceLevels<float> levels32() const { return {depth, L32}; }
+  TopPtrs top_ptrs() const {
+    return {L64[0].x, L64[0].rhs, L64[0].screen, L32.empty() ? nullptr : L32[0].rhs,
+            L32.empty() ? nullptr : L32[0].x};
+  }
+  const double* unit_points(const mpr_params& p) const {
+    return (p.stages & MPR_STAGE_NORMALIZE) ? unit : pts;
+  }
+  // pipeline stages; all enqueue on `s` without host synchronisation
+  void setup(const mpr_params& p, hipStream_t s);
+  void finish(const mpr_params& p, hipStream_t s);
+  void enqueue_extract(bool world, hipStream_t s);
+  void readback(hipStream_t s);
   void enqueue(const mpr_params& p);
-  void enqueue_extract(bool world);
+  // after the stream finished: if the mesh outgrew its buffers, grow them to the exact
+  // counts and re-run the extraction on `s` (synchronously)
+  void resolve_overflow(hipStream_t s);
+  void result(mpr_result* r) const;
+  uint64_t mesh_generation = 0;  // bumped when the mesh buffers move (graphs must re-record)
   ~Job();
 };
+
+void run_cycles(const BatchedVcycle<double>& v64, const BatchedVcycle<float>* v32,
+                const DefectTables* dt, const mpr_params& p, hipStream_t s);
+
+void Job::resolve_overflow(hipStream_t s) {
+  if (!ran || !(last.stages & MPR_STAGE_EXTRACT)) return;
+  const uint64_t nv = host->counts[0], nf = 2ull * host->counts[1];
+  if (nv <= vcap && nf <= fcap) return;
+  const uint64_t nvc = std::max<uint64_t>(nv, vcap), nfc = std::max<uint64_t>(nf, fcap);
+  void *v = nullptr, *d = nullptr, *f = nullptr;
+  MPR_HIP(hipMalloc(&v, 24 * nvc));
+  MPR_HIP(hipMalloc(&d, 8 * nvc));
+  MPR_HIP(hipMalloc(&f, 24 * nfc));
+  mesh_bufs.push_back(v);
+  mesh_bufs.push_back(d);
+  mesh_bufs.push_back(f);
+  verts = (double*)v;
+  vdens = (double*)d;
+  faces = (int64_t*)f;
+  vcap = nvc;
+  fcap = nfc;
+  ++mesh_generation;
+  if (exec) {  // the captured graph points at the old buffers
+    MPR_HIP(hipGraphExecDestroy(exec));
+    MPR_HIP(hipGraphDestroy(graph));
+    exec = nullptr;
+    graph = nullptr;
+  }
+  enqueue_extract((last.stages & MPR_STAGE_WORLD) != 0, s);
+  MPR_HIP(hipMemcpyAsync(host->counts, d_counts, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  MPR_HIP(hipStreamSynchronize(s));
+}
+
+void Job::result(mpr_result* r) const {
+  if (!r) return;
+  r->accepted = (int64_t)host->accepted;
+  r->vertex_count = host->counts[0];
+  r->face_count = 2ll * host->counts[1];
+  r->iso_value = host->iso;
+  r->residual_norm = host->resnorm;
+  for (int a = 0; a < 3; ++a) r->origin[a] = host->params[a];
+  r->width = host->params[3];
+}
+
+// A fixed set of jobs whose V-cycles run together: one launch per V-cycle step and grid
+// size class covers every job (mpr_batch in include/mpr_hip.h).
+struct Batch {
+  std::vector<Job*> jobs;
+  int precision = 0;
+  hipStream_t stream = nullptr;
+  DevArena arena;
+  BatchedVcycle<double> vc64;
+  BatchedVcycle<float> vc32;
+  DefectTables dtab;
+  std::vector<mpr_params> last;
+  std::vector<uint64_t> generations;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  void build();
+  void enqueue(const mpr_params* p);
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  ~Batch() {
+    drop_graph();
+    arena.release();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+void Batch::build() {
+  std::vector<InstanceLevels<double>> i64;
+  std::vector<InstanceLevels<float>> i32;
+  std::vector<int> depths;
+  std::vector<TopPtrs> tops;
+  for (Job* j : jobs) {
+    i64.push_back(j->levels64());
+    if (precision == MPR_PRECISION_MIXED) {
+      i32.push_back(j->levels32());
+      depths.push_back(j->depth);
+      tops.push_back(j->top_ptrs());
+    }
+  }
+  vc64.build(i64, arena.alloc<unsigned char>(BatchedVcycle<double>::table_bytes(i64)));
+  if (precision == MPR_PRECISION_MIXED) {
+    vc32.build(i32, arena.alloc<unsigned char>(BatchedVcycle<float>::table_bytes(i32)));
+    dtab.build(depths, tops, arena.alloc<unsigned char>(DefectTables::table_bytes(tops.size())));
+  }
+}
+
+void Batch::enqueue(const mpr_params* p) {
+  hipStream_t s = stream;
+  for (size_t i = 0; i < jobs.size(); ++i) jobs[i]->setup(p[i], s);
+  if (p[0].stages & MPR_STAGE_SOLVE)
+    run_cycles(vc64, precision == MPR_PRECISION_MIXED ? &vc32 : nullptr, &dtab, p[0], s);
+  for (size_t i = 0; i < jobs.size(); ++i) {
+    jobs[i]->finish(p[i], s);
+    if (p[i].stages & MPR_STAGE_EXTRACT)
+      jobs[i]->enqueue_extract((p[i].stages & MPR_STAGE_WORLD) != 0, s);
+    jobs[i]->readback(s);
+  }
+}
+
+// V-cycles of a set of jobs (core.py:274-285): FP64 as the reference, or FP32 V-cycles on
+// the correction equation inside an FP64 defect-correction loop.
+void run_cycles(const BatchedVcycle<double>& v64, const BatchedVcycle<float>* v32,
+                const DefectTables* dt, const mpr_params& p, hipStream_t s) {
+  for (int c = 0; c < p.cycles; ++c) {
+    if (v32) {
+      dt->defect(p.point_weight, s);  // r = b - A x (FP64), stored as FP32
+      v32->run(p.point_weight, p.pre_smooth, p.post_smooth, true, s);  // e = V32(r), e0 = 0
+      dt->correct(s);                 // x += e
+    } else {
+      v64.run(p.point_weight, p.pre_smooth, p.post_smooth, false, s);
+    }
+  }
+}
 
 void Job::plan(DevArena& a) {
   n = (1 << depth) + 1;
@@ -216,8 +324,24 @@ void Job::plan(DevArena& a) {
   verts = a.alloc<double>(3 * vcap);
   vdens = a.alloc<double>(vcap);
   faces = a.alloc<int64_t>(3 * fcap);
-  cfrom64 = coarse_start(L64);
-  if (!L32.empty()) cfrom32 = coarse_start(L32);
+  // single-instance V-cycle tables
+  const std::vector<InstanceLevels<double>> i64{levels64()};
+  void* t64 = a.alloc<unsigned char>(BatchedVcycle<double>::table_bytes(i64));
+  void* t32 = nullptr;
+  void* td = nullptr;
+  std::vector<InstanceLevels<float>> i32;
+  if (precision == MPR_PRECISION_MIXED) {
+    i32.push_back(levels32());
+    t32 = a.alloc<unsigned char>(BatchedVcycle<float>::table_bytes(i32));
+    td = a.alloc<unsigned char>(DefectTables::table_bytes(1));
+  }
+  if (!a.dry) {
+    vc64.build(i64, t64);
+    if (precision == MPR_PRECISION_MIXED) {
+      vc32.build(i32, t32);
+      dtab.build({depth}, {top_ptrs()}, td);
+    }
+  }
 }
 
 Job::~Job() {
@@ -230,8 +354,7 @@ Job::~Job() {
   if (stream) (void)hipStreamDestroy(stream);
 }
 
-void Job::enqueue_extract(bool world) {
-  hipStream_t s = stream;
+void Job::enqueue_extract(bool world, hipStream_t s) {
   launch_cells_count(field, n, d_iso, cell_tc, s);
   launch_scan_exclusive(cell_tc, cell_to, cell_tiles(n), &d_counts[0], s);
   launch_cells_emit<int32_t>(field, density, n, d_iso, cell_to, cell_map, verts, vdens, vcap,
@@ -241,57 +364,46 @@ void Job::enqueue_extract(bool world) {
   launch_edges_emit<int32_t>(field, n, d_iso, cell_map, edge_to, faces, fcap, s);
 }
 
-void Job::enqueue(const mpr_params& p) {
-  hipStream_t s = stream;
+// normalise, splat, assemble, screening pyramid and initial guess (core.py:229-273)
+void Job::setup(const mpr_params& p, hipStream_t s) {
   const uint64_t M = (uint64_t)n * n * n;
-  const double* upts = pts;
-  if (p.stages & MPR_STAGE_NORMALIZE) {
-    launch_normalize(pts, P, p.scale, bbox_part, d_params, unit, s);
-    upts = unit;
-  }
-  if (p.stages & MPR_STAGE_SOLVE) {
-    const double pw = p.point_weight;
-    launch_fill<double>(density, 0.0, M, s);
-    launch_fill<double>(screen, 0.0, M, s);
-    launch_fill<double>(vx, 0.0, M, s);
-    launch_fill<double>(vy, 0.0, M, s);
-    launch_fill<double>(vz, 0.0, M, s);
-    launch_fill<unsigned long long>(d_accepted, 0ull, 1, s);
-    launch_splat(upts, nrm, P, n, p.confidence, density, vx, vy, vz, screen, p.samples_per_node,
-                 d_accepted, s);
-    launch_assemble(vx, vy, vz, screen, (p.stages & MPR_STAGE_GRIDS) ? rhs : nullptr, L64[0].rhs,
-                    L64[0].screen, density, p.samples_per_node, geoms[0], pw, 0.5, s);
-    for (size_t l = 1; l < L64.size(); ++l)
-      launch_restrict<double>(L64[l - 1].screen, geoms[l - 1], L64[l].screen, geoms[l], 4.0, s);
-    if (P) {
-      launch_fill<double>(L64[0].x, 0.5, 2ull * geoms[0].C, s);
-      if (precision == MPR_PRECISION_MIXED)
-        for (size_t l = 0; l < L32.size(); ++l)
-          launch_convert<double, float>(L64[l].screen, L32[l].screen, 2ull * geoms[l].C, s);
-      for (int c = 0; c < p.cycles; ++c) {
-        if (precision == MPR_PRECISION_MIXED) {
-          // defect correction: r = b - A x in FP64, e = V32(r) from e = 0, x += e
-          launch_residual<double, float>(L64[0].x, L64[0].rhs, L64[0].screen, L32[0].rhs,
-                                         geoms[0], pw, nullptr, s);
-          vcycle<float>(L32, 0, cfrom32, true, pw, p.pre_smooth, p.post_smooth, s);
-          launch_add_correction(L64[0].x, L32[0].x, 2ull * geoms[0].C, s);
-        } else {
-          vcycle<double>(L64, 0, cfrom64, false, pw, p.pre_smooth, p.post_smooth, s);
-        }
-      }
-    } else {
-      launch_fill<double>(L64[0].x, 0.0, 2ull * geoms[0].C, s);  // core.py:286-287
-    }
-    launch_residual<double, double>(L64[0].x, L64[0].rhs, L64[0].screen, L64[0].res, geoms[0], pw,
-                                    norm_part, s);
-    launch_norm_finish(norm_part, norm_nparts, d_resnorm, s);
-    launch_split_to_nat<double>(L64[0].x, field, geoms[0], s);
-    if (P)
-      launch_iso(field, upts, nrm, P, n, p.confidence, iso_part, d_iso, s);
-    else
-      launch_fill<double>(d_iso, 0.0, 1, s);
-  }
-  if (p.stages & MPR_STAGE_EXTRACT) enqueue_extract((p.stages & MPR_STAGE_WORLD) != 0);
+  if (p.stages & MPR_STAGE_NORMALIZE) launch_normalize(pts, P, p.scale, bbox_part, d_params, unit, s);
+  if (!(p.stages & MPR_STAGE_SOLVE)) return;
+  const double* upts = unit_points(p);
+  launch_fill<double>(density, 0.0, M, s);
+  launch_fill<double>(screen, 0.0, M, s);
+  launch_fill<double>(vx, 0.0, M, s);
+  launch_fill<double>(vy, 0.0, M, s);
+  launch_fill<double>(vz, 0.0, M, s);
+  launch_fill<unsigned long long>(d_accepted, 0ull, 1, s);
+  launch_splat(upts, nrm, P, n, p.confidence, density, vx, vy, vz, screen, p.samples_per_node,
+               d_accepted, s);
+  launch_assemble(vx, vy, vz, screen, (p.stages & MPR_STAGE_GRIDS) ? rhs : nullptr, L64[0].rhs,
+                  L64[0].screen, density, p.samples_per_node, geoms[0], p.point_weight, 0.5, s);
+  for (size_t l = 1; l < L64.size(); ++l)
+    launch_restrict<double>(L64[l - 1].screen, geoms[l - 1], L64[l].screen, geoms[l], 4.0, s);
+  // initial guess 0.5 (core.py:270); no samples: the solution is 0 (core.py:286-287), which
+  // the V-cycles then leave at 0 (zero right-hand side and screening)
+  launch_fill<double>(L64[0].x, P ? 0.5 : 0.0, 2ull * geoms[0].C, s);
+  if (precision == MPR_PRECISION_MIXED)
+    for (size_t l = 0; l < L32.size(); ++l)
+      launch_convert<double, float>(L64[l].screen, L32[l].screen, 2ull * geoms[l].C, s);
+}
+
+// final FP64 residual norm, natural-layout field and iso value (core.py:289-303)
+void Job::finish(const mpr_params& p, hipStream_t s) {
+  if (!(p.stages & MPR_STAGE_SOLVE)) return;
+  launch_residual<double, double>(L64[0].x, L64[0].rhs, L64[0].screen, L64[0].res, geoms[0],
+                                  p.point_weight, norm_part, s);
+  launch_norm_finish(norm_part, norm_nparts, d_resnorm, s);
+  launch_split_to_nat<double>(L64[0].x, field, geoms[0], s);
+  if (P)
+    launch_iso(field, unit_points(p), nrm, P, n, p.confidence, iso_part, d_iso, s);
+  else
+    launch_fill<double>(d_iso, 0.0, 1, s);
+}
+
+void Job::readback(hipStream_t s) {
   MPR_HIP(hipMemcpyAsync(&host->accepted, d_accepted, sizeof(unsigned long long),
                          hipMemcpyDeviceToHost, s));
   MPR_HIP(hipMemcpyAsync(host->counts, d_counts, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -300,9 +412,20 @@ void Job::enqueue(const mpr_params& p) {
   MPR_HIP(hipMemcpyAsync(host->params, d_params, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
 }
 
+void Job::enqueue(const mpr_params& p) {
+  hipStream_t s = stream;
+  setup(p, s);
+  if ((p.stages & MPR_STAGE_SOLVE) && P)
+    run_cycles(vc64, precision == MPR_PRECISION_MIXED ? &vc32 : nullptr, &dtab, p, s);
+  finish(p, s);
+  if (p.stages & MPR_STAGE_EXTRACT) enqueue_extract((p.stages & MPR_STAGE_WORLD) != 0, s);
+  readback(s);
+}
+
 }  // namespace mpr
 
 struct mpr_job : mpr::Job {};
+struct mpr_batch : mpr::Batch {};
 
 struct mpr_mesh {
   int n = 0;
@@ -518,44 +641,8 @@ int mpr_job_wait(mpr_job* job, mpr_result* r) {
   if (!job) return -1;
   MPR_GUARD_BEGIN
   MPR_HIP(hipStreamSynchronize(job->stream));
-  if (job->ran && (job->last.stages & MPR_STAGE_EXTRACT)) {
-    const uint64_t nv = job->host->counts[0], nf = 2ull * job->host->counts[1];
-    if (nv > job->vcap || nf > job->fcap) {
-      // grow the mesh buffers to the exact counts and re-run the extraction
-      const uint64_t vcap = std::max<uint64_t>(nv, job->vcap), fcap = std::max<uint64_t>(nf, job->fcap);
-      void *v = nullptr, *d = nullptr, *f = nullptr;
-      MPR_HIP(hipMalloc(&v, 24 * vcap));
-      MPR_HIP(hipMalloc(&d, 8 * vcap));
-      MPR_HIP(hipMalloc(&f, 24 * fcap));
-      job->mesh_bufs.push_back(v);
-      job->mesh_bufs.push_back(d);
-      job->mesh_bufs.push_back(f);
-      job->verts = (double*)v;
-      job->vdens = (double*)d;
-      job->faces = (int64_t*)f;
-      job->vcap = vcap;
-      job->fcap = fcap;
-      if (job->exec) {  // the captured graph points at the old buffers
-        MPR_HIP(hipGraphExecDestroy(job->exec));
-        MPR_HIP(hipGraphDestroy(job->graph));
-        job->exec = nullptr;
-        job->graph = nullptr;
-      }
-      job->enqueue_extract((job->last.stages & MPR_STAGE_WORLD) != 0);
-      MPR_HIP(hipMemcpyAsync(job->host->counts, job->d_counts, 2 * sizeof(uint32_t),
-                             hipMemcpyDeviceToHost, job->stream));
-      MPR_HIP(hipStreamSynchronize(job->stream));
-    }
-  }
-  if (r) {
-    r->accepted = (int64_t)job->host->accepted;
-    r->vertex_count = job->host->counts[0];
-    r->face_count = 2ll * job->host->counts[1];
-    r->iso_value = job->host->iso;
-    r->residual_norm = job->host->resnorm;
-    for (int a = 0; a < 3; ++a) r->origin[a] = job->host->params[a];
-    r->width = job->host->params[3];
-  }
+  job->resolve_overflow(job->stream);
+  job->result(r);
   return 0;
   MPR_GUARD_END
 }
@@ -686,6 +773,141 @@ int mpr_job_fetch_octree(mpr_job* job, int64_t* depth, int64_t* morton, int64_t*
   MPR_HIP(hipStreamSynchronize(s));
   return 0;
   MPR_GUARD_END
+}
+
+static bool batch_params_ok(const mpr_batch* b, const mpr_params* p) {
+  for (size_t i = 0; i < b->jobs.size(); ++i) {
+    if (!valid_params(&p[i])) return false;
+    if (p[i].cycles != p[0].cycles || p[i].pre_smooth != p[0].pre_smooth ||
+        p[i].post_smooth != p[0].post_smooth || p[i].point_weight != p[0].point_weight ||
+        (p[i].stages & MPR_STAGE_SOLVE) != (p[0].stages & MPR_STAGE_SOLVE))
+      return false;
+  }
+  return true;
+}
+
+int mpr_batch_create(mpr_job* const* jobs, int32_t count, mpr_batch** out) {
+  if (!out || !jobs || count < 1) {
+    set_error("a batch needs at least one job");
+    return -1;
+  }
+  *out = nullptr;
+  for (int i = 0; i < count; ++i)
+    if (!jobs[i] || jobs[i]->precision != jobs[0]->precision) {
+      set_error("batch jobs must be non-null and share one precision");
+      return -1;
+    }
+  mpr_batch* b = new mpr_batch();
+  try {
+    b->precision = jobs[0]->precision;
+    for (int i = 0; i < count; ++i) b->jobs.push_back(jobs[i]);
+    MPR_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    for (Job* j : b->jobs) MPR_HIP(hipStreamSynchronize(j->stream));
+    b->build();
+    *out = b;
+    return 0;
+  } catch (const HipError& e) {
+    set_error(e.msg);
+    delete b;
+    return -2;
+  }
+}
+
+int mpr_batch_destroy(mpr_batch* b) {
+  if (!b) return 0;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  delete b;
+  return 0;
+}
+
+int mpr_batch_run(mpr_batch* b, const mpr_params* params) {
+  if (!b || !params || !batch_params_ok(b, params)) {
+    set_error("batch parameters must be valid and share cycles, smoothing and point weight");
+    return -1;
+  }
+  MPR_GUARD_BEGIN
+  for (Job* j : b->jobs) MPR_HIP(hipStreamSynchronize(j->stream));  // order after uploads
+  b->last.assign(params, params + b->jobs.size());
+  for (size_t i = 0; i < b->jobs.size(); ++i) {
+    b->jobs[i]->last = params[i];
+    b->jobs[i]->grids_kept = (params[i].stages & MPR_STAGE_GRIDS) != 0;
+    b->jobs[i]->ran = true;
+  }
+  b->enqueue(params);
+  return 0;
+  MPR_GUARD_END
+}
+
+int mpr_batch_capture(mpr_batch* b, const mpr_params* params) {
+  if (!b || !params || !batch_params_ok(b, params)) {
+    set_error("batch parameters must be valid and share cycles, smoothing and point weight");
+    return -1;
+  }
+  MPR_GUARD_BEGIN
+  b->drop_graph();
+  for (Job* j : b->jobs) MPR_HIP(hipStreamSynchronize(j->stream));
+  MPR_HIP(hipStreamSynchronize(b->stream));
+  MPR_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  try {
+    b->enqueue(params);
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(b->stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  MPR_HIP(hipStreamEndCapture(b->stream, &b->graph));
+  MPR_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  b->last.assign(params, params + b->jobs.size());
+  b->generations.clear();
+  for (Job* j : b->jobs) b->generations.push_back(j->mesh_generation);
+  return 0;
+  MPR_GUARD_END
+}
+
+int mpr_batch_launch(mpr_batch* b) {
+  if (!b || !b->exec) {
+    set_error("batch has no captured graph");
+    return -1;
+  }
+  MPR_GUARD_BEGIN
+  bool stale = false;
+  for (size_t i = 0; i < b->jobs.size(); ++i)
+    stale = stale || b->jobs[i]->mesh_generation != b->generations[i];
+  if (stale) {  // a job's mesh buffers grew since the capture: record again
+    std::vector<mpr_params> p = b->last;
+    if (int rc = mpr_batch_capture(b, p.data())) return rc;
+  }
+  for (Job* j : b->jobs) MPR_HIP(hipStreamSynchronize(j->stream));
+  for (size_t i = 0; i < b->jobs.size(); ++i) {
+    b->jobs[i]->last = b->last[i];
+    b->jobs[i]->grids_kept = (b->last[i].stages & MPR_STAGE_GRIDS) != 0;
+    b->jobs[i]->ran = true;
+  }
+  MPR_HIP(hipGraphLaunch(b->exec, b->stream));
+  return 0;
+  MPR_GUARD_END
+}
+
+int mpr_batch_wait(mpr_batch* b, mpr_result* results) {
+  if (!b) return -1;
+  MPR_GUARD_BEGIN
+  MPR_HIP(hipStreamSynchronize(b->stream));
+  for (size_t i = 0; i < b->jobs.size(); ++i) {
+    b->jobs[i]->resolve_overflow(b->stream);
+    b->jobs[i]->result(results ? &results[i] : nullptr);
+  }
+  return 0;
+  MPR_GUARD_END
+}
+
+int mpr_batch_query(mpr_batch* b) {
+  if (!b) return -1;
+  hipError_t e = hipStreamQuery(b->stream);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  set_error(hipGetErrorString(e));
+  return -2;
 }
 
 int mpr_job_time_kernel(mpr_job* job, int32_t kernel, int32_t reps, double* ms_per_launch,

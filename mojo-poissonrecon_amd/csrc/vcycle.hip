@@ -1,0 +1,145 @@
+// Host side of the batched V-cycle: builds the per-class pointer tables for a set of
+// instances and issues the schedule (one launch per step and size class).
+#include <algorithm>
+#include <cstring>
+
+#include "mpr_batch.h"
+
+namespace mpr {
+
+namespace {
+
+struct TableWriter {
+  std::vector<unsigned char> host;
+  char* device_base;
+  template <class E>
+  E* put(const std::vector<E>& entries) {
+    if (entries.empty()) return nullptr;
+    size_t off = (host.size() + 63) & ~size_t(63);
+    host.resize(off + sizeof(E) * entries.size());
+    std::memcpy(host.data() + off, entries.data(), sizeof(E) * entries.size());
+    return reinterpret_cast<E*>(device_base + off);
+  }
+};
+
+template <class T>
+size_t table_words(const std::vector<InstanceLevels<T>>& inst) {
+  size_t steps = 0;
+  for (const auto& i : inst) steps += (size_t)i.depth + 1;
+  return steps;
+}
+
+}  // namespace
+
+template <class T>
+size_t BatchedVcycle<T>::table_bytes(const std::vector<InstanceLevels<T>>& inst) {
+  const size_t n = inst.size();
+  return sizeof(StepPtrs<T>) * 2 * table_words(inst) + 2 * sizeof(CoarseEntry<T>) * n +
+         64 * 64;
+}
+
+template <class T>
+void BatchedVcycle<T>::build(const std::vector<InstanceLevels<T>>& inst, void* table_mem) {
+  smax = 1;
+  for (const auto& i : inst) smax = std::max(smax, i.depth);
+  sc = coarse_b_top_class<T>();
+  geom_.assign(smax + 1, Geom());
+  for (int s = 0; s <= smax; ++s) geom_[s] = make_geom((1 << s) + 1);
+  TableWriter w{{}, static_cast<char*>(table_mem)};
+  step_.assign(smax + 1, nullptr);
+  step_count_.assign(smax + 1, 0);
+  top_.assign(smax + 1, nullptr);
+  top_count_.assign(smax + 1, 0);
+  for (int s = sc + 1; s <= smax; ++s) {
+    std::vector<StepPtrs<T>> steps, tops;
+    for (const auto& i : inst) {
+      if (i.depth < s) continue;
+      const int k = i.depth - s;
+      const LevelPtrs<T>& a = i.levels[k];
+      const LevelPtrs<T>& b = i.levels[k + 1];
+      StepPtrs<T> e{a.x, a.rhs, a.screen, a.res, b.x, b.rhs};
+      steps.push_back(e);
+      if (i.depth == s) tops.push_back(e);
+    }
+    step_[s] = w.put(steps);
+    step_count_[s] = (uint32_t)steps.size();
+    top_[s] = w.put(tops);
+    top_count_[s] = (uint32_t)tops.size();
+  }
+  std::vector<CoarseEntry<T>> ce, cz;
+  for (const auto& i : inst) {
+    CoarseEntry<T> e;
+    std::memset(&e, 0, sizeof(e));
+    e.top = std::min(i.depth, sc);
+    const int k = i.depth - e.top;
+    e.x = i.levels[k].x;
+    e.rhs = i.levels[k].rhs;
+    for (int s = 1; s <= e.top; ++s) e.screen[s] = i.levels[i.depth - s].screen;
+    e.zero_top = i.depth > sc ? 1 : 0;
+    ce.push_back(e);
+    e.zero_top = 1;
+    cz.push_back(e);
+  }
+  coarse_ = w.put(ce);
+  coarse_zero_ = w.put(cz);
+  coarse_count_ = (uint32_t)ce.size();
+  coarse_class_ = 1;
+  for (const auto& e : ce) coarse_class_ = std::max(coarse_class_, e.top);
+  if (w.host.size() > table_bytes(inst)) throw HipError{"batched V-cycle table overflow"};
+  MPR_HIP(hipMemcpy(table_mem, w.host.data(), w.host.size(), hipMemcpyHostToDevice));
+}
+
+template <class T>
+void BatchedVcycle<T>::run(double pw, int pre, int post, bool zero_tops, hipStream_t s) const {
+  if (zero_tops)
+    for (int c = sc + 1; c <= smax; ++c) launch_b_fill_x<T>(top_[c], top_count_[c], geom_[c], s);
+  for (int c = smax; c > sc; --c) {
+    if (!step_count_[c]) continue;
+    launch_b_relax<T>(step_[c], step_count_[c], geom_[c], pw, pre, s);
+    launch_b_residual<T>(step_[c], step_count_[c], geom_[c], pw, s);
+    // solution[l+1].fill(0) (core.py:174) is fused into the restriction; the coarse
+    // kernel zeroes its own top level
+    launch_b_restrict<T>(step_[c], step_count_[c], geom_[c], geom_[c - 1], c - 1 > sc, s);
+  }
+  launch_b_coarse<T>(zero_tops ? coarse_zero_ : coarse_, coarse_count_, coarse_class_, pw, pre,
+                     post, s);
+  for (int c = sc + 1; c <= smax; ++c) {
+    if (!step_count_[c]) continue;
+    launch_b_prolong<T>(step_[c], step_count_[c], geom_[c - 1], geom_[c], s);
+    launch_b_relax<T>(step_[c], step_count_[c], geom_[c], pw, post, s);
+  }
+}
+
+void DefectTables::build(const std::vector<int>& depths, const std::vector<TopPtrs>& tops,
+                         void* table_mem) {
+  int dmax = 1;
+  for (int d : depths) dmax = std::max(dmax, d);
+  geom_.assign(dmax + 1, Geom());
+  tab_.assign(dmax + 1, nullptr);
+  count_.assign(dmax + 1, 0);
+  TableWriter w{{}, static_cast<char*>(table_mem)};
+  for (int d = 1; d <= dmax; ++d) {
+    std::vector<TopPtrs> e;
+    for (size_t i = 0; i < depths.size(); ++i)
+      if (depths[i] == d) e.push_back(tops[i]);
+    geom_[d] = make_geom((1 << d) + 1);
+    tab_[d] = w.put(e);
+    count_[d] = (uint32_t)e.size();
+  }
+  if (w.host.size() > table_bytes(depths.size())) throw HipError{"defect table overflow"};
+  if (!w.host.empty())
+    MPR_HIP(hipMemcpy(table_mem, w.host.data(), w.host.size(), hipMemcpyHostToDevice));
+}
+
+void DefectTables::defect(double pw, hipStream_t s) const {
+  for (size_t d = 1; d < count_.size(); ++d) launch_b_defect(tab_[d], count_[d], geom_[d], pw, s);
+}
+
+void DefectTables::correct(hipStream_t s) const {
+  for (size_t d = 1; d < count_.size(); ++d) launch_b_correct(tab_[d], count_[d], geom_[d], s);
+}
+
+template class BatchedVcycle<double>;
+template class BatchedVcycle<float>;
+
+}  // namespace mpr

@@ -20,7 +20,8 @@ BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "libmpr_hip.so")
 ARCH = os.environ.get("MPR_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["grid_kernels.hip", "point_kernels.hip", "extract_kernels.hip", "solver.hip",
+SOURCES = ["grid_kernels.hip", "batch_kernels.hip", "vcycle.hip", "point_kernels.hip",
+           "extract_kernels.hip", "solver.hip",
            "capi_compat.hip"]
 
 FLAGS = [

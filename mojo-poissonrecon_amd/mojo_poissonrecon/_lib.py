@@ -99,6 +99,13 @@ _PIPELINE = {
     "mpr_job_device_points": ([P, _PP, _PP], ctypes.c_int),
     "mpr_job_set_count": ([P, I], ctypes.c_int),
     "mpr_job_time_kernel": ([P, S, S, ctypes.POINTER(F), ctypes.POINTER(F)], ctypes.c_int),
+    "mpr_batch_create": ([P, S, _PP], ctypes.c_int),
+    "mpr_batch_destroy": ([P], ctypes.c_int),
+    "mpr_batch_run": ([P, ctypes.POINTER(mpr_params)], ctypes.c_int),
+    "mpr_batch_capture": ([P, ctypes.POINTER(mpr_params)], ctypes.c_int),
+    "mpr_batch_launch": ([P], ctypes.c_int),
+    "mpr_batch_query": ([P], ctypes.c_int),
+    "mpr_batch_wait": ([P, ctypes.POINTER(mpr_result)], ctypes.c_int),
     "mpr_extract": ([P, P, I, F, _PP, _IP, _IP], ctypes.c_int),
     "mpr_mesh_fetch": ([P, P, P, P], ctypes.c_int),
     "mpr_mesh_destroy": ([P], ctypes.c_int),

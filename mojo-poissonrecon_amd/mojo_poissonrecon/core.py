@@ -364,6 +364,31 @@ def reconstruct(
     precision: str = "fp64",
 ) -> Mesh:
     """Reconstruct a watertight surface from oriented points (core.py:388-426)."""
+    req = prepare_reconstruct(points, normals, depth=depth, point_weight=point_weight,
+                              confidence=confidence, cycles=cycles, scale=scale,
+                              samples_per_node=samples_per_node, precision=precision)
+    if req is None:
+        return Mesh(np.empty((0, 3), np.float64), np.empty((0, 3), np.int64), np.empty(0))
+    job = _jobs.get(req.depth, len(req.points), req.precision)
+    job.set_points(req.points, req.normals)
+    job.run(req.params)
+    return mesh_from_job(job, job.wait(), len(req.points))
+
+
+@dataclass(frozen=True)
+class ReconstructRequest:
+    """A validated reconstruct() call: filtered rows, depth, precision and device params."""
+    points: np.ndarray
+    normals: np.ndarray
+    depth: int
+    precision: int
+    params: _lib.mpr_params
+
+
+def prepare_reconstruct(points, normals, *, depth=6, point_weight=4.0, confidence=False,
+                        cycles=4, scale=1.1, samples_per_node=1.5, precision="fp64"):
+    """Validation of reconstruct() (core.py:400-411 and the solver checks of
+    core.py:211-220), in the reference's order. Returns None when no row survives."""
     points = _points(points)
     normals = _points(normals)
     if points.shape != normals.shape:
@@ -373,8 +398,7 @@ def reconstruct(
         raise ValueError("scale must be greater than 1")
     points, normals = _filter(points, normals)
     if not len(points):
-        return Mesh(np.empty((0, 3), np.float64), np.empty((0, 3), np.int64), np.empty(0))
-    # the parameter checks solve_screened_poisson applies (core.py:211-220)
+        return None
     depth = _integer_parameter("depth", depth)
     cycles = _integer_parameter("cycles", cycles)
     point_weight = _finite_parameter("point_weight", point_weight)
@@ -386,14 +410,15 @@ def reconstruct(
         raise ValueError("cycles must be positive")
     if point_weight < 0 or samples_per_node <= 0:
         raise ValueError("point_weight must be non-negative and samples_per_node positive")
-
-    job = _jobs.get(depth, len(points), mode)
-    job.set_points(points, normals)
     stages = (_lib.STAGE_NORMALIZE | _lib.STAGE_SOLVE | _lib.STAGE_EXTRACT | _lib.STAGE_WORLD)
-    job.run(Job.params(stages, point_weight, samples_per_node, scale, cycles, confidence))
-    result = job.wait()
-    if result.accepted != len(points):
-        raise RuntimeError(f"GPU splat accepted {result.accepted} of {len(points)} validated samples")
+    params = Job.params(stages, point_weight, samples_per_node, scale, cycles, confidence)
+    return ReconstructRequest(points, normals, depth, mode, params)
+
+
+def mesh_from_job(job: Job, result: _lib.mpr_result, count: int) -> Mesh:
+    """The world-space Mesh of a finished reconstruct run (core.py:422-426)."""
+    if result.accepted != count:
+        raise RuntimeError(f"GPU splat accepted {result.accepted} of {count} validated samples")
     vertices, faces, density = job.fetch_mesh(result)
     if not len(vertices):
         return Mesh(np.empty((0, 3), np.float64), np.empty((0, 3), np.int64), np.empty(0),

@@ -1,0 +1,214 @@
+"""Continuous-batching request loop for one GPU.
+
+Requests (oriented point clouds with a depth) are admitted while device memory and slots
+allow: an admitted request takes an idle device job of its depth from a pool (or a new
+one, if its HBM footprint fits the budget), uploads its points on that job's stream and
+enqueues the whole reconstruction without waiting. Requests admitted in the same tick run
+as one batch (one launch per V-cycle step for all of them). Work of different requests
+overlaps on the device; completed meshes are read back as soon as their stream finishes,
+which frees the slot and memory for the next arrivals. Latency is measured from a
+request's arrival to its mesh being on the host.
+
+The FP64 single-request reconstruct() stays the reference path; each served result is
+the same as that request's solo run up to the reordering of FP64 atomic additions in the
+splat.
+"""
+
+from __future__ import annotations
+
+import itertools
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .batch import Batch
+from .core import Job, Mesh, mesh_from_job, prepare_reconstruct
+
+
+@dataclass
+class Request:
+    id: int
+    points: np.ndarray
+    normals: np.ndarray
+    depth: int
+    params: _lib.mpr_params
+    precision: int
+    arrival: float
+    admitted: float = 0.0
+    completed: float = 0.0
+    mesh: Mesh | None = None
+
+    @property
+    def latency(self) -> float:
+        return self.completed - self.arrival
+
+
+@dataclass
+class _Flight:
+    requests: list
+    jobs: list
+    batch: Batch | None = None
+
+
+@dataclass
+class ServerStats:
+    completed: int = 0
+    batches: int = 0
+    peak_inflight: int = 0
+    peak_bytes: int = 0
+    latencies: list = field(default_factory=list)
+
+
+class ReconstructionServer:
+    """Admits reconstruction requests by HBM footprint and overlaps them on the device."""
+
+    def __init__(self, *, hbm_budget: int | None = None, max_inflight: int = 12,
+                 max_batch: int = 8, precision: str = "mixed"):
+        self.precision = precision
+        self.max_inflight = max_inflight
+        self.max_batch = max_batch
+        if hbm_budget is None:
+            total = _lib.ctypes.c_int64()
+            _lib.check(_lib.lib().mpr_device_info(None, 0, _lib.ctypes.byref(total), None),
+                       "mpr_device_info")
+            hbm_budget = int(0.7 * total.value)
+        self.hbm_budget = hbm_budget
+        self._pool: list[Job] = []        # idle jobs
+        self._busy: set[int] = set()      # ids of jobs in flight
+        self._bytes = 0
+        self._waiting: list[Request] = []
+        self._flights: list[_Flight] = []
+        self._ids = itertools.count()
+        self.stats = ServerStats()
+
+    # ---- public ---------------------------------------------------------------------
+    def submit(self, points, normals, *, depth: int, arrival: float | None = None,
+               **kwargs) -> Request | None:
+        kwargs.setdefault("precision", self.precision)
+        req = prepare_reconstruct(points, normals, depth=depth, **kwargs)
+        now = time.perf_counter() if arrival is None else arrival
+        if req is None:
+            return None
+        r = Request(next(self._ids), req.points, req.normals, req.depth, req.params,
+                    req.precision, now)
+        self._waiting.append(r)
+        return r
+
+    def step(self) -> list[Request]:
+        """Admit what fits, launch it, and collect finished requests (non-blocking)."""
+        done = self._collect()
+        self._admit()
+        return done
+
+    def drain(self) -> list[Request]:
+        out = []
+        while self._waiting or self._flights:
+            out += self.step()
+            if self._flights and not out:
+                time.sleep(0)
+        return out
+
+    def close(self) -> None:
+        for f in self._flights:
+            if f.batch is not None:
+                f.batch.wait()
+            else:
+                for j in f.jobs:
+                    j.wait()
+        for f in self._flights:
+            if f.batch is not None:
+                f.batch.close()
+        self._flights.clear()
+        for j in self._pool:
+            j.close()
+        self._pool.clear()
+
+    @property
+    def inflight(self) -> int:
+        return sum(len(f.requests) for f in self._flights)
+
+    # ---- internals ------------------------------------------------------------------
+    def _take_job(self, r: Request) -> Job | None:
+        best = None
+        for j in self._pool:
+            if (id(j) not in self._busy and j.depth == r.depth and j.precision == r.precision
+                    and j.max_points >= len(r.points)):
+                if best is None or j.max_points < best.max_points:
+                    best = j
+        if best is not None:
+            return best
+        need = Job.footprint(r.depth, len(r.points), r.precision)
+        # evict idle jobs until the new one fits the budget
+        idle = [j for j in self._pool if id(j) not in self._busy]
+        idle.sort(key=lambda j: Job.footprint(j.depth, j.max_points, j.precision))
+        while self._bytes + need > self.hbm_budget and idle:
+            victim = idle.pop()
+            self._bytes -= Job.footprint(victim.depth, victim.max_points, victim.precision)
+            self._pool.remove(victim)
+            victim.close()
+        if self._bytes + need > self.hbm_budget:
+            return None
+        job = Job(r.depth, len(r.points), r.precision)
+        self._pool.append(job)
+        self._bytes += need
+        self.stats.peak_bytes = max(self.stats.peak_bytes, self._bytes)
+        return job
+
+    def _admit(self) -> None:
+        group: list[tuple[Request, Job]] = []
+        while self._waiting and self.inflight + len(group) < self.max_inflight:
+            r = self._waiting[0]
+            job = self._take_job(r)
+            if job is None:
+                break  # memory is full: wait for a completion (FIFO admission)
+            self._waiting.pop(0)
+            self._busy.add(id(job))
+            job.set_points(r.points, r.normals)
+            r.admitted = time.perf_counter()
+            group.append((r, job))
+            if len(group) == self.max_batch:
+                self._launch(group)
+                group = []
+        if group:
+            self._launch(group)
+        self.stats.peak_inflight = max(self.stats.peak_inflight, self.inflight)
+
+    def _launch(self, group) -> None:
+        reqs = [r for r, _ in group]
+        jobs = [j for _, j in group]
+        uniform = all(r.precision == reqs[0].precision for r in reqs) and all(
+            (r.params.cycles, r.params.pre_smooth, r.params.post_smooth, r.params.point_weight)
+            == (reqs[0].params.cycles, reqs[0].params.pre_smooth, reqs[0].params.post_smooth,
+                reqs[0].params.point_weight) for r in reqs)
+        if len(group) > 1 and uniform:
+            batch = Batch(jobs)
+            batch.run([r.params for r in reqs])
+            self._flights.append(_Flight(reqs, jobs, batch))
+            self.stats.batches += 1
+        else:
+            for r, j in group:
+                j.run(r.params)
+                self._flights.append(_Flight([r], [j]))
+
+    def _collect(self) -> list[Request]:
+        done = []
+        still = []
+        for f in self._flights:
+            ready = f.batch.done() if f.batch is not None else f.jobs[0].done()
+            if not ready:
+                still.append(f)
+                continue
+            results = f.batch.wait() if f.batch is not None else [f.jobs[0].wait()]
+            for r, j, res in zip(f.requests, f.jobs, results):
+                r.mesh = mesh_from_job(j, res, len(r.points))
+                r.completed = time.perf_counter()
+                self._busy.discard(id(j))
+                self.stats.completed += 1
+                self.stats.latencies.append(r.latency)
+                done.append(r)
+            if f.batch is not None:
+                f.batch.close()
+        self._flights = still
+        return done

@@ -1,0 +1,148 @@
+"""GPU tests of the extensions: mixed precision, batched reconstruction, graph replay of a
+batch, and the continuous-batching server (each result the same alone and in the mix).
+
+Tolerances. FP64 atomic additions in the splat make two runs of the same input differ in
+the last bits of the grids, so "the same" is checked as: iso value within 1e-12 relative,
+and meshes whose vertices agree within 1e-9 of the bounding-box diagonal (vertex-to-nearest
+-vertex, both directions) with vertex counts within 2. Mixed precision against FP64: iso
+within 1e-6 relative and the same Hausdorff bound scaled to 1e-4 of the diagonal.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import mojo_poissonrecon as mpr
+import oracle
+from mojo_poissonrecon import shapes
+from mojo_poissonrecon.batch import Batch, reconstruct_batch
+from mojo_poissonrecon.core import Job, prepare_reconstruct
+from mojo_poissonrecon.serve import ReconstructionServer
+
+pytestmark = pytest.mark.gpu
+
+
+def hausdorff_bound(a, b):
+    from scipy.spatial import cKDTree
+    if not len(a) or not len(b):
+        return 0.0 if len(a) == len(b) else np.inf
+    return max(cKDTree(b).query(a)[0].max(), cKDTree(a).query(b)[0].max())
+
+
+def same_mesh(m1, m2, diag, tol=1e-9):
+    assert abs(len(m1.vertices) - len(m2.vertices)) <= 2
+    assert hausdorff_bound(m1.vertices, m2.vertices) <= tol * diag
+
+
+def clouds(n):
+    out = []
+    makers = (shapes.sphere, shapes.torus, shapes.superquadric)
+    for i in range(n):
+        p, q = makers[i % 3](1500 + 700 * i, seed=i)
+        p, q = shapes.perturb(p, q, seed=i, sigma=0.001 * (i % 4), degrees=2.0)
+        out.append((p, q))
+    return out
+
+
+def diag_of(p):
+    return float(np.linalg.norm(p.max(axis=0) - p.min(axis=0)))
+
+
+def test_mixed_precision_tracks_fp64():
+    p, n = shapes.torus_workload(30000, seed=3)
+    for depth in (5, 7):
+        m64 = mpr.reconstruct(p, n, depth=depth, confidence=True, precision="fp64")
+        m32 = mpr.reconstruct(p, n, depth=depth, confidence=True, precision="mixed")
+        assert m32.iso_value == pytest.approx(m64.iso_value, rel=1e-6)
+        same_mesh(m32, m64, diag_of(p), tol=1e-4)
+        s64 = mpr.solve_screened_poisson(*oracle.normalize(p, 1.1)[:1], n, depth=depth,
+                                         confidence=True)
+        s32 = mpr.solve_screened_poisson(*oracle.normalize(p, 1.1)[:1], n, depth=depth,
+                                         confidence=True, precision="mixed")
+        assert np.abs(s32.field - s64.field).max() <= 1e-5 * np.abs(s64.field).max()
+        assert s32.residual_norm == pytest.approx(s64.residual_norm, rel=1e-3)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_equals_solo_runs(precision):
+    cs = clouds(7)
+    depths = [2, 4, 5, 6, 7, 5, 6]
+    batch = reconstruct_batch(cs, depths=depths, precision=precision)
+    for (p, n), d, m in zip(cs, depths, batch):
+        solo = mpr.reconstruct(p, n, depth=d, precision=precision)
+        assert m.iso_value == pytest.approx(solo.iso_value, rel=1e-12)
+        assert m.residual_norm == pytest.approx(solo.residual_norm, rel=1e-9)
+        same_mesh(m, solo, diag_of(p))
+
+
+def test_batch_fp64_matches_oracle_and_handles_empty_clouds():
+    p, n = shapes.sphere(2000, 4)
+    empty = (np.zeros((3, 3)), np.zeros((3, 3)))  # zero normals: no valid rows
+    out = reconstruct_batch([(p, n), empty, (p, n)], depths=[5, 5, 6], precision="fp64")
+    want = oracle.reconstruct(p, n, depth=5)
+    assert out[0].iso_value == pytest.approx(want["iso_value"], rel=1e-10)
+    assert np.array_equal(out[0].faces, want["faces"])
+    assert out[1].is_empty
+    assert not out[2].is_empty
+
+
+def test_batch_graph_replays_match_eager():
+    cs = clouds(5)
+    depths = [3, 5, 6, 7, 4]
+    reqs = [prepare_reconstruct(p, n, depth=d, precision="mixed") for (p, n), d in zip(cs, depths)]
+    jobs = [Job(r.depth, len(r.points), r.precision) for r in reqs]
+    for j, r in zip(jobs, reqs):
+        j.set_points(r.points, r.normals)
+    b = Batch(jobs)
+    b.run([r.params for r in reqs])
+    eager = [(res.iso_value, j.fetch_mesh(res)) for j, res in zip(jobs, b.wait())]
+    b.capture([r.params for r in reqs])
+    for _ in range(3):
+        b.launch()
+        for (iso, (v, f, d)), j, res in zip(eager, jobs, b.wait()):
+            assert res.iso_value == pytest.approx(iso, rel=1e-12)
+            v2, f2, _ = j.fetch_mesh(res)
+            assert abs(len(v2) - len(v)) <= 2
+            assert hausdorff_bound(v2, v) <= 1e-9 * 3.0
+    b.close()
+    for j in jobs:
+        j.close()
+
+
+def test_server_results_equal_solo_runs():
+    rng = np.random.default_rng(0)
+    server = ReconstructionServer(max_inflight=6, max_batch=3, precision="mixed")
+    cs = clouds(10)
+    depths = [int(rng.choice([5, 6, 7])) for _ in cs]
+    reqs = {}
+    done = []
+    for i, ((p, n), d) in enumerate(zip(cs, depths)):
+        r = server.submit(p, n, depth=d)
+        reqs[r.id] = (p, n, d)
+        if i % 4 == 3:  # bursts of arrivals: admitted together as a batch
+            done += server.step()
+    done += server.drain()
+    server.close()
+    assert len(done) == len(cs)
+    assert server.stats.batches >= 1
+    for r in done:
+        p, n, d = reqs[r.id]
+        solo = mpr.reconstruct(p, n, depth=d, precision="mixed")
+        assert r.mesh.iso_value == pytest.approx(solo.iso_value, rel=1e-12)
+        same_mesh(r.mesh, solo, diag_of(p))
+        assert r.latency > 0
+
+
+def test_server_admission_respects_memory_budget():
+    p, n = shapes.sphere(3000, 1)
+    fp = Job.footprint(6, len(p), 1)
+    server = ReconstructionServer(hbm_budget=int(2.5 * fp), max_inflight=8, max_batch=1,
+                                  precision="mixed")
+    for i in range(6):
+        server.submit(p, n, depth=6)
+    server.step()
+    assert server.inflight <= 2
+    done = server.drain()
+    assert len(done) == 6 and server.stats.peak_bytes <= 2.5 * fp
+    server.close()
