@@ -5,6 +5,8 @@
 // Per node the arithmetic is the single-instance element functions, so an instance gets
 // bit-identical results alone or inside any batch.
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 #include "mpr_batch.h"
 #include "stencil.h"
@@ -27,29 +29,59 @@ __device__ __forceinline__ BlockMap block_map(uint32_t nbi) {
   return m;
 }
 
+// Streaming kernels handle kNodesPerThread nodes per lane (t, t + 256, ...) with every load
+// issued before any store (NodeLoads in stencil.h).
+constexpr uint32_t kSpan = kThreads * kNodesPerThread;
+
 template <class T>
 __global__ __launch_bounds__(kThreads) void kb_relax_color(const StepPtrs<T>* tab, uint32_t count,
                                                            uint32_t nbi, Geom g, int c, T pw, T h2) {
-  const BlockMap m = block_map(nbi);
-  if (m.slot >= count || m.t >= g.C) return;
-  int x, y, z;
-  if (!split_decode(g, c, m.t, x, y, z)) return;
-  const StepPtrs<T> p = tab[m.slot];
-  relax_node(p.x, p.rhs, p.screen, g, (uint32_t)c * g.C + m.t, x, y, z, pw, h2);
+  const uint32_t per = gridDim.x >> 3;
+  const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  const uint32_t slot = lb / nbi;
+  if (slot >= count) return;
+  const uint32_t base = (lb - slot * nbi) * kSpan + threadIdx.x;
+  const StepPtrs<T> p = tab[slot];
+  const gptr<T> X = as_global(p.x);
+  const gptr<const T> R = as_global(p.rhs), S = as_global(p.screen);
+  NodeLoads<T> L[kNodesPerThread];
+#pragma unroll
+  for (int k = 0; k < kNodesPerThread; ++k) load_node(X, R, S, g, c, base + k * kThreads, false, L[k]);
+#pragma unroll
+  for (int k = 0; k < kNodesPerThread; ++k) {
+    T v;
+    if (loaded_relax(L[k], pw, h2, v)) X[L[k].idx] = v;
+  }
+}
+
+template <class T, class PO>
+__device__ __forceinline__ void residual_span(gptr<const T> X, gptr<const T> R, gptr<const T> S,
+                                              PO out, const Geom& g, uint32_t base, T pw, T h2) {
+  NodeLoads<T> L[kNodesPerThread];
+#pragma unroll
+  for (int k = 0; k < kNodesPerThread; ++k) {
+    const uint32_t t = base + k * kThreads;
+    const int c = t >= g.C;
+    load_node(X, R, S, g, c, t - (uint32_t)c * g.C, true, L[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < kNodesPerThread; ++k) {
+    const uint32_t t = base + k * kThreads;
+    if (t >= 2u * g.C) continue;
+    out[t] = L[k].ok ? loaded_residual(L[k], pw, h2) : T(0);
+  }
 }
 
 template <class T>
 __global__ __launch_bounds__(kThreads) void kb_residual(const StepPtrs<T>* tab, uint32_t count,
                                                         uint32_t nbi, Geom g, T pw, T h2) {
-  const BlockMap m = block_map(nbi);
-  if (m.slot >= count || m.t >= 2u * g.C) return;
-  const StepPtrs<T> p = tab[m.slot];
-  const int c = m.t >= g.C;
-  int x, y, z;
-  if (split_decode(g, c, m.t - (uint32_t)c * g.C, x, y, z))
-    p.res[m.t] = residual_node(p.x, p.rhs, p.screen, g, m.t, x, y, z, pw, h2);
-  else
-    p.res[m.t] = T(0);
+  const uint32_t per = gridDim.x >> 3;
+  const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  const uint32_t slot = lb / nbi;
+  if (slot >= count) return;
+  const StepPtrs<T> p = tab[slot];
+  residual_span<T>(as_global((const T*)p.x), as_global(p.rhs), as_global(p.screen),
+                   as_global(p.res), g, (lb - slot * nbi) * kSpan + threadIdx.x, pw, h2);
 }
 
 // restrict the residual into the next-coarser rhs and zero the next-coarser x
@@ -76,7 +108,7 @@ __global__ __launch_bounds__(kThreads) void kb_prolong_add(const StepPtrs<T>* ta
   const int c = m.t >= gf.C;
   int x, y, z;
   if (!split_decode(gf, c, m.t - (uint32_t)c * gf.C, x, y, z)) return;
-  p.x[m.t] += prolong_node(p.xc, gc, x, y, z);
+  p.x[m.t] += prolong_node<T>(p.xc, gc, x, y, z);
 }
 
 template <class T>
@@ -88,17 +120,25 @@ __global__ __launch_bounds__(kThreads) void kb_fill_x(const StepPtrs<T>* tab, ui
 }
 
 // defect correction at the top level: r32 = b - A x in FP64, written as FP32
+struct ToFloat {  // stores a double as float through a global pointer
+  gptr<float> p;
+  struct Ref {
+    gptr<float> q;
+    __device__ void operator=(double v) const { *q = (float)v; }
+  };
+  __device__ Ref operator[](uint32_t i) const { return Ref{p + i}; }
+};
+
 __global__ __launch_bounds__(kThreads) void kb_defect(const TopPtrs* tab, uint32_t count,
                                                       uint32_t nbi, Geom g, double pw, double h2) {
-  const BlockMap m = block_map(nbi);
-  if (m.slot >= count || m.t >= 2u * g.C) return;
-  const TopPtrs p = tab[m.slot];
-  const int c = m.t >= g.C;
-  int x, y, z;
-  if (split_decode(g, c, m.t - (uint32_t)c * g.C, x, y, z))
-    p.r32[m.t] = (float)residual_node(p.x64, p.rhs64, p.screen64, g, m.t, x, y, z, pw, h2);
-  else
-    p.r32[m.t] = 0.0f;
+  const uint32_t per = gridDim.x >> 3;
+  const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  const uint32_t slot = lb / nbi;
+  if (slot >= count) return;
+  const TopPtrs p = tab[slot];
+  residual_span<double>(as_global((const double*)p.x64), as_global(p.rhs64),
+                        as_global(p.screen64), ToFloat{as_global(p.r32)}, g,
+                        (lb - slot * nbi) * kSpan + threadIdx.x, pw, h2);
 }
 
 __global__ __launch_bounds__(kThreads) void kb_correct(const TopPtrs* tab, uint32_t count,
@@ -109,12 +149,24 @@ __global__ __launch_bounds__(kThreads) void kb_correct(const TopPtrs* tab, uint3
   p.x64[m.t] += (double)p.e32[m.t];
 }
 
+// Geometry and LDS word offset of every coarse size class (n = 2^s + 1). Offsets count
+// elements from class 1 upwards, so they are the same for float and double and for every
+// top class; the table lives in constant memory and is read with uniform (scalar) loads.
+struct CoarseClassInfo {
+  Geom g;
+  uint32_t off = 0;
+};
+__constant__ CoarseClassInfo g_coarse_info[kMaxCoarseClass + 1];
+
 template <class T>
 __device__ __forceinline__ void wg_relax(T* x, const T* rhs, const T* scr, const Geom& g, T pw,
                                          int iterations) {
   const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
+#pragma unroll 1
   for (int it = 0; it < iterations; ++it)
+#pragma unroll 1
     for (int c = 0; c < 2; ++c) {
+#pragma unroll 1
       for (uint32_t t = threadIdx.x; t < g.C; t += blockDim.x) {
         int xx, yy, zz;
         if (split_decode(g, c, t, xx, yy, zz))
@@ -124,83 +176,90 @@ __device__ __forceinline__ void wg_relax(T* x, const T* rhs, const T* scr, const
     }
 }
 
-// Size class S of the per-instance LDS V-cycle (core.py:141-193). Classes are template
-// parameters so geometries and LDS offsets are compile-time-indexed kernel arguments.
-template <class T, int S>
-__device__ void wg_class(const CoarseArgsB<T>& a, const CoarseEntry<T>& e, T* sm) {
-  if constexpr (S >= 1 && S <= kMaxCoarseClass) {
-    if (S > e.top) {  // this instance's hierarchy starts below S
-      wg_class<T, S - 1>(a, e, sm);
-      return;
+// One workgroup per instance: the instance's V-cycle from its top coarse class down
+// (core.py:141-193 as a down loop and an up loop), every level resident in LDS. Only the
+// top x (unless it starts at zero), the top rhs and the screening weights are read from
+// HBM, and only the top x is written back.
+template <class T>
+__global__ __launch_bounds__(kCoarseThreads) void kb_coarse(const CoarseEntry<T>* tab, T pw,
+                                                            int pre, int post) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* sm = reinterpret_cast<T*>(smem_raw);
+  const CoarseEntry<T> e = tab[blockIdx.x];
+  const int top = e.top;
+  {
+    const uint32_t m0 = 2u * g_coarse_info[top].g.C;
+    T* x0 = sm + g_coarse_info[top].off;
+    for (uint32_t i = threadIdx.x; i < m0; i += blockDim.x) {
+      x0[i] = e.zero_top ? T(0) : e.x[i];
+      x0[m0 + i] = e.rhs[i];
     }
-    const Geom g = a.geom[S];
+#pragma unroll 1
+    for (int s = 1; s <= top; ++s) {
+      const uint32_t m = 2u * g_coarse_info[s].g.C;
+      T* scr = sm + g_coarse_info[s].off + 2 * m;
+      const T* src = e.screen[s];
+      for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) scr[i] = src[i];
+    }
+  }
+  __syncthreads();
+#pragma unroll 1
+  for (int s = top; s >= 2; --s) {  // pre-smooth, residual, restrict, zero the coarser x
+    const Geom g = g_coarse_info[s].g;
     const uint32_t m = 2u * g.C;
-    T* x = sm + a.off[S];
+    T* x = sm + g_coarse_info[s].off;
     T* rhs = x + m;
     T* scr = rhs + m;
     T* res = scr + m;
-    if (S == 1) {  // coarsest (n = 3): 40 iterations (core.py:152-156)
-      wg_relax(x, rhs, scr, g, a.pw, 40);
-      return;
-    }
-    wg_relax(x, rhs, scr, g, a.pw, a.pre);
+    wg_relax(x, rhs, scr, g, pw, pre);
     const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
+#pragma unroll 1
     for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) {
       const int c = t >= g.C;
       int xx, yy, zz;
       if (split_decode(g, c, t - (uint32_t)c * g.C, xx, yy, zz))
-        res[t] = residual_node(x, rhs, scr, g, t, xx, yy, zz, a.pw, h2);
+        res[t] = residual_node(x, rhs, scr, g, t, xx, yy, zz, pw, h2);
     }
     __syncthreads();
-    if constexpr (S > 1) {
-      const Geom gc = a.geom[S - 1];
-      const uint32_t mc = 2u * gc.C;
-      T* xc = sm + a.off[S - 1];
-      T* rhsc = xc + mc;
-      for (uint32_t t = threadIdx.x; t < mc; t += blockDim.x) {
-        const int c = t >= gc.C;
-        int X, Y, Z;
-        if (split_decode(gc, c, t - (uint32_t)c * gc.C, X, Y, Z))
-          rhsc[t] = restrict_node(res, g, X, Y, Z, T(1));
-        xc[t] = T(0);
-      }
-      __syncthreads();
-      wg_class<T, S - 1>(a, e, sm);
-      for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) {
-        const int c = t >= g.C;
-        int xx, yy, zz;
-        if (split_decode(g, c, t - (uint32_t)c * g.C, xx, yy, zz))
-          x[t] += prolong_node(xc, gc, xx, yy, zz);
-      }
-      __syncthreads();
-      wg_relax(x, rhs, scr, g, a.pw, a.post);
+    const Geom gc = g_coarse_info[s - 1].g;
+    const uint32_t mc = 2u * gc.C;
+    T* xc = sm + g_coarse_info[s - 1].off;
+    T* rhsc = xc + mc;
+#pragma unroll 1
+    for (uint32_t t = threadIdx.x; t < mc; t += blockDim.x) {
+      const int c = t >= gc.C;
+      int X, Y, Z;
+      if (split_decode(gc, c, t - (uint32_t)c * gc.C, X, Y, Z))
+        rhsc[t] = restrict_node(res, g, X, Y, Z, T(1));
+      xc[t] = T(0);
     }
+    __syncthreads();
   }
-}
-
-// one workgroup per instance: the whole V-cycle from the instance's coarse top class down,
-// with every level in LDS; only the top x is read (unless zeroed) and written back
-template <class T>
-__global__ __launch_bounds__(kCoarseThreads) void kb_coarse(CoarseArgsB<T> a,
-                                                            const CoarseEntry<T>* tab) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* sm = reinterpret_cast<T*>(smem_raw);
-  const CoarseEntry<T> e = tab[blockIdx.x];
-  const Geom g0 = a.geom[e.top];
-  const uint32_t m0 = 2u * g0.C;
-  T* x0 = sm + a.off[e.top];
-  for (uint32_t i = threadIdx.x; i < m0; i += blockDim.x) {
-    x0[i] = e.zero_top ? T(0) : e.x[i];
-    x0[m0 + i] = e.rhs[i];
+  {  // coarsest (n = 3): 40 iterations (core.py:152-156)
+    const Geom g = g_coarse_info[1].g;
+    const uint32_t m = 2u * g.C;
+    T* x = sm + g_coarse_info[1].off;
+    wg_relax(x, x + m, x + 2 * m, g, pw, 40);
   }
-  for (int s = 1; s <= e.top; ++s) {
-    const uint32_t m = 2u * a.geom[s].C;
-    T* scr = sm + a.off[s] + 2 * m;
-    const T* src = e.screen[s];
-    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) scr[i] = src[i];
+#pragma unroll 1
+  for (int s = 2; s <= top; ++s) {  // prolong-add, post-smooth
+    const Geom g = g_coarse_info[s].g;
+    const Geom gc = g_coarse_info[s - 1].g;
+    const uint32_t m = 2u * g.C;
+    T* x = sm + g_coarse_info[s].off;
+    const T* xc = sm + g_coarse_info[s - 1].off;
+#pragma unroll 1
+    for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) {
+      const int c = t >= g.C;
+      int xx, yy, zz;
+      if (split_decode(g, c, t - (uint32_t)c * g.C, xx, yy, zz))
+        x[t] += prolong_node<T>(xc, gc, xx, yy, zz);
+    }
+    __syncthreads();
+    wg_relax(x, x + m, x + 2 * m, g, pw, post);
   }
-  __syncthreads();
-  wg_class<T, kMaxCoarseClass>(a, e, sm);
+  const uint32_t m0 = 2u * g_coarse_info[top].g.C;
+  const T* x0 = sm + g_coarse_info[top].off;
   for (uint32_t i = threadIdx.x; i < m0; i += blockDim.x) e.x[i] = x0[i];
 }
 
@@ -212,7 +271,7 @@ template <class T>
 void launch_b_relax(const StepPtrs<T>* tab, uint32_t count, const Geom& g, double pw,
                     int iterations, hipStream_t s) {
   if (!count || !iterations) return;
-  const uint32_t nbi = blocks_for(g.C, kThreads);
+  const uint32_t nbi = blocks_for(g.C, kSpan);
   const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
   for (int it = 0; it < iterations; ++it)
     for (int c = 0; c < 2; ++c) {
@@ -226,7 +285,7 @@ template <class T>
 void launch_b_residual(const StepPtrs<T>* tab, uint32_t count, const Geom& g, double pw,
                        hipStream_t s) {
   if (!count) return;
-  const uint32_t nbi = blocks_for(2ull * g.C, kThreads);
+  const uint32_t nbi = blocks_for(2ull * g.C, kSpan);
   const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
   hipLaunchKernelGGL(kb_residual<T>, dim3(padded8((uint64_t)nbi * count)), dim3(kThreads), 0, s,
                      tab, count, nbi, g, (T)pw, h2);
@@ -264,7 +323,7 @@ void launch_b_fill_x(const StepPtrs<T>* tab, uint32_t count, const Geom& g, hipS
 
 void launch_b_defect(const TopPtrs* tab, uint32_t count, const Geom& g, double pw, hipStream_t s) {
   if (!count) return;
-  const uint32_t nbi = blocks_for(2ull * g.C, kThreads);
+  const uint32_t nbi = blocks_for(2ull * g.C, kSpan);
   const double h2 = (double)(g.n - 1) * (double)(g.n - 1);
   hipLaunchKernelGGL(kb_defect, dim3(padded8((uint64_t)nbi * count)), dim3(kThreads), 0, s, tab,
                      count, nbi, g, pw, h2);
@@ -300,28 +359,35 @@ template <class T>
 void launch_b_coarse(const CoarseEntry<T>* tab, uint32_t count, int top_class, double pw, int pre,
                      int post, hipStream_t s) {
   if (!count) return;
-  static bool attr_set = false;
-  if (!attr_set) {
-    MPR_HIP(hipFuncSetAttribute((const void*)kb_coarse<T>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, kCoarseLdsBytes));
-    attr_set = true;
-  }
-  CoarseArgsB<T> a;
-  uint32_t off = 0;
-  for (int c = 0; c <= kMaxCoarseClass; ++c) {
-    a.geom[c] = make_geom((1 << c) + 1);
-    a.off[c] = 0;
-  }
-  for (int c = 1; c <= top_class; ++c) {
-    a.off[c] = off;
-    off += 4u * 2u * a.geom[c].C;
-  }
-  a.pw = (T)pw;
-  a.pre = pre;
-  a.post = post;
   const size_t bytes = coarse_b_lds_bytes<T>(top_class);
-  hipLaunchKernelGGL(kb_coarse<T>, dim3(count), dim3(kCoarseThreads), bytes, s, a, tab);
+  hipLaunchKernelGGL(kb_coarse<T>, dim3(count), dim3(kCoarseThreads), bytes, s, tab, (T)pw, pre,
+                     post);
   MPR_LAUNCH_CHECK();
+}
+
+// Uploads the coarse class table and raises the coarse kernels' LDS limit. Runs once per
+// process, outside any stream capture (from the table builders).
+void init_coarse_kernels() {
+  static std::once_flag once;
+  static std::string failure;
+  std::call_once(once, [] {
+    CoarseClassInfo info[kMaxCoarseClass + 1];
+    uint32_t off = 0;
+    for (int c = 0; c <= kMaxCoarseClass; ++c) {
+      info[c].g = make_geom((1 << c) + 1);
+      info[c].off = c >= 1 ? off : 0;
+      if (c >= 1) off += 4u * 2u * info[c].g.C;
+    }
+    if (hipMemcpyToSymbol(HIP_SYMBOL(g_coarse_info), info, sizeof(info)) != hipSuccess ||
+        hipFuncSetAttribute((const void*)kb_coarse<double>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)kCoarseLdsBytes) != hipSuccess ||
+        hipFuncSetAttribute((const void*)kb_coarse<float>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)kCoarseLdsBytes) != hipSuccess)
+      failure = "coarse kernel initialisation failed";
+  });
+  if (!failure.empty()) throw HipError{failure};
 }
 
 #define MPR_BINST(T)                                                                           \

@@ -149,7 +149,7 @@ double mpr_residual_f64(const double* solution, const double* rhs, const double*
     const Geom g = make_geom((int32_t)n);
     const size_t M = (size_t)n * n * n, W = 2ull * g.C;
     DevBuf<double> nat(M), x(W), r(W), s(W), out(W), norm(1);
-    DevBuf<double> part(blocks_for(W, kThreads));
+    DevBuf<double> part(blocks_for(W, kThreads * kNodesPerThread));
     h2d(nat.p, solution, M);
     launch_nat_to_split<double, double>(nat.p, x.p, g, kNull);
     h2d(nat.p, rhs, M);

@@ -36,11 +36,15 @@ __global__ __launch_bounds__(kThreads) void k_fill(T* p, T v, uint64_t count) {
 template <class T>
 __global__ __launch_bounds__(kThreads) void k_relax_color(T* X, const T* R, const T* S, Geom g,
                                                           int c, T pw, T h2) {
-  uint32_t t = logical_block() * kThreads + threadIdx.x;
-  if (t >= g.C) return;
-  int x, y, z;
-  if (!split_decode(g, c, t, x, y, z)) return;
-  relax_node(X, R, S, g, (uint32_t)c * g.C + t, x, y, z, pw, h2);
+  const uint32_t base = logical_block() * (kThreads * kNodesPerThread) + threadIdx.x;
+  NodeLoads<T> L[kNodesPerThread];
+#pragma unroll
+  for (int k = 0; k < kNodesPerThread; ++k) load_node(X, R, S, g, c, base + k * kThreads, false, L[k]);
+#pragma unroll
+  for (int k = 0; k < kNodesPerThread; ++k) {
+    T v;
+    if (loaded_relax(L[k], pw, h2, v)) X[L[k].idx] = v;
+  }
 }
 
 __device__ __forceinline__ double block_sum(double v, double* lds) {
@@ -61,23 +65,32 @@ __global__ __launch_bounds__(kThreads) void k_residual(const T* X, const T* R, c
                                                        double* partials) {
   __shared__ double lds[kThreads / 64];
   const uint32_t lb = logical_block();
-  const uint32_t t = lb * kThreads + threadIdx.x;
-  double sq = 0.0;
-  if (t < 2u * g.C) {
+  const uint32_t base = lb * (kThreads * kNodesPerThread) + threadIdx.x;
+  NodeLoads<T> L[kNodesPerThread];
+  uint32_t tt[kNodesPerThread];
+#pragma unroll
+  for (int k = 0; k < kNodesPerThread; ++k) {
+    const uint32_t t = base + k * kThreads;
+    tt[k] = t;
     const int c = t >= g.C;
-    const uint32_t tc = t - (uint32_t)c * g.C;
-    int x, y, z;
-    if (split_decode(g, c, tc, x, y, z)) {
-      T r = residual_node(X, R, S, g, t, x, y, z, pw, h2);
-      out[t] = (TR)r;
-      sq = (double)r * (double)r;
+    load_node(X, R, S, g, c, t - (uint32_t)c * g.C, true, L[k]);
+    if (t >= 2u * g.C) L[k].ok = false;
+  }
+  double sq = 0.0;
+#pragma unroll
+  for (int k = 0; k < kNodesPerThread; ++k) {
+    if (tt[k] >= 2u * g.C) continue;
+    if (L[k].ok) {
+      const T r = loaded_residual(L[k], pw, h2);
+      out[tt[k]] = (TR)r;
+      sq += (double)r * (double)r;
     } else {
-      out[t] = TR(0);
+      out[tt[k]] = TR(0);
     }
   }
   if (partials) {
     double total = block_sum(sq, lds);
-    if (threadIdx.x == 0 && lb * kThreads < 2u * g.C) partials[lb] = total;
+    if (threadIdx.x == 0 && lb * (kThreads * kNodesPerThread) < 2u * g.C) partials[lb] = total;
   }
 }
 
@@ -113,7 +126,7 @@ __global__ __launch_bounds__(kThreads) void k_prolong_add(const T* Cc, Geom gc, 
   const int c = t >= gf.C;
   int x, y, z;
   if (!split_decode(gf, c, t - (uint32_t)c * gf.C, x, y, z)) return;
-  F[t] += prolong_node(Cc, gc, x, y, z);
+  F[t] += prolong_node<T>(Cc, gc, x, y, z);
 }
 
 template <class TI, class TO>
@@ -193,105 +206,6 @@ __global__ __launch_bounds__(kThreads) void k_assemble(const double* vx, const d
   if (density) density[t] = density[t] / spn;
 }
 
-template <class T>
-__device__ __forceinline__ void block_relax(T* x, const T* rhs, const T* scr, const Geom& g, T pw,
-                                            int iterations) {
-  const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
-  for (int it = 0; it < iterations; ++it)
-    for (int c = 0; c < 2; ++c) {
-      for (uint32_t t = threadIdx.x; t < g.C; t += blockDim.x) {
-        int xx, yy, zz;
-        if (split_decode(g, c, t, xx, yy, zz))
-          relax_node(x, rhs, scr, g, (uint32_t)c * g.C + t, xx, yy, zz, pw, h2);
-      }
-      __syncthreads();
-    }
-}
-
-template <class T>
-struct CoarseArgs {
-  LevelPtrs<T> lv[kMaxCoarseLevels];  // global buffers (only the top x/rhs and the screens
-                                      // are read, only the top x is written back)
-  uint32_t off[kMaxCoarseLevels];     // LDS word offset of each level's x|rhs|screen|res
-  int count;
-  int zero_top;
-  T pw;
-  int pre, post;
-};
-
-// Level L of the single-workgroup V-cycle (core.py:141-193). Levels are template
-// parameters so every geometry and LDS offset is a compile-time-indexed kernel argument
-// (scalar registers) rather than a private array.
-template <class T, int L>
-__device__ void coarse_level(const CoarseArgs<T>& a, T* sm) {
-  if constexpr (L < kMaxCoarseLevels) {
-    const Geom g = a.lv[L].g;
-    const uint32_t m = 2u * g.C;
-    T* x = sm + a.off[L];
-    T* rhs = x + m;
-    T* scr = rhs + m;
-    T* res = scr + m;
-    if (L + 1 >= a.count) {  // coarsest: 40 iterations (core.py:152-156)
-      block_relax(x, rhs, scr, g, a.pw, 40);
-      return;
-    }
-    block_relax(x, rhs, scr, g, a.pw, a.pre);
-    const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
-    for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) {
-      const int c = t >= g.C;
-      int xx, yy, zz;
-      if (split_decode(g, c, t - (uint32_t)c * g.C, xx, yy, zz))
-        res[t] = residual_node(x, rhs, scr, g, t, xx, yy, zz, a.pw, h2);
-    }
-    __syncthreads();
-    const Geom gc = a.lv[L + 1].g;
-    const uint32_t mc = 2u * gc.C;
-    T* xc = sm + a.off[L + 1];
-    T* rhsc = xc + mc;
-    for (uint32_t t = threadIdx.x; t < mc; t += blockDim.x) {
-      const int c = t >= gc.C;
-      int X, Y, Z;
-      if (split_decode(gc, c, t - (uint32_t)c * gc.C, X, Y, Z))
-        rhsc[t] = restrict_node(res, g, X, Y, Z, T(1));
-      xc[t] = T(0);
-    }
-    __syncthreads();
-    coarse_level<T, L + 1>(a, sm);
-    for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) {
-      const int c = t >= g.C;
-      int xx, yy, zz;
-      if (split_decode(g, c, t - (uint32_t)c * g.C, xx, yy, zz))
-        x[t] += prolong_node(xc, gc, xx, yy, zz);
-    }
-    __syncthreads();
-    block_relax(x, rhs, scr, g, a.pw, a.post);
-  }
-}
-
-// One workgroup runs the whole V-cycle below (and including) the top level it is given,
-// with all levels resident in LDS.
-template <class T>
-__global__ __launch_bounds__(1024) void k_coarse_vcycle(CoarseArgs<T> a) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* sm = reinterpret_cast<T*>(smem_raw);
-  const uint32_t m0 = 2u * a.lv[0].g.C;
-  T* x0 = sm + a.off[0];
-  for (uint32_t i = threadIdx.x; i < m0; i += blockDim.x) {
-    x0[i] = a.zero_top ? T(0) : a.lv[0].x[i];
-    x0[m0 + i] = a.lv[0].rhs[i];
-  }
-  for (int l = 0; l < a.count; ++l) {
-    const uint32_t m = 2u * a.lv[l].g.C;
-    T* scr = sm + a.off[l] + 2 * m;
-    const T* src = a.lv[l].screen;
-    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) scr[i] = src[i];
-  }
-  __syncthreads();
-  coarse_level<T, 0>(a, sm);
-  T* dst = a.lv[0].x;
-  for (uint32_t i = threadIdx.x; i < m0; i += blockDim.x) dst[i] = x0[i];
-}
-
 }  // namespace
 
 template <class T>
@@ -306,7 +220,7 @@ template <class T>
 void launch_relax_color(T* X, const T* R, const T* S, const Geom& g, double pw, int c,
                         hipStream_t s) {
   const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
-  const unsigned nb = xcd_grid(g.C, kThreads);
+  const unsigned nb = xcd_grid(g.C, kThreads * kNodesPerThread);
   hipLaunchKernelGGL(k_relax_color<T>, dim3(nb), dim3(kThreads), 0, s, X, R, S, g, c, (T)pw, h2);
   MPR_LAUNCH_CHECK();
 }
@@ -322,11 +236,11 @@ template <class T, class TR>
 int launch_residual(const T* X, const T* R, const T* S, TR* out, const Geom& g, double pw,
                     double* partials, hipStream_t s) {
   const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
-  const unsigned nb = xcd_grid(2ull * g.C, kThreads);
+  const unsigned nb = xcd_grid(2ull * g.C, kThreads * kNodesPerThread);
   hipLaunchKernelGGL((k_residual<T, TR>), dim3(nb), dim3(kThreads), 0, s, X, R, S, out, g,
                      (T)pw, h2, partials);
   MPR_LAUNCH_CHECK();
-  return (int)blocks_for(2ull * g.C, kThreads);
+  return (int)blocks_for(2ull * g.C, kThreads * kNodesPerThread);
 }
 
 void launch_norm_finish(const double* partials, int count, double* out, hipStream_t s) {
@@ -391,48 +305,6 @@ void launch_assemble(const double* vx, const double* vy, const double* vz,
   MPR_LAUNCH_CHECK();
 }
 
-template <class T>
-size_t coarse_lds_bytes(const Geom* geoms, int count) {
-  size_t words = 0;
-  for (int l = 0; l < count; ++l) words += 4ull * 2ull * geoms[l].C;
-  return words * sizeof(T);
-}
-
-template <class T>
-bool launch_coarse_vcycle(const LevelPtrs<T>* levels, int count, bool zero_top, double pw,
-                          int pre, int post, hipStream_t s) {
-  if (count < 1 || count > kMaxCoarseLevels) return false;
-  Geom geoms[kMaxCoarseLevels];
-  for (int l = 0; l < count; ++l) geoms[l] = levels[l].g;
-  const size_t bytes = coarse_lds_bytes<T>(geoms, count);
-  if (bytes > 160 * 1024) return false;
-  static bool attr_set = false;
-  if (!attr_set) {
-    MPR_HIP(hipFuncSetAttribute((const void*)k_coarse_vcycle<T>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    attr_set = true;
-  }
-  CoarseArgs<T> a;
-  uint32_t off = 0;
-  for (int l = 0; l < count; ++l) {
-    a.lv[l] = levels[l];
-    a.off[l] = off;
-    off += 4u * 2u * levels[l].g.C;
-  }
-  for (int l = count; l < kMaxCoarseLevels; ++l) {
-    a.lv[l] = levels[count - 1];
-    a.off[l] = 0;
-  }
-  a.count = count;
-  a.zero_top = zero_top ? 1 : 0;
-  a.pw = (T)pw;
-  a.pre = pre;
-  a.post = post;
-  hipLaunchKernelGGL(k_coarse_vcycle<T>, dim3(1), dim3(1024), bytes, s, a);
-  MPR_LAUNCH_CHECK();
-  return true;
-}
-
 #define MPR_INST(T)                                                                          \
   template void launch_fill<T>(T*, T, uint64_t, hipStream_t);                                \
   template void launch_relax<T>(T*, const T*, const T*, const Geom&, double, int, hipStream_t); \
@@ -440,10 +312,7 @@ bool launch_coarse_vcycle(const LevelPtrs<T>* levels, int count, bool zero_top, 
                                       hipStream_t);                                            \
   template void launch_restrict<T>(const T*, const Geom&, T*, const Geom&, double, hipStream_t); \
   template void launch_prolong_add<T>(const T*, const Geom&, T*, const Geom&, hipStream_t);  \
-  template void launch_split_to_nat<T>(const T*, double*, const Geom&, hipStream_t);         \
-  template bool launch_coarse_vcycle<T>(const LevelPtrs<T>*, int, bool, double, int, int,    \
-                                        hipStream_t);                                        \
-  template size_t coarse_lds_bytes<T>(const Geom*, int);
+  template void launch_split_to_nat<T>(const T*, double*, const Geom&, hipStream_t);
 MPR_INST(double)
 MPR_INST(float)
 #undef MPR_INST

@@ -44,14 +44,6 @@ struct CoarseEntry {
 };
 
 template <class T>
-struct CoarseArgsB {
-  Geom geom[kMaxCoarseClass + 1];
-  uint32_t off[kMaxCoarseClass + 1];  // LDS word offset of each class's x|rhs|screen|res
-  T pw;
-  int pre, post;
-};
-
-template <class T>
 void launch_b_relax(const StepPtrs<T>* tab, uint32_t count, const Geom& g, double pw,
                     int iterations, hipStream_t s);
 template <class T>
@@ -70,6 +62,8 @@ void launch_b_correct(const TopPtrs* tab, uint32_t count, const Geom& g, hipStre
 template <class T>
 void launch_b_coarse(const CoarseEntry<T>* tab, uint32_t count, int top_class, double pw, int pre,
                      int post, hipStream_t s);
+// uploads the coarse class table; call outside stream capture before the first launch
+void init_coarse_kernels();
 // largest coarse class whose LDS-resident hierarchy fits one workgroup
 template <class T> int coarse_b_top_class();
 

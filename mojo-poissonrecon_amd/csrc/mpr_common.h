@@ -46,7 +46,7 @@ struct HipError {
 // Granlund & Montgomery 1994), used to decode flattened thread indices.
 struct FastDiv {
   uint32_t d, m, s;
-  MPR_HD FastDiv() : d(1), m(1), s(0) {}
+  MPR_HD constexpr FastDiv() : d(1), m(1), s(0) {}
   explicit FastDiv(uint32_t div) : d(div) {
     uint32_t l = 0;
     while ((1ull << l) < div) ++l;
@@ -65,10 +65,10 @@ struct FastDiv {
 
 // geometry of one grid level in the colour-split layout
 struct Geom {
-  int32_t n;     // nodes per side
-  int32_t H;     // slots per colour row, (n+1)/2
-  uint32_t C;    // words per colour block, n*n*H
-  uint32_t nH;   // n*H
+  int32_t n = 0;     // nodes per side
+  int32_t H = 0;     // slots per colour row, (n+1)/2
+  uint32_t C = 0;    // words per colour block, n*n*H
+  uint32_t nH = 0;   // n*H
   FastDiv div_nH, div_H, div_C;
 };
 
@@ -107,5 +107,18 @@ MPR_HD bool split_decode(const Geom& g, int c, uint32_t t, int& x, int& y, int& 
 inline unsigned blocks_for(uint64_t work, unsigned threads) {
   return (unsigned)((work + threads - 1) / threads);
 }
+
+// Pointer into global memory (address space 1). Buffers reached through a device pointer
+// table are generic pointers to the compiler; re-typing them lets it emit global_load/store
+// instead of the slower generic flat_load/store.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+
+template <class T>
+__device__ __forceinline__ gptr<T> as_global(T* p) {
+  return (gptr<T>)p;
+}
+
+constexpr int kNodesPerThread = 2;  // streaming kernels: nodes per lane (loads hoisted)
 
 }  // namespace mpr

@@ -7,7 +7,6 @@
 namespace mpr {
 
 constexpr int kThreads = 256;
-constexpr int kMaxCoarseLevels = 8;
 
 template <class T>
 struct LevelPtrs {
@@ -52,13 +51,6 @@ void launch_assemble(const double* vx, const double* vy, const double* vz,
                      const double* screen_nat, double* rhs_nat, double* rhs_split,
                      double* screen_split, double* density, double spn, const Geom& g,
                      double pw, double target, hipStream_t s);
-// Runs the V-cycle of levels[0..count) (top level first, last level n <= 3) inside one
-// workgroup with every level resident in LDS. Returns false if the levels do not fit.
-template <class T>
-bool launch_coarse_vcycle(const LevelPtrs<T>* levels, int count, bool zero_top, double pw,
-                          int pre, int post, hipStream_t s);
-template <class T> size_t coarse_lds_bytes(const Geom* geoms, int count);
-
 // ---- point kernels (point_kernels.hip) --------------------------------------------------
 int bbox_partial_count(uint32_t P);
 // core.py:377-385 on the device: params = [origin x, y, z, width], unit = (p - origin)/width

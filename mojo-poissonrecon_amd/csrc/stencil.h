@@ -65,6 +65,81 @@ MPR_HD T residual_node(const T* X, const T* R, const T* S, const Geom& g, uint32
   return R[idx] - applied;
 }
 
+// ---- two-phase form for streaming kernels ---------------------------------------------
+// A lane first issues every load of its NPT nodes (neighbours from the other colour, rhs,
+// screen, own value), then computes and stores. Within a colour sweep no node reads a node
+// that is written, so hoisting all loads above the stores is exact, and it doubles the
+// bytes each lane keeps in flight. The arithmetic is relax_node / residual_node's.
+template <class T>
+struct NodeLoads {
+  T v[6];       // neighbours x-1, x+1, y-1, y+1, z-1, z+1 (own value where absent)
+  T r, s, self;
+  uint32_t idx;
+  uint32_t flags;  // bit j: neighbour j exists
+  bool ok;         // a real node (not a pad slot or past the end)
+};
+
+// t: index inside colour block c (values >= g.C are clamped and marked not ok)
+template <class T, class PX, class PC>
+MPR_HD void load_node(PX X, PC R, PC S, const Geom& g, int c, uint32_t t, bool want_self,
+                      NodeLoads<T>& L) {
+  const bool in = t < g.C;
+  t = in ? t : g.C - 1;
+  int x, y, z;
+  L.ok = split_decode(g, c, t, x, y, z) && in;
+  const int n = g.n;
+  const bool bx0 = x > 0, bx1 = x + 1 < n, by0 = y > 0, by1 = y + 1 < n, bz0 = z > 0,
+             bz1 = z + 1 < n;
+  const uint32_t self = (uint32_t)c * g.C + t;
+  L.idx = self;
+  L.flags = (uint32_t)bx0 | ((uint32_t)bx1 << 1) | ((uint32_t)by0 << 2) | ((uint32_t)by1 << 3) |
+            ((uint32_t)bz0 << 4) | ((uint32_t)bz1 << 5);
+  L.v[0] = X[bx0 ? split_index(g, x - 1, y, z) : self];
+  L.v[1] = X[bx1 ? split_index(g, x + 1, y, z) : self];
+  L.v[2] = X[by0 ? split_index(g, x, y - 1, z) : self];
+  L.v[3] = X[by1 ? split_index(g, x, y + 1, z) : self];
+  L.v[4] = X[bz0 ? split_index(g, x, y, z - 1) : self];
+  L.v[5] = X[bz1 ? split_index(g, x, y, z + 1) : self];
+  L.r = R[self];
+  L.s = S[self];
+  L.self = want_self ? X[self] : T(0);
+}
+
+template <class T>
+MPR_HD T loaded_sum(const NodeLoads<T>& L, int& deg) {
+  T s = T(0);
+  int d = 0;
+#pragma unroll
+  for (int j = 0; j < 6; ++j)
+    if ((L.flags >> j) & 1u) {
+      s += L.v[j];
+      ++d;
+    }
+  deg = d;
+  return s;
+}
+
+// relax_node on loaded values; returns whether the node is written
+template <class T>
+MPR_HD bool loaded_relax(const NodeLoads<T>& L, T pw, T h2, T& out) {
+  int deg;
+  const T s = loaded_sum(L, deg);
+  const T screening = pw * L.s * h2;
+  const T diag = (T)deg * h2 + screening;
+  out = (L.r + s * h2) / diag;
+  return L.ok && diag > T(1.0e-30);
+}
+
+// residual_node on loaded values
+template <class T>
+MPR_HD T loaded_residual(const NodeLoads<T>& L, T pw, T h2) {
+  int deg;
+  const T s = loaded_sum(L, deg);
+  const T screening = pw * L.s * h2;
+  const T applied = ((T)deg * h2 + screening) * L.self - s * h2;
+  return L.r - applied;
+}
+
 // Full-weighting restriction of coarse node (X,Y,Z) (capi.mojo:556-673): weights 2 at the
 // centre and 1 at the sides per axis, terms in (oz, oy, ox) order, divided by the sum of
 // the in-grid weights, then multiplied by `scale` (1 for residuals, 4 for the screening
@@ -72,8 +147,8 @@ MPR_HD T residual_node(const T* X, const T* R, const T* S, const Geom& g, uint32
 //
 // As in neighbour_sum, the 27 loads use clamped (in-grid) coordinates and only the
 // accumulation is guarded, so all loads can be in flight at once.
-template <class T>
-MPR_HD T restrict_node(const T* F, const Geom& gf, int X, int Y, int Z, T scale) {
+template <class T, class PF>
+MPR_HD T restrict_node(PF F, const Geom& gf, int X, int Y, int Z, T scale) {
   const int nf = gf.n;
   T v[27];
 #pragma unroll
@@ -120,8 +195,8 @@ MPR_HD T restrict_node(const T* F, const Geom& gf, int X, int Y, int Z, T scale)
 }
 
 // Trilinear prolongation value at fine node (x,y,z) (capi.mojo:676-742).
-template <class T>
-MPR_HD T prolong_node(const T* Cg, const Geom& gc, int x, int y, int z) {
+template <class T, class PC>
+MPR_HD T prolong_node(PC Cg, const Geom& gc, int x, int y, int z) {
   const int nc = gc.n;
   int x0 = x >> 1, y0 = y >> 1, z0 = z >> 1;
   int x1 = x0 + 1 < nc - 1 ? x0 + 1 : nc - 1;

@@ -40,6 +40,7 @@ size_t BatchedVcycle<T>::table_bytes(const std::vector<InstanceLevels<T>>& inst)
 
 template <class T>
 void BatchedVcycle<T>::build(const std::vector<InstanceLevels<T>>& inst, void* table_mem) {
+  init_coarse_kernels();
   smax = 1;
   for (const auto& i : inst) smax = std::max(smax, i.depth);
   sc = coarse_b_top_class<T>();
