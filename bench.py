@@ -175,8 +175,31 @@ def serve_stream(requests: int, qps: float, seed: int):
         "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "batches": server.stats.batches,
         "peak_inflight": server.stats.peak_inflight,
         "peak_hbm_gb": round(server.stats.peak_bytes / 1e9, 2),
+        "jobs_created": server.stats.jobs_created,
+        "host_s": {"prepare": round(server.stats.host_prepare, 3),
+                   "launch": round(server.stats.host_launch, 3),
+                   "collect": round(server.stats.host_collect, 3), "wall": round(wall, 3)},
         "depth_mix": "6:7:8:9 = 4:3:2:1", "points": "5k-500k log-uniform",
     }
+
+
+def time_kernels(jobs, reps: int) -> dict:
+    """Dominant kernels at the finest level, timed with HIP events on each job's own stream
+    (the stream the pipeline launches them on)."""
+    from mojo_poissonrecon import _lib
+
+    kernels = {}
+    for kname, kid, job in (("rb_sweep_f64", 8, jobs["fp64"]), ("rb_sweep_f32", 7, jobs["mixed"]),
+                            ("residual_f32", 9, jobs["mixed"]), ("residual_f64", 2, jobs["fp64"]),
+                            ("restrict_f64", 3, jobs["fp64"]), ("prolong_f64", 4, jobs["fp64"]),
+                            ("defect_f64_f32", 6, jobs["mixed"])):
+        ms, nbytes = _lib.ctypes.c_double(), _lib.ctypes.c_double()
+        _lib.check(_lib.lib().mpr_job_time_kernel(job.handle, kid, reps, _lib.ctypes.byref(ms),
+                                                   _lib.ctypes.byref(nbytes)),
+                   "mpr_job_time_kernel")
+        kernels[kname] = dict(ms=ms.value, bytes=nbytes.value,
+                              gbs=nbytes.value / (ms.value * 1e-3) / 1e9)
+    return kernels
 
 
 def main() -> int:
@@ -193,6 +216,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the 64-batch and serving runs")
     ap.add_argument("--kernel-reps", type=int, default=40)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="one reconstruction per precision, then only the kernel timings "
+                         "(a short program for hardware-counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -227,6 +253,13 @@ def main() -> int:
                            nv=int(r.vertex_count), iso=r.iso_value, res=r.residual_norm)
         jobs[name] = job
         log(f"{name}: {modes[name]['value']:.2f} recon/s, p50 {modes[name]['p50']} ms")
+    if args.kernels_only:
+        kernels = time_kernels(jobs, args.kernel_reps)
+        print(json.dumps({k: {"us": round(v["ms"] * 1e3, 2), "GBps": round(v["gbs"], 1)}
+                          for k, v in kernels.items()}), flush=True)
+        for job in jobs.values():
+            job.close()
+        return 0
     # graph replay of the mixed single request
     jm = jobs["mixed"]
     jm.capture(params)
@@ -249,19 +282,7 @@ def main() -> int:
     diag = float(np.linalg.norm(points.max(axis=0) - points.min(axis=0)))
     hd = hausdorff_bound(meshes["mixed"][0], meshes["fp64"][0]) / diag
 
-    # dominant kernels, timed on the jobs' own streams with HIP events
-    kernels = {}
-    for kname, kid, job in (("rb_sweep_f64", 0, jobs["fp64"]), ("rb_sweep_f32", 1, jobs["mixed"]),
-                            ("residual_f64", 2, jobs["fp64"]), ("restrict_f64", 3, jobs["fp64"]),
-                            ("prolong_f64", 4, jobs["fp64"]), ("residual_f32", 5, jobs["mixed"]),
-                            ("defect_f64_f32", 6, jobs["mixed"])):
-        ms, nbytes = _lib.ctypes.c_double(), _lib.ctypes.c_double()
-        _lib.check(_lib.lib().mpr_job_time_kernel(job.handle, kid, args.kernel_reps,
-                                                   _lib.ctypes.byref(ms),
-                                                   _lib.ctypes.byref(nbytes)),
-                   "mpr_job_time_kernel")
-        kernels[kname] = dict(ms=ms.value, bytes=nbytes.value,
-                              gbs=nbytes.value / (ms.value * 1e-3) / 1e9)
+    kernels = time_kernels(jobs, args.kernel_reps)
     for job in jobs.values():
         job.close()
     dom = kernels["rb_sweep_f32"]
@@ -317,8 +338,7 @@ def main() -> int:
         "hausdorff_mixed_vs_fp64_over_diag": hd,
         "mesh": {"vertices": modes["mixed"]["nv"], "fp64_vertices": fp64["nv"]},
         "roofline": {
-            "kernel": "rb_sweep_f32 (k_relax_color<float> / kb_relax_color<float>, finest level, "
-                      "one colour)",
+            "kernel": "rb_sweep_f32: kb_relax_color<float>, one colour of the finest level",
             "bound": "hbm",
             "achieved": round(dom["gbs"], 1),
             "peak": HBM_PEAK_GBS,

@@ -182,8 +182,9 @@ int mpr_batch_wait(mpr_batch *batch, mpr_result *results);
  * times on the job's finest level (after a run) and returns the mean milliseconds per
  * launch and the algorithmic HBM bytes one launch moves. Kernels: 0 FP64 red-black colour
  * sweep, 1 FP32 colour sweep (mixed jobs), 2 FP64 residual, 3 FP64 restriction,
- * 4 FP64 prolongation, 5 FP32 residual, 6 FP64->FP32 defect residual. No reference
- * counterpart (measurement hook). */
+ * 4 FP64 prolongation, 5 FP32 residual, 6 FP64->FP32 defect residual; the batched V-cycle
+ * kernels the pipeline itself launches: 7 FP32 colour sweep, 8 FP64 colour sweep, 9 FP32
+ * residual. No reference counterpart (measurement hook). */
 int mpr_job_time_kernel(mpr_job *job, int32_t kernel, int32_t reps, double *ms_per_launch,
                         double *bytes_per_launch);
 

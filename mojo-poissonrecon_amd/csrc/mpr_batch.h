@@ -86,6 +86,14 @@ class BatchedVcycle {
   // One V-cycle of every instance (core.py:141-193). zero_tops: every instance's top x
   // starts at zero (the correction equation of the defect-correction loop).
   void run(double pw, int pre, int post, bool zero_tops, hipStream_t s) const;
+  // the V-cycle's own steps at the finest class alone (for kernel timing): `iterations`
+  // red-black sweeps (two launches each), or one residual launch
+  void relax_finest(double pw, int iterations, hipStream_t s) const {
+    launch_b_relax<T>(step_[smax], step_count_[smax], geom_[smax], pw, iterations, s);
+  }
+  void residual_finest(double pw, hipStream_t s) const {
+    launch_b_residual<T>(step_[smax], step_count_[smax], geom_[smax], pw, s);
+  }
   int smax = 0, sc = 0;
 
  private:

@@ -912,11 +912,12 @@ int mpr_batch_query(mpr_batch* b) {
 
 int mpr_job_time_kernel(mpr_job* job, int32_t kernel, int32_t reps, double* ms_per_launch,
                         double* bytes_per_launch) {
-  if (!job || reps < 1 || kernel < 0 || kernel > 6) {
+  if (!job || reps < 1 || kernel < 0 || kernel > 9) {
     set_error("invalid kernel timing request");
     return -1;
   }
-  if ((kernel == 1 || kernel == 5 || kernel == 6) && job->L32.empty()) {
+  if ((kernel == 1 || kernel == 5 || kernel == 6 || kernel == 7 || kernel == 9) &&
+      job->L32.empty()) {
     set_error("FP32 kernels need a mixed-precision job");
     return -1;
   }
@@ -928,6 +929,7 @@ int mpr_job_time_kernel(mpr_job* job, int32_t kernel, int32_t reps, double* ms_p
   const double pw = job->ran ? job->last.point_weight : 4.0;
   auto& a = job->L64[0];
   double bytes = 0;
+  int launches = reps;
   hipEvent_t e0, e1;
   MPR_HIP(hipEventCreate(&e0));
   MPR_HIP(hipEventCreate(&e1));
@@ -968,13 +970,30 @@ int mpr_job_time_kernel(mpr_job* job, int32_t kernel, int32_t reps, double* ms_p
         break;
     }
   }
+  // the V-cycle's batched kernels at the finest class, as the pipeline launches them
+  switch (kernel) {
+    case 7:
+      job->vc32.relax_finest(pw, reps, s);
+      bytes = 4.0 * 4.0 * M0 / 2.0;
+      launches = 2 * reps;
+      break;
+    case 8:
+      job->vc64.relax_finest(pw, reps, s);
+      bytes = 4.0 * 8.0 * M0 / 2.0;
+      launches = 2 * reps;
+      break;
+    case 9:
+      for (int r = 0; r < reps; ++r) job->vc32.residual_finest(pw, s);
+      bytes = 4.0 * 4.0 * M0;
+      break;
+  }
   MPR_HIP(hipEventRecord(e1, s));
   MPR_HIP(hipEventSynchronize(e1));
   float ms = 0.f;
   MPR_HIP(hipEventElapsedTime(&ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  if (ms_per_launch) *ms_per_launch = (double)ms / reps;
+  if (ms_per_launch) *ms_per_launch = (double)ms / launches;
   if (bytes_per_launch) *bytes_per_launch = bytes;
   return 0;
   MPR_GUARD_END

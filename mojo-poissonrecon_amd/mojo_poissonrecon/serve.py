@@ -58,7 +58,18 @@ class ServerStats:
     batches: int = 0
     peak_inflight: int = 0
     peak_bytes: int = 0
+    jobs_created: int = 0
     latencies: list = field(default_factory=list)
+    # host seconds spent in each part of the loop (validation, upload + launch, readback)
+    host_prepare: float = 0.0
+    host_launch: float = 0.0
+    host_collect: float = 0.0
+
+
+def job_capacity(points: int) -> int:
+    """Point capacity of a pooled job: the next power of two, at least 16384, so that a
+    pool job serves later clouds of similar size instead of only the one it was made for."""
+    return max(16384, 1 << max(0, int(points) - 1).bit_length())
 
 
 class ReconstructionServer:
@@ -87,8 +98,10 @@ class ReconstructionServer:
     def submit(self, points, normals, *, depth: int, arrival: float | None = None,
                **kwargs) -> Request | None:
         kwargs.setdefault("precision", self.precision)
+        t0 = time.perf_counter()
         req = prepare_reconstruct(points, normals, depth=depth, **kwargs)
-        now = time.perf_counter() if arrival is None else arrival
+        self.stats.host_prepare += time.perf_counter() - t0
+        now = t0 if arrival is None else arrival
         if req is None:
             return None
         r = Request(next(self._ids), req.points, req.normals, req.depth, req.params,
@@ -98,8 +111,12 @@ class ReconstructionServer:
 
     def step(self) -> list[Request]:
         """Admit what fits, launch it, and collect finished requests (non-blocking)."""
+        t0 = time.perf_counter()
         done = self._collect()
+        t1 = time.perf_counter()
         self._admit()
+        self.stats.host_collect += t1 - t0
+        self.stats.host_launch += time.perf_counter() - t1
         return done
 
     def drain(self) -> list[Request]:
@@ -139,7 +156,8 @@ class ReconstructionServer:
                     best = j
         if best is not None:
             return best
-        need = Job.footprint(r.depth, len(r.points), r.precision)
+        capacity = job_capacity(len(r.points))
+        need = Job.footprint(r.depth, capacity, r.precision)
         # evict idle jobs until the new one fits the budget
         idle = [j for j in self._pool if id(j) not in self._busy]
         idle.sort(key=lambda j: Job.footprint(j.depth, j.max_points, j.precision))
@@ -150,7 +168,8 @@ class ReconstructionServer:
             victim.close()
         if self._bytes + need > self.hbm_budget:
             return None
-        job = Job(r.depth, len(r.points), r.precision)
+        job = Job(r.depth, capacity, r.precision)
+        self.stats.jobs_created += 1
         self._pool.append(job)
         self._bytes += need
         self.stats.peak_bytes = max(self.stats.peak_bytes, self._bytes)

@@ -138,8 +138,34 @@ def test_vectorised_mesh_normals_equal_the_per_face_loop_bitwise():
     assert np.array_equal(p, wp) and np.array_equal(n, wn)
 
 
+def test_row_filter_matches_reference_restatement_on_adversarial_rows():
+    # core.py:73-83 with huge, tiny, zero, infinite and NaN components
+    rng = np.random.default_rng(0)
+    P = rng.normal(size=(20000, 3)) * 10.0 ** rng.integers(-300, 300, size=(20000, 1))
+    N = rng.normal(size=(20000, 3)) * 10.0 ** rng.integers(-320, 308, size=(20000, 1))
+    N[::7] = 0
+    N[1::11, 0] = np.inf
+    P[2::13, 1] = np.nan
+    N[3::17] = [0, 0, 5e-324]
+    N[4::19] = [1.7e308, -1.7e308, 1e308]
+    N[5::23] = [-0.0, 0.0, -0.0]
+    with np.errstate(all="ignore"):
+        got = mpr.core._valid_oriented_rows(P, N)
+        want = oracle.valid_oriented_rows(P, N)
+    assert np.array_equal(got, want)
+    clean = rng.normal(size=(100, 3))
+    assert mpr.core._valid_oriented_rows(clean, clean).all()
+
+
 def test_host_normalisation_matches_reference_restatement():
     p = np.random.default_rng(1).normal(size=(1000, 3)) * [3.0, 1.0, 0.5] + 7.0
     a = mpr.core._normalize(p, 1.1)
     b = oracle.normalize(p, 1.1)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
+
+
+def test_server_job_capacity_rounds_to_powers_of_two():
+    from mojo_poissonrecon.serve import job_capacity
+    assert job_capacity(0) == 16384 and job_capacity(3000) == 16384
+    assert job_capacity(16384) == 16384 and job_capacity(16385) == 32768
+    assert job_capacity(500_000) == 524288

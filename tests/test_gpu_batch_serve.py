@@ -18,7 +18,7 @@ import oracle
 from mojo_poissonrecon import shapes
 from mojo_poissonrecon.batch import Batch, reconstruct_batch
 from mojo_poissonrecon.core import Job, prepare_reconstruct
-from mojo_poissonrecon.serve import ReconstructionServer
+from mojo_poissonrecon.serve import ReconstructionServer, job_capacity
 
 pytestmark = pytest.mark.gpu
 
@@ -136,7 +136,7 @@ def test_server_results_equal_solo_runs():
 
 def test_server_admission_respects_memory_budget():
     p, n = shapes.sphere(3000, 1)
-    fp = Job.footprint(6, len(p), 1)
+    fp = Job.footprint(6, job_capacity(len(p)), 1)
     server = ReconstructionServer(hbm_budget=int(2.5 * fp), max_inflight=8, max_batch=1,
                                   precision="mixed")
     for i in range(6):
