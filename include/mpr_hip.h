@@ -57,6 +57,15 @@ void mpr_relax_f64(double *solution, const double *rhs, const double *screen, in
 double mpr_residual_f64(const double *solution, const double *rhs, const double *screen,
                         double *residual, int64_t n, double point_weight);
 
+/* FP32 instantiations of mpr_relax_f64 / mpr_residual_f64 (the smoother and residual of
+ * the mixed-precision V-cycle, run by the same kernels) on host float32 grids. No reference
+ * counterpart (the reference is FP64 only); they let the FP32 kernels be checked against a
+ * float32 restatement. Return 0, or -1 with mpr_last_error() set. */
+int mpr_relax_f32(float *solution, const float *rhs, const float *screen, int64_t n,
+                  double point_weight, int64_t iterations);
+int mpr_residual_f32(const float *solution, const float *rhs, const float *screen,
+                     float *residual, int64_t n, double point_weight);
+
 /* Replaces mpr_restrict_f64 (src/capi.mojo:556-673; _lib.py:27). */
 void mpr_restrict_f64(const double *fine, double *coarse, int64_t fine_n);
 

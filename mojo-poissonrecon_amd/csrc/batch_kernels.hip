@@ -84,6 +84,57 @@ __global__ __launch_bounds__(kThreads) void kb_residual(const StepPtrs<T>* tab, 
                    as_global(p.res), g, (lb - slot * nbi) * kSpan + threadIdx.x, pw, h2);
 }
 
+// Slot-group forms of the sweep, residual and defect for levels with 16-byte aligned rows
+// (Geom::H a multiple of 4): one lane per 16 bytes of a colour row (stencil.h).
+template <class T>
+constexpr int kGroup = 16 / (int)sizeof(T);
+
+template <class T>
+__global__ __launch_bounds__(kThreads) void kb_relax_group(const StepPtrs<T>* tab, uint32_t count,
+                                                           uint32_t nbi, Geom g, uint32_t lanes,
+                                                           int c, T pw, T h2) {
+  constexpr int V = kGroup<T>;
+  const uint32_t per = gridDim.x >> 3;
+  const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  const uint32_t slot = lb / nbi;
+  if (slot >= count) return;
+  const StepPtrs<T> p = tab[slot];
+  SlotGroup<T, V> G;
+  load_group<T, V>(as_global((const T*)p.x), as_global(p.rhs), as_global(p.screen), g, c,
+                   (lb - slot * nbi) * kThreads + threadIdx.x, lanes, false, G);
+  T out[V];
+  const uint32_t mask = group_relax<T, V>(G, g.n, pw, h2, out);
+  store_group<T, T, V>(as_global(p.x) + G.own, out, mask);
+}
+
+// residual of both colours: lanes [0, lanes) cover colour 0, [lanes, 2 lanes) colour 1
+template <class T, class TO>
+__device__ __forceinline__ void residual_group(gptr<const T> X, gptr<const T> R, gptr<const T> S,
+                                               gptr<TO> out, const Geom& g, uint32_t lane,
+                                               uint32_t lanes, T pw, T h2) {
+  constexpr int V = kGroup<T>;
+  const int c = lane >= lanes;
+  SlotGroup<T, V> G;
+  load_group<T, V>(X, R, S, g, c, lane - (uint32_t)c * lanes, lanes, true, G);
+  T v[V];
+  group_residual<T, V>(G, g.n, pw, h2, v);
+  store_group<T, TO, V>(out + G.own, v, G.in ? (1u << V) - 1 : 0u);
+}
+
+template <class T>
+__global__ __launch_bounds__(kThreads) void kb_residual_group(const StepPtrs<T>* tab,
+                                                              uint32_t count, uint32_t nbi, Geom g,
+                                                              uint32_t lanes, T pw, T h2) {
+  const uint32_t per = gridDim.x >> 3;
+  const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  const uint32_t slot = lb / nbi;
+  if (slot >= count) return;
+  const StepPtrs<T> p = tab[slot];
+  residual_group<T, T>(as_global((const T*)p.x), as_global(p.rhs), as_global(p.screen),
+                       as_global(p.res), g, (lb - slot * nbi) * kThreads + threadIdx.x, lanes, pw,
+                       h2);
+}
+
 // restrict the residual into the next-coarser rhs and zero the next-coarser x
 template <class T>
 __global__ __launch_bounds__(kThreads) void kb_restrict(const StepPtrs<T>* tab, uint32_t count,
@@ -139,6 +190,19 @@ __global__ __launch_bounds__(kThreads) void kb_defect(const TopPtrs* tab, uint32
   residual_span<double>(as_global((const double*)p.x64), as_global(p.rhs64),
                         as_global(p.screen64), ToFloat{as_global(p.r32)}, g,
                         (lb - slot * nbi) * kSpan + threadIdx.x, pw, h2);
+}
+
+__global__ __launch_bounds__(kThreads) void kb_defect_group(const TopPtrs* tab, uint32_t count,
+                                                            uint32_t nbi, Geom g, uint32_t lanes,
+                                                            double pw, double h2) {
+  const uint32_t per = gridDim.x >> 3;
+  const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  const uint32_t slot = lb / nbi;
+  if (slot >= count) return;
+  const TopPtrs p = tab[slot];
+  residual_group<double, float>(as_global((const double*)p.x64), as_global(p.rhs64),
+                                as_global(p.screen64), as_global(p.r32), g,
+                                (lb - slot * nbi) * kThreads + threadIdx.x, lanes, pw, h2);
 }
 
 __global__ __launch_bounds__(kThreads) void kb_correct(const TopPtrs* tab, uint32_t count,
@@ -271,8 +335,19 @@ template <class T>
 void launch_b_relax(const StepPtrs<T>* tab, uint32_t count, const Geom& g, double pw,
                     int iterations, hipStream_t s) {
   if (!count || !iterations) return;
-  const uint32_t nbi = blocks_for(g.C, kSpan);
   const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
+  if (group_layout(g)) {
+    const uint32_t lanes = g.C / (uint32_t)kGroup<T>;
+    const uint32_t nbi = blocks_for(lanes, kThreads);
+    for (int it = 0; it < iterations; ++it)
+      for (int c = 0; c < 2; ++c) {
+        hipLaunchKernelGGL(kb_relax_group<T>, dim3(padded8((uint64_t)nbi * count)),
+                           dim3(kThreads), 0, s, tab, count, nbi, g, lanes, c, (T)pw, h2);
+        MPR_LAUNCH_CHECK();
+      }
+    return;
+  }
+  const uint32_t nbi = blocks_for(g.C, kSpan);
   for (int it = 0; it < iterations; ++it)
     for (int c = 0; c < 2; ++c) {
       hipLaunchKernelGGL(kb_relax_color<T>, dim3(padded8((uint64_t)nbi * count)), dim3(kThreads),
@@ -285,8 +360,16 @@ template <class T>
 void launch_b_residual(const StepPtrs<T>* tab, uint32_t count, const Geom& g, double pw,
                        hipStream_t s) {
   if (!count) return;
-  const uint32_t nbi = blocks_for(2ull * g.C, kSpan);
   const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
+  if (group_layout(g)) {
+    const uint32_t lanes = g.C / (uint32_t)kGroup<T>;
+    const uint32_t nbi = blocks_for(2ull * lanes, kThreads);
+    hipLaunchKernelGGL(kb_residual_group<T>, dim3(padded8((uint64_t)nbi * count)),
+                       dim3(kThreads), 0, s, tab, count, nbi, g, lanes, (T)pw, h2);
+    MPR_LAUNCH_CHECK();
+    return;
+  }
+  const uint32_t nbi = blocks_for(2ull * g.C, kSpan);
   hipLaunchKernelGGL(kb_residual<T>, dim3(padded8((uint64_t)nbi * count)), dim3(kThreads), 0, s,
                      tab, count, nbi, g, (T)pw, h2);
   MPR_LAUNCH_CHECK();
@@ -323,8 +406,16 @@ void launch_b_fill_x(const StepPtrs<T>* tab, uint32_t count, const Geom& g, hipS
 
 void launch_b_defect(const TopPtrs* tab, uint32_t count, const Geom& g, double pw, hipStream_t s) {
   if (!count) return;
-  const uint32_t nbi = blocks_for(2ull * g.C, kSpan);
   const double h2 = (double)(g.n - 1) * (double)(g.n - 1);
+  if (group_layout(g)) {
+    const uint32_t lanes = g.C / (uint32_t)kGroup<double>;
+    const uint32_t nbi = blocks_for(2ull * lanes, kThreads);
+    hipLaunchKernelGGL(kb_defect_group, dim3(padded8((uint64_t)nbi * count)), dim3(kThreads), 0,
+                       s, tab, count, nbi, g, lanes, pw, h2);
+    MPR_LAUNCH_CHECK();
+    return;
+  }
+  const uint32_t nbi = blocks_for(2ull * g.C, kSpan);
   hipLaunchKernelGGL(kb_defect, dim3(padded8((uint64_t)nbi * count)), dim3(kThreads), 0, s, tab,
                      count, nbi, g, pw, h2);
   MPR_LAUNCH_CHECK();

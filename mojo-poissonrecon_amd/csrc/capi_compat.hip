@@ -51,6 +51,20 @@ void report(const char* fn, const std::string& msg) {
 const double kNaN = std::numeric_limits<double>::quiet_NaN();
 const hipStream_t kNull = nullptr;
 
+void upload_f32_as_split(const float* host, float* split, const Geom& g, DevBuf<double>& nat,
+                         size_t M) {
+  std::vector<double> wide(host, host + M);
+  h2d(nat.p, wide.data(), M);
+  launch_nat_to_split<double, float>(nat.p, split, g, kNull);
+}
+void download_split_f32(const float* split, float* host, const Geom& g, DevBuf<double>& nat,
+                        size_t M) {
+  launch_split_to_nat<float>(split, nat.p, g, kNull);
+  std::vector<double> wide(M);
+  d2h(wide.data(), nat.p, M);
+  for (size_t i = 0; i < M; ++i) host[i] = (float)wide[i];
+}
+
 }  // namespace
 
 extern "C" {
@@ -149,7 +163,7 @@ double mpr_residual_f64(const double* solution, const double* rhs, const double*
     const Geom g = make_geom((int32_t)n);
     const size_t M = (size_t)n * n * n, W = 2ull * g.C;
     DevBuf<double> nat(M), x(W), r(W), s(W), out(W), norm(1);
-    DevBuf<double> part(blocks_for(W, kThreads * kNodesPerThread));
+    DevBuf<double> part((size_t)residual_partial_count(g, sizeof(double)));
     h2d(nat.p, solution, M);
     launch_nat_to_split<double, double>(nat.p, x.p, g, kNull);
     h2d(nat.p, rhs, M);
@@ -167,6 +181,56 @@ double mpr_residual_f64(const double* solution, const double* rhs, const double*
   } catch (const HipError& e) {
     report(__func__, e.msg);
     return kNaN;
+  }
+}
+
+// FP32 instantiations of the smoother and residual (the mixed-precision V-cycle's
+// kernels) on host float32 grids: staged through the same layout conversions.
+int mpr_relax_f32(float* solution, const float* rhs, const float* screen, int64_t n,
+                  double point_weight, int64_t iterations) {
+  if (n < 1 || n > 1025 || iterations < 0) {
+    set_error("mpr_relax_f32: n must be in [1, 1025] and iterations non-negative");
+    return -1;
+  }
+  if (ensure_device()) return -1;
+  try {
+    const Geom g = make_geom((int32_t)n);
+    const size_t M = (size_t)n * n * n, W = 2ull * g.C;
+    DevBuf<double> nat(M);
+    DevBuf<float> x(W), r(W), s(W);
+    upload_f32_as_split(solution, x.p, g, nat, M);
+    upload_f32_as_split(rhs, r.p, g, nat, M);
+    upload_f32_as_split(screen, s.p, g, nat, M);
+    launch_relax<float>(x.p, r.p, s.p, g, point_weight, (int)iterations, kNull);
+    download_split_f32(x.p, solution, g, nat, M);
+    return 0;
+  } catch (const HipError& e) {
+    report(__func__, e.msg);
+    return -1;
+  }
+}
+
+int mpr_residual_f32(const float* solution, const float* rhs, const float* screen,
+                     float* residual, int64_t n, double point_weight) {
+  if (n < 1 || n > 1025) {
+    set_error("mpr_residual_f32: n must be in [1, 1025]");
+    return -1;
+  }
+  if (ensure_device()) return -1;
+  try {
+    const Geom g = make_geom((int32_t)n);
+    const size_t M = (size_t)n * n * n, W = 2ull * g.C;
+    DevBuf<double> nat(M);
+    DevBuf<float> x(W), r(W), s(W), out(W);
+    upload_f32_as_split(solution, x.p, g, nat, M);
+    upload_f32_as_split(rhs, r.p, g, nat, M);
+    upload_f32_as_split(screen, s.p, g, nat, M);
+    launch_residual<float, float>(x.p, r.p, s.p, out.p, g, point_weight, nullptr, kNull);
+    download_split_f32(out.p, residual, g, nat, M);
+    return 0;
+  } catch (const HipError& e) {
+    report(__func__, e.msg);
+    return -1;
   }
 }
 

@@ -47,6 +47,19 @@ __global__ __launch_bounds__(kThreads) void k_relax_color(T* X, const T* R, cons
   }
 }
 
+// slot-group form (one lane per 16 bytes of a colour row) for levels with aligned rows
+template <class T>
+__global__ __launch_bounds__(kThreads) void k_relax_group(T* X, const T* R, const T* S, Geom g,
+                                                          uint32_t lanes, int c, T pw, T h2) {
+  constexpr int V = 16 / (int)sizeof(T);
+  SlotGroup<T, V> G;
+  load_group<T, V>(as_global((const T*)X), as_global(R), as_global(S), g, c,
+                   logical_block() * kThreads + threadIdx.x, lanes, false, G);
+  T out[V];
+  const uint32_t mask = group_relax<T, V>(G, g.n, pw, h2, out);
+  store_group<T, T, V>(as_global(X) + G.own, out, mask);
+}
+
 __device__ __forceinline__ double block_sum(double v, double* lds) {
   // fixed-order reduction: wavefront butterfly, then the 4 wave totals in order
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -91,6 +104,34 @@ __global__ __launch_bounds__(kThreads) void k_residual(const T* X, const T* R, c
   if (partials) {
     double total = block_sum(sq, lds);
     if (threadIdx.x == 0 && lb * (kThreads * kNodesPerThread) < 2u * g.C) partials[lb] = total;
+  }
+}
+
+// residual over both colours in slot groups (lanes [0, lanes) colour 0, then colour 1);
+// block partials of the squared norm as k_residual
+template <class T, class TR>
+__global__ __launch_bounds__(kThreads) void k_residual_group(const T* X, const T* R, const T* S,
+                                                             TR* out, Geom g, uint32_t lanes,
+                                                             T pw, T h2, double* partials) {
+  constexpr int V = 16 / (int)sizeof(T);
+  __shared__ double lds[kThreads / 64];
+  const uint32_t lb = logical_block();
+  const uint32_t lane = lb * kThreads + threadIdx.x;
+  const int c = lane >= lanes;
+  SlotGroup<T, V> G;
+  load_group<T, V>(as_global(X), as_global(R), as_global(S), g, c, lane - (uint32_t)c * lanes,
+                   lanes, true, G);
+  if (lane >= 2u * lanes) G.in = false;
+  T v[V];
+  group_residual<T, V>(G, g.n, pw, h2, v);
+  store_group<T, TR, V>(as_global(out) + G.own, v, G.in ? (1u << V) - 1 : 0u);
+  if (partials) {
+    double sq = 0.0;
+    if (G.in)
+#pragma unroll
+      for (int j = 0; j < V; ++j) sq += (double)v[j] * (double)v[j];
+    const double total = block_sum(sq, lds);
+    if (threadIdx.x == 0 && lb * kThreads < 2u * lanes) partials[lb] = total;
   }
 }
 
@@ -220,6 +261,13 @@ template <class T>
 void launch_relax_color(T* X, const T* R, const T* S, const Geom& g, double pw, int c,
                         hipStream_t s) {
   const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
+  if (group_layout(g)) {
+    const uint32_t lanes = g.C / (uint32_t)(16 / sizeof(T));
+    hipLaunchKernelGGL(k_relax_group<T>, dim3(xcd_grid(lanes, kThreads)), dim3(kThreads), 0, s, X,
+                       R, S, g, lanes, c, (T)pw, h2);
+    MPR_LAUNCH_CHECK();
+    return;
+  }
   const unsigned nb = xcd_grid(g.C, kThreads * kNodesPerThread);
   hipLaunchKernelGGL(k_relax_color<T>, dim3(nb), dim3(kThreads), 0, s, X, R, S, g, c, (T)pw, h2);
   MPR_LAUNCH_CHECK();
@@ -236,10 +284,22 @@ template <class T, class TR>
 int launch_residual(const T* X, const T* R, const T* S, TR* out, const Geom& g, double pw,
                     double* partials, hipStream_t s) {
   const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
+  if (group_layout(g)) {
+    const uint32_t lanes = g.C / (uint32_t)(16 / sizeof(T));
+    hipLaunchKernelGGL((k_residual_group<T, TR>), dim3(xcd_grid(2ull * lanes, kThreads)),
+                       dim3(kThreads), 0, s, X, R, S, out, g, lanes, (T)pw, h2, partials);
+    MPR_LAUNCH_CHECK();
+    return (int)blocks_for(2ull * lanes, kThreads);
+  }
   const unsigned nb = xcd_grid(2ull * g.C, kThreads * kNodesPerThread);
   hipLaunchKernelGGL((k_residual<T, TR>), dim3(nb), dim3(kThreads), 0, s, X, R, S, out, g,
                      (T)pw, h2, partials);
   MPR_LAUNCH_CHECK();
+  return (int)blocks_for(2ull * g.C, kThreads * kNodesPerThread);
+}
+
+int residual_partial_count(const Geom& g, size_t word_bytes) {
+  if (group_layout(g)) return (int)blocks_for(2ull * (g.C / (uint32_t)(16 / word_bytes)), kThreads);
   return (int)blocks_for(2ull * g.C, kThreads * kNodesPerThread);
 }
 

@@ -5,9 +5,10 @@
 // * natural: (n,n,n) row-major [z][y][x], index (z*n + y)*n + x — the layout of every grid
 //   that crosses the API (capi.mojo:41-48).
 // * colour-split: the solver's layout. Node (x,y,z) has colour c = (x+y+z)&1. Each colour
-//   owns a block of C = n*n*H words, H = (n+1)/2, holding row (z,y) of that colour as
-//   H consecutive words, node x at slot x>>1 (rows whose first node of colour c is at
-//   x = 1 have one unused pad slot at the end). A red-black sweep of colour c then reads
+//   owns a block of C = n*n*H words, H >= (n+1)/2, holding row (z,y) of that colour as
+//   H consecutive words, node x at slot x>>1; the slots past the row's last node are
+//   unused pads (one for rows whose first node of colour c is at x = 1, plus the
+//   alignment padding of make_geom). A red-black sweep of colour c then reads
 //   and writes colour c contiguously and reads colour 1-c contiguously: no half-used
 //   cache lines, so a full Gauss-Seidel iteration moves 4 words per node.
 #pragma once
@@ -66,16 +67,23 @@ struct FastDiv {
 // geometry of one grid level in the colour-split layout
 struct Geom {
   int32_t n = 0;     // nodes per side
-  int32_t H = 0;     // slots per colour row, (n+1)/2
+  int32_t H = 0;     // slots per colour row, >= (n+1)/2 (make_geom)
   uint32_t C = 0;    // words per colour block, n*n*H
   uint32_t nH = 0;   // n*H
   FastDiv div_nH, div_H, div_C;
 };
 
+// Rows of levels with n >= 33 are padded to a multiple of 4 slots, so every colour row
+// starts on a 16-byte boundary (FP32 and FP64) and the streaming kernels read whole 16-byte
+// chunks (group_layout). Smaller levels, including every level the LDS coarse solve holds,
+// keep H = (n+1)/2.
+constexpr int32_t kGroupMinN = 33;
+
 inline Geom make_geom(int32_t n) {
   Geom g;
   g.n = n;
   g.H = (n + 1) / 2;
+  if (n >= kGroupMinN) g.H = (g.H + 3) & ~3;
   g.nH = (uint32_t)n * (uint32_t)g.H;
   g.C = g.nH * (uint32_t)n;
   g.div_nH = FastDiv(g.nH);
@@ -83,6 +91,9 @@ inline Geom make_geom(int32_t n) {
   g.div_C = FastDiv(g.C);
   return g;
 }
+
+// whether a level's rows are 16-byte aligned chunks (the slot-group kernels' precondition)
+MPR_HD bool group_layout(const Geom& g) { return (g.H & 3) == 0; }
 
 MPR_HD int64_t nat_index(int x, int y, int z, int n) { return ((int64_t)z * n + y) * n + x; }
 

@@ -31,6 +31,8 @@ void launch_relax_color(T* X, const T* R, const T* S, const Geom& g, double pw, 
 template <class T, class TR>
 int launch_residual(const T* X, const T* R, const T* S, TR* out, const Geom& g, double pw,
                     double* partials, hipStream_t s);
+// number of squared-norm partials launch_residual writes for a level of word_bytes words
+int residual_partial_count(const Geom& g, size_t word_bytes);
 // out = sqrt(sum partials) with a fixed summation order
 void launch_norm_finish(const double* partials, int count, double* out, hipStream_t s);
 template <class T>

@@ -309,7 +309,7 @@ void Job::plan(DevArena& a) {
   d_resnorm = a.alloc<double>(1);
   d_params = a.alloc<double>(4);
   d_counts = a.alloc<uint32_t>(2);
-  norm_nparts = (int)blocks_for(2ull * geoms[0].C, kThreads * kNodesPerThread);
+  norm_nparts = residual_partial_count(geoms[0], sizeof(double));
   norm_part = a.alloc<double>(norm_nparts);
   bbox_part = a.alloc<double>(6ull * bbox_partial_count(std::max<uint32_t>(1u, max_points)));
   iso_part = a.alloc<double>(2ull * iso_partial_count(std::max<uint32_t>(1u, max_points)));

@@ -6,6 +6,8 @@
 // kernels and the single-workgroup coarse-level kernel both call these.
 #pragma once
 
+#include <utility>
+
 #include "mpr_common.h"
 
 namespace mpr {
@@ -138,6 +140,186 @@ MPR_HD T loaded_residual(const NodeLoads<T>& L, T pw, T h2) {
   const T screening = pw * L.s * h2;
   const T applied = ((T)deg * h2 + screening) * L.self - s * h2;
   return L.r - applied;
+}
+
+// ---- slot-group form: V consecutive slots of one colour row per lane -------------------
+// On levels whose row length H is a multiple of 4 (make_geom pads rows of n >= 33), a lane
+// takes slots k0..k0+V-1 of row (y,z) in colour c and reads every operand with 16-byte
+// loads: the same-row neighbours (x-1, x+1) are the other colour's slots k0..k0+V-1 plus
+// one edge slot (k0-1 for rows whose first colour-c node is at even x, k0+V otherwise),
+// and the y/z neighbours are the same V slots of the other colour's rows (y+-1, z+-1).
+// Per node the arithmetic is relax_node / residual_node's, in the same order.
+template <class T>
+struct Chunk16;
+template <>
+struct Chunk16<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct Chunk16<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+};
+
+template <class T, int V>
+struct SlotGroup {
+  static constexpr int W = 16 / (int)sizeof(T);  // elements per 16-byte chunk
+  static constexpr int NC = V / W;               // chunks per operand
+  static_assert(V % W == 0, "a slot group is a whole number of 16-byte chunks");
+  using CT = typename Chunk16<T>::type;
+  CT row[NC], ym[NC], yp[NC], zm[NC], zp[NC], r[NC], s[NC], self[NC];
+  T edge;
+  uint32_t own;  // index of the group's first slot (colour block offset included)
+  int x0, y, z, p;
+  bool in;  // lane inside the level (lanes past the end load clamped and store nothing)
+};
+
+template <class T, class CT, int NC>
+__device__ __forceinline__ void load_chunks(CT (&d)[NC], gptr<const T> p) {
+  constexpr int W = 16 / (int)sizeof(T);
+#pragma unroll
+  for (int i = 0; i < NC; ++i) d[i] = *(const __attribute__((address_space(1))) CT*)(p + i * W);
+}
+
+// lane: index of the group inside colour block c; lanes = C / V
+template <class T, int V>
+__device__ __forceinline__ void load_group(gptr<const T> X, gptr<const T> R, gptr<const T> S,
+                                           const Geom& g, int c, uint32_t lane, uint32_t lanes,
+                                           bool want_self, SlotGroup<T, V>& G) {
+  G.in = lane < lanes;
+  lane = G.in ? lane : lanes - 1;
+  const uint32_t s0 = lane * (uint32_t)V;
+  const uint32_t z = g.div_nH.div(s0);
+  const uint32_t rr = s0 - z * g.nH;
+  const uint32_t y = g.div_H.div(rr);
+  const uint32_t k0 = rr - y * (uint32_t)g.H;
+  const int p = (c ^ (int)y ^ (int)z) & 1;
+  const uint32_t o = (uint32_t)(1 - c) * g.C + s0;
+  G.own = (uint32_t)c * g.C + s0;
+  G.x0 = 2 * (int)k0 + p;
+  G.y = (int)y;
+  G.z = (int)z;
+  G.p = p;
+  const int n = g.n;
+  load_chunks<T>(G.row, X + o);
+  const uint32_t eo = p == 0 ? (k0 > 0 ? o - 1 : o) : (k0 + V < (uint32_t)g.H ? o + V : o);
+  G.edge = X[eo];
+  load_chunks<T>(G.ym, X + (y > 0 ? o - (uint32_t)g.H : o));
+  load_chunks<T>(G.yp, X + ((int)y + 1 < n ? o + (uint32_t)g.H : o));
+  load_chunks<T>(G.zm, X + (z > 0 ? o - g.nH : o));
+  load_chunks<T>(G.zp, X + ((int)z + 1 < n ? o + g.nH : o));
+  load_chunks<T>(G.r, R + G.own);
+  load_chunks<T>(G.s, S + G.own);
+  if (want_self) load_chunks<T>(G.self, X + G.own);
+}
+
+// element J of an operand (J is a compile-time constant, so every access is static)
+template <int J, class T, class CT, int NC>
+__device__ __forceinline__ T elem(const CT (&d)[NC]) {
+  constexpr int W = 16 / (int)sizeof(T);
+  return d[J / W][J % W];
+}
+
+// neighbour sum and degree of slot J, terms in the order of neighbour_sum
+template <int J, class T, int V>
+__device__ __forceinline__ T group_sum(const SlotGroup<T, V>& G, int n, int& deg) {
+  const T cur = elem<J, T>(G.row);
+  T prev, next;
+  if constexpr (J == 0)
+    prev = G.edge;
+  else
+    prev = elem<J - 1, T>(G.row);
+  if constexpr (J == V - 1)
+    next = G.edge;
+  else
+    next = elem<J + 1, T>(G.row);
+  const T v0 = G.p ? cur : prev;  // x - 1
+  const T v1 = G.p ? next : cur;  // x + 1
+  const int x = G.x0 + 2 * J;
+  const bool bx0 = x > 0, bx1 = x + 1 < n, by0 = G.y > 0, by1 = G.y + 1 < n, bz0 = G.z > 0,
+             bz1 = G.z + 1 < n;
+  T s = T(0);
+  if (bx0) s += v0;
+  if (bx1) s += v1;
+  if (by0) s += elem<J, T>(G.ym);
+  if (by1) s += elem<J, T>(G.yp);
+  if (bz0) s += elem<J, T>(G.zm);
+  if (bz1) s += elem<J, T>(G.zp);
+  deg = (int)bx0 + (int)bx1 + (int)by0 + (int)by1 + (int)bz0 + (int)bz1;
+  return s;
+}
+
+template <int J, class T, int V>
+__device__ __forceinline__ void group_relax_one(const SlotGroup<T, V>& G, int n, T pw, T h2,
+                                                T (&out)[V], uint32_t& mask) {
+  int deg;
+  const T s = group_sum<J, T, V>(G, n, deg);
+  const T screening = pw * elem<J, T>(G.s) * h2;
+  const T diag = (T)deg * h2 + screening;
+  out[J] = (elem<J, T>(G.r) + s * h2) / diag;
+  if (G.in && G.x0 + 2 * J < n && diag > T(1.0e-30)) mask |= 1u << J;
+}
+
+template <int J, class T, int V>
+__device__ __forceinline__ T group_residual_one(const SlotGroup<T, V>& G, int n, T pw, T h2) {
+  int deg;
+  const T s = group_sum<J, T, V>(G, n, deg);
+  const T screening = pw * elem<J, T>(G.s) * h2;
+  const T applied = ((T)deg * h2 + screening) * elem<J, T>(G.self) - s * h2;
+  return G.x0 + 2 * J < n ? elem<J, T>(G.r) - applied : T(0);
+}
+
+template <class T, int V, int... J>
+__device__ __forceinline__ uint32_t group_relax_all(const SlotGroup<T, V>& G, int n, T pw, T h2,
+                                                    T (&out)[V],
+                                                    std::integer_sequence<int, J...>) {
+  uint32_t mask = 0;
+  (group_relax_one<J, T, V>(G, n, pw, h2, out, mask), ...);
+  return mask;
+}
+
+// Relaxed values of the group's V slots; returns the mask of slots to store.
+template <class T, int V>
+__device__ __forceinline__ uint32_t group_relax(const SlotGroup<T, V>& G, int n, T pw, T h2,
+                                                T (&out)[V]) {
+  return group_relax_all<T, V>(G, n, pw, h2, out, std::make_integer_sequence<int, V>{});
+}
+
+template <class T, int V, int... J>
+__device__ __forceinline__ void group_residual_all(const SlotGroup<T, V>& G, int n, T pw, T h2,
+                                                   T (&out)[V],
+                                                   std::integer_sequence<int, J...>) {
+  ((out[J] = group_residual_one<J, T, V>(G, n, pw, h2)), ...);
+}
+
+// Residuals of the group's V slots (0 in pad slots, as the scalar kernels write).
+template <class T, int V>
+__device__ __forceinline__ void group_residual(const SlotGroup<T, V>& G, int n, T pw, T h2,
+                                               T (&out)[V]) {
+  group_residual_all<T, V>(G, n, pw, h2, out, std::make_integer_sequence<int, V>{});
+}
+
+// Stores V values at p (16-byte aligned): whole chunks when every slot is stored, else
+// slot by slot.
+template <class T, class U, int V>
+__device__ __forceinline__ void store_group(__attribute__((address_space(1))) U* p,
+                                            const T (&v)[V], uint32_t mask) {
+  if constexpr (sizeof(U) == sizeof(T)) {
+    using CT = typename SlotGroup<T, V>::CT;
+    constexpr int W = SlotGroup<T, V>::W;
+    if (mask == (1u << V) - 1) {
+#pragma unroll
+      for (int i = 0; i < V / W; ++i) {
+        CT c;
+#pragma unroll
+        for (int j = 0; j < W; ++j) c[j] = v[i * W + j];
+        *(__attribute__((address_space(1))) CT*)(p + i * W) = c;
+      }
+      return;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < V; ++j)
+    if ((mask >> j) & 1u) p[j] = (U)v[j];
 }
 
 // Full-weighting restriction of coarse node (X,Y,Z) (capi.mojo:556-673): weights 2 at the

@@ -133,6 +133,61 @@ def residual(solution, rhs, screen, point_weight: float):
     return out, norm
 
 
+def _neighbour_sum_np(x: np.ndarray):
+    """capi.mojo:282-306 over a whole grid in x's dtype: terms x-1, x+1, y-1, y+1, z-1, z+1
+    (grids are [z, y, x]); an absent neighbour skips its addition."""
+    n = x.shape[0]
+    idx = np.arange(n)
+    s = np.zeros_like(x)
+    deg = np.zeros(x.shape, np.int64)
+    for axis in (2, 1, 0):
+        for shift in (1, -1):  # roll by +1 brings index i-1 to i (the "-1" neighbour)
+            v = np.roll(x, shift, axis=axis)
+            ok = idx > 0 if shift == 1 else idx < n - 1
+            ok = ok.reshape([n if a == axis else 1 for a in range(3)])
+            ok = np.broadcast_to(ok, x.shape)
+            s = np.where(ok, s + v, s)
+            deg += ok
+    return s, deg
+
+
+def relax_np(solution, rhs, screen, point_weight: float, iterations: int) -> np.ndarray:
+    """capi.mojo:309-410 in the arrays' own dtype (float32 restates the mixed-precision
+    smoother): red-black sweeps, colour 0 first, x = (rhs + h^-2 sum) / diag where
+    diag = deg h^-2 + (alpha screen) h^-2 > 1e-30. Every colour-c node reads only colour
+    1-c nodes, so a whole-grid update per colour is the serial sweep's result."""
+    x = np.array(solution, copy=True)
+    dt = x.dtype.type
+    n = x.shape[0]
+    h2 = dt(float(n - 1) * float(n - 1))
+    pw = dt(point_weight)
+    zz, yy, xx = np.indices(x.shape)
+    colour = (xx + yy + zz) & 1
+    screening = pw * screen * h2
+    for _ in range(iterations):
+        for c in (0, 1):
+            s, deg = _neighbour_sum_np(x)
+            diag = deg.astype(dt) * h2 + screening
+            with np.errstate(all="ignore"):
+                new = (rhs + s * h2) / diag
+            m = (colour == c) & (diag > dt(1.0e-30))
+            x[m] = new[m]
+    return x
+
+
+def residual_np(solution, rhs, screen, point_weight: float) -> np.ndarray:
+    """capi.mojo:413-470 in the arrays' own dtype: r = rhs - ((deg h^-2 + alpha s h^-2) x
+    - h^-2 sum)."""
+    x = np.asarray(solution)
+    dt = x.dtype.type
+    n = x.shape[0]
+    h2 = dt(float(n - 1) * float(n - 1))
+    pw = dt(point_weight)
+    s, deg = _neighbour_sum_np(x)
+    applied = (deg.astype(dt) * h2 + pw * screen * h2) * x - s * h2
+    return rhs - applied
+
+
 def restrict(fine) -> np.ndarray:
     fine = _f(fine)
     cn = (fine.shape[0] + 1) // 2

@@ -28,7 +28,7 @@ def rng_grids(n, seed):
     return rng.normal(size=(n, n, n)), rng.normal(size=(n, n, n)), rng.random((n, n, n))
 
 
-@pytest.mark.parametrize("n", [2, 9, 10, 65])
+@pytest.mark.parametrize("n", [2, 9, 10, 33, 34, 65, 66, 129])
 def test_relax_bit_exact(n):
     x, rhs, screen = rng_grids(n, 71)
     expected = oracle.relax(x, rhs, screen, 4.0, 3)
@@ -37,7 +37,7 @@ def test_relax_bit_exact(n):
     assert np.array_equal(actual, expected)
 
 
-@pytest.mark.parametrize("n", [9, 65, 129])
+@pytest.mark.parametrize("n", [9, 34, 65, 129])
 def test_residual_bit_exact_and_norm(n):
     x, rhs, screen = rng_grids(n, n)
     expected, norm = oracle.residual(x, rhs, screen, 4.0)
@@ -46,6 +46,32 @@ def test_residual_bit_exact_and_norm(n):
     assert np.array_equal(actual, expected)
     assert got == pytest.approx(norm, rel=1e-14)
     assert got == pytest.approx(np.linalg.norm(expected), rel=2e-15)
+
+
+def f32_grids(n, seed):
+    x, rhs, screen = rng_grids(n, seed)
+    return x.astype(np.float32), rhs.astype(np.float32), screen.astype(np.float32)
+
+
+@pytest.mark.parametrize("n", [3, 9, 17, 33, 34, 65, 66])
+def test_relax_f32_bit_exact_against_float32_restatement(n):
+    # the mixed-precision smoother: scalar kernels below n = 33, 16-byte slot groups above
+    x, rhs, screen = f32_grids(n, 300 + n)
+    expected = oracle.relax_np(x, rhs, screen, 4.0, 3)
+    actual = x.copy()
+    assert lib().mpr_relax_f32(actual.ctypes.data, rhs.ctypes.data, screen.ctypes.data, n, 4.0,
+                               3) == 0
+    assert np.array_equal(actual, expected)
+
+
+@pytest.mark.parametrize("n", [9, 33, 34, 65])
+def test_residual_f32_bit_exact_against_float32_restatement(n):
+    x, rhs, screen = f32_grids(n, 400 + n)
+    expected = oracle.residual_np(x, rhs, screen, 4.0)
+    actual = np.empty_like(x)
+    assert lib().mpr_residual_f32(x.ctypes.data, rhs.ctypes.data, screen.ctypes.data,
+                                  actual.ctypes.data, n, 4.0) == 0
+    assert np.array_equal(actual, expected)
 
 
 @pytest.mark.parametrize("n", [3, 9, 10, 17, 65])

@@ -309,3 +309,21 @@ def test_empty_solve_is_zero():
     sol = oracle.solve(np.empty((0, 3)), np.empty((0, 3)), 3)
     assert sol["residual_norm"] == 0.0 and sol["iso_value"] == 0.0
     assert not sol["field"].any()
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 9, 10])
+def test_numpy_smoother_restatement_equals_c_oracle_in_float64(n):
+    # pins relax_np / residual_np, whose float32 instantiation checks the FP32 kernels
+    rng = np.random.default_rng(50 + n)
+    x, rhs = rng.normal(size=(n, n, n)), rng.normal(size=(n, n, n))
+    screen = np.where(rng.random((n, n, n)) < 0.3, rng.random((n, n, n)), 0.0)
+    assert np.array_equal(oracle.relax_np(x, rhs, screen, 4.0, 3),
+                          oracle.relax(x, rhs, screen, 4.0, 3))
+    assert np.array_equal(oracle.residual_np(x, rhs, screen, 4.0),
+                          oracle.residual(x, rhs, screen, 4.0)[0])
+    x32, r32, s32 = x.astype(np.float32), rhs.astype(np.float32), screen.astype(np.float32)
+    got = oracle.relax_np(x32, r32, s32, 4.0, 3)
+    assert got.dtype == np.float32
+    want = oracle.relax(x32.astype(np.float64), r32.astype(np.float64), s32.astype(np.float64),
+                        4.0, 3)
+    assert np.allclose(got, want, rtol=1e-4, atol=1e-5 * np.abs(want).max())
