@@ -37,6 +37,11 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the roofline kernel (kb_relax_group<float>, one colour of the
+# 257^3 level): FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from rocprofv3 counter runs, the factors
+# calibrated on a plain elementwise copy (profiles/kb_relax_group_f32_depth8_pmc.txt).
+# Applies to the depth-8 workload only.
+ROOFLINE_TRAFFIC_BYTES = {8: 141_465_323}
 
 
 def log(*args):
@@ -216,6 +221,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the 64-batch and serving runs")
     ap.add_argument("--kernel-reps", type=int, default=40)
+    ap.add_argument("--only", choices=("fp64", "mixed"),
+                    help="time one single-request mode and print its line (for per-mode "
+                         "kernel traces)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="one reconstruction per precision, then only the kernel timings "
                          "(a short program for hardware-counter runs)")
@@ -238,7 +246,10 @@ def main() -> int:
     modes = {}
     meshes = {}
     jobs = {}
-    for name, prec in (("fp64", _lib.PRECISION_FP64), ("mixed", _lib.PRECISION_MIXED)):
+    runs = (("fp64", _lib.PRECISION_FP64), ("mixed", _lib.PRECISION_MIXED))
+    if args.only:
+        runs = tuple(r for r in runs if r[0] == args.only)
+    for name, prec in runs:
         job = Job(args.depth, len(points), prec)
         job.set_points(points, normals)
 
@@ -253,6 +264,14 @@ def main() -> int:
                            nv=int(r.vertex_count), iso=r.iso_value, res=r.residual_norm)
         jobs[name] = job
         log(f"{name}: {modes[name]['value']:.2f} recon/s, p50 {modes[name]['p50']} ms")
+    if args.only:
+        m = modes[args.only]
+        print(json.dumps({"mode": args.only, "recon_per_s": round(m["value"], 3),
+                          "p50_ms": m["p50"], "p99_ms": m["p99"], "vertices": m["nv"],
+                          "steps": args.steps, "warmup": args.warmup}), flush=True)
+        for job in jobs.values():
+            job.close()
+        return 0
     if args.kernels_only:
         kernels = time_kernels(jobs, args.kernel_reps)
         print(json.dumps({k: {"us": round(v["ms"] * 1e3, 2), "GBps": round(v["gbs"], 1)}
@@ -338,13 +357,15 @@ def main() -> int:
         "hausdorff_mixed_vs_fp64_over_diag": hd,
         "mesh": {"vertices": modes["mixed"]["nv"], "fp64_vertices": fp64["nv"]},
         "roofline": {
-            "kernel": "rb_sweep_f32: kb_relax_color<float>, one colour of the finest level",
+            "kernel": "rb_sweep_f32: kb_relax_group<float>, one colour of the finest level",
+            "algorithmic_bytes": round(dom["bytes"]),
             "bound": "hbm",
             "achieved": round(dom["gbs"], 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": ROOFLINE_TRAFFIC_BYTES.get(args.depth),
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)",
         },
         "kernels": {k: {"us": round(v["ms"] * 1e3, 2), "GBps": round(v["gbs"], 1)}
                     for k, v in kernels.items()},
