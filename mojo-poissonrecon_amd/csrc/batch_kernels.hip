@@ -162,6 +162,25 @@ __global__ __launch_bounds__(kThreads) void kb_prolong_add(const StepPtrs<T>* ta
   p.x[m.t] += prolong_node<T>(p.xc, gc, x, y, z);
 }
 
+// slot-group prolong-add of colours c_first .. c_first + ncol - 1 (lanes per colour each)
+template <class T>
+__global__ __launch_bounds__(kThreads) void kb_prolong_group(const StepPtrs<T>* tab,
+                                                             uint32_t count, uint32_t nbi, Geom gc,
+                                                             Geom gf, uint32_t lanes, int c_first,
+                                                             int ncol) {
+  const uint32_t per = gridDim.x >> 3;
+  const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  const uint32_t slot = lb / nbi;
+  if (slot >= count) return;
+  const StepPtrs<T> p = tab[slot];
+  const uint32_t lane = (lb - slot * nbi) * kThreads + threadIdx.x;
+  // lanes past the end keep a valid colour and load clamped (prolong_group stores nothing)
+  const int k = (ncol == 2 && lane >= lanes) ? 1 : 0;
+  const uint32_t l = lane < (uint32_t)ncol * lanes ? lane - (uint32_t)k * lanes : lanes;
+  prolong_group<T, kGroup<T>>(as_global((const T*)p.xc), gc, as_global(p.x), gf, c_first + k,
+                              l, lanes);
+}
+
 template <class T>
 __global__ __launch_bounds__(kThreads) void kb_fill_x(const StepPtrs<T>* tab, uint32_t count,
                                                       uint32_t nbi, uint32_t words) {
@@ -387,8 +406,17 @@ void launch_b_restrict(const StepPtrs<T>* tab, uint32_t count, const Geom& gf, c
 
 template <class T>
 void launch_b_prolong(const StepPtrs<T>* tab, uint32_t count, const Geom& gc, const Geom& gf,
-                      hipStream_t s) {
+                      bool colour1_only, hipStream_t s) {
   if (!count) return;
+  if (group_layout(gf)) {
+    const uint32_t lanes = gf.C / (uint32_t)kGroup<T>;
+    const int ncol = colour1_only ? 1 : 2;
+    const uint32_t nbi = blocks_for((uint64_t)ncol * lanes, kThreads);
+    hipLaunchKernelGGL(kb_prolong_group<T>, dim3(padded8((uint64_t)nbi * count)), dim3(kThreads),
+                       0, s, tab, count, nbi, gc, gf, lanes, colour1_only ? 1 : 0, ncol);
+    MPR_LAUNCH_CHECK();
+    return;
+  }
   const uint32_t nbi = blocks_for(2ull * gf.C, kThreads);
   hipLaunchKernelGGL(kb_prolong_add<T>, dim3(padded8((uint64_t)nbi * count)), dim3(kThreads), 0,
                      s, tab, count, nbi, gc, gf);
@@ -489,7 +517,7 @@ void init_coarse_kernels() {
   template void launch_b_restrict<T>(const StepPtrs<T>*, uint32_t, const Geom&, const Geom&,   \
                                      bool, hipStream_t);                                       \
   template void launch_b_prolong<T>(const StepPtrs<T>*, uint32_t, const Geom&, const Geom&,    \
-                                    hipStream_t);                                              \
+                                    bool, hipStream_t);                                        \
   template void launch_b_fill_x<T>(const StepPtrs<T>*, uint32_t, const Geom&, hipStream_t);    \
   template void launch_b_coarse<T>(const CoarseEntry<T>*, uint32_t, int, double, int, int,     \
                                    hipStream_t);                                               \

@@ -160,6 +160,18 @@ __global__ __launch_bounds__(kThreads) void k_restrict(const T* F, Geom gf, T* C
   Cc[t] = restrict_node(F, gf, X, Y, Z, scale);
 }
 
+// slot-group prolong-add over both colours (lanes [0, lanes) colour 0, then colour 1)
+template <class T>
+__global__ __launch_bounds__(kThreads) void k_prolong_group(const T* Cc, Geom gc, T* F, Geom gf,
+                                                            uint32_t lanes) {
+  constexpr int V = 16 / (int)sizeof(T);
+  const uint32_t lane = logical_block() * kThreads + threadIdx.x;
+  const int c = lane >= lanes;
+  const uint32_t l = lane - (uint32_t)c * lanes;
+  prolong_group<T, V>(as_global(Cc), gc, as_global(F), gf, c, lane < 2u * lanes ? l : lanes,
+                      lanes);
+}
+
 template <class T>
 __global__ __launch_bounds__(kThreads) void k_prolong_add(const T* Cc, Geom gc, T* F, Geom gf) {
   const uint32_t t = logical_block() * kThreads + threadIdx.x;
@@ -318,6 +330,13 @@ void launch_restrict(const T* F, const Geom& gf, T* Cc, const Geom& gc, double s
 
 template <class T>
 void launch_prolong_add(const T* Cc, const Geom& gc, T* F, const Geom& gf, hipStream_t s) {
+  if (group_layout(gf)) {
+    const uint32_t lanes = gf.C / (uint32_t)(16 / sizeof(T));
+    hipLaunchKernelGGL(k_prolong_group<T>, dim3(xcd_grid(2ull * lanes, kThreads)), dim3(kThreads),
+                       0, s, Cc, gc, F, gf, lanes);
+    MPR_LAUNCH_CHECK();
+    return;
+  }
   const unsigned nb = xcd_grid(2ull * gf.C, kThreads);
   hipLaunchKernelGGL(k_prolong_add<T>, dim3(nb), dim3(kThreads), 0, s, Cc, gc, F, gf);
   MPR_LAUNCH_CHECK();

@@ -52,9 +52,11 @@ void launch_b_residual(const StepPtrs<T>* tab, uint32_t count, const Geom& g, do
 template <class T>
 void launch_b_restrict(const StepPtrs<T>* tab, uint32_t count, const Geom& gf, const Geom& gc,
                        bool zero_coarse, hipStream_t s);
+// colour1_only: prolong only colour 1, for a level whose next step is a colour-0 sweep
+// (which overwrites every colour-0 node without reading it)
 template <class T>
 void launch_b_prolong(const StepPtrs<T>* tab, uint32_t count, const Geom& gc, const Geom& gf,
-                      hipStream_t s);
+                      bool colour1_only, hipStream_t s);
 template <class T>
 void launch_b_fill_x(const StepPtrs<T>* tab, uint32_t count, const Geom& g, hipStream_t s);
 void launch_b_defect(const TopPtrs* tab, uint32_t count, const Geom& g, double pw, hipStream_t s);

@@ -322,6 +322,59 @@ __device__ __forceinline__ void store_group(__attribute__((address_space(1))) U*
     if ((mask >> j) & 1u) p[j] = (U)v[j];
 }
 
+// Prolong-add of a slot group: fine slots k0..k0+V-1 of row (y,z) in colour c. All V nodes
+// have the same x parity p, so they share tx, ty, tz and the four coarse rows
+// (y0|y1, z0|z1); node j needs coarse x = k0+j and min(k0+j+1, nc-1), so each row is read
+// once at x = k0..k0+V (coarse nodes alternate colour along x). Per node the arithmetic is
+// prolong_node's followed by fine += value.
+template <class T, int V>
+__device__ __forceinline__ void prolong_group(gptr<const T> XC, const Geom& gc, gptr<T> F,
+                                              const Geom& gf, int c, uint32_t lane,
+                                              uint32_t lanes) {
+  const bool in = lane < lanes;
+  lane = in ? lane : lanes - 1;
+  const uint32_t s0 = lane * (uint32_t)V;
+  const uint32_t z = gf.div_nH.div(s0);
+  const uint32_t rr = s0 - z * gf.nH;
+  const uint32_t y = gf.div_H.div(rr);
+  const uint32_t k0 = rr - y * (uint32_t)gf.H;
+  const int p = (c ^ (int)y ^ (int)z) & 1;
+  const uint32_t own = (uint32_t)c * gf.C + s0;
+  typename SlotGroup<T, V>::CT xf[SlotGroup<T, V>::NC];
+  load_chunks<T>(xf, (gptr<const T>)F + own);
+  const int nc = gc.n;
+  const int y0 = (int)y >> 1, z0 = (int)z >> 1;
+  const int y1 = y0 + 1 < nc - 1 ? y0 + 1 : nc - 1;
+  const int z1 = z0 + 1 < nc - 1 ? z0 + 1 : nc - 1;
+  const T tx = p ? T(0.5) : T(0), ty = (y & 1) ? T(0.5) : T(0), tz = (z & 1) ? T(0.5) : T(0);
+  T cv[4][V + 1];  // rows (y0,z0), (y1,z0), (y0,z1), (y1,z1) at coarse x = k0..k0+V
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int yr = (r & 1) ? y1 : y0, zr = (r & 2) ? z1 : z0;
+    const uint32_t row = ((uint32_t)zr * (uint32_t)nc + (uint32_t)yr) * (uint32_t)gc.H;
+#pragma unroll
+    for (int i = 0; i <= V; ++i) {
+      const int X = (int)k0 + i < nc - 1 ? (int)k0 + i : nc - 1;
+      const uint32_t cc = (uint32_t)((X + yr + zr) & 1);
+      cv[r][i] = XC[cc * gc.C + row + (uint32_t)(X >> 1)];
+    }
+  }
+  T out[V];
+  uint32_t mask = 0;
+#pragma unroll
+  for (int j = 0; j < V; ++j) {
+    const T c00 = cv[0][j] * (T(1) - tx) + cv[0][j + 1] * tx;
+    const T c10 = cv[1][j] * (T(1) - tx) + cv[1][j + 1] * tx;
+    const T c01 = cv[2][j] * (T(1) - tx) + cv[2][j + 1] * tx;
+    const T c11 = cv[3][j] * (T(1) - tx) + cv[3][j + 1] * tx;
+    const T c0 = c00 * (T(1) - ty) + c10 * ty;
+    const T c1 = c01 * (T(1) - ty) + c11 * ty;
+    out[j] = xf[j / SlotGroup<T, V>::W][j % SlotGroup<T, V>::W] + (c0 * (T(1) - tz) + c1 * tz);
+    if (in && 2 * ((int)k0 + j) + p < gf.n) mask |= 1u << j;
+  }
+  store_group<T, T, V>(F + own, out, mask);
+}
+
 // Full-weighting restriction of coarse node (X,Y,Z) (capi.mojo:556-673): weights 2 at the
 // centre and 1 at the sides per axis, terms in (oz, oy, ox) order, divided by the sum of
 // the in-grid weights, then multiplied by `scale` (1 for residuals, 4 for the screening

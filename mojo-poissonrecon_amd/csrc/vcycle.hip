@@ -106,7 +106,11 @@ void BatchedVcycle<T>::run(double pw, int pre, int post, bool zero_tops, hipStre
                      post, s);
   for (int c = sc + 1; c <= smax; ++c) {
     if (!step_count_[c]) continue;
-    launch_b_prolong<T>(step_[c], step_count_[c], geom_[c - 1], geom_[c], s);
+    // The post-smoothing's first colour-0 sweep overwrites every colour-0 node without
+    // reading it (its diagonal is >= 3 h^-2 > 0 for point_weight >= 0), so only colour 1
+    // needs the correction.
+    launch_b_prolong<T>(step_[c], step_count_[c], geom_[c - 1], geom_[c], post > 0 && pw >= 0.0,
+                        s);
     launch_b_relax<T>(step_[c], step_count_[c], geom_[c], pw, post, s);
   }
 }

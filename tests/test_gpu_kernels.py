@@ -83,7 +83,7 @@ def test_restrict_bit_exact(n):
     assert np.array_equal(coarse, oracle.restrict(fine))
 
 
-@pytest.mark.parametrize("n", [3, 9, 10, 65])
+@pytest.mark.parametrize("n", [3, 9, 10, 33, 34, 65, 66, 129])
 def test_prolong_add_bit_exact(n):
     rng = np.random.default_rng(5)
     cn = (n + 1) // 2
