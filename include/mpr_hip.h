@@ -65,6 +65,10 @@ int mpr_relax_f32(float *solution, const float *rhs, const float *screen, int64_
                   double point_weight, int64_t iterations);
 int mpr_residual_f32(const float *solution, const float *rhs, const float *screen,
                      float *residual, int64_t n, double point_weight);
+/* FP32 instantiations of mpr_restrict_f64 / mpr_prolong_add_f64 (the mixed-precision
+ * V-cycle's grid transfers), same contract. No reference counterpart. */
+int mpr_restrict_f32(const float *fine, float *coarse, int64_t fine_n);
+int mpr_prolong_add_f32(const float *coarse, float *fine, int64_t fine_n);
 
 /* Replaces mpr_restrict_f64 (src/capi.mojo:556-673; _lib.py:27). */
 void mpr_restrict_f64(const double *fine, double *coarse, int64_t fine_n);
@@ -193,7 +197,8 @@ int mpr_batch_wait(mpr_batch *batch, mpr_result *results);
  * sweep, 1 FP32 colour sweep (mixed jobs), 2 FP64 residual, 3 FP64 restriction,
  * 4 FP64 prolongation, 5 FP32 residual, 6 FP64->FP32 defect residual; the batched V-cycle
  * kernels the pipeline itself launches: 7 FP32 colour sweep, 8 FP64 colour sweep, 9 FP32
- * residual. No reference counterpart (measurement hook). */
+ * residual, 10 FP32 restriction, 11 FP32 prolongation of colour 1, 12 FP64 restriction.
+ * No reference counterpart (measurement hook). */
 int mpr_job_time_kernel(mpr_job *job, int32_t kernel, int32_t reps, double *ms_per_launch,
                         double *bytes_per_launch);
 

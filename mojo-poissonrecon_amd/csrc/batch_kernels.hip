@@ -151,6 +151,24 @@ __global__ __launch_bounds__(kThreads) void kb_restrict(const StepPtrs<T>* tab, 
 }
 
 template <class T>
+__global__ __launch_bounds__(kThreads) void kb_restrict_group(const StepPtrs<T>* tab,
+                                                              uint32_t count, uint32_t nbi, Geom gf,
+                                                              Geom gc, uint32_t lanes,
+                                                              int zero_coarse) {
+  const uint32_t per = gridDim.x >> 3;
+  const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  const uint32_t slot = lb / nbi;
+  if (slot >= count) return;
+  const StepPtrs<T> p = tab[slot];
+  const uint32_t lane = (lb - slot * nbi) * kThreads + threadIdx.x;
+  const int c = lane >= lanes;
+  const uint32_t l = lane < 2u * lanes ? lane - (uint32_t)c * lanes : lanes;
+  restrict_group<T, kGroup<T>>(as_global((const T*)p.res), gf, as_global(p.rhsc),
+                               zero_coarse ? as_global(p.xc) : gptr<T>(nullptr), gc, c, l, lanes,
+                               T(1));
+}
+
+template <class T>
 __global__ __launch_bounds__(kThreads) void kb_prolong_add(const StepPtrs<T>* tab, uint32_t count,
                                                            uint32_t nbi, Geom gc, Geom gf) {
   const BlockMap m = block_map(nbi);
@@ -241,22 +259,106 @@ struct CoarseClassInfo {
 };
 __constant__ CoarseClassInfo g_coarse_info[kMaxCoarseClass + 1];
 
+// Per-node constants of one smoothing call on an LDS-resident level: the LDS indices of the
+// six neighbours (the node's own slot where one is absent), which of them exist, the rhs
+// and the diagonal deg h^-2 + (alpha screen) h^-2 -- all fixed while x is iterated, so
+// they are computed once per call instead of once per sweep.
+template <class T>
+struct CoarseNode {
+  uint32_t idx, nb[6], flags;
+  T r, diag;
+  bool live;  // a real node whose update is applied (diag > 1e-30, as relax_node)
+};
+
+template <class T>
+__device__ __forceinline__ void coarse_node(const T* rhs, const T* scr, const Geom& g, int c,
+                                            uint32_t t, T pw, T h2, CoarseNode<T>& N) {
+  int x = 0, y = 0, z = 0;
+  const bool in = t < g.C;
+  const bool ok = in && split_decode(g, c, in ? t : 0u, x, y, z);
+  const int n = g.n;
+  const bool bx0 = x > 0, bx1 = x + 1 < n, by0 = y > 0, by1 = y + 1 < n, bz0 = z > 0,
+             bz1 = z + 1 < n;
+  const uint32_t self = (uint32_t)c * g.C + (in ? t : 0u);
+  N.idx = self;
+  N.nb[0] = bx0 ? split_index(g, x - 1, y, z) : self;
+  N.nb[1] = bx1 ? split_index(g, x + 1, y, z) : self;
+  N.nb[2] = by0 ? split_index(g, x, y - 1, z) : self;
+  N.nb[3] = by1 ? split_index(g, x, y + 1, z) : self;
+  N.nb[4] = bz0 ? split_index(g, x, y, z - 1) : self;
+  N.nb[5] = bz1 ? split_index(g, x, y, z + 1) : self;
+  N.flags = (uint32_t)bx0 | ((uint32_t)bx1 << 1) | ((uint32_t)by0 << 2) | ((uint32_t)by1 << 3) |
+            ((uint32_t)bz0 << 4) | ((uint32_t)bz1 << 5);
+  const int deg = (int)bx0 + (int)bx1 + (int)by0 + (int)by1 + (int)bz0 + (int)bz1;
+  const T screening = pw * scr[self] * h2;
+  N.diag = (T)deg * h2 + screening;
+  N.r = rhs[self];
+  N.live = ok && N.diag > T(1.0e-30);
+}
+
+// relax_node's update of one node from its constants
+template <class T>
+__device__ __forceinline__ void coarse_update(T* x, const CoarseNode<T>& N, T h2) {
+  if (!N.live) return;
+  T v[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) v[j] = x[N.nb[j]];
+  T s = T(0);
+#pragma unroll
+  for (int j = 0; j < 6; ++j)
+    if ((N.flags >> j) & 1u) s += v[j];
+  x[N.idx] = (N.r + s * h2) / N.diag;
+}
+
+// Red-black sweeps of an LDS-resident level by the workgroup (capi.mojo:309-410). Levels
+// with at most 128 slots per colour (n <= 5) are swept by the first wavefront alone, with
+// wavefront-level ordering between the colours instead of workgroup barriers; larger ones
+// by all 1024 lanes, at most three nodes per lane and colour (n <= 17).
 template <class T>
 __device__ __forceinline__ void wg_relax(T* x, const T* rhs, const T* scr, const Geom& g, T pw,
                                          int iterations) {
+  constexpr int kPer = 3;
   const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
+  if (g.C <= 2u * 64u) {
+    if (threadIdx.x < 64) {
+      CoarseNode<T> N[2][2];
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int k = 0; k < 2; ++k) coarse_node(rhs, scr, g, c, threadIdx.x + 64u * k, pw, h2, N[c][k]);
 #pragma unroll 1
-  for (int it = 0; it < iterations; ++it)
-#pragma unroll 1
-    for (int c = 0; c < 2; ++c) {
-#pragma unroll 1
-      for (uint32_t t = threadIdx.x; t < g.C; t += blockDim.x) {
-        int xx, yy, zz;
-        if (split_decode(g, c, t, xx, yy, zz))
-          relax_node(x, rhs, scr, g, (uint32_t)c * g.C + t, xx, yy, zz, pw, h2);
-      }
-      __syncthreads();
+      for (int it = 0; it < iterations; ++it)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+#pragma unroll
+          for (int k = 0; k < 2; ++k) coarse_update(x, N[c][k], h2);
+          // one wavefront: its LDS accesses complete in program order
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
     }
+    __syncthreads();
+    return;
+  }
+  {
+    // n <= 17 (kMaxCoarseClass 4): C <= 17 * 17 * 9 = 2601 <= kPer * kCoarseThreads
+    static_assert(kMaxCoarseClass <= 4, "coarse levels above n = 17 need more nodes per lane");
+    CoarseNode<T> N[2][kPer];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int k = 0; k < kPer; ++k)
+        coarse_node(rhs, scr, g, c, threadIdx.x + (uint32_t)kCoarseThreads * k, pw, h2, N[c][k]);
+#pragma unroll 1
+    for (int it = 0; it < iterations; ++it)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) coarse_update(x, N[c][k], h2);
+        __syncthreads();
+      }
+  }
 }
 
 // One workgroup per instance: the instance's V-cycle from its top coarse class down
@@ -273,7 +375,7 @@ __global__ __launch_bounds__(kCoarseThreads) void kb_coarse(const CoarseEntry<T>
   {
     const uint32_t m0 = 2u * g_coarse_info[top].g.C;
     T* x0 = sm + g_coarse_info[top].off;
-    for (uint32_t i = threadIdx.x; i < m0; i += blockDim.x) {
+    for (uint32_t i = threadIdx.x; i < m0; i += kCoarseThreads) {
       x0[i] = e.zero_top ? T(0) : e.x[i];
       x0[m0 + i] = e.rhs[i];
     }
@@ -282,7 +384,7 @@ __global__ __launch_bounds__(kCoarseThreads) void kb_coarse(const CoarseEntry<T>
       const uint32_t m = 2u * g_coarse_info[s].g.C;
       T* scr = sm + g_coarse_info[s].off + 2 * m;
       const T* src = e.screen[s];
-      for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) scr[i] = src[i];
+      for (uint32_t i = threadIdx.x; i < m; i += kCoarseThreads) scr[i] = src[i];
     }
   }
   __syncthreads();
@@ -297,7 +399,7 @@ __global__ __launch_bounds__(kCoarseThreads) void kb_coarse(const CoarseEntry<T>
     wg_relax(x, rhs, scr, g, pw, pre);
     const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
 #pragma unroll 1
-    for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) {
+    for (uint32_t t = threadIdx.x; t < m; t += kCoarseThreads) {
       const int c = t >= g.C;
       int xx, yy, zz;
       if (split_decode(g, c, t - (uint32_t)c * g.C, xx, yy, zz))
@@ -309,7 +411,7 @@ __global__ __launch_bounds__(kCoarseThreads) void kb_coarse(const CoarseEntry<T>
     T* xc = sm + g_coarse_info[s - 1].off;
     T* rhsc = xc + mc;
 #pragma unroll 1
-    for (uint32_t t = threadIdx.x; t < mc; t += blockDim.x) {
+    for (uint32_t t = threadIdx.x; t < mc; t += kCoarseThreads) {
       const int c = t >= gc.C;
       int X, Y, Z;
       if (split_decode(gc, c, t - (uint32_t)c * gc.C, X, Y, Z))
@@ -332,7 +434,7 @@ __global__ __launch_bounds__(kCoarseThreads) void kb_coarse(const CoarseEntry<T>
     T* x = sm + g_coarse_info[s].off;
     const T* xc = sm + g_coarse_info[s - 1].off;
 #pragma unroll 1
-    for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) {
+    for (uint32_t t = threadIdx.x; t < m; t += kCoarseThreads) {
       const int c = t >= g.C;
       int xx, yy, zz;
       if (split_decode(g, c, t - (uint32_t)c * g.C, xx, yy, zz))
@@ -343,7 +445,7 @@ __global__ __launch_bounds__(kCoarseThreads) void kb_coarse(const CoarseEntry<T>
   }
   const uint32_t m0 = 2u * g_coarse_info[top].g.C;
   const T* x0 = sm + g_coarse_info[top].off;
-  for (uint32_t i = threadIdx.x; i < m0; i += blockDim.x) e.x[i] = x0[i];
+  for (uint32_t i = threadIdx.x; i < m0; i += kCoarseThreads) e.x[i] = x0[i];
 }
 
 inline unsigned padded8(uint64_t blocks) { return (unsigned)((blocks + 7u) & ~7ull); }
@@ -398,6 +500,14 @@ template <class T>
 void launch_b_restrict(const StepPtrs<T>* tab, uint32_t count, const Geom& gf, const Geom& gc,
                        bool zero_coarse, hipStream_t s) {
   if (!count) return;
+  if (sizeof(T) == 4 && group_layout(gc) && group_layout(gf)) {  // as launch_restrict
+    const uint32_t lanes = gc.C / (uint32_t)kGroup<T>;
+    const uint32_t nbi = blocks_for(2ull * lanes, kThreads);
+    hipLaunchKernelGGL(kb_restrict_group<T>, dim3(padded8((uint64_t)nbi * count)),
+                       dim3(kThreads), 0, s, tab, count, nbi, gf, gc, lanes, zero_coarse ? 1 : 0);
+    MPR_LAUNCH_CHECK();
+    return;
+  }
   const uint32_t nbi = blocks_for(2ull * gc.C, kThreads);
   hipLaunchKernelGGL(kb_restrict<T>, dim3(padded8((uint64_t)nbi * count)), dim3(kThreads), 0, s,
                      tab, count, nbi, gf, gc, zero_coarse ? 1 : 0);

@@ -234,6 +234,51 @@ int mpr_residual_f32(const float* solution, const float* rhs, const float* scree
   }
 }
 
+int mpr_restrict_f32(const float* fine, float* coarse, int64_t fine_n) {
+  if (fine_n < 1 || fine_n > 1025) {
+    set_error("mpr_restrict_f32: fine_n must be in [1, 1025]");
+    return -1;
+  }
+  if (ensure_device()) return -1;
+  try {
+    const int64_t cn = (fine_n + 1) / 2;
+    const Geom gf = make_geom((int32_t)fine_n), gc = make_geom((int32_t)cn);
+    const size_t Mf = (size_t)fine_n * fine_n * fine_n, Mc = (size_t)cn * cn * cn;
+    DevBuf<double> nat(Mf);
+    DevBuf<float> f(2ull * gf.C), c(2ull * gc.C);
+    upload_f32_as_split(fine, f.p, gf, nat, Mf);
+    launch_restrict<float>(f.p, gf, c.p, gc, 1.0, kNull);
+    download_split_f32(c.p, coarse, gc, nat, Mc);
+    return 0;
+  } catch (const HipError& e) {
+    report(__func__, e.msg);
+    return -1;
+  }
+}
+
+int mpr_prolong_add_f32(const float* coarse, float* fine, int64_t fine_n) {
+  if (fine_n < 2 || fine_n > 1025) {
+    set_error("mpr_prolong_add_f32: fine_n must be in [2, 1025]");
+    return -1;
+  }
+  if (ensure_device()) return -1;
+  try {
+    const int64_t cn = (fine_n + 1) / 2;
+    const Geom gf = make_geom((int32_t)fine_n), gc = make_geom((int32_t)cn);
+    const size_t Mf = (size_t)fine_n * fine_n * fine_n, Mc = (size_t)cn * cn * cn;
+    DevBuf<double> nat(Mf);
+    DevBuf<float> f(2ull * gf.C), c(2ull * gc.C);
+    upload_f32_as_split(coarse, c.p, gc, nat, Mc);
+    upload_f32_as_split(fine, f.p, gf, nat, Mf);
+    launch_prolong_add<float>(c.p, gc, f.p, gf, kNull);
+    download_split_f32(f.p, fine, gf, nat, Mf);
+    return 0;
+  } catch (const HipError& e) {
+    report(__func__, e.msg);
+    return -1;
+  }
+}
+
 void mpr_restrict_f64(const double* fine, double* coarse, int64_t fine_n) {
   if (fine_n < 1 || ensure_device()) return;
   try {

@@ -160,6 +160,17 @@ __global__ __launch_bounds__(kThreads) void k_restrict(const T* F, Geom gf, T* C
   Cc[t] = restrict_node(F, gf, X, Y, Z, scale);
 }
 
+// slot-group restriction over both coarse colours (lanes [0, lanes) colour 0, then 1)
+template <class T>
+__global__ __launch_bounds__(kThreads) void k_restrict_group(const T* F, Geom gf, T* Cc, Geom gc,
+                                                             uint32_t lanes, T scale) {
+  constexpr int V = 16 / (int)sizeof(T);
+  const uint32_t lane = logical_block() * kThreads + threadIdx.x;
+  const int c = lane >= lanes;
+  const uint32_t l = lane < 2u * lanes ? lane - (uint32_t)c * lanes : lanes;
+  restrict_group<T, V>(as_global(F), gf, as_global(Cc), gptr<T>(nullptr), gc, c, l, lanes, scale);
+}
+
 // slot-group prolong-add over both colours (lanes [0, lanes) colour 0, then colour 1)
 template <class T>
 __global__ __launch_bounds__(kThreads) void k_prolong_group(const T* Cc, Geom gc, T* F, Geom gf,
@@ -323,6 +334,15 @@ void launch_norm_finish(const double* partials, int count, double* out, hipStrea
 template <class T>
 void launch_restrict(const T* F, const Geom& gf, T* Cc, const Geom& gc, double scale,
                      hipStream_t s) {
+  // FP32 only: with 8-byte words the grouped form reads 22.5 loads per coarse node against
+  // the scalar form's 27 and measured slower
+  if (sizeof(T) == 4 && group_layout(gc) && group_layout(gf)) {
+    const uint32_t lanes = gc.C / (uint32_t)(16 / sizeof(T));
+    hipLaunchKernelGGL(k_restrict_group<T>, dim3(xcd_grid(2ull * lanes, kThreads)),
+                       dim3(kThreads), 0, s, F, gf, Cc, gc, lanes, (T)scale);
+    MPR_LAUNCH_CHECK();
+    return;
+  }
   const unsigned nb = xcd_grid(2ull * gc.C, kThreads);
   hipLaunchKernelGGL(k_restrict<T>, dim3(nb), dim3(kThreads), 0, s, F, gf, Cc, gc, (T)scale);
   MPR_LAUNCH_CHECK();

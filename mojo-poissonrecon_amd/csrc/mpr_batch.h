@@ -96,6 +96,13 @@ class BatchedVcycle {
   void residual_finest(double pw, hipStream_t s) const {
     launch_b_residual<T>(step_[smax], step_count_[smax], geom_[smax], pw, s);
   }
+  void restrict_finest(hipStream_t s) const {
+    launch_b_restrict<T>(step_[smax], step_count_[smax], geom_[smax], geom_[smax - 1], false, s);
+  }
+  void prolong_finest(bool colour1_only, hipStream_t s) const {
+    launch_b_prolong<T>(step_[smax], step_count_[smax], geom_[smax - 1], geom_[smax],
+                        colour1_only, s);
+  }
   int smax = 0, sc = 0;
 
  private:

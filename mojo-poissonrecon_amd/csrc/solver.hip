@@ -912,11 +912,12 @@ int mpr_batch_query(mpr_batch* b) {
 
 int mpr_job_time_kernel(mpr_job* job, int32_t kernel, int32_t reps, double* ms_per_launch,
                         double* bytes_per_launch) {
-  if (!job || reps < 1 || kernel < 0 || kernel > 9) {
+  if (!job || reps < 1 || kernel < 0 || kernel > 12) {
     set_error("invalid kernel timing request");
     return -1;
   }
-  if ((kernel == 1 || kernel == 5 || kernel == 6 || kernel == 7 || kernel == 9) &&
+  if ((kernel == 1 || kernel == 5 || kernel == 6 || kernel == 7 || kernel == 9 ||
+       kernel == 10 || kernel == 11) &&
       job->L32.empty()) {
     set_error("FP32 kernels need a mixed-precision job");
     return -1;
@@ -985,6 +986,18 @@ int mpr_job_time_kernel(mpr_job* job, int32_t kernel, int32_t reps, double* ms_p
     case 9:
       for (int r = 0; r < reps; ++r) job->vc32.residual_finest(pw, s);
       bytes = 4.0 * 4.0 * M0;
+      break;
+    case 10:
+      for (int r = 0; r < reps; ++r) job->vc32.restrict_finest(s);
+      bytes = 4.0 * (M0 + M1);
+      break;
+    case 11:  // colour 1 only: read coarse and fine colour 1, write fine colour 1
+      for (int r = 0; r < reps; ++r) job->vc32.prolong_finest(true, s);
+      bytes = 4.0 * (M1 + M0);
+      break;
+    case 12:
+      for (int r = 0; r < reps; ++r) job->vc64.restrict_finest(s);
+      bytes = 8.0 * (M0 + M1);
       break;
   }
   MPR_HIP(hipEventRecord(e1, s));

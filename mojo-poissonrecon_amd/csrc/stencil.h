@@ -375,6 +375,107 @@ __device__ __forceinline__ void prolong_group(gptr<const T> XC, const Geom& gc, 
   store_group<T, T, V>(F + own, out, mask);
 }
 
+// Full-weighting restriction of a slot group of coarse nodes: coarse slots K0..K0+V-1 of
+// row (Y,Z) in colour cc, nodes X_j = 2(K0+j) + P. Coarse node j is centred at fine
+// x = 2 X_j, whose colour in fine row (yy,zz) is q = (yy+zz)&1 at slot 2(K0+j)+P; its x-1
+// and x+1 neighbours are the other colour's slots 2(K0+j)+P-1 and 2(K0+j)+P. So each of the
+// nine fine rows is read as two 16-byte chunks of both colours starting at slot 2 K0, plus
+// one slot before them. Terms are accumulated per coarse node in restrict_node's
+// (oz, oy, ox) order with its weights, so the result is restrict_node's. Needs 16-byte
+// aligned rows on both levels.
+template <class T, int V>
+__device__ __forceinline__ void restrict_group(gptr<const T> F, const Geom& gf, gptr<T> RC,
+                                               gptr<T> XC, const Geom& gc, int cc, uint32_t lane,
+                                               uint32_t lanes, T scale) {
+  using CT = typename SlotGroup<T, V>::CT;
+  constexpr int W = SlotGroup<T, V>::W;
+  constexpr int NR = 2 * V / W;  // chunks per fine row and colour
+  const bool in = lane < lanes;
+  lane = in ? lane : lanes - 1;
+  const uint32_t s0 = lane * (uint32_t)V;
+  const uint32_t Z = gc.div_nH.div(s0);
+  const uint32_t rr = s0 - Z * gc.nH;
+  const uint32_t Y = gc.div_H.div(rr);
+  const uint32_t K0 = rr - Y * (uint32_t)gc.H;
+  const int P = (cc ^ (int)Y ^ (int)Z) & 1;
+  const int nf = gf.n;
+  const uint32_t base = 2u * K0;
+  // every load first (9 rows x 2 colours x NR chunks + 9 edge slots), then the sums
+  CT aq[9][NR], ao[9][NR];
+  T before[9];
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    const int oz = r / 3 - 1, oy = r % 3 - 1;
+    const int zz = 2 * (int)Z + oz, yy = 2 * (int)Y + oy;
+    const int zc = zz < 0 ? 0 : (zz >= nf ? nf - 1 : zz);
+    const int yc = yy < 0 ? 0 : (yy >= nf ? nf - 1 : yy);
+    const uint32_t q = (uint32_t)((yc + zc) & 1);
+    const uint32_t row = ((uint32_t)zc * (uint32_t)nf + (uint32_t)yc) * (uint32_t)gf.H;
+    const uint32_t rq = q * gf.C + row, ro = (1u - q) * gf.C + row;
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      // chunks past the row end only hold pad slots (never used): read the row start
+      const uint32_t slot = base + (uint32_t)(i * W) + W <= (uint32_t)gf.H ? base + i * W : 0u;
+      aq[r][i] = *(const __attribute__((address_space(1))) CT*)(F + rq + slot);
+      ao[r][i] = *(const __attribute__((address_space(1))) CT*)(F + ro + slot);
+    }
+    before[r] = F[ro + (base > 0 ? base - 1 : 0)];
+  }
+  T sum[V], wsum[V];
+#pragma unroll
+  for (int j = 0; j < V; ++j) sum[j] = wsum[j] = T(0);
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    const int oz = r / 3 - 1, oy = r % 3 - 1;
+    const int zz = 2 * (int)Z + oz, yy = 2 * (int)Y + oy;
+    const bool iny = zz >= 0 && zz < nf && yy >= 0 && yy < nf;
+    const T wz = oz == 0 ? T(2) : T(1), wy = oy == 0 ? T(2) : T(1);
+#pragma unroll
+    for (int j = 0; j < V; ++j) {
+      const T q0 = aq[r][(2 * j) / W][(2 * j) % W], q1 = aq[r][(2 * j + 1) / W][(2 * j + 1) % W];
+      const T o0 = ao[r][(2 * j) / W][(2 * j) % W], o1 = ao[r][(2 * j + 1) / W][(2 * j + 1) % W];
+      T om;
+      if (j == 0)
+        om = before[r];
+      else
+        om = ao[r][(2 * j - 1) / W][(2 * j - 1) % W];
+      const T vm = P ? o0 : om;  // x - 1
+      const T v0 = P ? q1 : q0;  // x
+      const T vp = P ? o1 : o0;  // x + 1
+      const int xc = 2 * (2 * ((int)K0 + j) + P);
+      const bool inm = iny && xc - 1 >= 0, in0 = iny && xc < nf, inp = iny && xc + 1 < nf;
+      const T wm = T(1) * wy * wz, w0 = T(2) * wy * wz, wp = T(1) * wy * wz;
+      if (inm) {
+        sum[j] += vm * wm;
+        wsum[j] += wm;
+      }
+      if (in0) {
+        sum[j] += v0 * w0;
+        wsum[j] += w0;
+      }
+      if (inp) {
+        sum[j] += vp * wp;
+        wsum[j] += wp;
+      }
+    }
+  }
+  T out[V];
+  uint32_t mask = 0;
+#pragma unroll
+  for (int j = 0; j < V; ++j) {
+    out[j] = sum[j] / wsum[j] * scale;
+    if (in && 2 * ((int)K0 + j) + P < gc.n) mask |= 1u << j;
+  }
+  const uint32_t own = (uint32_t)cc * gc.C + s0;
+  store_group<T, T, V>(RC + own, out, mask);
+  if (XC) {
+    T zero[V];
+#pragma unroll
+    for (int j = 0; j < V; ++j) zero[j] = T(0);
+    store_group<T, T, V>(XC + own, zero, in ? (1u << V) - 1 : 0u);
+  }
+}
+
 // Full-weighting restriction of coarse node (X,Y,Z) (capi.mojo:556-673): weights 2 at the
 // centre and 1 at the sides per axis, terms in (oz, oy, ox) order, divided by the sum of
 // the in-grid weights, then multiplied by `scale` (1 for residuals, 4 for the screening

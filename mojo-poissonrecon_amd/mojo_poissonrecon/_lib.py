@@ -108,6 +108,8 @@ _PIPELINE = {
     "mpr_batch_wait": ([P, ctypes.POINTER(mpr_result)], ctypes.c_int),
     "mpr_relax_f32": ([P, P, P, I, F, I], ctypes.c_int),
     "mpr_residual_f32": ([P, P, P, P, I, F], ctypes.c_int),
+    "mpr_restrict_f32": ([P, P, I], ctypes.c_int),
+    "mpr_prolong_add_f32": ([P, P, I], ctypes.c_int),
     "mpr_extract": ([P, P, I, F, _PP, _IP, _IP], ctypes.c_int),
     "mpr_mesh_fetch": ([P, P, P, P], ctypes.c_int),
     "mpr_mesh_destroy": ([P], ctypes.c_int),

@@ -203,6 +203,52 @@ def prolong_add(coarse, fine) -> np.ndarray:
     return out
 
 
+def restrict_np(fine, scale: float = 1.0) -> np.ndarray:
+    """capi.mojo:556-673 in the array's own dtype: coarse (X,Y,Z) = sum of in-grid fine
+    values at (2X+ox, 2Y+oy, 2Z+oz) times wx wy wz (2 at the centre, 1 aside), terms in
+    (oz, oy, ox) order, divided by the sum of those weights, times `scale`."""
+    fine = np.asarray(fine)
+    dt = fine.dtype.type
+    nf = fine.shape[0]
+    nc = (nf + 1) // 2
+    Z, Y, X = np.indices((nc, nc, nc))
+    total = np.zeros((nc, nc, nc), dtype=fine.dtype)
+    wsum = np.zeros((nc, nc, nc), dtype=fine.dtype)
+    for oz in (-1, 0, 1):
+        for oy in (-1, 0, 1):
+            for ox in (-1, 0, 1):
+                zz, yy, xx = 2 * Z + oz, 2 * Y + oy, 2 * X + ox
+                ok = (zz >= 0) & (zz < nf) & (yy >= 0) & (yy < nf) & (xx >= 0) & (xx < nf)
+                v = fine[np.clip(zz, 0, nf - 1), np.clip(yy, 0, nf - 1), np.clip(xx, 0, nf - 1)]
+                w = dt(2 if ox == 0 else 1) * dt(2 if oy == 0 else 1) * dt(2 if oz == 0 else 1)
+                total = np.where(ok, total + v * w, total)
+                wsum = np.where(ok, wsum + w, wsum)
+    return total / wsum * dt(scale)
+
+
+def prolong_np(coarse, fine) -> np.ndarray:
+    """capi.mojo:676-742 in the arrays' own dtype: fine(x,y,z) += trilinear value of the
+    coarse grid at (x/2, y/2, z/2), lerped along x, then y, then z, upper index clamped."""
+    coarse = np.asarray(coarse)
+    out = np.array(fine, copy=True)
+    dt = out.dtype.type
+    n = out.shape[0]
+    nc = coarse.shape[0]
+    z, y, x = np.indices(out.shape)
+    x0, y0, z0 = x >> 1, y >> 1, z >> 1
+    x1, y1, z1 = np.minimum(x0 + 1, nc - 1), np.minimum(y0 + 1, nc - 1), np.minimum(z0 + 1, nc - 1)
+    half, zero, one = dt(0.5), dt(0.0), dt(1.0)
+    tx, ty, tz = (np.where(v & 1, half, zero) for v in (x, y, z))
+    c = lambda zi, yi, xi: coarse[zi, yi, xi]  # noqa: E731
+    c00 = c(z0, y0, x0) * (one - tx) + c(z0, y0, x1) * tx
+    c10 = c(z0, y1, x0) * (one - tx) + c(z0, y1, x1) * tx
+    c01 = c(z1, y0, x0) * (one - tx) + c(z1, y0, x1) * tx
+    c11 = c(z1, y1, x0) * (one - tx) + c(z1, y1, x1) * tx
+    c0 = c00 * (one - ty) + c10 * ty
+    c1 = c01 * (one - ty) + c11 * ty
+    return out + (c0 * (one - tz) + c1 * tz)
+
+
 def iso_value(field, points, normals, confidence: bool) -> float:
     field, points, normals = map(_f, (field, points, normals))
     return lib().oracle_iso_value(_p(field), _p(points), _p(normals), len(points),

@@ -74,7 +74,27 @@ def test_residual_f32_bit_exact_against_float32_restatement(n):
     assert np.array_equal(actual, expected)
 
 
-@pytest.mark.parametrize("n", [3, 9, 10, 17, 65])
+@pytest.mark.parametrize("n", [3, 9, 33, 65, 66, 129])
+def test_restrict_f32_bit_exact_against_float32_restatement(n):
+    fine = np.random.default_rng(500 + n).normal(size=(n, n, n)).astype(np.float32)
+    cn = (n + 1) // 2
+    coarse = np.empty((cn, cn, cn), dtype=np.float32)
+    assert lib().mpr_restrict_f32(fine.ctypes.data, coarse.ctypes.data, n) == 0
+    assert np.array_equal(coarse, oracle.restrict_np(fine))
+
+
+@pytest.mark.parametrize("n", [3, 9, 33, 34, 65, 66, 129])
+def test_prolong_f32_bit_exact_against_float32_restatement(n):
+    rng = np.random.default_rng(600 + n)
+    cn = (n + 1) // 2
+    coarse = rng.normal(size=(cn, cn, cn)).astype(np.float32)
+    fine = rng.normal(size=(n, n, n)).astype(np.float32)
+    expected = oracle.prolong_np(coarse, fine)
+    assert lib().mpr_prolong_add_f32(coarse.ctypes.data, fine.ctypes.data, n) == 0
+    assert np.array_equal(fine, expected)
+
+
+@pytest.mark.parametrize("n", [3, 9, 10, 17, 65, 66, 129, 130])
 def test_restrict_bit_exact(n):
     fine = np.random.default_rng(11).normal(size=(n, n, n))
     cn = (n + 1) // 2

@@ -327,3 +327,15 @@ def test_numpy_smoother_restatement_equals_c_oracle_in_float64(n):
     want = oracle.relax(x32.astype(np.float64), r32.astype(np.float64), s32.astype(np.float64),
                         4.0, 3)
     assert np.allclose(got, want, rtol=1e-4, atol=1e-5 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 9, 10, 17])
+def test_numpy_transfer_restatements_equal_c_oracle_in_float64(n):
+    # pins restrict_np / prolong_np, whose float32 instantiation checks the FP32 kernels
+    rng = np.random.default_rng(90 + n)
+    fine = rng.normal(size=(n, n, n))
+    cn = (n + 1) // 2
+    coarse = rng.normal(size=(cn, cn, cn))
+    assert np.array_equal(oracle.restrict_np(fine), oracle.restrict(fine))
+    assert np.array_equal(oracle.restrict_np(fine, 4.0), 4.0 * oracle.restrict(fine))
+    assert np.array_equal(oracle.prolong_np(coarse, fine), oracle.prolong_add(coarse, fine))
