@@ -26,6 +26,32 @@ __device__ __forceinline__ uint32_t block_prefix256(bool flag, uint32_t* lds4, u
   return base + in_wave;
 }
 
+// Exclusive prefix sum of `v` over the 256 threads of a block in thread order (wave64
+// shuffle scan, then the 4 wave totals in LDS); `total` receives the block's sum. Every
+// thread of the block must call it.
+__device__ __forceinline__ uint32_t block_exclusive256(uint32_t v, uint32_t* lds4,
+                                                       uint32_t& total) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  uint32_t inc = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t t = __shfl_up(inc, off, 64);
+    if (lane >= (uint32_t)off) inc += t;
+  }
+  if (lane == 63) lds4[wave] = inc;
+  __syncthreads();
+  uint32_t base = 0, tot = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < 4; ++w) {
+    const uint32_t c = lds4[w];
+    base += w < wave ? c : 0u;
+    tot += c;
+  }
+  __syncthreads();
+  total = tot;
+  return base + inc - v;
+}
+
 // Sum of a per-thread count over a 256-thread block (result valid in every thread).
 __device__ __forceinline__ uint32_t block_count256(uint32_t v, uint32_t* lds4) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);

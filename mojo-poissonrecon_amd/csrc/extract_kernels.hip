@@ -278,6 +278,247 @@ __global__ __launch_bounds__(kThreads) void k_edges_emit(const double* f, int n,
   }
 }
 
+// ---- sign-bit extraction (the device pipeline) -----------------------------------------
+// One pass packs s(x,y,z) = field < iso into 32-bit words along x (word w of node row
+// (y,z) holds x = 32w .. 32w+31). Cell activity (some but not all of the 8 corner bits
+// set) and edge sign changes are then whole-word operations, 32 cells or edges per lane, in
+// the reference's raster and pass order. A cell's vertex index is the exclusive count of
+// active cells before its 32-cell run plus the active cells before it in the run, so no
+// per-cell index map is stored; the field and density are read only at active cells.
+struct BitGeom {
+  uint32_t n, cells, bw, cw;  // node words and cell runs per row
+  uint32_t r0, r1, r2;        // rows of the three edge passes
+  FastDiv d_bw, d_cw, d_cells, d_cm1;
+};
+
+__device__ __forceinline__ uint32_t node_word(const uint32_t* bits, const BitGeom& G, uint32_t y,
+                                              uint32_t z, uint32_t w) {
+  return w < G.bw ? bits[(z * G.n + y) * G.bw + w] : 0u;
+}
+
+// bits of nodes x = 32w+k (lo) and x = 32w+k+1 (hi) of node row (y,z)
+__device__ __forceinline__ void node_pair(const uint32_t* bits, const BitGeom& G, uint32_t y,
+                                          uint32_t z, uint32_t w, uint32_t& lo, uint32_t& hi) {
+  lo = node_word(bits, G, y, z, w);
+  hi = (lo >> 1) | (node_word(bits, G, y, z, w + 1) << 31);
+}
+
+__device__ __forceinline__ uint32_t low_bits(uint32_t count) {
+  return count >= 32u ? 0xffffffffu : ((1u << count) - 1u);
+}
+
+__global__ __launch_bounds__(kThreads) void k_sign_bits(const double* f, const double* isop,
+                                                        BitGeom G, uint32_t* bits) {
+  const uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t wi = (uint32_t)(i >> 5), k = (uint32_t)(i & 31u);
+  const uint32_t words = G.n * G.n * G.bw;
+  const uint32_t row = G.d_bw.div(wi < words ? wi : 0u);
+  const uint32_t x = 32u * (wi - row * G.bw) + k;
+  const bool in = wi < words && x < G.n;
+  const bool below = in && f[in ? (uint64_t)row * G.n + x : 0ull] < *isop;
+  const uint64_t bal = __ballot(below);
+  if (k == 0 && wi < words) bits[wi] = (uint32_t)(bal >> (threadIdx.x & 32u));
+}
+
+// active-cell mask of cell run r (cells x = 32w .. 32w+31 of cell row (y,z))
+__device__ __forceinline__ uint32_t cell_mask(const uint32_t* bits, const BitGeom& G, uint32_t r,
+                                              uint32_t& y, uint32_t& z, uint32_t& w) {
+  const uint32_t row = G.d_cw.div(r);
+  w = r - row * G.cw;
+  z = G.d_cells.div(row);
+  y = row - z * G.cells;
+  uint32_t all = 0xffffffffu, any = 0u;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    uint32_t lo, hi;
+    node_pair(bits, G, y + (c & 1), z + (c >> 1), w, lo, hi);
+    all &= lo & hi;
+    any |= lo | hi;
+  }
+  return (any & ~all) & low_bits(G.cells - 32u * w);
+}
+
+__global__ __launch_bounds__(kThreads) void k_bit_cells_count(const uint32_t* bits, BitGeom G,
+                                                              uint32_t* cmask, uint32_t* counts) {
+  __shared__ uint32_t lds4[4];
+  const uint32_t r = blockIdx.x * kThreads + threadIdx.x;
+  uint32_t m = 0;
+  if (r < G.cells * G.cells * G.cw) {
+    uint32_t y, z, w;
+    m = cell_mask(bits, G, r, y, z, w);
+    cmask[r] = m;
+  }
+  const uint32_t c = block_count256((uint32_t)__popc(m), lds4);
+  if (threadIdx.x == 0) counts[blockIdx.x] = c;
+}
+
+// vertices of the active cells in raster order; coff[r] = index of run r's first vertex
+__global__ __launch_bounds__(kThreads) void k_bit_cells_emit(
+    const double* f, const double* dens, const double* isop, BitGeom G, const uint32_t* cmask,
+    const uint32_t* tile_offsets, uint32_t* coff, double* verts, double* vdens, uint64_t capacity,
+    const double* xform) {
+  __shared__ uint32_t lds4[4];
+  const uint32_t r = blockIdx.x * kThreads + threadIdx.x;
+  const bool live = r < G.cells * G.cells * G.cw;
+  uint32_t m = live ? cmask[r] : 0u;
+  uint32_t total;
+  const uint32_t pre = block_exclusive256((uint32_t)__popc(m), lds4, total);
+  if (!live) return;
+  const uint32_t base = tile_offsets[blockIdx.x] + pre;
+  coff[r] = base;
+  const uint32_t row = G.d_cw.div(r);
+  const uint32_t w = r - row * G.cw;
+  const uint32_t z = G.d_cells.div(row);
+  const uint32_t y = row - z * G.cells;
+  const double iso = *isop;
+  uint32_t rank = 0;
+  while (m) {
+    const uint32_t k = (uint32_t)__ffs(m) - 1u;
+    m &= m - 1u;
+    const uint64_t idx = (uint64_t)base + rank++;
+    if (idx >= capacity) continue;
+    double v[3], d;
+    dual_vertex(f, dens, (int)(32u * w + k), (int)y, (int)z, (int)G.n, iso, v, d);
+    if (xform) {
+      for (int a = 0; a < 3; ++a) v[a] = v[a] * xform[3] + xform[a];
+    }
+    verts[3 * idx] = v[0];
+    verts[3 * idx + 1] = v[1];
+    verts[3 * idx + 2] = v[2];
+    vdens[idx] = d;
+  }
+}
+
+// Sign-change mask of edge run e over the three passes (capi.mojo:1090-1128): pass 0 edges
+// along x at y,z in [1, cells-1], x in [0, cells-1]; pass 1 along y at z in [1, cells-1],
+// y in [0, cells-1], x in [1, cells-1]; pass 2 along z at z in [0, cells-1], y, x in
+// [1, cells-1]; rows in raster (z, y) order. `below` receives the lower node's bits (the
+// face orientation, capi.mojo:1044-1072).
+__device__ __forceinline__ uint32_t edge_mask(const uint32_t* bits, const BitGeom& G, uint32_t e,
+                                              int& axis, uint32_t& y, uint32_t& z, uint32_t& w,
+                                              uint32_t& below) {
+  const uint32_t c = G.cells, cm1 = c - 1u;
+  uint32_t row;
+  if (e < G.r0 * G.cw) {
+    axis = 0;
+    row = G.d_cw.div(e);
+    w = e - row * G.cw;
+    const uint32_t zz = G.d_cm1.div(row);
+    y = 1u + (row - zz * cm1);
+    z = 1u + zz;
+    uint32_t lo, hi;
+    node_pair(bits, G, y, z, w, lo, hi);
+    below = lo;
+    return (lo ^ hi) & low_bits(c - 32u * w);
+  }
+  e -= G.r0 * G.cw;
+  uint32_t other;
+  if (e < G.r1 * G.cw) {
+    axis = 1;
+    row = G.d_cw.div(e);
+    w = e - row * G.cw;
+    const uint32_t zz = G.d_cells.div(row);
+    y = row - zz * c;
+    z = 1u + zz;
+    other = node_word(bits, G, y + 1u, z, w);
+  } else {
+    e -= G.r1 * G.cw;
+    axis = 2;
+    row = G.d_cw.div(e);
+    w = e - row * G.cw;
+    const uint32_t zz = G.d_cm1.div(row);
+    y = 1u + (row - zz * cm1);
+    z = zz;
+    other = node_word(bits, G, y, z + 1u, w);
+  }
+  below = node_word(bits, G, y, z, w);
+  const uint32_t valid = low_bits(c - 32u * w) & (w == 0 ? ~1u : 0xffffffffu);  // x >= 1
+  return (below ^ other) & valid;
+}
+
+__device__ __forceinline__ uint32_t edge_runs(const BitGeom& G) {
+  return (G.r0 + G.r1 + G.r2) * G.cw;
+}
+
+__global__ __launch_bounds__(kThreads) void k_bit_edges_count(const uint32_t* bits, BitGeom G,
+                                                              uint32_t* counts) {
+  __shared__ uint32_t lds4[4];
+  const uint32_t e = blockIdx.x * kThreads + threadIdx.x;
+  uint32_t m = 0;
+  if (e < edge_runs(G)) {
+    int axis;
+    uint32_t y, z, w, below;
+    m = edge_mask(bits, G, e, axis, y, z, w, below);
+  }
+  const uint32_t c = block_count256((uint32_t)__popc(m), lds4);
+  if (threadIdx.x == 0) counts[blockIdx.x] = c;
+}
+
+__device__ __forceinline__ int64_t vertex_of(const BitGeom& G, const uint32_t* cmask,
+                                             const uint32_t* coff, uint32_t x, uint32_t y,
+                                             uint32_t z) {
+  const uint32_t r = (z * G.cells + y) * G.cw + (x >> 5);
+  return (int64_t)coff[r] + __popc(cmask[r] & low_bits(x & 31u));
+}
+
+__global__ __launch_bounds__(kThreads) void k_bit_edges_emit(const uint32_t* bits, BitGeom G,
+                                                             const uint32_t* cmask,
+                                                             const uint32_t* coff,
+                                                             const uint32_t* tile_offsets,
+                                                             int64_t* faces, uint64_t capacity) {
+  __shared__ uint32_t lds4[4];
+  const uint32_t e = blockIdx.x * kThreads + threadIdx.x;
+  uint32_t m = 0, y = 0, z = 0, w = 0, below = 0;
+  int axis = 0;
+  if (e < edge_runs(G)) m = edge_mask(bits, G, e, axis, y, z, w, below);
+  uint32_t total;
+  const uint32_t pre = block_exclusive256((uint32_t)__popc(m), lds4, total);
+  uint64_t quad = (uint64_t)tile_offsets[blockIdx.x] + pre;
+  while (m) {
+    const uint32_t k = (uint32_t)__ffs(m) - 1u;
+    m &= m - 1u;
+    const uint64_t fc = 2ull * quad++;
+    if (fc + 2 > capacity) continue;
+    const uint32_t x = 32u * w + k;
+    uint32_t cx[4], cy[4], cz[4];
+    if (axis == 0) {
+      const uint32_t t[4][3] = {{x, y - 1, z - 1}, {x, y, z - 1}, {x, y, z}, {x, y - 1, z}};
+      for (int j = 0; j < 4; ++j) cx[j] = t[j][0], cy[j] = t[j][1], cz[j] = t[j][2];
+    } else if (axis == 1) {
+      const uint32_t t[4][3] = {{x - 1, y, z - 1}, {x - 1, y, z}, {x, y, z}, {x, y, z - 1}};
+      for (int j = 0; j < 4; ++j) cx[j] = t[j][0], cy[j] = t[j][1], cz[j] = t[j][2];
+    } else {
+      const uint32_t t[4][3] = {{x - 1, y - 1, z}, {x, y - 1, z}, {x, y, z}, {x - 1, y, z}};
+      for (int j = 0; j < 4; ++j) cx[j] = t[j][0], cy[j] = t[j][1], cz[j] = t[j][2];
+    }
+    int64_t q[4];
+    for (int j = 0; j < 4; ++j) q[j] = vertex_of(G, cmask, coff, cx[j], cy[j], cz[j]);
+    int64_t* o = faces + 3 * fc;
+    if ((below >> k) & 1u) {
+      o[0] = q[0]; o[1] = q[3]; o[2] = q[2]; o[3] = q[0]; o[4] = q[2]; o[5] = q[1];
+    } else {
+      o[0] = q[0]; o[1] = q[1]; o[2] = q[2]; o[3] = q[0]; o[4] = q[2]; o[5] = q[3];
+    }
+  }
+}
+
+BitGeom make_bit_geom(int n) {
+  BitGeom G;
+  G.n = (uint32_t)n;
+  G.cells = (uint32_t)(n - 1);
+  G.bw = (G.n + 31u) / 32u;
+  G.cw = std::max<uint32_t>(1u, (G.cells + 31u) / 32u);
+  const uint32_t c = G.cells, cm1 = c ? c - 1u : 0u;
+  G.r0 = cm1 * cm1;
+  G.r1 = cm1 * c;
+  G.r2 = c * cm1;
+  G.d_bw = FastDiv(G.bw);
+  G.d_cw = FastDiv(G.cw);
+  G.d_cells = FastDiv(std::max<uint32_t>(1u, c));
+  G.d_cm1 = FastDiv(std::max<uint32_t>(1u, cm1));
+  return G;
+}
+
 EdgeGeom make_edge_geom(int n) {
   EdgeGeom G;
   G.cells = (uint32_t)(n - 1);
@@ -343,6 +584,42 @@ void launch_edges_emit(const double* f, int n, const double* iso, const CM* cm,
                        hipStream_t s) {
   hipLaunchKernelGGL(k_edges_emit<CM>, dim3(edge_tiles(n)), dim3(kThreads), 0, s, f, n, iso, cm,
                      make_edge_geom(n), tile_offsets, faces, capacity);
+  MPR_LAUNCH_CHECK();
+}
+
+BitSurface bit_surface_sizes(int n) {
+  const BitGeom G = make_bit_geom(n);
+  BitSurface b;
+  b.bit_words = (uint64_t)G.n * G.n * G.bw;
+  b.runs = (uint64_t)G.cells * G.cells * G.cw;
+  b.cell_tiles = (uint32_t)std::max<uint64_t>(1, (b.runs + kThreads - 1) / kThreads);
+  b.edge_tiles = (uint32_t)std::max<uint64_t>(
+      1, ((uint64_t)(G.r0 + G.r1 + G.r2) * G.cw + kThreads - 1) / kThreads);
+  return b;
+}
+
+void launch_bit_extract(const double* f, const double* dens, int n, const double* iso,
+                        const BitSurface& b, uint32_t* counts, double* verts, double* vdens,
+                        uint64_t vcap, int64_t* faces, uint64_t fcap, const double* xform,
+                        hipStream_t s) {
+  const BitGeom G = make_bit_geom(n);
+  const uint64_t lanes = b.bit_words * 32ull;
+  hipLaunchKernelGGL(k_sign_bits, dim3((unsigned)((lanes + kThreads - 1) / kThreads)),
+                     dim3(kThreads), 0, s, f, iso, G, b.bits);
+  MPR_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_bit_cells_count, dim3(b.cell_tiles), dim3(kThreads), 0, s, b.bits, G,
+                     b.cmask, b.cell_tc);
+  MPR_LAUNCH_CHECK();
+  launch_scan_exclusive(b.cell_tc, b.cell_to, b.cell_tiles, &counts[0], s);
+  hipLaunchKernelGGL(k_bit_cells_emit, dim3(b.cell_tiles), dim3(kThreads), 0, s, f, dens, iso, G,
+                     b.cmask, b.cell_to, b.coff, verts, vdens, vcap, xform);
+  MPR_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_bit_edges_count, dim3(b.edge_tiles), dim3(kThreads), 0, s, b.bits, G,
+                     b.edge_tc);
+  MPR_LAUNCH_CHECK();
+  launch_scan_exclusive(b.edge_tc, b.edge_to, b.edge_tiles, &counts[1], s);
+  hipLaunchKernelGGL(k_bit_edges_emit, dim3(b.edge_tiles), dim3(kThreads), 0, s, b.bits, G,
+                     b.cmask, b.coff, b.edge_to, faces, fcap);
   MPR_LAUNCH_CHECK();
 }
 

@@ -89,6 +89,22 @@ void launch_iso(const double* field, const double* pts, const double* nrm, uint3
 uint32_t cell_tiles(int n);
 uint32_t edge_tiles(int n);
 // iso is a device pointer: the pipeline computes it on the GPU
+// Device buffers of the sign-bit extraction (launch_bit_extract) for an n^3 field.
+struct BitSurface {
+  uint64_t bit_words = 0, runs = 0;  // sizes of bits and of cmask / coff
+  uint32_t cell_tiles = 0, edge_tiles = 0;
+  uint32_t *bits = nullptr, *cmask = nullptr, *coff = nullptr;
+  uint32_t *cell_tc = nullptr, *cell_to = nullptr, *edge_tc = nullptr, *edge_to = nullptr;
+};
+BitSurface bit_surface_sizes(int n);
+// Dual-grid extraction of the device pipeline (capi.mojo:839-1129, same vertex indices and
+// face order): counts[0] = vertices, counts[1] = quads; vertices past vcap and faces past
+// fcap are counted but not written. xform: optional [origin x, y, z, width] world transform.
+void launch_bit_extract(const double* f, const double* dens, int n, const double* iso,
+                        const BitSurface& b, uint32_t* counts, double* verts, double* vdens,
+                        uint64_t vcap, int64_t* faces, uint64_t fcap, const double* xform,
+                        hipStream_t s);
+
 void launch_cells_count(const double* f, int n, const double* iso, uint32_t* counts, hipStream_t s);
 void launch_cells_chunk_counts(const double* f, int n, const double* iso, int64_t* chunks,
                                hipStream_t s);

@@ -99,8 +99,7 @@ struct Job {
   double *norm_part = nullptr, *bbox_part = nullptr, *iso_part = nullptr;
   int norm_nparts = 0;
   // extraction
-  uint32_t *cell_tc = nullptr, *cell_to = nullptr, *edge_tc = nullptr, *edge_to = nullptr;
-  int32_t* cell_map = nullptr;
+  BitSurface surf;
   double *verts = nullptr, *vdens = nullptr;
   int64_t* faces = nullptr;
   uint64_t vcap = 0, fcap = 0;
@@ -313,12 +312,15 @@ void Job::plan(DevArena& a) {
   norm_part = a.alloc<double>(norm_nparts);
   bbox_part = a.alloc<double>(6ull * bbox_partial_count(std::max<uint32_t>(1u, max_points)));
   iso_part = a.alloc<double>(2ull * iso_partial_count(std::max<uint32_t>(1u, max_points)));
-  cell_tc = a.alloc<uint32_t>(cell_tiles(n));
-  cell_to = a.alloc<uint32_t>(cell_tiles(n));
-  edge_tc = a.alloc<uint32_t>(edge_tiles(n));
-  edge_to = a.alloc<uint32_t>(edge_tiles(n));
+  surf = bit_surface_sizes(n);
+  surf.bits = a.alloc<uint32_t>(surf.bit_words);
+  surf.cmask = a.alloc<uint32_t>(surf.runs);
+  surf.coff = a.alloc<uint32_t>(surf.runs);
+  surf.cell_tc = a.alloc<uint32_t>(surf.cell_tiles);
+  surf.cell_to = a.alloc<uint32_t>(surf.cell_tiles);
+  surf.edge_tc = a.alloc<uint32_t>(surf.edge_tiles);
+  surf.edge_to = a.alloc<uint32_t>(surf.edge_tiles);
   const uint64_t cells = (uint64_t)(n - 1);
-  cell_map = a.alloc<int32_t>(cells * cells * cells);
   vcap = std::min<uint64_t>(cells * cells * cells, std::max<uint64_t>(1u << 16, 12ull * n * n));
   fcap = 5 * vcap / 2 + 16;
   verts = a.alloc<double>(3 * vcap);
@@ -355,13 +357,8 @@ Job::~Job() {
 }
 
 void Job::enqueue_extract(bool world, hipStream_t s) {
-  launch_cells_count(field, n, d_iso, cell_tc, s);
-  launch_scan_exclusive(cell_tc, cell_to, cell_tiles(n), &d_counts[0], s);
-  launch_cells_emit<int32_t>(field, density, n, d_iso, cell_to, cell_map, verts, vdens, vcap,
-                             world ? d_params : nullptr, s);
-  launch_edges_count<int32_t>(field, n, d_iso, cell_map, edge_tc, s);
-  launch_scan_exclusive(edge_tc, edge_to, edge_tiles(n), &d_counts[1], s);
-  launch_edges_emit<int32_t>(field, n, d_iso, cell_map, edge_to, faces, fcap, s);
+  launch_bit_extract(field, density, n, d_iso, surf, d_counts, verts, vdens, vcap, faces, fcap,
+                     world ? d_params : nullptr, s);
 }
 
 // normalise, splat, assemble, screening pyramid and initial guess (core.py:229-273)
@@ -1030,34 +1027,34 @@ int mpr_extract(const double* field, const double* density, int64_t n, double is
     double* f = scratch.alloc<double>(M);
     double* d = scratch.alloc<double>(M);
     double* isod = scratch.alloc<double>(1);
-    uint32_t* ctc = scratch.alloc<uint32_t>(cell_tiles((int)n));
-    uint32_t* cto = scratch.alloc<uint32_t>(cell_tiles((int)n));
-    uint32_t* etc_ = scratch.alloc<uint32_t>(edge_tiles((int)n));
-    uint32_t* eto = scratch.alloc<uint32_t>(edge_tiles((int)n));
     uint32_t* counts = scratch.alloc<uint32_t>(2);
-    int32_t* cm = scratch.alloc<int32_t>(cells * cells * cells);
+    BitSurface surf = bit_surface_sizes((int)n);
+    surf.bits = scratch.alloc<uint32_t>(surf.bit_words);
+    surf.cmask = scratch.alloc<uint32_t>(surf.runs);
+    surf.coff = scratch.alloc<uint32_t>(surf.runs);
+    surf.cell_tc = scratch.alloc<uint32_t>(surf.cell_tiles);
+    surf.cell_to = scratch.alloc<uint32_t>(surf.cell_tiles);
+    surf.edge_tc = scratch.alloc<uint32_t>(surf.edge_tiles);
+    surf.edge_to = scratch.alloc<uint32_t>(surf.edge_tiles);
     MPR_HIP(hipMemcpyAsync(f, field, 8 * M, hipMemcpyHostToDevice, s));
     if (density)
       MPR_HIP(hipMemcpyAsync(d, density, 8 * M, hipMemcpyHostToDevice, s));
     else
       launch_fill<double>(d, 0.0, M, s);
     MPR_HIP(hipMemcpyAsync(isod, &iso, 8, hipMemcpyHostToDevice, s));
-    launch_cells_count(f, (int)n, isod, ctc, s);
-    launch_scan_exclusive(ctc, cto, cell_tiles((int)n), counts, s);
+    // count (nothing fits a zero capacity), then emit into exactly sized buffers
+    launch_bit_extract(f, d, (int)n, isod, surf, counts, nullptr, nullptr, 0, nullptr, 0, nullptr,
+                       s);
     uint32_t hc[2] = {0, 0};
-    MPR_HIP(hipMemcpyAsync(hc, counts, 4, hipMemcpyDeviceToHost, s));
+    MPR_HIP(hipMemcpyAsync(hc, counts, 8, hipMemcpyDeviceToHost, s));
     MPR_HIP(hipStreamSynchronize(s));
     m->nv = hc[0];
+    m->nq = hc[1];
     m->verts = m->arena.alloc<double>(3 * std::max<uint64_t>(1, m->nv));
     m->vdens = m->arena.alloc<double>(std::max<uint64_t>(1, m->nv));
-    launch_cells_emit<int32_t>(f, d, (int)n, isod, cto, cm, m->verts, m->vdens, m->nv, nullptr, s);
-    launch_edges_count<int32_t>(f, (int)n, isod, cm, etc_, s);
-    launch_scan_exclusive(etc_, eto, edge_tiles((int)n), counts + 1, s);
-    MPR_HIP(hipMemcpyAsync(hc + 1, counts + 1, 4, hipMemcpyDeviceToHost, s));
-    MPR_HIP(hipStreamSynchronize(s));
-    m->nq = hc[1];
     m->faces = m->arena.alloc<int64_t>(6 * std::max<uint64_t>(1, m->nq));
-    launch_edges_emit<int32_t>(f, (int)n, isod, cm, eto, m->faces, 2 * m->nq, s);
+    launch_bit_extract(f, d, (int)n, isod, surf, counts, m->verts, m->vdens, m->nv, m->faces,
+                       2 * m->nq, nullptr, s);
     MPR_HIP(hipStreamSynchronize(s));
     scratch.release();
     if (vertex_count) *vertex_count = (int64_t)m->nv;

@@ -104,6 +104,20 @@ def test_extract_isosurface_bit_exact_on_identical_field(n):
     assert np.array_equal(mesh.density, d)
 
 
+@pytest.mark.parametrize("n", [3, 32, 33, 34, 65, 66])
+def test_extract_random_field_bit_exact(n):
+    # dense sign changes around every 32-cell run boundary of the device's bit masks
+    rng = np.random.default_rng(1000 + n)
+    field = rng.normal(size=(n, n, n))
+    density = rng.random(field.shape)
+    mesh = mpr.extract_isosurface(field, iso_value=0.1, density=density)
+    v, f, d = oracle.extract_isosurface(field, 0.1, density)
+    assert len(v) > 0
+    assert np.array_equal(mesh.vertices, v)
+    assert np.array_equal(mesh.faces, f)
+    assert np.array_equal(mesh.density, d)
+
+
 def test_extract_empty_and_full():
     field = np.ones((9, 9, 9))
     assert mpr.extract_isosurface(field, iso_value=0.0).is_empty
