@@ -352,11 +352,12 @@ __global__ __launch_bounds__(kThreads) void k_bit_cells_count(const uint32_t* bi
   if (threadIdx.x == 0) counts[blockIdx.x] = c;
 }
 
-// vertices of the active cells in raster order; coff[r] = index of run r's first vertex
-__global__ __launch_bounds__(kThreads) void k_bit_cells_emit(
-    const double* f, const double* dens, const double* isop, BitGeom G, const uint32_t* cmask,
-    const uint32_t* tile_offsets, uint32_t* coff, double* verts, double* vdens, uint64_t capacity,
-    const double* xform) {
+// Raster-order indices of the active cells: coff[r] = index of run r's first vertex, and
+// vcell[idx] = the cell (x, run) of vertex idx for the vertices below the capacity.
+__global__ __launch_bounds__(kThreads) void k_bit_cells_index(BitGeom G, const uint32_t* cmask,
+                                                              const uint32_t* tile_offsets,
+                                                              uint32_t* coff, uint32_t* vcell,
+                                                              uint64_t capacity) {
   __shared__ uint32_t lds4[4];
   const uint32_t r = blockIdx.x * kThreads + threadIdx.x;
   const bool live = r < G.cells * G.cells * G.cw;
@@ -366,27 +367,38 @@ __global__ __launch_bounds__(kThreads) void k_bit_cells_emit(
   if (!live) return;
   const uint32_t base = tile_offsets[blockIdx.x] + pre;
   coff[r] = base;
+  uint64_t idx = base;
+  while (m && idx < capacity) {
+    const uint32_t k = (uint32_t)__ffs(m) - 1u;
+    m &= m - 1u;
+    vcell[idx++] = r * 32u + k;
+  }
+}
+
+// one lane per vertex: the dual vertex of its cell (capi.mojo:958-1038)
+__global__ __launch_bounds__(kThreads) void k_bit_vertices(const double* f, const double* dens,
+                                                           const double* isop, BitGeom G,
+                                                           const uint32_t* vcell,
+                                                           const uint32_t* count, double* verts,
+                                                           double* vdens, uint64_t capacity,
+                                                           const double* xform) {
+  const uint64_t idx = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (idx >= capacity || idx >= *count) return;
+  const uint32_t c = vcell[idx];
+  const uint32_t r = c >> 5, k = c & 31u;
   const uint32_t row = G.d_cw.div(r);
   const uint32_t w = r - row * G.cw;
   const uint32_t z = G.d_cells.div(row);
   const uint32_t y = row - z * G.cells;
-  const double iso = *isop;
-  uint32_t rank = 0;
-  while (m) {
-    const uint32_t k = (uint32_t)__ffs(m) - 1u;
-    m &= m - 1u;
-    const uint64_t idx = (uint64_t)base + rank++;
-    if (idx >= capacity) continue;
-    double v[3], d;
-    dual_vertex(f, dens, (int)(32u * w + k), (int)y, (int)z, (int)G.n, iso, v, d);
-    if (xform) {
-      for (int a = 0; a < 3; ++a) v[a] = v[a] * xform[3] + xform[a];
-    }
-    verts[3 * idx] = v[0];
-    verts[3 * idx + 1] = v[1];
-    verts[3 * idx + 2] = v[2];
-    vdens[idx] = d;
+  double v[3], d;
+  dual_vertex(f, dens, (int)(32u * w + k), (int)y, (int)z, (int)G.n, *isop, v, d);
+  if (xform) {
+    for (int a = 0; a < 3; ++a) v[a] = v[a] * xform[3] + xform[a];
   }
+  verts[3 * idx] = v[0];
+  verts[3 * idx + 1] = v[1];
+  verts[3 * idx + 2] = v[2];
+  vdens[idx] = d;
 }
 
 // Sign-change mask of edge run e over the three passes (capi.mojo:1090-1128): pass 0 edges
@@ -611,9 +623,16 @@ void launch_bit_extract(const double* f, const double* dens, int n, const double
                      b.cmask, b.cell_tc);
   MPR_LAUNCH_CHECK();
   launch_scan_exclusive(b.cell_tc, b.cell_to, b.cell_tiles, &counts[0], s);
-  hipLaunchKernelGGL(k_bit_cells_emit, dim3(b.cell_tiles), dim3(kThreads), 0, s, f, dens, iso, G,
-                     b.cmask, b.cell_to, b.coff, verts, vdens, vcap, xform);
+  hipLaunchKernelGGL(k_bit_cells_index, dim3(b.cell_tiles), dim3(kThreads), 0, s, G, b.cmask,
+                     b.cell_to, b.coff, b.vcell, std::min<uint64_t>(vcap, b.vcell_cap));
   MPR_LAUNCH_CHECK();
+  const uint64_t vlanes = std::min<uint64_t>(vcap, b.vcell_cap);
+  if (vlanes) {
+    hipLaunchKernelGGL(k_bit_vertices, dim3((unsigned)((vlanes + kThreads - 1) / kThreads)),
+                       dim3(kThreads), 0, s, f, dens, iso, G, b.vcell, &counts[0], verts, vdens,
+                       vlanes, xform);
+    MPR_LAUNCH_CHECK();
+  }
   hipLaunchKernelGGL(k_bit_edges_count, dim3(b.edge_tiles), dim3(kThreads), 0, s, b.bits, G,
                      b.edge_tc);
   MPR_LAUNCH_CHECK();

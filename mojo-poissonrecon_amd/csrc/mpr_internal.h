@@ -94,6 +94,8 @@ struct BitSurface {
   uint64_t bit_words = 0, runs = 0;  // sizes of bits and of cmask / coff
   uint32_t cell_tiles = 0, edge_tiles = 0;
   uint32_t *bits = nullptr, *cmask = nullptr, *coff = nullptr;
+  uint32_t* vcell = nullptr;  // cell of each vertex, vcell_cap entries (>= the vertex capacity)
+  uint64_t vcell_cap = 0;
   uint32_t *cell_tc = nullptr, *cell_to = nullptr, *edge_tc = nullptr, *edge_to = nullptr;
 };
 BitSurface bit_surface_sizes(int n);

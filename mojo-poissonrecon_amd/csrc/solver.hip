@@ -150,16 +150,20 @@ void Job::resolve_overflow(hipStream_t s) {
   const uint64_t nv = host->counts[0], nf = 2ull * host->counts[1];
   if (nv <= vcap && nf <= fcap) return;
   const uint64_t nvc = std::max<uint64_t>(nv, vcap), nfc = std::max<uint64_t>(nf, fcap);
-  void *v = nullptr, *d = nullptr, *f = nullptr;
+  void *v = nullptr, *d = nullptr, *f = nullptr, *c = nullptr;
   MPR_HIP(hipMalloc(&v, 24 * nvc));
-  MPR_HIP(hipMalloc(&d, 8 * nvc));
-  MPR_HIP(hipMalloc(&f, 24 * nfc));
   mesh_bufs.push_back(v);
+  MPR_HIP(hipMalloc(&d, 8 * nvc));
   mesh_bufs.push_back(d);
+  MPR_HIP(hipMalloc(&f, 24 * nfc));
   mesh_bufs.push_back(f);
+  MPR_HIP(hipMalloc(&c, 4 * nvc));
+  mesh_bufs.push_back(c);
   verts = (double*)v;
   vdens = (double*)d;
   faces = (int64_t*)f;
+  surf.vcell = (uint32_t*)c;
+  surf.vcell_cap = nvc;
   vcap = nvc;
   fcap = nfc;
   ++mesh_generation;
@@ -323,6 +327,8 @@ void Job::plan(DevArena& a) {
   const uint64_t cells = (uint64_t)(n - 1);
   vcap = std::min<uint64_t>(cells * cells * cells, std::max<uint64_t>(1u << 16, 12ull * n * n));
   fcap = 5 * vcap / 2 + 16;
+  surf.vcell_cap = vcap;
+  surf.vcell = a.alloc<uint32_t>(vcap);
   verts = a.alloc<double>(3 * vcap);
   vdens = a.alloc<double>(vcap);
   faces = a.alloc<int64_t>(3 * fcap);
@@ -1050,6 +1056,8 @@ int mpr_extract(const double* field, const double* density, int64_t n, double is
     MPR_HIP(hipStreamSynchronize(s));
     m->nv = hc[0];
     m->nq = hc[1];
+    surf.vcell_cap = m->nv;
+    surf.vcell = m->arena.alloc<uint32_t>(std::max<uint64_t>(1, m->nv));
     m->verts = m->arena.alloc<double>(3 * std::max<uint64_t>(1, m->nv));
     m->vdens = m->arena.alloc<double>(std::max<uint64_t>(1, m->nv));
     m->faces = m->arena.alloc<int64_t>(6 * std::max<uint64_t>(1, m->nq));

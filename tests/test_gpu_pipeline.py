@@ -193,3 +193,31 @@ def test_job_reuse_with_smaller_input_and_graph_replay():
         assert np.array_equal(f, outs[1][1][1])
         assert np.allclose(v, outs[1][1][0], rtol=0, atol=1e-12)
     job.close()
+
+
+def test_job_mesh_overflow_regrows_and_matches_extraction():
+    # a noise cloud whose mesh outgrows the job's preallocated vertex buffer: the job grows
+    # its buffers and re-extracts; the result equals a standalone extraction of its field
+    from mojo_poissonrecon import _lib
+    from mojo_poissonrecon.core import Job
+
+    rng = np.random.default_rng(9)
+    depth, count = 6, 30000
+    points = rng.random((count, 3)) * 0.9 + 0.05
+    normals = rng.normal(size=(count, 3))
+    n = (1 << depth) + 1
+    vcap = min((n - 1) ** 3, max(1 << 16, 12 * n * n))
+    job = Job(depth, count)
+    try:
+        job.set_points(points, normals)
+        job.run(Job.params(_lib.STAGE_SOLVE | _lib.STAGE_EXTRACT))
+        r = job.wait()
+        assert r.vertex_count > vcap
+        v, f, d = job.fetch_mesh(r)
+        field, density = job.fetch_grids(rhs=False)[:2]
+    finally:
+        job.close()
+    mesh = mpr.extract_isosurface(field, iso_value=r.iso_value, density=density)
+    assert np.array_equal(v, mesh.vertices)
+    assert np.array_equal(f, mesh.faces)
+    assert np.array_equal(d, mesh.density)
