@@ -33,9 +33,16 @@ __device__ __forceinline__ BlockMap block_map(uint32_t nbi) {
 // issued before any store (NodeLoads in stencil.h).
 constexpr uint32_t kSpan = kThreads * kNodesPerThread;
 
+// whether an instance's x counts as zero in this launch (ZeroStart)
+template <class T>
+__device__ __forceinline__ bool start_zero(const StepPtrs<T>& p, int zero_start) {
+  return zero_start == kZeroAll || (zero_start == kZeroBelowTop && !p.top);
+}
+
 template <class T>
 __global__ __launch_bounds__(kThreads) void kb_relax_color(const StepPtrs<T>* tab, uint32_t count,
-                                                           uint32_t nbi, Geom g, int c, T pw, T h2) {
+                                                           uint32_t nbi, Geom g, int c, T pw, T h2,
+                                                           int zero_start) {
   const uint32_t per = gridDim.x >> 3;
   const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
   const uint32_t slot = lb / nbi;
@@ -44,9 +51,11 @@ __global__ __launch_bounds__(kThreads) void kb_relax_color(const StepPtrs<T>* ta
   const StepPtrs<T> p = tab[slot];
   const gptr<T> X = as_global(p.x);
   const gptr<const T> R = as_global(p.rhs), S = as_global(p.screen);
+  const bool zero = start_zero(p, zero_start);
   NodeLoads<T> L[kNodesPerThread];
 #pragma unroll
-  for (int k = 0; k < kNodesPerThread; ++k) load_node(X, R, S, g, c, base + k * kThreads, false, L[k]);
+  for (int k = 0; k < kNodesPerThread; ++k)
+    load_node(X, R, S, g, c, base + k * kThreads, false, L[k], zero);
 #pragma unroll
   for (int k = 0; k < kNodesPerThread; ++k) {
     T v;
@@ -92,7 +101,7 @@ constexpr int kGroup = 16 / (int)sizeof(T);
 template <class T>
 __global__ __launch_bounds__(kThreads) void kb_relax_group(const StepPtrs<T>* tab, uint32_t count,
                                                            uint32_t nbi, Geom g, uint32_t lanes,
-                                                           int c, T pw, T h2) {
+                                                           int c, T pw, T h2, int zero_start) {
   constexpr int V = kGroup<T>;
   const uint32_t per = gridDim.x >> 3;
   const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
@@ -101,7 +110,8 @@ __global__ __launch_bounds__(kThreads) void kb_relax_group(const StepPtrs<T>* ta
   const StepPtrs<T> p = tab[slot];
   SlotGroup<T, V> G;
   load_group<T, V>(as_global((const T*)p.x), as_global(p.rhs), as_global(p.screen), g, c,
-                   (lb - slot * nbi) * kThreads + threadIdx.x, lanes, false, G);
+                   (lb - slot * nbi) * kThreads + threadIdx.x, lanes, false, G,
+                   start_zero(p, zero_start));
   T out[V];
   const uint32_t mask = group_relax<T, V>(G, g.n, pw, h2, out);
   store_group<T, T, V>(as_global(p.x) + G.own, out, mask);
@@ -454,7 +464,7 @@ inline unsigned padded8(uint64_t blocks) { return (unsigned)((blocks + 7u) & ~7u
 
 template <class T>
 void launch_b_relax(const StepPtrs<T>* tab, uint32_t count, const Geom& g, double pw,
-                    int iterations, hipStream_t s) {
+                    int iterations, hipStream_t s, int zero_start) {
   if (!count || !iterations) return;
   const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
   if (group_layout(g)) {
@@ -463,7 +473,8 @@ void launch_b_relax(const StepPtrs<T>* tab, uint32_t count, const Geom& g, doubl
     for (int it = 0; it < iterations; ++it)
       for (int c = 0; c < 2; ++c) {
         hipLaunchKernelGGL(kb_relax_group<T>, dim3(padded8((uint64_t)nbi * count)),
-                           dim3(kThreads), 0, s, tab, count, nbi, g, lanes, c, (T)pw, h2);
+                           dim3(kThreads), 0, s, tab, count, nbi, g, lanes, c, (T)pw, h2,
+                           it == 0 && c == 0 ? zero_start : (int)kZeroNone);
         MPR_LAUNCH_CHECK();
       }
     return;
@@ -472,7 +483,8 @@ void launch_b_relax(const StepPtrs<T>* tab, uint32_t count, const Geom& g, doubl
   for (int it = 0; it < iterations; ++it)
     for (int c = 0; c < 2; ++c) {
       hipLaunchKernelGGL(kb_relax_color<T>, dim3(padded8((uint64_t)nbi * count)), dim3(kThreads),
-                         0, s, tab, count, nbi, g, c, (T)pw, h2);
+                         0, s, tab, count, nbi, g, c, (T)pw, h2,
+                         it == 0 && c == 0 ? zero_start : (int)kZeroNone);
       MPR_LAUNCH_CHECK();
     }
 }
@@ -621,7 +633,7 @@ void init_coarse_kernels() {
 
 #define MPR_BINST(T)                                                                           \
   template void launch_b_relax<T>(const StepPtrs<T>*, uint32_t, const Geom&, double, int,      \
-                                  hipStream_t);                                                \
+                                  hipStream_t, int);                                           \
   template void launch_b_residual<T>(const StepPtrs<T>*, uint32_t, const Geom&, double,        \
                                      hipStream_t);                                             \
   template void launch_b_restrict<T>(const StepPtrs<T>*, uint32_t, const Geom&, const Geom&,   \

@@ -21,7 +21,12 @@ struct StepPtrs {
   T* res;
   T* xc;    // next-coarser x
   T* rhsc;  // next-coarser rhs
+  int top;  // this size class is the instance's finest level
 };
+
+// First sweep of a V-cycle level from a known-zero x: 0 x as stored, 1 zero for instances
+// below their finest level, 2 zero for every instance
+enum ZeroStart { kZeroNone = 0, kZeroBelowTop = 1, kZeroAll = 2 };
 
 // top-level buffers of one instance for the FP64 defect-correction loop
 struct TopPtrs {
@@ -43,9 +48,11 @@ struct CoarseEntry {
   const T* screen[kMaxCoarseClass + 1];
 };
 
+// zero_start: the first colour-0 sweep treats x as zero (ZeroStart); the colour-1 sweep
+// after it overwrites every colour-1 node, so x needs no zero fill
 template <class T>
 void launch_b_relax(const StepPtrs<T>* tab, uint32_t count, const Geom& g, double pw,
-                    int iterations, hipStream_t s);
+                    int iterations, hipStream_t s, int zero_start = kZeroNone);
 template <class T>
 void launch_b_residual(const StepPtrs<T>* tab, uint32_t count, const Geom& g, double pw,
                        hipStream_t s);

@@ -84,7 +84,7 @@ struct NodeLoads {
 // t: index inside colour block c (values >= g.C are clamped and marked not ok)
 template <class T, class PX, class PC>
 MPR_HD void load_node(PX X, PC R, PC S, const Geom& g, int c, uint32_t t, bool want_self,
-                      NodeLoads<T>& L) {
+                      NodeLoads<T>& L, bool x_zero = false) {
   const bool in = t < g.C;
   t = in ? t : g.C - 1;
   int x, y, z;
@@ -96,12 +96,17 @@ MPR_HD void load_node(PX X, PC R, PC S, const Geom& g, int c, uint32_t t, bool w
   L.idx = self;
   L.flags = (uint32_t)bx0 | ((uint32_t)bx1 << 1) | ((uint32_t)by0 << 2) | ((uint32_t)by1 << 3) |
             ((uint32_t)bz0 << 4) | ((uint32_t)bz1 << 5);
-  L.v[0] = X[bx0 ? split_index(g, x - 1, y, z) : self];
-  L.v[1] = X[bx1 ? split_index(g, x + 1, y, z) : self];
-  L.v[2] = X[by0 ? split_index(g, x, y - 1, z) : self];
-  L.v[3] = X[by1 ? split_index(g, x, y + 1, z) : self];
-  L.v[4] = X[bz0 ? split_index(g, x, y, z - 1) : self];
-  L.v[5] = X[bz1 ? split_index(g, x, y, z + 1) : self];
+  if (x_zero) {  // as load_group: the other colour is known to be zero
+#pragma unroll
+    for (int j = 0; j < 6; ++j) L.v[j] = T(0);
+  } else {
+    L.v[0] = X[bx0 ? split_index(g, x - 1, y, z) : self];
+    L.v[1] = X[bx1 ? split_index(g, x + 1, y, z) : self];
+    L.v[2] = X[by0 ? split_index(g, x, y - 1, z) : self];
+    L.v[3] = X[by1 ? split_index(g, x, y + 1, z) : self];
+    L.v[4] = X[bz0 ? split_index(g, x, y, z - 1) : self];
+    L.v[5] = X[bz1 ? split_index(g, x, y, z + 1) : self];
+  }
   L.r = R[self];
   L.s = S[self];
   L.self = want_self ? X[self] : T(0);
@@ -180,11 +185,14 @@ __device__ __forceinline__ void load_chunks(CT (&d)[NC], gptr<const T> p) {
   for (int i = 0; i < NC; ++i) d[i] = *(const __attribute__((address_space(1))) CT*)(p + i * W);
 }
 
-// lane: index of the group inside colour block c; lanes = C / V
+// lane: index of the group inside colour block c; lanes = C / V. x_zero: the other
+// colour's x is known to be zero (the first sweep from a zero start), so its loads are
+// skipped and the neighbour values are +0, exactly what a zero-filled x would give.
 template <class T, int V>
 __device__ __forceinline__ void load_group(gptr<const T> X, gptr<const T> R, gptr<const T> S,
                                            const Geom& g, int c, uint32_t lane, uint32_t lanes,
-                                           bool want_self, SlotGroup<T, V>& G) {
+                                           bool want_self, SlotGroup<T, V>& G,
+                                           bool x_zero = false) {
   G.in = lane < lanes;
   lane = G.in ? lane : lanes - 1;
   const uint32_t s0 = lane * (uint32_t)V;
@@ -200,13 +208,20 @@ __device__ __forceinline__ void load_group(gptr<const T> X, gptr<const T> R, gpt
   G.z = (int)z;
   G.p = p;
   const int n = g.n;
-  load_chunks<T>(G.row, X + o);
-  const uint32_t eo = p == 0 ? (k0 > 0 ? o - 1 : o) : (k0 + V < (uint32_t)g.H ? o + V : o);
-  G.edge = X[eo];
-  load_chunks<T>(G.ym, X + (y > 0 ? o - (uint32_t)g.H : o));
-  load_chunks<T>(G.yp, X + ((int)y + 1 < n ? o + (uint32_t)g.H : o));
-  load_chunks<T>(G.zm, X + (z > 0 ? o - g.nH : o));
-  load_chunks<T>(G.zp, X + ((int)z + 1 < n ? o + g.nH : o));
+  if (x_zero) {
+#pragma unroll
+    for (int i = 0; i < SlotGroup<T, V>::NC; ++i)
+      G.row[i] = G.ym[i] = G.yp[i] = G.zm[i] = G.zp[i] = typename SlotGroup<T, V>::CT(T(0));
+    G.edge = T(0);
+  } else {
+    load_chunks<T>(G.row, X + o);
+    const uint32_t eo = p == 0 ? (k0 > 0 ? o - 1 : o) : (k0 + V < (uint32_t)g.H ? o + V : o);
+    G.edge = X[eo];
+    load_chunks<T>(G.ym, X + (y > 0 ? o - (uint32_t)g.H : o));
+    load_chunks<T>(G.yp, X + ((int)y + 1 < n ? o + (uint32_t)g.H : o));
+    load_chunks<T>(G.zm, X + (z > 0 ? o - g.nH : o));
+    load_chunks<T>(G.zp, X + ((int)z + 1 < n ? o + g.nH : o));
+  }
   load_chunks<T>(G.r, R + G.own);
   load_chunks<T>(G.s, S + G.own);
   if (want_self) load_chunks<T>(G.self, X + G.own);

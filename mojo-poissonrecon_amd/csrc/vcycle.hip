@@ -58,7 +58,7 @@ void BatchedVcycle<T>::build(const std::vector<InstanceLevels<T>>& inst, void* t
       const int k = i.depth - s;
       const LevelPtrs<T>& a = i.levels[k];
       const LevelPtrs<T>& b = i.levels[k + 1];
-      StepPtrs<T> e{a.x, a.rhs, a.screen, a.res, b.x, b.rhs};
+      StepPtrs<T> e{a.x, a.rhs, a.screen, a.res, b.x, b.rhs, i.depth == s ? 1 : 0};
       steps.push_back(e);
       if (i.depth == s) tops.push_back(e);
     }
@@ -92,15 +92,22 @@ void BatchedVcycle<T>::build(const std::vector<InstanceLevels<T>>& inst, void* t
 
 template <class T>
 void BatchedVcycle<T>::run(double pw, int pre, int post, bool zero_tops, hipStream_t s) const {
-  if (zero_tops)
+  // A level that starts at zero (core.py:174 solution[l+1].fill(0); every level of the
+  // correction equation when zero_tops) needs no fill when pre-smoothing runs: the first
+  // colour-0 sweep reads the other colour as zero, and every node of that colour has a
+  // diagonal >= 3 h^-2 > 0 (point_weight >= 0), so the two colour sweeps write every node
+  // before anything reads it. Otherwise the levels are zero-filled as the reference does
+  // (the coarse kernel zeroes its own top level).
+  const bool fill = pre <= 0 || pw < 0.0;
+  if (zero_tops && fill)
     for (int c = sc + 1; c <= smax; ++c) launch_b_fill_x<T>(top_[c], top_count_[c], geom_[c], s);
+  const int zero_start = fill ? kZeroNone : (zero_tops ? kZeroAll : kZeroBelowTop);
   for (int c = smax; c > sc; --c) {
     if (!step_count_[c]) continue;
-    launch_b_relax<T>(step_[c], step_count_[c], geom_[c], pw, pre, s);
+    launch_b_relax<T>(step_[c], step_count_[c], geom_[c], pw, pre, s, zero_start);
     launch_b_residual<T>(step_[c], step_count_[c], geom_[c], pw, s);
-    // solution[l+1].fill(0) (core.py:174) is fused into the restriction; the coarse
-    // kernel zeroes its own top level
-    launch_b_restrict<T>(step_[c], step_count_[c], geom_[c], geom_[c - 1], c - 1 > sc, s);
+    launch_b_restrict<T>(step_[c], step_count_[c], geom_[c], geom_[c - 1], fill && c - 1 > sc,
+                         s);
   }
   launch_b_coarse<T>(zero_tops ? coarse_zero_ : coarse_, coarse_count_, coarse_class_, pw, pre,
                      post, s);
