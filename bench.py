@@ -223,7 +223,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the 64-batch and serving runs")
     ap.add_argument("--kernel-reps", type=int, default=40)
-    ap.add_argument("--only", choices=("fp64", "mixed"),
+    ap.add_argument("--only", choices=("fp64", "mixed", "mixed_batch"),
                     help="time one single-request mode and print its line (for per-mode "
                          "kernel traces)")
     ap.add_argument("--kernels-only", action="store_true",
@@ -248,6 +248,14 @@ def main() -> int:
     modes = {}
     meshes = {}
     jobs = {}
+    if args.only == "mixed_batch":
+        bclouds = [shapes.torus_workload(args.points, seed=s) for s in range(args.batch)]
+        per_launch, blat, _ = run_batch_mode(bclouds, [args.depth] * args.batch, "mixed",
+                                             args.steps)
+        print(json.dumps({"mode": "mixed_batch", "batch": args.batch,
+                          "recon_per_s": round(args.batch / per_launch, 3),
+                          "p50_ms": pct(blat, 50), "steps": args.steps}), flush=True)
+        return 0
     runs = (("fp64", _lib.PRECISION_FP64), ("mixed", _lib.PRECISION_MIXED))
     if args.only:
         runs = tuple(r for r in runs if r[0] == args.only)

@@ -88,6 +88,11 @@ void mpr_octree_codes_f64(const double* points, int64_t point_count, int64_t dep
 int64_t mpr_splat_f64(const double* points, const double* normals, int64_t point_count, int64_t n,
                       int64_t confidence, double* density, double* vx, double* vy, double* vz,
                       double* screen, double samples_per_node) {
+  if (point_count < 0 || point_count > kMaxPoints || n < 2) {
+    report(__func__, "point_count must be in [0, 2^28] and n >= 2");
+    return -1;
+  }
+  if (point_count == 0) return 0;
   if (ensure_device()) return -1;
   try {
     const size_t M = (size_t)n * n * n;
@@ -101,8 +106,20 @@ int64_t mpr_splat_f64(const double* points, const double* normals, int64_t point
     h2d(z.p, vz, M);
     h2d(s.p, screen, M);
     launch_fill<unsigned long long>(acc.p, 0ull, 1, kNull);
+    SplatScratch sc = splat_scratch_sizes((uint32_t)point_count);
+    DevBuf<uint64_t> keys(sc.pairs), ktmp(sc.pairs);
+    DevBuf<uint32_t> ids(sc.pairs), itmp(sc.pairs), hist(sc.radix_words), offs(sc.radix_words);
+    DevBuf<double> w1(sc.pairs), c2(3ull * sc.pairs);
+    sc.keys = keys.p;
+    sc.ktmp = ktmp.p;
+    sc.ids = ids.p;
+    sc.itmp = itmp.p;
+    sc.hist = hist.p;
+    sc.offs = offs.p;
+    sc.w1 = w1.p;
+    sc.c2 = c2.p;
     launch_splat(p.p, q.p, (uint32_t)point_count, (int)n, (int)(confidence != 0), d.p, x.p, y.p,
-                 z.p, s.p, samples_per_node, acc.p, kNull);
+                 z.p, s.p, samples_per_node, acc.p, sc, kNull);
     unsigned long long accepted = 0;
     d2h(&accepted, acc.p, 1);
     d2h(density, d.p, M);

@@ -129,8 +129,8 @@ class DefectTables {
  public:
   void build(const std::vector<int>& depths, const std::vector<TopPtrs>& tops, void* table_mem);
   static size_t table_bytes(size_t count) { return sizeof(TopPtrs) * count + 64 * 16; }
-  void defect(double pw, hipStream_t s) const;
-  void correct(hipStream_t s) const;
+  void defect(double pw, hipStream_t s) const;  // r32 = b - A x64
+  void correct(hipStream_t s) const;             // x64 += e32
 
  private:
   std::vector<Geom> geom_;

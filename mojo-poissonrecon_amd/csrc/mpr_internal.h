@@ -60,7 +60,9 @@ void launch_normalize(const double* pts, uint32_t P, double scale, double* part,
                       double* unit, hipStream_t s);
 void launch_octree_codes(const double* pts, uint32_t P, int depth, int64_t* codes, hipStream_t s);
 void launch_scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t count, uint32_t* total,
-                           hipStream_t s);
+                           hipStream_t s, uint32_t* scratch = nullptr);
+// words of `scratch` that let launch_scan_exclusive use many workgroups for `count` values
+uint32_t scan_scratch_words(uint32_t count);
 size_t radix_scratch_words(uint32_t n);
 void launch_radix_sort(uint64_t* keys, uint32_t* vals, uint64_t* k_tmp, uint32_t* v_tmp,
                        uint32_t n, int bits, uint32_t* hist, uint32_t* offs, hipStream_t s);
@@ -78,9 +80,26 @@ void launch_level_emit(const uint64_t* keys, uint32_t n, int d, int depth, const
 void launch_point_leaf(const uint32_t* perm, const uint32_t* uidx, uint32_t n,
                        const int64_t* offsets, int depth, int64_t* point_leaf, hipStream_t s);
 // capi.mojo:102-224; density/screen/vx/vy/vz accumulate (callers zero them)
+// Largest point count of one job or splat call (8 splat contributions per point are indexed
+// with 32 bits).
+constexpr int64_t kMaxPoints = int64_t(1) << 28;
+
+// Device scratch of the deterministic splat for up to pairs / 8 points.
+struct SplatScratch {
+  uint32_t pairs = 0;        // 8 per point
+  size_t radix_words = 0;    // words of hist and of offs
+  uint64_t *keys = nullptr, *ktmp = nullptr;
+  uint32_t *ids = nullptr, *itmp = nullptr, *hist = nullptr, *offs = nullptr;
+  double* w1 = nullptr;  // pairs
+  double* c2 = nullptr;  // 3 * pairs
+};
+SplatScratch splat_scratch_sizes(uint32_t P);
+// Both splat passes (capi.mojo:102-224), accumulating into the five grids in the reference's
+// point order; `accepted` (device) is incremented by the accepted point count.
 void launch_splat(const double* pts, const double* nrm, uint32_t P, int n, int confidence,
                   double* density, double* vx, double* vy, double* vz, double* screen,
-                  double spn, unsigned long long* accepted, hipStream_t s);
+                  double spn, unsigned long long* accepted, const SplatScratch& b,
+                  hipStream_t s);
 int iso_partial_count(uint32_t P);
 void launch_iso(const double* field, const double* pts, const double* nrm, uint32_t P, int n,
                 int confidence, double* part, double* iso, hipStream_t s);

@@ -320,15 +320,86 @@ __global__ __launch_bounds__(1024) void k_scan_exclusive(const uint32_t* in, uin
   if (threadIdx.x == 1023 && total) *total = part[1023];
 }
 
+// Multi-block exclusive scan for long inputs: per-chunk sums, a one-block scan of the sums,
+// then each chunk scanned locally from its offset.
+constexpr uint32_t kScanChunk = 4096;  // 1024 lanes x 4 values
+
+__device__ __forceinline__ uint32_t block_exclusive1024(uint32_t v, uint32_t* lds16,
+                                                        uint32_t& total) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  uint32_t inc = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t t = __shfl_up(inc, off, 64);
+    if (lane >= (uint32_t)off) inc += t;
+  }
+  if (lane == 63) lds16[wave] = inc;
+  __syncthreads();
+  uint32_t base = 0, tot = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < 16; ++w) {
+    const uint32_t c = lds16[w];
+    base += w < wave ? c : 0u;
+    tot += c;
+  }
+  __syncthreads();
+  total = tot;
+  return base + inc - v;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_chunk_sums(const uint32_t* in, uint32_t count,
+                                                          uint32_t* sums) {
+  __shared__ uint32_t lds16[16];
+  const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x * 4u;
+  uint32_t v = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k)
+    if (base + k < count) v += in[base + k];
+  uint32_t total;
+  block_exclusive1024(v, lds16, total);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_chunks(const uint32_t* in, uint32_t* out,
+                                                      uint32_t count, const uint32_t* offsets) {
+  __shared__ uint32_t lds16[16];
+  const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x * 4u;
+  uint32_t v[4], sum = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) {
+    v[k] = base + k < count ? in[base + k] : 0u;
+    sum += v[k];
+  }
+  uint32_t total;
+  uint32_t run = offsets[blockIdx.x] + block_exclusive1024(sum, lds16, total);
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) {
+    if (base + k < count) out[base + k] = run;
+    run += v[k];
+  }
+}
+
 // ---- splat (capi.mojo:102-224) ---------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_splat_pass1(const double* pts, const double* nrm,
+// Deterministic, in the reference's order. Every accepted point adds to its 8 corner nodes
+// (dz, dy, dx order) and the reference applies those additions point by point. Here each
+// (node, point) contribution becomes a pair (key = node, id = 8 p + corner) in point order;
+// a stable radix sort by node keeps each node's contributions in point order, and one lane
+// per node then adds them to the grid value sequentially -- the same additions in the same
+// order as the serial loop, so the grids are bit-identical to it and no atomics are used.
+// Pass 2 needs the finished density, so its contributions are computed after pass 1's sums
+// and summed along the same sorted pairs.
+
+// per point: accepted? and the trilinear cell of pass 1 (capi.mojo:127-173)
+__global__ __launch_bounds__(kThreads) void k_splat_pairs(const double* pts, const double* nrm,
                                                           uint32_t P, int n, int confidence,
-                                                          double* density, double* screen,
+                                                          uint64_t* keys, uint32_t* ids,
+                                                          double* w1, uint64_t sentinel,
                                                           unsigned long long* accepted) {
   __shared__ uint32_t lds4[4];
   const uint32_t p = blockIdx.x * kThreads + threadIdx.x;
   uint32_t ok = 0;
   if (p < P) {
+    const uint64_t base = 8ull * p;
     const double length = norm3(nrm[3ull * p], nrm[3ull * p + 1], nrm[3ull * p + 2]);
     if (length > 0.0) {
       ok = 1;
@@ -344,18 +415,23 @@ __global__ __launch_bounds__(kThreads) void k_splat_pass1(const double* pts, con
       if (iy >= res) { iy = res - 1; gy = (double)res; }
       if (iz >= res) { iz = res - 1; gz = (double)res; }
       const double fx = gx - (double)ix, fy = gy - (double)iy, fz = gz - (double)iz;
+      int k = 0;
       for (int dz = 0; dz < 2; ++dz) {
         const double wz = dz ? fz : 1.0 - fz;
         for (int dy = 0; dy < 2; ++dy) {
           const double wy = dy ? fy : 1.0 - fy;
-          for (int dx = 0; dx < 2; ++dx) {
+          for (int dx = 0; dx < 2; ++dx, ++k) {
             const double wx = dx ? fx : 1.0 - fx;
-            const double w = wx * wy * wz * sw;
-            const int64_t idx = nat_index((int)ix + dx, (int)iy + dy, (int)iz + dz, n);
-            unsafeAtomicAdd(&density[idx], 2.0 * w);
-            unsafeAtomicAdd(&screen[idx], w);
+            keys[base + k] = (uint64_t)nat_index((int)ix + dx, (int)iy + dy, (int)iz + dz, n);
+            ids[base + k] = (uint32_t)(base + k);
+            w1[base + k] = wx * wy * wz * sw;
           }
         }
+      }
+    } else {
+      for (int k = 0; k < 8; ++k) {
+        keys[base + k] = sentinel;  // sorts after every node and is skipped
+        ids[base + k] = (uint32_t)(base + k);
       }
     }
   }
@@ -363,10 +439,31 @@ __global__ __launch_bounds__(kThreads) void k_splat_pass1(const double* pts, con
   if (threadIdx.x == 0 && c) atomicAdd(accepted, (unsigned long long)c);
 }
 
-__global__ __launch_bounds__(kThreads) void k_splat_pass2(const double* pts, const double* nrm,
-                                                          uint32_t P, int n, int confidence,
-                                                          const double* density, double* vx,
-                                                          double* vy, double* vz, double spn) {
+// one lane per node: density += 2 w and screen += w over its sorted contributions
+__global__ __launch_bounds__(kThreads) void k_splat_sum1(const uint64_t* keys,
+                                                         const uint32_t* ids, const double* w1,
+                                                         uint32_t count, uint64_t M,
+                                                         double* density, double* screen) {
+  const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
+  if (j >= count) return;
+  const uint64_t key = keys[j];
+  if (key >= M || (j > 0 && keys[j - 1] == key)) return;
+  double d = density[key], sc = screen[key];
+  for (uint32_t i = j; i < count && keys[i] == key; ++i) {
+    const double w = w1[ids[i]];
+    d += 2.0 * w;
+    sc += w;
+  }
+  density[key] = d;
+  screen[key] = sc;
+}
+
+// pass 2 contributions (capi.mojo:176-223): n^ * (w (n-1) / factor) for each corner
+__global__ __launch_bounds__(kThreads) void k_splat_pass2_pairs(const double* pts,
+                                                                const double* nrm, uint32_t P,
+                                                                int n, int confidence,
+                                                                const double* density,
+                                                                double* c2, double spn) {
   const uint32_t p = blockIdx.x * kThreads + threadIdx.x;
   if (p >= P) return;
   double nx = nrm[3ull * p], ny = nrm[3ull * p + 1], nz = nrm[3ull * p + 2];
@@ -400,21 +497,43 @@ __global__ __launch_bounds__(kThreads) void k_splat_pass2(const double* pts, con
   }
   const double q = local / spn;
   const double factor = q > 1.0 ? q : 1.0;
+  double* out = c2 + 24ull * p;
+  int k = 0;
   for (int dz = 0; dz < 2; ++dz) {
     const double wz = dz ? fz : 1.0 - fz;
     for (int dy = 0; dy < 2; ++dy) {
       const double wy = dy ? fy : 1.0 - fy;
-      for (int dx = 0; dx < 2; ++dx) {
+      for (int dx = 0; dx < 2; ++dx, ++k) {
         const double wx = dx ? fx : 1.0 - fx;
         const double w = wx * wy * wz * sw;
-        const int64_t idx = nat_index((int)ix + dx, (int)iy + dy, (int)iz + dz, n);
         const double vw = w * (double)res / factor;
-        unsafeAtomicAdd(&vx[idx], -(nx * vw));
-        unsafeAtomicAdd(&vy[idx], -(ny * vw));
-        unsafeAtomicAdd(&vz[idx], -(nz * vw));
+        out[3 * k] = nx * vw;
+        out[3 * k + 1] = ny * vw;
+        out[3 * k + 2] = nz * vw;
       }
     }
   }
+}
+
+// one lane per node: v -= n^ vw over its sorted contributions
+__global__ __launch_bounds__(kThreads) void k_splat_sum2(const uint64_t* keys,
+                                                         const uint32_t* ids, const double* c2,
+                                                         uint32_t count, uint64_t M, double* vx,
+                                                         double* vy, double* vz) {
+  const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
+  if (j >= count) return;
+  const uint64_t key = keys[j];
+  if (key >= M || (j > 0 && keys[j - 1] == key)) return;
+  double a = vx[key], b = vy[key], c = vz[key];
+  for (uint32_t i = j; i < count && keys[i] == key; ++i) {
+    const double* q = c2 + 3ull * ids[i];
+    a -= q[0];
+    b -= q[1];
+    c -= q[2];
+  }
+  vx[key] = a;
+  vy[key] = b;
+  vz[key] = c;
 }
 
 // ---- iso value (capi.mojo:745-795) -----------------------------------------------------
@@ -513,14 +632,27 @@ void launch_octree_codes(const double* pts, uint32_t P, int depth, int64_t* code
 }
 
 void launch_scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t count, uint32_t* total,
-                           hipStream_t s) {
+                           hipStream_t s, uint32_t* scratch) {
+  if (count > kScanChunk && scratch) {
+    const uint32_t chunks = (count + kScanChunk - 1) / kScanChunk;
+    hipLaunchKernelGGL(k_scan_chunk_sums, dim3(chunks), dim3(1024), 0, s, in, count, scratch);
+    MPR_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, scratch, scratch, chunks,
+                       total);
+    MPR_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_chunks, dim3(chunks), dim3(1024), 0, s, in, out, count, scratch);
+    MPR_LAUNCH_CHECK();
+    return;
+  }
   hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, in, out, count, total);
   MPR_LAUNCH_CHECK();
 }
 
+uint32_t scan_scratch_words(uint32_t count) { return (count + kScanChunk - 1) / kScanChunk + 1; }
+
 size_t radix_scratch_words(uint32_t n) {
   const uint32_t nb = std::max<uint32_t>(1u, (n + kRadixTile - 1) / kRadixTile);
-  return 256ull * nb;
+  return 256ull * nb + scan_scratch_words(256u * nb);  // histogram, then the scan's sums
 }
 
 // Sorts (keys, vals) by the low `bits` key bits, stably. The result ends in (keys, vals);
@@ -536,7 +668,7 @@ void launch_radix_sort(uint64_t* keys, uint32_t* vals, uint64_t* k_tmp, uint32_t
     const int shift = 8 * p;
     hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kThreads), 0, s, ka, n, shift, hist, nb);
     MPR_LAUNCH_CHECK();
-    launch_scan_exclusive(hist, offs, 256u * nb, nullptr, s);
+    launch_scan_exclusive(hist, offs, 256u * nb, nullptr, s, hist + 256ull * nb);
     hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kThreads), 0, s, ka, va, kb, vb, n, shift,
                        offs, nb);
     MPR_LAUNCH_CHECK();
@@ -593,16 +725,36 @@ void launch_point_leaf(const uint32_t* perm, const uint32_t* uidx, uint32_t n,
   MPR_LAUNCH_CHECK();
 }
 
+SplatScratch splat_scratch_sizes(uint32_t P) {
+  SplatScratch b;
+  b.pairs = 8u * std::max<uint32_t>(1u, P);
+  b.radix_words = radix_scratch_words(b.pairs);
+  return b;
+}
+
 void launch_splat(const double* pts, const double* nrm, uint32_t P, int n, int confidence,
                   double* density, double* vx, double* vy, double* vz, double* screen,
-                  double spn, unsigned long long* accepted, hipStream_t s) {
+                  double spn, unsigned long long* accepted, const SplatScratch& b,
+                  hipStream_t s) {
   if (!P) return;
-  const unsigned nb = blocks_for(P, kThreads);
-  hipLaunchKernelGGL(k_splat_pass1, dim3(nb), dim3(kThreads), 0, s, pts, nrm, P, n, confidence,
-                     density, screen, accepted);
+  const uint32_t pairs = 8u * P;
+  if (pairs > b.pairs) throw HipError{"splat scratch smaller than the point count"};
+  const uint64_t M = (uint64_t)n * n * n;
+  int bits = 1;
+  while ((1ull << bits) <= M) ++bits;  // the sentinel M sorts after every node
+  const unsigned nb = blocks_for(P, kThreads), np = blocks_for(pairs, kThreads);
+  hipLaunchKernelGGL(k_splat_pairs, dim3(nb), dim3(kThreads), 0, s, pts, nrm, P, n, confidence,
+                     b.keys, b.ids, b.w1, M, accepted);
   MPR_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_splat_pass2, dim3(nb), dim3(kThreads), 0, s, pts, nrm, P, n, confidence,
-                     density, vx, vy, vz, spn);
+  launch_radix_sort(b.keys, b.ids, b.ktmp, b.itmp, pairs, bits, b.hist, b.offs, s);
+  hipLaunchKernelGGL(k_splat_sum1, dim3(np), dim3(kThreads), 0, s, b.keys, b.ids, b.w1, pairs, M,
+                     density, screen);
+  MPR_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_splat_pass2_pairs, dim3(nb), dim3(kThreads), 0, s, pts, nrm, P, n,
+                     confidence, density, b.c2, spn);
+  MPR_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_splat_sum2, dim3(np), dim3(kThreads), 0, s, b.keys, b.ids, b.c2, pairs, M,
+                     vx, vy, vz);
   MPR_LAUNCH_CHECK();
 }
 

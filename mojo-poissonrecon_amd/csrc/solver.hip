@@ -100,6 +100,7 @@ struct Job {
   int norm_nparts = 0;
   // extraction
   BitSurface surf;
+  SplatScratch splat;
   double *verts = nullptr, *vdens = nullptr;
   int64_t* faces = nullptr;
   uint64_t vcap = 0, fcap = 0;
@@ -316,6 +317,15 @@ void Job::plan(DevArena& a) {
   norm_part = a.alloc<double>(norm_nparts);
   bbox_part = a.alloc<double>(6ull * bbox_partial_count(std::max<uint32_t>(1u, max_points)));
   iso_part = a.alloc<double>(2ull * iso_partial_count(std::max<uint32_t>(1u, max_points)));
+  splat = splat_scratch_sizes(std::max<uint32_t>(1u, max_points));
+  splat.keys = a.alloc<uint64_t>(splat.pairs);
+  splat.ktmp = a.alloc<uint64_t>(splat.pairs);
+  splat.ids = a.alloc<uint32_t>(splat.pairs);
+  splat.itmp = a.alloc<uint32_t>(splat.pairs);
+  splat.hist = a.alloc<uint32_t>(splat.radix_words);
+  splat.offs = a.alloc<uint32_t>(splat.radix_words);
+  splat.w1 = a.alloc<double>(splat.pairs);
+  splat.c2 = a.alloc<double>(3ull * splat.pairs);
   surf = bit_surface_sizes(n);
   surf.bits = a.alloc<uint32_t>(surf.bit_words);
   surf.cmask = a.alloc<uint32_t>(surf.runs);
@@ -380,7 +390,7 @@ void Job::setup(const mpr_params& p, hipStream_t s) {
   launch_fill<double>(vz, 0.0, M, s);
   launch_fill<unsigned long long>(d_accepted, 0ull, 1, s);
   launch_splat(upts, nrm, P, n, p.confidence, density, vx, vy, vz, screen, p.samples_per_node,
-               d_accepted, s);
+               d_accepted, splat, s);
   launch_assemble(vx, vy, vz, screen, (p.stages & MPR_STAGE_GRIDS) ? rhs : nullptr, L64[0].rhs,
                   L64[0].screen, density, p.samples_per_node, geoms[0], p.point_weight, 0.5, s);
   for (size_t l = 1; l < L64.size(); ++l)
@@ -481,7 +491,7 @@ int mpr_device_info(char* name, int64_t name_len, int64_t* total_bytes, int64_t*
 }
 
 int64_t mpr_job_footprint(int32_t depth, int64_t max_points, int32_t precision) {
-  if (depth < 1 || depth > 10 || max_points < 0 || max_points > 0xffffffffll) return -1;
+  if (depth < 1 || depth > 10 || max_points < 0 || max_points > kMaxPoints) return -1;
   Job j;
   j.depth = depth;
   j.max_points = (uint32_t)max_points;
@@ -503,8 +513,8 @@ int mpr_job_create(int32_t depth, int64_t max_points, int32_t precision, mpr_job
     set_error("depth must be between 1 and 10");
     return -1;
   }
-  if (max_points < 0 || max_points > 0x7fffffffll) {
-    set_error("max_points out of range");
+  if (max_points < 0 || max_points > kMaxPoints) {
+    set_error("max_points must be between 0 and 2^28");
     return -1;
   }
   if (precision != MPR_PRECISION_FP64 && precision != MPR_PRECISION_MIXED) {

@@ -20,11 +20,6 @@ from mojo_poissonrecon import shapes
 pytestmark = pytest.mark.gpu
 
 
-def rel_close(a, b, rtol):
-    scale = max(np.abs(b).max(), 1e-300)
-    return np.abs(a - b).max() <= rtol * scale
-
-
 @pytest.mark.parametrize("depth,count,confidence", [(4, 500, False), (5, 2000, True),
                                                     (6, 8000, False), (7, 20000, True)])
 def test_solve_matches_oracle(depth, count, confidence):
@@ -33,12 +28,15 @@ def test_solve_matches_oracle(depth, count, confidence):
     unit, _, _ = oracle.normalize(p, 1.1)
     got = mpr.solve_screened_poisson(unit, n, depth=depth, confidence=confidence)
     want = oracle.solve(unit, n, depth, confidence=confidence)
-    assert rel_close(got.density, want["density"], 1e-13)
-    assert rel_close(got.screen, want["screen"], 1e-13)
-    assert rel_close(got.rhs, want["rhs"], 1e-11)
-    assert rel_close(got.field, want["field"], 1e-10)
-    assert got.iso_value == pytest.approx(want["iso_value"], rel=1e-10)
-    assert got.residual_norm == pytest.approx(want["residual_norm"], rel=1e-8)
+    # the splat sums in the reference's point order and every later stage repeats its
+    # arithmetic, so the grids are bit-identical; the two global reductions (iso value and
+    # residual norm) add in another order
+    assert np.array_equal(got.density, want["density"])
+    assert np.array_equal(got.screen, want["screen"])
+    assert np.array_equal(got.rhs, want["rhs"])
+    assert np.array_equal(got.field, want["field"])
+    assert got.iso_value == pytest.approx(want["iso_value"], rel=1e-12)
+    assert got.residual_norm == pytest.approx(want["residual_norm"], rel=1e-12)
     tree = oracle.build_octree(unit, depth)
     for name in ("depth", "morton", "parent", "child_start", "child_count", "point_leaf"):
         assert np.array_equal(getattr(got.octree, name), tree[name]), name
