@@ -119,7 +119,7 @@ int64_t mpr_splat_f64(const double* points, const double* normals, int64_t point
     sc.w1 = w1.p;
     sc.c2 = c2.p;
     launch_splat(p.p, q.p, (uint32_t)point_count, (int)n, (int)(confidence != 0), d.p, x.p, y.p,
-                 z.p, s.p, samples_per_node, acc.p, sc, kNull);
+                 z.p, s.p, samples_per_node, acc.p, sc, nullptr, kNull);
     unsigned long long accepted = 0;
     d2h(&accepted, acc.p, 1);
     d2h(density, d.p, M);
@@ -145,7 +145,7 @@ void mpr_assemble_f64(const double* vx, const double* vy, const double* vz, cons
     h2d(z.p, vz, M);
     h2d(s.p, screen, M);
     launch_assemble(x.p, y.p, z.p, s.p, r.p, nullptr, nullptr, nullptr, 1.0,
-                    make_geom((int32_t)n), point_weight, target_value, kNull);
+                    make_geom((int32_t)n), point_weight, target_value, nullptr, nullptr, kNull);
     d2h(rhs, r.p, M);
   } catch (const HipError& e) {
     report(__func__, e.msg);

@@ -232,12 +232,21 @@ __global__ __launch_bounds__(kThreads) void k_add_correction(double* X, const fl
   for (; i < count; i += stride) X[i] += (double)E[i];
 }
 
+// value of a splat grid at node i: unset `touched` bits mean the node received nothing
+__device__ __forceinline__ double splat_value(const double* v, const uint32_t* touched,
+                                              uint64_t i) {
+  if (touched && !((touched[i >> 5] >> (i & 31u)) & 1u)) return 0.0;
+  return v[i];
+}
+
 __global__ __launch_bounds__(kThreads) void k_assemble(const double* vx, const double* vy,
                                                        const double* vz, const double* scr,
                                                        double* rhs_nat, double* rhs_split,
                                                        double* scr_split, double* density,
                                                        double spn, Geom g, FastDiv div_n,
-                                                       double pw, double target) {
+                                                       double pw, double target,
+                                                       const uint32_t* touched,
+                                                       double* scr_nat_out) {
   const uint32_t t = logical_block() * kThreads + threadIdx.x;
   const int n = g.n;
   if (t >= (uint32_t)n * n * n) return;
@@ -248,26 +257,28 @@ __global__ __launch_bounds__(kThreads) void k_assemble(const double* vx, const d
   const double h_inv = (double)(n - 1);
   const double h_inv2 = h_inv * h_inv;
   const int64_t sy = n, sz = (int64_t)n * n;
+  const uint32_t* tb = touched;
   double div = 0.0;
-  if (x == 0) div += (vx[t + 1] - vx[t]) * h_inv;
-  else if (x == n - 1) div += (vx[t] - vx[t - 1]) * h_inv;
-  else div += (vx[t + 1] - vx[t - 1]) * (0.5 * h_inv);
-  if (y == 0) div += (vy[t + sy] - vy[t]) * h_inv;
-  else if (y == n - 1) div += (vy[t] - vy[t - sy]) * h_inv;
-  else div += (vy[t + sy] - vy[t - sy]) * (0.5 * h_inv);
-  if (z == 0) div += (vz[t + sz] - vz[t]) * h_inv;
-  else if (z == n - 1) div += (vz[t] - vz[t - sz]) * h_inv;
-  else div += (vz[t + sz] - vz[t - sz]) * (0.5 * h_inv);
-  const double s = scr[t];
+  if (x == 0) div += (splat_value(vx, tb, t + 1) - splat_value(vx, tb, t)) * h_inv;
+  else if (x == n - 1) div += (splat_value(vx, tb, t) - splat_value(vx, tb, t - 1)) * h_inv;
+  else div += (splat_value(vx, tb, t + 1) - splat_value(vx, tb, t - 1)) * (0.5 * h_inv);
+  if (y == 0) div += (splat_value(vy, tb, t + sy) - splat_value(vy, tb, t)) * h_inv;
+  else if (y == n - 1) div += (splat_value(vy, tb, t) - splat_value(vy, tb, t - sy)) * h_inv;
+  else div += (splat_value(vy, tb, t + sy) - splat_value(vy, tb, t - sy)) * (0.5 * h_inv);
+  if (z == 0) div += (splat_value(vz, tb, t + sz) - splat_value(vz, tb, t)) * h_inv;
+  else if (z == n - 1) div += (splat_value(vz, tb, t) - splat_value(vz, tb, t - sz)) * h_inv;
+  else div += (splat_value(vz, tb, t + sz) - splat_value(vz, tb, t - sz)) * (0.5 * h_inv);
+  const double s = splat_value(scr, tb, t);
   const double screening = pw * s * h_inv2;
   const double r = -div + screening * target;
   if (rhs_nat) rhs_nat[t] = r;
+  if (scr_nat_out) scr_nat_out[t] = s;
   if (rhs_split) {
     const uint32_t si = split_index(g, x, y, z);
     rhs_split[si] = r;
     scr_split[si] = s;
   }
-  if (density) density[t] = density[t] / spn;
+  if (density) density[t] = splat_value(density, tb, t) / spn;
 }
 
 }  // namespace
@@ -395,12 +406,13 @@ void launch_add_correction(double* X, const float* E, uint64_t count, hipStream_
 void launch_assemble(const double* vx, const double* vy, const double* vz,
                      const double* screen_nat, double* rhs_nat, double* rhs_split,
                      double* screen_split, double* density, double spn, const Geom& g,
-                     double pw, double target, hipStream_t s) {
+                     double pw, double target, const uint32_t* touched, double* screen_nat_out,
+                     hipStream_t s) {
   const uint64_t m = (uint64_t)g.n * g.n * g.n;
   const unsigned nb = xcd_grid(m, kThreads);
   hipLaunchKernelGGL(k_assemble, dim3(nb), dim3(kThreads), 0, s, vx, vy, vz, screen_nat,
                      rhs_nat, rhs_split, screen_split, density, spn, g, FastDiv((uint32_t)g.n),
-                     pw, target);
+                     pw, target, touched, screen_nat_out);
   MPR_LAUNCH_CHECK();
 }
 
@@ -416,6 +428,7 @@ MPR_INST(double)
 MPR_INST(float)
 #undef MPR_INST
 template void launch_fill<int32_t>(int32_t*, int32_t, uint64_t, hipStream_t);
+template void launch_fill<uint32_t>(uint32_t*, uint32_t, uint64_t, hipStream_t);
 template void launch_fill<int64_t>(int64_t*, int64_t, uint64_t, hipStream_t);
 template void launch_fill<unsigned long long>(unsigned long long*, unsigned long long, uint64_t,
                                               hipStream_t);

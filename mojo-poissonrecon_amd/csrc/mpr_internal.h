@@ -49,10 +49,14 @@ void launch_add_correction(double* X, const float* E, uint64_t count, hipStream_
 // layout (rhs_nat, optional) and the split layout (rhs_split); also copies the screening
 // weights into the split layout and, if density is non-null, divides it by spn in place
 // (core.py:255).
+// RHS assembly (capi.mojo:227-279) from the natural splat grids; rhs_nat / rhs_split /
+// screen_split / screen_nat_out / density (/= spn, core.py:255) are written when non-null.
+// touched: the splat's node bits; unset nodes read as +0.0 (their grids were not filled).
 void launch_assemble(const double* vx, const double* vy, const double* vz,
                      const double* screen_nat, double* rhs_nat, double* rhs_split,
                      double* screen_split, double* density, double spn, const Geom& g,
-                     double pw, double target, hipStream_t s);
+                     double pw, double target, const uint32_t* touched, double* screen_nat_out,
+                     hipStream_t s);
 // ---- point kernels (point_kernels.hip) --------------------------------------------------
 int bbox_partial_count(uint32_t P);
 // core.py:377-385 on the device: params = [origin x, y, z, width], unit = (p - origin)/width
@@ -95,11 +99,14 @@ struct SplatScratch {
 };
 SplatScratch splat_scratch_sizes(uint32_t P);
 // Both splat passes (capi.mojo:102-224), accumulating into the five grids in the reference's
-// point order; `accepted` (device) is incremented by the accepted point count.
+// point order; `accepted` (device) is incremented by the accepted point count. With
+// `touched` (ceil(n^3 / 32) zeroed words) the grids are taken as zero-filled without being
+// filled: only the nodes that receive contributions are written, and their bits are set
+// (readers treat an unset node as +0.0).
 void launch_splat(const double* pts, const double* nrm, uint32_t P, int n, int confidence,
                   double* density, double* vx, double* vy, double* vz, double* screen,
                   double spn, unsigned long long* accepted, const SplatScratch& b,
-                  hipStream_t s);
+                  uint32_t* touched, hipStream_t s);
 int iso_partial_count(uint32_t P);
 void launch_iso(const double* field, const double* pts, const double* nrm, uint32_t P, int n,
                 int confidence, double* part, double* iso, hipStream_t s);

@@ -439,16 +439,25 @@ __global__ __launch_bounds__(kThreads) void k_splat_pairs(const double* pts, con
   if (threadIdx.x == 0 && c) atomicAdd(accepted, (unsigned long long)c);
 }
 
-// one lane per node: density += 2 w and screen += w over its sorted contributions
+// one lane per node: density += 2 w and screen += w over its sorted contributions. With
+// `touched`, the grids count as zero-filled (the sums start at +0.0, as on the reference's
+// fresh grids) and the node's bit is set instead of reading the grid.
 __global__ __launch_bounds__(kThreads) void k_splat_sum1(const uint64_t* keys,
                                                          const uint32_t* ids, const double* w1,
                                                          uint32_t count, uint64_t M,
-                                                         double* density, double* screen) {
+                                                         double* density, double* screen,
+                                                         uint32_t* touched) {
   const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
   if (j >= count) return;
   const uint64_t key = keys[j];
   if (key >= M || (j > 0 && keys[j - 1] == key)) return;
-  double d = density[key], sc = screen[key];
+  double d = 0.0, sc = 0.0;
+  if (touched) {
+    atomicOr(&touched[key >> 5], 1u << (key & 31u));
+  } else {
+    d = density[key];
+    sc = screen[key];
+  }
   for (uint32_t i = j; i < count && keys[i] == key; ++i) {
     const double w = w1[ids[i]];
     d += 2.0 * w;
@@ -515,16 +524,22 @@ __global__ __launch_bounds__(kThreads) void k_splat_pass2_pairs(const double* pt
   }
 }
 
-// one lane per node: v -= n^ vw over its sorted contributions
+// one lane per node: v -= n^ vw over its sorted contributions (from +0.0 when the grids
+// count as zero-filled, as k_splat_sum1)
 __global__ __launch_bounds__(kThreads) void k_splat_sum2(const uint64_t* keys,
                                                          const uint32_t* ids, const double* c2,
                                                          uint32_t count, uint64_t M, double* vx,
-                                                         double* vy, double* vz) {
+                                                         double* vy, double* vz, int from_zero) {
   const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
   if (j >= count) return;
   const uint64_t key = keys[j];
   if (key >= M || (j > 0 && keys[j - 1] == key)) return;
-  double a = vx[key], b = vy[key], c = vz[key];
+  double a = 0.0, b = 0.0, c = 0.0;
+  if (!from_zero) {
+    a = vx[key];
+    b = vy[key];
+    c = vz[key];
+  }
   for (uint32_t i = j; i < count && keys[i] == key; ++i) {
     const double* q = c2 + 3ull * ids[i];
     a -= q[0];
@@ -735,7 +750,7 @@ SplatScratch splat_scratch_sizes(uint32_t P) {
 void launch_splat(const double* pts, const double* nrm, uint32_t P, int n, int confidence,
                   double* density, double* vx, double* vy, double* vz, double* screen,
                   double spn, unsigned long long* accepted, const SplatScratch& b,
-                  hipStream_t s) {
+                  uint32_t* touched, hipStream_t s) {
   if (!P) return;
   const uint32_t pairs = 8u * P;
   if (pairs > b.pairs) throw HipError{"splat scratch smaller than the point count"};
@@ -748,13 +763,13 @@ void launch_splat(const double* pts, const double* nrm, uint32_t P, int n, int c
   MPR_LAUNCH_CHECK();
   launch_radix_sort(b.keys, b.ids, b.ktmp, b.itmp, pairs, bits, b.hist, b.offs, s);
   hipLaunchKernelGGL(k_splat_sum1, dim3(np), dim3(kThreads), 0, s, b.keys, b.ids, b.w1, pairs, M,
-                     density, screen);
+                     density, screen, touched);
   MPR_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_splat_pass2_pairs, dim3(nb), dim3(kThreads), 0, s, pts, nrm, P, n,
                      confidence, density, b.c2, spn);
   MPR_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_splat_sum2, dim3(np), dim3(kThreads), 0, s, b.keys, b.ids, b.c2, pairs, M,
-                     vx, vy, vz);
+                     vx, vy, vz, touched ? 1 : 0);
   MPR_LAUNCH_CHECK();
 }
 

@@ -101,6 +101,7 @@ struct Job {
   // extraction
   BitSurface surf;
   SplatScratch splat;
+  uint32_t* touched = nullptr;  // splat node bits, ceil(n^3 / 32) words
   double *verts = nullptr, *vdens = nullptr;
   int64_t* faces = nullptr;
   uint64_t vcap = 0, fcap = 0;
@@ -317,6 +318,7 @@ void Job::plan(DevArena& a) {
   norm_part = a.alloc<double>(norm_nparts);
   bbox_part = a.alloc<double>(6ull * bbox_partial_count(std::max<uint32_t>(1u, max_points)));
   iso_part = a.alloc<double>(2ull * iso_partial_count(std::max<uint32_t>(1u, max_points)));
+  touched = a.alloc<uint32_t>((M + 31) / 32);
   splat = splat_scratch_sizes(std::max<uint32_t>(1u, max_points));
   splat.keys = a.alloc<uint64_t>(splat.pairs);
   splat.ktmp = a.alloc<uint64_t>(splat.pairs);
@@ -383,16 +385,16 @@ void Job::setup(const mpr_params& p, hipStream_t s) {
   if (p.stages & MPR_STAGE_NORMALIZE) launch_normalize(pts, P, p.scale, bbox_part, d_params, unit, s);
   if (!(p.stages & MPR_STAGE_SOLVE)) return;
   const double* upts = unit_points(p);
-  launch_fill<double>(density, 0.0, M, s);
-  launch_fill<double>(screen, 0.0, M, s);
-  launch_fill<double>(vx, 0.0, M, s);
-  launch_fill<double>(vy, 0.0, M, s);
-  launch_fill<double>(vz, 0.0, M, s);
+  // the splat grids are not zero-filled: the splat marks the nodes it writes and the
+  // assembly reads every other node as zero (and writes the full density grid)
+  launch_fill<uint32_t>(touched, 0u, (M + 31) / 32, s);
   launch_fill<unsigned long long>(d_accepted, 0ull, 1, s);
   launch_splat(upts, nrm, P, n, p.confidence, density, vx, vy, vz, screen, p.samples_per_node,
-               d_accepted, splat, s);
-  launch_assemble(vx, vy, vz, screen, (p.stages & MPR_STAGE_GRIDS) ? rhs : nullptr, L64[0].rhs,
-                  L64[0].screen, density, p.samples_per_node, geoms[0], p.point_weight, 0.5, s);
+               d_accepted, splat, touched, s);
+  const bool grids = (p.stages & MPR_STAGE_GRIDS) != 0;
+  launch_assemble(vx, vy, vz, screen, grids ? rhs : nullptr, L64[0].rhs, L64[0].screen, density,
+                  p.samples_per_node, geoms[0], p.point_weight, 0.5, touched,
+                  grids ? screen : nullptr, s);
   for (size_t l = 1; l < L64.size(); ++l)
     launch_restrict<double>(L64[l - 1].screen, geoms[l - 1], L64[l].screen, geoms[l], 4.0, s);
   // initial guess 0.5 (core.py:270); no samples: the solution is 0 (core.py:286-287), which
@@ -681,8 +683,8 @@ int mpr_job_fetch_mesh(mpr_job* job, double* vertices, double* vertex_density, i
 
 int mpr_job_fetch_grids(mpr_job* job, double* field, double* density, double* screen, double* rhs) {
   if (!job) return -1;
-  if (rhs && !job->grids_kept) {
-    set_error("rhs was not kept: run with MPR_STAGE_GRIDS");
+  if ((rhs || screen) && !job->grids_kept) {
+    set_error("rhs and screen are kept only by runs with MPR_STAGE_GRIDS");
     return -1;
   }
   MPR_GUARD_BEGIN

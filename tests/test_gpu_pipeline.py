@@ -212,7 +212,7 @@ def test_job_mesh_overflow_regrows_and_matches_extraction():
         r = job.wait()
         assert r.vertex_count > vcap
         v, f, d = job.fetch_mesh(r)
-        field, density = job.fetch_grids(rhs=False)[:2]
+        field, density = job.fetch_grids(kept=False)[:2]
     finally:
         job.close()
     mesh = mpr.extract_isosurface(field, iso_value=r.iso_value, density=density)
