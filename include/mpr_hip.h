@@ -112,7 +112,7 @@ enum {
   MPR_STAGE_NORMALIZE = 1,  /* points are world space: bbox normalise on device (core.py:377-385) */
   MPR_STAGE_SOLVE = 2,      /* splat, assemble, multigrid, residual, iso value */
   MPR_STAGE_EXTRACT = 4,    /* dual-grid mesh of the solution at the iso value */
-  MPR_STAGE_GRIDS = 8,      /* also keep natural-layout screen and rhs for mpr_job_fetch_grids */
+  MPR_STAGE_GRIDS = 8,      /* also write natural-layout field, screen, rhs for fetching */
   MPR_STAGE_WORLD = 16      /* write mesh vertices in world space (core.py:425) */
 };
 
@@ -155,7 +155,7 @@ int mpr_job_wait(mpr_job *job, mpr_result *result);
 /* Non-blocking completion test: 1 done, 0 running, <0 error. */
 int mpr_job_query(mpr_job *job);
 int mpr_job_fetch_mesh(mpr_job *job, double *vertices, double *vertex_density, int64_t *faces);
-/* Any pointer may be NULL. screen and rhs require MPR_STAGE_GRIDS on the run. */
+/* Any pointer may be NULL. field, screen and rhs require MPR_STAGE_GRIDS on the run. */
 int mpr_job_fetch_grids(mpr_job *job, double *field, double *density, double *screen,
                         double *rhs);
 /* Breadth-first octree of the job's (unit-cube) points (core.py:86-127); synchronises and

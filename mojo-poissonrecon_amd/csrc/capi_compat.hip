@@ -145,7 +145,8 @@ void mpr_assemble_f64(const double* vx, const double* vy, const double* vz, cons
     h2d(z.p, vz, M);
     h2d(s.p, screen, M);
     launch_assemble(x.p, y.p, z.p, s.p, r.p, nullptr, nullptr, nullptr, 1.0,
-                    make_geom((int32_t)n), point_weight, target_value, nullptr, nullptr, kNull);
+                    make_geom((int32_t)n), point_weight, target_value, nullptr, nullptr,
+                    nullptr, kNull);
     d2h(rhs, r.p, M);
   } catch (const HipError& e) {
     report(__func__, e.msg);
@@ -342,7 +343,7 @@ double mpr_iso_value_f64(const double* field, const double* points, const double
     h2d(f.p, field, M);
     h2d(p.p, points, 3 * point_count);
     h2d(q.p, normals, 3 * point_count);
-    launch_iso(f.p, p.p, q.p, (uint32_t)point_count, (int)n, (int)(confidence != 0), part.p, iso.p,
+    launch_iso(natural_field(f.p, (int)n), p.p, q.p, (uint32_t)point_count, (int)n, (int)(confidence != 0), part.p, iso.p,
                kNull);
     double value = kNaN;
     d2h(&value, iso.p, 1);

@@ -68,13 +68,12 @@ __global__ __launch_bounds__(kThreads) void k_cells_chunk_counts(const double* f
 }
 
 // vertex of an active cell: mean of the edge roots in x-, y-, z-axis order (capi.mojo:958-1038)
-__device__ __forceinline__ void dual_vertex(const double* f, const double* dens, int x, int y,
+__device__ __forceinline__ void dual_vertex(const FieldView& f, const double* dens, int x, int y,
                                             int z, int n, double iso, double out[3],
                                             double& vdens) {
   double v[8];
 #pragma unroll
-  for (int c = 0; c < 8; ++c)
-    v[c] = f[nat_index(x + (c & 1), y + ((c >> 1) & 1), z + ((c >> 2) & 1), n)];
+  for (int c = 0; c < 8; ++c) v[c] = f.at(x + (c & 1), y + ((c >> 1) & 1), z + ((c >> 2) & 1));
   double sx = 0.0, sy = 0.0, sz = 0.0;
   int roots = 0;
 #pragma unroll
@@ -147,7 +146,10 @@ __global__ __launch_bounds__(kThreads) void k_cells_emit(const double* f, const 
         const uint64_t idx = (uint64_t)running + pre;
         if (idx < capacity) {
           double v[3], d;
-          dual_vertex(f, dens, x, y, z, n, iso, v, d);
+          FieldView fv;
+          fv.nat = f;
+          fv.g.n = n;
+          dual_vertex(fv, dens, x, y, z, n, iso, v, d);
           if (xform) {
             for (int a = 0; a < 3; ++a) v[a] = v[a] * xform[3] + xform[a];
           }
@@ -288,7 +290,7 @@ __global__ __launch_bounds__(kThreads) void k_edges_emit(const double* f, int n,
 struct BitGeom {
   uint32_t n, cells, bw, cw;  // node words and cell runs per row
   uint32_t r0, r1, r2;        // rows of the three edge passes
-  FastDiv d_bw, d_cw, d_cells, d_cm1;
+  FastDiv d_bw, d_cw, d_cells, d_cm1, d_n;
 };
 
 __device__ __forceinline__ uint32_t node_word(const uint32_t* bits, const BitGeom& G, uint32_t y,
@@ -307,7 +309,7 @@ __device__ __forceinline__ uint32_t low_bits(uint32_t count) {
   return count >= 32u ? 0xffffffffu : ((1u << count) - 1u);
 }
 
-__global__ __launch_bounds__(kThreads) void k_sign_bits(const double* f, const double* isop,
+__global__ __launch_bounds__(kThreads) void k_sign_bits(FieldView f, const double* isop,
                                                         BitGeom G, uint32_t* bits) {
   const uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
   const uint32_t wi = (uint32_t)(i >> 5), k = (uint32_t)(i & 31u);
@@ -315,7 +317,8 @@ __global__ __launch_bounds__(kThreads) void k_sign_bits(const double* f, const d
   const uint32_t row = G.d_bw.div(wi < words ? wi : 0u);
   const uint32_t x = 32u * (wi - row * G.bw) + k;
   const bool in = wi < words && x < G.n;
-  const bool below = in && f[in ? (uint64_t)row * G.n + x : 0ull] < *isop;
+  const uint32_t z = G.d_n.div(row), y = row - z * G.n;
+  const bool below = in && f.at(in ? (int)x : 0, (int)y, (int)z) < *isop;
   const uint64_t bal = __ballot(below);
   if (k == 0 && wi < words) bits[wi] = (uint32_t)(bal >> (threadIdx.x & 32u));
 }
@@ -376,7 +379,7 @@ __global__ __launch_bounds__(kThreads) void k_bit_cells_index(BitGeom G, const u
 }
 
 // one lane per vertex: the dual vertex of its cell (capi.mojo:958-1038)
-__global__ __launch_bounds__(kThreads) void k_bit_vertices(const double* f, const double* dens,
+__global__ __launch_bounds__(kThreads) void k_bit_vertices(FieldView f, const double* dens,
                                                            const double* isop, BitGeom G,
                                                            const uint32_t* vcell,
                                                            const uint32_t* count, double* verts,
@@ -528,6 +531,7 @@ BitGeom make_bit_geom(int n) {
   G.d_cw = FastDiv(G.cw);
   G.d_cells = FastDiv(std::max<uint32_t>(1u, c));
   G.d_cm1 = FastDiv(std::max<uint32_t>(1u, cm1));
+  G.d_n = FastDiv(G.n);
   return G;
 }
 
@@ -610,7 +614,7 @@ BitSurface bit_surface_sizes(int n) {
   return b;
 }
 
-void launch_bit_extract(const double* f, const double* dens, int n, const double* iso,
+void launch_bit_extract(const FieldView& f, const double* dens, int n, const double* iso,
                         const BitSurface& b, uint32_t* counts, double* verts, double* vdens,
                         uint64_t vcap, int64_t* faces, uint64_t fcap, const double* xform,
                         hipStream_t s) {

@@ -95,9 +95,9 @@ __global__ __launch_bounds__(kThreads) void k_residual(const T* X, const T* R, c
     if (tt[k] >= 2u * g.C) continue;
     if (L[k].ok) {
       const T r = loaded_residual(L[k], pw, h2);
-      out[tt[k]] = (TR)r;
+      if (out) out[tt[k]] = (TR)r;
       sq += (double)r * (double)r;
-    } else {
+    } else if (out) {
       out[tt[k]] = TR(0);
     }
   }
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(kThreads) void k_residual_group(const T* X, const T
   if (lane >= 2u * lanes) G.in = false;
   T v[V];
   group_residual<T, V>(G, g.n, pw, h2, v);
-  store_group<T, TR, V>(as_global(out) + G.own, v, G.in ? (1u << V) - 1 : 0u);
+  if (out) store_group<T, TR, V>(as_global(out) + G.own, v, G.in ? (1u << V) - 1 : 0u);
   if (partials) {
     double sq = 0.0;
     if (G.in)
@@ -151,13 +151,15 @@ __global__ __launch_bounds__(1024) void k_norm_finish(const double* partials, in
 
 template <class T>
 __global__ __launch_bounds__(kThreads) void k_restrict(const T* F, Geom gf, T* Cc, Geom gc,
-                                                       T scale) {
+                                                       T scale, float* copy32) {
   const uint32_t t = logical_block() * kThreads + threadIdx.x;
   if (t >= 2u * gc.C) return;
   const int c = t >= gc.C;
   int X, Y, Z;
   if (!split_decode(gc, c, t - (uint32_t)c * gc.C, X, Y, Z)) return;
-  Cc[t] = restrict_node(F, gf, X, Y, Z, scale);
+  const T v = restrict_node(F, gf, X, Y, Z, scale);
+  Cc[t] = v;
+  if (copy32) copy32[t] = (float)v;
 }
 
 // slot-group restriction over both coarse colours (lanes [0, lanes) colour 0, then 1)
@@ -246,7 +248,7 @@ __global__ __launch_bounds__(kThreads) void k_assemble(const double* vx, const d
                                                        double spn, Geom g, FastDiv div_n,
                                                        double pw, double target,
                                                        const uint32_t* touched,
-                                                       double* scr_nat_out) {
+                                                       double* scr_nat_out, float* scr_split32) {
   const uint32_t t = logical_block() * kThreads + threadIdx.x;
   const int n = g.n;
   if (t >= (uint32_t)n * n * n) return;
@@ -277,6 +279,7 @@ __global__ __launch_bounds__(kThreads) void k_assemble(const double* vx, const d
     const uint32_t si = split_index(g, x, y, z);
     rhs_split[si] = r;
     scr_split[si] = s;
+    if (scr_split32) scr_split32[si] = (float)s;
   }
   if (density) density[t] = splat_value(density, tb, t) / spn;
 }
@@ -344,10 +347,10 @@ void launch_norm_finish(const double* partials, int count, double* out, hipStrea
 
 template <class T>
 void launch_restrict(const T* F, const Geom& gf, T* Cc, const Geom& gc, double scale,
-                     hipStream_t s) {
+                     hipStream_t s, float* copy32) {
   // FP32 only: with 8-byte words the grouped form reads 22.5 loads per coarse node against
   // the scalar form's 27 and measured slower
-  if (sizeof(T) == 4 && group_layout(gc) && group_layout(gf)) {
+  if (sizeof(T) == 4 && !copy32 && group_layout(gc) && group_layout(gf)) {
     const uint32_t lanes = gc.C / (uint32_t)(16 / sizeof(T));
     hipLaunchKernelGGL(k_restrict_group<T>, dim3(xcd_grid(2ull * lanes, kThreads)),
                        dim3(kThreads), 0, s, F, gf, Cc, gc, lanes, (T)scale);
@@ -355,7 +358,8 @@ void launch_restrict(const T* F, const Geom& gf, T* Cc, const Geom& gc, double s
     return;
   }
   const unsigned nb = xcd_grid(2ull * gc.C, kThreads);
-  hipLaunchKernelGGL(k_restrict<T>, dim3(nb), dim3(kThreads), 0, s, F, gf, Cc, gc, (T)scale);
+  hipLaunchKernelGGL(k_restrict<T>, dim3(nb), dim3(kThreads), 0, s, F, gf, Cc, gc, (T)scale,
+                     copy32);
   MPR_LAUNCH_CHECK();
 }
 
@@ -407,12 +411,12 @@ void launch_assemble(const double* vx, const double* vy, const double* vz,
                      const double* screen_nat, double* rhs_nat, double* rhs_split,
                      double* screen_split, double* density, double spn, const Geom& g,
                      double pw, double target, const uint32_t* touched, double* screen_nat_out,
-                     hipStream_t s) {
+                     float* screen_split32, hipStream_t s) {
   const uint64_t m = (uint64_t)g.n * g.n * g.n;
   const unsigned nb = xcd_grid(m, kThreads);
   hipLaunchKernelGGL(k_assemble, dim3(nb), dim3(kThreads), 0, s, vx, vy, vz, screen_nat,
                      rhs_nat, rhs_split, screen_split, density, spn, g, FastDiv((uint32_t)g.n),
-                     pw, target, touched, screen_nat_out);
+                     pw, target, touched, screen_nat_out, screen_split32);
   MPR_LAUNCH_CHECK();
 }
 
@@ -421,7 +425,8 @@ void launch_assemble(const double* vx, const double* vy, const double* vz,
   template void launch_relax<T>(T*, const T*, const T*, const Geom&, double, int, hipStream_t); \
   template void launch_relax_color<T>(T*, const T*, const T*, const Geom&, double, int,         \
                                       hipStream_t);                                            \
-  template void launch_restrict<T>(const T*, const Geom&, T*, const Geom&, double, hipStream_t); \
+  template void launch_restrict<T>(const T*, const Geom&, T*, const Geom&, double, hipStream_t,  \
+                                   float*);                                                     \
   template void launch_prolong_add<T>(const T*, const Geom&, T*, const Geom&, hipStream_t);  \
   template void launch_split_to_nat<T>(const T*, double*, const Geom&, hipStream_t);
 MPR_INST(double)

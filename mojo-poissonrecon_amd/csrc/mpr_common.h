@@ -115,6 +115,30 @@ MPR_HD bool split_decode(const Geom& g, int c, uint32_t t, int& x, int& y, int& 
   return x < g.n;
 }
 
+// A scalar n^3 grid in either layout: natural [z][y][x] (split == nullptr) or colour-split.
+struct FieldView {
+  const double* nat = nullptr;
+  const double* split = nullptr;
+  Geom g;  // g.n is the side in both layouts; the rest describes the split layout
+  MPR_HD double at(int x, int y, int z) const {
+    return split ? split[split_index(g, x, y, z)] : nat[nat_index(x, y, z, g.n)];
+  }
+};
+
+inline FieldView natural_field(const double* f, int n) {
+  FieldView v;
+  v.nat = f;
+  v.g.n = n;
+  return v;
+}
+
+inline FieldView split_field(const double* f, const Geom& g) {
+  FieldView v;
+  v.split = f;
+  v.g = g;
+  return v;
+}
+
 inline unsigned blocks_for(uint64_t work, unsigned threads) {
   return (unsigned)((work + threads - 1) / threads);
 }

@@ -37,7 +37,7 @@ int residual_partial_count(const Geom& g, size_t word_bytes);
 void launch_norm_finish(const double* partials, int count, double* out, hipStream_t s);
 template <class T>
 void launch_restrict(const T* F, const Geom& gf, T* Cc, const Geom& gc, double scale,
-                     hipStream_t s);
+                     hipStream_t s, float* copy32 = nullptr);  // copy32: also store as float
 template <class T>
 void launch_prolong_add(const T* Cc, const Geom& gc, T* F, const Geom& gf, hipStream_t s);
 template <class TI, class TO>
@@ -50,13 +50,14 @@ void launch_add_correction(double* X, const float* E, uint64_t count, hipStream_
 // weights into the split layout and, if density is non-null, divides it by spn in place
 // (core.py:255).
 // RHS assembly (capi.mojo:227-279) from the natural splat grids; rhs_nat / rhs_split /
-// screen_split / screen_nat_out / density (/= spn, core.py:255) are written when non-null.
+// screen_split (+ its float copy) / screen_nat_out / density (/= spn, core.py:255) are
+// written when non-null.
 // touched: the splat's node bits; unset nodes read as +0.0 (their grids were not filled).
 void launch_assemble(const double* vx, const double* vy, const double* vz,
                      const double* screen_nat, double* rhs_nat, double* rhs_split,
                      double* screen_split, double* density, double spn, const Geom& g,
                      double pw, double target, const uint32_t* touched, double* screen_nat_out,
-                     hipStream_t s);
+                     float* screen_split32, hipStream_t s);
 // ---- point kernels (point_kernels.hip) --------------------------------------------------
 int bbox_partial_count(uint32_t P);
 // core.py:377-385 on the device: params = [origin x, y, z, width], unit = (p - origin)/width
@@ -108,7 +109,7 @@ void launch_splat(const double* pts, const double* nrm, uint32_t P, int n, int c
                   double spn, unsigned long long* accepted, const SplatScratch& b,
                   uint32_t* touched, hipStream_t s);
 int iso_partial_count(uint32_t P);
-void launch_iso(const double* field, const double* pts, const double* nrm, uint32_t P, int n,
+void launch_iso(const FieldView& field, const double* pts, const double* nrm, uint32_t P, int n,
                 int confidence, double* part, double* iso, hipStream_t s);
 
 // ---- extraction kernels (extract_kernels.hip) --------------------------------------------
@@ -128,7 +129,7 @@ BitSurface bit_surface_sizes(int n);
 // Dual-grid extraction of the device pipeline (capi.mojo:839-1129, same vertex indices and
 // face order): counts[0] = vertices, counts[1] = quads; vertices past vcap and faces past
 // fcap are counted but not written. xform: optional [origin x, y, z, width] world transform.
-void launch_bit_extract(const double* f, const double* dens, int n, const double* iso,
+void launch_bit_extract(const FieldView& f, const double* dens, int n, const double* iso,
                         const BitSurface& b, uint32_t* counts, double* verts, double* vdens,
                         uint64_t vcap, int64_t* faces, uint64_t fcap, const double* xform,
                         hipStream_t s);

@@ -552,7 +552,7 @@ __global__ __launch_bounds__(kThreads) void k_splat_sum2(const uint64_t* keys,
 }
 
 // ---- iso value (capi.mojo:745-795) -----------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_iso_partials(const double* field,
+__global__ __launch_bounds__(kThreads) void k_iso_partials(FieldView field,
                                                            const double* pts, const double* nrm,
                                                            uint32_t P, int n, int confidence,
                                                            double* part) {
@@ -580,7 +580,7 @@ __global__ __launch_bounds__(kThreads) void k_iso_partials(const double* field,
         const double wy = dy ? fy : 1.0 - fy;
         for (int dx = 0; dx < 2; ++dx) {
           const double wx = dx ? fx : 1.0 - fx;
-          value += field[nat_index((int)ix + dx, (int)iy + dy, (int)iz + dz, n)] * wx * wy * wz;
+          value += field.at((int)ix + dx, (int)iy + dy, (int)iz + dz) * wx * wy * wz;
         }
       }
     }
@@ -775,7 +775,7 @@ void launch_splat(const double* pts, const double* nrm, uint32_t P, int n, int c
 
 int iso_partial_count(uint32_t P) { return (int)std::max<uint32_t>(1u, blocks_for(P, kThreads)); }
 
-void launch_iso(const double* field, const double* pts, const double* nrm, uint32_t P, int n,
+void launch_iso(const FieldView& field, const double* pts, const double* nrm, uint32_t P, int n,
                 int confidence, double* part, double* iso, hipStream_t s) {
   const int nb = iso_partial_count(P);
   hipLaunchKernelGGL(k_iso_partials, dim3(nb), dim3(kThreads), 0, s, field, pts, nrm, P, n,

@@ -375,7 +375,8 @@ Job::~Job() {
 }
 
 void Job::enqueue_extract(bool world, hipStream_t s) {
-  launch_bit_extract(field, density, n, d_iso, surf, d_counts, verts, vdens, vcap, faces, fcap,
+  launch_bit_extract(split_field(L64[0].x, geoms[0]), density, n, d_iso, surf, d_counts, verts,
+                     vdens, vcap, faces, fcap,
                      world ? d_params : nullptr, s);
 }
 
@@ -392,28 +393,31 @@ void Job::setup(const mpr_params& p, hipStream_t s) {
   launch_splat(upts, nrm, P, n, p.confidence, density, vx, vy, vz, screen, p.samples_per_node,
                d_accepted, splat, touched, s);
   const bool grids = (p.stages & MPR_STAGE_GRIDS) != 0;
+  const bool mixed = precision == MPR_PRECISION_MIXED;
+  // the FP32 screening weights of the mixed V-cycle are written beside the FP64 ones
   launch_assemble(vx, vy, vz, screen, grids ? rhs : nullptr, L64[0].rhs, L64[0].screen, density,
                   p.samples_per_node, geoms[0], p.point_weight, 0.5, touched,
-                  grids ? screen : nullptr, s);
+                  grids ? screen : nullptr, mixed ? L32[0].screen : nullptr, s);
   for (size_t l = 1; l < L64.size(); ++l)
-    launch_restrict<double>(L64[l - 1].screen, geoms[l - 1], L64[l].screen, geoms[l], 4.0, s);
+    launch_restrict<double>(L64[l - 1].screen, geoms[l - 1], L64[l].screen, geoms[l], 4.0, s,
+                            mixed ? L32[l].screen : nullptr);
   // initial guess 0.5 (core.py:270); no samples: the solution is 0 (core.py:286-287), which
   // the V-cycles then leave at 0 (zero right-hand side and screening)
   launch_fill<double>(L64[0].x, P ? 0.5 : 0.0, 2ull * geoms[0].C, s);
-  if (precision == MPR_PRECISION_MIXED)
-    for (size_t l = 0; l < L32.size(); ++l)
-      launch_convert<double, float>(L64[l].screen, L32[l].screen, 2ull * geoms[l].C, s);
 }
 
 // final FP64 residual norm, natural-layout field and iso value (core.py:289-303)
 void Job::finish(const mpr_params& p, hipStream_t s) {
   if (!(p.stages & MPR_STAGE_SOLVE)) return;
-  launch_residual<double, double>(L64[0].x, L64[0].rhs, L64[0].screen, L64[0].res, geoms[0],
-                                  p.point_weight, norm_part, s);
+  // only the norm of the final residual is reported; the iso value and the extraction read
+  // the solution in the solver's layout, so the natural field is written for fetching only
+  launch_residual<double, double>(L64[0].x, L64[0].rhs, L64[0].screen, (double*)nullptr,
+                                  geoms[0], p.point_weight, norm_part, s);
   launch_norm_finish(norm_part, norm_nparts, d_resnorm, s);
-  launch_split_to_nat<double>(L64[0].x, field, geoms[0], s);
+  if (p.stages & MPR_STAGE_GRIDS) launch_split_to_nat<double>(L64[0].x, field, geoms[0], s);
   if (P)
-    launch_iso(field, unit_points(p), nrm, P, n, p.confidence, iso_part, d_iso, s);
+    launch_iso(split_field(L64[0].x, geoms[0]), unit_points(p), nrm, P, n, p.confidence,
+               iso_part, d_iso, s);
   else
     launch_fill<double>(d_iso, 0.0, 1, s);
 }
@@ -683,8 +687,8 @@ int mpr_job_fetch_mesh(mpr_job* job, double* vertices, double* vertex_density, i
 
 int mpr_job_fetch_grids(mpr_job* job, double* field, double* density, double* screen, double* rhs) {
   if (!job) return -1;
-  if ((rhs || screen) && !job->grids_kept) {
-    set_error("rhs and screen are kept only by runs with MPR_STAGE_GRIDS");
+  if ((field || rhs || screen) && !job->grids_kept) {
+    set_error("field, screen and rhs are kept only by runs with MPR_STAGE_GRIDS");
     return -1;
   }
   MPR_GUARD_BEGIN
@@ -1061,8 +1065,8 @@ int mpr_extract(const double* field, const double* density, int64_t n, double is
       launch_fill<double>(d, 0.0, M, s);
     MPR_HIP(hipMemcpyAsync(isod, &iso, 8, hipMemcpyHostToDevice, s));
     // count (nothing fits a zero capacity), then emit into exactly sized buffers
-    launch_bit_extract(f, d, (int)n, isod, surf, counts, nullptr, nullptr, 0, nullptr, 0, nullptr,
-                       s);
+    launch_bit_extract(natural_field(f, (int)n), d, (int)n, isod, surf, counts, nullptr, nullptr,
+                       0, nullptr, 0, nullptr, s);
     uint32_t hc[2] = {0, 0};
     MPR_HIP(hipMemcpyAsync(hc, counts, 8, hipMemcpyDeviceToHost, s));
     MPR_HIP(hipStreamSynchronize(s));
@@ -1073,7 +1077,8 @@ int mpr_extract(const double* field, const double* density, int64_t n, double is
     m->verts = m->arena.alloc<double>(3 * std::max<uint64_t>(1, m->nv));
     m->vdens = m->arena.alloc<double>(std::max<uint64_t>(1, m->nv));
     m->faces = m->arena.alloc<int64_t>(6 * std::max<uint64_t>(1, m->nq));
-    launch_bit_extract(f, d, (int)n, isod, surf, counts, m->verts, m->vdens, m->nv, m->faces,
+    launch_bit_extract(natural_field(f, (int)n), d, (int)n, isod, surf, counts, m->verts,
+                       m->vdens, m->nv, m->faces,
                        2 * m->nq, nullptr, s);
     MPR_HIP(hipStreamSynchronize(s));
     scratch.release();

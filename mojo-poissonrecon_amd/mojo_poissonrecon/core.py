@@ -191,10 +191,10 @@ class Job:
         return vertices, faces, density
 
     def fetch_grids(self, kept: bool = True):
-        """[field, density, screen, rhs]; screen and rhs (None unless `kept`) need a run
-        with STAGE_GRIDS."""
+        """[field, density, screen, rhs]; field, screen and rhs (None unless `kept`) need a
+        run with STAGE_GRIDS."""
         n = (1 << self.depth) + 1
-        grids = [np.empty((n, n, n), np.float64) if i < 2 or kept else None for i in range(4)]
+        grids = [np.empty((n, n, n), np.float64) if i == 1 or kept else None for i in range(4)]
         check(self._lib.mpr_job_fetch_grids(self.handle,
                                             *(ptr(g) if g is not None else None for g in grids)),
               "mpr_job_fetch_grids")

@@ -208,11 +208,11 @@ def test_job_mesh_overflow_regrows_and_matches_extraction():
     job = Job(depth, count)
     try:
         job.set_points(points, normals)
-        job.run(Job.params(_lib.STAGE_SOLVE | _lib.STAGE_EXTRACT))
+        job.run(Job.params(_lib.STAGE_SOLVE | _lib.STAGE_EXTRACT | _lib.STAGE_GRIDS))
         r = job.wait()
         assert r.vertex_count > vcap
         v, f, d = job.fetch_mesh(r)
-        field, density = job.fetch_grids(kept=False)[:2]
+        field, density = job.fetch_grids()[:2]
     finally:
         job.close()
     mesh = mpr.extract_isosurface(field, iso_value=r.iso_value, density=density)
