@@ -17,8 +17,8 @@ Modes, each timed over --steps iterations after --warmup:
   mixed_batch --batch depth-8 reconstructions (different seeds) as one batched graph launch
 `value` is the mixed_batch recon/s. Also reported: the 64-instance batch of uneven depths
 and sizes (depths U{5..8}, points log-U[2k, 200k], sphere/torus/superquadric) with its
-per-instance error against FP64, a served request stream, the dominant kernel's roofline
-and the CPU oracle baseline. Prints one JSON line.
+per-instance error against FP64, the depth-9 2M-point mixed/FP64 pair, a served request
+stream, the dominant kernel's roofline and the CPU oracle baseline. Prints one JSON line.
 """
 
 from __future__ import annotations
@@ -139,10 +139,48 @@ def run_batch_mode(clouds, depths, precision, reps):
     return total / reps, np.array(lat), meshes
 
 
-def serve_stream(requests: int, qps: float, seed: int):
+def depth9_run(steps: int):
+    """Depth-9 workload: 2,000,000 superquadric points (exponents 0.3/0.3, half-extents
+    0.8/0.6/0.4, sigma 0.001 diag, 3 deg normal noise, unit normals), mixed precision and
+    all-FP64 single requests at the same depth, with both final FP64 residual norms."""
+    from mojo_poissonrecon import _lib, shapes
+    from mojo_poissonrecon.core import Job
+
+    p, n = shapes.superquadric_workload(2_000_000, seed=0)
+    params = Job.params(_lib.STAGE_NORMALIZE | _lib.STAGE_SOLVE | _lib.STAGE_EXTRACT
+                        | _lib.STAGE_WORLD)
+    out, meshes = {}, {}
+    for name, prec in (("fp64", _lib.PRECISION_FP64), ("mixed", _lib.PRECISION_MIXED)):
+        job = Job(9, len(p), prec)
+        job.set_points(p, n)
+
+        def step(job=job):
+            job.run(params)
+            job.wait()
+
+        total, lat = timed(step, steps, 1)
+        r = job.wait()
+        meshes[name] = job.fetch_mesh(r)[0]
+        out[name] = {"recon_per_s": round(steps / total, 3), "p50_ms": pct(lat, 50),
+                     "residual_norm": r.residual_norm, "iso_value": r.iso_value,
+                     "vertices": int(r.vertex_count),
+                     "footprint_gb": round(Job.footprint(9, len(p), prec) / 1e9, 2)}
+        job.close()
+    diag = float(np.linalg.norm(p.max(axis=0) - p.min(axis=0)))
+    return {"workload": "superquadric 2,000,000 points, depth 9 (513^3), sigma 0.001 diag, "
+                        "3 deg normal noise", **out,
+            "hausdorff_mixed_vs_fp64_over_diag": hausdorff_bound(meshes["mixed"], meshes["fp64"])
+            / diag,
+            "residual_ratio_mixed_over_fp64": out["mixed"]["residual_norm"]
+            / out["fp64"]["residual_norm"]}
+
+
+def serve_stream(requests: int, qps: float, seed: int, warmup: int = 0):
     """Poisson arrivals at `qps`; each request a cloud of 5k-500k points (log-uniform) on a
     random shape, randomly rotated and scaled into [-1,1]^3, depth from {6,7,8,9} weighted
-    4:3:2:1. A pool of pre-generated clouds is rotated per request to bound host time."""
+    4:3:2:1. A pool of pre-generated clouds is rotated per request to bound host time. The
+    first `warmup` requests of the same stream fill the server's job pool (device buffers
+    are allocated on first use) and are excluded from the latency percentiles."""
     from mojo_poissonrecon import shapes
     from mojo_poissonrecon.serve import ReconstructionServer
 
@@ -153,17 +191,21 @@ def serve_stream(requests: int, qps: float, seed: int):
         npts = int(round(math.exp(rng.uniform(math.log(5e3), math.log(5e5)))))
         p, n = makers[i % 3](npts, seed=7000 + i)
         pool.append(shapes.perturb(p, n, seed=7000 + i, sigma=0.001, degrees=2.0))
-    gaps = rng.exponential(1.0 / qps, requests)
+    total = requests + warmup
+    gaps = rng.exponential(1.0 / qps, total)
     arrivals = np.cumsum(gaps)
-    depth_choice = rng.choice([6, 7, 8, 9], size=requests, p=[0.4, 0.3, 0.2, 0.1])
-    picks = rng.integers(0, len(pool), requests)
+    depth_choice = rng.choice([6, 7, 8, 9], size=total, p=[0.4, 0.3, 0.2, 0.1])
+    picks = rng.integers(0, len(pool), total)
     server = ReconstructionServer(max_inflight=16, max_batch=8, precision="mixed")
     done = []
     start = time.perf_counter()
     nxt = 0
-    while len(done) < requests:
+    measured_from = None
+    while len(done) < total:
         now = time.perf_counter() - start
-        while nxt < requests and arrivals[nxt] <= now:
+        if measured_from is None and nxt >= warmup:
+            measured_from = now
+        while nxt < total and arrivals[nxt] <= now:
             p, n = pool[picks[nxt]]
             rot = shapes.random_rotation(rng)
             s = float(rng.uniform(0.5, 1.0))
@@ -174,9 +216,15 @@ def serve_stream(requests: int, qps: float, seed: int):
         done += server.step()
     wall = time.perf_counter() - start
     server.close()
-    lat = np.array([r.latency for r in done])
+    measured = [r for r in done if r.id >= warmup]
+    lat = np.array([r.latency for r in measured])
+    cold = np.array([r.latency for r in done if r.id < warmup]) if warmup else np.array([0.0])
+    last = max(r.completed for r in measured) - start
+    first = min(r.arrival for r in measured) - start
     return {
-        "requests": requests, "qps_offered": qps, "recon_per_s": round(requests / wall, 2),
+        "requests": requests, "warmup_requests": warmup, "qps_offered": qps,
+        "recon_per_s": round(len(measured) / (last - first), 2),
+        "warmup_p99_ms": pct(cold, 99),
         "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "batches": server.stats.batches,
         "peak_inflight": server.stats.peak_inflight,
         "peak_hbm_gb": round(server.stats.peak_bytes / 1e9, 2),
@@ -220,6 +268,10 @@ def main() -> int:
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-qps", type=float, default=100.0)
+    ap.add_argument("--serve-warmup", type=int, default=100,
+                    help="requests that fill the job pool before the measured stream")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run only the served stream and print its line")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the 64-batch and serving runs")
     ap.add_argument("--kernel-reps", type=int, default=40)
@@ -248,6 +300,10 @@ def main() -> int:
     modes = {}
     meshes = {}
     jobs = {}
+    if args.serve_only:
+        line = serve_stream(args.serve_requests, args.serve_qps, seed=2, warmup=args.serve_warmup)
+        print(json.dumps({"mode": "serve", **line}), flush=True)
+        return 0
     if args.only == "mixed_batch":
         bclouds = [shapes.torus_workload(args.points, seed=s) for s in range(args.batch)]
         per_launch, blat, _ = run_batch_mode(bclouds, [args.depth] * args.batch, "mixed",
@@ -333,7 +389,10 @@ def main() -> int:
             "launch_ms": round(t32 * 1e3, 3),
         }
         log("batch64:", extras["batch64"])
-        extras["serve"] = serve_stream(args.serve_requests, args.serve_qps, seed=2)
+        extras["depth9"] = depth9_run(3)
+        log("depth9:", extras["depth9"])
+        extras["serve"] = serve_stream(args.serve_requests, args.serve_qps, seed=2,
+                                       warmup=args.serve_warmup)
         log("serve:", extras["serve"])
 
     mixed_b, fp64 = modes["mixed_batch"], modes["fp64"]

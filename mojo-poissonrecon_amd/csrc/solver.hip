@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -193,11 +194,66 @@ void Job::result(mpr_result* r) const {
 
 // A fixed set of jobs whose V-cycles run together: one launch per V-cycle step and grid
 // size class covers every job (mpr_batch in include/mpr_hip.h).
+// Streams and table buffers reused across batches: forming and closing a batch in a
+// serving loop then allocates and frees no device memory after warm-up (a device free can
+// wait for work queued on other streams).
+class BatchResources {
+ public:
+  static BatchResources& get() {
+    static BatchResources r;
+    return r;
+  }
+  hipStream_t take_stream() {
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      if (!streams_.empty()) {
+        hipStream_t s = streams_.back();
+        streams_.pop_back();
+        return s;
+      }
+    }
+    hipStream_t s = nullptr;
+    MPR_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return s;
+  }
+  void give_stream(hipStream_t s) {
+    std::lock_guard<std::mutex> lock(mu_);
+    streams_.push_back(s);
+  }
+  // a buffer of at least `bytes` (capacity a power of two, at least 64 KiB)
+  void* take_buffer(size_t bytes, size_t& cap) {
+    cap = 64 * 1024;
+    while (cap < bytes) cap <<= 1;
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      auto it = buffers_.find(cap);
+      if (it != buffers_.end()) {
+        void* p = it->second;
+        buffers_.erase(it);
+        return p;
+      }
+    }
+    void* p = nullptr;
+    MPR_HIP(hipMalloc(&p, cap));
+    return p;
+  }
+  void give_buffer(void* p, size_t cap) {
+    std::lock_guard<std::mutex> lock(mu_);
+    buffers_.emplace(cap, p);
+  }
+
+ private:
+  std::mutex mu_;
+  std::vector<hipStream_t> streams_;
+  std::multimap<size_t, void*> buffers_;
+};
+
 struct Batch {
   std::vector<Job*> jobs;
   int precision = 0;
   hipStream_t stream = nullptr;
-  DevArena arena;
+  void* tables = nullptr;  // pooled (BatchResources)
+  size_t tables_cap = 0;
   BatchedVcycle<double> vc64;
   BatchedVcycle<float> vc32;
   DefectTables dtab;
@@ -216,8 +272,8 @@ struct Batch {
   }
   ~Batch() {
     drop_graph();
-    arena.release();
-    if (stream) (void)hipStreamDestroy(stream);
+    if (tables) BatchResources::get().give_buffer(tables, tables_cap);
+    if (stream) BatchResources::get().give_stream(stream);
   }
 };
 
@@ -234,10 +290,17 @@ void Batch::build() {
       tops.push_back(j->top_ptrs());
     }
   }
-  vc64.build(i64, arena.alloc<unsigned char>(BatchedVcycle<double>::table_bytes(i64)));
-  if (precision == MPR_PRECISION_MIXED) {
-    vc32.build(i32, arena.alloc<unsigned char>(BatchedVcycle<float>::table_bytes(i32)));
-    dtab.build(depths, tops, arena.alloc<unsigned char>(DefectTables::table_bytes(tops.size())));
+  const bool mixed = precision == MPR_PRECISION_MIXED;
+  auto round = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t b64 = round(BatchedVcycle<double>::table_bytes(i64));
+  const size_t b32 = mixed ? round(BatchedVcycle<float>::table_bytes(i32)) : 0;
+  const size_t bd = mixed ? round(DefectTables::table_bytes(tops.size())) : 0;
+  tables = BatchResources::get().take_buffer(b64 + b32 + bd, tables_cap);
+  unsigned char* t = static_cast<unsigned char*>(tables);
+  vc64.build(i64, t);
+  if (mixed) {
+    vc32.build(i32, t + b64);
+    dtab.build(depths, tops, t + b64 + b32);
   }
 }
 
@@ -820,7 +883,7 @@ int mpr_batch_create(mpr_job* const* jobs, int32_t count, mpr_batch** out) {
   try {
     b->precision = jobs[0]->precision;
     for (int i = 0; i < count; ++i) b->jobs.push_back(jobs[i]);
-    MPR_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    b->stream = BatchResources::get().take_stream();
     for (Job* j : b->jobs) MPR_HIP(hipStreamSynchronize(j->stream));
     b->build();
     *out = b;

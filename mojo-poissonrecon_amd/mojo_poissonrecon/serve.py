@@ -3,15 +3,15 @@
 Requests (oriented point clouds with a depth) are admitted while device memory and slots
 allow: an admitted request takes an idle device job of its depth from a pool (or a new
 one, if its HBM footprint fits the budget), uploads its points on that job's stream and
-enqueues the whole reconstruction without waiting. Requests admitted in the same tick run
-as one batch (one launch per V-cycle step for all of them). Work of different requests
-overlaps on the device; completed meshes are read back as soon as their stream finishes,
-which frees the slot and memory for the next arrivals. Latency is measured from a
-request's arrival to its mesh being on the host.
+enqueues the whole reconstruction without waiting. Requests of one depth admitted in the
+same tick run as one batch (one launch per V-cycle step for all of them). Work of
+different requests overlaps on the device; completed meshes are read back as soon as their
+stream finishes, which frees the slot and memory for the next arrivals. Latency is
+measured from a request's arrival to its mesh being on the host.
 
 The FP64 single-request reconstruct() stays the reference path; each served result is
-the same as that request's solo run up to the reordering of FP64 atomic additions in the
-splat.
+bit-identical to that request's solo run (every stage is deterministic and an instance's
+arithmetic does not depend on its batch).
 """
 
 from __future__ import annotations
@@ -176,8 +176,11 @@ class ReconstructionServer:
         return job
 
     def _admit(self) -> None:
-        group: list[tuple[Request, Job]] = []
-        while self._waiting and self.inflight + len(group) < self.max_inflight:
+        # one batch per depth: a batch completes as a whole, so shallow requests are not
+        # held back by a deeper one admitted in the same tick
+        groups: dict[int, list[tuple[Request, Job]]] = {}
+        pending = 0
+        while self._waiting and self.inflight + pending < self.max_inflight:
             r = self._waiting[0]
             job = self._take_job(r)
             if job is None:
@@ -186,12 +189,16 @@ class ReconstructionServer:
             self._busy.add(id(job))
             job.set_points(r.points, r.normals)
             r.admitted = time.perf_counter()
+            group = groups.setdefault(r.depth, [])
             group.append((r, job))
+            pending += 1
             if len(group) == self.max_batch:
                 self._launch(group)
-                group = []
-        if group:
-            self._launch(group)
+                pending -= len(group)
+                groups[r.depth] = []
+        for group in groups.values():
+            if group:
+                self._launch(group)
         self.stats.peak_inflight = max(self.stats.peak_inflight, self.inflight)
 
     def _launch(self, group) -> None:
