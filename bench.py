@@ -275,6 +275,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the 64-batch and serving runs")
     ap.add_argument("--kernel-reps", type=int, default=40)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="launch the batched kernels with the default workgroup size")
     ap.add_argument("--only", choices=("fp64", "mixed", "mixed_batch"),
                     help="time one single-request mode and print its line (for per-mode "
                          "kernel traces)")
@@ -293,6 +295,12 @@ def main() -> int:
 
     if not os.path.exists(_build.LIB):
         mpr.build()
+    if not args.no_autotune:
+        from mojo_poissonrecon import tune
+        t0 = time.perf_counter()
+        chosen = tune.autotune(max_depth=max(5, min(args.depth, 10)), reps=20)
+        log(f"autotune {time.perf_counter() - t0:.2f} s:",
+            " ".join(f"{k}/{p}/{s}={b}" for (k, p, s), b in sorted(chosen.items())))
 
     points, normals = shapes.torus_workload(args.points, seed=0)
     stages = (_lib.STAGE_NORMALIZE | _lib.STAGE_SOLVE | _lib.STAGE_EXTRACT | _lib.STAGE_WORLD)

@@ -9,6 +9,7 @@
 #include <string>
 
 #include "mpr_batch.h"
+#include "mpr_tune.h"
 #include "stencil.h"
 
 namespace mpr {
@@ -98,10 +99,11 @@ __global__ __launch_bounds__(kThreads) void kb_residual(const StepPtrs<T>* tab, 
 template <class T>
 constexpr int kGroup = 16 / (int)sizeof(T);
 
-template <class T>
-__global__ __launch_bounds__(kThreads) void kb_relax_group(const StepPtrs<T>* tab, uint32_t count,
-                                                           uint32_t nbi, Geom g, uint32_t lanes,
-                                                           int c, T pw, T h2, int zero_start) {
+// B: workgroup size (chosen per kernel, precision and size class by the tuner, mpr_tune.h)
+template <class T, int B>
+__global__ __launch_bounds__(B) void kb_relax_group(const StepPtrs<T>* tab, uint32_t count,
+                                                    uint32_t nbi, Geom g, uint32_t lanes, int c,
+                                                    T pw, T h2, int zero_start) {
   constexpr int V = kGroup<T>;
   const uint32_t per = gridDim.x >> 3;
   const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
@@ -110,7 +112,7 @@ __global__ __launch_bounds__(kThreads) void kb_relax_group(const StepPtrs<T>* ta
   const StepPtrs<T> p = tab[slot];
   SlotGroup<T, V> G;
   load_group<T, V>(as_global((const T*)p.x), as_global(p.rhs), as_global(p.screen), g, c,
-                   (lb - slot * nbi) * kThreads + threadIdx.x, lanes, false, G,
+                   (lb - slot * nbi) * B + threadIdx.x, lanes, false, G,
                    start_zero(p, zero_start));
   T out[V];
   const uint32_t mask = group_relax<T, V>(G, g.n, pw, h2, out);
@@ -131,18 +133,17 @@ __device__ __forceinline__ void residual_group(gptr<const T> X, gptr<const T> R,
   store_group<T, TO, V>(out + G.own, v, G.in ? (1u << V) - 1 : 0u);
 }
 
-template <class T>
-__global__ __launch_bounds__(kThreads) void kb_residual_group(const StepPtrs<T>* tab,
-                                                              uint32_t count, uint32_t nbi, Geom g,
-                                                              uint32_t lanes, T pw, T h2) {
+template <class T, int B>
+__global__ __launch_bounds__(B) void kb_residual_group(const StepPtrs<T>* tab, uint32_t count,
+                                                       uint32_t nbi, Geom g, uint32_t lanes, T pw,
+                                                       T h2) {
   const uint32_t per = gridDim.x >> 3;
   const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
   const uint32_t slot = lb / nbi;
   if (slot >= count) return;
   const StepPtrs<T> p = tab[slot];
   residual_group<T, T>(as_global((const T*)p.x), as_global(p.rhs), as_global(p.screen),
-                       as_global(p.res), g, (lb - slot * nbi) * kThreads + threadIdx.x, lanes, pw,
-                       h2);
+                       as_global(p.res), g, (lb - slot * nbi) * B + threadIdx.x, lanes, pw, h2);
 }
 
 // restrict the residual into the next-coarser rhs and zero the next-coarser x
@@ -239,9 +240,10 @@ __global__ __launch_bounds__(kThreads) void kb_defect(const TopPtrs* tab, uint32
                         (lb - slot * nbi) * kSpan + threadIdx.x, pw, h2);
 }
 
-__global__ __launch_bounds__(kThreads) void kb_defect_group(const TopPtrs* tab, uint32_t count,
-                                                            uint32_t nbi, Geom g, uint32_t lanes,
-                                                            double pw, double h2) {
+template <int B>
+__global__ __launch_bounds__(B) void kb_defect_group(const TopPtrs* tab, uint32_t count,
+                                                     uint32_t nbi, Geom g, uint32_t lanes,
+                                                     double pw, double h2) {
   const uint32_t per = gridDim.x >> 3;
   const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
   const uint32_t slot = lb / nbi;
@@ -249,7 +251,7 @@ __global__ __launch_bounds__(kThreads) void kb_defect_group(const TopPtrs* tab, 
   const TopPtrs p = tab[slot];
   residual_group<double, float>(as_global((const double*)p.x64), as_global(p.rhs64),
                                 as_global(p.screen64), as_global(p.r32), g,
-                                (lb - slot * nbi) * kThreads + threadIdx.x, lanes, pw, h2);
+                                (lb - slot * nbi) * B + threadIdx.x, lanes, pw, h2);
 }
 
 __global__ __launch_bounds__(kThreads) void kb_correct(const TopPtrs* tab, uint32_t count,
@@ -460,6 +462,16 @@ __global__ __launch_bounds__(kCoarseThreads) void kb_coarse(const CoarseEntry<T>
 
 inline unsigned padded8(uint64_t blocks) { return (unsigned)((blocks + 7u) & ~7ull); }
 
+// launch KERNEL<..., B> with the tuned workgroup size B (one of kTuneCandidates)
+#define MPR_WITH_BLOCK(BLOCK, LAUNCH) \
+  switch (BLOCK) {                    \
+    case 64: LAUNCH(64); break;       \
+    case 128: LAUNCH(128); break;     \
+    case 512: LAUNCH(512); break;     \
+    case 1024: LAUNCH(1024); break;   \
+    default: LAUNCH(256); break;      \
+  }
+
 }  // namespace
 
 template <class T>
@@ -469,12 +481,16 @@ void launch_b_relax(const StepPtrs<T>* tab, uint32_t count, const Geom& g, doubl
   const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
   if (group_layout(g)) {
     const uint32_t lanes = g.C / (uint32_t)kGroup<T>;
-    const uint32_t nbi = blocks_for(lanes, kThreads);
+    const int block = tuned_block(kTuneRelax, sizeof(T) == 8, size_class(g));
+    const uint32_t nbi = blocks_for(lanes, (unsigned)block);
     for (int it = 0; it < iterations; ++it)
       for (int c = 0; c < 2; ++c) {
-        hipLaunchKernelGGL(kb_relax_group<T>, dim3(padded8((uint64_t)nbi * count)),
-                           dim3(kThreads), 0, s, tab, count, nbi, g, lanes, c, (T)pw, h2,
-                           it == 0 && c == 0 ? zero_start : (int)kZeroNone);
+        const int zs = it == 0 && c == 0 ? zero_start : (int)kZeroNone;
+#define MPR_LAUNCH_RELAX(B)                                                                 \
+  hipLaunchKernelGGL((kb_relax_group<T, B>), dim3(padded8((uint64_t)nbi * count)), dim3(B), \
+                     0, s, tab, count, nbi, g, lanes, c, (T)pw, h2, zs)
+        MPR_WITH_BLOCK(block, MPR_LAUNCH_RELAX)
+#undef MPR_LAUNCH_RELAX
         MPR_LAUNCH_CHECK();
       }
     return;
@@ -496,9 +512,13 @@ void launch_b_residual(const StepPtrs<T>* tab, uint32_t count, const Geom& g, do
   const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
   if (group_layout(g)) {
     const uint32_t lanes = g.C / (uint32_t)kGroup<T>;
-    const uint32_t nbi = blocks_for(2ull * lanes, kThreads);
-    hipLaunchKernelGGL(kb_residual_group<T>, dim3(padded8((uint64_t)nbi * count)),
-                       dim3(kThreads), 0, s, tab, count, nbi, g, lanes, (T)pw, h2);
+    const int block = tuned_block(kTuneResidual, sizeof(T) == 8, size_class(g));
+    const uint32_t nbi = blocks_for(2ull * lanes, (unsigned)block);
+#define MPR_LAUNCH_RESIDUAL(B)                                                                 \
+  hipLaunchKernelGGL((kb_residual_group<T, B>), dim3(padded8((uint64_t)nbi * count)), dim3(B), \
+                     0, s, tab, count, nbi, g, lanes, (T)pw, h2)
+    MPR_WITH_BLOCK(block, MPR_LAUNCH_RESIDUAL)
+#undef MPR_LAUNCH_RESIDUAL
     MPR_LAUNCH_CHECK();
     return;
   }
@@ -559,9 +579,13 @@ void launch_b_defect(const TopPtrs* tab, uint32_t count, const Geom& g, double p
   const double h2 = (double)(g.n - 1) * (double)(g.n - 1);
   if (group_layout(g)) {
     const uint32_t lanes = g.C / (uint32_t)kGroup<double>;
-    const uint32_t nbi = blocks_for(2ull * lanes, kThreads);
-    hipLaunchKernelGGL(kb_defect_group, dim3(padded8((uint64_t)nbi * count)), dim3(kThreads), 0,
-                       s, tab, count, nbi, g, lanes, pw, h2);
+    const int block = tuned_block(kTuneDefect, true, size_class(g));
+    const uint32_t nbi = blocks_for(2ull * lanes, (unsigned)block);
+#define MPR_LAUNCH_DEFECT(B)                                                                 \
+  hipLaunchKernelGGL((kb_defect_group<B>), dim3(padded8((uint64_t)nbi * count)), dim3(B), 0, \
+                     s, tab, count, nbi, g, lanes, pw, h2)
+    MPR_WITH_BLOCK(block, MPR_LAUNCH_DEFECT)
+#undef MPR_LAUNCH_DEFECT
     MPR_LAUNCH_CHECK();
     return;
   }

@@ -4,8 +4,8 @@ The reference reconstructs one cloud per call. Here a batch of jobs (one per clo
 depths) runs its V-cycles together: every V-cycle step is one kernel launch over all jobs
 that have a grid of that size, and the whole batched run can be recorded once as a HIP
 graph and replayed with a single launch. Each cloud's arithmetic is exactly what its solo
-reconstruct() performs, so results agree with solo runs up to the reordering of the
-splat's FP64 atomic additions.
+reconstruct() performs (every stage is deterministic), so results are bit-identical to solo
+runs.
 """
 
 from __future__ import annotations

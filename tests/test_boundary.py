@@ -169,3 +169,20 @@ def test_server_job_capacity_rounds_to_powers_of_two():
     assert job_capacity(0) == 16384 and job_capacity(3000) == 16384
     assert job_capacity(16384) == 16384 and job_capacity(16385) == 32768
     assert job_capacity(500_000) == 524288
+
+
+def test_workgroup_size_table_without_a_device():
+    from mojo_poissonrecon import tune
+    from mojo_poissonrecon._lib import DeviceError
+    assert tune.tuned_block("relax", "fp32", 8) == 256
+    tune.set_tuned_block("relax", "fp32", 8, 512)
+    try:
+        assert tune.tuned_block("relax", "fp32", 8) == 512
+        assert tune.tuned_block("relax", "fp64", 8) == 256
+    finally:
+        tune.set_tuned_block("relax", "fp32", 8, 256)
+    for bad in (0, 32, 200, 2048):
+        with pytest.raises(DeviceError):
+            tune.set_tuned_block("residual", "fp64", 6, bad)
+    with pytest.raises(DeviceError):
+        tune.tuned_block("defect", "fp64", 11)

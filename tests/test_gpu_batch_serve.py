@@ -153,3 +153,25 @@ def test_server_admission_respects_memory_budget():
     done = server.drain()
     assert len(done) == 6 and server.stats.peak_bytes <= 2.5 * fp
     server.close()
+
+
+def test_workgroup_sizes_do_not_change_results():
+    from mojo_poissonrecon import tune
+    p, n = shapes.torus_workload(20000, seed=5)
+    saved = {(k, t, s): tune.tuned_block(k, t, s)
+             for k in tune.KERNELS for t in ("fp32", "fp64") for s in range(5, 8)}
+    try:
+        base = {prec: mpr.reconstruct(p, n, depth=7, precision=prec) for prec in ("fp64", "mixed")}
+        for block in (64, 1024):
+            for key in saved:
+                tune.set_tuned_block(*key, block)
+            for prec, m in base.items():
+                identical_mesh(mpr.reconstruct(p, n, depth=7, precision=prec), m)
+        chosen = tune.autotune(max_depth=7, reps=3)
+        assert set(chosen.values()) <= set(tune.CANDIDATES)
+        assert len(chosen) == 3 * 2 + 3 * 2 + 3  # relax, residual x2 precisions; defect fp64
+        for prec, m in base.items():
+            identical_mesh(mpr.reconstruct(p, n, depth=7, precision=prec), m)
+    finally:
+        for key, b in saved.items():
+            tune.set_tuned_block(*key, b)
