@@ -112,12 +112,12 @@ enum {
   MPR_STAGE_NORMALIZE = 1,  /* points are world space: bbox normalise on device (core.py:377-385) */
   MPR_STAGE_SOLVE = 2,      /* splat, assemble, multigrid, residual, iso value */
   MPR_STAGE_EXTRACT = 4,    /* dual-grid mesh of the solution at the iso value */
-  MPR_STAGE_GRIDS = 8,      /* also write natural-layout field, screen, rhs for fetching */
+  MPR_STAGE_GRIDS = 8,      /* also write natural-layout field, density, screen, rhs for fetching */
   MPR_STAGE_WORLD = 16      /* write mesh vertices in world space (core.py:425) */
 };
 
 typedef struct mpr_params {
-  double point_weight;     /* alpha, default 4.0 */
+  double point_weight;     /* alpha, default 4.0; finite and >= 0 (-0.0 runs as +0.0) */
   double samples_per_node; /* default 1.5 */
   double scale;            /* bbox scale for MPR_STAGE_NORMALIZE, default 1.1 */
   int32_t cycles;          /* V-cycles, default 4 */
@@ -155,7 +155,9 @@ int mpr_job_wait(mpr_job *job, mpr_result *result);
 /* Non-blocking completion test: 1 done, 0 running, <0 error. */
 int mpr_job_query(mpr_job *job);
 int mpr_job_fetch_mesh(mpr_job *job, double *vertices, double *vertex_density, int64_t *faces);
-/* Any pointer may be NULL. field, screen and rhs require MPR_STAGE_GRIDS on the run. */
+/* Any pointer may be NULL. Each grid requires MPR_STAGE_GRIDS on the run (without it the
+ * assembly writes only the solver-layout grids near the splat and leaves the density as
+ * splatted). */
 int mpr_job_fetch_grids(mpr_job *job, double *field, double *density, double *screen,
                         double *rhs);
 /* Breadth-first octree of the job's (unit-cube) points (core.py:86-127); synchronises and

@@ -68,7 +68,7 @@ __global__ __launch_bounds__(kThreads) void k_cells_chunk_counts(const double* f
 }
 
 // vertex of an active cell: mean of the edge roots in x-, y-, z-axis order (capi.mojo:958-1038)
-__device__ __forceinline__ void dual_vertex(const FieldView& f, const double* dens, int x, int y,
+__device__ __forceinline__ void dual_vertex(const FieldView& f, const DensityView& dens, int x, int y,
                                             int z, int n, double iso, double out[3],
                                             double& vdens) {
   double v[8];
@@ -112,7 +112,7 @@ __device__ __forceinline__ void dual_vertex(const FieldView& f, const double* de
   double ds = 0.0;
 #pragma unroll
   for (int c = 0; c < 8; ++c)
-    ds += dens[nat_index(x + (c & 1), y + ((c >> 1) & 1), z + ((c >> 2) & 1), n)];
+    ds += dens.at(nat_index(x + (c & 1), y + ((c >> 1) & 1), z + ((c >> 2) & 1), n));
   vdens = ds * 0.125;
 }
 
@@ -149,7 +149,7 @@ __global__ __launch_bounds__(kThreads) void k_cells_emit(const double* f, const 
           FieldView fv;
           fv.nat = f;
           fv.g.n = n;
-          dual_vertex(fv, dens, x, y, z, n, iso, v, d);
+          dual_vertex(fv, DensityView{dens, nullptr, 1.0}, x, y, z, n, iso, v, d);
           if (xform) {
             for (int a = 0; a < 3; ++a) v[a] = v[a] * xform[3] + xform[a];
           }
@@ -379,7 +379,7 @@ __global__ __launch_bounds__(kThreads) void k_bit_cells_index(BitGeom G, const u
 }
 
 // one lane per vertex: the dual vertex of its cell (capi.mojo:958-1038)
-__global__ __launch_bounds__(kThreads) void k_bit_vertices(FieldView f, const double* dens,
+__global__ __launch_bounds__(kThreads) void k_bit_vertices(FieldView f, DensityView dens,
                                                            const double* isop, BitGeom G,
                                                            const uint32_t* vcell,
                                                            const uint32_t* count, double* verts,
@@ -614,7 +614,7 @@ BitSurface bit_surface_sizes(int n) {
   return b;
 }
 
-void launch_bit_extract(const FieldView& f, const double* dens, int n, const double* iso,
+void launch_bit_extract(const FieldView& f, const DensityView& dens, int n, const double* iso,
                         const BitSurface& b, uint32_t* counts, double* verts, double* vdens,
                         uint64_t vcap, int64_t* faces, uint64_t fcap, const double* xform,
                         hipStream_t s) {

@@ -241,6 +241,33 @@ __device__ __forceinline__ double splat_value(const double* v, const uint32_t* t
   return v[i];
 }
 
+// 32 node bits starting at node b of the union of two bit sets (nodes outside [0, 32 words)
+// read as unset)
+__device__ __forceinline__ uint32_t bits_at(const uint32_t* a, const uint32_t* b, int64_t bit,
+                                            int64_t words) {
+  const int64_t q = bit >> 5;  // arithmetic shift: floor for negative offsets
+  const uint32_t r = (uint32_t)(bit & 31);
+  const uint32_t lo = (q >= 0 && q < words) ? (a[q] | b[q]) : 0u;
+  if (!r) return lo;
+  const uint32_t hi = (q + 1 >= 0 && q + 1 < words) ? (a[q + 1] | b[q + 1]) : 0u;
+  return (lo >> r) | (hi << (32u - r));
+}
+
+// need = nodes whose 7-point stencil (linear offsets 0, +-1, +-n, +-n^2) meets a node of
+// cur or prev. Linear offsets also reach across row and plane ends, which only adds nodes
+// (a superset is harmless: an added node is written with the value the assembly computes).
+__global__ __launch_bounds__(kThreads) void k_need_mask(const uint32_t* cur, const uint32_t* prev,
+                                                        int64_t words, int64_t sy, int64_t sz,
+                                                        uint32_t* need) {
+  const int64_t w = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (w >= words) return;
+  const int64_t b = 32 * w;
+  need[w] = bits_at(cur, prev, b, words) | bits_at(cur, prev, b + 1, words) |
+            bits_at(cur, prev, b - 1, words) | bits_at(cur, prev, b + sy, words) |
+            bits_at(cur, prev, b - sy, words) | bits_at(cur, prev, b + sz, words) |
+            bits_at(cur, prev, b - sz, words);
+}
+
 __global__ __launch_bounds__(kThreads) void k_assemble(const double* vx, const double* vy,
                                                        const double* vz, const double* scr,
                                                        double* rhs_nat, double* rhs_split,
@@ -248,10 +275,15 @@ __global__ __launch_bounds__(kThreads) void k_assemble(const double* vx, const d
                                                        double spn, Geom g, FastDiv div_n,
                                                        double pw, double target,
                                                        const uint32_t* touched,
-                                                       double* scr_nat_out, float* scr_split32) {
+                                                       double* scr_nat_out, float* scr_split32,
+                                                       const uint32_t* need, uint32_t* prev) {
   const uint32_t t = logical_block() * kThreads + threadIdx.x;
   const int n = g.n;
   if (t >= (uint32_t)n * n * n) return;
+  // record this run's touched bits for the next run's need mask (this run's need mask was
+  // computed from the old ones by the previous kernel), then skip nodes outside it
+  if (prev && (t & 31u) == 0) prev[t >> 5] = touched[t >> 5];
+  if (need && !((need[t >> 5] >> (t & 31u)) & 1u)) return;
   uint32_t zy = div_n.div(t);
   const int x = (int)(t - zy * (uint32_t)n);
   const int z = (int)div_n.div(zy);
@@ -411,12 +443,30 @@ void launch_assemble(const double* vx, const double* vy, const double* vz,
                      const double* screen_nat, double* rhs_nat, double* rhs_split,
                      double* screen_split, double* density, double spn, const Geom& g,
                      double pw, double target, const uint32_t* touched, double* screen_nat_out,
-                     float* screen_split32, hipStream_t s) {
+                     float* screen_split32, const TouchedHistory* hist, hipStream_t s) {
   const uint64_t m = (uint64_t)g.n * g.n * g.n;
+  const uint32_t* need = nullptr;
+  uint32_t* prev = nullptr;
+  if (hist) {
+    if (!touched || !rhs_split) throw HipError{"assembly history needs the touched bits"};
+    prev = hist->prev;
+  }
+  if (hist && !hist->write_all) {
+    // only the split outputs keep the zero-input value at untouched nodes; the
+    // natural-layout outputs are written by the dense assembly
+    if (rhs_nat || density || screen_nat_out)
+      throw HipError{"sparse assembly writes the split outputs only"};
+    const int64_t words = (int64_t)((m + 31) / 32);
+    hipLaunchKernelGGL(k_need_mask, dim3(blocks_for((uint64_t)words, kThreads)), dim3(kThreads),
+                       0, s, touched, hist->prev, words, (int64_t)g.n, (int64_t)g.n * g.n,
+                       hist->need);
+    MPR_LAUNCH_CHECK();
+    need = hist->need;
+  }
   const unsigned nb = xcd_grid(m, kThreads);
   hipLaunchKernelGGL(k_assemble, dim3(nb), dim3(kThreads), 0, s, vx, vy, vz, screen_nat,
                      rhs_nat, rhs_split, screen_split, density, spn, g, FastDiv((uint32_t)g.n),
-                     pw, target, touched, screen_nat_out, screen_split32);
+                     pw, target, touched, screen_nat_out, screen_split32, need, prev);
   MPR_LAUNCH_CHECK();
 }
 

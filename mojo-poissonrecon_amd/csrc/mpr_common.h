@@ -125,6 +125,21 @@ struct FieldView {
   }
 };
 
+// The density grid (natural layout) as the extraction reads it. With `touched` it is the
+// splat's raw grid: nodes whose bit is unset read as zero (the grid was not zero-filled) and
+// every value is divided by samples_per_node, the exact value of core.py:255's in-place
+// `density /= samples_per_node`. Without it, the values are read as stored.
+struct DensityView {
+  const double* p = nullptr;
+  const uint32_t* touched = nullptr;
+  double spn = 1.0;
+  MPR_HD double at(uint64_t i) const {
+    if (!touched) return p[i];
+    const double raw = ((touched[i >> 5] >> (i & 31u)) & 1u) ? p[i] : 0.0;
+    return raw / spn;
+  }
+};
+
 inline FieldView natural_field(const double* f, int n) {
   FieldView v;
   v.nat = f;

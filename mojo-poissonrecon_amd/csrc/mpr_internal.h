@@ -53,11 +53,26 @@ void launch_add_correction(double* X, const float* E, uint64_t count, hipStream_
 // screen_split (+ its float copy) / screen_nat_out / density (/= spn, core.py:255) are
 // written when non-null.
 // touched: the splat's node bits; unset nodes read as +0.0 (their grids were not filled).
+//
+// Sparse assembly (hist != null, write_all false; split outputs only, density left as
+// splatted): a node whose 7-point stencil holds no touched node gets
+// rhs = -(+0) + pw * (+0) * h^-2 * target = +0.0 and screen = +0.0 (pw >= +0, target >= 0),
+// the same in every run. So only nodes whose stencil meets this run's or the previous run's
+// touched nodes are written; every other node keeps the +0.0 an earlier run wrote.
+// `prev` holds the previous assembly's touched bits (all ones before the first, so that one
+// writes every node) and is updated in place by every assembly given `hist` (write_all
+// true: a dense assembly that still records its touched bits); `need` is scratch of the
+// same size.
+struct TouchedHistory {
+  uint32_t* prev;
+  uint32_t* need;
+  bool write_all;
+};
 void launch_assemble(const double* vx, const double* vy, const double* vz,
                      const double* screen_nat, double* rhs_nat, double* rhs_split,
                      double* screen_split, double* density, double spn, const Geom& g,
                      double pw, double target, const uint32_t* touched, double* screen_nat_out,
-                     float* screen_split32, hipStream_t s);
+                     float* screen_split32, const TouchedHistory* hist, hipStream_t s);
 // ---- point kernels (point_kernels.hip) --------------------------------------------------
 int bbox_partial_count(uint32_t P);
 // core.py:377-385 on the device: params = [origin x, y, z, width], unit = (p - origin)/width
@@ -129,7 +144,7 @@ BitSurface bit_surface_sizes(int n);
 // Dual-grid extraction of the device pipeline (capi.mojo:839-1129, same vertex indices and
 // face order): counts[0] = vertices, counts[1] = quads; vertices past vcap and faces past
 // fcap are counted but not written. xform: optional [origin x, y, z, width] world transform.
-void launch_bit_extract(const FieldView& f, const double* dens, int n, const double* iso,
+void launch_bit_extract(const FieldView& f, const DensityView& dens, int n, const double* iso,
                         const BitSurface& b, uint32_t* counts, double* verts, double* vdens,
                         uint64_t vcap, int64_t* faces, uint64_t fcap, const double* xform,
                         hipStream_t s);

@@ -103,13 +103,19 @@ struct Job {
   BitSurface surf;
   SplatScratch splat;
   uint32_t* touched = nullptr;  // splat node bits, ceil(n^3 / 32) words
+  // the previous assembly's touched bits and the sparse assembly's need mask (same size)
+  uint32_t *prev_touched = nullptr, *need_bits = nullptr;
+  // how the density grid holds the last solve's values: raw splat sums read through
+  // `touched` and divided by dens_spn, or normalised in place (runs with MPR_STAGE_GRIDS)
+  bool dens_raw = false;
+  double dens_spn = 1.0;
   double *verts = nullptr, *vdens = nullptr;
   int64_t* faces = nullptr;
   uint64_t vcap = 0, fcap = 0;
   std::vector<void*> mesh_bufs;  // grown extraction outputs (owned separately)
   // host results
   HostScalars* host = nullptr;
-  mpr_params last{};
+  mpr_params last{}, captured{};
   bool ran = false, grids_kept = false;
   // graph
   hipGraph_t graph = nullptr;
@@ -135,6 +141,31 @@ struct Job {
   void setup(const mpr_params& p, hipStream_t s);
   void finish(const mpr_params& p, hipStream_t s);
   void enqueue_extract(bool world, hipStream_t s);
+  // the parameters of the work last enqueued or launched on this job (eagerly or as a graph)
+  void adopt(const mpr_params& p) {
+    last = p;
+    grids_kept = (p.stages & MPR_STAGE_GRIDS) != 0;
+    if (p.stages & MPR_STAGE_SOLVE) {
+      dens_raw = !grids_kept;
+      dens_spn = p.samples_per_node;
+    }
+  }
+  struct State {
+    mpr_params last;
+    bool grids_kept, dens_raw;
+    double dens_spn;
+  };
+  State state() const { return {last, grids_kept, dens_raw, dens_spn}; }
+  void restore(const State& st) {
+    last = st.last;
+    grids_kept = st.grids_kept;
+    dens_raw = st.dens_raw;
+    dens_spn = st.dens_spn;
+  }
+  DensityView density_view() const {
+    return dens_raw ? DensityView{density, touched, dens_spn}
+                    : DensityView{density, nullptr, 1.0};
+  }
   void readback(hipStream_t s);
   void enqueue(const mpr_params& p);
   // after the stream finished: if the mesh outgrew its buffers, grow them to the exact
@@ -382,6 +413,8 @@ void Job::plan(DevArena& a) {
   bbox_part = a.alloc<double>(6ull * bbox_partial_count(std::max<uint32_t>(1u, max_points)));
   iso_part = a.alloc<double>(2ull * iso_partial_count(std::max<uint32_t>(1u, max_points)));
   touched = a.alloc<uint32_t>((M + 31) / 32);
+  prev_touched = a.alloc<uint32_t>((M + 31) / 32);
+  need_bits = a.alloc<uint32_t>((M + 31) / 32);
   splat = splat_scratch_sizes(std::max<uint32_t>(1u, max_points));
   splat.keys = a.alloc<uint64_t>(splat.pairs);
   splat.ktmp = a.alloc<uint64_t>(splat.pairs);
@@ -438,9 +471,8 @@ Job::~Job() {
 }
 
 void Job::enqueue_extract(bool world, hipStream_t s) {
-  launch_bit_extract(split_field(L64[0].x, geoms[0]), density, n, d_iso, surf, d_counts, verts,
-                     vdens, vcap, faces, fcap,
-                     world ? d_params : nullptr, s);
+  launch_bit_extract(split_field(L64[0].x, geoms[0]), density_view(), n, d_iso, surf, d_counts,
+                     verts, vdens, vcap, faces, fcap, world ? d_params : nullptr, s);
 }
 
 // normalise, splat, assemble, screening pyramid and initial guess (core.py:229-273)
@@ -450,17 +482,22 @@ void Job::setup(const mpr_params& p, hipStream_t s) {
   if (!(p.stages & MPR_STAGE_SOLVE)) return;
   const double* upts = unit_points(p);
   // the splat grids are not zero-filled: the splat marks the nodes it writes and the
-  // assembly reads every other node as zero (and writes the full density grid)
+  // assembly and the extraction read every other node as zero
   launch_fill<uint32_t>(touched, 0u, (M + 31) / 32, s);
   launch_fill<unsigned long long>(d_accepted, 0ull, 1, s);
   launch_splat(upts, nrm, P, n, p.confidence, density, vx, vy, vz, screen, p.samples_per_node,
                d_accepted, splat, touched, s);
   const bool grids = (p.stages & MPR_STAGE_GRIDS) != 0;
   const bool mixed = precision == MPR_PRECISION_MIXED;
-  // the FP32 screening weights of the mixed V-cycle are written beside the FP64 ones
-  launch_assemble(vx, vy, vz, screen, grids ? rhs : nullptr, L64[0].rhs, L64[0].screen, density,
-                  p.samples_per_node, geoms[0], p.point_weight, 0.5, touched,
-                  grids ? screen : nullptr, mixed ? L32[0].screen : nullptr, s);
+  // Without MPR_STAGE_GRIDS the assembly writes only the nodes near this run's and the
+  // previous run's splat (the rest of the right-hand side and screening weights stay +0.0)
+  // and leaves the density raw; with it, every grid is written in natural layout too. The
+  // FP32 screening weights of the mixed V-cycle are written beside the FP64 ones.
+  const TouchedHistory hist{prev_touched, need_bits, grids};
+  launch_assemble(vx, vy, vz, screen, grids ? rhs : nullptr, L64[0].rhs, L64[0].screen,
+                  grids ? density : nullptr, p.samples_per_node, geoms[0], p.point_weight, 0.5,
+                  touched, grids ? screen : nullptr, mixed ? L32[0].screen : nullptr, &hist, s);
+  adopt(p);  // the extraction enqueued next reads the density as this assembly leaves it
   for (size_t l = 1; l < L64.size(); ++l)
     launch_restrict<double>(L64[l - 1].screen, geoms[l - 1], L64[l].screen, geoms[l], 4.0, s,
                             mixed ? L32[l].screen : nullptr);
@@ -538,7 +575,18 @@ using namespace mpr;
 
 static bool valid_params(const mpr_params* p) {
   if (!p) return false;
-  return p->cycles >= 0 && p->pre_smooth >= 0 && p->post_smooth >= 0 && p->samples_per_node > 0;
+  // core.py:213-220: finite, point_weight >= 0, samples_per_node > 0
+  return p->cycles >= 0 && p->pre_smooth >= 0 && p->post_smooth >= 0 &&
+         std::isfinite(p->samples_per_node) && p->samples_per_node > 0 &&
+         std::isfinite(p->point_weight) && p->point_weight >= 0.0;
+}
+
+// A point weight of -0.0 runs as +0.0 (only the sign of zero right-hand-side values could
+// differ), so that every assembly gives the same +0.0 at nodes away from the splat.
+static mpr_params canonical(const mpr_params& p) {
+  mpr_params q = p;
+  q.point_weight = p.point_weight + 0.0;
+  return q;
 }
 
 extern "C" {
@@ -604,6 +652,9 @@ int mpr_job_create(int32_t depth, int64_t max_points, int32_t precision, mpr_job
   MPR_HIP(hipMemcpyAsync(j->d_params, ident, sizeof(ident), hipMemcpyHostToDevice, j->stream));
   MPR_HIP(hipMemsetAsync(j->d_counts, 0, 2 * sizeof(uint32_t), j->stream));
   MPR_HIP(hipMemsetAsync(j->d_accepted, 0, sizeof(unsigned long long), j->stream));
+  // no assembly yet: the first one writes every node
+  const uint64_t words = ((uint64_t)j->n * j->n * j->n + 31) / 32;
+  launch_fill<uint32_t>(j->prev_touched, 0xffffffffu, words, j->stream);
   MPR_HIP(hipStreamSynchronize(j->stream));
   *out = j;
   return 0;
@@ -658,9 +709,9 @@ int mpr_job_run(mpr_job* job, const mpr_params* params) {
     return -1;
   }
   MPR_GUARD_BEGIN
-  job->last = *params;
-  job->grids_kept = (params->stages & MPR_STAGE_GRIDS) != 0;
-  job->enqueue(*params);
+  const mpr_params p = canonical(*params);
+  job->adopt(p);
+  job->enqueue(p);
   job->ran = true;
   return 0;
   MPR_GUARD_END
@@ -681,10 +732,13 @@ int mpr_job_capture(mpr_job* job, const mpr_params* params) {
     job->graph = nullptr;
   }
   MPR_HIP(hipStreamSynchronize(job->stream));
+  const mpr_params p = canonical(*params);
+  const Job::State before = job->state();
   MPR_HIP(hipStreamBeginCapture(job->stream, hipStreamCaptureModeThreadLocal));
   try {
-    job->enqueue(*params);
+    job->enqueue(p);
   } catch (...) {
+    job->restore(before);
     hipGraph_t g = nullptr;
     (void)hipStreamEndCapture(job->stream, &g);
     if (g) (void)hipGraphDestroy(g);
@@ -692,8 +746,9 @@ int mpr_job_capture(mpr_job* job, const mpr_params* params) {
   }
   MPR_HIP(hipStreamEndCapture(job->stream, &job->graph));
   MPR_HIP(hipGraphInstantiate(&job->exec, job->graph, nullptr, nullptr, 0));
-  job->last = *params;
-  job->grids_kept = (params->stages & MPR_STAGE_GRIDS) != 0;
+  // recording ran nothing: the job's state is still that of its last executed work
+  job->captured = p;
+  job->restore(before);
   return 0;
   MPR_GUARD_END
 }
@@ -705,6 +760,7 @@ int mpr_job_launch(mpr_job* job) {
   }
   MPR_GUARD_BEGIN
   MPR_HIP(hipGraphLaunch(job->exec, job->stream));
+  job->adopt(job->captured);
   job->ran = true;
   return 0;
   MPR_GUARD_END
@@ -750,8 +806,8 @@ int mpr_job_fetch_mesh(mpr_job* job, double* vertices, double* vertex_density, i
 
 int mpr_job_fetch_grids(mpr_job* job, double* field, double* density, double* screen, double* rhs) {
   if (!job) return -1;
-  if ((field || rhs || screen) && !job->grids_kept) {
-    set_error("field, screen and rhs are kept only by runs with MPR_STAGE_GRIDS");
+  if ((field || density || rhs || screen) && !job->grids_kept) {
+    set_error("field, density, screen and rhs are kept only by runs with MPR_STAGE_GRIDS");
     return -1;
   }
   MPR_GUARD_BEGIN
@@ -909,13 +965,13 @@ int mpr_batch_run(mpr_batch* b, const mpr_params* params) {
   }
   MPR_GUARD_BEGIN
   for (Job* j : b->jobs) MPR_HIP(hipStreamSynchronize(j->stream));  // order after uploads
-  b->last.assign(params, params + b->jobs.size());
+  b->last.clear();
+  for (size_t i = 0; i < b->jobs.size(); ++i) b->last.push_back(canonical(params[i]));
   for (size_t i = 0; i < b->jobs.size(); ++i) {
-    b->jobs[i]->last = params[i];
-    b->jobs[i]->grids_kept = (params[i].stages & MPR_STAGE_GRIDS) != 0;
+    b->jobs[i]->adopt(b->last[i]);
     b->jobs[i]->ran = true;
   }
-  b->enqueue(params);
+  b->enqueue(b->last.data());
   return 0;
   MPR_GUARD_END
 }
@@ -929,18 +985,29 @@ int mpr_batch_capture(mpr_batch* b, const mpr_params* params) {
   b->drop_graph();
   for (Job* j : b->jobs) MPR_HIP(hipStreamSynchronize(j->stream));
   MPR_HIP(hipStreamSynchronize(b->stream));
+  std::vector<mpr_params> cp;
+  std::vector<Job::State> before;
+  for (size_t i = 0; i < b->jobs.size(); ++i) {
+    cp.push_back(canonical(params[i]));
+    before.push_back(b->jobs[i]->state());
+  }
+  auto restore = [&] {  // recording ran nothing: each job keeps its executed work's state
+    for (size_t i = 0; i < b->jobs.size(); ++i) b->jobs[i]->restore(before[i]);
+  };
   MPR_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   try {
-    b->enqueue(params);
+    b->enqueue(cp.data());
   } catch (...) {
+    restore();
     hipGraph_t g = nullptr;
     (void)hipStreamEndCapture(b->stream, &g);
     if (g) (void)hipGraphDestroy(g);
     throw;
   }
+  restore();
   MPR_HIP(hipStreamEndCapture(b->stream, &b->graph));
   MPR_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-  b->last.assign(params, params + b->jobs.size());
+  b->last = cp;
   b->generations.clear();
   for (Job* j : b->jobs) b->generations.push_back(j->mesh_generation);
   return 0;
@@ -962,8 +1029,7 @@ int mpr_batch_launch(mpr_batch* b) {
   }
   for (Job* j : b->jobs) MPR_HIP(hipStreamSynchronize(j->stream));
   for (size_t i = 0; i < b->jobs.size(); ++i) {
-    b->jobs[i]->last = b->last[i];
-    b->jobs[i]->grids_kept = (b->last[i].stages & MPR_STAGE_GRIDS) != 0;
+    b->jobs[i]->adopt(b->last[i]);
     b->jobs[i]->ran = true;
   }
   MPR_HIP(hipGraphLaunch(b->exec, b->stream));
@@ -1128,8 +1194,8 @@ int mpr_extract(const double* field, const double* density, int64_t n, double is
       launch_fill<double>(d, 0.0, M, s);
     MPR_HIP(hipMemcpyAsync(isod, &iso, 8, hipMemcpyHostToDevice, s));
     // count (nothing fits a zero capacity), then emit into exactly sized buffers
-    launch_bit_extract(natural_field(f, (int)n), d, (int)n, isod, surf, counts, nullptr, nullptr,
-                       0, nullptr, 0, nullptr, s);
+    launch_bit_extract(natural_field(f, (int)n), DensityView{d, nullptr, 1.0}, (int)n, isod,
+                       surf, counts, nullptr, nullptr, 0, nullptr, 0, nullptr, s);
     uint32_t hc[2] = {0, 0};
     MPR_HIP(hipMemcpyAsync(hc, counts, 8, hipMemcpyDeviceToHost, s));
     MPR_HIP(hipStreamSynchronize(s));
@@ -1140,9 +1206,8 @@ int mpr_extract(const double* field, const double* density, int64_t n, double is
     m->verts = m->arena.alloc<double>(3 * std::max<uint64_t>(1, m->nv));
     m->vdens = m->arena.alloc<double>(std::max<uint64_t>(1, m->nv));
     m->faces = m->arena.alloc<int64_t>(6 * std::max<uint64_t>(1, m->nq));
-    launch_bit_extract(natural_field(f, (int)n), d, (int)n, isod, surf, counts, m->verts,
-                       m->vdens, m->nv, m->faces,
-                       2 * m->nq, nullptr, s);
+    launch_bit_extract(natural_field(f, (int)n), DensityView{d, nullptr, 1.0}, (int)n, isod,
+                       surf, counts, m->verts, m->vdens, m->nv, m->faces, 2 * m->nq, nullptr, s);
     MPR_HIP(hipStreamSynchronize(s));
     scratch.release();
     if (vertex_count) *vertex_count = (int64_t)m->nv;

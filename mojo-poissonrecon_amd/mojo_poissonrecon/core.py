@@ -190,13 +190,11 @@ class Job:
               "mpr_job_fetch_mesh")
         return vertices, faces, density
 
-    def fetch_grids(self, kept: bool = True):
-        """[field, density, screen, rhs]; field, screen and rhs (None unless `kept`) need a
-        run with STAGE_GRIDS."""
+    def fetch_grids(self):
+        """[field, density, screen, rhs] of the last run, which needs STAGE_GRIDS."""
         n = (1 << self.depth) + 1
-        grids = [np.empty((n, n, n), np.float64) if i == 1 or kept else None for i in range(4)]
-        check(self._lib.mpr_job_fetch_grids(self.handle,
-                                            *(ptr(g) if g is not None else None for g in grids)),
+        grids = [np.empty((n, n, n), np.float64) for _ in range(4)]
+        check(self._lib.mpr_job_fetch_grids(self.handle, *(ptr(g) for g in grids)),
               "mpr_job_fetch_grids")
         return grids
 

@@ -219,3 +219,98 @@ def test_job_mesh_overflow_regrows_and_matches_extraction():
     assert np.array_equal(v, mesh.vertices)
     assert np.array_equal(f, mesh.faces)
     assert np.array_equal(d, mesh.density)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_sparse_assembly_over_a_sequence_equals_fresh_dense_runs(precision):
+    # Without STAGE_GRIDS a job's assembly writes only the nodes whose stencil meets this or
+    # the previous run's splat (every other node keeps the +0.0 an earlier run wrote) and the
+    # extraction reads the raw splat density through the splat's node bits. A sequence of
+    # different clouds on one job, with a dense STAGE_GRIDS run, an empty cloud and a graph
+    # replay in between, must give bit for bit what a fresh job's dense run gives.
+    from mojo_poissonrecon import _lib
+    from mojo_poissonrecon.core import Job
+    depth = 6
+    base = _lib.STAGE_SOLVE | _lib.STAGE_EXTRACT
+    clouds = []
+    for maker, count, seed in ((shapes.sphere, 4000, 1), (shapes.torus, 6000, 2),
+                               (shapes.superquadric, 5000, 3)):
+        p, n = maker(count, seed=seed)
+        clouds.append((oracle.normalize(p, 1.1)[0], n))
+    empty = (np.zeros((0, 3)), np.zeros((0, 3)))
+    seq = [(clouds[0], 0), (clouds[1], 0), (empty, 0), (clouds[2], _lib.STAGE_GRIDS),
+           (clouds[1], 0), (clouds[0], 0)]
+
+    def dense_fresh(cloud, params_grids):
+        j = Job(depth, 6000, precision)
+        try:
+            j.set_points(*cloud)
+            j.run(params_grids)
+            r = j.wait()
+            return r, j.fetch_mesh(r)
+        finally:
+            j.close()
+
+    job = Job(depth, 6000, precision)
+    try:
+        for cloud, extra in seq:
+            job.set_points(*cloud)
+            job.run(Job.params(base | extra, confidence=True))
+            r = job.wait()
+            v, f, d = job.fetch_mesh(r)
+            rw, (vw, fw, dw) = dense_fresh(cloud, Job.params(base | _lib.STAGE_GRIDS,
+                                                             confidence=True))
+            assert r.iso_value == rw.iso_value and r.residual_norm == rw.residual_norm
+            assert np.array_equal(v, vw) and np.array_equal(f, fw) and np.array_equal(d, dw)
+            if not len(cloud[0]):
+                assert len(v) == 0
+        # a sparse graph recorded now, an eager dense run with other samples_per_node, then
+        # the replay: the replay's mesh is the sparse run's (its density divided by its own
+        # samples_per_node)
+        job.set_points(*clouds[2])
+        sparse = Job.params(base, samples_per_node=2.0)
+        job.run(sparse)
+        r0 = job.wait()
+        want = job.fetch_mesh(r0)
+        job.capture(sparse)
+        job.run(Job.params(base | _lib.STAGE_GRIDS, samples_per_node=1.0))
+        job.wait()
+        for _ in range(2):
+            job.launch()
+            r = job.wait()
+            got = job.fetch_mesh(r)
+            assert r.iso_value == r0.iso_value
+            assert all(np.array_equal(a, b) for a, b in zip(got, want))
+    finally:
+        job.close()
+
+
+def test_fetch_grids_needs_a_grids_run():
+    from mojo_poissonrecon import _lib
+    from mojo_poissonrecon.core import Job
+    p, n = shapes.sphere(2000, 4)
+    job = Job(5, 2000)
+    try:
+        job.set_points(oracle.normalize(p, 1.1)[0], n)
+        job.run(Job.params(_lib.STAGE_SOLVE))
+        job.wait()
+        with pytest.raises(_lib.DeviceError, match="STAGE_GRIDS"):
+            job.fetch_grids()
+        job.run(Job.params(_lib.STAGE_SOLVE | _lib.STAGE_GRIDS))
+        job.wait()
+        field, density, screen, rhs = job.fetch_grids()
+        assert density.sum() > 0 and np.isfinite(field).all()
+    finally:
+        job.close()
+
+
+def test_job_rejects_negative_or_non_finite_point_weight():
+    from mojo_poissonrecon import _lib
+    from mojo_poissonrecon.core import Job
+    job = Job(4, 16)
+    try:
+        for pw in (-1.0, float("nan"), float("inf")):
+            with pytest.raises(_lib.DeviceError):
+                job.run(Job.params(_lib.STAGE_SOLVE, point_weight=pw))
+    finally:
+        job.close()
