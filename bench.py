@@ -247,7 +247,8 @@ def time_kernels(jobs, reps: int) -> dict:
                             ("restrict_f64", 3, jobs["fp64"]), ("prolong_f64", 4, jobs["fp64"]),
                             ("defect_f64_f32", 6, jobs["mixed"]),
                             ("restrict_f32", 10, jobs["mixed"]), ("prolong_f32_c1", 11, jobs["mixed"]),
-                            ("restrict_f64_batched", 12, jobs["fp64"])):
+                            ("restrict_f64_batched", 12, jobs["fp64"]),
+                            ("defect_f64_f32_batched", 13, jobs["mixed"])):
         ms, nbytes = _lib.ctypes.c_double(), _lib.ctypes.c_double()
         _lib.check(_lib.lib().mpr_job_time_kernel(job.handle, kid, reps, _lib.ctypes.byref(ms),
                                                    _lib.ctypes.byref(nbytes)),

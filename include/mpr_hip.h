@@ -199,8 +199,10 @@ int mpr_batch_wait(mpr_batch *batch, mpr_result *results);
  * sweep, 1 FP32 colour sweep (mixed jobs), 2 FP64 residual, 3 FP64 restriction,
  * 4 FP64 prolongation, 5 FP32 residual, 6 FP64->FP32 defect residual; the batched V-cycle
  * kernels the pipeline itself launches: 7 FP32 colour sweep, 8 FP64 colour sweep, 9 FP32
- * residual, 10 FP32 restriction, 11 FP32 prolongation of colour 1, 12 FP64 restriction.
- * No reference counterpart (measurement hook). */
+ * residual, 10 FP32 restriction, 11 FP32 prolongation of colour 1, 12 FP64 restriction,
+ * 13 FP64->FP32 defect residual. Kernels 7, 8, 9 and 13 skip the right-hand-side and
+ * screening words that are +0.0 away from the splat (live-slot bits), and their byte
+ * counts include only the words they load. No reference counterpart (measurement hook). */
 int mpr_job_time_kernel(mpr_job *job, int32_t kernel, int32_t reps, double *ms_per_launch,
                         double *bytes_per_launch);
 

@@ -113,7 +113,7 @@ __global__ __launch_bounds__(B) void kb_relax_group(const StepPtrs<T>* tab, uint
   SlotGroup<T, V> G;
   load_group<T, V>(as_global((const T*)p.x), as_global(p.rhs), as_global(p.screen), g, c,
                    (lb - slot * nbi) * B + threadIdx.x, lanes, false, G,
-                   start_zero(p, zero_start));
+                   start_zero(p, zero_start), LiveMask{p.live, p.live_rhs != 0});
   T out[V];
   const uint32_t mask = group_relax<T, V>(G, g.n, pw, h2, out);
   store_group<T, T, V>(as_global(p.x) + G.own, out, mask);
@@ -123,11 +123,11 @@ __global__ __launch_bounds__(B) void kb_relax_group(const StepPtrs<T>* tab, uint
 template <class T, class TO>
 __device__ __forceinline__ void residual_group(gptr<const T> X, gptr<const T> R, gptr<const T> S,
                                                gptr<TO> out, const Geom& g, uint32_t lane,
-                                               uint32_t lanes, T pw, T h2) {
+                                               uint32_t lanes, T pw, T h2, LiveMask live) {
   constexpr int V = kGroup<T>;
   const int c = lane >= lanes;
   SlotGroup<T, V> G;
-  load_group<T, V>(X, R, S, g, c, lane - (uint32_t)c * lanes, lanes, true, G);
+  load_group<T, V>(X, R, S, g, c, lane - (uint32_t)c * lanes, lanes, true, G, false, live);
   T v[V];
   group_residual<T, V>(G, g.n, pw, h2, v);
   store_group<T, TO, V>(out + G.own, v, G.in ? (1u << V) - 1 : 0u);
@@ -143,7 +143,8 @@ __global__ __launch_bounds__(B) void kb_residual_group(const StepPtrs<T>* tab, u
   if (slot >= count) return;
   const StepPtrs<T> p = tab[slot];
   residual_group<T, T>(as_global((const T*)p.x), as_global(p.rhs), as_global(p.screen),
-                       as_global(p.res), g, (lb - slot * nbi) * B + threadIdx.x, lanes, pw, h2);
+                       as_global(p.res), g, (lb - slot * nbi) * B + threadIdx.x, lanes, pw, h2,
+                       LiveMask{p.live, p.live_rhs != 0});
 }
 
 // restrict the residual into the next-coarser rhs and zero the next-coarser x
@@ -251,7 +252,8 @@ __global__ __launch_bounds__(B) void kb_defect_group(const TopPtrs* tab, uint32_
   const TopPtrs p = tab[slot];
   residual_group<double, float>(as_global((const double*)p.x64), as_global(p.rhs64),
                                 as_global(p.screen64), as_global(p.r32), g,
-                                (lb - slot * nbi) * B + threadIdx.x, lanes, pw, h2);
+                                (lb - slot * nbi) * B + threadIdx.x, lanes, pw, h2,
+                                LiveMask{p.live, true});
 }
 
 __global__ __launch_bounds__(kThreads) void kb_correct(const TopPtrs* tab, uint32_t count,

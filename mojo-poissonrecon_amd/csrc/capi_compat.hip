@@ -146,7 +146,7 @@ void mpr_assemble_f64(const double* vx, const double* vy, const double* vz, cons
     h2d(s.p, screen, M);
     launch_assemble(x.p, y.p, z.p, s.p, r.p, nullptr, nullptr, nullptr, 1.0,
                     make_geom((int32_t)n), point_weight, target_value, nullptr, nullptr,
-                    nullptr, nullptr, kNull);
+                    nullptr, nullptr, nullptr, kNull);
     d2h(rhs, r.p, M);
   } catch (const HipError& e) {
     report(__func__, e.msg);

@@ -112,7 +112,8 @@ __global__ __launch_bounds__(kThreads) void k_residual(const T* X, const T* R, c
 template <class T, class TR>
 __global__ __launch_bounds__(kThreads) void k_residual_group(const T* X, const T* R, const T* S,
                                                              TR* out, Geom g, uint32_t lanes,
-                                                             T pw, T h2, double* partials) {
+                                                             T pw, T h2, double* partials,
+                                                             const uint32_t* live) {
   constexpr int V = 16 / (int)sizeof(T);
   __shared__ double lds[kThreads / 64];
   const uint32_t lb = logical_block();
@@ -120,7 +121,7 @@ __global__ __launch_bounds__(kThreads) void k_residual_group(const T* X, const T
   const int c = lane >= lanes;
   SlotGroup<T, V> G;
   load_group<T, V>(as_global(X), as_global(R), as_global(S), g, c, lane - (uint32_t)c * lanes,
-                   lanes, true, G);
+                   lanes, true, G, false, LiveMask{live, true});
   if (lane >= 2u * lanes) G.in = false;
   T v[V];
   group_residual<T, V>(G, g.n, pw, h2, v);
@@ -276,7 +277,8 @@ __global__ __launch_bounds__(kThreads) void k_assemble(const double* vx, const d
                                                        double pw, double target,
                                                        const uint32_t* touched,
                                                        double* scr_nat_out, float* scr_split32,
-                                                       const uint32_t* need, uint32_t* prev) {
+                                                       const uint32_t* need, uint32_t* prev,
+                                                       uint32_t* live) {
   const uint32_t t = logical_block() * kThreads + threadIdx.x;
   const int n = g.n;
   if (t >= (uint32_t)n * n * n) return;
@@ -312,6 +314,9 @@ __global__ __launch_bounds__(kThreads) void k_assemble(const double* vx, const d
     rhs_split[si] = r;
     scr_split[si] = s;
     if (scr_split32) scr_split32[si] = (float)s;
+    // live slots: rhs or screen not +0.0 (bit patterns, so -0.0 counts as live)
+    if (live && (__double_as_longlong(r) | __double_as_longlong(s)) != 0)
+      atomicOr(&live[si >> 5], 1u << (si & 31u));
   }
   if (density) density[t] = splat_value(density, tb, t) / spn;
 }
@@ -351,12 +356,12 @@ void launch_relax(T* X, const T* R, const T* S, const Geom& g, double pw, int it
 
 template <class T, class TR>
 int launch_residual(const T* X, const T* R, const T* S, TR* out, const Geom& g, double pw,
-                    double* partials, hipStream_t s) {
+                    double* partials, hipStream_t s, const uint32_t* live) {
   const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
   if (group_layout(g)) {
     const uint32_t lanes = g.C / (uint32_t)(16 / sizeof(T));
     hipLaunchKernelGGL((k_residual_group<T, TR>), dim3(xcd_grid(2ull * lanes, kThreads)),
-                       dim3(kThreads), 0, s, X, R, S, out, g, lanes, (T)pw, h2, partials);
+                       dim3(kThreads), 0, s, X, R, S, out, g, lanes, (T)pw, h2, partials, live);
     MPR_LAUNCH_CHECK();
     return (int)blocks_for(2ull * lanes, kThreads);
   }
@@ -443,7 +448,8 @@ void launch_assemble(const double* vx, const double* vy, const double* vz,
                      const double* screen_nat, double* rhs_nat, double* rhs_split,
                      double* screen_split, double* density, double spn, const Geom& g,
                      double pw, double target, const uint32_t* touched, double* screen_nat_out,
-                     float* screen_split32, const TouchedHistory* hist, hipStream_t s) {
+                     float* screen_split32, const TouchedHistory* hist, uint32_t* live,
+                     hipStream_t s) {
   const uint64_t m = (uint64_t)g.n * g.n * g.n;
   const uint32_t* need = nullptr;
   uint32_t* prev = nullptr;
@@ -466,7 +472,7 @@ void launch_assemble(const double* vx, const double* vy, const double* vz,
   const unsigned nb = xcd_grid(m, kThreads);
   hipLaunchKernelGGL(k_assemble, dim3(nb), dim3(kThreads), 0, s, vx, vy, vz, screen_nat,
                      rhs_nat, rhs_split, screen_split, density, spn, g, FastDiv((uint32_t)g.n),
-                     pw, target, touched, screen_nat_out, screen_split32, need, prev);
+                     pw, target, touched, screen_nat_out, screen_split32, need, prev, live);
   MPR_LAUNCH_CHECK();
 }
 
@@ -488,11 +494,14 @@ template void launch_fill<int64_t>(int64_t*, int64_t, uint64_t, hipStream_t);
 template void launch_fill<unsigned long long>(unsigned long long*, unsigned long long, uint64_t,
                                               hipStream_t);
 template int launch_residual<double, double>(const double*, const double*, const double*,
-                                             double*, const Geom&, double, double*, hipStream_t);
+                                             double*, const Geom&, double, double*, hipStream_t,
+                                             const uint32_t*);
 template int launch_residual<double, float>(const double*, const double*, const double*, float*,
-                                            const Geom&, double, double*, hipStream_t);
+                                            const Geom&, double, double*, hipStream_t,
+                                            const uint32_t*);
 template int launch_residual<float, float>(const float*, const float*, const float*, float*,
-                                           const Geom&, double, double*, hipStream_t);
+                                           const Geom&, double, double*, hipStream_t,
+                                           const uint32_t*);
 template void launch_nat_to_split<double, double>(const double*, double*, const Geom&,
                                                   hipStream_t);
 template void launch_nat_to_split<double, float>(const double*, float*, const Geom&,

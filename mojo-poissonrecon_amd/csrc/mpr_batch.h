@@ -22,6 +22,10 @@ struct StepPtrs {
   T* xc;    // next-coarser x
   T* rhsc;  // next-coarser rhs
   int top;  // this size class is the instance's finest level
+  // live slots of the finest level (LiveMask, stencil.h; null elsewhere) and whether they
+  // also cover its rhs (the assembled right-hand side, not the mixed V-cycle's defect)
+  int live_rhs;
+  const uint32_t* live;
 };
 
 // First sweep of a V-cycle level from a known-zero x: 0 x as stored, 1 zero for instances
@@ -35,6 +39,7 @@ struct TopPtrs {
   const double* screen64;
   float* r32;
   const float* e32;
+  const uint32_t* live;  // live slots of rhs64 / screen64 (LiveMask, stencil.h), or null
 };
 
 // one instance of the LDS-resident coarse V-cycle: its top coarse class, whether that
@@ -82,6 +87,8 @@ template <class T>
 struct InstanceLevels {
   int depth;
   std::vector<LevelPtrs<T>> levels;
+  const uint32_t* live = nullptr;  // live slots of levels[0] (LiveMask, stencil.h)
+  bool live_rhs = false;           // ... covering levels[0].rhs too
 };
 
 // Device tables + schedule for the V-cycles of a fixed set of instances.

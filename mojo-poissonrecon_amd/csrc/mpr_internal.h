@@ -28,9 +28,10 @@ void launch_relax_color(T* X, const T* R, const T* S, const Geom& g, double pw, 
                         hipStream_t s);
 // r = rhs - A x written as TR into `out` (split layout). When `partials` is non-null each
 // block also writes the sum of squares of its residuals there; returns the block count.
+// live: the level's live-slot bits covering R and S (LiveMask, stencil.h), or null.
 template <class T, class TR>
 int launch_residual(const T* X, const T* R, const T* S, TR* out, const Geom& g, double pw,
-                    double* partials, hipStream_t s);
+                    double* partials, hipStream_t s, const uint32_t* live = nullptr);
 // number of squared-norm partials launch_residual writes for a level of word_bytes words
 int residual_partial_count(const Geom& g, size_t word_bytes);
 // out = sqrt(sum partials) with a fixed summation order
@@ -62,7 +63,8 @@ void launch_add_correction(double* X, const float* E, uint64_t count, hipStream_
 // `prev` holds the previous assembly's touched bits (all ones before the first, so that one
 // writes every node) and is updated in place by every assembly given `hist` (write_all
 // true: a dense assembly that still records its touched bits); `need` is scratch of the
-// same size.
+// same size. live (nullable, zeroed by the caller): the split layout's live-slot bits
+// (LiveMask, stencil.h) are set where rhs or screen is written non-zero.
 struct TouchedHistory {
   uint32_t* prev;
   uint32_t* need;
@@ -72,7 +74,8 @@ void launch_assemble(const double* vx, const double* vy, const double* vz,
                      const double* screen_nat, double* rhs_nat, double* rhs_split,
                      double* screen_split, double* density, double spn, const Geom& g,
                      double pw, double target, const uint32_t* touched, double* screen_nat_out,
-                     float* screen_split32, const TouchedHistory* hist, hipStream_t s);
+                     float* screen_split32, const TouchedHistory* hist, uint32_t* live,
+                     hipStream_t s);
 // ---- point kernels (point_kernels.hip) --------------------------------------------------
 int bbox_partial_count(uint32_t P);
 // core.py:377-385 on the device: params = [origin x, y, z, width], unit = (p - origin)/width

@@ -105,6 +105,8 @@ struct Job {
   uint32_t* touched = nullptr;  // splat node bits, ceil(n^3 / 32) words
   // the previous assembly's touched bits and the sparse assembly's need mask (same size)
   uint32_t *prev_touched = nullptr, *need_bits = nullptr;
+  // live slots of the finest level (LiveMask, stencil.h): rhs or screen non-zero
+  uint32_t* live_bits = nullptr;
   // how the density grid holds the last solve's values: raw splat sums read through
   // `touched` and divided by dens_spn, or normalised in place (runs with MPR_STAGE_GRIDS)
   bool dens_raw = false;
@@ -128,12 +130,15 @@ struct Job {
           *o_ccount = nullptr, *o_leaf = nullptr;
 
   void plan(DevArena& a);
-  InstanceLevels<double> levels64() const { return {depth, L64}; }
-  InstanceLevels<float> levels32() const { return {depth, L32}; }
+  // the finest level's live slots cover the assembled rhs and screen (FP64) and the FP32
+  // screen; the mixed V-cycle's FP32 rhs is the (dense) defect
+  InstanceLevels<double> levels64() const { return {depth, L64, live_bits, true}; }
+  InstanceLevels<float> levels32() const { return {depth, L32, live_bits, false}; }
   TopPtrs top_ptrs() const {
     return {L64[0].x, L64[0].rhs, L64[0].screen, L32.empty() ? nullptr : L32[0].rhs,
-            L32.empty() ? nullptr : L32[0].x};
+            L32.empty() ? nullptr : L32[0].x, live_bits};
   }
+  uint64_t live_words() const { return (2ull * geoms[0].C + 31) / 32; }
   const double* unit_points(const mpr_params& p) const {
     return (p.stages & MPR_STAGE_NORMALIZE) ? unit : pts;
   }
@@ -415,6 +420,7 @@ void Job::plan(DevArena& a) {
   touched = a.alloc<uint32_t>((M + 31) / 32);
   prev_touched = a.alloc<uint32_t>((M + 31) / 32);
   need_bits = a.alloc<uint32_t>((M + 31) / 32);
+  live_bits = a.alloc<uint32_t>(live_words());
   splat = splat_scratch_sizes(std::max<uint32_t>(1u, max_points));
   splat.keys = a.alloc<uint64_t>(splat.pairs);
   splat.ktmp = a.alloc<uint64_t>(splat.pairs);
@@ -493,10 +499,14 @@ void Job::setup(const mpr_params& p, hipStream_t s) {
   // previous run's splat (the rest of the right-hand side and screening weights stay +0.0)
   // and leaves the density raw; with it, every grid is written in natural layout too. The
   // FP32 screening weights of the mixed V-cycle are written beside the FP64 ones.
+  // The live-slot bits let the finest level's kernels skip loading the +0.0 rhs and screen
+  // words away from the splat.
   const TouchedHistory hist{prev_touched, need_bits, grids};
+  launch_fill<uint32_t>(live_bits, 0u, live_words(), s);
   launch_assemble(vx, vy, vz, screen, grids ? rhs : nullptr, L64[0].rhs, L64[0].screen,
                   grids ? density : nullptr, p.samples_per_node, geoms[0], p.point_weight, 0.5,
-                  touched, grids ? screen : nullptr, mixed ? L32[0].screen : nullptr, &hist, s);
+                  touched, grids ? screen : nullptr, mixed ? L32[0].screen : nullptr, &hist,
+                  live_bits, s);
   adopt(p);  // the extraction enqueued next reads the density as this assembly leaves it
   for (size_t l = 1; l < L64.size(); ++l)
     launch_restrict<double>(L64[l - 1].screen, geoms[l - 1], L64[l].screen, geoms[l], 4.0, s,
@@ -512,7 +522,7 @@ void Job::finish(const mpr_params& p, hipStream_t s) {
   // only the norm of the final residual is reported; the iso value and the extraction read
   // the solution in the solver's layout, so the natural field is written for fetching only
   launch_residual<double, double>(L64[0].x, L64[0].rhs, L64[0].screen, (double*)nullptr,
-                                  geoms[0], p.point_weight, norm_part, s);
+                                  geoms[0], p.point_weight, norm_part, s, live_bits);
   launch_norm_finish(norm_part, norm_nparts, d_resnorm, s);
   if (p.stages & MPR_STAGE_GRIDS) launch_split_to_nat<double>(L64[0].x, field, geoms[0], s);
   if (P)
@@ -1058,14 +1068,31 @@ int mpr_batch_query(mpr_batch* b) {
   return -2;
 }
 
+// fraction of the finest level's V-slot groups with a live slot (the groups whose rhs and
+// screen words the pipeline's kernels load), for the kernels' algorithmic byte counts
+static double live_group_fraction(const mpr_job* job, int V) {
+  std::vector<uint32_t> bits(job->live_words());
+  MPR_HIP(hipMemcpyAsync(bits.data(), job->live_bits, 4 * bits.size(), hipMemcpyDeviceToHost,
+                         job->stream));
+  MPR_HIP(hipStreamSynchronize(job->stream));
+  const uint64_t slots = 2ull * job->geoms[0].C, groups = slots / V;
+  const uint32_t gm = (1u << V) - 1u;
+  uint64_t live = 0;
+  for (uint64_t k = 0; k < groups; ++k) {
+    const uint64_t b = k * V;
+    live += ((bits[b >> 5] >> (b & 31)) & gm) != 0;
+  }
+  return groups ? (double)live / (double)groups : 1.0;
+}
+
 int mpr_job_time_kernel(mpr_job* job, int32_t kernel, int32_t reps, double* ms_per_launch,
                         double* bytes_per_launch) {
-  if (!job || reps < 1 || kernel < 0 || kernel > 12) {
+  if (!job || reps < 1 || kernel < 0 || kernel > 13) {
     set_error("invalid kernel timing request");
     return -1;
   }
   if ((kernel == 1 || kernel == 5 || kernel == 6 || kernel == 7 || kernel == 9 ||
-       kernel == 10 || kernel == 11) &&
+       kernel == 10 || kernel == 11 || kernel == 13) &&
       job->L32.empty()) {
     set_error("FP32 kernels need a mixed-precision job");
     return -1;
@@ -1079,6 +1106,13 @@ int mpr_job_time_kernel(mpr_job* job, int32_t kernel, int32_t reps, double* ms_p
   auto& a = job->L64[0];
   double bytes = 0;
   int launches = reps;
+  // the pipeline's finest-level kernels skip the rhs / screen words of groups without a
+  // live slot (not the standalone kernels 0-6); mask words: one bit per slot
+  const bool top_live = group_layout(g0) && (kernel == 7 || kernel == 8 || kernel == 9 ||
+                                             kernel == 13);
+  const double live = top_live ? live_group_fraction(job, kernel == 8 || kernel == 13 ? 2 : 4)
+                               : 1.0;
+  const double mask = top_live ? M0 / 8.0 : 0.0;
   hipEvent_t e0, e1;
   MPR_HIP(hipEventCreate(&e0));
   MPR_HIP(hipEventCreate(&e1));
@@ -1121,19 +1155,19 @@ int mpr_job_time_kernel(mpr_job* job, int32_t kernel, int32_t reps, double* ms_p
   }
   // the V-cycle's batched kernels at the finest class, as the pipeline launches them
   switch (kernel) {
-    case 7:
+    case 7:  // other colour x, rhs (the dense defect), x written; screen of live groups
       job->vc32.relax_finest(pw, reps, s);
-      bytes = 4.0 * 4.0 * M0 / 2.0;
+      bytes = (3.0 + live) * 4.0 * M0 / 2.0 + mask / 2.0;
       launches = 2 * reps;
       break;
-    case 8:
+    case 8:  // other colour x, x written; rhs and screen of live groups
       job->vc64.relax_finest(pw, reps, s);
-      bytes = 4.0 * 8.0 * M0 / 2.0;
+      bytes = (2.0 + 2.0 * live) * 8.0 * M0 / 2.0 + mask / 2.0;
       launches = 2 * reps;
       break;
-    case 9:
+    case 9:  // x, rhs, residual written; screen of live groups
       for (int r = 0; r < reps; ++r) job->vc32.residual_finest(pw, s);
-      bytes = 4.0 * 4.0 * M0;
+      bytes = (3.0 + live) * 4.0 * M0 + mask;
       break;
     case 10:
       for (int r = 0; r < reps; ++r) job->vc32.restrict_finest(s);
@@ -1146,6 +1180,10 @@ int mpr_job_time_kernel(mpr_job* job, int32_t kernel, int32_t reps, double* ms_p
     case 12:
       for (int r = 0; r < reps; ++r) job->vc64.restrict_finest(s);
       bytes = 8.0 * (M0 + M1);
+      break;
+    case 13:  // x64 read, r32 written; rhs64 and screen64 of live groups
+      for (int r = 0; r < reps; ++r) job->dtab.defect(pw, s);
+      bytes = (8.0 + 4.0 + 16.0 * live) * M0 + mask;
       break;
   }
   MPR_HIP(hipEventRecord(e1, s));

@@ -185,6 +185,16 @@ __device__ __forceinline__ void load_chunks(CT (&d)[NC], gptr<const T> p) {
   for (int i = 0; i < NC; ++i) d[i] = *(const __attribute__((address_space(1))) CT*)(p + i * W);
 }
 
+// Live slots of a level whose right-hand side and screening weights are exactly +0.0 away
+// from the splat: bit k of `bits` (k = colour * C + slot, the split index) is set where the
+// assembled rhs or screen of slot k is non-zero. A group whose V bits are all clear reads
+// the screening weights (and, with `rhs`, the right-hand side) as +0.0 without loading them:
+// the values it would load. bits == nullptr: load everything.
+struct LiveMask {
+  const uint32_t* bits = nullptr;
+  bool rhs = false;
+};
+
 // lane: index of the group inside colour block c; lanes = C / V. x_zero: the other
 // colour's x is known to be zero (the first sweep from a zero start), so its loads are
 // skipped and the neighbour values are +0, exactly what a zero-filled x would give.
@@ -192,7 +202,7 @@ template <class T, int V>
 __device__ __forceinline__ void load_group(gptr<const T> X, gptr<const T> R, gptr<const T> S,
                                            const Geom& g, int c, uint32_t lane, uint32_t lanes,
                                            bool want_self, SlotGroup<T, V>& G,
-                                           bool x_zero = false) {
+                                           bool x_zero = false, LiveMask live = LiveMask()) {
   G.in = lane < lanes;
   lane = G.in ? lane : lanes - 1;
   const uint32_t s0 = lane * (uint32_t)V;
@@ -222,8 +232,22 @@ __device__ __forceinline__ void load_group(gptr<const T> X, gptr<const T> R, gpt
     load_chunks<T>(G.zm, X + (z > 0 ? o - g.nH : o));
     load_chunks<T>(G.zp, X + ((int)z + 1 < n ? o + g.nH : o));
   }
-  load_chunks<T>(G.r, R + G.own);
-  load_chunks<T>(G.s, S + G.own);
+  // own is a multiple of V (C is a multiple of 4), so the group's V bits share one word
+  const bool dead =
+      live.bits && !((live.bits[G.own >> 5] >> (G.own & 31u)) & ((1u << V) - 1u));
+  using CT = typename SlotGroup<T, V>::CT;
+  if (dead && live.rhs) {
+#pragma unroll
+    for (int i = 0; i < SlotGroup<T, V>::NC; ++i) G.r[i] = CT(T(0));
+  } else {
+    load_chunks<T>(G.r, R + G.own);
+  }
+  if (dead) {
+#pragma unroll
+    for (int i = 0; i < SlotGroup<T, V>::NC; ++i) G.s[i] = CT(T(0));
+  } else {
+    load_chunks<T>(G.s, S + G.own);
+  }
   if (want_self) load_chunks<T>(G.self, X + G.own);
 }
 

@@ -58,7 +58,9 @@ void BatchedVcycle<T>::build(const std::vector<InstanceLevels<T>>& inst, void* t
       const int k = i.depth - s;
       const LevelPtrs<T>& a = i.levels[k];
       const LevelPtrs<T>& b = i.levels[k + 1];
-      StepPtrs<T> e{a.x, a.rhs, a.screen, a.res, b.x, b.rhs, i.depth == s ? 1 : 0};
+      const bool top = i.depth == s;
+      StepPtrs<T> e{a.x, a.rhs, a.screen, a.res, b.x, b.rhs, top ? 1 : 0,
+                    top && i.live_rhs ? 1 : 0, top ? i.live : nullptr};
       steps.push_back(e);
       if (i.depth == s) tops.push_back(e);
     }
