@@ -38,10 +38,10 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the roofline kernel (kb_relax_group<float>, one colour of the
-# 257^3 level): FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from rocprofv3 counter runs, the factors
-# calibrated on a plain elementwise copy (profiles/kb_relax_group_f32_depth8_pmc.txt).
-# Applies to the depth-8 workload only.
-ROOFLINE_TRAFFIC_BYTES = {8: 141_465_323}
+# 257^3 finest level of all instances of the batch), keyed by (depth, batch):
+# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from rocprofv3 counter runs, the factors calibrated on a
+# plain elementwise copy (profiles/kb_relax_group_f32_depth8_pmc.txt).
+ROOFLINE_TRAFFIC_BYTES = {}
 
 
 def log(*args):
@@ -110,8 +110,10 @@ def uneven_batch(count: int, seed: int):
     return clouds, depths
 
 
-def run_batch_mode(clouds, depths, precision, reps):
-    """Records the batched run as one graph, replays it; returns (seconds per launch, meshes)."""
+def run_batch_mode(clouds, depths, precision, reps, kernel_reps=0, kernels=None):
+    """Records the batched run as one graph, replays it; returns (seconds per launch, latencies,
+    meshes). With `kernels` [(name, id)] it also times those batched kernels afterwards on the
+    batch's stream (include/mpr_hip.h mpr_batch_time_kernel) into the dict `kernels_out`."""
     from mojo_poissonrecon.batch import Batch
     from mojo_poissonrecon.core import Job, mesh_from_job, prepare_reconstruct
 
@@ -133,9 +135,21 @@ def run_batch_mode(clouds, depths, precision, reps):
         lat.append(time.perf_counter() - s)
     total = time.perf_counter() - t0
     meshes = [mesh_from_job(j, res, len(r.points)) for j, res, r in zip(jobs, results, reqs)]
+    timed_kernels = {}
+    for name, kid in kernels or ():
+        from mojo_poissonrecon import _lib
+        ms, nbytes = _lib.ctypes.c_double(), _lib.ctypes.c_double()
+        _lib.check(_lib.lib().mpr_batch_time_kernel(b.handle, kid, kernel_reps,
+                                                     _lib.ctypes.byref(ms),
+                                                     _lib.ctypes.byref(nbytes)),
+                   "mpr_batch_time_kernel")
+        timed_kernels[name] = dict(ms=ms.value, bytes=nbytes.value,
+                                   gbs=nbytes.value / (ms.value * 1e-3) / 1e9)
     b.close()
     for j in jobs:
         j.close()
+    if kernels:
+        return total / reps, np.array(lat), meshes, timed_kernels
     return total / reps, np.array(lat), meshes
 
 
@@ -315,11 +329,17 @@ def main() -> int:
         return 0
     if args.only == "mixed_batch":
         bclouds = [shapes.torus_workload(args.points, seed=s) for s in range(args.batch)]
-        per_launch, blat, _ = run_batch_mode(bclouds, [args.depth] * args.batch, "mixed",
-                                             args.steps)
+        per_launch, blat, _, bk = run_batch_mode(
+            bclouds, [args.depth] * args.batch, "mixed", args.steps,
+            kernel_reps=args.kernel_reps,
+            kernels=[("rb_sweep_f32", 7), ("residual_f32", 9), ("defect_f64_f32", 13)])
         print(json.dumps({"mode": "mixed_batch", "batch": args.batch,
                           "recon_per_s": round(args.batch / per_launch, 3),
-                          "p50_ms": pct(blat, 50), "steps": args.steps}), flush=True)
+                          "p50_ms": pct(blat, 50), "steps": args.steps,
+                          "kernels_batch": {k: {"us": round(v["ms"] * 1e3, 2),
+                                                "GBps": round(v["gbs"], 1),
+                                                "bytes": round(v["bytes"])}
+                                            for k, v in bk.items()}}), flush=True)
         return 0
     runs = (("fp64", _lib.PRECISION_FP64), ("mixed", _lib.PRECISION_MIXED))
     if args.only:
@@ -367,8 +387,10 @@ def main() -> int:
     log(f"mixed_graph: {modes['mixed_graph']['value']:.2f} recon/s")
     # batched: --batch depth-8 reconstructions of the workload family in one graph launch
     bclouds = [shapes.torus_workload(args.points, seed=s) for s in range(args.batch)]
-    per_launch, blat, bmeshes = run_batch_mode(bclouds, [args.depth] * args.batch, "mixed",
-                                               max(2, args.steps // 2))
+    per_launch, blat, bmeshes, bkernels = run_batch_mode(
+        bclouds, [args.depth] * args.batch, "mixed", max(2, args.steps // 2),
+        kernel_reps=args.kernel_reps,
+        kernels=[("rb_sweep_f32", 7), ("residual_f32", 9), ("defect_f64_f32", 13)])
     modes["mixed_batch"] = dict(value=args.batch / per_launch, p50=pct(blat, 50),
                                 p99=pct(blat, 99))
     log(f"mixed_batch x{args.batch}: {modes['mixed_batch']['value']:.2f} recon/s")
@@ -379,7 +401,9 @@ def main() -> int:
     kernels = time_kernels(jobs, args.kernel_reps)
     for job in jobs.values():
         job.close()
-    dom = kernels["rb_sweep_f32"]
+    # the dominant kernel of the headline workload: the batched FP32 colour sweep over the
+    # finest level of all --batch instances, timed on the batch's stream
+    dom = bkernels["rb_sweep_f32"]
 
     extras = {}
     if not args.no_extras:
@@ -435,18 +459,21 @@ def main() -> int:
         "hausdorff_mixed_vs_fp64_over_diag": hd,
         "mesh": {"vertices": modes["mixed"]["nv"], "fp64_vertices": fp64["nv"]},
         "roofline": {
-            "kernel": "rb_sweep_f32: kb_relax_group<float>, one colour of the finest level",
+            "kernel": f"kb_relax_group<float>: one colour of the finest level of all {args.batch} "
+                      "instances of the batch (one launch)",
             "algorithmic_bytes": round(dom["bytes"]),
             "bound": "hbm",
             "achieved": round(dom["gbs"], 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4),
-            "traffic": ROOFLINE_TRAFFIC_BYTES.get(args.depth),
+            "traffic": ROOFLINE_TRAFFIC_BYTES.get((args.depth, args.batch)),
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)",
         },
         "kernels": {k: {"us": round(v["ms"] * 1e3, 2), "GBps": round(v["gbs"], 1)}
                     for k, v in kernels.items()},
+        "kernels_batch": {k: {"us": round(v["ms"] * 1e3, 2), "GBps": round(v["gbs"], 1),
+                              "bytes": round(v["bytes"])} for k, v in bkernels.items()},
     }
     line.update(extras)
     if not args.no_cpu_baseline:

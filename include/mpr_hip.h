@@ -192,6 +192,13 @@ int mpr_batch_launch(mpr_batch *batch);
 int mpr_batch_query(mpr_batch *batch);
 /* waits, resolves extraction overflow per job; results: one per job (may be NULL) */
 int mpr_batch_wait(mpr_batch *batch, mpr_result *results);
+/* Kernel timing of a batch on its own stream with HIP events (as mpr_job_time_kernel, after
+ * a run): one launch covers every job of the batch's finest depth. Kernels 7 (FP32 colour
+ * sweep), 9 (FP32 residual) and 13 (FP64->FP32 defect; jobs of one depth) for a mixed
+ * batch, 8 (FP64 colour sweep) for an FP64 one. bytes_per_launch sums the jobs' algorithmic
+ * bytes. No reference counterpart (measurement hook). */
+int mpr_batch_time_kernel(mpr_batch *batch, int32_t kernel, int32_t reps, double *ms_per_launch,
+                          double *bytes_per_launch);
 
 /* Kernel timing on the job's own stream with HIP events: launches kernel `kernel` `reps`
  * times on the job's finest level (after a run) and returns the mean milliseconds per

@@ -119,18 +119,22 @@ __global__ __launch_bounds__(B) void kb_relax_group(const StepPtrs<T>* tab, uint
   store_group<T, T, V>(as_global(p.x) + G.own, out, mask);
 }
 
-// residual of both colours: lanes [0, lanes) cover colour 0, [lanes, 2 lanes) colour 1
+// residual of slot group `lane` in both colours. One lane takes the same slots of both
+// colour blocks, so the words each colour reads as neighbours it also reads as its own
+// nodes at the same time (from cache), and x streams from memory once.
 template <class T, class TO>
 __device__ __forceinline__ void residual_group(gptr<const T> X, gptr<const T> R, gptr<const T> S,
                                                gptr<TO> out, const Geom& g, uint32_t lane,
                                                uint32_t lanes, T pw, T h2, LiveMask live) {
   constexpr int V = kGroup<T>;
-  const int c = lane >= lanes;
-  SlotGroup<T, V> G;
-  load_group<T, V>(X, R, S, g, c, lane - (uint32_t)c * lanes, lanes, true, G, false, live);
-  T v[V];
-  group_residual<T, V>(G, g.n, pw, h2, v);
-  store_group<T, TO, V>(out + G.own, v, G.in ? (1u << V) - 1 : 0u);
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    SlotGroup<T, V> G;
+    load_group<T, V>(X, R, S, g, c, lane, lanes, true, G, false, live);
+    T v[V];
+    group_residual<T, V>(G, g.n, pw, h2, v);
+    store_group<T, TO, V>(out + G.own, v, G.in ? (1u << V) - 1 : 0u);
+  }
 }
 
 template <class T, int B>
@@ -155,11 +159,13 @@ __global__ __launch_bounds__(kThreads) void kb_restrict(const StepPtrs<T>* tab, 
   const BlockMap m = block_map(nbi);
   if (m.slot >= count || m.t >= 2u * gc.C) return;
   const StepPtrs<T> p = tab[m.slot];
-  const int c = m.t >= gc.C;
+  // colours interleaved (neighbouring threads restrict neighbouring coarse nodes, which
+  // read overlapping fine nodes of both colours)
+  const int c = (int)(m.t & 1u);
+  const uint32_t i = (uint32_t)c * gc.C + (m.t >> 1);
   int X, Y, Z;
-  if (split_decode(gc, c, m.t - (uint32_t)c * gc.C, X, Y, Z))
-    p.rhsc[m.t] = restrict_node(p.res, gf, X, Y, Z, T(1));
-  if (zero_coarse) p.xc[m.t] = T(0);
+  if (split_decode(gc, c, m.t >> 1, X, Y, Z)) p.rhsc[i] = restrict_node(p.res, gf, X, Y, Z, T(1));
+  if (zero_coarse) p.xc[i] = T(0);
 }
 
 template <class T>
@@ -173,10 +179,8 @@ __global__ __launch_bounds__(kThreads) void kb_restrict_group(const StepPtrs<T>*
   if (slot >= count) return;
   const StepPtrs<T> p = tab[slot];
   const uint32_t lane = (lb - slot * nbi) * kThreads + threadIdx.x;
-  const int c = lane >= lanes;
-  const uint32_t l = lane < 2u * lanes ? lane - (uint32_t)c * lanes : lanes;
   restrict_group<T, kGroup<T>>(as_global((const T*)p.res), gf, as_global(p.rhsc),
-                               zero_coarse ? as_global(p.xc) : gptr<T>(nullptr), gc, c, l, lanes,
+                               zero_coarse ? as_global(p.xc) : gptr<T>(nullptr), gc, lane, lanes,
                                T(1));
 }
 
@@ -515,7 +519,7 @@ void launch_b_residual(const StepPtrs<T>* tab, uint32_t count, const Geom& g, do
   if (group_layout(g)) {
     const uint32_t lanes = g.C / (uint32_t)kGroup<T>;
     const int block = tuned_block(kTuneResidual, sizeof(T) == 8, size_class(g));
-    const uint32_t nbi = blocks_for(2ull * lanes, (unsigned)block);
+    const uint32_t nbi = blocks_for(lanes, (unsigned)block);  // both colours per lane
 #define MPR_LAUNCH_RESIDUAL(B)                                                                 \
   hipLaunchKernelGGL((kb_residual_group<T, B>), dim3(padded8((uint64_t)nbi * count)), dim3(B), \
                      0, s, tab, count, nbi, g, lanes, (T)pw, h2)
@@ -536,7 +540,7 @@ void launch_b_restrict(const StepPtrs<T>* tab, uint32_t count, const Geom& gf, c
   if (!count) return;
   if (sizeof(T) == 4 && group_layout(gc) && group_layout(gf)) {  // as launch_restrict
     const uint32_t lanes = gc.C / (uint32_t)kGroup<T>;
-    const uint32_t nbi = blocks_for(2ull * lanes, kThreads);
+    const uint32_t nbi = blocks_for(lanes, kThreads);  // both coarse colours per lane
     hipLaunchKernelGGL(kb_restrict_group<T>, dim3(padded8((uint64_t)nbi * count)),
                        dim3(kThreads), 0, s, tab, count, nbi, gf, gc, lanes, zero_coarse ? 1 : 0);
     MPR_LAUNCH_CHECK();
@@ -582,7 +586,7 @@ void launch_b_defect(const TopPtrs* tab, uint32_t count, const Geom& g, double p
   if (group_layout(g)) {
     const uint32_t lanes = g.C / (uint32_t)kGroup<double>;
     const int block = tuned_block(kTuneDefect, true, size_class(g));
-    const uint32_t nbi = blocks_for(2ull * lanes, (unsigned)block);
+    const uint32_t nbi = blocks_for(lanes, (unsigned)block);  // both colours per lane
 #define MPR_LAUNCH_DEFECT(B)                                                                 \
   hipLaunchKernelGGL((kb_defect_group<B>), dim3(padded8((uint64_t)nbi * count)), dim3(B), 0, \
                      s, tab, count, nbi, g, lanes, pw, h2)

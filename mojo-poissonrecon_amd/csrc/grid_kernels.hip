@@ -107,8 +107,9 @@ __global__ __launch_bounds__(kThreads) void k_residual(const T* X, const T* R, c
   }
 }
 
-// residual over both colours in slot groups (lanes [0, lanes) colour 0, then colour 1);
-// block partials of the squared norm as k_residual
+// residual of slot group `lane` in both colours (one lane takes the same slots of both
+// colour blocks, so x streams from memory once); block partials of the squared norm as
+// k_residual (colour 0 then colour 1 per lane)
 template <class T, class TR>
 __global__ __launch_bounds__(kThreads) void k_residual_group(const T* X, const T* R, const T* S,
                                                              TR* out, Geom g, uint32_t lanes,
@@ -118,21 +119,22 @@ __global__ __launch_bounds__(kThreads) void k_residual_group(const T* X, const T
   __shared__ double lds[kThreads / 64];
   const uint32_t lb = logical_block();
   const uint32_t lane = lb * kThreads + threadIdx.x;
-  const int c = lane >= lanes;
-  SlotGroup<T, V> G;
-  load_group<T, V>(as_global(X), as_global(R), as_global(S), g, c, lane - (uint32_t)c * lanes,
-                   lanes, true, G, false, LiveMask{live, true});
-  if (lane >= 2u * lanes) G.in = false;
-  T v[V];
-  group_residual<T, V>(G, g.n, pw, h2, v);
-  if (out) store_group<T, TR, V>(as_global(out) + G.own, v, G.in ? (1u << V) - 1 : 0u);
-  if (partials) {
-    double sq = 0.0;
+  double sq = 0.0;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    SlotGroup<T, V> G;
+    load_group<T, V>(as_global(X), as_global(R), as_global(S), g, c, lane, lanes, true, G,
+                     false, LiveMask{live, true});
+    T v[V];
+    group_residual<T, V>(G, g.n, pw, h2, v);
+    if (out) store_group<T, TR, V>(as_global(out) + G.own, v, G.in ? (1u << V) - 1 : 0u);
     if (G.in)
 #pragma unroll
       for (int j = 0; j < V; ++j) sq += (double)v[j] * (double)v[j];
+  }
+  if (partials) {
     const double total = block_sum(sq, lds);
-    if (threadIdx.x == 0 && lb * kThreads < 2u * lanes) partials[lb] = total;
+    if (threadIdx.x == 0 && lb * kThreads < lanes) partials[lb] = total;
   }
 }
 
@@ -155,23 +157,24 @@ __global__ __launch_bounds__(kThreads) void k_restrict(const T* F, Geom gf, T* C
                                                        T scale, float* copy32) {
   const uint32_t t = logical_block() * kThreads + threadIdx.x;
   if (t >= 2u * gc.C) return;
-  const int c = t >= gc.C;
+  // colours interleaved (neighbouring threads restrict neighbouring coarse nodes, which
+  // read overlapping fine nodes of both colours)
+  const int c = (int)(t & 1u);
+  const uint32_t i = (uint32_t)c * gc.C + (t >> 1);
   int X, Y, Z;
-  if (!split_decode(gc, c, t - (uint32_t)c * gc.C, X, Y, Z)) return;
+  if (!split_decode(gc, c, t >> 1, X, Y, Z)) return;
   const T v = restrict_node(F, gf, X, Y, Z, scale);
-  Cc[t] = v;
-  if (copy32) copy32[t] = (float)v;
+  Cc[i] = v;
+  if (copy32) copy32[i] = (float)v;
 }
 
-// slot-group restriction over both coarse colours (lanes [0, lanes) colour 0, then 1)
+// slot-group restriction, both coarse colours of slot group `lane` per lane
 template <class T>
 __global__ __launch_bounds__(kThreads) void k_restrict_group(const T* F, Geom gf, T* Cc, Geom gc,
                                                              uint32_t lanes, T scale) {
   constexpr int V = 16 / (int)sizeof(T);
   const uint32_t lane = logical_block() * kThreads + threadIdx.x;
-  const int c = lane >= lanes;
-  const uint32_t l = lane < 2u * lanes ? lane - (uint32_t)c * lanes : lanes;
-  restrict_group<T, V>(as_global(F), gf, as_global(Cc), gptr<T>(nullptr), gc, c, l, lanes, scale);
+  restrict_group<T, V>(as_global(F), gf, as_global(Cc), gptr<T>(nullptr), gc, lane, lanes, scale);
 }
 
 // slot-group prolong-add over both colours (lanes [0, lanes) colour 0, then colour 1)
@@ -360,10 +363,10 @@ int launch_residual(const T* X, const T* R, const T* S, TR* out, const Geom& g, 
   const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
   if (group_layout(g)) {
     const uint32_t lanes = g.C / (uint32_t)(16 / sizeof(T));
-    hipLaunchKernelGGL((k_residual_group<T, TR>), dim3(xcd_grid(2ull * lanes, kThreads)),
+    hipLaunchKernelGGL((k_residual_group<T, TR>), dim3(xcd_grid(lanes, kThreads)),
                        dim3(kThreads), 0, s, X, R, S, out, g, lanes, (T)pw, h2, partials, live);
     MPR_LAUNCH_CHECK();
-    return (int)blocks_for(2ull * lanes, kThreads);
+    return (int)blocks_for(lanes, kThreads);
   }
   const unsigned nb = xcd_grid(2ull * g.C, kThreads * kNodesPerThread);
   hipLaunchKernelGGL((k_residual<T, TR>), dim3(nb), dim3(kThreads), 0, s, X, R, S, out, g,
@@ -373,7 +376,7 @@ int launch_residual(const T* X, const T* R, const T* S, TR* out, const Geom& g, 
 }
 
 int residual_partial_count(const Geom& g, size_t word_bytes) {
-  if (group_layout(g)) return (int)blocks_for(2ull * (g.C / (uint32_t)(16 / word_bytes)), kThreads);
+  if (group_layout(g)) return (int)blocks_for(g.C / (uint32_t)(16 / word_bytes), kThreads);
   return (int)blocks_for(2ull * g.C, kThreads * kNodesPerThread);
 }
 
@@ -389,7 +392,7 @@ void launch_restrict(const T* F, const Geom& gf, T* Cc, const Geom& gc, double s
   // the scalar form's 27 and measured slower
   if (sizeof(T) == 4 && !copy32 && group_layout(gc) && group_layout(gf)) {
     const uint32_t lanes = gc.C / (uint32_t)(16 / sizeof(T));
-    hipLaunchKernelGGL(k_restrict_group<T>, dim3(xcd_grid(2ull * lanes, kThreads)),
+    hipLaunchKernelGGL(k_restrict_group<T>, dim3(xcd_grid(lanes, kThreads)),
                        dim3(kThreads), 0, s, F, gf, Cc, gc, lanes, (T)scale);
     MPR_LAUNCH_CHECK();
     return;

@@ -1085,6 +1085,28 @@ static double live_group_fraction(const mpr_job* job, int V) {
   return groups ? (double)live / (double)groups : 1.0;
 }
 
+// algorithmic bytes of one launch of pipeline kernel 7, 8, 9 or 13 (mpr_job_time_kernel) on
+// the job's finest level: words each kernel must move, with the rhs / screen words of
+// groups without a live slot left out (the kernels skip them) and one mask bit per slot
+static double pipeline_kernel_bytes(const mpr_job* job, int kernel) {
+  const Geom& g0 = job->geoms[0];
+  const double M0 = (double)g0.n * g0.n * g0.n;
+  const bool skip = group_layout(g0);
+  const double live = skip ? live_group_fraction(job, kernel == 8 || kernel == 13 ? 2 : 4) : 1.0;
+  const double mask = skip ? M0 / 8.0 : 0.0;
+  switch (kernel) {
+    case 7:  // one colour: other colour x, rhs (the dense defect), x written; live screen
+      return (3.0 + live) * 4.0 * M0 / 2.0 + mask / 2.0;
+    case 8:  // one colour: other colour x, x written; live rhs and screen
+      return (2.0 + 2.0 * live) * 8.0 * M0 / 2.0 + mask / 2.0;
+    case 9:  // x, rhs, residual written; live screen
+      return (3.0 + live) * 4.0 * M0 + mask;
+    case 13:  // x64 read, r32 written; live rhs64 and screen64
+      return (8.0 + 4.0 + 16.0 * live) * M0 + mask;
+  }
+  return 0.0;
+}
+
 int mpr_job_time_kernel(mpr_job* job, int32_t kernel, int32_t reps, double* ms_per_launch,
                         double* bytes_per_launch) {
   if (!job || reps < 1 || kernel < 0 || kernel > 13) {
@@ -1104,15 +1126,11 @@ int mpr_job_time_kernel(mpr_job* job, int32_t kernel, int32_t reps, double* ms_p
   const double M0 = (double)g0.n * g0.n * g0.n, M1 = (double)g1.n * g1.n * g1.n;
   const double pw = job->ran ? job->last.point_weight : 4.0;
   auto& a = job->L64[0];
-  double bytes = 0;
+  // the pipeline kernels' byte counts read the live-slot bits back: before the timed region
+  double bytes = (kernel == 7 || kernel == 8 || kernel == 9 || kernel == 13)
+                     ? pipeline_kernel_bytes(job, kernel)
+                     : 0.0;
   int launches = reps;
-  // the pipeline's finest-level kernels skip the rhs / screen words of groups without a
-  // live slot (not the standalone kernels 0-6); mask words: one bit per slot
-  const bool top_live = group_layout(g0) && (kernel == 7 || kernel == 8 || kernel == 9 ||
-                                             kernel == 13);
-  const double live = top_live ? live_group_fraction(job, kernel == 8 || kernel == 13 ? 2 : 4)
-                               : 1.0;
-  const double mask = top_live ? M0 / 8.0 : 0.0;
   hipEvent_t e0, e1;
   MPR_HIP(hipEventCreate(&e0));
   MPR_HIP(hipEventCreate(&e1));
@@ -1155,19 +1173,16 @@ int mpr_job_time_kernel(mpr_job* job, int32_t kernel, int32_t reps, double* ms_p
   }
   // the V-cycle's batched kernels at the finest class, as the pipeline launches them
   switch (kernel) {
-    case 7:  // other colour x, rhs (the dense defect), x written; screen of live groups
+    case 7:
       job->vc32.relax_finest(pw, reps, s);
-      bytes = (3.0 + live) * 4.0 * M0 / 2.0 + mask / 2.0;
       launches = 2 * reps;
       break;
-    case 8:  // other colour x, x written; rhs and screen of live groups
+    case 8:
       job->vc64.relax_finest(pw, reps, s);
-      bytes = (2.0 + 2.0 * live) * 8.0 * M0 / 2.0 + mask / 2.0;
       launches = 2 * reps;
       break;
-    case 9:  // x, rhs, residual written; screen of live groups
+    case 9:
       for (int r = 0; r < reps; ++r) job->vc32.residual_finest(pw, s);
-      bytes = (3.0 + live) * 4.0 * M0 + mask;
       break;
     case 10:
       for (int r = 0; r < reps; ++r) job->vc32.restrict_finest(s);
@@ -1181,9 +1196,67 @@ int mpr_job_time_kernel(mpr_job* job, int32_t kernel, int32_t reps, double* ms_p
       for (int r = 0; r < reps; ++r) job->vc64.restrict_finest(s);
       bytes = 8.0 * (M0 + M1);
       break;
-    case 13:  // x64 read, r32 written; rhs64 and screen64 of live groups
+    case 13:
       for (int r = 0; r < reps; ++r) job->dtab.defect(pw, s);
-      bytes = (8.0 + 4.0 + 16.0 * live) * M0 + mask;
+      break;
+  }
+  MPR_HIP(hipEventRecord(e1, s));
+  MPR_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  MPR_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ms_per_launch) *ms_per_launch = (double)ms / launches;
+  if (bytes_per_launch) *bytes_per_launch = bytes;
+  return 0;
+  MPR_GUARD_END
+}
+
+int mpr_batch_time_kernel(mpr_batch* b, int32_t kernel, int32_t reps, double* ms_per_launch,
+                          double* bytes_per_launch) {
+  if (!b || reps < 1 || (kernel != 7 && kernel != 8 && kernel != 9 && kernel != 13)) {
+    set_error("batch kernel timing: kernel 7, 8, 9 or 13 and reps >= 1");
+    return -1;
+  }
+  const bool mixed = b->precision == MPR_PRECISION_MIXED;
+  if ((kernel == 8) == mixed) {
+    set_error("batch kernel timing: kernel 8 needs an FP64 batch, 7, 9 and 13 a mixed one");
+    return -1;
+  }
+  for (Job* j : b->jobs)
+    if (kernel == 13 && j->depth != b->jobs[0]->depth) {
+      set_error("batch kernel timing: kernel 13 needs jobs of one depth");
+      return -1;
+    }
+  MPR_GUARD_BEGIN
+  hipStream_t s = b->stream;
+  for (Job* j : b->jobs) MPR_HIP(hipStreamSynchronize(j->stream));
+  MPR_HIP(hipStreamSynchronize(s));
+  const double pw = b->last.empty() ? 4.0 : b->last[0].point_weight;
+  const int smax = mixed ? b->vc32.smax : b->vc64.smax;
+  // one launch covers every instance of the batch's finest class
+  double bytes = 0.0;
+  for (Job* j : b->jobs)
+    if (j->depth == smax) bytes += pipeline_kernel_bytes(static_cast<const mpr_job*>(j), kernel);
+  hipEvent_t e0, e1;
+  MPR_HIP(hipEventCreate(&e0));
+  MPR_HIP(hipEventCreate(&e1));
+  MPR_HIP(hipEventRecord(e0, s));
+  int launches = reps;
+  switch (kernel) {
+    case 7:
+      b->vc32.relax_finest(pw, reps, s);
+      launches = 2 * reps;
+      break;
+    case 8:
+      b->vc64.relax_finest(pw, reps, s);
+      launches = 2 * reps;
+      break;
+    case 9:
+      for (int r = 0; r < reps; ++r) b->vc32.residual_finest(pw, s);
+      break;
+    case 13:
+      for (int r = 0; r < reps; ++r) b->dtab.defect(pw, s);
       break;
   }
   MPR_HIP(hipEventRecord(e1, s));

@@ -421,10 +421,11 @@ __device__ __forceinline__ void prolong_group(gptr<const T> XC, const Geom& gc, 
 // nine fine rows is read as two 16-byte chunks of both colours starting at slot 2 K0, plus
 // one slot before them. Terms are accumulated per coarse node in restrict_node's
 // (oz, oy, ox) order with its weights, so the result is restrict_node's. Needs 16-byte
-// aligned rows on both levels.
+// aligned rows on both levels. Both coarse colours' groups at slots K0.. of row (Y,Z) read
+// the same fine chunks, so one lane restricts both from one set of loads.
 template <class T, int V>
 __device__ __forceinline__ void restrict_group(gptr<const T> F, const Geom& gf, gptr<T> RC,
-                                               gptr<T> XC, const Geom& gc, int cc, uint32_t lane,
+                                               gptr<T> XC, const Geom& gc, uint32_t lane,
                                                uint32_t lanes, T scale) {
   using CT = typename SlotGroup<T, V>::CT;
   constexpr int W = SlotGroup<T, V>::W;
@@ -436,7 +437,6 @@ __device__ __forceinline__ void restrict_group(gptr<const T> F, const Geom& gf, 
   const uint32_t rr = s0 - Z * gc.nH;
   const uint32_t Y = gc.div_H.div(rr);
   const uint32_t K0 = rr - Y * (uint32_t)gc.H;
-  const int P = (cc ^ (int)Y ^ (int)Z) & 1;
   const int nf = gf.n;
   const uint32_t base = 2u * K0;
   // every load first (9 rows x 2 colours x NR chunks + 9 edge slots), then the sums
@@ -460,58 +460,64 @@ __device__ __forceinline__ void restrict_group(gptr<const T> F, const Geom& gf, 
     }
     before[r] = F[ro + (base > 0 ? base - 1 : 0)];
   }
-  T sum[V], wsum[V];
 #pragma unroll
-  for (int j = 0; j < V; ++j) sum[j] = wsum[j] = T(0);
+  for (int cc = 0; cc < 2; ++cc) {
+    const int P = (cc ^ (int)Y ^ (int)Z) & 1;
+    T sum[V], wsum[V];
 #pragma unroll
-  for (int r = 0; r < 9; ++r) {
-    const int oz = r / 3 - 1, oy = r % 3 - 1;
-    const int zz = 2 * (int)Z + oz, yy = 2 * (int)Y + oy;
-    const bool iny = zz >= 0 && zz < nf && yy >= 0 && yy < nf;
-    const T wz = oz == 0 ? T(2) : T(1), wy = oy == 0 ? T(2) : T(1);
+    for (int j = 0; j < V; ++j) sum[j] = wsum[j] = T(0);
 #pragma unroll
-    for (int j = 0; j < V; ++j) {
-      const T q0 = aq[r][(2 * j) / W][(2 * j) % W], q1 = aq[r][(2 * j + 1) / W][(2 * j + 1) % W];
-      const T o0 = ao[r][(2 * j) / W][(2 * j) % W], o1 = ao[r][(2 * j + 1) / W][(2 * j + 1) % W];
-      T om;
-      if (j == 0)
-        om = before[r];
-      else
-        om = ao[r][(2 * j - 1) / W][(2 * j - 1) % W];
-      const T vm = P ? o0 : om;  // x - 1
-      const T v0 = P ? q1 : q0;  // x
-      const T vp = P ? o1 : o0;  // x + 1
-      const int xc = 2 * (2 * ((int)K0 + j) + P);
-      const bool inm = iny && xc - 1 >= 0, in0 = iny && xc < nf, inp = iny && xc + 1 < nf;
-      const T wm = T(1) * wy * wz, w0 = T(2) * wy * wz, wp = T(1) * wy * wz;
-      if (inm) {
-        sum[j] += vm * wm;
-        wsum[j] += wm;
-      }
-      if (in0) {
-        sum[j] += v0 * w0;
-        wsum[j] += w0;
-      }
-      if (inp) {
-        sum[j] += vp * wp;
-        wsum[j] += wp;
+    for (int r = 0; r < 9; ++r) {
+      const int oz = r / 3 - 1, oy = r % 3 - 1;
+      const int zz = 2 * (int)Z + oz, yy = 2 * (int)Y + oy;
+      const bool iny = zz >= 0 && zz < nf && yy >= 0 && yy < nf;
+      const T wz = oz == 0 ? T(2) : T(1), wy = oy == 0 ? T(2) : T(1);
+#pragma unroll
+      for (int j = 0; j < V; ++j) {
+        const T q0 = aq[r][(2 * j) / W][(2 * j) % W];
+        const T q1 = aq[r][(2 * j + 1) / W][(2 * j + 1) % W];
+        const T o0 = ao[r][(2 * j) / W][(2 * j) % W];
+        const T o1 = ao[r][(2 * j + 1) / W][(2 * j + 1) % W];
+        T om;
+        if (j == 0)
+          om = before[r];
+        else
+          om = ao[r][(2 * j - 1) / W][(2 * j - 1) % W];
+        const T vm = P ? o0 : om;  // x - 1
+        const T v0 = P ? q1 : q0;  // x
+        const T vp = P ? o1 : o0;  // x + 1
+        const int xc = 2 * (2 * ((int)K0 + j) + P);
+        const bool inm = iny && xc - 1 >= 0, in0 = iny && xc < nf, inp = iny && xc + 1 < nf;
+        const T wm = T(1) * wy * wz, w0 = T(2) * wy * wz, wp = T(1) * wy * wz;
+        if (inm) {
+          sum[j] += vm * wm;
+          wsum[j] += wm;
+        }
+        if (in0) {
+          sum[j] += v0 * w0;
+          wsum[j] += w0;
+        }
+        if (inp) {
+          sum[j] += vp * wp;
+          wsum[j] += wp;
+        }
       }
     }
-  }
-  T out[V];
-  uint32_t mask = 0;
+    T out[V];
+    uint32_t mask = 0;
 #pragma unroll
-  for (int j = 0; j < V; ++j) {
-    out[j] = sum[j] / wsum[j] * scale;
-    if (in && 2 * ((int)K0 + j) + P < gc.n) mask |= 1u << j;
-  }
-  const uint32_t own = (uint32_t)cc * gc.C + s0;
-  store_group<T, T, V>(RC + own, out, mask);
-  if (XC) {
-    T zero[V];
+    for (int j = 0; j < V; ++j) {
+      out[j] = sum[j] / wsum[j] * scale;
+      if (in && 2 * ((int)K0 + j) + P < gc.n) mask |= 1u << j;
+    }
+    const uint32_t own = (uint32_t)cc * gc.C + s0;
+    store_group<T, T, V>(RC + own, out, mask);
+    if (XC) {
+      T zero[V];
 #pragma unroll
-    for (int j = 0; j < V; ++j) zero[j] = T(0);
-    store_group<T, T, V>(XC + own, zero, in ? (1u << V) - 1 : 0u);
+      for (int j = 0; j < V; ++j) zero[j] = T(0);
+      store_group<T, T, V>(XC + own, zero, in ? (1u << V) - 1 : 0u);
+    }
   }
 }
 

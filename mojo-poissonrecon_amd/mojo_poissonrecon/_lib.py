@@ -106,6 +106,7 @@ _PIPELINE = {
     "mpr_batch_destroy": ([P], ctypes.c_int),
     "mpr_batch_run": ([P, ctypes.POINTER(mpr_params)], ctypes.c_int),
     "mpr_batch_capture": ([P, ctypes.POINTER(mpr_params)], ctypes.c_int),
+    "mpr_batch_time_kernel": ([P, S, S, ctypes.POINTER(F), ctypes.POINTER(F)], ctypes.c_int),
     "mpr_batch_launch": ([P], ctypes.c_int),
     "mpr_batch_query": ([P], ctypes.c_int),
     "mpr_batch_wait": ([P, ctypes.POINTER(mpr_result)], ctypes.c_int),
