@@ -213,21 +213,26 @@ int mpr_batch_time_kernel(mpr_batch *batch, int32_t kernel, int32_t reps, double
 int mpr_job_time_kernel(mpr_job *job, int32_t kernel, int32_t reps, double *ms_per_launch,
                         double *bytes_per_launch);
 
-/* Workgroup-size autotuner of the batched slot-group kernels (colour sweep = 0, residual = 1,
+/* Launch-shape autotuner of the batched slot-group kernels (colour sweep = 0, residual = 1,
  * FP64->FP32 defect residual = 2) on levels n = 2^s + 1, s = 5..max_depth (max_depth in
- * [5, 10]): times the workgroup sizes 64, 128, 256, 512 and 1024 per (kernel, precision, s)
- * on up to 8 synthetic instances (within 4 GiB, at least one instance; it allocates and frees
- * them), 7 interleaved rounds of `reps` launches each, and keeps the one with the fastest
- * median round for later launches in this process if it beats the default (256) by 1.5%.
- * Results do not depend on the workgroup size. If
- * `table` is not NULL it receives the chosen sizes, int32[3][2][11] indexed
- * [kernel][fp64][s]. Run it before capturing batch graphs (captured launches keep the size
- * they were recorded with). No reference counterpart (launch tuning). */
-int mpr_autotune(int32_t max_depth, int32_t reps, int32_t *table);
+ * [5, 10]): times the workgroup sizes 64, 128, 256, 512 and 1024 (for the colour sweep also
+ * with 1 or 2 slot groups per lane) per (kernel, precision, s) on up to 8 synthetic
+ * instances (within 4 GiB, at least one instance; it allocates and frees them), 7
+ * interleaved rounds of `reps` launches each, and keeps the shape with the fastest median
+ * round for later launches in this process if it beats the default (256, 1) by 1.5%.
+ * Results do not depend on the launch shape. If `table` / `groups` are not NULL they
+ * receive the chosen workgroup sizes / groups per lane, int32[3][2][11] indexed
+ * [kernel][fp64][s]. Run it before capturing graphs (captured launches keep the shape they
+ * were recorded with). No reference counterpart (launch tuning). */
+int mpr_autotune(int32_t max_depth, int32_t reps, int32_t *table, int32_t *groups);
 /* current workgroup size of (kernel, fp64, s), or -1 */
 int mpr_tune_get(int32_t kernel, int32_t fp64, int32_t size_class);
 /* sets it (block one of 64, 128, 256, 512, 1024); 0 or -1 */
 int mpr_tune_set(int32_t kernel, int32_t fp64, int32_t size_class, int32_t block);
+/* slot groups per lane of (kernel, fp64, s) (1 or 2; only the colour sweep, kernel 0, has
+ * a choice), or -1; and its setter (0 or -1) */
+int mpr_tune_get_groups(int32_t kernel, int32_t fp64, int32_t size_class);
+int mpr_tune_set_groups(int32_t kernel, int32_t fp64, int32_t size_class, int32_t groups);
 
 /* Extraction of a caller grid (core.py:309-374): field and density host (n,n,n) float64
  * (density may be NULL); returns counts; then fetch and destroy. */

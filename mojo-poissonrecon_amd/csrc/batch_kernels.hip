@@ -99,8 +99,10 @@ __global__ __launch_bounds__(kThreads) void kb_residual(const StepPtrs<T>* tab, 
 template <class T>
 constexpr int kGroup = 16 / (int)sizeof(T);
 
-// B: workgroup size (chosen per kernel, precision and size class by the tuner, mpr_tune.h)
-template <class T, int B>
+// B: workgroup size; NG: consecutive slot groups per lane (all of a lane's loads are issued
+// before its arithmetic, so NG = 2 doubles the loads in flight). Both are chosen per
+// precision and size class by the tuner (mpr_tune.h).
+template <class T, int B, int NG>
 __global__ __launch_bounds__(B) void kb_relax_group(const StepPtrs<T>* tab, uint32_t count,
                                                     uint32_t nbi, Geom g, uint32_t lanes, int c,
                                                     T pw, T h2, int zero_start) {
@@ -110,13 +112,19 @@ __global__ __launch_bounds__(B) void kb_relax_group(const StepPtrs<T>* tab, uint
   const uint32_t slot = lb / nbi;
   if (slot >= count) return;
   const StepPtrs<T> p = tab[slot];
-  SlotGroup<T, V> G;
-  load_group<T, V>(as_global((const T*)p.x), as_global(p.rhs), as_global(p.screen), g, c,
-                   (lb - slot * nbi) * B + threadIdx.x, lanes, false, G,
-                   start_zero(p, zero_start), LiveMask{p.live, p.live_rhs != 0});
-  T out[V];
-  const uint32_t mask = group_relax<T, V>(G, g.n, pw, h2, out);
-  store_group<T, T, V>(as_global(p.x) + G.own, out, mask);
+  const uint32_t l0 = ((lb - slot * nbi) * B + threadIdx.x) * NG;
+  const LiveMask live{p.live, p.live_rhs != 0};
+  SlotGroup<T, V> G[NG];
+#pragma unroll
+  for (int k = 0; k < NG; ++k)
+    load_group<T, V>(as_global((const T*)p.x), as_global(p.rhs), as_global(p.screen), g, c,
+                     l0 + (uint32_t)k, lanes, false, G[k], start_zero(p, zero_start), live);
+#pragma unroll
+  for (int k = 0; k < NG; ++k) {
+    T out[V];
+    const uint32_t mask = group_relax<T, V>(G[k], g.n, pw, h2, out);
+    store_group<T, T, V>(as_global(p.x) + G[k].own, out, mask);
+  }
 }
 
 // residual of slot group `lane` in both colours. One lane takes the same slots of both
@@ -488,15 +496,24 @@ void launch_b_relax(const StepPtrs<T>* tab, uint32_t count, const Geom& g, doubl
   if (group_layout(g)) {
     const uint32_t lanes = g.C / (uint32_t)kGroup<T>;
     const int block = tuned_block(kTuneRelax, sizeof(T) == 8, size_class(g));
-    const uint32_t nbi = blocks_for(lanes, (unsigned)block);
+    const int ng = tuned_groups(kTuneRelax, sizeof(T) == 8, size_class(g));
+    const uint32_t nbi = blocks_for((lanes + ng - 1) / ng, (unsigned)block);
     for (int it = 0; it < iterations; ++it)
       for (int c = 0; c < 2; ++c) {
         const int zs = it == 0 && c == 0 ? zero_start : (int)kZeroNone;
-#define MPR_LAUNCH_RELAX(B)                                                                 \
-  hipLaunchKernelGGL((kb_relax_group<T, B>), dim3(padded8((uint64_t)nbi * count)), dim3(B), \
-                     0, s, tab, count, nbi, g, lanes, c, (T)pw, h2, zs)
-        MPR_WITH_BLOCK(block, MPR_LAUNCH_RELAX)
-#undef MPR_LAUNCH_RELAX
+#define MPR_LAUNCH_RELAX_NG(B, NG)                                                 \
+  hipLaunchKernelGGL((kb_relax_group<T, B, NG>), dim3(padded8((uint64_t)nbi * count)), \
+                     dim3(B), 0, s, tab, count, nbi, g, lanes, c, (T)pw, h2, zs)
+#define MPR_LAUNCH_RELAX1(B) MPR_LAUNCH_RELAX_NG(B, 1)
+#define MPR_LAUNCH_RELAX2(B) MPR_LAUNCH_RELAX_NG(B, 2)
+        if (ng == 2) {
+          MPR_WITH_BLOCK(block, MPR_LAUNCH_RELAX2)
+        } else {
+          MPR_WITH_BLOCK(block, MPR_LAUNCH_RELAX1)
+        }
+#undef MPR_LAUNCH_RELAX1
+#undef MPR_LAUNCH_RELAX2
+#undef MPR_LAUNCH_RELAX_NG
         MPR_LAUNCH_CHECK();
       }
     return;

@@ -1,7 +1,8 @@
-// Launch-shape tuning of the batched slot-group kernels: the workgroup size of each kernel,
-// precision and grid size class is looked up in a process-wide table, which the autotuner
-// (mpr_autotune in include/mpr_hip.h) fills by timing the candidates on the device. The
-// per-node arithmetic does not depend on the workgroup size, so results do not change.
+// Launch-shape tuning of the batched slot-group kernels: the workgroup size (and, for the
+// colour sweep, the slot groups per lane) of each kernel, precision and grid size class is
+// looked up in a process-wide table, which the autotuner (mpr_autotune in include/mpr_hip.h)
+// fills by timing the candidates on the device. The per-node arithmetic does not depend on
+// the launch shape, so results do not change.
 #pragma once
 
 #include "mpr_common.h"
@@ -23,5 +24,9 @@ inline int size_class(const Geom& g) {
 int tuned_block(int kernel, bool fp64, int size_class);
 void set_tuned_block(int kernel, bool fp64, int size_class, int block);
 bool valid_block(int block);
+// slot groups per lane (1 or 2; the colour sweep only, 1 for the other kernels)
+constexpr int kTuneGroupCandidates[] = {1, 2};
+int tuned_groups(int kernel, bool fp64, int size_class);
+void set_tuned_groups(int kernel, bool fp64, int size_class, int groups);
 
 }  // namespace mpr

@@ -284,10 +284,18 @@ class BatchResources {
   std::multimap<size_t, void*> buffers_;
 };
 
+// Jobs' own stages (normalise .. pyramid, and residual norm .. readback) are chains of
+// mostly small kernels; a batch runs them on up to kBatchSideStreams side streams forked
+// from and joined back into the batch stream (parallel branches of a captured graph), so the
+// jobs' chains overlap. The V-cycles run on the batch stream, one launch per step for all.
+constexpr size_t kBatchSideStreams = 8;
+
 struct Batch {
   std::vector<Job*> jobs;
   int precision = 0;
   hipStream_t stream = nullptr;
+  std::vector<hipStream_t> side;  // pooled (BatchResources)
+  std::vector<hipEvent_t> events;  // per phase: fork, then one join per side stream
   void* tables = nullptr;  // pooled (BatchResources)
   size_t tables_cap = 0;
   BatchedVcycle<double> vc64;
@@ -300,6 +308,10 @@ struct Batch {
 
   void build();
   void enqueue(const mpr_params* p);
+  // runs work(i, stream) for every job on the side streams between a fork from and a join
+  // into the batch stream; phase selects the events (0 before the V-cycles, 1 after)
+  template <class F>
+  void fork_join(int phase, F&& work);
   void drop_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -309,9 +321,27 @@ struct Batch {
   ~Batch() {
     drop_graph();
     if (tables) BatchResources::get().give_buffer(tables, tables_cap);
+    for (hipStream_t q : side) {
+      (void)hipStreamSynchronize(q);
+      BatchResources::get().give_stream(q);
+    }
+    for (hipEvent_t e : events) (void)hipEventDestroy(e);
     if (stream) BatchResources::get().give_stream(stream);
   }
 };
+
+template <class F>
+void Batch::fork_join(int phase, F&& work) {
+  const size_t k = side.size();
+  hipEvent_t* ev = events.data() + (size_t)phase * (k + 1);
+  MPR_HIP(hipEventRecord(ev[0], stream));
+  for (size_t j = 0; j < k; ++j) MPR_HIP(hipStreamWaitEvent(side[j], ev[0], 0));
+  for (size_t i = 0; i < jobs.size(); ++i) work(i, side[i % k]);
+  for (size_t j = 0; j < k; ++j) {
+    MPR_HIP(hipEventRecord(ev[1 + j], side[j]));
+    MPR_HIP(hipStreamWaitEvent(stream, ev[1 + j], 0));
+  }
+}
 
 void Batch::build() {
   std::vector<InstanceLevels<double>> i64;
@@ -341,16 +371,15 @@ void Batch::build() {
 }
 
 void Batch::enqueue(const mpr_params* p) {
-  hipStream_t s = stream;
-  for (size_t i = 0; i < jobs.size(); ++i) jobs[i]->setup(p[i], s);
+  fork_join(0, [&](size_t i, hipStream_t q) { jobs[i]->setup(p[i], q); });
   if (p[0].stages & MPR_STAGE_SOLVE)
-    run_cycles(vc64, precision == MPR_PRECISION_MIXED ? &vc32 : nullptr, &dtab, p[0], s);
-  for (size_t i = 0; i < jobs.size(); ++i) {
-    jobs[i]->finish(p[i], s);
+    run_cycles(vc64, precision == MPR_PRECISION_MIXED ? &vc32 : nullptr, &dtab, p[0], stream);
+  fork_join(1, [&](size_t i, hipStream_t q) {
+    jobs[i]->finish(p[i], q);
     if (p[i].stages & MPR_STAGE_EXTRACT)
-      jobs[i]->enqueue_extract((p[i].stages & MPR_STAGE_WORLD) != 0, s);
-    jobs[i]->readback(s);
-  }
+      jobs[i]->enqueue_extract((p[i].stages & MPR_STAGE_WORLD) != 0, q);
+    jobs[i]->readback(q);
+  });
 }
 
 // V-cycles of a set of jobs (core.py:274-285): FP64 as the reference, or FP32 V-cycles on
@@ -950,6 +979,13 @@ int mpr_batch_create(mpr_job* const* jobs, int32_t count, mpr_batch** out) {
     b->precision = jobs[0]->precision;
     for (int i = 0; i < count; ++i) b->jobs.push_back(jobs[i]);
     b->stream = BatchResources::get().take_stream();
+    const size_t k = std::min<size_t>(kBatchSideStreams, b->jobs.size());
+    for (size_t j = 0; j < k; ++j) b->side.push_back(BatchResources::get().take_stream());
+    for (size_t e = 0; e < 2 * (k + 1); ++e) {
+      hipEvent_t ev = nullptr;
+      MPR_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      b->events.push_back(ev);
+    }
     for (Job* j : b->jobs) MPR_HIP(hipStreamSynchronize(j->stream));
     b->build();
     *out = b;

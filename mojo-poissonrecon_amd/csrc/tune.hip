@@ -14,12 +14,16 @@ int ensure_device();
 namespace {
 
 std::atomic<int> g_blocks[kTuneKernels][2][kTuneClasses];
+std::atomic<int> g_groups[kTuneKernels][2][kTuneClasses];
 
 struct TableInit {
   TableInit() {
     for (auto& k : g_blocks)
       for (auto& t : k)
         for (auto& c : t) c.store(kTuneDefault);
+    for (auto& k : g_groups)
+      for (auto& t : k)
+        for (auto& c : t) c.store(1);
   }
 } g_table_init;
 
@@ -53,34 +57,48 @@ double time_launches(F&& launch, int reps, hipStream_t s, hipEvent_t e0, hipEven
 constexpr int kTuneRounds = 7;
 constexpr double kTuneMargin = 0.015;
 
+// (workgroup size, slot groups per lane) candidates; the default shape is (256, 1)
+struct Shape {
+  int block, groups;
+};
+
 template <class F>
 void tune_one(int kernel, bool fp64, int cls, F&& launch, int reps, hipStream_t s) {
-  constexpr int kCand = sizeof(kTuneCandidates) / sizeof(int);
+  std::vector<Shape> cand;
+  for (int g : kTuneGroupCandidates) {
+    if (g != 1 && kernel != kTuneRelax) continue;
+    for (int b : kTuneCandidates) cand.push_back({b, g});
+  }
+  const size_t nc = cand.size();
+  auto apply = [&](const Shape& sh) {
+    set_tuned_block(kernel, fp64, cls, sh.block);
+    set_tuned_groups(kernel, fp64, cls, sh.groups);
+  };
   hipEvent_t e0, e1;
   MPR_HIP(hipEventCreate(&e0));
   MPR_HIP(hipEventCreate(&e1));
-  std::vector<double> ms[kCand];
-  for (int c = 0; c < kCand; ++c) {  // warm-up: code objects loaded, caches settled
-    set_tuned_block(kernel, fp64, cls, kTuneCandidates[c]);
+  std::vector<std::vector<double>> ms(nc);
+  for (size_t c = 0; c < nc; ++c) {  // warm-up: code objects loaded, caches settled
+    apply(cand[c]);
     launch();
   }
   for (int round = 0; round < kTuneRounds; ++round)
-    for (int c = 0; c < kCand; ++c) {
-      set_tuned_block(kernel, fp64, cls, kTuneCandidates[c]);
+    for (size_t c = 0; c < nc; ++c) {
+      apply(cand[c]);
       ms[c].push_back(time_launches(launch, reps, s, e0, e1));
     }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  double med[kCand];
-  int best = 0, dflt = 0;
-  for (int c = 0; c < kCand; ++c) {
+  std::vector<double> med(nc);
+  size_t best = 0, dflt = 0;
+  for (size_t c = 0; c < nc; ++c) {
     std::sort(ms[c].begin(), ms[c].end());
     med[c] = ms[c][ms[c].size() / 2];
     if (med[c] < med[best]) best = c;
-    if (kTuneCandidates[c] == kTuneDefault) dflt = c;
+    if (cand[c].block == kTuneDefault && cand[c].groups == 1) dflt = c;
   }
   if (med[best] > (1.0 - kTuneMargin) * med[dflt]) best = dflt;
-  set_tuned_block(kernel, fp64, cls, kTuneCandidates[best]);
+  apply(cand[best]);
 }
 
 // Up to kTuneInstances instances of one class with their own buffers, as a batch runs them,
@@ -141,13 +159,26 @@ void set_tuned_block(int kernel, bool fp64, int size_class, int block) {
   g_blocks[kernel][fp64 ? 1 : 0][size_class].store(block, std::memory_order_relaxed);
 }
 
+int tuned_groups(int kernel, bool fp64, int size_class) {
+  if (kernel < 0 || kernel >= kTuneKernels || size_class < 0 || size_class >= kTuneClasses)
+    return 1;
+  return g_groups[kernel][fp64 ? 1 : 0][size_class].load(std::memory_order_relaxed);
+}
+
+void set_tuned_groups(int kernel, bool fp64, int size_class, int groups) {
+  if (kernel != kTuneRelax || size_class < 0 || size_class >= kTuneClasses ||
+      (groups != 1 && groups != 2))
+    return;
+  g_groups[kernel][fp64 ? 1 : 0][size_class].store(groups, std::memory_order_relaxed);
+}
+
 }  // namespace mpr
 
 using namespace mpr;
 
 extern "C" {
 
-int mpr_autotune(int32_t max_depth, int32_t reps, int32_t* table) {
+int mpr_autotune(int32_t max_depth, int32_t reps, int32_t* table, int32_t* groups) {
   if (max_depth < 5 || max_depth > 10 || reps < 1) {
     set_error("mpr_autotune: max_depth must be in [5, 10] and reps positive");
     return -1;
@@ -167,11 +198,12 @@ int mpr_autotune(int32_t max_depth, int32_t reps, int32_t* table) {
     set_error(e.msg);
     return -2;
   }
-  if (table)
-    for (int k = 0; k < kTuneKernels; ++k)
-      for (int t = 0; t < 2; ++t)
-        for (int c = 0; c < kTuneClasses; ++c)
-          table[(k * 2 + t) * kTuneClasses + c] = tuned_block(k, t == 1, c);
+  for (int k = 0; k < kTuneKernels; ++k)
+    for (int t = 0; t < 2; ++t)
+      for (int c = 0; c < kTuneClasses; ++c) {
+        if (table) table[(k * 2 + t) * kTuneClasses + c] = tuned_block(k, t == 1, c);
+        if (groups) groups[(k * 2 + t) * kTuneClasses + c] = tuned_groups(k, t == 1, c);
+      }
   return 0;
 }
 
@@ -190,6 +222,24 @@ int mpr_tune_set(int32_t kernel, int32_t fp64, int32_t size_class, int32_t block
     return -1;
   }
   set_tuned_block(kernel, fp64 != 0, size_class, block);
+  return 0;
+}
+
+int mpr_tune_get_groups(int32_t kernel, int32_t fp64, int32_t size_class) {
+  if (kernel < 0 || kernel >= kTuneKernels || size_class < 0 || size_class >= kTuneClasses) {
+    set_error("mpr_tune_get_groups: no such kernel or size class");
+    return -1;
+  }
+  return tuned_groups(kernel, fp64 != 0, size_class);
+}
+
+int mpr_tune_set_groups(int32_t kernel, int32_t fp64, int32_t size_class, int32_t groups) {
+  if (kernel != kTuneRelax || size_class < 0 || size_class >= kTuneClasses ||
+      (groups != 1 && groups != 2)) {
+    set_error("mpr_tune_set_groups: kernel 0 (colour sweep), size class 0..10, groups 1 or 2");
+    return -1;
+  }
+  set_tuned_groups(kernel, fp64 != 0, size_class, groups);
   return 0;
 }
 

@@ -186,3 +186,12 @@ def test_workgroup_size_table_without_a_device():
             tune.set_tuned_block("residual", "fp64", 6, bad)
     with pytest.raises(DeviceError):
         tune.tuned_block("defect", "fp64", 11)
+    assert tune.tuned_groups("relax", "fp32", 8) == 1
+    tune.set_tuned_groups("relax", "fp32", 8, 2)
+    try:
+        assert tune.tuned_groups("relax", "fp32", 8) == 2
+    finally:
+        tune.set_tuned_groups("relax", "fp32", 8, 1)
+    for kernel, groups in (("relax", 3), ("residual", 2)):
+        with pytest.raises(DeviceError):
+            tune.set_tuned_groups(kernel, "fp32", 8, groups)

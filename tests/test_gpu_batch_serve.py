@@ -155,23 +155,28 @@ def test_server_admission_respects_memory_budget():
     server.close()
 
 
-def test_workgroup_sizes_do_not_change_results():
+def test_launch_shapes_do_not_change_results():
     from mojo_poissonrecon import tune
     p, n = shapes.torus_workload(20000, seed=5)
-    saved = {(k, t, s): tune.tuned_block(k, t, s)
-             for k in tune.KERNELS for t in ("fp32", "fp64") for s in range(5, 8)}
+    keys = [(k, t, s) for k in tune.KERNELS for t in ("fp32", "fp64") for s in range(5, 8)]
+    saved = {key: (tune.tuned_block(*key), tune.tuned_groups(*key)) for key in keys}
     try:
         base = {prec: mpr.reconstruct(p, n, depth=7, precision=prec) for prec in ("fp64", "mixed")}
-        for block in (64, 1024):
-            for key in saved:
+        for block, groups in ((64, 2), (1024, 1), (128, 2)):
+            for key in keys:
                 tune.set_tuned_block(*key, block)
+                if key[0] == "relax":
+                    tune.set_tuned_groups(*key, groups)
             for prec, m in base.items():
                 identical_mesh(mpr.reconstruct(p, n, depth=7, precision=prec), m)
         chosen = tune.autotune(max_depth=7, reps=3)
-        assert set(chosen.values()) <= set(tune.CANDIDATES)
+        assert {b for b, _ in chosen.values()} <= set(tune.CANDIDATES)
+        assert {g for _, g in chosen.values()} <= set(tune.GROUP_CANDIDATES)
         assert len(chosen) == 3 * 2 + 3 * 2 + 3  # relax, residual x2 precisions; defect fp64
         for prec, m in base.items():
             identical_mesh(mpr.reconstruct(p, n, depth=7, precision=prec), m)
     finally:
-        for key, b in saved.items():
+        for key, (b, g) in saved.items():
             tune.set_tuned_block(*key, b)
+            if key[0] == "relax":
+                tune.set_tuned_groups(*key, g)
