@@ -107,7 +107,7 @@ int64_t mpr_splat_f64(const double* points, const double* normals, int64_t point
     h2d(s.p, screen, M);
     launch_fill<unsigned long long>(acc.p, 0ull, 1, kNull);
     SplatScratch sc = splat_scratch_sizes((uint32_t)point_count);
-    DevBuf<uint64_t> keys(sc.pairs), ktmp(sc.pairs);
+    DevBuf<uint32_t> keys(sc.pairs), ktmp(sc.pairs);
     DevBuf<uint32_t> ids(sc.pairs), itmp(sc.pairs), hist(sc.radix_words), offs(sc.radix_words);
     DevBuf<double> w1(sc.pairs), c2(3ull * sc.pairs);
     sc.keys = keys.p;

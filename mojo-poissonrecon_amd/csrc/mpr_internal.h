@@ -87,8 +87,9 @@ void launch_scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t count, ui
 // words of `scratch` that let launch_scan_exclusive use many workgroups for `count` values
 uint32_t scan_scratch_words(uint32_t count);
 size_t radix_scratch_words(uint32_t n);
-void launch_radix_sort(uint64_t* keys, uint32_t* vals, uint64_t* k_tmp, uint32_t* v_tmp,
-                       uint32_t n, int bits, uint32_t* hist, uint32_t* offs, hipStream_t s);
+template <class K>  // uint32_t (splat node keys) or uint64_t (Morton keys)
+void launch_radix_sort(K* keys, uint32_t* vals, K* k_tmp, uint32_t* v_tmp, uint32_t n, int bits,
+                       uint32_t* hist, uint32_t* offs, hipStream_t s);
 void launch_leaf_keys(const double* pts, uint32_t P, int depth, uint64_t* keys, uint32_t* perm,
                       hipStream_t s);
 uint32_t compaction_tiles(uint64_t n);
@@ -111,7 +112,7 @@ constexpr int64_t kMaxPoints = int64_t(1) << 28;
 struct SplatScratch {
   uint32_t pairs = 0;        // 8 per point
   size_t radix_words = 0;    // words of hist and of offs
-  uint64_t *keys = nullptr, *ktmp = nullptr;
+  uint32_t *keys = nullptr, *ktmp = nullptr;  // node index of each (node, point) pair
   uint32_t *ids = nullptr, *itmp = nullptr, *hist = nullptr, *offs = nullptr;
   double* w1 = nullptr;  // pairs
   double* c2 = nullptr;  // 3 * pairs

@@ -155,9 +155,9 @@ __global__ __launch_bounds__(kThreads) void k_leaf_keys(const double* pts, uint3
 constexpr int kRadixItems = 8;                        // items per thread per tile
 constexpr int kRadixTile = kThreads * kRadixItems;    // 2048 keys per block
 
-__global__ __launch_bounds__(kThreads) void k_radix_hist(const uint64_t* keys, uint32_t n,
-                                                         int shift, uint32_t* hist,
-                                                         uint32_t nblocks) {
+template <class K>
+__global__ __launch_bounds__(kThreads) void k_radix_hist(const K* keys, uint32_t n, int shift,
+                                                         uint32_t* hist, uint32_t nblocks) {
   __shared__ uint32_t h[256];
   h[threadIdx.x] = 0;
   __syncthreads();
@@ -170,10 +170,10 @@ __global__ __launch_bounds__(kThreads) void k_radix_hist(const uint64_t* keys, u
   hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];  // digit-major
 }
 
-__global__ __launch_bounds__(kThreads) void k_radix_scatter(const uint64_t* kin,
-                                                            const uint32_t* vin, uint64_t* kout,
-                                                            uint32_t* vout, uint32_t n, int shift,
-                                                            const uint32_t* offsets,
+template <class K>
+__global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* kin, const uint32_t* vin,
+                                                            K* kout, uint32_t* vout, uint32_t n,
+                                                            int shift, const uint32_t* offsets,
                                                             uint32_t nblocks) {
   __shared__ uint32_t wave_cnt[4][256];
   __shared__ uint32_t running[256];
@@ -185,7 +185,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const uint64_t* kin,
     __syncthreads();
     const uint32_t i = base + r * kThreads + threadIdx.x;
     const bool valid = i < n;
-    uint64_t key = 0;
+    K key = 0;
     uint32_t val = 0, digit = 0;
     if (valid) {
       key = kin[i];
@@ -392,8 +392,8 @@ __global__ __launch_bounds__(1024) void k_scan_chunks(const uint32_t* in, uint32
 // per point: accepted? and the trilinear cell of pass 1 (capi.mojo:127-173)
 __global__ __launch_bounds__(kThreads) void k_splat_pairs(const double* pts, const double* nrm,
                                                           uint32_t P, int n, int confidence,
-                                                          uint64_t* keys, uint32_t* ids,
-                                                          double* w1, uint64_t sentinel,
+                                                          uint32_t* keys, uint32_t* ids,
+                                                          double* w1, uint32_t sentinel,
                                                           unsigned long long* accepted) {
   __shared__ uint32_t lds4[4];
   const uint32_t p = blockIdx.x * kThreads + threadIdx.x;
@@ -422,7 +422,7 @@ __global__ __launch_bounds__(kThreads) void k_splat_pairs(const double* pts, con
           const double wy = dy ? fy : 1.0 - fy;
           for (int dx = 0; dx < 2; ++dx, ++k) {
             const double wx = dx ? fx : 1.0 - fx;
-            keys[base + k] = (uint64_t)nat_index((int)ix + dx, (int)iy + dy, (int)iz + dz, n);
+            keys[base + k] = (uint32_t)nat_index((int)ix + dx, (int)iy + dy, (int)iz + dz, n);
             ids[base + k] = (uint32_t)(base + k);
             w1[base + k] = wx * wy * wz * sw;
           }
@@ -442,14 +442,14 @@ __global__ __launch_bounds__(kThreads) void k_splat_pairs(const double* pts, con
 // one lane per node: density += 2 w and screen += w over its sorted contributions. With
 // `touched`, the grids count as zero-filled (the sums start at +0.0, as on the reference's
 // fresh grids) and the node's bit is set instead of reading the grid.
-__global__ __launch_bounds__(kThreads) void k_splat_sum1(const uint64_t* keys,
+__global__ __launch_bounds__(kThreads) void k_splat_sum1(const uint32_t* keys,
                                                          const uint32_t* ids, const double* w1,
-                                                         uint32_t count, uint64_t M,
+                                                         uint32_t count, uint32_t M,
                                                          double* density, double* screen,
                                                          uint32_t* touched) {
   const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
   if (j >= count) return;
-  const uint64_t key = keys[j];
+  const uint32_t key = keys[j];
   if (key >= M || (j > 0 && keys[j - 1] == key)) return;
   double d = 0.0, sc = 0.0;
   if (touched) {
@@ -526,13 +526,13 @@ __global__ __launch_bounds__(kThreads) void k_splat_pass2_pairs(const double* pt
 
 // one lane per node: v -= n^ vw over its sorted contributions (from +0.0 when the grids
 // count as zero-filled, as k_splat_sum1)
-__global__ __launch_bounds__(kThreads) void k_splat_sum2(const uint64_t* keys,
+__global__ __launch_bounds__(kThreads) void k_splat_sum2(const uint32_t* keys,
                                                          const uint32_t* ids, const double* c2,
-                                                         uint32_t count, uint64_t M, double* vx,
+                                                         uint32_t count, uint32_t M, double* vx,
                                                          double* vy, double* vz, int from_zero) {
   const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
   if (j >= count) return;
-  const uint64_t key = keys[j];
+  const uint32_t key = keys[j];
   if (key >= M || (j > 0 && keys[j - 1] == key)) return;
   double a = 0.0, b = 0.0, c = 0.0;
   if (!from_zero) {
@@ -672,29 +672,35 @@ size_t radix_scratch_words(uint32_t n) {
 
 // Sorts (keys, vals) by the low `bits` key bits, stably. The result ends in (keys, vals);
 // (k_tmp, v_tmp) are ping-pong buffers; hist and offs hold radix_scratch_words(n) words.
-void launch_radix_sort(uint64_t* keys, uint32_t* vals, uint64_t* k_tmp, uint32_t* v_tmp,
-                       uint32_t n, int bits, uint32_t* hist, uint32_t* offs, hipStream_t s) {
+template <class K>
+void launch_radix_sort(K* keys, uint32_t* vals, K* k_tmp, uint32_t* v_tmp, uint32_t n, int bits,
+                       uint32_t* hist, uint32_t* offs, hipStream_t s) {
   if (n <= 1) return;
+  if (bits > 8 * (int)sizeof(K)) throw HipError{"radix sort: more key bits than the key holds"};
   const uint32_t nb = (n + kRadixTile - 1) / kRadixTile;
   const int passes = (bits + 7) / 8;
-  uint64_t *ka = keys, *kb = k_tmp;
+  K *ka = keys, *kb = k_tmp;
   uint32_t *va = vals, *vb = v_tmp;
   for (int p = 0; p < passes; ++p) {
     const int shift = 8 * p;
-    hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kThreads), 0, s, ka, n, shift, hist, nb);
+    hipLaunchKernelGGL(k_radix_hist<K>, dim3(nb), dim3(kThreads), 0, s, ka, n, shift, hist, nb);
     MPR_LAUNCH_CHECK();
     launch_scan_exclusive(hist, offs, 256u * nb, nullptr, s, hist + 256ull * nb);
-    hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kThreads), 0, s, ka, va, kb, vb, n, shift,
-                       offs, nb);
+    hipLaunchKernelGGL(k_radix_scatter<K>, dim3(nb), dim3(kThreads), 0, s, ka, va, kb, vb, n,
+                       shift, offs, nb);
     MPR_LAUNCH_CHECK();
     std::swap(ka, kb);
     std::swap(va, vb);
   }
   if (ka != keys) {
-    MPR_HIP(hipMemcpyAsync(keys, ka, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, s));
+    MPR_HIP(hipMemcpyAsync(keys, ka, sizeof(K) * n, hipMemcpyDeviceToDevice, s));
     MPR_HIP(hipMemcpyAsync(vals, va, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
   }
 }
+template void launch_radix_sort<uint32_t>(uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t,
+                                          int, uint32_t*, uint32_t*, hipStream_t);
+template void launch_radix_sort<uint64_t>(uint64_t*, uint32_t*, uint64_t*, uint32_t*, uint32_t,
+                                          int, uint32_t*, uint32_t*, hipStream_t);
 
 void launch_leaf_keys(const double* pts, uint32_t P, int depth, uint64_t* keys, uint32_t* perm,
                       hipStream_t s) {
@@ -754,9 +760,10 @@ void launch_splat(const double* pts, const double* nrm, uint32_t P, int n, int c
   if (!P) return;
   const uint32_t pairs = 8u * P;
   if (pairs > b.pairs) throw HipError{"splat scratch smaller than the point count"};
-  const uint64_t M = (uint64_t)n * n * n;
+  // node keys and the sentinel M (sorts after every node) fit 32 bits: M <= 1025^3 < 2^31
+  const uint32_t M = (uint32_t)((uint64_t)n * n * n);
   int bits = 1;
-  while ((1ull << bits) <= M) ++bits;  // the sentinel M sorts after every node
+  while ((1ull << bits) <= M) ++bits;
   const unsigned nb = blocks_for(P, kThreads), np = blocks_for(pairs, kThreads);
   hipLaunchKernelGGL(k_splat_pairs, dim3(nb), dim3(kThreads), 0, s, pts, nrm, P, n, confidence,
                      b.keys, b.ids, b.w1, M, accepted);

@@ -451,8 +451,8 @@ void Job::plan(DevArena& a) {
   need_bits = a.alloc<uint32_t>((M + 31) / 32);
   live_bits = a.alloc<uint32_t>(live_words());
   splat = splat_scratch_sizes(std::max<uint32_t>(1u, max_points));
-  splat.keys = a.alloc<uint64_t>(splat.pairs);
-  splat.ktmp = a.alloc<uint64_t>(splat.pairs);
+  splat.keys = a.alloc<uint32_t>(splat.pairs);
+  splat.ktmp = a.alloc<uint32_t>(splat.pairs);
   splat.ids = a.alloc<uint32_t>(splat.pairs);
   splat.itmp = a.alloc<uint32_t>(splat.pairs);
   splat.hist = a.alloc<uint32_t>(splat.radix_words);
@@ -1128,6 +1128,7 @@ static double pipeline_kernel_bytes(const mpr_job* job, int kernel) {
   const Geom& g0 = job->geoms[0];
   const double M0 = (double)g0.n * g0.n * g0.n;
   const bool skip = group_layout(g0);
+  // slots per lane: 4 in FP32, 2 in FP64
   const double live = skip ? live_group_fraction(job, kernel == 8 || kernel == 13 ? 2 : 4) : 1.0;
   const double mask = skip ? M0 / 8.0 : 0.0;
   switch (kernel) {

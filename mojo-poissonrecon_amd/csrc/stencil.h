@@ -338,7 +338,8 @@ __device__ __forceinline__ void group_residual(const SlotGroup<T, V>& G, int n, 
 }
 
 // Stores V values at p (16-byte aligned): whole chunks when every slot is stored, else
-// slot by slot.
+// slot by slot. Values narrower than T (the FP64 defect written as FP32) go slot by slot,
+// which measured faster than one converted vector store.
 template <class T, class U, int V>
 __device__ __forceinline__ void store_group(__attribute__((address_space(1))) U* p,
                                             const T (&v)[V], uint32_t mask) {
