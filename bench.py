@@ -41,7 +41,7 @@ HBM_PEAK_GBS = 8000.0
 # 257^3 finest level of all instances of the batch), keyed by (depth, batch):
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from rocprofv3 counter runs, the factors calibrated on a
 # plain elementwise copy (profiles/kb_relax_group_f32_depth8_pmc.txt).
-ROOFLINE_TRAFFIC_BYTES = {}
+ROOFLINE_TRAFFIC_BYTES = {(8, 8): 872_873_816}
 
 
 def log(*args):

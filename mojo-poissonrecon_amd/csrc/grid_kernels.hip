@@ -257,6 +257,19 @@ __device__ __forceinline__ uint32_t bits_at(const uint32_t* a, const uint32_t* b
   return (lo >> r) | (hi << (32u - r));
 }
 
+// any of bits [b, b + len) set (len <= 64, all inside the mask)
+__device__ __forceinline__ bool any_bit(const uint32_t* w, uint32_t b, uint32_t len) {
+  const uint32_t e = b + len, q0 = b >> 5, qe = (e - 1) >> 5;
+  uint32_t acc = 0;
+  for (uint32_t q = q0; q <= qe; ++q) {
+    uint32_t m = w[q];
+    if (q == q0) m &= ~0u << (b & 31u);
+    if (q == qe && (e & 31u)) m &= (1u << (e & 31u)) - 1u;
+    acc |= m;
+  }
+  return acc != 0;
+}
+
 // need = nodes whose 7-point stencil (linear offsets 0, +-1, +-n, +-n^2) meets a node of
 // cur or prev. Linear offsets also reach across row and plane ends, which only adds nodes
 // (a superset is harmless: an added node is written with the value the assembly computes).
@@ -272,56 +285,102 @@ __global__ __launch_bounds__(kThreads) void k_need_mask(const uint32_t* cur, con
             bits_at(cur, prev, b - sz, words);
 }
 
-__global__ __launch_bounds__(kThreads) void k_assemble(const double* vx, const double* vy,
-                                                       const double* vz, const double* scr,
-                                                       double* rhs_nat, double* rhs_split,
-                                                       double* scr_split, double* density,
-                                                       double spn, Geom g, FastDiv div_n,
-                                                       double pw, double target,
-                                                       const uint32_t* touched,
-                                                       double* scr_nat_out, float* scr_split32,
-                                                       const uint32_t* need, uint32_t* prev,
-                                                       uint32_t* live) {
-  const uint32_t t = logical_block() * kThreads + threadIdx.x;
+// outputs and parameters of the assembly (capi.mojo:227-279); null outputs are not written
+struct AssembleArgs {
+  const double *vx, *vy, *vz, *scr;
+  double *rhs_nat, *rhs_split, *scr_split, *density, *scr_nat_out;
+  float* scr_split32;
+  const uint32_t* touched;
+  uint32_t* live;
+  double spn, pw, target;
+};
+
+// assembly of node t = (x, y, z)
+__device__ __forceinline__ void assemble_node(const AssembleArgs& a, const Geom& g, uint32_t t,
+                                              int x, int y, int z) {
   const int n = g.n;
-  if (t >= (uint32_t)n * n * n) return;
-  // record this run's touched bits for the next run's need mask (this run's need mask was
-  // computed from the old ones by the previous kernel), then skip nodes outside it
-  if (prev && (t & 31u) == 0) prev[t >> 5] = touched[t >> 5];
-  if (need && !((need[t >> 5] >> (t & 31u)) & 1u)) return;
-  uint32_t zy = div_n.div(t);
-  const int x = (int)(t - zy * (uint32_t)n);
-  const int z = (int)div_n.div(zy);
-  const int y = (int)(zy - (uint32_t)z * (uint32_t)n);
   const double h_inv = (double)(n - 1);
   const double h_inv2 = h_inv * h_inv;
   const int64_t sy = n, sz = (int64_t)n * n;
-  const uint32_t* tb = touched;
+  const uint32_t* tb = a.touched;
   double div = 0.0;
-  if (x == 0) div += (splat_value(vx, tb, t + 1) - splat_value(vx, tb, t)) * h_inv;
-  else if (x == n - 1) div += (splat_value(vx, tb, t) - splat_value(vx, tb, t - 1)) * h_inv;
-  else div += (splat_value(vx, tb, t + 1) - splat_value(vx, tb, t - 1)) * (0.5 * h_inv);
-  if (y == 0) div += (splat_value(vy, tb, t + sy) - splat_value(vy, tb, t)) * h_inv;
-  else if (y == n - 1) div += (splat_value(vy, tb, t) - splat_value(vy, tb, t - sy)) * h_inv;
-  else div += (splat_value(vy, tb, t + sy) - splat_value(vy, tb, t - sy)) * (0.5 * h_inv);
-  if (z == 0) div += (splat_value(vz, tb, t + sz) - splat_value(vz, tb, t)) * h_inv;
-  else if (z == n - 1) div += (splat_value(vz, tb, t) - splat_value(vz, tb, t - sz)) * h_inv;
-  else div += (splat_value(vz, tb, t + sz) - splat_value(vz, tb, t - sz)) * (0.5 * h_inv);
-  const double s = splat_value(scr, tb, t);
-  const double screening = pw * s * h_inv2;
-  const double r = -div + screening * target;
-  if (rhs_nat) rhs_nat[t] = r;
-  if (scr_nat_out) scr_nat_out[t] = s;
-  if (rhs_split) {
+  if (x == 0) div += (splat_value(a.vx, tb, t + 1) - splat_value(a.vx, tb, t)) * h_inv;
+  else if (x == n - 1) div += (splat_value(a.vx, tb, t) - splat_value(a.vx, tb, t - 1)) * h_inv;
+  else div += (splat_value(a.vx, tb, t + 1) - splat_value(a.vx, tb, t - 1)) * (0.5 * h_inv);
+  if (y == 0) div += (splat_value(a.vy, tb, t + sy) - splat_value(a.vy, tb, t)) * h_inv;
+  else if (y == n - 1) div += (splat_value(a.vy, tb, t) - splat_value(a.vy, tb, t - sy)) * h_inv;
+  else div += (splat_value(a.vy, tb, t + sy) - splat_value(a.vy, tb, t - sy)) * (0.5 * h_inv);
+  if (z == 0) div += (splat_value(a.vz, tb, t + sz) - splat_value(a.vz, tb, t)) * h_inv;
+  else if (z == n - 1) div += (splat_value(a.vz, tb, t) - splat_value(a.vz, tb, t - sz)) * h_inv;
+  else div += (splat_value(a.vz, tb, t + sz) - splat_value(a.vz, tb, t - sz)) * (0.5 * h_inv);
+  const double s = splat_value(a.scr, tb, t);
+  const double screening = a.pw * s * h_inv2;
+  const double r = -div + screening * a.target;
+  if (a.rhs_nat) a.rhs_nat[t] = r;
+  if (a.scr_nat_out) a.scr_nat_out[t] = s;
+  if (a.rhs_split) {
     const uint32_t si = split_index(g, x, y, z);
-    rhs_split[si] = r;
-    scr_split[si] = s;
-    if (scr_split32) scr_split32[si] = (float)s;
+    a.rhs_split[si] = r;
+    a.scr_split[si] = s;
+    if (a.scr_split32) a.scr_split32[si] = (float)s;
     // live slots: rhs or screen not +0.0 (bit patterns, so -0.0 counts as live)
-    if (live && (__double_as_longlong(r) | __double_as_longlong(s)) != 0)
-      atomicOr(&live[si >> 5], 1u << (si & 31u));
+    if (a.live && (__double_as_longlong(r) | __double_as_longlong(s)) != 0)
+      atomicOr(&a.live[si >> 5], 1u << (si & 31u));
   }
-  if (density) density[t] = splat_value(density, tb, t) / spn;
+  if (a.density) a.density[t] = splat_value(a.density, tb, t) / a.spn;
+}
+
+// dense assembly: every node; with `prev`, also records the touched bits for the next run
+__global__ __launch_bounds__(kThreads) void k_assemble(AssembleArgs a, Geom g, FastDiv div_n,
+                                                       uint32_t* prev) {
+  const uint32_t t = logical_block() * kThreads + threadIdx.x;
+  const int n = g.n;
+  if (t >= (uint32_t)n * n * n) return;
+  if (prev && (t & 31u) == 0) prev[t >> 5] = a.touched[t >> 5];
+  const uint32_t zy = div_n.div(t);
+  const int x = (int)(t - zy * (uint32_t)n);
+  const int z = (int)div_n.div(zy);
+  const int y = (int)(zy - (uint32_t)z * (uint32_t)n);
+  assemble_node(a, g, t, x, y, z);
+}
+
+// Sparse assembly, step 1: lists the 64-node row chunks (chunk k of row zy: x in
+// [64k, 64k+64)) that hold a node of the need mask (list order is free: chunks are
+// independent), and records this run's touched bits for the next run's need mask (the need
+// mask was computed from the old ones by the previous kernel).
+__global__ __launch_bounds__(kThreads) void k_need_chunks(const uint32_t* need,
+                                                          const uint32_t* touched,
+                                                          uint32_t* prev, uint32_t words, int n,
+                                                          uint32_t per_row, uint32_t chunks,
+                                                          uint32_t* list, uint32_t* count) {
+  const uint32_t tid = blockIdx.x * kThreads + threadIdx.x;
+  for (uint32_t w = tid; w < words; w += gridDim.x * kThreads) prev[w] = touched[w];
+  if (tid >= chunks) return;
+  const uint32_t zy = tid / per_row, k = tid - zy * per_row;
+  const uint32_t x0 = 64u * k;
+  const uint32_t len = min(64u, (uint32_t)n - x0);
+  if (any_bit(need, zy * (uint32_t)n + x0, len)) list[atomicAdd(count, 1u)] = tid;
+}
+
+// Sparse assembly, step 2: one wavefront per listed chunk (a lane per node), for every
+// listed chunk: nodes of a chunk outside the need mask get the assembly's own value, +0.0
+__global__ __launch_bounds__(kThreads) void k_assemble_chunks(AssembleArgs a, Geom g,
+                                                              uint32_t per_row,
+                                                              const uint32_t* list,
+                                                              const uint32_t* count) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t waves = gridDim.x * (kThreads / 64);
+  const uint32_t total = *count;
+  const int n = g.n;
+  for (uint32_t i = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); i < total; i += waves) {
+    const uint32_t chunk = list[i];
+    const uint32_t zy = chunk / per_row, k = chunk - zy * per_row;
+    const int x = (int)(64u * k + lane);
+    if (x >= n) continue;
+    const int z = (int)(zy / (uint32_t)n);
+    const int y = (int)(zy - (uint32_t)z * (uint32_t)n);
+    assemble_node(a, g, zy * (uint32_t)n + (uint32_t)x, x, y, z);
+  }
 }
 
 }  // namespace
@@ -454,28 +513,35 @@ void launch_assemble(const double* vx, const double* vy, const double* vz,
                      float* screen_split32, const TouchedHistory* hist, uint32_t* live,
                      hipStream_t s) {
   const uint64_t m = (uint64_t)g.n * g.n * g.n;
-  const uint32_t* need = nullptr;
-  uint32_t* prev = nullptr;
-  if (hist) {
-    if (!touched || !rhs_split) throw HipError{"assembly history needs the touched bits"};
-    prev = hist->prev;
-  }
-  if (hist && !hist->write_all) {
-    // only the split outputs keep the zero-input value at untouched nodes; the
-    // natural-layout outputs are written by the dense assembly
-    if (rhs_nat || density || screen_nat_out)
-      throw HipError{"sparse assembly writes the split outputs only"};
-    const int64_t words = (int64_t)((m + 31) / 32);
-    hipLaunchKernelGGL(k_need_mask, dim3(blocks_for((uint64_t)words, kThreads)), dim3(kThreads),
-                       0, s, touched, hist->prev, words, (int64_t)g.n, (int64_t)g.n * g.n,
-                       hist->need);
+  const AssembleArgs a{vx, vy, vz, screen_nat, rhs_nat, rhs_split, screen_split, density,
+                       screen_nat_out, screen_split32, touched, live, spn, pw, target};
+  if (hist && (!touched || !rhs_split)) throw HipError{"assembly history needs the touched bits"};
+  if (!hist || hist->write_all) {
+    hipLaunchKernelGGL(k_assemble, dim3(xcd_grid(m, kThreads)), dim3(kThreads), 0, s, a, g,
+                       FastDiv((uint32_t)g.n), hist ? hist->prev : nullptr);
     MPR_LAUNCH_CHECK();
-    need = hist->need;
+    return;
   }
-  const unsigned nb = xcd_grid(m, kThreads);
-  hipLaunchKernelGGL(k_assemble, dim3(nb), dim3(kThreads), 0, s, vx, vy, vz, screen_nat,
-                     rhs_nat, rhs_split, screen_split, density, spn, g, FastDiv((uint32_t)g.n),
-                     pw, target, touched, screen_nat_out, screen_split32, need, prev, live);
+  // only the split outputs keep the zero-input value at untouched nodes; the natural-layout
+  // outputs are written by the dense assembly
+  if (rhs_nat || density || screen_nat_out)
+    throw HipError{"sparse assembly writes the split outputs only"};
+  const uint32_t words = (uint32_t)((m + 31) / 32);
+  hipLaunchKernelGGL(k_need_mask, dim3(blocks_for(words, kThreads)), dim3(kThreads), 0, s,
+                     touched, hist->prev, (int64_t)words, (int64_t)g.n, (int64_t)g.n * g.n,
+                     hist->need);
+  MPR_LAUNCH_CHECK();
+  const uint32_t per_row = assemble_chunks_per_row(g.n);
+  const uint32_t chunks = (uint32_t)g.n * (uint32_t)g.n * per_row;
+  launch_fill<uint32_t>(hist->count, 0u, 1, s);
+  hipLaunchKernelGGL(k_need_chunks, dim3(blocks_for(chunks, kThreads)), dim3(kThreads), 0, s,
+                     hist->need, touched, hist->prev, words, g.n, per_row, chunks, hist->list,
+                     hist->count);
+  MPR_LAUNCH_CHECK();
+  // a fixed grid (graph-capturable) whose wavefronts walk the chunk list
+  const unsigned nb = std::min<unsigned>(blocks_for(chunks, kThreads / 64), 4096u);
+  hipLaunchKernelGGL(k_assemble_chunks, dim3(nb), dim3(kThreads), 0, s, a, g, per_row,
+                     hist->list, hist->count);
   MPR_LAUNCH_CHECK();
 }
 

@@ -58,8 +58,9 @@ void launch_add_correction(double* X, const float* E, uint64_t count, hipStream_
 // Sparse assembly (hist != null, write_all false; split outputs only, density left as
 // splatted): a node whose 7-point stencil holds no touched node gets
 // rhs = -(+0) + pw * (+0) * h^-2 * target = +0.0 and screen = +0.0 (pw >= +0, target >= 0),
-// the same in every run. So only nodes whose stencil meets this run's or the previous run's
-// touched nodes are written; every other node keeps the +0.0 an earlier run wrote.
+// the same in every run. So only the 64-node row chunks holding a node whose stencil meets
+// this run's or the previous run's touched nodes are written, one wavefront per listed
+// chunk; every other node keeps the +0.0 an earlier run wrote.
 // `prev` holds the previous assembly's touched bits (all ones before the first, so that one
 // writes every node) and is updated in place by every assembly given `hist` (write_all
 // true: a dense assembly that still records its touched bits); `need` is scratch of the
@@ -69,7 +70,10 @@ struct TouchedHistory {
   uint32_t* prev;
   uint32_t* need;
   bool write_all;
+  uint32_t* list;   // n * n * assemble_chunks_per_row(n) words: the 64-node row chunks written
+  uint32_t* count;  // one word: their number
 };
+inline uint32_t assemble_chunks_per_row(int n) { return (uint32_t)(n + 63) / 64u; }
 void launch_assemble(const double* vx, const double* vy, const double* vz,
                      const double* screen_nat, double* rhs_nat, double* rhs_split,
                      double* screen_split, double* density, double spn, const Geom& g,

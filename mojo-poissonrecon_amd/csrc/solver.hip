@@ -105,6 +105,7 @@ struct Job {
   uint32_t* touched = nullptr;  // splat node bits, ceil(n^3 / 32) words
   // the previous assembly's touched bits and the sparse assembly's need mask (same size)
   uint32_t *prev_touched = nullptr, *need_bits = nullptr;
+  uint32_t *chunk_list = nullptr, *chunk_count = nullptr;  // the sparse assembly's chunks
   // live slots of the finest level (LiveMask, stencil.h): rhs or screen non-zero
   uint32_t* live_bits = nullptr;
   // how the density grid holds the last solve's values: raw splat sums read through
@@ -449,6 +450,8 @@ void Job::plan(DevArena& a) {
   touched = a.alloc<uint32_t>((M + 31) / 32);
   prev_touched = a.alloc<uint32_t>((M + 31) / 32);
   need_bits = a.alloc<uint32_t>((M + 31) / 32);
+  chunk_list = a.alloc<uint32_t>((uint64_t)n * n * assemble_chunks_per_row(n));
+  chunk_count = a.alloc<uint32_t>(1);
   live_bits = a.alloc<uint32_t>(live_words());
   splat = splat_scratch_sizes(std::max<uint32_t>(1u, max_points));
   splat.keys = a.alloc<uint32_t>(splat.pairs);
@@ -530,7 +533,7 @@ void Job::setup(const mpr_params& p, hipStream_t s) {
   // FP32 screening weights of the mixed V-cycle are written beside the FP64 ones.
   // The live-slot bits let the finest level's kernels skip loading the +0.0 rhs and screen
   // words away from the splat.
-  const TouchedHistory hist{prev_touched, need_bits, grids};
+  const TouchedHistory hist{prev_touched, need_bits, grids, chunk_list, chunk_count};
   launch_fill<uint32_t>(live_bits, 0u, live_words(), s);
   launch_assemble(vx, vy, vz, screen, grids ? rhs : nullptr, L64[0].rhs, L64[0].screen,
                   grids ? density : nullptr, p.samples_per_node, geoms[0], p.point_weight, 0.5,

@@ -42,9 +42,9 @@ def test_solve_matches_oracle(depth, count, confidence):
         assert np.array_equal(getattr(got.octree, name), tree[name]), name
 
 
-def test_solve_is_bit_exact_when_splat_has_no_collisions():
-    # points in distinct cells far apart: no two atomic adds hit one node, so the whole
-    # solve is the reference's arithmetic and must match the oracle bit for bit
+def test_solve_of_isolated_points_is_bit_exact():
+    # points in distinct cells far apart (every node receives one point's contributions):
+    # the whole solve is the reference's arithmetic and must match the oracle bit for bit
     g = np.linspace(0.1, 0.9, 4)
     unit = np.array([[x, y, z] for x in g for y in g for z in g])
     nrm = unit - 0.5
