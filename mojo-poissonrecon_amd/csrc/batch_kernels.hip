@@ -99,13 +99,61 @@ __global__ __launch_bounds__(kThreads) void kb_residual(const StepPtrs<T>* tab, 
 template <class T>
 constexpr int kGroup = 16 / (int)sizeof(T);
 
+// x64 += e for the group's colour-1 slots (e1: this sweep's new values, written slots in
+// mask1) and the colour-0 slots at the same positions (their finished values, loaded as
+// the sweep's same-row neighbours), as kb_correct does for every slot.
+template <int V>
+__device__ __forceinline__ void add_correction(double* x64, const SlotGroup<float, V>& G,
+                                               const Geom& g, const float (&e1)[V],
+                                               uint32_t mask1) {
+  if (!G.in) return;
+  using D2 = double __attribute__((ext_vector_type(2)));
+  gptr<double> X = as_global(x64);
+  const uint32_t own1 = G.own, own0 = G.own - g.C;
+  // colour-0 node j of this row is at x = x0 + 1 - 2p + 2j (its parity is 1 - p)
+  const int x00 = G.x0 + 1 - 2 * G.p;
+  uint32_t mask0 = 0;
+#pragma unroll
+  for (int j = 0; j < V; ++j)
+    if (x00 + 2 * j < g.n) mask0 |= 1u << j;
+  float e0[V];
+#pragma unroll
+  for (int j = 0; j < V; ++j) e0[j] = G.row[j / 4][j % 4];
+  constexpr uint32_t full = (1u << V) - 1u;
+  if (mask0 == full && mask1 == full) {
+    D2 a[V / 2], b[V / 2];
+#pragma unroll
+    for (int i = 0; i < V / 2; ++i) {
+      a[i] = *(const __attribute__((address_space(1))) D2*)(X + own0 + 2 * i);
+      b[i] = *(const __attribute__((address_space(1))) D2*)(X + own1 + 2 * i);
+    }
+#pragma unroll
+    for (int i = 0; i < V / 2; ++i) {
+      a[i][0] += (double)e0[2 * i];
+      a[i][1] += (double)e0[2 * i + 1];
+      b[i][0] += (double)e1[2 * i];
+      b[i][1] += (double)e1[2 * i + 1];
+      *(__attribute__((address_space(1))) D2*)(X + own0 + 2 * i) = a[i];
+      *(__attribute__((address_space(1))) D2*)(X + own1 + 2 * i) = b[i];
+    }
+    return;
+  }
+#pragma unroll
+  for (int j = 0; j < V; ++j) {
+    if ((mask0 >> j) & 1u) X[own0 + j] += (double)e0[j];
+    if ((mask1 >> j) & 1u) X[own1 + j] += (double)e1[j];
+  }
+}
+
 // B: workgroup size; NG: consecutive slot groups per lane (all of a lane's loads are issued
 // before its arithmetic, so NG = 2 doubles the loads in flight). Both are chosen per
-// precision and size class by the tuner (mpr_tune.h).
+// precision and size class by the tuner (mpr_tune.h). correct (the last colour-1 sweep of
+// a mixed-precision top level): entries with an x64 pointer get x64 += e for both colours
+// instead of storing e (the next V-cycle starts from zero and never reads it).
 template <class T, int B, int NG>
 __global__ __launch_bounds__(B) void kb_relax_group(const StepPtrs<T>* tab, uint32_t count,
                                                     uint32_t nbi, Geom g, uint32_t lanes, int c,
-                                                    T pw, T h2, int zero_start) {
+                                                    T pw, T h2, int zero_start, int correct) {
   constexpr int V = kGroup<T>;
   const uint32_t per = gridDim.x >> 3;
   const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
@@ -123,6 +171,12 @@ __global__ __launch_bounds__(B) void kb_relax_group(const StepPtrs<T>* tab, uint
   for (int k = 0; k < NG; ++k) {
     T out[V];
     const uint32_t mask = group_relax<T, V>(G[k], g.n, pw, h2, out);
+    if constexpr (sizeof(T) == 4) {
+      if (correct && p.x64) {
+        add_correction<V>(p.x64, G[k], g, out, mask);
+        continue;
+      }
+    }
     store_group<T, T, V>(as_global(p.x) + G[k].own, out, mask);
   }
 }
@@ -490,7 +544,7 @@ inline unsigned padded8(uint64_t blocks) { return (unsigned)((blocks + 7u) & ~7u
 
 template <class T>
 void launch_b_relax(const StepPtrs<T>* tab, uint32_t count, const Geom& g, double pw,
-                    int iterations, hipStream_t s, int zero_start) {
+                    int iterations, hipStream_t s, int zero_start, bool correct) {
   if (!count || !iterations) return;
   const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
   if (group_layout(g)) {
@@ -501,9 +555,10 @@ void launch_b_relax(const StepPtrs<T>* tab, uint32_t count, const Geom& g, doubl
     for (int it = 0; it < iterations; ++it)
       for (int c = 0; c < 2; ++c) {
         const int zs = it == 0 && c == 0 ? zero_start : (int)kZeroNone;
+        const int corr = correct && sizeof(T) == 4 && it == iterations - 1 && c == 1 ? 1 : 0;
 #define MPR_LAUNCH_RELAX_NG(B, NG)                                                 \
   hipLaunchKernelGGL((kb_relax_group<T, B, NG>), dim3(padded8((uint64_t)nbi * count)), \
-                     dim3(B), 0, s, tab, count, nbi, g, lanes, c, (T)pw, h2, zs)
+                     dim3(B), 0, s, tab, count, nbi, g, lanes, c, (T)pw, h2, zs, corr)
 #define MPR_LAUNCH_RELAX1(B) MPR_LAUNCH_RELAX_NG(B, 1)
 #define MPR_LAUNCH_RELAX2(B) MPR_LAUNCH_RELAX_NG(B, 2)
         if (ng == 2) {
@@ -680,7 +735,7 @@ void init_coarse_kernels() {
 
 #define MPR_BINST(T)                                                                           \
   template void launch_b_relax<T>(const StepPtrs<T>*, uint32_t, const Geom&, double, int,      \
-                                  hipStream_t, int);                                           \
+                                  hipStream_t, int, bool);                                     \
   template void launch_b_residual<T>(const StepPtrs<T>*, uint32_t, const Geom&, double,        \
                                      hipStream_t);                                             \
   template void launch_b_restrict<T>(const StepPtrs<T>*, uint32_t, const Geom&, const Geom&,   \

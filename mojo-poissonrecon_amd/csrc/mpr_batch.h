@@ -26,6 +26,9 @@ struct StepPtrs {
   // also cover its rhs (the assembled right-hand side, not the mixed V-cycle's defect)
   int live_rhs;
   const uint32_t* live;
+  // the mixed-precision loop's FP64 solution of an instance at its finest level (null
+  // elsewhere): the top level's last post-smoothing sweep adds the correction into it
+  double* x64;
 };
 
 // First sweep of a V-cycle level from a known-zero x: 0 x as stored, 1 zero for instances
@@ -55,9 +58,12 @@ struct CoarseEntry {
 
 // zero_start: the first colour-0 sweep treats x as zero (ZeroStart); the colour-1 sweep
 // after it overwrites every colour-1 node, so x needs no zero fill
+// correct: the last colour-1 sweep also adds the finished correction of every entry with
+// an x64 pointer into it (x64 += e for both colours) and does not store e there (FP32 only)
 template <class T>
 void launch_b_relax(const StepPtrs<T>* tab, uint32_t count, const Geom& g, double pw,
-                    int iterations, hipStream_t s, int zero_start = kZeroNone);
+                    int iterations, hipStream_t s, int zero_start = kZeroNone,
+                    bool correct = false);
 template <class T>
 void launch_b_residual(const StepPtrs<T>* tab, uint32_t count, const Geom& g, double pw,
                        hipStream_t s);
@@ -89,6 +95,7 @@ struct InstanceLevels {
   std::vector<LevelPtrs<T>> levels;
   const uint32_t* live = nullptr;  // live slots of levels[0] (LiveMask, stencil.h)
   bool live_rhs = false;           // ... covering levels[0].rhs too
+  double* x64 = nullptr;           // FP32 correction levels: the FP64 solution it corrects
 };
 
 // Device tables + schedule for the V-cycles of a fixed set of instances.
@@ -100,8 +107,15 @@ class BatchedVcycle {
   void build(const std::vector<InstanceLevels<T>>& inst, void* table_mem);
   static size_t table_bytes(const std::vector<InstanceLevels<T>>& inst);
   // One V-cycle of every instance (core.py:141-193). zero_tops: every instance's top x
-  // starts at zero (the correction equation of the defect-correction loop).
-  void run(double pw, int pre, int post, bool zero_tops, hipStream_t s) const;
+  // starts at zero (the correction equation of the defect-correction loop). fuse_correct
+  // (FP32, post > 0): the top level's last post-smoothing sweep adds the correction into
+  // the instance's x64 for every instance whose finest class is above unfused_class(); the
+  // others need DefectTables::correct(s, unfused_class()).
+  void run(double pw, int pre, int post, bool zero_tops, hipStream_t s,
+           bool fuse_correct = false) const;
+  // highest finest class without the fused correction: the LDS coarse classes and the
+  // levels below 33^3, which the slot-group sweep does not cover
+  int unfused_class() const { return sc > 4 ? sc : 4; }
   // the V-cycle's own steps at the finest class alone (for kernel timing): `iterations`
   // red-black sweeps (two launches each), or one residual launch
   void relax_finest(double pw, int iterations, hipStream_t s) const {
@@ -137,7 +151,8 @@ class DefectTables {
   void build(const std::vector<int>& depths, const std::vector<TopPtrs>& tops, void* table_mem);
   static size_t table_bytes(size_t count) { return sizeof(TopPtrs) * count + 64 * 16; }
   void defect(double pw, hipStream_t s) const;  // r32 = b - A x64
-  void correct(hipStream_t s) const;             // x64 += e32
+  // x64 += e32 for the instances of finest class <= max_class (all by default)
+  void correct(hipStream_t s, int max_class = 1 << 30) const;
 
  private:
   std::vector<Geom> geom_;

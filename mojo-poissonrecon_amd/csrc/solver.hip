@@ -134,7 +134,7 @@ struct Job {
   // the finest level's live slots cover the assembled rhs and screen (FP64) and the FP32
   // screen; the mixed V-cycle's FP32 rhs is the (dense) defect
   InstanceLevels<double> levels64() const { return {depth, L64, live_bits, true}; }
-  InstanceLevels<float> levels32() const { return {depth, L32, live_bits, false}; }
+  InstanceLevels<float> levels32() const { return {depth, L32, live_bits, false, L64[0].x}; }
   TopPtrs top_ptrs() const {
     return {L64[0].x, L64[0].rhs, L64[0].screen, L32.empty() ? nullptr : L32[0].rhs,
             L32.empty() ? nullptr : L32[0].x, live_bits};
@@ -390,8 +390,11 @@ void run_cycles(const BatchedVcycle<double>& v64, const BatchedVcycle<float>* v3
   for (int c = 0; c < p.cycles; ++c) {
     if (v32) {
       dt->defect(p.point_weight, s);  // r = b - A x (FP64), stored as FP32
-      v32->run(p.point_weight, p.pre_smooth, p.post_smooth, true, s);  // e = V32(r), e0 = 0
-      dt->correct(s);                 // x += e
+      // e = V32(r) from e = 0, and x += e: with post-smoothing the top level's last sweep
+      // adds e into x for instances deeper than the coarse classes, the rest use kb_correct
+      const bool fuse = p.post_smooth > 0;
+      v32->run(p.point_weight, p.pre_smooth, p.post_smooth, true, s, fuse);
+      dt->correct(s, fuse ? v32->unfused_class() : 1 << 30);
     } else {
       v64.run(p.point_weight, p.pre_smooth, p.post_smooth, false, s);
     }
