@@ -162,6 +162,8 @@ __global__ __launch_bounds__(B) void kb_relax_group(const StepPtrs<T>* tab, uint
   const StepPtrs<T> p = tab[slot];
   const uint32_t l0 = ((lb - slot * nbi) * B + threadIdx.x) * NG;
   const LiveMask live{p.live, p.live_rhs != 0};
+  // the edge slot is loaded, not exchanged between lanes (group_edges): the shuffle waits
+  // for the row chunk and measured slower here, though faster in the residual
   SlotGroup<T, V> G[NG];
 #pragma unroll
   for (int k = 0; k < NG; ++k)
@@ -191,11 +193,12 @@ __device__ __forceinline__ void residual_group(gptr<const T> X, gptr<const T> R,
   constexpr int V = kGroup<T>;
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
-    SlotGroup<T, V> G;
-    load_group<T, V>(X, R, S, g, c, lane, lanes, true, G, false, live);
+    SlotGroup<T, V> G[1];
+    load_group<T, V>(X, R, S, g, c, lane, lanes, true, G[0], false, live, false);
+    group_edges<T, V, 1>(X, g, c, G, false);
     T v[V];
-    group_residual<T, V>(G, g.n, pw, h2, v);
-    store_group<T, TO, V>(out + G.own, v, G.in ? (1u << V) - 1 : 0u);
+    group_residual<T, V>(G[0], g.n, pw, h2, v);
+    store_group<T, TO, V>(out + G[0].own, v, G[0].in ? (1u << V) - 1 : 0u);
   }
 }
 

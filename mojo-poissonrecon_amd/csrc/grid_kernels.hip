@@ -122,13 +122,14 @@ __global__ __launch_bounds__(kThreads) void k_residual_group(const T* X, const T
   double sq = 0.0;
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
-    SlotGroup<T, V> G;
-    load_group<T, V>(as_global(X), as_global(R), as_global(S), g, c, lane, lanes, true, G,
-                     false, LiveMask{live, true});
+    SlotGroup<T, V> G[1];
+    load_group<T, V>(as_global(X), as_global(R), as_global(S), g, c, lane, lanes, true, G[0],
+                     false, LiveMask{live, true}, false);
+    group_edges<T, V, 1>(as_global(X), g, c, G, false);
     T v[V];
-    group_residual<T, V>(G, g.n, pw, h2, v);
-    if (out) store_group<T, TR, V>(as_global(out) + G.own, v, G.in ? (1u << V) - 1 : 0u);
-    if (G.in)
+    group_residual<T, V>(G[0], g.n, pw, h2, v);
+    if (out) store_group<T, TR, V>(as_global(out) + G[0].own, v, G[0].in ? (1u << V) - 1 : 0u);
+    if (G[0].in)
 #pragma unroll
       for (int j = 0; j < V; ++j) sq += (double)v[j] * (double)v[j];
   }

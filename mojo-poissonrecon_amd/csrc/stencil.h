@@ -202,7 +202,8 @@ template <class T, int V>
 __device__ __forceinline__ void load_group(gptr<const T> X, gptr<const T> R, gptr<const T> S,
                                            const Geom& g, int c, uint32_t lane, uint32_t lanes,
                                            bool want_self, SlotGroup<T, V>& G,
-                                           bool x_zero = false, LiveMask live = LiveMask()) {
+                                           bool x_zero = false, LiveMask live = LiveMask(),
+                                           bool load_edge = true) {
   G.in = lane < lanes;
   lane = G.in ? lane : lanes - 1;
   const uint32_t s0 = lane * (uint32_t)V;
@@ -225,8 +226,10 @@ __device__ __forceinline__ void load_group(gptr<const T> X, gptr<const T> R, gpt
     G.edge = T(0);
   } else {
     load_chunks<T>(G.row, X + o);
-    const uint32_t eo = p == 0 ? (k0 > 0 ? o - 1 : o) : (k0 + V < (uint32_t)g.H ? o + V : o);
-    G.edge = X[eo];
+    if (load_edge) {
+      const uint32_t eo = p == 0 ? (k0 > 0 ? o - 1 : o) : (k0 + V < (uint32_t)g.H ? o + V : o);
+      G.edge = X[eo];
+    }
     load_chunks<T>(G.ym, X + (y > 0 ? o - (uint32_t)g.H : o));
     load_chunks<T>(G.yp, X + ((int)y + 1 < n ? o + (uint32_t)g.H : o));
     load_chunks<T>(G.zm, X + (z > 0 ? o - g.nH : o));
@@ -249,6 +252,48 @@ __device__ __forceinline__ void load_group(gptr<const T> X, gptr<const T> R, gpt
     load_chunks<T>(G.s, S + G.own);
   }
   if (want_self) load_chunks<T>(G.self, X + G.own);
+}
+
+// Edge slots of a lane's NG consecutive slot groups (group index lane * NG + k, the lanes of
+// a wavefront consecutive, all in colour c, loaded with load_edge = false). A group's edge
+// is slot k0 - 1 (x parity 0) or k0 + V (parity 1) of the other colour's row: the last slot
+// of the previous group's or the first slot of the next group's row chunk whenever it is
+// inside the row, taken from this lane or a neighbouring lane; only the first and last
+// lanes of a wavefront load it. Every lane must call this (the exchange is wavefront-wide).
+template <class T, int V, int NG>
+__device__ __forceinline__ void group_edges(gptr<const T> X, const Geom& g, int c,
+                                            SlotGroup<T, V> (&G)[NG], bool x_zero) {
+  constexpr int W = SlotGroup<T, V>::W;
+  constexpr int NC = SlotGroup<T, V>::NC;
+  const T prev_lane = __shfl_up(G[NG - 1].row[NC - 1][W - 1], 1, 64);
+  const T next_lane = __shfl_down(G[0].row[0][0], 1, 64);
+  const int lid = (int)(threadIdx.x & 63u);
+#pragma unroll
+  for (int k = 0; k < NG; ++k) {
+    if (x_zero) {
+      G[k].edge = T(0);
+      continue;
+    }
+    const uint32_t k0 = (uint32_t)(G[k].x0 - G[k].p) >> 1;
+    const uint32_t o = (uint32_t)(1 - c) * g.C + (G[k].own - (uint32_t)c * g.C);
+    if (G[k].p == 0) {
+      const bool in_row = k0 > 0;
+      if (k > 0)
+        G[k].edge = G[k - 1].row[NC - 1][W - 1];
+      else if (lid > 0 || !in_row)
+        G[k].edge = prev_lane;
+      else
+        G[k].edge = X[o - 1];
+    } else {
+      const bool in_row = k0 + V < (uint32_t)g.H;
+      if (k < NG - 1)
+        G[k].edge = G[k + 1].row[0][0];
+      else if (lid < 63 || !in_row)
+        G[k].edge = next_lane;
+      else
+        G[k].edge = X[o + V];
+    }
+  }
 }
 
 // element J of an operand (J is a compile-time constant, so every access is static)
