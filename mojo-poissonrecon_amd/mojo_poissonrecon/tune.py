@@ -1,10 +1,10 @@
 """Launch-shape tuning of the batched multigrid kernels (include/mpr_hip.h mpr_autotune).
 
 The colour sweep, residual and FP64->FP32 defect kernels of every V-cycle (solo and batched)
-launch with a workgroup size (and, for the colour sweep, a number of slot groups per lane)
-looked up per (kernel, precision, grid size class). autotune() times the candidates on the
-device and keeps the fastest for the rest of the process. The per-node arithmetic does not
-depend on the launch shape, so results stay bit-identical.
+launch with a workgroup size (and, for the colour sweep, 1 or 2 slot groups per lane)
+looked up per (kernel, precision, grid size class). autotune() times the candidates on the device and keeps the fastest for the rest of
+the process. The per-node arithmetic does not depend on the launch shape, so results stay
+bit-identical.
 """
 
 from __future__ import annotations

@@ -192,6 +192,6 @@ def test_workgroup_size_table_without_a_device():
         assert tune.tuned_groups("relax", "fp32", 8) == 2
     finally:
         tune.set_tuned_groups("relax", "fp32", 8, 1)
-    for kernel, groups in (("relax", 3), ("residual", 2)):
+    for kernel, groups in (("relax", 3), ("relax", 0), ("residual", 2)):
         with pytest.raises(DeviceError):
             tune.set_tuned_groups(kernel, "fp32", 8, groups)
