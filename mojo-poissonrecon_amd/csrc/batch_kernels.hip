@@ -191,15 +191,17 @@ __device__ __forceinline__ void residual_group(gptr<const T> X, gptr<const T> R,
                                                gptr<TO> out, const Geom& g, uint32_t lane,
                                                uint32_t lanes, T pw, T h2, LiveMask live) {
   constexpr int V = kGroup<T>;
-#pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    SlotGroup<T, V> G[1];
-    load_group<T, V>(X, R, S, g, c, lane, lanes, true, G[0], false, live, false);
-    group_edges<T, V, 1>(X, g, c, G, false);
-    T v[V];
-    group_residual<T, V>(G[0], g.n, pw, h2, v);
-    store_group<T, TO, V>(out + G[0].own, v, G[0].in ? (1u << V) - 1 : 0u);
-  }
+  SlotGroup<T, V> G0[1], G1[1];
+  load_group<T, V>(X, R, S, g, 0, lane, lanes, true, G0[0], false, live, false);
+  // colour 1 at the same slots: its row and own chunks are colour 0's own and row chunks
+  load_group<T, V>(X, R, S, g, 1, lane, lanes, true, G1[0], false, live, false, G0);
+  group_edges<T, V, 1>(X, g, 0, G0, false);
+  group_edges<T, V, 1>(X, g, 1, G1, false);
+  T v[V];
+  group_residual<T, V>(G0[0], g.n, pw, h2, v);
+  store_group<T, TO, V>(out + G0[0].own, v, G0[0].in ? (1u << V) - 1 : 0u);
+  group_residual<T, V>(G1[0], g.n, pw, h2, v);
+  store_group<T, TO, V>(out + G1[0].own, v, G1[0].in ? (1u << V) - 1 : 0u);
 }
 
 template <class T, int B>

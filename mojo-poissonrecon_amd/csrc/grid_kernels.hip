@@ -120,16 +120,20 @@ __global__ __launch_bounds__(kThreads) void k_residual_group(const T* X, const T
   const uint32_t lb = logical_block();
   const uint32_t lane = lb * kThreads + threadIdx.x;
   double sq = 0.0;
+  SlotGroup<T, V> G[2][1];
+  load_group<T, V>(as_global(X), as_global(R), as_global(S), g, 0, lane, lanes, true, G[0][0],
+                   false, LiveMask{live, true}, false);
+  // colour 1 at the same slots: its row and own chunks are colour 0's own and row chunks
+  load_group<T, V>(as_global(X), as_global(R), as_global(S), g, 1, lane, lanes, true, G[1][0],
+                   false, LiveMask{live, true}, false, G[0]);
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
-    SlotGroup<T, V> G[1];
-    load_group<T, V>(as_global(X), as_global(R), as_global(S), g, c, lane, lanes, true, G[0],
-                     false, LiveMask{live, true}, false);
-    group_edges<T, V, 1>(as_global(X), g, c, G, false);
+    group_edges<T, V, 1>(as_global(X), g, c, G[c], false);
     T v[V];
-    group_residual<T, V>(G[0], g.n, pw, h2, v);
-    if (out) store_group<T, TR, V>(as_global(out) + G[0].own, v, G[0].in ? (1u << V) - 1 : 0u);
-    if (G[0].in)
+    group_residual<T, V>(G[c][0], g.n, pw, h2, v);
+    if (out)
+      store_group<T, TR, V>(as_global(out) + G[c][0].own, v, G[c][0].in ? (1u << V) - 1 : 0u);
+    if (G[c][0].in)
 #pragma unroll
       for (int j = 0; j < V; ++j) sq += (double)v[j] * (double)v[j];
   }

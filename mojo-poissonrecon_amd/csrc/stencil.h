@@ -203,7 +203,8 @@ __device__ __forceinline__ void load_group(gptr<const T> X, gptr<const T> R, gpt
                                            const Geom& g, int c, uint32_t lane, uint32_t lanes,
                                            bool want_self, SlotGroup<T, V>& G,
                                            bool x_zero = false, LiveMask live = LiveMask(),
-                                           bool load_edge = true) {
+                                           bool load_edge = true,
+                                           const SlotGroup<T, V>* partner = nullptr) {
   G.in = lane < lanes;
   lane = G.in ? lane : lanes - 1;
   const uint32_t s0 = lane * (uint32_t)V;
@@ -225,7 +226,14 @@ __device__ __forceinline__ void load_group(gptr<const T> X, gptr<const T> R, gpt
       G.row[i] = G.ym[i] = G.yp[i] = G.zm[i] = G.zp[i] = typename SlotGroup<T, V>::CT(T(0));
     G.edge = T(0);
   } else {
-    load_chunks<T>(G.row, X + o);
+    // partner: the other colour's group at the same slots (loaded with want_self), whose own
+    // chunk is this group's row and whose row is this group's own chunk
+    if (partner) {
+#pragma unroll
+      for (int i = 0; i < SlotGroup<T, V>::NC; ++i) G.row[i] = partner->self[i];
+    } else {
+      load_chunks<T>(G.row, X + o);
+    }
     if (load_edge) {
       const uint32_t eo = p == 0 ? (k0 > 0 ? o - 1 : o) : (k0 + V < (uint32_t)g.H ? o + V : o);
       G.edge = X[eo];
@@ -251,7 +259,14 @@ __device__ __forceinline__ void load_group(gptr<const T> X, gptr<const T> R, gpt
   } else {
     load_chunks<T>(G.s, S + G.own);
   }
-  if (want_self) load_chunks<T>(G.self, X + G.own);
+  if (want_self) {
+    if (partner && !x_zero) {
+#pragma unroll
+      for (int i = 0; i < SlotGroup<T, V>::NC; ++i) G.self[i] = partner->row[i];
+    } else {
+      load_chunks<T>(G.self, X + G.own);
+    }
+  }
 }
 
 // Edge slots of a lane's NG consecutive slot groups (group index lane * NG + k, the lanes of
