@@ -101,11 +101,13 @@ constexpr int kGroup = 16 / (int)sizeof(T);
 
 // x64 += e for the group's colour-1 slots (e1: this sweep's new values, written slots in
 // mask1) and the colour-0 slots at the same positions (their finished values, loaded as
-// the sweep's same-row neighbours), as kb_correct does for every slot.
+// the sweep's same-row neighbours), as kb_correct does for every slot. x_initial: x64 holds
+// the initial guess kInitialX implicitly (it was never filled), so x64 = kInitialX + e is
+// stored without a load: the sum x64 += e would give.
 template <int V>
 __device__ __forceinline__ void add_correction(double* x64, const SlotGroup<float, V>& G,
                                                const Geom& g, const float (&e1)[V],
-                                               uint32_t mask1) {
+                                               uint32_t mask1, bool x_initial) {
   if (!G.in) return;
   using D2 = double __attribute__((ext_vector_type(2)));
   gptr<double> X = as_global(x64);
@@ -124,8 +126,12 @@ __device__ __forceinline__ void add_correction(double* x64, const SlotGroup<floa
     D2 a[V / 2], b[V / 2];
 #pragma unroll
     for (int i = 0; i < V / 2; ++i) {
-      a[i] = *(const __attribute__((address_space(1))) D2*)(X + own0 + 2 * i);
-      b[i] = *(const __attribute__((address_space(1))) D2*)(X + own1 + 2 * i);
+      if (x_initial) {
+        a[i] = b[i] = D2(kInitialX);
+      } else {
+        a[i] = *(const __attribute__((address_space(1))) D2*)(X + own0 + 2 * i);
+        b[i] = *(const __attribute__((address_space(1))) D2*)(X + own1 + 2 * i);
+      }
     }
 #pragma unroll
     for (int i = 0; i < V / 2; ++i) {
@@ -140,8 +146,8 @@ __device__ __forceinline__ void add_correction(double* x64, const SlotGroup<floa
   }
 #pragma unroll
   for (int j = 0; j < V; ++j) {
-    if ((mask0 >> j) & 1u) X[own0 + j] += (double)e0[j];
-    if ((mask1 >> j) & 1u) X[own1 + j] += (double)e1[j];
+    if ((mask0 >> j) & 1u) X[own0 + j] = (x_initial ? kInitialX : X[own0 + j]) + (double)e0[j];
+    if ((mask1 >> j) & 1u) X[own1 + j] = (x_initial ? kInitialX : X[own1 + j]) + (double)e1[j];
   }
 }
 
@@ -149,11 +155,13 @@ __device__ __forceinline__ void add_correction(double* x64, const SlotGroup<floa
 // before its arithmetic, so NG = 2 doubles the loads in flight). Both are chosen per
 // precision and size class by the tuner (mpr_tune.h). correct (the last colour-1 sweep of
 // a mixed-precision top level): entries with an x64 pointer get x64 += e for both colours
-// instead of storing e (the next V-cycle starts from zero and never reads it).
+// instead of storing e (the next V-cycle starts from zero and never reads it). first: this
+// is the first cycle's correction, so an entry whose x_initial flag is set holds kInitialX.
 template <class T, int B, int NG>
 __global__ __launch_bounds__(B) void kb_relax_group(const StepPtrs<T>* tab, uint32_t count,
                                                     uint32_t nbi, Geom g, uint32_t lanes, int c,
-                                                    T pw, T h2, int zero_start, int correct) {
+                                                    T pw, T h2, int zero_start, int correct,
+                                                    int first) {
   constexpr int V = kGroup<T>;
   const uint32_t per = gridDim.x >> 3;
   const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
@@ -175,7 +183,7 @@ __global__ __launch_bounds__(B) void kb_relax_group(const StepPtrs<T>* tab, uint
     const uint32_t mask = group_relax<T, V>(G[k], g.n, pw, h2, out);
     if constexpr (sizeof(T) == 4) {
       if (correct && p.x64) {
-        add_correction<V>(p.x64, G[k], g, out, mask);
+        add_correction<V>(p.x64, G[k], g, out, mask, first && p.x_initial && *p.x_initial != 0);
         continue;
       }
     }
@@ -186,17 +194,20 @@ __global__ __launch_bounds__(B) void kb_relax_group(const StepPtrs<T>* tab, uint
 // residual of slot group `lane` in both colours. One lane takes the same slots of both
 // colour blocks, so the words each colour reads as neighbours it also reads as its own
 // nodes at the same time (from cache), and x streams from memory once.
-template <class T, class TO>
+// KNOWN: every x word holds xv (load_group); a compile-time choice, so the loading form
+// keeps its load schedule.
+template <class T, class TO, bool KNOWN = false>
 __device__ __forceinline__ void residual_group(gptr<const T> X, gptr<const T> R, gptr<const T> S,
                                                gptr<TO> out, const Geom& g, uint32_t lane,
-                                               uint32_t lanes, T pw, T h2, LiveMask live) {
+                                               uint32_t lanes, T pw, T h2, LiveMask live,
+                                               T xv = T(0)) {
   constexpr int V = kGroup<T>;
   SlotGroup<T, V> G0[1], G1[1];
-  load_group<T, V>(X, R, S, g, 0, lane, lanes, true, G0[0], false, live, false);
+  load_group<T, V>(X, R, S, g, 0, lane, lanes, true, G0[0], KNOWN, live, false, nullptr, xv);
   // colour 1 at the same slots: its row and own chunks are colour 0's own and row chunks
-  load_group<T, V>(X, R, S, g, 1, lane, lanes, true, G1[0], false, live, false, G0);
-  group_edges<T, V, 1>(X, g, 0, G0, false);
-  group_edges<T, V, 1>(X, g, 1, G1, false);
+  load_group<T, V>(X, R, S, g, 1, lane, lanes, true, G1[0], KNOWN, live, false, G0, xv);
+  group_edges<T, V, 1>(X, g, 0, G0, KNOWN, xv);
+  group_edges<T, V, 1>(X, g, 1, G1, KNOWN, xv);
   T v[V];
   group_residual<T, V>(G0[0], g.n, pw, h2, v);
   store_group<T, TO, V>(out + G0[0].own, v, G0[0].in ? (1u << V) - 1 : 0u);
@@ -312,7 +323,9 @@ __global__ __launch_bounds__(kThreads) void kb_defect(const TopPtrs* tab, uint32
                         (lb - slot * nbi) * kSpan + threadIdx.x, pw, h2);
 }
 
-template <int B>
+// FIRST: the first cycle's defect, where an entry whose x_initial flag is set reads x64 as
+// kInitialX everywhere (it was never filled)
+template <int B, bool FIRST>
 __global__ __launch_bounds__(B) void kb_defect_group(const TopPtrs* tab, uint32_t count,
                                                      uint32_t nbi, Geom g, uint32_t lanes,
                                                      double pw, double h2) {
@@ -321,10 +334,15 @@ __global__ __launch_bounds__(B) void kb_defect_group(const TopPtrs* tab, uint32_
   const uint32_t slot = lb / nbi;
   if (slot >= count) return;
   const TopPtrs p = tab[slot];
-  residual_group<double, float>(as_global((const double*)p.x64), as_global(p.rhs64),
-                                as_global(p.screen64), as_global(p.r32), g,
-                                (lb - slot * nbi) * B + threadIdx.x, lanes, pw, h2,
-                                LiveMask{p.live, true});
+  const uint32_t lane = (lb - slot * nbi) * B + threadIdx.x;
+  if (FIRST && p.x_initial && *p.x_initial != 0)
+    residual_group<double, float, true>(as_global((const double*)p.x64), as_global(p.rhs64),
+                                        as_global(p.screen64), as_global(p.r32), g, lane, lanes,
+                                        pw, h2, LiveMask{p.live, true}, kInitialX);
+  else
+    residual_group<double, float>(as_global((const double*)p.x64), as_global(p.rhs64),
+                                  as_global(p.screen64), as_global(p.r32), g, lane, lanes, pw,
+                                  h2, LiveMask{p.live, true});
 }
 
 __global__ __launch_bounds__(kThreads) void kb_correct(const TopPtrs* tab, uint32_t count,
@@ -549,7 +567,7 @@ inline unsigned padded8(uint64_t blocks) { return (unsigned)((blocks + 7u) & ~7u
 
 template <class T>
 void launch_b_relax(const StepPtrs<T>* tab, uint32_t count, const Geom& g, double pw,
-                    int iterations, hipStream_t s, int zero_start, bool correct) {
+                    int iterations, hipStream_t s, int zero_start, bool correct, bool first) {
   if (!count || !iterations) return;
   const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
   if (group_layout(g)) {
@@ -563,7 +581,8 @@ void launch_b_relax(const StepPtrs<T>* tab, uint32_t count, const Geom& g, doubl
         const int corr = correct && sizeof(T) == 4 && it == iterations - 1 && c == 1 ? 1 : 0;
 #define MPR_LAUNCH_RELAX_NG(B, NG)                                                 \
   hipLaunchKernelGGL((kb_relax_group<T, B, NG>), dim3(padded8((uint64_t)nbi * count)), \
-                     dim3(B), 0, s, tab, count, nbi, g, lanes, c, (T)pw, h2, zs, corr)
+                     dim3(B), 0, s, tab, count, nbi, g, lanes, c, (T)pw, h2, zs, corr, \
+                     first ? 1 : 0)
 #define MPR_LAUNCH_RELAX1(B) MPR_LAUNCH_RELAX_NG(B, 1)
 #define MPR_LAUNCH_RELAX2(B) MPR_LAUNCH_RELAX_NG(B, 2)
         if (ng == 2) {
@@ -657,18 +676,27 @@ void launch_b_fill_x(const StepPtrs<T>* tab, uint32_t count, const Geom& g, hipS
   MPR_LAUNCH_CHECK();
 }
 
-void launch_b_defect(const TopPtrs* tab, uint32_t count, const Geom& g, double pw, hipStream_t s) {
+void launch_b_defect(const TopPtrs* tab, uint32_t count, const Geom& g, double pw, bool first,
+                     hipStream_t s) {
   if (!count) return;
   const double h2 = (double)(g.n - 1) * (double)(g.n - 1);
   if (group_layout(g)) {
     const uint32_t lanes = g.C / (uint32_t)kGroup<double>;
     const int block = tuned_block(kTuneDefect, true, size_class(g));
     const uint32_t nbi = blocks_for(lanes, (unsigned)block);  // both colours per lane
-#define MPR_LAUNCH_DEFECT(B)                                                                 \
-  hipLaunchKernelGGL((kb_defect_group<B>), dim3(padded8((uint64_t)nbi * count)), dim3(B), 0, \
-                     s, tab, count, nbi, g, lanes, pw, h2)
-    MPR_WITH_BLOCK(block, MPR_LAUNCH_DEFECT)
-#undef MPR_LAUNCH_DEFECT
+#define MPR_LAUNCH_DEFECT_F(B, F)                                                           \
+  hipLaunchKernelGGL((kb_defect_group<B, F>), dim3(padded8((uint64_t)nbi * count)), dim3(B), \
+                     0, s, tab, count, nbi, g, lanes, pw, h2)
+#define MPR_LAUNCH_DEFECT1(B) MPR_LAUNCH_DEFECT_F(B, true)
+#define MPR_LAUNCH_DEFECT0(B) MPR_LAUNCH_DEFECT_F(B, false)
+    if (first) {
+      MPR_WITH_BLOCK(block, MPR_LAUNCH_DEFECT1)
+    } else {
+      MPR_WITH_BLOCK(block, MPR_LAUNCH_DEFECT0)
+    }
+#undef MPR_LAUNCH_DEFECT0
+#undef MPR_LAUNCH_DEFECT1
+#undef MPR_LAUNCH_DEFECT_F
     MPR_LAUNCH_CHECK();
     return;
   }
@@ -740,7 +768,7 @@ void init_coarse_kernels() {
 
 #define MPR_BINST(T)                                                                           \
   template void launch_b_relax<T>(const StepPtrs<T>*, uint32_t, const Geom&, double, int,      \
-                                  hipStream_t, int, bool);                                     \
+                                  hipStream_t, int, bool, bool);                               \
   template void launch_b_residual<T>(const StepPtrs<T>*, uint32_t, const Geom&, double,        \
                                      hipStream_t);                                             \
   template void launch_b_restrict<T>(const StepPtrs<T>*, uint32_t, const Geom&, const Geom&,   \

@@ -29,11 +29,15 @@ struct StepPtrs {
   // the mixed-precision loop's FP64 solution of an instance at its finest level (null
   // elsewhere): the top level's last post-smoothing sweep adds the correction into it
   double* x64;
+  const int* x_initial;  // with x64: the instance's initial-guess flag (TopPtrs)
 };
 
 // First sweep of a V-cycle level from a known-zero x: 0 x as stored, 1 zero for instances
 // below their finest level, 2 zero for every instance
 enum ZeroStart { kZeroNone = 0, kZeroBelowTop = 1, kZeroAll = 2 };
+
+// Initial guess of the solve (core.py:270)
+constexpr double kInitialX = 0.5;
 
 // top-level buffers of one instance for the FP64 defect-correction loop
 struct TopPtrs {
@@ -43,6 +47,10 @@ struct TopPtrs {
   float* r32;
   const float* e32;
   const uint32_t* live;  // live slots of rhs64 / screen64 (LiveMask, stencil.h), or null
+  // device flag, set for a run that did not fill x64 with kInitialX: the first cycle's
+  // defect reads x64 as kInitialX and its fused correction stores kInitialX + e (only for
+  // instances whose correction is fused, BatchedVcycle::unfused_class)
+  const int* x_initial;
 };
 
 // one instance of the LDS-resident coarse V-cycle: its top coarse class, whether that
@@ -59,11 +67,12 @@ struct CoarseEntry {
 // zero_start: the first colour-0 sweep treats x as zero (ZeroStart); the colour-1 sweep
 // after it overwrites every colour-1 node, so x needs no zero fill
 // correct: the last colour-1 sweep also adds the finished correction of every entry with
-// an x64 pointer into it (x64 += e for both colours) and does not store e there (FP32 only)
+// an x64 pointer into it (x64 += e for both colours) and does not store e there (FP32 only);
+// first: that is the first cycle's correction (TopPtrs::x_initial)
 template <class T>
 void launch_b_relax(const StepPtrs<T>* tab, uint32_t count, const Geom& g, double pw,
                     int iterations, hipStream_t s, int zero_start = kZeroNone,
-                    bool correct = false);
+                    bool correct = false, bool first = false);
 template <class T>
 void launch_b_residual(const StepPtrs<T>* tab, uint32_t count, const Geom& g, double pw,
                        hipStream_t s);
@@ -77,7 +86,9 @@ void launch_b_prolong(const StepPtrs<T>* tab, uint32_t count, const Geom& gc, co
                       bool colour1_only, hipStream_t s);
 template <class T>
 void launch_b_fill_x(const StepPtrs<T>* tab, uint32_t count, const Geom& g, hipStream_t s);
-void launch_b_defect(const TopPtrs* tab, uint32_t count, const Geom& g, double pw, hipStream_t s);
+// first: the first cycle's defect (TopPtrs::x_initial)
+void launch_b_defect(const TopPtrs* tab, uint32_t count, const Geom& g, double pw, bool first,
+                     hipStream_t s);
 void launch_b_correct(const TopPtrs* tab, uint32_t count, const Geom& g, hipStream_t s);
 template <class T>
 void launch_b_coarse(const CoarseEntry<T>* tab, uint32_t count, int top_class, double pw, int pre,
@@ -96,7 +107,15 @@ struct InstanceLevels {
   const uint32_t* live = nullptr;  // live slots of levels[0] (LiveMask, stencil.h)
   bool live_rhs = false;           // ... covering levels[0].rhs too
   double* x64 = nullptr;           // FP32 correction levels: the FP64 solution it corrects
+  const int* x_initial = nullptr;  // ... and its initial-guess flag (TopPtrs)
 };
+
+// highest finest class whose mixed-precision correction is not fused into the FP32 sweep
+// (BatchedVcycle<float>::unfused_class)
+inline int mixed_unfused_class() {
+  const int c = coarse_b_top_class<float>();
+  return c > 4 ? c : 4;
+}
 
 // Device tables + schedule for the V-cycles of a fixed set of instances.
 template <class T>
@@ -110,9 +129,10 @@ class BatchedVcycle {
   // starts at zero (the correction equation of the defect-correction loop). fuse_correct
   // (FP32, post > 0): the top level's last post-smoothing sweep adds the correction into
   // the instance's x64 for every instance whose finest class is above unfused_class(); the
-  // others need DefectTables::correct(s, unfused_class()).
+  // others need DefectTables::correct(s, unfused_class()). first: this is the first cycle
+  // of the defect-correction loop (TopPtrs::x_initial).
   void run(double pw, int pre, int post, bool zero_tops, hipStream_t s,
-           bool fuse_correct = false) const;
+           bool fuse_correct = false, bool first = false) const;
   // highest finest class without the fused correction: the LDS coarse classes and the
   // levels below 33^3, which the slot-group sweep does not cover
   int unfused_class() const { return sc > 4 ? sc : 4; }
@@ -150,7 +170,8 @@ class DefectTables {
  public:
   void build(const std::vector<int>& depths, const std::vector<TopPtrs>& tops, void* table_mem);
   static size_t table_bytes(size_t count) { return sizeof(TopPtrs) * count + 64 * 16; }
-  void defect(double pw, hipStream_t s) const;  // r32 = b - A x64
+  // r32 = b - A x64; first: the first cycle of the loop (TopPtrs::x_initial)
+  void defect(double pw, hipStream_t s, bool first = false) const;
   // x64 += e32 for the instances of finest class <= max_class (all by default)
   void correct(hipStream_t s, int max_class = 1 << 30) const;
 

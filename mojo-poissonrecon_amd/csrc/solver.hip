@@ -108,6 +108,7 @@ struct Job {
   uint32_t *chunk_list = nullptr, *chunk_count = nullptr;  // the sparse assembly's chunks
   // live slots of the finest level (LiveMask, stencil.h): rhs or screen non-zero
   uint32_t* live_bits = nullptr;
+  int* x_initial = nullptr;  // the FP64 solution was not filled with kInitialX (TopPtrs)
   // how the density grid holds the last solve's values: raw splat sums read through
   // `touched` and divided by dens_spn, or normalised in place (runs with MPR_STAGE_GRIDS)
   bool dens_raw = false;
@@ -134,10 +135,12 @@ struct Job {
   // the finest level's live slots cover the assembled rhs and screen (FP64) and the FP32
   // screen; the mixed V-cycle's FP32 rhs is the (dense) defect
   InstanceLevels<double> levels64() const { return {depth, L64, live_bits, true}; }
-  InstanceLevels<float> levels32() const { return {depth, L32, live_bits, false, L64[0].x}; }
+  InstanceLevels<float> levels32() const {
+    return {depth, L32, live_bits, false, L64[0].x, x_initial};
+  }
   TopPtrs top_ptrs() const {
     return {L64[0].x, L64[0].rhs, L64[0].screen, L32.empty() ? nullptr : L32[0].rhs,
-            L32.empty() ? nullptr : L32[0].x, live_bits};
+            L32.empty() ? nullptr : L32[0].x, live_bits, x_initial};
   }
   uint64_t live_words() const { return (2ull * geoms[0].C + 31) / 32; }
   const double* unit_points(const mpr_params& p) const {
@@ -389,11 +392,11 @@ void run_cycles(const BatchedVcycle<double>& v64, const BatchedVcycle<float>* v3
                 const DefectTables* dt, const mpr_params& p, hipStream_t s) {
   for (int c = 0; c < p.cycles; ++c) {
     if (v32) {
-      dt->defect(p.point_weight, s);  // r = b - A x (FP64), stored as FP32
+      dt->defect(p.point_weight, s, c == 0);  // r = b - A x (FP64), stored as FP32
       // e = V32(r) from e = 0, and x += e: with post-smoothing the top level's last sweep
       // adds e into x for instances deeper than the coarse classes, the rest use kb_correct
       const bool fuse = p.post_smooth > 0;
-      v32->run(p.point_weight, p.pre_smooth, p.post_smooth, true, s, fuse);
+      v32->run(p.point_weight, p.pre_smooth, p.post_smooth, true, s, fuse, c == 0);
       dt->correct(s, fuse ? v32->unfused_class() : 1 << 30);
     } else {
       v64.run(p.point_weight, p.pre_smooth, p.post_smooth, false, s);
@@ -456,6 +459,7 @@ void Job::plan(DevArena& a) {
   chunk_list = a.alloc<uint32_t>((uint64_t)n * n * assemble_chunks_per_row(n));
   chunk_count = a.alloc<uint32_t>(1);
   live_bits = a.alloc<uint32_t>(live_words());
+  x_initial = a.alloc<int>(1);
   splat = splat_scratch_sizes(std::max<uint32_t>(1u, max_points));
   splat.keys = a.alloc<uint32_t>(splat.pairs);
   splat.ktmp = a.alloc<uint32_t>(splat.pairs);
@@ -547,8 +551,13 @@ void Job::setup(const mpr_params& p, hipStream_t s) {
     launch_restrict<double>(L64[l - 1].screen, geoms[l - 1], L64[l].screen, geoms[l], 4.0, s,
                             mixed ? L32[l].screen : nullptr);
   // initial guess 0.5 (core.py:270); no samples: the solution is 0 (core.py:286-287), which
-  // the V-cycles then leave at 0 (zero right-hand side and screening)
-  launch_fill<double>(L64[0].x, P ? 0.5 : 0.0, 2ull * geoms[0].C, s);
+  // the V-cycles then leave at 0 (zero right-hand side and screening). When the first
+  // mixed-precision cycle's defect and fused correction cover this job, they read the
+  // guess as the constant and write every node of x, so x is not filled.
+  const bool implicit = mixed && P && p.cycles > 0 && p.post_smooth > 0 &&
+                        depth > mixed_unfused_class();
+  launch_fill<int>(x_initial, implicit ? 1 : 0, 1, s);
+  if (!implicit) launch_fill<double>(L64[0].x, P ? kInitialX : 0.0, 2ull * geoms[0].C, s);
 }
 
 // final FP64 residual norm, natural-layout field and iso value (core.py:289-303)

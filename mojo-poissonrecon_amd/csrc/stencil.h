@@ -195,16 +195,17 @@ struct LiveMask {
   bool rhs = false;
 };
 
-// lane: index of the group inside colour block c; lanes = C / V. x_zero: the other
-// colour's x is known to be zero (the first sweep from a zero start), so its loads are
-// skipped and the neighbour values are +0, exactly what a zero-filled x would give.
+// lane: index of the group inside colour block c; lanes = C / V. x_known: every x word is
+// known to hold xv (+0 for the first sweep from a zero start, the initial guess before the
+// first defect), so the x loads are skipped and the values are exactly what they would load.
 template <class T, int V>
 __device__ __forceinline__ void load_group(gptr<const T> X, gptr<const T> R, gptr<const T> S,
                                            const Geom& g, int c, uint32_t lane, uint32_t lanes,
                                            bool want_self, SlotGroup<T, V>& G,
-                                           bool x_zero = false, LiveMask live = LiveMask(),
+                                           bool x_known = false, LiveMask live = LiveMask(),
                                            bool load_edge = true,
-                                           const SlotGroup<T, V>* partner = nullptr) {
+                                           const SlotGroup<T, V>* partner = nullptr,
+                                           T xv = T(0)) {
   G.in = lane < lanes;
   lane = G.in ? lane : lanes - 1;
   const uint32_t s0 = lane * (uint32_t)V;
@@ -220,11 +221,11 @@ __device__ __forceinline__ void load_group(gptr<const T> X, gptr<const T> R, gpt
   G.z = (int)z;
   G.p = p;
   const int n = g.n;
-  if (x_zero) {
+  if (x_known) {
 #pragma unroll
     for (int i = 0; i < SlotGroup<T, V>::NC; ++i)
-      G.row[i] = G.ym[i] = G.yp[i] = G.zm[i] = G.zp[i] = typename SlotGroup<T, V>::CT(T(0));
-    G.edge = T(0);
+      G.row[i] = G.ym[i] = G.yp[i] = G.zm[i] = G.zp[i] = typename SlotGroup<T, V>::CT(xv);
+    G.edge = xv;
   } else {
     // partner: the other colour's group at the same slots (loaded with want_self), whose own
     // chunk is this group's row and whose row is this group's own chunk
@@ -260,7 +261,10 @@ __device__ __forceinline__ void load_group(gptr<const T> X, gptr<const T> R, gpt
     load_chunks<T>(G.s, S + G.own);
   }
   if (want_self) {
-    if (partner && !x_zero) {
+    if (x_known) {
+#pragma unroll
+      for (int i = 0; i < SlotGroup<T, V>::NC; ++i) G.self[i] = CT(xv);
+    } else if (partner) {
 #pragma unroll
       for (int i = 0; i < SlotGroup<T, V>::NC; ++i) G.self[i] = partner->row[i];
     } else {
@@ -275,9 +279,11 @@ __device__ __forceinline__ void load_group(gptr<const T> X, gptr<const T> R, gpt
 // of the previous group's or the first slot of the next group's row chunk whenever it is
 // inside the row, taken from this lane or a neighbouring lane; only the first and last
 // lanes of a wavefront load it. Every lane must call this (the exchange is wavefront-wide).
+// x_known: x is the constant xv (load_group).
 template <class T, int V, int NG>
 __device__ __forceinline__ void group_edges(gptr<const T> X, const Geom& g, int c,
-                                            SlotGroup<T, V> (&G)[NG], bool x_zero) {
+                                            SlotGroup<T, V> (&G)[NG], bool x_known,
+                                            T xv = T(0)) {
   constexpr int W = SlotGroup<T, V>::W;
   constexpr int NC = SlotGroup<T, V>::NC;
   const T prev_lane = __shfl_up(G[NG - 1].row[NC - 1][W - 1], 1, 64);
@@ -285,8 +291,8 @@ __device__ __forceinline__ void group_edges(gptr<const T> X, const Geom& g, int 
   const int lid = (int)(threadIdx.x & 63u);
 #pragma unroll
   for (int k = 0; k < NG; ++k) {
-    if (x_zero) {
-      G[k].edge = T(0);
+    if (x_known) {
+      G[k].edge = xv;
       continue;
     }
     const uint32_t k0 = (uint32_t)(G[k].x0 - G[k].p) >> 1;

@@ -133,7 +133,7 @@ void tune_class(int cls, int reps, hipStream_t s) {
                    (const double*)scr.p + k * W, r32.p + k * W, e32.p + k * W});
     DevArray<TopPtrs> ttab(K);
     MPR_HIP(hipMemcpy(ttab.p, t.data(), K * sizeof(t[0]), hipMemcpyHostToDevice));
-    tune_one(kTuneDefect, true, cls, [&] { launch_b_defect(ttab.p, K, g, 4.0, s); }, reps, s);
+    tune_one(kTuneDefect, true, cls, [&] { launch_b_defect(ttab.p, K, g, 4.0, false, s); }, reps, s);
   }
   MPR_HIP(hipStreamSynchronize(s));
 }

@@ -60,7 +60,8 @@ void BatchedVcycle<T>::build(const std::vector<InstanceLevels<T>>& inst, void* t
       const LevelPtrs<T>& b = i.levels[k + 1];
       const bool top = i.depth == s;
       StepPtrs<T> e{a.x, a.rhs, a.screen, a.res, b.x, b.rhs, top ? 1 : 0,
-                    top && i.live_rhs ? 1 : 0, top ? i.live : nullptr, top ? i.x64 : nullptr};
+                    top && i.live_rhs ? 1 : 0, top ? i.live : nullptr, top ? i.x64 : nullptr,
+                    top ? i.x_initial : nullptr};
       steps.push_back(e);
       if (i.depth == s) tops.push_back(e);
     }
@@ -94,7 +95,7 @@ void BatchedVcycle<T>::build(const std::vector<InstanceLevels<T>>& inst, void* t
 
 template <class T>
 void BatchedVcycle<T>::run(double pw, int pre, int post, bool zero_tops, hipStream_t s,
-                           bool fuse_correct) const {
+                           bool fuse_correct, bool first) const {
   // A level that starts at zero (core.py:174 solution[l+1].fill(0); every level of the
   // correction equation when zero_tops) needs no fill when pre-smoothing runs: the first
   // colour-0 sweep reads the other colour as zero, and every node of that colour has a
@@ -122,7 +123,7 @@ void BatchedVcycle<T>::run(double pw, int pre, int post, bool zero_tops, hipStre
     launch_b_prolong<T>(step_[c], step_count_[c], geom_[c - 1], geom_[c], post > 0 && pw >= 0.0,
                         s);
     launch_b_relax<T>(step_[c], step_count_[c], geom_[c], pw, post, s, kZeroNone,
-                      fuse_correct && post > 0);
+                      fuse_correct && post > 0, first);
   }
 }
 
@@ -147,8 +148,9 @@ void DefectTables::build(const std::vector<int>& depths, const std::vector<TopPt
     MPR_HIP(hipMemcpy(table_mem, w.host.data(), w.host.size(), hipMemcpyHostToDevice));
 }
 
-void DefectTables::defect(double pw, hipStream_t s) const {
-  for (size_t d = 1; d < count_.size(); ++d) launch_b_defect(tab_[d], count_[d], geom_[d], pw, s);
+void DefectTables::defect(double pw, hipStream_t s, bool first) const {
+  for (size_t d = 1; d < count_.size(); ++d)
+    launch_b_defect(tab_[d], count_[d], geom_[d], pw, first, s);
 }
 
 void DefectTables::correct(hipStream_t s, int max_class) const {

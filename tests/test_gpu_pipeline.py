@@ -285,6 +285,44 @@ def test_sparse_assembly_over_a_sequence_equals_fresh_dense_runs(precision):
         job.close()
 
 
+def test_mixed_runs_with_and_without_the_initial_fill_match_fresh_jobs():
+    # A mixed-precision run whose first cycle fuses the correction into the FP32 sweep
+    # (post-smoothing, a grid above the LDS-resident classes) does not fill the FP64
+    # solution with the initial guess 0.5: the first defect reads it as 0.5 and the first
+    # correction stores 0.5 + e. Runs on one job that take the filled path (no
+    # post-smoothing, no cycles) and runs that do not, in any order, give bit for bit what
+    # a fresh job gives with the same parameters.
+    from mojo_poissonrecon import _lib
+    from mojo_poissonrecon.core import Job
+    p, n = shapes.torus(5000, seed=4)
+    unit = oracle.normalize(p, 1.1)[0]
+    stages = _lib.STAGE_SOLVE | _lib.STAGE_EXTRACT
+    runs = [{}, {"post_smooth": 0}, {"post_smooth": 2, "cycles": 1}, {"cycles": 0},
+            {"pre_smooth": 0}, {}]
+
+    def run(job, kw):
+        job.set_points(unit, n)
+        job.run(Job.params(stages, **kw))
+        r = job.wait()
+        return r, job.fetch_mesh(r)
+
+    job = Job(6, 5000, _lib.PRECISION_MIXED)
+    try:
+        for kw in runs:
+            r, mesh = run(job, kw)
+            fresh = Job(6, 5000, _lib.PRECISION_MIXED)
+            try:
+                rw, want = run(fresh, kw)
+            finally:
+                fresh.close()
+            assert r.iso_value == rw.iso_value and r.residual_norm == rw.residual_norm, kw
+            assert all(np.array_equal(a, b) for a, b in zip(mesh, want)), kw
+            if kw.get("cycles", 4):
+                assert len(mesh[0]) > 0
+    finally:
+        job.close()
+
+
 def test_fetch_grids_needs_a_grids_run():
     from mojo_poissonrecon import _lib
     from mojo_poissonrecon.core import Job
