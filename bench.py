@@ -315,7 +315,7 @@ def main() -> int:
         t0 = time.perf_counter()
         chosen = tune.autotune(max_depth=max(5, min(args.depth, 10)), reps=20)
         log(f"autotune {time.perf_counter() - t0:.2f} s:",
-            " ".join(f"{k}/{p}/{s}={b}x{g}" for (k, p, s), (b, g) in sorted(chosen.items())))
+            " ".join(f"{k}/{p}/{s}={b}x{g}x{z}" for (k, p, s), (b, g, z) in sorted(chosen.items())))
 
     points, normals = shapes.torus_workload(args.points, seed=0)
     stages = (_lib.STAGE_NORMALIZE | _lib.STAGE_SOLVE | _lib.STAGE_EXTRACT | _lib.STAGE_WORLD)

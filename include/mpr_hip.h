@@ -216,15 +216,18 @@ int mpr_job_time_kernel(mpr_job *job, int32_t kernel, int32_t reps, double *ms_p
 /* Launch-shape autotuner of the batched slot-group kernels (colour sweep = 0, residual = 1,
  * FP64->FP32 defect residual = 2) on levels n = 2^s + 1, s = 5..max_depth (max_depth in
  * [5, 10]): times the workgroup sizes 64, 128, 256, 512 and 1024 (for the colour sweep also
- * with 1 or 2 slot groups per lane) per (kernel, precision, s) on up to 8 synthetic
+ * with 2 slot groups per lane, or with one slot group in each of 2 or 4 consecutive planes
+ * per lane, the z-column form) per (kernel, precision, s) on up to 8 synthetic
  * instances (within 4 GiB, at least one instance; it allocates and frees them), 7
  * interleaved rounds of `reps` launches each, and keeps the shape with the fastest median
- * round for later launches in this process if it beats the default (256, 1) by 1.5%.
- * Results do not depend on the launch shape. If `table` / `groups` are not NULL they
- * receive the chosen workgroup sizes / groups per lane, int32[3][2][11] indexed
- * [kernel][fp64][s]. Run it before capturing graphs (captured launches keep the shape they
- * were recorded with). No reference counterpart (launch tuning). */
-int mpr_autotune(int32_t max_depth, int32_t reps, int32_t *table, int32_t *groups);
+ * round for later launches in this process if it beats the default (256, 1, 1) by 1.5%.
+ * Results do not depend on the launch shape. If `table` / `groups` / `planes` are not NULL
+ * they receive the chosen workgroup sizes / groups per lane / planes per lane,
+ * int32[3][2][11] indexed [kernel][fp64][s]. Run it before capturing graphs (captured
+ * launches keep the shape they were recorded with). No reference counterpart (launch
+ * tuning). */
+int mpr_autotune(int32_t max_depth, int32_t reps, int32_t *table, int32_t *groups,
+                 int32_t *planes);
 /* current workgroup size of (kernel, fp64, s), or -1 */
 int mpr_tune_get(int32_t kernel, int32_t fp64, int32_t size_class);
 /* sets it (block one of 64, 128, 256, 512, 1024); 0 or -1 */
@@ -233,6 +236,11 @@ int mpr_tune_set(int32_t kernel, int32_t fp64, int32_t size_class, int32_t block
  * a choice), or -1; and its setter (0 or -1) */
 int mpr_tune_get_groups(int32_t kernel, int32_t fp64, int32_t size_class);
 int mpr_tune_set_groups(int32_t kernel, int32_t fp64, int32_t size_class, int32_t groups);
+/* planes per lane of (kernel, fp64, s) (1, 2 or 4; only the colour sweep has a choice;
+ * above 1 the sweep runs in z-column form with one slot group per plane, whatever the
+ * groups setting), or -1; and its setter (0 or -1) */
+int mpr_tune_get_planes(int32_t kernel, int32_t fp64, int32_t size_class);
+int mpr_tune_set_planes(int32_t kernel, int32_t fp64, int32_t size_class, int32_t planes);
 
 /* Extraction of a caller grid (core.py:309-374): field and density host (n,n,n) float64
  * (density may be NULL); returns counts; then fetch and destroy. */

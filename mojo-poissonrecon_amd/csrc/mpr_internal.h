@@ -91,9 +91,11 @@ void launch_scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t count, ui
 // words of `scratch` that let launch_scan_exclusive use many workgroups for `count` values
 uint32_t scan_scratch_words(uint32_t count);
 size_t radix_scratch_words(uint32_t n);
+// gather_dst (optional, n doubles): gather_src[vals[i]] in the sorted order
 template <class K>  // uint32_t (splat node keys) or uint64_t (Morton keys)
 void launch_radix_sort(K* keys, uint32_t* vals, K* k_tmp, uint32_t* v_tmp, uint32_t n, int bits,
-                       uint32_t* hist, uint32_t* offs, hipStream_t s);
+                       uint32_t* hist, uint32_t* offs, hipStream_t s,
+                       const double* gather_src = nullptr, double* gather_dst = nullptr);
 void launch_leaf_keys(const double* pts, uint32_t P, int depth, uint64_t* keys, uint32_t* perm,
                       hipStream_t s);
 uint32_t compaction_tiles(uint64_t n);

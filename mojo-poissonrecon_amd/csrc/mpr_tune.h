@@ -28,5 +28,10 @@ bool valid_block(int block);
 constexpr int kTuneGroupCandidates[] = {1, 2};
 int tuned_groups(int kernel, bool fp64, int size_class);
 void set_tuned_groups(int kernel, bool fp64, int size_class, int groups);
+// planes per lane of the colour sweep's z-column form (1: the plain form; 2 or 4 with one
+// slot group per plane; 1 for the other kernels)
+constexpr int kTunePlaneCandidates[] = {1, 2, 4};
+int tuned_planes(int kernel, bool fp64, int size_class);
+void set_tuned_planes(int kernel, bool fp64, int size_class, int planes);
 
 }  // namespace mpr

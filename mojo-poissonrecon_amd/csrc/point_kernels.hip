@@ -170,11 +170,14 @@ __global__ __launch_bounds__(kThreads) void k_radix_hist(const K* keys, uint32_t
   hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];  // digit-major
 }
 
+// gdst (optional): gdst[pos] = gsrc[val] beside each moved pair, a payload gathered into
+// the sorted order
 template <class K>
 __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* kin, const uint32_t* vin,
                                                             K* kout, uint32_t* vout, uint32_t n,
                                                             int shift, const uint32_t* offsets,
-                                                            uint32_t nblocks) {
+                                                            uint32_t nblocks, const double* gsrc,
+                                                            double* gdst) {
   __shared__ uint32_t wave_cnt[4][256];
   __shared__ uint32_t running[256];
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -206,6 +209,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* kin, const 
       for (uint32_t w = 0; w < wave; ++w) pos += wave_cnt[w][digit];
       kout[pos] = key;
       vout[pos] = val;
+      if (gdst) gdst[pos] = gsrc[val];
     }
     __syncthreads();
     running[threadIdx.x] += wave_cnt[0][threadIdx.x] + wave_cnt[1][threadIdx.x] +
@@ -439,18 +443,49 @@ __global__ __launch_bounds__(kThreads) void k_splat_pairs(const double* pts, con
   if (threadIdx.x == 0 && c) atomicAdd(accepted, (unsigned long long)c);
 }
 
-// one lane per node: density += 2 w and screen += w over its sorted contributions. With
-// `touched`, the grids count as zero-filled (the sums start at +0.0, as on the reference's
-// fresh grids) and the node's bit is set instead of reading the grid.
-__global__ __launch_bounds__(kThreads) void k_splat_sum1(const uint32_t* keys,
-                                                         const uint32_t* ids, const double* w1,
+// Runs of equal keys in the sorted pairs: position j starts a run when its key differs from
+// position j - 1's. The lane of a starting position learns from a wavefront ballot of the run
+// starts where the run's part in this wavefront ends, so it can issue those loads together;
+// only a run that reaches the wavefront's end is followed on key by key. Every lane of the
+// wavefront must call this (it exchanges keys).
+struct RunStart {
+  bool head;     // position j starts the run of a node (key < M)
+  bool open;     // the run may continue past `end`, the wavefront's end
+  uint32_t key;
+  uint32_t end;  // first position past the run's part in this wavefront (at most count)
+};
+
+__device__ __forceinline__ RunStart run_start(const uint32_t* keys, uint32_t j, uint32_t count,
+                                              uint32_t M) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const bool valid = j < count;
+  const uint32_t key = valid ? keys[j] : 0xffffffffu;
+  uint32_t prev = __shfl_up(key, 1, 64);
+  if (lane == 0) prev = valid && j > 0 ? keys[j - 1] : ~key;
+  const bool starts = !valid || key != prev;
+  const uint64_t sm = __ballot(starts);
+  const uint64_t above = lane == 63u ? 0ull : sm & (~0ull << (lane + 1u));
+  RunStart r;
+  r.key = key;
+  r.head = valid && starts && key < M;
+  r.open = above == 0ull;
+  r.end = (j - lane) + (r.open ? 64u : (uint32_t)__builtin_ctzll(above));
+  if (r.end > count) r.end = count;
+  return r;
+}
+
+// one lane per node: density += 2 w and screen += w over its contributions, in point order
+// (ws: pass 1's weights in the sorted order). With `touched`, the grids count as
+// zero-filled (the sums start at +0.0, as on the reference's fresh grids) and the node's bit
+// is set instead of reading the grid.
+__global__ __launch_bounds__(kThreads) void k_splat_sum1(const uint32_t* keys, const double* ws,
                                                          uint32_t count, uint32_t M,
                                                          double* density, double* screen,
                                                          uint32_t* touched) {
   const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
-  if (j >= count) return;
-  const uint32_t key = keys[j];
-  if (key >= M || (j > 0 && keys[j - 1] == key)) return;
+  const RunStart r = run_start(keys, j, count, M);
+  if (!r.head) return;
+  const uint32_t key = r.key;
   double d = 0.0, sc = 0.0;
   if (touched) {
     atomicOr(&touched[key >> 5], 1u << (key & 31u));
@@ -458,11 +493,29 @@ __global__ __launch_bounds__(kThreads) void k_splat_sum1(const uint32_t* keys,
     d = density[key];
     sc = screen[key];
   }
-  for (uint32_t i = j; i < count && keys[i] == key; ++i) {
-    const double w = w1[ids[i]];
+  uint32_t i = j;
+  for (; i + 4 <= r.end; i += 4) {
+    const double w0 = ws[i], w1 = ws[i + 1], w2 = ws[i + 2], w3 = ws[i + 3];
+    d += 2.0 * w0;
+    sc += w0;
+    d += 2.0 * w1;
+    sc += w1;
+    d += 2.0 * w2;
+    sc += w2;
+    d += 2.0 * w3;
+    sc += w3;
+  }
+  for (; i < r.end; ++i) {
+    const double w = ws[i];
     d += 2.0 * w;
     sc += w;
   }
+  if (r.open)
+    for (; i < count && keys[i] == key; ++i) {
+      const double w = ws[i];
+      d += 2.0 * w;
+      sc += w;
+    }
   density[key] = d;
   screen[key] = sc;
 }
@@ -524,28 +577,54 @@ __global__ __launch_bounds__(kThreads) void k_splat_pass2_pairs(const double* pt
   }
 }
 
-// one lane per node: v -= n^ vw over its sorted contributions (from +0.0 when the grids
-// count as zero-filled, as k_splat_sum1)
+// one lane per node: v -= n^ vw over its contributions in point order (from +0.0 when the
+// grids count as zero-filled, as k_splat_sum1)
 __global__ __launch_bounds__(kThreads) void k_splat_sum2(const uint32_t* keys,
                                                          const uint32_t* ids, const double* c2,
                                                          uint32_t count, uint32_t M, double* vx,
                                                          double* vy, double* vz, int from_zero) {
   const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
-  if (j >= count) return;
-  const uint32_t key = keys[j];
-  if (key >= M || (j > 0 && keys[j - 1] == key)) return;
+  const RunStart r = run_start(keys, j, count, M);
+  if (!r.head) return;
+  const uint32_t key = r.key;
   double a = 0.0, b = 0.0, c = 0.0;
   if (!from_zero) {
     a = vx[key];
     b = vy[key];
     c = vz[key];
   }
-  for (uint32_t i = j; i < count && keys[i] == key; ++i) {
+  uint32_t i = j;
+  for (; i + 4 <= r.end; i += 4) {
+    const double* q[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) q[k] = c2 + 3ull * ids[i + k];
+    double v[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      v[k][0] = q[k][0];
+      v[k][1] = q[k][1];
+      v[k][2] = q[k][2];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      a -= v[k][0];
+      b -= v[k][1];
+      c -= v[k][2];
+    }
+  }
+  for (; i < r.end; ++i) {
     const double* q = c2 + 3ull * ids[i];
     a -= q[0];
     b -= q[1];
     c -= q[2];
   }
+  if (r.open)
+    for (; i < count && keys[i] == key; ++i) {
+      const double* q = c2 + 3ull * ids[i];
+      a -= q[0];
+      b -= q[1];
+      c -= q[2];
+    }
   vx[key] = a;
   vy[key] = b;
   vz[key] = c;
@@ -672,10 +751,13 @@ size_t radix_scratch_words(uint32_t n) {
 
 // Sorts (keys, vals) by the low `bits` key bits, stably. The result ends in (keys, vals);
 // (k_tmp, v_tmp) are ping-pong buffers; hist and offs hold radix_scratch_words(n) words.
+// gather_dst (optional, n words): gather_dst[i] = gather_src[vals[i]] in the sorted order,
+// written by the last pass.
 template <class K>
 void launch_radix_sort(K* keys, uint32_t* vals, K* k_tmp, uint32_t* v_tmp, uint32_t n, int bits,
-                       uint32_t* hist, uint32_t* offs, hipStream_t s) {
-  if (n <= 1) return;
+                       uint32_t* hist, uint32_t* offs, hipStream_t s, const double* gather_src,
+                       double* gather_dst) {
+  if (n == 0 || (n == 1 && !gather_dst)) return;
   if (bits > 8 * (int)sizeof(K)) throw HipError{"radix sort: more key bits than the key holds"};
   const uint32_t nb = (n + kRadixTile - 1) / kRadixTile;
   const int passes = (bits + 7) / 8;
@@ -686,8 +768,10 @@ void launch_radix_sort(K* keys, uint32_t* vals, K* k_tmp, uint32_t* v_tmp, uint3
     hipLaunchKernelGGL(k_radix_hist<K>, dim3(nb), dim3(kThreads), 0, s, ka, n, shift, hist, nb);
     MPR_LAUNCH_CHECK();
     launch_scan_exclusive(hist, offs, 256u * nb, nullptr, s, hist + 256ull * nb);
+    const bool last = p == passes - 1;
     hipLaunchKernelGGL(k_radix_scatter<K>, dim3(nb), dim3(kThreads), 0, s, ka, va, kb, vb, n,
-                       shift, offs, nb);
+                       shift, offs, nb, last ? gather_src : nullptr,
+                       last ? gather_dst : nullptr);
     MPR_LAUNCH_CHECK();
     std::swap(ka, kb);
     std::swap(va, vb);
@@ -698,9 +782,11 @@ void launch_radix_sort(K* keys, uint32_t* vals, K* k_tmp, uint32_t* v_tmp, uint3
   }
 }
 template void launch_radix_sort<uint32_t>(uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t,
-                                          int, uint32_t*, uint32_t*, hipStream_t);
+                                          int, uint32_t*, uint32_t*, hipStream_t, const double*,
+                                          double*);
 template void launch_radix_sort<uint64_t>(uint64_t*, uint32_t*, uint64_t*, uint32_t*, uint32_t,
-                                          int, uint32_t*, uint32_t*, hipStream_t);
+                                          int, uint32_t*, uint32_t*, hipStream_t, const double*,
+                                          double*);
 
 void launch_leaf_keys(const double* pts, uint32_t P, int depth, uint64_t* keys, uint32_t* perm,
                       hipStream_t s) {
@@ -768,8 +854,11 @@ void launch_splat(const double* pts, const double* nrm, uint32_t P, int n, int c
   hipLaunchKernelGGL(k_splat_pairs, dim3(nb), dim3(kThreads), 0, s, pts, nrm, P, n, confidence,
                      b.keys, b.ids, b.w1, M, accepted);
   MPR_LAUNCH_CHECK();
-  launch_radix_sort(b.keys, b.ids, b.ktmp, b.itmp, pairs, bits, b.hist, b.offs, s);
-  hipLaunchKernelGGL(k_splat_sum1, dim3(np), dim3(kThreads), 0, s, b.keys, b.ids, b.w1, pairs, M,
+  // the sort's last pass also lays pass 1's weights out in the sorted order, in the space
+  // of pass 2's contributions (written after k_splat_sum1 has read them)
+  double* ws = b.c2;
+  launch_radix_sort(b.keys, b.ids, b.ktmp, b.itmp, pairs, bits, b.hist, b.offs, s, b.w1, ws);
+  hipLaunchKernelGGL(k_splat_sum1, dim3(np), dim3(kThreads), 0, s, b.keys, ws, pairs, M,
                      density, screen, touched);
   MPR_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_splat_pass2_pairs, dim3(nb), dim3(kThreads), 0, s, pts, nrm, P, n,

@@ -66,6 +66,8 @@ struct DevArena {
   ~DevArena() { release(); }
 };
 
+// the scalars a run reports, laid out alike on the device and in pinned host memory so that
+// one copy reads them back
 struct HostScalars {
   unsigned long long accepted;
   uint32_t counts[2];  // active cells, quads
@@ -93,7 +95,8 @@ struct Job {
   BatchedVcycle<double> vc64;
   BatchedVcycle<float> vc32;
   DefectTables dtab;
-  // scalars and partial sums
+  // scalars (fields of one device HostScalars block) and partial sums
+  HostScalars* d_scalars = nullptr;
   unsigned long long* d_accepted = nullptr;
   double *d_iso = nullptr, *d_resnorm = nullptr, *d_params = nullptr;
   uint32_t* d_counts = nullptr;
@@ -444,11 +447,14 @@ void Job::plan(DevArena& a) {
       L32.push_back(f);
     }
   }
-  d_accepted = a.alloc<unsigned long long>(1);
-  d_iso = a.alloc<double>(1);
-  d_resnorm = a.alloc<double>(1);
-  d_params = a.alloc<double>(4);
-  d_counts = a.alloc<uint32_t>(2);
+  d_scalars = a.alloc<HostScalars>(1);
+  if (d_scalars) {
+    d_accepted = &d_scalars->accepted;
+    d_counts = d_scalars->counts;
+    d_iso = &d_scalars->iso;
+    d_resnorm = &d_scalars->resnorm;
+    d_params = d_scalars->params;
+  }
   norm_nparts = residual_partial_count(geoms[0], sizeof(double));
   norm_part = a.alloc<double>(norm_nparts);
   bbox_part = a.alloc<double>(6ull * bbox_partial_count(std::max<uint32_t>(1u, max_points)));
@@ -577,12 +583,7 @@ void Job::finish(const mpr_params& p, hipStream_t s) {
 }
 
 void Job::readback(hipStream_t s) {
-  MPR_HIP(hipMemcpyAsync(&host->accepted, d_accepted, sizeof(unsigned long long),
-                         hipMemcpyDeviceToHost, s));
-  MPR_HIP(hipMemcpyAsync(host->counts, d_counts, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  MPR_HIP(hipMemcpyAsync(&host->iso, d_iso, sizeof(double), hipMemcpyDeviceToHost, s));
-  MPR_HIP(hipMemcpyAsync(&host->resnorm, d_resnorm, sizeof(double), hipMemcpyDeviceToHost, s));
-  MPR_HIP(hipMemcpyAsync(host->params, d_params, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+  MPR_HIP(hipMemcpyAsync(host, d_scalars, sizeof(HostScalars), hipMemcpyDeviceToHost, s));
 }
 
 void Job::enqueue(const mpr_params& p) {

@@ -15,6 +15,7 @@ namespace {
 
 std::atomic<int> g_blocks[kTuneKernels][2][kTuneClasses];
 std::atomic<int> g_groups[kTuneKernels][2][kTuneClasses];
+std::atomic<int> g_planes[kTuneKernels][2][kTuneClasses];
 
 struct TableInit {
   TableInit() {
@@ -22,6 +23,9 @@ struct TableInit {
       for (auto& t : k)
         for (auto& c : t) c.store(kTuneDefault);
     for (auto& k : g_groups)
+      for (auto& t : k)
+        for (auto& c : t) c.store(1);
+    for (auto& k : g_planes)
       for (auto& t : k)
         for (auto& c : t) c.store(1);
   }
@@ -57,9 +61,10 @@ double time_launches(F&& launch, int reps, hipStream_t s, hipEvent_t e0, hipEven
 constexpr int kTuneRounds = 7;
 constexpr double kTuneMargin = 0.015;
 
-// (workgroup size, slot groups per lane) candidates; the default shape is (256, 1)
+// (workgroup size, slot groups per lane, z-column planes per lane) candidates; the default
+// shape is (256, 1, 1)
 struct Shape {
-  int block, groups;
+  int block, groups, planes;
 };
 
 template <class F>
@@ -67,12 +72,18 @@ void tune_one(int kernel, bool fp64, int cls, F&& launch, int reps, hipStream_t 
   std::vector<Shape> cand;
   for (int g : kTuneGroupCandidates) {
     if (g != 1 && kernel != kTuneRelax) continue;
-    for (int b : kTuneCandidates) cand.push_back({b, g});
+    for (int b : kTuneCandidates) cand.push_back({b, g, 1});
   }
+  if (kernel == kTuneRelax)
+    for (int z : kTunePlaneCandidates) {
+      if (z == 1) continue;
+      for (int b : kTuneCandidates) cand.push_back({b, 1, z});
+    }
   const size_t nc = cand.size();
   auto apply = [&](const Shape& sh) {
     set_tuned_block(kernel, fp64, cls, sh.block);
     set_tuned_groups(kernel, fp64, cls, sh.groups);
+    set_tuned_planes(kernel, fp64, cls, sh.planes);
   };
   hipEvent_t e0, e1;
   MPR_HIP(hipEventCreate(&e0));
@@ -95,7 +106,7 @@ void tune_one(int kernel, bool fp64, int cls, F&& launch, int reps, hipStream_t 
     std::sort(ms[c].begin(), ms[c].end());
     med[c] = ms[c][ms[c].size() / 2];
     if (med[c] < med[best]) best = c;
-    if (cand[c].block == kTuneDefault && cand[c].groups == 1) dflt = c;
+    if (cand[c].block == kTuneDefault && cand[c].groups == 1 && cand[c].planes == 1) dflt = c;
   }
   if (med[best] > (1.0 - kTuneMargin) * med[dflt]) best = dflt;
   apply(cand[best]);
@@ -172,13 +183,33 @@ void set_tuned_groups(int kernel, bool fp64, int size_class, int groups) {
   g_groups[kernel][fp64 ? 1 : 0][size_class].store(groups, std::memory_order_relaxed);
 }
 
+bool valid_planes(int planes) {
+  for (int z : kTunePlaneCandidates)
+    if (z == planes) return true;
+  return false;
+}
+
+int tuned_planes(int kernel, bool fp64, int size_class) {
+  if (kernel < 0 || kernel >= kTuneKernels || size_class < 0 || size_class >= kTuneClasses)
+    return 1;
+  return g_planes[kernel][fp64 ? 1 : 0][size_class].load(std::memory_order_relaxed);
+}
+
+void set_tuned_planes(int kernel, bool fp64, int size_class, int planes) {
+  if (kernel != kTuneRelax || size_class < 0 || size_class >= kTuneClasses ||
+      !valid_planes(planes))
+    return;
+  g_planes[kernel][fp64 ? 1 : 0][size_class].store(planes, std::memory_order_relaxed);
+}
+
 }  // namespace mpr
 
 using namespace mpr;
 
 extern "C" {
 
-int mpr_autotune(int32_t max_depth, int32_t reps, int32_t* table, int32_t* groups) {
+int mpr_autotune(int32_t max_depth, int32_t reps, int32_t* table, int32_t* groups,
+                 int32_t* planes) {
   if (max_depth < 5 || max_depth > 10 || reps < 1) {
     set_error("mpr_autotune: max_depth must be in [5, 10] and reps positive");
     return -1;
@@ -203,6 +234,7 @@ int mpr_autotune(int32_t max_depth, int32_t reps, int32_t* table, int32_t* group
       for (int c = 0; c < kTuneClasses; ++c) {
         if (table) table[(k * 2 + t) * kTuneClasses + c] = tuned_block(k, t == 1, c);
         if (groups) groups[(k * 2 + t) * kTuneClasses + c] = tuned_groups(k, t == 1, c);
+        if (planes) planes[(k * 2 + t) * kTuneClasses + c] = tuned_planes(k, t == 1, c);
       }
   return 0;
 }
@@ -240,6 +272,24 @@ int mpr_tune_set_groups(int32_t kernel, int32_t fp64, int32_t size_class, int32_
     return -1;
   }
   set_tuned_groups(kernel, fp64 != 0, size_class, groups);
+  return 0;
+}
+
+int mpr_tune_get_planes(int32_t kernel, int32_t fp64, int32_t size_class) {
+  if (kernel < 0 || kernel >= kTuneKernels || size_class < 0 || size_class >= kTuneClasses) {
+    set_error("mpr_tune_get_planes: no such kernel or size class");
+    return -1;
+  }
+  return tuned_planes(kernel, fp64 != 0, size_class);
+}
+
+int mpr_tune_set_planes(int32_t kernel, int32_t fp64, int32_t size_class, int32_t planes) {
+  if (kernel != kTuneRelax || size_class < 0 || size_class >= kTuneClasses ||
+      !valid_planes(planes)) {
+    set_error("mpr_tune_set_planes: kernel 0 (colour sweep), size class 0..10, planes 1/2/4");
+    return -1;
+  }
+  set_tuned_planes(kernel, fp64 != 0, size_class, planes);
   return 0;
 }
 

@@ -99,11 +99,13 @@ _PIPELINE = {
     "mpr_job_device_points": ([P, _PP, _PP], ctypes.c_int),
     "mpr_job_set_count": ([P, I], ctypes.c_int),
     "mpr_job_time_kernel": ([P, S, S, ctypes.POINTER(F), ctypes.POINTER(F)], ctypes.c_int),
-    "mpr_autotune": ([S, S, P, P], ctypes.c_int),
+    "mpr_autotune": ([S, S, P, P, P], ctypes.c_int),
     "mpr_tune_get": ([S, S, S], ctypes.c_int),
     "mpr_tune_set": ([S, S, S, S], ctypes.c_int),
     "mpr_tune_get_groups": ([S, S, S], ctypes.c_int),
     "mpr_tune_set_groups": ([S, S, S, S], ctypes.c_int),
+    "mpr_tune_get_planes": ([S, S, S], ctypes.c_int),
+    "mpr_tune_set_planes": ([S, S, S, S], ctypes.c_int),
     "mpr_batch_create": ([P, S, _PP], ctypes.c_int),
     "mpr_batch_destroy": ([P], ctypes.c_int),
     "mpr_batch_run": ([P, ctypes.POINTER(mpr_params)], ctypes.c_int),

@@ -1,9 +1,10 @@
 """Launch-shape tuning of the batched multigrid kernels (include/mpr_hip.h mpr_autotune).
 
 The colour sweep, residual and FP64->FP32 defect kernels of every V-cycle (solo and batched)
-launch with a workgroup size (and, for the colour sweep, 1 or 2 slot groups per lane)
-looked up per (kernel, precision, grid size class). autotune() times the candidates on the device and keeps the fastest for the rest of
-the process. The per-node arithmetic does not depend on the launch shape, so results stay
+launch with a workgroup size looked up per (kernel, precision, grid size class); the colour
+sweep also with 1 or 2 slot groups per lane, or in z-column form with one slot group in each
+of 2 or 4 consecutive planes per lane. autotune() times the candidates on the device and
+keeps the fastest for the rest of the process. The per-node arithmetic does not depend on the launch shape, so results stay
 bit-identical.
 """
 
@@ -18,23 +19,27 @@ from ._lib import check, lib, load_library
 KERNELS = ("relax", "residual", "defect")
 CANDIDATES = (64, 128, 256, 512, 1024)
 GROUP_CANDIDATES = (1, 2)
+PLANE_CANDIDATES = (1, 2, 4)
 SIZE_CLASSES = 11
 
 
 def autotune(max_depth: int = 8, reps: int = 20) -> dict:
     """Times every candidate launch shape for grid classes 5..max_depth and keeps the
-    fastest. Returns {(kernel, precision, class): (block, groups per lane)}."""
+    fastest. Returns {(kernel, precision, class): (block, groups per lane, planes per lane)}."""
     blocks = np.zeros((len(KERNELS), 2, SIZE_CLASSES), np.int32)
     groups = np.zeros_like(blocks)
+    planes = np.zeros_like(blocks)
     check(lib().mpr_autotune(int(max_depth), int(reps), blocks.ctypes.data_as(ctypes.c_void_p),
-                             groups.ctypes.data_as(ctypes.c_void_p)), "mpr_autotune")
+                             groups.ctypes.data_as(ctypes.c_void_p),
+                             planes.ctypes.data_as(ctypes.c_void_p)), "mpr_autotune")
     out = {}
     for k, name in enumerate(KERNELS):
         for t, prec in enumerate(("fp32", "fp64")):
             if name == "defect" and prec == "fp32":
                 continue
             for s in range(5, max_depth + 1):
-                out[(name, prec, s)] = (int(blocks[k, t, s]), int(groups[k, t, s]))
+                out[(name, prec, s)] = (int(blocks[k, t, s]), int(groups[k, t, s]),
+                                        int(planes[k, t, s]))
     return out
 
 
@@ -62,3 +67,14 @@ def tuned_groups(kernel: str, precision: str, size_class: int) -> int:
 def set_tuned_groups(kernel: str, precision: str, size_class: int, groups: int) -> None:
     check(load_library().mpr_tune_set_groups(*_key(kernel, precision, size_class), int(groups)),
           "mpr_tune_set_groups")
+
+
+def tuned_planes(kernel: str, precision: str, size_class: int) -> int:
+    v = load_library().mpr_tune_get_planes(*_key(kernel, precision, size_class))
+    check(-1 if v < 0 else 0, "mpr_tune_get_planes")
+    return v
+
+
+def set_tuned_planes(kernel: str, precision: str, size_class: int, planes: int) -> None:
+    check(load_library().mpr_tune_set_planes(*_key(kernel, precision, size_class), int(planes)),
+          "mpr_tune_set_planes")

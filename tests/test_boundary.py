@@ -195,3 +195,13 @@ def test_workgroup_size_table_without_a_device():
     for kernel, groups in (("relax", 3), ("relax", 0), ("residual", 2)):
         with pytest.raises(DeviceError):
             tune.set_tuned_groups(kernel, "fp32", 8, groups)
+    assert tune.tuned_planes("relax", "fp64", 7) == 1
+    tune.set_tuned_planes("relax", "fp64", 7, 4)
+    try:
+        assert tune.tuned_planes("relax", "fp64", 7) == 4
+        assert tune.tuned_planes("relax", "fp32", 7) == 1
+    finally:
+        tune.set_tuned_planes("relax", "fp64", 7, 1)
+    for kernel, planes in (("relax", 3), ("relax", 8), ("relax", 0), ("defect", 2)):
+        with pytest.raises(DeviceError):
+            tune.set_tuned_planes(kernel, "fp64", 7, planes)

@@ -159,24 +159,32 @@ def test_launch_shapes_do_not_change_results():
     from mojo_poissonrecon import tune
     p, n = shapes.torus_workload(20000, seed=5)
     keys = [(k, t, s) for k in tune.KERNELS for t in ("fp32", "fp64") for s in range(5, 8)]
-    saved = {key: (tune.tuned_block(*key), tune.tuned_groups(*key)) for key in keys}
+    saved = {key: (tune.tuned_block(*key), tune.tuned_groups(*key), tune.tuned_planes(*key))
+             for key in keys}
     try:
         base = {prec: mpr.reconstruct(p, n, depth=7, precision=prec) for prec in ("fp64", "mixed")}
-        for block, groups in ((64, 2), (1024, 1), (128, 2)):
+        # (block, groups, planes): planes > 1 is the sweep's z-column form; 129 = 4 * 32 + 1
+        # and 65 = 4 * 16 + 1 nodes per side leave a last column of one plane
+        for block, groups, planes in ((64, 2, 1), (1024, 1, 1), (128, 2, 1), (256, 1, 2),
+                                      (64, 1, 4), (1024, 1, 4)):
             for key in keys:
                 tune.set_tuned_block(*key, block)
                 if key[0] == "relax":
                     tune.set_tuned_groups(*key, groups)
+                    tune.set_tuned_planes(*key, planes)
             for prec, m in base.items():
                 identical_mesh(mpr.reconstruct(p, n, depth=7, precision=prec), m)
         chosen = tune.autotune(max_depth=7, reps=3)
-        assert {b for b, _ in chosen.values()} <= set(tune.CANDIDATES)
-        assert {g for _, g in chosen.values()} <= set(tune.GROUP_CANDIDATES)
+        assert {b for b, _, _ in chosen.values()} <= set(tune.CANDIDATES)
+        assert {g for _, g, _ in chosen.values()} <= set(tune.GROUP_CANDIDATES)
+        assert {z for _, _, z in chosen.values()} <= set(tune.PLANE_CANDIDATES)
+        assert all(g == 1 or z == 1 for _, g, z in chosen.values())
         assert len(chosen) == 3 * 2 + 3 * 2 + 3  # relax, residual x2 precisions; defect fp64
         for prec, m in base.items():
             identical_mesh(mpr.reconstruct(p, n, depth=7, precision=prec), m)
     finally:
-        for key, (b, g) in saved.items():
+        for key, (b, g, z) in saved.items():
             tune.set_tuned_block(*key, b)
             if key[0] == "relax":
                 tune.set_tuned_groups(*key, g)
+                tune.set_tuned_planes(*key, z)
