@@ -368,7 +368,7 @@ template <class T>
 __global__ __launch_bounds__(kThreads) void kb_prolong_group(const StepPtrs<T>* tab,
                                                              uint32_t count, uint32_t nbi, Geom gc,
                                                              Geom gf, uint32_t lanes, int c_first,
-                                                             int ncol) {
+                                                             int ncol, int vec_coarse) {
   const uint32_t per = gridDim.x >> 3;
   const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
   const uint32_t slot = lb / nbi;
@@ -379,7 +379,7 @@ __global__ __launch_bounds__(kThreads) void kb_prolong_group(const StepPtrs<T>* 
   const int k = (ncol == 2 && lane >= lanes) ? 1 : 0;
   const uint32_t l = lane < (uint32_t)ncol * lanes ? lane - (uint32_t)k * lanes : lanes;
   prolong_group<T, kGroup<T>>(as_global((const T*)p.xc), gc, as_global(p.x), gf, c_first + k,
-                              l, lanes);
+                              l, lanes, vec_coarse != 0);
 }
 
 template <class T>
@@ -767,7 +767,8 @@ void launch_b_prolong(const StepPtrs<T>* tab, uint32_t count, const Geom& gc, co
     const int ncol = colour1_only ? 1 : 2;
     const uint32_t nbi = blocks_for((uint64_t)ncol * lanes, kThreads);
     hipLaunchKernelGGL(kb_prolong_group<T>, dim3(padded8((uint64_t)nbi * count)), dim3(kThreads),
-                       0, s, tab, count, nbi, gc, gf, lanes, colour1_only ? 1 : 0, ncol);
+                       0, s, tab, count, nbi, gc, gf, lanes, colour1_only ? 1 : 0, ncol,
+                       group_layout(gc) ? 1 : 0);
     MPR_LAUNCH_CHECK();
     return;
   }

@@ -185,13 +185,13 @@ __global__ __launch_bounds__(kThreads) void k_restrict_group(const T* F, Geom gf
 // slot-group prolong-add over both colours (lanes [0, lanes) colour 0, then colour 1)
 template <class T>
 __global__ __launch_bounds__(kThreads) void k_prolong_group(const T* Cc, Geom gc, T* F, Geom gf,
-                                                            uint32_t lanes) {
+                                                            uint32_t lanes, int vec_coarse) {
   constexpr int V = 16 / (int)sizeof(T);
   const uint32_t lane = logical_block() * kThreads + threadIdx.x;
   const int c = lane >= lanes;
   const uint32_t l = lane - (uint32_t)c * lanes;
   prolong_group<T, V>(as_global(Cc), gc, as_global(F), gf, c, lane < 2u * lanes ? l : lanes,
-                      lanes);
+                      lanes, vec_coarse != 0);
 }
 
 template <class T>
@@ -472,7 +472,7 @@ void launch_prolong_add(const T* Cc, const Geom& gc, T* F, const Geom& gf, hipSt
   if (group_layout(gf)) {
     const uint32_t lanes = gf.C / (uint32_t)(16 / sizeof(T));
     hipLaunchKernelGGL(k_prolong_group<T>, dim3(xcd_grid(2ull * lanes, kThreads)), dim3(kThreads),
-                       0, s, Cc, gc, F, gf, lanes);
+                       0, s, Cc, gc, F, gf, lanes, group_layout(gc) ? 1 : 0);
     MPR_LAUNCH_CHECK();
     return;
   }

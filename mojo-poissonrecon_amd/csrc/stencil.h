@@ -432,11 +432,15 @@ __device__ __forceinline__ void store_group(__attribute__((address_space(1))) U*
 // have the same x parity p, so they share tx, ty, tz and the four coarse rows
 // (y0|y1, z0|z1); node j needs coarse x = k0+j and min(k0+j+1, nc-1), so each row is read
 // once at x = k0..k0+V (coarse nodes alternate colour along x). Per node the arithmetic is
-// prolong_node's followed by fine += value.
+// prolong_node's followed by fine += value. vec_coarse (FP32, coarse rows 16-byte aligned):
+// coarse x = k0..k0+4 are slots k0/2..k0/2+2 of one colour and k0/2..k0/2+1 of the other
+// (k0 is a multiple of 4), read as two 8-byte loads and one scalar per row instead of five
+// scalars; x past the last node takes the last node's value in registers, as the clamped
+// index does.
 template <class T, int V>
 __device__ __forceinline__ void prolong_group(gptr<const T> XC, const Geom& gc, gptr<T> F,
                                               const Geom& gf, int c, uint32_t lane,
-                                              uint32_t lanes) {
+                                              uint32_t lanes, bool vec_coarse = false) {
   const bool in = lane < lanes;
   lane = in ? lane : lanes - 1;
   const uint32_t s0 = lane * (uint32_t)V;
@@ -454,15 +458,45 @@ __device__ __forceinline__ void prolong_group(gptr<const T> XC, const Geom& gc, 
   const int z1 = z0 + 1 < nc - 1 ? z0 + 1 : nc - 1;
   const T tx = p ? T(0.5) : T(0), ty = (y & 1) ? T(0.5) : T(0), tz = (z & 1) ? T(0.5) : T(0);
   T cv[4][V + 1];  // rows (y0,z0), (y1,z0), (y0,z1), (y1,z1) at coarse x = k0..k0+V
+  bool loaded = false;
+  if constexpr (sizeof(T) == 4 && V == 4) {
+    if (vec_coarse) {
+      using F2 = float __attribute__((ext_vector_type(2)));
+      const uint32_t h = k0 >> 1;
+      const int m = nc - 1 - (int)k0;  // x = k0 + m is the last coarse node
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int yr = (r & 1) ? y1 : y0, zr = (r & 2) ? z1 : z0;
-    const uint32_t row = ((uint32_t)zr * (uint32_t)nc + (uint32_t)yr) * (uint32_t)gc.H;
+      for (int r = 0; r < 4; ++r) {
+        const int yr = (r & 1) ? y1 : y0, zr = (r & 2) ? z1 : z0;
+        const uint32_t row = ((uint32_t)zr * (uint32_t)nc + (uint32_t)yr) * (uint32_t)gc.H;
+        const uint32_t ca = (uint32_t)((yr + zr) & 1);  // colour of the even x = k0
+        const uint32_t oa = ca * gc.C + row + h, ob = (1u - ca) * gc.C + row + h;
+        const F2 a = *(const __attribute__((address_space(1))) F2*)(XC + oa);
+        const F2 b = *(const __attribute__((address_space(1))) F2*)(XC + ob);
+        const T a2 = XC[oa + 2];
+        const T raw[V + 1] = {a[0], b[0], a[1], b[1], a2};
 #pragma unroll
-    for (int i = 0; i <= V; ++i) {
-      const int X = (int)k0 + i < nc - 1 ? (int)k0 + i : nc - 1;
-      const uint32_t cc = (uint32_t)((X + yr + zr) & 1);
-      cv[r][i] = XC[cc * gc.C + row + (uint32_t)(X >> 1)];
+        for (int i = 0; i <= V; ++i) {
+          T v = raw[i];
+#pragma unroll
+          for (int q = 0; q < i; ++q)
+            if (i > m && q == m) v = raw[q];
+          cv[r][i] = v;
+        }
+      }
+      loaded = true;
+    }
+  }
+  if (!loaded) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int yr = (r & 1) ? y1 : y0, zr = (r & 2) ? z1 : z0;
+      const uint32_t row = ((uint32_t)zr * (uint32_t)nc + (uint32_t)yr) * (uint32_t)gc.H;
+#pragma unroll
+      for (int i = 0; i <= V; ++i) {
+        const int X = (int)k0 + i < nc - 1 ? (int)k0 + i : nc - 1;
+        const uint32_t cc = (uint32_t)((X + yr + zr) & 1);
+        cv[r][i] = XC[cc * gc.C + row + (uint32_t)(X >> 1)];
+      }
     }
   }
   T out[V];
