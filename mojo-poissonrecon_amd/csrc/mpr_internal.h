@@ -91,11 +91,14 @@ void launch_scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t count, ui
 // words of `scratch` that let launch_scan_exclusive use many workgroups for `count` values
 uint32_t scan_scratch_words(uint32_t count);
 size_t radix_scratch_words(uint32_t n);
-// gather_dst (optional, n doubles): gather_src[vals[i]] in the sorted order
+// gather_dst (optional, n doubles): gather_src[vals[i]] in the sorted order. With
+// result_keys and result_vals the sorted pairs stay in whichever buffers the last pass
+// wrote (keys/vals or k_tmp/v_tmp), which they are set to; without, they end in keys/vals.
 template <class K>  // uint32_t (splat node keys) or uint64_t (Morton keys)
 void launch_radix_sort(K* keys, uint32_t* vals, K* k_tmp, uint32_t* v_tmp, uint32_t n, int bits,
                        uint32_t* hist, uint32_t* offs, hipStream_t s,
-                       const double* gather_src = nullptr, double* gather_dst = nullptr);
+                       const double* gather_src = nullptr, double* gather_dst = nullptr,
+                       K** result_keys = nullptr, uint32_t** result_vals = nullptr);
 void launch_leaf_keys(const double* pts, uint32_t P, int depth, uint64_t* keys, uint32_t* perm,
                       hipStream_t s);
 uint32_t compaction_tiles(uint64_t n);

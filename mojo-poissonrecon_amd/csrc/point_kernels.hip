@@ -151,40 +151,48 @@ __global__ __launch_bounds__(kThreads) void k_leaf_keys(const double* pts, uint3
   perm[i] = i;
 }
 
-// ---- stable LSB radix sort, 8-bit digits ---------------------------------------------
+// ---- stable LSB radix sort, 8- or 9-bit digits -----------------------------------------
 constexpr int kRadixItems = 8;                        // items per thread per tile
 constexpr int kRadixTile = kThreads * kRadixItems;    // 2048 keys per block
+constexpr int kRadixMaxBins = 512;                    // 9-bit digits
 
-template <class K>
+// per-block digit counts, digit-major: hist[d * nblocks + block]
+template <class K, int DB>
 __global__ __launch_bounds__(kThreads) void k_radix_hist(const K* keys, uint32_t n, int shift,
                                                          uint32_t* hist, uint32_t nblocks) {
-  __shared__ uint32_t h[256];
-  h[threadIdx.x] = 0;
+  constexpr uint32_t kBins = 1u << DB;
+  __shared__ uint32_t h[kBins];
+  for (uint32_t b = threadIdx.x; b < kBins; b += kThreads) h[b] = 0;
   __syncthreads();
   const uint32_t base = blockIdx.x * kRadixTile;
   for (int r = 0; r < kRadixItems; ++r) {
     const uint32_t i = base + r * kThreads + threadIdx.x;
-    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & (kBins - 1u)], 1u);
   }
   __syncthreads();
-  hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];  // digit-major
+  for (uint32_t b = threadIdx.x; b < kBins; b += kThreads) hist[b * nblocks + blockIdx.x] = h[b];
 }
 
-// gdst (optional): gdst[pos] = gsrc[val] beside each moved pair, a payload gathered into
-// the sorted order
-template <class K>
+// Moves each key to offsets[digit][block] + its rank among the block's keys of that digit
+// (wavefront ballots on the DB digit bits give the rank inside a wavefront). gdst
+// (optional): gdst[pos] = gsrc[val] beside each moved pair, a payload gathered into the
+// sorted order.
+template <class K, int DB>
 __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* kin, const uint32_t* vin,
                                                             K* kout, uint32_t* vout, uint32_t n,
                                                             int shift, const uint32_t* offsets,
                                                             uint32_t nblocks, const double* gsrc,
                                                             double* gdst) {
-  __shared__ uint32_t wave_cnt[4][256];
-  __shared__ uint32_t running[256];
+  constexpr uint32_t kBins = 1u << DB;
+  __shared__ uint32_t wave_cnt[4][kBins];
+  __shared__ uint32_t running[kBins];
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  running[threadIdx.x] = offsets[threadIdx.x * nblocks + blockIdx.x];
+  for (uint32_t b = threadIdx.x; b < kBins; b += kThreads)
+    running[b] = offsets[b * nblocks + blockIdx.x];
   const uint32_t base = blockIdx.x * kRadixTile;
   for (int r = 0; r < kRadixItems; ++r) {
-    for (int w = 0; w < 4; ++w) wave_cnt[w][threadIdx.x] = 0;
+    for (uint32_t b = threadIdx.x; b < kBins; b += kThreads)
+      for (int w = 0; w < 4; ++w) wave_cnt[w][b] = 0;
     __syncthreads();
     const uint32_t i = base + r * kThreads + threadIdx.x;
     const bool valid = i < n;
@@ -193,10 +201,10 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* kin, const 
     if (valid) {
       key = kin[i];
       val = vin[i];
-      digit = (uint32_t)(key >> shift) & 255u;
+      digit = (uint32_t)(key >> shift) & (kBins - 1u);
     }
     uint64_t match = __ballot(valid);
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < DB; ++b) {
       const bool bit = (digit >> b) & 1u;
       const uint64_t bal = __ballot(bit);
       match &= bit ? bal : ~bal;
@@ -212,8 +220,8 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* kin, const 
       if (gdst) gdst[pos] = gsrc[val];
     }
     __syncthreads();
-    running[threadIdx.x] += wave_cnt[0][threadIdx.x] + wave_cnt[1][threadIdx.x] +
-                            wave_cnt[2][threadIdx.x] + wave_cnt[3][threadIdx.x];
+    for (uint32_t b = threadIdx.x; b < kBins; b += kThreads)
+      running[b] += wave_cnt[0][b] + wave_cnt[1][b] + wave_cnt[2][b] + wave_cnt[3][b];
     __syncthreads();
   }
 }
@@ -746,47 +754,69 @@ uint32_t scan_scratch_words(uint32_t count) { return (count + kScanChunk - 1) / 
 
 size_t radix_scratch_words(uint32_t n) {
   const uint32_t nb = std::max<uint32_t>(1u, (n + kRadixTile - 1) / kRadixTile);
-  return 256ull * nb + scan_scratch_words(256u * nb);  // histogram, then the scan's sums
+  // histogram, then the scan's sums
+  return (size_t)kRadixMaxBins * nb + scan_scratch_words((uint32_t)kRadixMaxBins * nb);
 }
 
-// Sorts (keys, vals) by the low `bits` key bits, stably. The result ends in (keys, vals);
+// Sorts (keys, vals) by the low `bits` key bits, stably, with 9-bit digits where that saves
+// a pass (25..27 bits: depth-8 node keys, depth-9 Morton codes) and 8-bit digits otherwise.
 // (k_tmp, v_tmp) are ping-pong buffers; hist and offs hold radix_scratch_words(n) words.
+// The result ends in (keys, vals), or, with result_keys / result_vals, in whichever pair of
+// buffers the last pass wrote (no copy back), which they then point at.
 // gather_dst (optional, n words): gather_dst[i] = gather_src[vals[i]] in the sorted order,
 // written by the last pass.
 template <class K>
 void launch_radix_sort(K* keys, uint32_t* vals, K* k_tmp, uint32_t* v_tmp, uint32_t n, int bits,
                        uint32_t* hist, uint32_t* offs, hipStream_t s, const double* gather_src,
-                       double* gather_dst) {
+                       double* gather_dst, K** result_keys, uint32_t** result_vals) {
+  if (result_keys) *result_keys = keys;
+  if (result_vals) *result_vals = vals;
   if (n == 0 || (n == 1 && !gather_dst)) return;
   if (bits > 8 * (int)sizeof(K)) throw HipError{"radix sort: more key bits than the key holds"};
   const uint32_t nb = (n + kRadixTile - 1) / kRadixTile;
-  const int passes = (bits + 7) / 8;
+  const int wide = (bits + 8) / 9 < (bits + 7) / 8;
+  const int db = wide ? 9 : 8;
+  const int passes = (bits + db - 1) / db;
+  const uint32_t bins = 1u << db;
   K *ka = keys, *kb = k_tmp;
   uint32_t *va = vals, *vb = v_tmp;
   for (int p = 0; p < passes; ++p) {
-    const int shift = 8 * p;
-    hipLaunchKernelGGL(k_radix_hist<K>, dim3(nb), dim3(kThreads), 0, s, ka, n, shift, hist, nb);
-    MPR_LAUNCH_CHECK();
-    launch_scan_exclusive(hist, offs, 256u * nb, nullptr, s, hist + 256ull * nb);
+    const int shift = db * p;
     const bool last = p == passes - 1;
-    hipLaunchKernelGGL(k_radix_scatter<K>, dim3(nb), dim3(kThreads), 0, s, ka, va, kb, vb, n,
-                       shift, offs, nb, last ? gather_src : nullptr,
-                       last ? gather_dst : nullptr);
+    const double* gs = last ? gather_src : nullptr;
+    double* gd = last ? gather_dst : nullptr;
+    if (wide)
+      hipLaunchKernelGGL((k_radix_hist<K, 9>), dim3(nb), dim3(kThreads), 0, s, ka, n, shift, hist,
+                         nb);
+    else
+      hipLaunchKernelGGL((k_radix_hist<K, 8>), dim3(nb), dim3(kThreads), 0, s, ka, n, shift, hist,
+                         nb);
+    MPR_LAUNCH_CHECK();
+    launch_scan_exclusive(hist, offs, bins * nb, nullptr, s, hist + (size_t)bins * nb);
+    if (wide)
+      hipLaunchKernelGGL((k_radix_scatter<K, 9>), dim3(nb), dim3(kThreads), 0, s, ka, va, kb, vb,
+                         n, shift, offs, nb, gs, gd);
+    else
+      hipLaunchKernelGGL((k_radix_scatter<K, 8>), dim3(nb), dim3(kThreads), 0, s, ka, va, kb, vb,
+                         n, shift, offs, nb, gs, gd);
     MPR_LAUNCH_CHECK();
     std::swap(ka, kb);
     std::swap(va, vb);
   }
-  if (ka != keys) {
+  if (result_keys && result_vals) {
+    *result_keys = ka;
+    *result_vals = va;
+  } else if (ka != keys) {
     MPR_HIP(hipMemcpyAsync(keys, ka, sizeof(K) * n, hipMemcpyDeviceToDevice, s));
     MPR_HIP(hipMemcpyAsync(vals, va, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
   }
 }
 template void launch_radix_sort<uint32_t>(uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t,
                                           int, uint32_t*, uint32_t*, hipStream_t, const double*,
-                                          double*);
+                                          double*, uint32_t**, uint32_t**);
 template void launch_radix_sort<uint64_t>(uint64_t*, uint32_t*, uint64_t*, uint32_t*, uint32_t,
                                           int, uint32_t*, uint32_t*, hipStream_t, const double*,
-                                          double*);
+                                          double*, uint64_t**, uint32_t**);
 
 void launch_leaf_keys(const double* pts, uint32_t P, int depth, uint64_t* keys, uint32_t* perm,
                       hipStream_t s) {
@@ -855,16 +885,19 @@ void launch_splat(const double* pts, const double* nrm, uint32_t P, int n, int c
                      b.keys, b.ids, b.w1, M, accepted);
   MPR_LAUNCH_CHECK();
   // the sort's last pass also lays pass 1's weights out in the sorted order, in the space
-  // of pass 2's contributions (written after k_splat_sum1 has read them)
+  // of pass 2's contributions (written after k_splat_sum1 has read them); the sorted pairs
+  // stay in whichever buffers the last pass wrote
   double* ws = b.c2;
-  launch_radix_sort(b.keys, b.ids, b.ktmp, b.itmp, pairs, bits, b.hist, b.offs, s, b.w1, ws);
-  hipLaunchKernelGGL(k_splat_sum1, dim3(np), dim3(kThreads), 0, s, b.keys, ws, pairs, M,
+  uint32_t *keys = nullptr, *ids = nullptr;
+  launch_radix_sort(b.keys, b.ids, b.ktmp, b.itmp, pairs, bits, b.hist, b.offs, s, b.w1, ws,
+                    &keys, &ids);
+  hipLaunchKernelGGL(k_splat_sum1, dim3(np), dim3(kThreads), 0, s, keys, ws, pairs, M,
                      density, screen, touched);
   MPR_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_splat_pass2_pairs, dim3(nb), dim3(kThreads), 0, s, pts, nrm, P, n,
                      confidence, density, b.c2, spn);
   MPR_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_splat_sum2, dim3(np), dim3(kThreads), 0, s, b.keys, b.ids, b.c2, pairs, M,
+  hipLaunchKernelGGL(k_splat_sum2, dim3(np), dim3(kThreads), 0, s, keys, ids, b.c2, pairs, M,
                      vx, vy, vz, touched ? 1 : 0);
   MPR_LAUNCH_CHECK();
 }
