@@ -156,7 +156,15 @@ constexpr int kRadixItems = 8;                        // items per thread per ti
 constexpr int kRadixTile = kThreads * kRadixItems;    // 2048 keys per block
 constexpr int kRadixMaxBins = 512;                    // 9-bit digits
 
-// per-block digit counts, digit-major: hist[d * nblocks + block]
+// Tile of a workgroup: the 8 XCDs take workgroups round-robin, so tile t goes to workgroup
+// 8 (t mod per) + t / per, giving each XCD a contiguous run of tiles. Neighbouring tiles
+// write neighbouring positions of every digit, which then meet in one XCD's L2.
+// nblocks (the grid) is a multiple of 8.
+__device__ __forceinline__ uint32_t radix_tile() {
+  return (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+}
+
+// per-tile digit counts, digit-major: hist[d * nblocks + tile]
 template <class K, int DB>
 __global__ __launch_bounds__(kThreads) void k_radix_hist(const K* keys, uint32_t n, int shift,
                                                          uint32_t* hist, uint32_t nblocks) {
@@ -164,19 +172,22 @@ __global__ __launch_bounds__(kThreads) void k_radix_hist(const K* keys, uint32_t
   __shared__ uint32_t h[kBins];
   for (uint32_t b = threadIdx.x; b < kBins; b += kThreads) h[b] = 0;
   __syncthreads();
-  const uint32_t base = blockIdx.x * kRadixTile;
+  const uint32_t tile = radix_tile();
+  const uint32_t base = tile * kRadixTile;
   for (int r = 0; r < kRadixItems; ++r) {
     const uint32_t i = base + r * kThreads + threadIdx.x;
     if (i < n) atomicAdd(&h[(keys[i] >> shift) & (kBins - 1u)], 1u);
   }
   __syncthreads();
-  for (uint32_t b = threadIdx.x; b < kBins; b += kThreads) hist[b * nblocks + blockIdx.x] = h[b];
+  for (uint32_t b = threadIdx.x; b < kBins; b += kThreads) hist[b * nblocks + tile] = h[b];
 }
 
-// Moves each key to offsets[digit][block] + its rank among the block's keys of that digit
-// (wavefront ballots on the DB digit bits give the rank inside a wavefront). gdst
-// (optional): gdst[pos] = gsrc[val] beside each moved pair, a payload gathered into the
-// sorted order.
+// Moves each key to offsets[digit][block] + its rank among the block's keys of that digit,
+// in input order (stable). Three phases with two workgroup barriers: (1) each wavefront
+// ranks its keys of every round by ballots on the DB digit bits and records its count per
+// (round, wavefront, digit); (2) one lane per digit turns the counts into running offsets
+// in input order (round-major, wavefront-minor); (3) every key is written. gdst (optional):
+// gdst[pos] = gsrc[val] beside each moved pair, a payload gathered into the sorted order.
 template <class K, int DB>
 __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* kin, const uint32_t* vin,
                                                             K* kout, uint32_t* vout, uint32_t n,
@@ -184,45 +195,54 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* kin, const 
                                                             uint32_t nblocks, const double* gsrc,
                                                             double* gdst) {
   constexpr uint32_t kBins = 1u << DB;
-  __shared__ uint32_t wave_cnt[4][kBins];
-  __shared__ uint32_t running[kBins];
+  constexpr int kWaves = kThreads / 64;
+  __shared__ uint32_t cnt[kRadixItems][kWaves][kBins];
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  for (uint32_t b = threadIdx.x; b < kBins; b += kThreads)
-    running[b] = offsets[b * nblocks + blockIdx.x];
-  const uint32_t base = blockIdx.x * kRadixTile;
+  uint32_t* flat = &cnt[0][0][0];
+  for (uint32_t i = threadIdx.x; i < (uint32_t)(kRadixItems * kWaves) * kBins; i += kThreads)
+    flat[i] = 0;
+  const uint32_t tile = radix_tile();
+  const uint32_t base = tile * kRadixTile;
+  K keys[kRadixItems];
+  uint32_t vals[kRadixItems], digit[kRadixItems], rank[kRadixItems];
+#pragma unroll
   for (int r = 0; r < kRadixItems; ++r) {
-    for (uint32_t b = threadIdx.x; b < kBins; b += kThreads)
-      for (int w = 0; w < 4; ++w) wave_cnt[w][b] = 0;
-    __syncthreads();
     const uint32_t i = base + r * kThreads + threadIdx.x;
-    const bool valid = i < n;
-    K key = 0;
-    uint32_t val = 0, digit = 0;
-    if (valid) {
-      key = kin[i];
-      val = vin[i];
-      digit = (uint32_t)(key >> shift) & (kBins - 1u);
-    }
+    keys[r] = i < n ? kin[i] : K(0);
+    vals[r] = i < n ? vin[i] : 0u;
+  }
+  __syncthreads();  // counts cleared
+#pragma unroll
+  for (int r = 0; r < kRadixItems; ++r) {
+    const bool valid = base + r * kThreads + threadIdx.x < n;
+    digit[r] = valid ? (uint32_t)(keys[r] >> shift) & (kBins - 1u) : 0u;
     uint64_t match = __ballot(valid);
     for (int b = 0; b < DB; ++b) {
-      const bool bit = (digit >> b) & 1u;
+      const bool bit = (digit[r] >> b) & 1u;
       const uint64_t bal = __ballot(bit);
       match &= bit ? bal : ~bal;
     }
-    const uint32_t rank = (uint32_t)__popcll(match & ((1ull << lane) - 1ull));
-    if (valid && rank == 0) wave_cnt[wave][digit] = (uint32_t)__popcll(match);
-    __syncthreads();
-    if (valid) {
-      uint32_t pos = running[digit] + rank;
-      for (uint32_t w = 0; w < wave; ++w) pos += wave_cnt[w][digit];
-      kout[pos] = key;
-      vout[pos] = val;
-      if (gdst) gdst[pos] = gsrc[val];
-    }
-    __syncthreads();
-    for (uint32_t b = threadIdx.x; b < kBins; b += kThreads)
-      running[b] += wave_cnt[0][b] + wave_cnt[1][b] + wave_cnt[2][b] + wave_cnt[3][b];
-    __syncthreads();
+    rank[r] = (uint32_t)__popcll(match & ((1ull << lane) - 1ull));
+    if (valid && rank[r] == 0) cnt[r][wave][digit[r]] = (uint32_t)__popcll(match);
+  }
+  __syncthreads();
+  for (uint32_t d = threadIdx.x; d < kBins; d += kThreads) {
+    uint32_t run = offsets[d * nblocks + tile];
+    for (int r = 0; r < kRadixItems; ++r)
+      for (int w = 0; w < kWaves; ++w) {
+        const uint32_t c = cnt[r][w][d];
+        cnt[r][w][d] = run;
+        run += c;
+      }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kRadixItems; ++r) {
+    if (base + r * kThreads + threadIdx.x >= n) continue;
+    const uint32_t pos = cnt[r][wave][digit[r]] + rank[r];
+    kout[pos] = keys[r];
+    vout[pos] = vals[r];
+    if (gdst) gdst[pos] = gsrc[vals[r]];
   }
 }
 
@@ -752,8 +772,13 @@ void launch_scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t count, ui
 
 uint32_t scan_scratch_words(uint32_t count) { return (count + kScanChunk - 1) / kScanChunk + 1; }
 
+// tiles of a sort of n keys, a multiple of 8 (radix_tile)
+inline uint32_t radix_tiles(uint32_t n) {
+  return ((std::max<uint32_t>(1u, (n + kRadixTile - 1) / kRadixTile) + 7u) / 8u) * 8u;
+}
+
 size_t radix_scratch_words(uint32_t n) {
-  const uint32_t nb = std::max<uint32_t>(1u, (n + kRadixTile - 1) / kRadixTile);
+  const uint32_t nb = radix_tiles(n);
   // histogram, then the scan's sums
   return (size_t)kRadixMaxBins * nb + scan_scratch_words((uint32_t)kRadixMaxBins * nb);
 }
@@ -773,7 +798,7 @@ void launch_radix_sort(K* keys, uint32_t* vals, K* k_tmp, uint32_t* v_tmp, uint3
   if (result_vals) *result_vals = vals;
   if (n == 0 || (n == 1 && !gather_dst)) return;
   if (bits > 8 * (int)sizeof(K)) throw HipError{"radix sort: more key bits than the key holds"};
-  const uint32_t nb = (n + kRadixTile - 1) / kRadixTile;
+  const uint32_t nb = radix_tiles(n);
   const int wide = (bits + 8) / 9 < (bits + 7) / 8;
   const int db = wide ? 9 : 8;
   const int passes = (bits + db - 1) / db;
