@@ -96,6 +96,43 @@ def test_batch_fp64_matches_oracle_and_handles_empty_clouds():
     assert not out[2].is_empty
 
 
+def test_mixed_batch_with_an_empty_job_matches_solo_runs():
+    # a job with no points keeps the filled solution (0) while the others in the batch start
+    # from the constant initial guess without a fill; each result equals its solo run
+    from mojo_poissonrecon import _lib
+    from mojo_poissonrecon.core import mesh_from_job
+    p, n = shapes.torus(3000, seed=6)
+    reqs = [prepare_reconstruct(p, n, depth=d, precision="mixed") for d in (5, 6)]
+
+    def solo(r):
+        j = Job(r.depth, len(r.points), r.precision)
+        try:
+            j.set_points(r.points, r.normals)
+            j.run(r.params)
+            return mesh_from_job(j, j.wait(), len(r.points))
+        finally:
+            j.close()
+
+    want = [solo(r) for r in reqs]
+    jobs = [Job(d, 3000, _lib.PRECISION_MIXED) for d in (5, 5, 6)]
+    try:
+        jobs[0].set_points(reqs[0].points, reqs[0].normals)
+        jobs[1].set_points(np.zeros((0, 3)), np.zeros((0, 3)))
+        jobs[2].set_points(reqs[1].points, reqs[1].normals)
+        b = Batch(jobs)
+        try:
+            b.run([reqs[0].params, reqs[0].params, reqs[1].params])
+            res = b.wait()
+        finally:
+            b.close()
+        assert res[1].vertex_count == 0 and res[1].face_count == 0
+        identical_mesh(mesh_from_job(jobs[0], res[0], len(reqs[0].points)), want[0])
+        identical_mesh(mesh_from_job(jobs[2], res[2], len(reqs[1].points)), want[1])
+    finally:
+        for j in jobs:
+            j.close()
+
+
 def test_batch_graph_replays_match_eager():
     cs = clouds(5)
     depths = [3, 5, 6, 7, 4]
