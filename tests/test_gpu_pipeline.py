@@ -42,6 +42,27 @@ def test_solve_matches_oracle(depth, count, confidence):
         assert np.array_equal(getattr(got.octree, name), tree[name]), name
 
 
+@pytest.mark.parametrize("point_weight,samples_per_node,cycles",
+                         [(0.0, 1.5, 4), (25.0, 1.0, 2), (4.0, 3.0, 1), (1e-3, 1.5, 5)])
+def test_solve_parameters_match_oracle(point_weight, samples_per_node, cycles):
+    # no screening (pure Poisson: the sweeps' diagonal is the degree alone), strong
+    # screening, other samples per node and cycle counts: grids bit-identical to the oracle
+    p, n = shapes.perturb(*shapes.torus(6000, 3), seed=3, sigma=0.002, degrees=5.0,
+                          length_range=(0.5, 1.5))
+    unit, _, _ = oracle.normalize(p, 1.1)
+    got = mpr.solve_screened_poisson(unit, n, depth=6, point_weight=point_weight,
+                                     samples_per_node=samples_per_node, cycles=cycles,
+                                     confidence=True)
+    want = oracle.solve(unit, n, 6, point_weight=point_weight, confidence=True, cycles=cycles,
+                        samples_per_node=samples_per_node)
+    assert np.array_equal(got.density, want["density"])
+    assert np.array_equal(got.screen, want["screen"])
+    assert np.array_equal(got.rhs, want["rhs"])
+    assert np.array_equal(got.field, want["field"])
+    assert got.iso_value == pytest.approx(want["iso_value"], rel=1e-12)
+    assert got.residual_norm == pytest.approx(want["residual_norm"], rel=1e-12)
+
+
 def test_solve_of_isolated_points_is_bit_exact():
     # points in distinct cells far apart (every node receives one point's contributions):
     # the whole solve is the reference's arithmetic and must match the oracle bit for bit
