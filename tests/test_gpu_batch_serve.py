@@ -75,6 +75,20 @@ def test_mixed_precision_tracks_fp64():
         assert s32.residual_norm == pytest.approx(s64.residual_norm, rel=1e-3)
 
 
+@pytest.mark.parametrize("point_weight,cycles", [(0.0, 4), (25.0, 2), (4.0, 1)])
+def test_mixed_precision_tracks_fp64_across_parameters(point_weight, cycles):
+    # no screening, strong screening and a single cycle: the mixed field stays within the
+    # same bounds of the FP64 field
+    p, n = shapes.torus_workload(20000, seed=5)
+    unit = oracle.normalize(p, 1.1)[0]
+    kw = dict(depth=6, confidence=True, point_weight=point_weight, cycles=cycles)
+    s64 = mpr.solve_screened_poisson(unit, n, **kw)
+    s32 = mpr.solve_screened_poisson(unit, n, precision="mixed", **kw)
+    assert s32.iso_value == pytest.approx(s64.iso_value, rel=1e-6)
+    assert np.abs(s32.field - s64.field).max() <= 1e-5 * np.abs(s64.field).max()
+    assert s32.residual_norm == pytest.approx(s64.residual_norm, rel=1e-3)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_batch_equals_solo_runs(precision):
     cs = clouds(7)
