@@ -137,6 +137,20 @@ def test_extract_random_field_bit_exact(n):
     assert np.array_equal(mesh.density, d)
 
 
+@pytest.mark.parametrize("n,iso", [(17, 0.0), (34, 0.0), (33, 1.0), (33, -1.0)])
+def test_extract_with_iso_ties_bit_exact(n, iso):
+    # values in {-1, 0, 1}: many corners equal the iso value, which counts as not below it
+    # (capi.mojo:839-1041); roots on edges that end exactly at the iso value
+    rng = np.random.default_rng(2000 + n)
+    field = rng.integers(-1, 2, size=(n, n, n)).astype(np.float64)
+    density = rng.random(field.shape)
+    mesh = mpr.extract_isosurface(field, iso_value=iso, density=density)
+    v, f, d = oracle.extract_isosurface(field, iso, density)
+    assert np.array_equal(mesh.vertices, v)
+    assert np.array_equal(mesh.faces, f)
+    assert np.array_equal(mesh.density, d)
+
+
 def test_extract_empty_and_full():
     field = np.ones((9, 9, 9))
     assert mpr.extract_isosurface(field, iso_value=0.0).is_empty
