@@ -77,6 +77,26 @@ def test_solve_of_isolated_points_is_bit_exact():
     assert np.array_equal(got.field, want["field"])
 
 
+def test_solve_with_boundary_and_duplicate_points_is_bit_exact():
+    # points on the faces, edges and corners of the unit cube (the splat clamps the cell
+    # index there, capi.mojo:127-173) and exact duplicates (equal node keys, summed in point
+    # order by the stable sort): the grids equal the oracle's bit for bit
+    rng = np.random.default_rng(17)
+    corners = np.array([[x, y, z] for x in (0.0, 1.0) for y in (0.0, 1.0) for z in (0.0, 1.0)])
+    faces = rng.random((300, 3))
+    faces[np.arange(300), rng.integers(0, 3, 300)] = rng.integers(0, 2, 300).astype(float)
+    inner = rng.random((400, 3)) * 0.8 + 0.1
+    unit = np.concatenate([corners, faces, inner, inner[:50], faces[:20]])
+    nrm = rng.normal(size=unit.shape)
+    for depth in (3, 5):
+        got = mpr.solve_screened_poisson(unit, nrm, depth=depth, confidence=True)
+        want = oracle.solve(unit, nrm, depth, confidence=True)
+        assert np.array_equal(got.density, want["density"])
+        assert np.array_equal(got.screen, want["screen"])
+        assert np.array_equal(got.rhs, want["rhs"])
+        assert np.array_equal(got.field, want["field"])
+
+
 def test_confidence_known_answers():
     # reference tests/test_kernels.py:46-58
     points = np.array([[0.25, 0.25, 0.25], [0.75, 0.75, 0.75]])
