@@ -99,6 +99,10 @@ const char *mpr_last_error(void);
 /* Device name, total HBM bytes and compute-unit count of the current GPU. */
 int mpr_device_info(char *name, int64_t name_len, int64_t *total_bytes, int64_t *cu_count);
 
+/* Free and total HBM bytes of the current GPU (hipMemGetInfo); either may be NULL. No
+ * reference counterpart (memory accounting for admission by footprint). */
+int mpr_device_memory(int64_t *free_bytes, int64_t *total_bytes);
+
 /* ---- 2. device-resident pipeline --------------------------------------------------------
  * The stages replace core.py:196-306 (solve_screened_poisson), core.py:309-374
  * (extract_isosurface) and core.py:388-426 (reconstruct) as one device pass. */
@@ -140,7 +144,9 @@ typedef struct mpr_result {
 
 typedef struct mpr_job mpr_job;
 
-/* Device bytes a job of this shape holds (used for admission by HBM footprint). */
+/* Device bytes a job of this shape holds (used for admission by HBM footprint): its
+ * buffers are carved out of one device allocation of exactly this size, made by
+ * mpr_job_create. A run whose mesh outgrows the preallocated mesh buffers adds larger ones. */
 int64_t mpr_job_footprint(int32_t depth, int64_t max_points, int32_t precision);
 int mpr_job_create(int32_t depth, int64_t max_points, int32_t precision, mpr_job **out);
 int mpr_job_destroy(mpr_job *job);

@@ -41,15 +41,37 @@ int ensure_device() {
 
 namespace {
 
+// Device buffers of one owner. Each buffer takes its size rounded up to 256 bytes, and a
+// buffer of 1 MiB or more starts on a 2 MiB boundary (as a separate allocation of that size
+// would). After reserve(total) the buffers are carved in order out of one allocation of
+// `total` bytes (the sum, padding included, that a dry run of the same requests counts), so
+// the owner's HBM footprint is that sum; otherwise each buffer is its own allocation.
 struct DevArena {
+  static constexpr size_t kBig = size_t(1) << 20, kPage = size_t(2) << 20;
   std::vector<void*> ptrs;
   size_t bytes = 0;
   bool dry = false;  // footprint accounting only
+  char* base = nullptr;
+  size_t cap = 0;
+  // one allocation of `total` bytes, which must include kPage for aligning its start
+  void reserve(size_t total) {
+    char* p = nullptr;
+    MPR_HIP(hipMalloc((void**)&p, total));
+    ptrs.push_back(p);
+    base = reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(p) + kPage - 1) & ~(kPage - 1));
+    cap = total - (size_t)(base - p);
+  }
   template <class T>
   T* alloc(size_t count) {
     const size_t b = std::max<size_t>(256, ((count * sizeof(T)) + 255) & ~size_t(255));
+    if (b >= kBig) bytes = (bytes + kPage - 1) & ~(kPage - 1);
+    const size_t off = bytes;
     bytes += b;
     if (dry) return nullptr;
+    if (base) {
+      if (bytes > cap) throw HipError{"device arena smaller than its buffers"};
+      return reinterpret_cast<T*>(base + off);
+    }
     void* p = nullptr;
     MPR_HIP(hipMalloc(&p, b));
     ptrs.push_back(p);
@@ -59,6 +81,8 @@ struct DevArena {
     for (void* p : ptrs) (void)hipFree(p);
     ptrs.clear();
     bytes = 0;
+    base = nullptr;
+    cap = 0;
   }
   DevArena() = default;
   DevArena(const DevArena&) = delete;
@@ -662,6 +686,17 @@ int mpr_device_info(char* name, int64_t name_len, int64_t* total_bytes, int64_t*
   MPR_GUARD_END
 }
 
+int mpr_device_memory(int64_t* free_bytes, int64_t* total_bytes) {
+  if (ensure_device()) return -1;
+  MPR_GUARD_BEGIN
+  size_t f = 0, t = 0;
+  MPR_HIP(hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = (int64_t)f;
+  if (total_bytes) *total_bytes = (int64_t)t;
+  return 0;
+  MPR_GUARD_END
+}
+
 int64_t mpr_job_footprint(int32_t depth, int64_t max_points, int32_t precision) {
   if (depth < 1 || depth > 10 || max_points < 0 || max_points > kMaxPoints) return -1;
   Job j;
@@ -675,7 +710,8 @@ int64_t mpr_job_footprint(int32_t depth, int64_t max_points, int32_t precision) 
   } catch (...) {
     return -1;
   }
-  return (int64_t)dry.bytes;
+  // the job's one allocation: its buffers and room to start them on a 2 MiB boundary
+  return (int64_t)(dry.bytes + DevArena::kPage);
 }
 
 int mpr_job_create(int32_t depth, int64_t max_points, int32_t precision, mpr_job** out) {
@@ -702,6 +738,10 @@ int mpr_job_create(int32_t depth, int64_t max_points, int32_t precision, mpr_job
   MPR_HIP(hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking));
   MPR_HIP(hipHostMalloc((void**)&j->host, sizeof(HostScalars), hipHostMallocDefault));
   std::memset(j->host, 0, sizeof(HostScalars));
+  // one allocation for every buffer of the job: its HBM footprint is mpr_job_footprint
+  const int64_t total = mpr_job_footprint(depth, max_points, precision);
+  if (total < 0) throw HipError{"job footprint"};
+  j->arena.reserve((size_t)total);
   j->plan(j->arena);
   const double ident[4] = {0.0, 0.0, 0.0, 1.0};
   MPR_HIP(hipMemcpyAsync(j->d_params, ident, sizeof(ident), hipMemcpyHostToDevice, j->stream));

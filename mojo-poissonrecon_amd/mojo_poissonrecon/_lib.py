@@ -83,6 +83,7 @@ _IP = ctypes.POINTER(ctypes.c_int64)
 _PIPELINE = {
     "mpr_last_error": ([], ctypes.c_char_p),
     "mpr_device_info": ([ctypes.c_char_p, I, _IP, _IP], ctypes.c_int),
+    "mpr_device_memory": ([_IP, _IP], ctypes.c_int),
     "mpr_job_footprint": ([S, I, S], I),
     "mpr_job_create": ([S, I, S, _PP], ctypes.c_int),
     "mpr_job_destroy": ([P], ctypes.c_int),

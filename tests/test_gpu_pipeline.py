@@ -378,6 +378,44 @@ def test_mixed_runs_with_and_without_the_initial_fill_match_fresh_jobs():
         job.close()
 
 
+FOOTPRINT_PROBE = """
+import ctypes, json, sys
+sys.path.insert(0, sys.argv[1])
+from mojo_poissonrecon import _lib
+from mojo_poissonrecon.core import Job
+lib = _lib.lib()
+free = ctypes.c_int64()
+def free_bytes():
+    _lib.check(lib.mpr_device_memory(ctypes.byref(free), None), "mpr_device_memory")
+    return free.value
+Job(4, 100).close()  # the runtime's one-time allocations come before the measurement
+out, jobs = [], []
+for depth, count, precision in ((4, 100, 0), (6, 20000, 0), (8, 200000, 1)):
+    before = free_bytes()
+    jobs.append(Job(depth, count, precision))
+    out.append((Job.footprint(depth, count, precision), before - free_bytes()))
+print(json.dumps(out))
+"""
+
+
+def test_job_holds_exactly_its_footprint():
+    # a job's buffers are carved out of one device allocation of mpr_job_footprint bytes, so
+    # admission by footprint (the server) counts what the device gives away, up to the
+    # rounding of that one allocation and the job stream's own memory. Measured in a fresh
+    # process with every job kept, so that no memory freed earlier is handed out again.
+    import json
+    import os
+    import subprocess
+    import sys
+    pkg = os.path.dirname(os.path.dirname(os.path.abspath(mpr.__file__)))
+    run = subprocess.run([sys.executable, "-c", FOOTPRINT_PROBE, pkg], capture_output=True,
+                         text=True, timeout=120)
+    assert run.returncode == 0, run.stderr
+    # besides the one allocation, a job's HIP stream takes a few MiB of its own
+    pairs = json.loads(run.stdout.strip().splitlines()[-1])
+    assert all(fp <= used <= fp + (8 << 20) for fp, used in pairs), pairs
+
+
 def test_fetch_grids_needs_a_grids_run():
     from mojo_poissonrecon import _lib
     from mojo_poissonrecon.core import Job
