@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import ctypes
 import threading
+from contextlib import contextmanager
 from dataclasses import dataclass
 
 import numpy as np
@@ -211,28 +212,42 @@ class Job:
 
 
 class _JobCache:
-    """Reuses one job per (depth, precision) so repeated calls do not re-allocate HBM."""
+    """Idle device jobs by (depth, precision), each lent to one caller at a time: repeated
+    calls do not re-allocate HBM, and concurrent calls (from several threads) never share a
+    job, as the reference's stateless calls never share buffers."""
 
     def __init__(self):
-        self._jobs: dict[tuple[int, int], Job] = {}
+        self._idle: dict[tuple[int, int], Job] = {}
         self._lock = threading.Lock()
 
-    def get(self, depth: int, count: int, precision: int) -> Job:
+    @contextmanager
+    def lend(self, depth: int, count: int, precision: int):
         key = (depth, precision)
         with self._lock:
-            job = self._jobs.get(key)
-            if job is None or job.max_points < count:
-                if job is not None:
-                    job.close()
-                job = Job(depth, max(count, 1), precision)
-                self._jobs[key] = job
-            return job
+            job = self._idle.pop(key, None)
+        if job is not None and job.max_points < count:
+            job.close()
+            job = None
+        if job is None:
+            job = Job(depth, max(count, 1), precision)
+        try:
+            yield job
+        except BaseException:
+            job.close()  # its device state is unknown after a failure
+            raise
+        with self._lock:
+            other = self._idle.get(key)
+            if other is None or other.max_points < job.max_points:
+                self._idle[key] = job
+                job = other
+        if job is not None:  # keep the larger of two returned jobs
+            job.close()
 
     def clear(self) -> None:
         with self._lock:
-            for job in self._jobs.values():
+            for job in self._idle.values():
                 job.close()
-            self._jobs.clear()
+            self._idle.clear()
 
 
 _jobs = _JobCache()
@@ -258,9 +273,9 @@ def build_octree(points, depth: int = 6) -> FlatOctree:
         zero = np.array([0], dtype=np.int64)
         return FlatOctree(zero, one, np.array([-1], np.int64), np.array([-1], np.int64), zero,
                           np.empty(0, np.int64))
-    job = _jobs.get(1, len(points), _lib.PRECISION_FP64)
-    job.set_points(points, points)
-    return job.octree(depth)
+    with _jobs.lend(1, len(points), _lib.PRECISION_FP64) as job:
+        job.set_points(points, points)
+        return job.octree(depth)
 
 
 def solve_screened_poisson(
@@ -295,15 +310,16 @@ def solve_screened_poisson(
     if len(points) and (np.any(points < 0) or np.any(points > 1)):
         raise ValueError("solver points must be inside [0, 1]^3")
 
-    job = _jobs.get(depth, len(points), mode)
-    job.set_points(points, normals)
-    job.run(Job.params(_lib.STAGE_SOLVE | _lib.STAGE_GRIDS, point_weight, samples_per_node,
-                       cycles=cycles, confidence=confidence))
-    result = job.wait()
-    if result.accepted != len(points):
-        raise RuntimeError(f"GPU splat accepted {result.accepted} of {len(points)} validated samples")
-    field, density, screen, rhs = job.fetch_grids()
-    tree = job.octree(depth) if len(points) else build_octree(points, depth)
+    with _jobs.lend(depth, len(points), mode) as job:
+        job.set_points(points, normals)
+        job.run(Job.params(_lib.STAGE_SOLVE | _lib.STAGE_GRIDS, point_weight, samples_per_node,
+                           cycles=cycles, confidence=confidence))
+        result = job.wait()
+        if result.accepted != len(points):
+            raise RuntimeError(
+                f"GPU splat accepted {result.accepted} of {len(points)} validated samples")
+        field, density, screen, rhs = job.fetch_grids()
+        tree = job.octree(depth) if len(points) else build_octree(points, depth)
     return GridSolution(field, density, screen, rhs, float(result.iso_value),
                         float(result.residual_norm), tree)
 
@@ -375,10 +391,10 @@ def reconstruct(
                               samples_per_node=samples_per_node, precision=precision)
     if req is None:
         return Mesh(np.empty((0, 3), np.float64), np.empty((0, 3), np.int64), np.empty(0))
-    job = _jobs.get(req.depth, len(req.points), req.precision)
-    job.set_points(req.points, req.normals)
-    job.run(req.params)
-    return mesh_from_job(job, job.wait(), len(req.points))
+    with _jobs.lend(req.depth, len(req.points), req.precision) as job:
+        job.set_points(req.points, req.normals)
+        job.run(req.params)
+        return mesh_from_job(job, job.wait(), len(req.points))
 
 
 @dataclass(frozen=True)

@@ -248,6 +248,33 @@ def test_job_reuse_with_smaller_input_and_graph_replay():
     job.close()
 
 
+def test_concurrent_calls_from_threads_match_single_calls():
+    # the public calls may run in several host threads at once, as the reference's stateless
+    # calls can: each call gets a device job of its own, and every result equals the same
+    # call made alone
+    from concurrent.futures import ThreadPoolExecutor
+    clouds = [shapes.sphere(3000, s) for s in range(3)] + [shapes.torus(4000, s) for s in range(3)]
+
+    def call(i):
+        p, n = clouds[i % len(clouds)]
+        return mpr.reconstruct(p, n, depth=6, precision="mixed" if i % 2 else "fp64")
+
+    with ThreadPoolExecutor(4) as pool:
+        got = list(pool.map(call, range(3 * len(clouds))))
+    for i, m in enumerate(got):
+        want = call(i)
+        assert m.iso_value == want.iso_value and m.residual_norm == want.residual_norm, i
+        assert np.array_equal(m.vertices, want.vertices), i
+        assert np.array_equal(m.faces, want.faces), i
+        assert np.array_equal(m.density, want.density), i
+    units = [oracle.normalize(p, 1.1)[0] for p, _ in clouds]
+    with ThreadPoolExecutor(3) as pool:
+        trees = list(pool.map(lambda u: mpr.build_octree(u, 6), units))
+    for u, t in zip(units, trees):
+        want = mpr.build_octree(u, 6)
+        assert np.array_equal(t.morton, want.morton) and np.array_equal(t.parent, want.parent)
+
+
 def test_job_mesh_overflow_regrows_and_matches_extraction():
     # a noise cloud whose mesh outgrows the job's preallocated vertex buffer: the job grows
     # its buffers and re-extracts; the result equals a standalone extraction of its field
