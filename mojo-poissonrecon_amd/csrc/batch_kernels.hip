@@ -285,7 +285,10 @@ __global__ __launch_bounds__(B) void kb_relax_column(const StepPtrs<T>* tab, uin
 // nodes at the same time (from cache), and x streams from memory once.
 // KNOWN: every x word holds xv (load_group); a compile-time choice, so the loading form
 // keeps its load schedule.
-template <class T, class TO, bool KNOWN = false>
+// C0: colour 0 only; the colour-1 residual words are left as they are (for a grid whose
+// last sweep was a colour-1 sweep, where the colour-1 residual is zero up to rounding and
+// the restriction reads it as zero).
+template <class T, class TO, bool KNOWN = false, bool C0 = false>
 __device__ __forceinline__ void residual_group(gptr<const T> X, gptr<const T> R, gptr<const T> S,
                                                gptr<TO> out, const Geom& g, uint32_t lane,
                                                uint32_t lanes, T pw, T h2, LiveMask live,
@@ -294,17 +297,18 @@ __device__ __forceinline__ void residual_group(gptr<const T> X, gptr<const T> R,
   SlotGroup<T, V> G0[1], G1[1];
   load_group<T, V>(X, R, S, g, 0, lane, lanes, true, G0[0], KNOWN, live, false, nullptr, xv);
   // colour 1 at the same slots: its row and own chunks are colour 0's own and row chunks
-  load_group<T, V>(X, R, S, g, 1, lane, lanes, true, G1[0], KNOWN, live, false, G0, xv);
+  if (!C0) load_group<T, V>(X, R, S, g, 1, lane, lanes, true, G1[0], KNOWN, live, false, G0, xv);
   group_edges<T, V, 1>(X, g, 0, G0, KNOWN, xv);
-  group_edges<T, V, 1>(X, g, 1, G1, KNOWN, xv);
+  if (!C0) group_edges<T, V, 1>(X, g, 1, G1, KNOWN, xv);
   T v[V];
   group_residual<T, V>(G0[0], g.n, pw, h2, v);
   store_group<T, TO, V>(out + G0[0].own, v, G0[0].in ? (1u << V) - 1 : 0u);
+  if (C0) return;
   group_residual<T, V>(G1[0], g.n, pw, h2, v);
   store_group<T, TO, V>(out + G1[0].own, v, G1[0].in ? (1u << V) - 1 : 0u);
 }
 
-template <class T, int B>
+template <class T, int B, bool C0>
 __global__ __launch_bounds__(B) void kb_residual_group(const StepPtrs<T>* tab, uint32_t count,
                                                        uint32_t nbi, Geom g, uint32_t lanes, T pw,
                                                        T h2) {
@@ -313,9 +317,10 @@ __global__ __launch_bounds__(B) void kb_residual_group(const StepPtrs<T>* tab, u
   const uint32_t slot = lb / nbi;
   if (slot >= count) return;
   const StepPtrs<T> p = tab[slot];
-  residual_group<T, T>(as_global((const T*)p.x), as_global(p.rhs), as_global(p.screen),
-                       as_global(p.res), g, (lb - slot * nbi) * B + threadIdx.x, lanes, pw, h2,
-                       LiveMask{p.live, p.live_rhs != 0});
+  residual_group<T, T, false, C0>(as_global((const T*)p.x), as_global(p.rhs),
+                                  as_global(p.screen), as_global(p.res), g,
+                                  (lb - slot * nbi) * B + threadIdx.x, lanes, pw, h2,
+                                  LiveMask{p.live, p.live_rhs != 0});
 }
 
 // restrict the residual into the next-coarser rhs and zero the next-coarser x
@@ -335,7 +340,8 @@ __global__ __launch_bounds__(kThreads) void kb_restrict(const StepPtrs<T>* tab, 
   if (zero_coarse) p.xc[i] = T(0);
 }
 
-template <class T>
+// C1Z: the fine colour-1 residual is read as zero (residual_group's C0)
+template <class T, bool C1Z>
 __global__ __launch_bounds__(kThreads) void kb_restrict_group(const StepPtrs<T>* tab,
                                                               uint32_t count, uint32_t nbi, Geom gf,
                                                               Geom gc, uint32_t lanes,
@@ -348,7 +354,7 @@ __global__ __launch_bounds__(kThreads) void kb_restrict_group(const StepPtrs<T>*
   const uint32_t lane = (lb - slot * nbi) * kThreads + threadIdx.x;
   restrict_group<T, kGroup<T>>(as_global((const T*)p.res), gf, as_global(p.rhsc),
                                zero_coarse ? as_global(p.xc) : gptr<T>(nullptr), gc, lane, lanes,
-                               T(1));
+                               T(1), C1Z);
 }
 
 template <class T>
@@ -719,16 +725,20 @@ void launch_b_relax(const StepPtrs<T>* tab, uint32_t count, const Geom& g, doubl
 
 template <class T>
 void launch_b_residual(const StepPtrs<T>* tab, uint32_t count, const Geom& g, double pw,
-                       hipStream_t s) {
+                       hipStream_t s, bool colour0_only) {
   if (!count) return;
   const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
   if (group_layout(g)) {
     const uint32_t lanes = g.C / (uint32_t)kGroup<T>;
     const int block = tuned_block(kTuneResidual, sizeof(T) == 8, size_class(g));
     const uint32_t nbi = blocks_for(lanes, (unsigned)block);  // both colours per lane
-#define MPR_LAUNCH_RESIDUAL(B)                                                                 \
-  hipLaunchKernelGGL((kb_residual_group<T, B>), dim3(padded8((uint64_t)nbi * count)), dim3(B), \
-                     0, s, tab, count, nbi, g, lanes, (T)pw, h2)
+#define MPR_LAUNCH_RESIDUAL(B)                                                               \
+  if (colour0_only)                                                                          \
+    hipLaunchKernelGGL((kb_residual_group<T, B, true>), dim3(padded8((uint64_t)nbi * count)),  \
+                       dim3(B), 0, s, tab, count, nbi, g, lanes, (T)pw, h2);                   \
+  else                                                                                       \
+    hipLaunchKernelGGL((kb_residual_group<T, B, false>), dim3(padded8((uint64_t)nbi * count)), \
+                       dim3(B), 0, s, tab, count, nbi, g, lanes, (T)pw, h2)
     MPR_WITH_BLOCK(block, MPR_LAUNCH_RESIDUAL)
 #undef MPR_LAUNCH_RESIDUAL
     MPR_LAUNCH_CHECK();
@@ -742,16 +752,21 @@ void launch_b_residual(const StepPtrs<T>* tab, uint32_t count, const Geom& g, do
 
 template <class T>
 void launch_b_restrict(const StepPtrs<T>* tab, uint32_t count, const Geom& gf, const Geom& gc,
-                       bool zero_coarse, hipStream_t s) {
+                       bool zero_coarse, hipStream_t s, bool colour1_zero) {
   if (!count) return;
   if (sizeof(T) == 4 && group_layout(gc) && group_layout(gf)) {  // as launch_restrict
     const uint32_t lanes = gc.C / (uint32_t)kGroup<T>;
     const uint32_t nbi = blocks_for(lanes, kThreads);  // both coarse colours per lane
-    hipLaunchKernelGGL(kb_restrict_group<T>, dim3(padded8((uint64_t)nbi * count)),
-                       dim3(kThreads), 0, s, tab, count, nbi, gf, gc, lanes, zero_coarse ? 1 : 0);
+    if (colour1_zero)
+      hipLaunchKernelGGL((kb_restrict_group<T, true>), dim3(padded8((uint64_t)nbi * count)),
+                         dim3(kThreads), 0, s, tab, count, nbi, gf, gc, lanes, zero_coarse ? 1 : 0);
+    else
+      hipLaunchKernelGGL((kb_restrict_group<T, false>), dim3(padded8((uint64_t)nbi * count)),
+                         dim3(kThreads), 0, s, tab, count, nbi, gf, gc, lanes, zero_coarse ? 1 : 0);
     MPR_LAUNCH_CHECK();
     return;
   }
+  if (colour1_zero) throw HipError{"colour-1-zero restriction needs the slot-group layout"};
   const uint32_t nbi = blocks_for(2ull * gc.C, kThreads);
   hipLaunchKernelGGL(kb_restrict<T>, dim3(padded8((uint64_t)nbi * count)), dim3(kThreads), 0, s,
                      tab, count, nbi, gf, gc, zero_coarse ? 1 : 0);
@@ -881,9 +896,9 @@ void init_coarse_kernels() {
   template void launch_b_relax<T>(const StepPtrs<T>*, uint32_t, const Geom&, double, int,      \
                                   hipStream_t, int, bool, bool);                               \
   template void launch_b_residual<T>(const StepPtrs<T>*, uint32_t, const Geom&, double,        \
-                                     hipStream_t);                                             \
+                                     hipStream_t, bool);                                       \
   template void launch_b_restrict<T>(const StepPtrs<T>*, uint32_t, const Geom&, const Geom&,   \
-                                     bool, hipStream_t);                                       \
+                                     bool, hipStream_t, bool);                                 \
   template void launch_b_prolong<T>(const StepPtrs<T>*, uint32_t, const Geom&, const Geom&,    \
                                     bool, hipStream_t);                                        \
   template void launch_b_fill_x<T>(const StepPtrs<T>*, uint32_t, const Geom&, hipStream_t);    \

@@ -73,12 +73,16 @@ template <class T>
 void launch_b_relax(const StepPtrs<T>* tab, uint32_t count, const Geom& g, double pw,
                     int iterations, hipStream_t s, int zero_start = kZeroNone,
                     bool correct = false, bool first = false);
+// colour0_only: the residual of colour 0 alone (slot-group layout only); colour1_zero: the
+// restriction reads the fine colour 1 as zero (FP32 with the slot-group layout on both grids
+// only). Together they serve a grid whose last sweep was a colour-1 sweep, after which the
+// colour-1 residual is zero up to rounding.
 template <class T>
 void launch_b_residual(const StepPtrs<T>* tab, uint32_t count, const Geom& g, double pw,
-                       hipStream_t s);
+                       hipStream_t s, bool colour0_only = false);
 template <class T>
 void launch_b_restrict(const StepPtrs<T>* tab, uint32_t count, const Geom& gf, const Geom& gc,
-                       bool zero_coarse, hipStream_t s);
+                       bool zero_coarse, hipStream_t s, bool colour1_zero = false);
 // colour1_only: prolong only colour 1, for a level whose next step is a colour-0 sweep
 // (which overwrites every colour-0 node without reading it)
 template <class T>

@@ -527,7 +527,7 @@ __device__ __forceinline__ void prolong_group(gptr<const T> XC, const Geom& gc, 
 template <class T, int V>
 __device__ __forceinline__ void restrict_group(gptr<const T> F, const Geom& gf, gptr<T> RC,
                                                gptr<T> XC, const Geom& gc, uint32_t lane,
-                                               uint32_t lanes, T scale) {
+                                               uint32_t lanes, T scale, bool c1_zero = false) {
   using CT = typename SlotGroup<T, V>::CT;
   constexpr int W = SlotGroup<T, V>::W;
   constexpr int NR = 2 * V / W;  // chunks per fine row and colour
@@ -552,14 +552,16 @@ __device__ __forceinline__ void restrict_group(gptr<const T> F, const Geom& gf, 
     const uint32_t q = (uint32_t)((yc + zc) & 1);
     const uint32_t row = ((uint32_t)zc * (uint32_t)nf + (uint32_t)yc) * (uint32_t)gf.H;
     const uint32_t rq = q * gf.C + row, ro = (1u - q) * gf.C + row;
+    // c1_zero: the fine colour-1 values are known to be zero and are not loaded
+    const bool zq = c1_zero && q == 1u, zo = c1_zero && q == 0u;
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
       // chunks past the row end only hold pad slots (never used): read the row start
       const uint32_t slot = base + (uint32_t)(i * W) + W <= (uint32_t)gf.H ? base + i * W : 0u;
-      aq[r][i] = *(const __attribute__((address_space(1))) CT*)(F + rq + slot);
-      ao[r][i] = *(const __attribute__((address_space(1))) CT*)(F + ro + slot);
+      aq[r][i] = zq ? CT(T(0)) : *(const __attribute__((address_space(1))) CT*)(F + rq + slot);
+      ao[r][i] = zo ? CT(T(0)) : *(const __attribute__((address_space(1))) CT*)(F + ro + slot);
     }
-    before[r] = F[ro + (base > 0 ? base - 1 : 0)];
+    before[r] = zo ? T(0) : F[ro + (base > 0 ? base - 1 : 0)];
   }
 #pragma unroll
   for (int cc = 0; cc < 2; ++cc) {
