@@ -145,8 +145,18 @@ class BatchedVcycle {
   void relax_finest(double pw, int iterations, hipStream_t s) const {
     launch_b_relax<T>(step_[smax], step_count_[smax], geom_[smax], pw, iterations, s);
   }
+  // as the V-cycle launches it after pre-smoothing
   void residual_finest(double pw, hipStream_t s) const {
-    launch_b_residual<T>(step_[smax], step_count_[smax], geom_[smax], pw, s);
+    launch_b_residual<T>(step_[smax], step_count_[smax], geom_[smax], pw, s,
+                         colour0_residual(smax, pw, 1));
+  }
+  // whether class c's residual computes colour 0 only and its restriction reads the fine
+  // colour 1 as zero: in FP32 after a pre-smoothing (which ends with a colour-1 sweep that
+  // zeroes the colour-1 residual up to rounding, its diagonal >= 3 h^-2 > 0 for
+  // point_weight >= 0), where both class c and c - 1 have the slot-group layout
+  bool colour0_residual(int c, double pw, int pre) const {
+    return sizeof(T) == 4 && pre > 0 && pw >= 0.0 && c > sc && group_layout(geom_[c]) &&
+           group_layout(geom_[c - 1]);
   }
   void restrict_finest(hipStream_t s) const {
     launch_b_restrict<T>(step_[smax], step_count_[smax], geom_[smax], geom_[smax - 1], false, s);

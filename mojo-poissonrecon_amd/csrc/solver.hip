@@ -1180,7 +1180,7 @@ static double live_group_fraction(const mpr_job* job, int V) {
 // algorithmic bytes of one launch of pipeline kernel 7, 8, 9 or 13 (mpr_job_time_kernel) on
 // the job's finest level: words each kernel must move, with the rhs / screen words of
 // groups without a live slot left out (the kernels skip them) and one mask bit per slot
-static double pipeline_kernel_bytes(const mpr_job* job, int kernel) {
+static double pipeline_kernel_bytes(const mpr_job* job, int kernel, double pw) {
   const Geom& g0 = job->geoms[0];
   const double M0 = (double)g0.n * g0.n * g0.n;
   const bool skip = group_layout(g0);
@@ -1192,8 +1192,12 @@ static double pipeline_kernel_bytes(const mpr_job* job, int kernel) {
       return (3.0 + live) * 4.0 * M0 / 2.0 + mask / 2.0;
     case 8:  // one colour: other colour x, x written; live rhs and screen
       return (2.0 + 2.0 * live) * 8.0 * M0 / 2.0 + mask / 2.0;
-    case 9:  // x, rhs, residual written; live screen
-      return (3.0 + live) * 4.0 * M0 + mask;
+    case 9:
+      // colour 0 only (FP32 after pre-smoothing, BatchedVcycle::colour0_residual): all of x,
+      // colour-0 rhs and residual written; live colour-0 screen
+      if (job->vc32.colour0_residual(job->vc32.smax, pw, 1))
+        return (4.0 + live) * 4.0 * M0 / 2.0 + mask / 2.0;
+      return (3.0 + live) * 4.0 * M0 + mask;  // x, rhs, residual written; live screen
     case 13:  // x64 read, r32 written; live rhs64 and screen64
       return (8.0 + 4.0 + 16.0 * live) * M0 + mask;
   }
@@ -1221,7 +1225,7 @@ int mpr_job_time_kernel(mpr_job* job, int32_t kernel, int32_t reps, double* ms_p
   auto& a = job->L64[0];
   // the pipeline kernels' byte counts read the live-slot bits back: before the timed region
   double bytes = (kernel == 7 || kernel == 8 || kernel == 9 || kernel == 13)
-                     ? pipeline_kernel_bytes(job, kernel)
+                     ? pipeline_kernel_bytes(job, kernel, pw)
                      : 0.0;
   int launches = reps;
   hipEvent_t e0, e1;
@@ -1330,7 +1334,7 @@ int mpr_batch_time_kernel(mpr_batch* b, int32_t kernel, int32_t reps, double* ms
   // one launch covers every instance of the batch's finest class
   double bytes = 0.0;
   for (Job* j : b->jobs)
-    if (j->depth == smax) bytes += pipeline_kernel_bytes(static_cast<const mpr_job*>(j), kernel);
+    if (j->depth == smax) bytes += pipeline_kernel_bytes(static_cast<const mpr_job*>(j), kernel, pw);
   hipEvent_t e0, e1;
   MPR_HIP(hipEventCreate(&e0));
   MPR_HIP(hipEventCreate(&e1));

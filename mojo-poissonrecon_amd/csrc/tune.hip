@@ -135,7 +135,11 @@ void tune_class(int cls, int reps, hipStream_t s) {
   MPR_HIP(hipMemcpy(tab.p, e.data(), K * sizeof(e[0]), hipMemcpyHostToDevice));
   const bool fp64 = sizeof(T) == 8;
   tune_one(kTuneRelax, fp64, cls, [&] { launch_b_relax<T>(tab.p, K, g, 4.0, 1, s); }, reps, s);
-  tune_one(kTuneResidual, fp64, cls, [&] { launch_b_residual<T>(tab.p, K, g, 4.0, s); }, reps, s);
+  // the residual in the form the V-cycle launches: FP32 classes whose next-coarser class
+  // also has the slot-group layout compute colour 0 only
+  const bool c0 = !fp64 && group_layout(g) && group_layout(gc);
+  tune_one(kTuneResidual, fp64, cls, [&] { launch_b_residual<T>(tab.p, K, g, 4.0, s, c0); }, reps,
+           s);
   if (fp64) {
     DevArray<float> r32(K * W), e32(K * W);
     std::vector<TopPtrs> t;

@@ -109,11 +109,9 @@ void BatchedVcycle<T>::run(double pw, int pre, int post, bool zero_tops, hipStre
   for (int c = smax; c > sc; --c) {
     if (!step_count_[c]) continue;
     launch_b_relax<T>(step_[c], step_count_[c], geom_[c], pw, pre, s, zero_start);
-    // In FP32, after a pre-smoothing that ends with a colour-1 sweep (a division by a
-    // diagonal >= 3 h^-2 > 0 at every colour-1 node for point_weight >= 0), the colour-1
-    // residual is zero up to rounding: only colour 0 is computed and restricted.
-    const bool c0 = sizeof(T) == 4 && pre > 0 && pw >= 0.0 && group_layout(geom_[c]) &&
-                    group_layout(geom_[c - 1]);
+    // FP32 after pre-smoothing: the colour-1 residual is zero up to rounding, and only
+    // colour 0 is computed and restricted
+    const bool c0 = colour0_residual(c, pw, pre);
     launch_b_residual<T>(step_[c], step_count_[c], geom_[c], pw, s, c0);
     launch_b_restrict<T>(step_[c], step_count_[c], geom_[c], geom_[c - 1], fill && c - 1 > sc,
                          s, c0);
