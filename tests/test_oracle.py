@@ -237,6 +237,29 @@ def test_red_black_relaxation_closed_form(n):
     assert np.allclose(actual, expected, rtol=0, atol=2e-15)
 
 
+@pytest.mark.parametrize("n", [9, 33, 65])
+@pytest.mark.parametrize("point_weight", [0.0, 4.0])
+def test_colour_one_residual_is_rounding_after_relaxation(n, point_weight):
+    # Each red-black iteration ends with a colour-1 sweep, which sets every colour-1 node to
+    # the value that zeroes its own equation (its diagonal is >= 3 h^-2 > 0 for
+    # point_weight >= 0). The colour-1 residual left behind is rounding, a few ulps of the
+    # terms it sums, while colour 0's is not: the premise on which the FP32 V-cycle computes
+    # and restricts colour 0 only after pre-smoothing.
+    rng = np.random.default_rng(n)
+    h2 = float((n - 1) ** 2)
+    x = rng.normal(size=(n, n, n))
+    rhs = rng.normal(size=(n, n, n)) * h2
+    screen = rng.random((n, n, n))
+    x = oracle.relax(x, rhs, screen, point_weight, 2)
+    r, _ = oracle.residual(x, rhs, screen, point_weight)
+    s, d = stencil_sum_degree(x)
+    terms = np.abs(rhs) + (d * h2 + point_weight * screen * h2) * np.abs(x) + np.abs(s) * h2
+    parity = np.indices((n, n, n)).sum(axis=0) & 1
+    eps = np.finfo(np.float64).eps
+    assert np.all(np.abs(r[parity == 1]) <= 8 * eps * terms[parity == 1])
+    assert np.abs(r[parity == 0]).max() > 1e-3 * terms[parity == 0].max()
+
+
 @pytest.mark.parametrize("n", [14, 65])
 def test_dual_vertex_count_closed_form(n):
     # reference tests/test_kernels.py:168-185
