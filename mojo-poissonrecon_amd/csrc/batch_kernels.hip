@@ -418,8 +418,70 @@ __global__ __launch_bounds__(kThreads) void kb_defect(const TopPtrs* tab, uint32
                         (lb - slot * nbi) * kSpan + threadIdx.x, pw, h2);
 }
 
-// FIRST: the first cycle's defect, where an entry whose x_initial flag is set reads x64 as
-// kInitialX everywhere (it was never filled)
+// The first defect of an entry whose x_initial flag is set: x holds kInitialX everywhere, so
+// a node needs only its own rhs and screening words. The arithmetic is residual_group's with
+// x known, in the same order: the neighbour sum of deg copies of kInitialX (0.5) is
+// deg * kInitialX exactly, and r = rhs - ((deg h2 + pw s h2) x - sum h2). Lanes as
+// residual_group<double>: the same 2 slots of both colours.
+__device__ __forceinline__ void initial_defect_group(gptr<const double> R, gptr<const double> S,
+                                                     gptr<float> out, const Geom& g,
+                                                     uint32_t lane, uint32_t lanes, double pw,
+                                                     double h2, const uint32_t* live) {
+  constexpr int V = kGroup<double>;
+  static_assert(V == 2, "one double2 per colour and lane");
+  if (lane >= lanes) return;
+  const uint32_t s0 = lane * (uint32_t)V;
+  const uint32_t z = g.div_nH.div(s0);
+  const uint32_t rr = s0 - z * g.nH;
+  const uint32_t y = g.div_H.div(rr);
+  const uint32_t k0 = rr - y * (uint32_t)g.H;
+  const int n = g.n;
+  const int dyz = (int)(y > 0) + (int)((int)y + 1 < n) + (int)(z > 0) + (int)((int)z + 1 < n);
+  using D2 = typename Chunk16<double>::type;
+  using F2 = float __attribute__((ext_vector_type(2)));
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const uint32_t own = (uint32_t)c * g.C + s0;
+    const int p = (c ^ (int)y ^ (int)z) & 1;
+    // a group without a live slot has rhs and screen exactly +0.0 (LiveMask)
+    const bool dead = live && !((live[own >> 5] >> (own & 31u)) & ((1u << V) - 1u));
+    D2 rv = D2(0.0), sv = D2(0.0);
+    if (!dead) {
+      rv = *(const __attribute__((address_space(1))) D2*)(R + own);
+      sv = *(const __attribute__((address_space(1))) D2*)(S + own);
+    }
+    float v[V];
+#pragma unroll
+    for (int j = 0; j < V; ++j) {
+      const int x = 2 * ((int)k0 + j) + p;
+      const int deg = (int)(x > 0) + (int)(x + 1 < n) + dyz;
+      const double screening = pw * (j ? sv.y : sv.x) * h2;
+      const double applied =
+          ((double)deg * h2 + screening) * kInitialX - ((double)deg * kInitialX) * h2;
+      v[j] = x < n ? (float)((j ? rv.y : rv.x) - applied) : 0.f;
+    }
+    *(__attribute__((address_space(1))) F2*)(out + own) = F2{v[0], v[1]};
+  }
+}
+
+// The first cycle's defect of the entries whose x_initial flag is set (x64 was never filled
+// and holds kInitialX everywhere); a kernel of its own, so that its few registers allow the
+// occupancy that hides its two dependent loads
+__global__ __launch_bounds__(kThreads) void kb_defect_initial(const TopPtrs* tab, uint32_t count,
+                                                              uint32_t nbi, Geom g, uint32_t lanes,
+                                                              double pw, double h2) {
+  const uint32_t per = gridDim.x >> 3;
+  const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  const uint32_t slot = lb / nbi;
+  if (slot >= count) return;
+  const TopPtrs p = tab[slot];
+  if (!p.x_initial || *p.x_initial == 0) return;
+  initial_defect_group(as_global(p.rhs64), as_global(p.screen64), as_global(p.r32), g,
+                       (lb - slot * nbi) * kThreads + threadIdx.x, lanes, pw, h2, p.live);
+}
+
+// FIRST: the first cycle's defect; entries whose x_initial flag is set are left to
+// kb_defect_initial
 template <int B, bool FIRST>
 __global__ __launch_bounds__(B) void kb_defect_group(const TopPtrs* tab, uint32_t count,
                                                      uint32_t nbi, Geom g, uint32_t lanes,
@@ -430,14 +492,10 @@ __global__ __launch_bounds__(B) void kb_defect_group(const TopPtrs* tab, uint32_
   if (slot >= count) return;
   const TopPtrs p = tab[slot];
   const uint32_t lane = (lb - slot * nbi) * B + threadIdx.x;
-  if (FIRST && p.x_initial && *p.x_initial != 0)
-    residual_group<double, float, true>(as_global((const double*)p.x64), as_global(p.rhs64),
-                                        as_global(p.screen64), as_global(p.r32), g, lane, lanes,
-                                        pw, h2, LiveMask{p.live, true}, kInitialX);
-  else
-    residual_group<double, float>(as_global((const double*)p.x64), as_global(p.rhs64),
-                                  as_global(p.screen64), as_global(p.r32), g, lane, lanes, pw,
-                                  h2, LiveMask{p.live, true});
+  if (FIRST && p.x_initial && *p.x_initial != 0) return;
+  residual_group<double, float>(as_global((const double*)p.x64), as_global(p.rhs64),
+                                as_global(p.screen64), as_global(p.r32), g, lane, lanes, pw, h2,
+                                LiveMask{p.live, true});
 }
 
 __global__ __launch_bounds__(kThreads) void kb_correct(const TopPtrs* tab, uint32_t count,
@@ -816,6 +874,11 @@ void launch_b_defect(const TopPtrs* tab, uint32_t count, const Geom& g, double p
 #define MPR_LAUNCH_DEFECT1(B) MPR_LAUNCH_DEFECT_F(B, true)
 #define MPR_LAUNCH_DEFECT0(B) MPR_LAUNCH_DEFECT_F(B, false)
     if (first) {
+      hipLaunchKernelGGL(kb_defect_initial, dim3(padded8((uint64_t)blocks_for(lanes, kThreads) *
+                                                         count)),
+                         dim3(kThreads), 0, s, tab, count, blocks_for(lanes, kThreads), g, lanes,
+                         pw, h2);
+      MPR_LAUNCH_CHECK();
       MPR_WITH_BLOCK(block, MPR_LAUNCH_DEFECT1)
     } else {
       MPR_WITH_BLOCK(block, MPR_LAUNCH_DEFECT0)
