@@ -42,7 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from rocprofv3 counter runs (the sweep timing's plain
 # launches), the factors calibrated on a plain elementwise copy
 # (profiles/kb_relax_group_f32_depth8_pmc.txt).
-ROOFLINE_TRAFFIC_BYTES = {(8, 8): 894_782_955}
+ROOFLINE_TRAFFIC_BYTES = {(8, 8): 894_785_904}
 
 
 def log(*args):
