@@ -248,6 +248,21 @@ int mpr_tune_set_groups(int32_t kernel, int32_t fp64, int32_t size_class, int32_
 int mpr_tune_get_planes(int32_t kernel, int32_t fp64, int32_t size_class);
 int mpr_tune_set_planes(int32_t kernel, int32_t fp64, int32_t size_class, int32_t planes);
 
+/* Allocation-free variant of mpr_iso_value_f64. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, serving loops
+ * that must not call hipMalloc). field ((n,n,n) float64), points and normals ((P,3) float64)
+ * and iso_value (one double, the result) are DEVICE pointers; the work is enqueued on
+ * `stream` (a hipStream_t; NULL is the default stream) and the call returns without
+ * waiting. The kernels are mpr_iso_value_f64's, so the value is the same bit for bit. Their
+ * partial sums live in a static device array of the library (1 MiB per GPU, held until the
+ * process ends), which bounds point_count to 2^24; n must be in [2, 1025]. Every call on a
+ * GPU shares that GPU's array, so calls that overlap on one GPU (on different streams or
+ * host threads) race. Allocates nothing (no hipMalloc, hipFree or allocating library
+ * call). Returns 0, or -1 with mpr_last_error() set. */
+int mpr_iso_value_f64_noalloc(const double *field, const double *points, const double *normals,
+                              int64_t point_count, int64_t n, int64_t confidence,
+                              double *iso_value, void *stream);
+
 /* Extraction of a caller grid (core.py:309-374): field and density host (n,n,n) float64
  * (density may be NULL); returns counts; then fetch and destroy. */
 typedef struct mpr_mesh mpr_mesh;
