@@ -65,15 +65,33 @@ def build(force: bool = False) -> str:
 _lib = None
 
 
+def _load(path: str) -> ctypes.CDLL:
+    handle = ctypes.CDLL(path)
+    for name, (args, res) in _SIGNATURES.items():
+        fn = getattr(handle, name)
+        fn.argtypes = args
+        fn.restype = res
+    return handle
+
+
 def lib() -> ctypes.CDLL:
     global _lib
     if _lib is None:
-        _lib = ctypes.CDLL(build())
-        for name, (args, res) in _SIGNATURES.items():
-            fn = getattr(_lib, name)
-            fn.argtypes = args
-            fn.restype = res
+        _lib = _load(build())
     return _lib
+
+
+def build_mutation(mutation: int, out_dir: str) -> ctypes.CDLL:
+    """A copy of the oracle built with -DORACLE_MUTATION=`mutation` (a deliberate slip in
+    the solver orchestration, mpr_oracle.c) into out_dir, for tests that show their checks
+    reject it. Never used as the checker."""
+    path = os.path.join(out_dir, f"liboracle_mutation{int(mutation)}.so")
+    cmd = [
+        "gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
+        f"-DORACLE_MUTATION={int(mutation)}", SRC, "-o", path, "-lm",
+    ]
+    subprocess.run(cmd, check=True, capture_output=True, text=True)
+    return _load(path)
 
 
 def _p(a: np.ndarray) -> int:
@@ -333,18 +351,19 @@ def build_octree(points, depth: int):
 
 
 def solve(points, normals, depth: int, point_weight: float = 4.0, confidence: bool = False,
-          cycles: int = 4, samples_per_node: float = 1.5) -> dict:
-    """core.py:222-306 on already-validated unit-cube rows (octree omitted)."""
+          cycles: int = 4, samples_per_node: float = 1.5, handle=None) -> dict:
+    """core.py:222-306 on already-validated unit-cube rows (octree omitted). `handle`: an
+    oracle library other than the default (build_mutation)."""
     points, normals = _f(points).reshape(-1, 3), _f(normals).reshape(-1, 3)
     valid = valid_oriented_rows(points, normals)
     points, normals = np.ascontiguousarray(points[valid]), np.ascontiguousarray(normals[valid])
     n = (1 << depth) + 1
     field, density, screen, rhs = (np.empty((n, n, n)) for _ in range(4))
     iso, rn = ctypes.c_double(), ctypes.c_double()
-    acc = lib().oracle_solve(_p(points), _p(normals), len(points), depth, float(point_weight),
-                             int(confidence), int(cycles), float(samples_per_node), _p(field),
-                             _p(density), _p(screen), _p(rhs), ctypes.addressof(iso),
-                             ctypes.addressof(rn))
+    acc = (handle or lib()).oracle_solve(
+        _p(points), _p(normals), len(points), depth, float(point_weight), int(confidence),
+        int(cycles), float(samples_per_node), _p(field), _p(density), _p(screen), _p(rhs),
+        ctypes.addressof(iso), ctypes.addressof(rn))
     return dict(field=field, density=density, screen=screen, rhs=rhs, iso_value=iso.value,
                 residual_norm=rn.value, accepted=acc, points=points, normals=normals)
 

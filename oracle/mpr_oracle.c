@@ -15,6 +15,14 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Test-only slips in the solver orchestration: tests/test_oracle_solve.py builds copies of
+ * this file with -DORACLE_MUTATION=k (k = 1..9) and shows that its checks reject each one.
+ * The oracle proper is built with the default, 0, which is the restatement itself. */
+#ifndef ORACLE_MUTATION
+#define ORACLE_MUTATION 0
+#endif
+#define MUTATED(k) (ORACLE_MUTATION == (k))
+
 static inline int64_t gidx(int64_t x, int64_t y, int64_t z, int64_t n) {
     return (z * n + y) * n + x; /* capi.mojo:41-43 */
 }
@@ -467,17 +475,22 @@ static void v_cycle(int level, int levels, double **sol, double **rhs, double **
                     const int64_t *ns, double pw, int pre, int post) {
     int64_t n = ns[level];
     if (n <= 3) {
-        oracle_relax(sol[level], rhs[level], scr[level], n, pw, 40);
+        oracle_relax(sol[level], rhs[level], scr[level], n, pw, MUTATED(7) ? 20 : 40);
         return;
     }
-    oracle_relax(sol[level], rhs[level], scr[level], n, pw, pre);
+    oracle_relax(sol[level], rhs[level], scr[level], n, pw, MUTATED(6) ? pre - 1 : pre);
     oracle_residual(sol[level], rhs[level], scr[level], res[level], n, pw);
-    oracle_restrict(res[level], rhs[level + 1], n);
+    oracle_restrict(MUTATED(9) ? sol[level] : res[level], rhs[level + 1], n);
     int64_t cn = ns[level + 1];
-    memset(sol[level + 1], 0, sizeof(double) * (size_t)(cn * cn * cn));
+    if (!MUTATED(3)) memset(sol[level + 1], 0, sizeof(double) * (size_t)(cn * cn * cn));
     v_cycle(level + 1, levels, sol, rhs, scr, res, ns, pw, pre, post);
-    oracle_prolong_add(sol[level + 1], sol[level], n);
-    oracle_relax(sol[level], rhs[level], scr[level], n, pw, post);
+    if (MUTATED(4)) {
+        oracle_relax(sol[level], rhs[level], scr[level], n, pw, post);
+        oracle_prolong_add(sol[level + 1], sol[level], n);
+    } else {
+        oracle_prolong_add(sol[level + 1], sol[level], n);
+        oracle_relax(sol[level], rhs[level], scr[level], n, pw, post);
+    }
     (void)levels;
 }
 
@@ -498,8 +511,9 @@ int64_t oracle_solve(const double *points, const double *normals, int64_t point_
     if (point_count) {
         accepted = oracle_splat(points, normals, point_count, n, confidence, density, vx, vy, vz,
                                 screen, samples_per_node);
-        for (size_t i = 0; i < m; ++i) density[i] /= samples_per_node;
-        oracle_assemble(vx, vy, vz, screen, rhs, n, point_weight, 0.5);
+        if (!MUTATED(5))
+            for (size_t i = 0; i < m; ++i) density[i] /= samples_per_node;
+        oracle_assemble(vx, vy, vz, screen, rhs, n, point_weight, MUTATED(8) ? 1.0 : 0.5);
     }
     free(vx); free(vy); free(vz);
 
@@ -514,10 +528,10 @@ int64_t oracle_solve(const double *points, const double *normals, int64_t point_
         rh[l] = calloc(ml, sizeof(double));
         rs[l] = malloc(ml * sizeof(double));
         sc[l] = l == 0 ? screen : malloc(ml * sizeof(double));
-        for (size_t i = 0; i < ml; ++i) sol[l][i] = 0.5;
+        for (size_t i = 0; i < ml; ++i) sol[l][i] = MUTATED(2) ? 0.0 : 0.5;
         if (l > 0) {
             oracle_restrict(sc[l - 1], sc[l], ns[l - 1]);
-            for (size_t i = 0; i < ml; ++i) sc[l][i] *= 4.0; /* core.py:267-269 */
+            for (size_t i = 0; i < ml; ++i) sc[l][i] *= MUTATED(1) ? 2.0 : 4.0; /* core.py:267-269 */
         }
     }
     memcpy(rh[0], rhs, m * sizeof(double));
