@@ -263,6 +263,18 @@ int mpr_iso_value_f64_noalloc(const double *field, const double *points, const d
                               int64_t point_count, int64_t n, int64_t confidence,
                               double *iso_value, void *stream);
 
+/* Distances between two triangle meshes (host buffers: vertices (V,3) float64, faces (F,3)
+ * int64), the accuracy metric of the reference's end-to-end tests
+ * (tests/test_reconstruction.py:50-62), computed exactly (point to triangle) on the GPU:
+ * out[0] = max over A's vertices of the distance to B's surface, out[1] = the same from B to
+ * A, out[2] and out[3] the corresponding means. The symmetric Hausdorff distance is
+ * max(out[0], out[1]); the vertex chamfer (out[2] + out[3]) / 2. A direction with no query
+ * vertex gives 0; one with vertices but no target triangle gives +inf. Returns 0, or -1 with
+ * mpr_last_error() set (face index out of range, non-finite vertex, device error). */
+int mpr_mesh_distance(const double *vertices_a, int64_t vertex_count_a, const int64_t *faces_a,
+                      int64_t face_count_a, const double *vertices_b, int64_t vertex_count_b,
+                      const int64_t *faces_b, int64_t face_count_b, double *out);
+
 /* Extraction of a caller grid (core.py:309-374): field and density host (n,n,n) float64
  * (density may be NULL); returns counts; then fetch and destroy. */
 typedef struct mpr_mesh mpr_mesh;

@@ -123,6 +123,7 @@ _PIPELINE = {
     "mpr_mesh_fetch": ([P, P, P, P], ctypes.c_int),
     "mpr_mesh_destroy": ([P], ctypes.c_int),
     "mpr_iso_value_f64_noalloc": ([P, P, P, I, I, I, P, P], ctypes.c_int),
+    "mpr_mesh_distance": ([P, I, P, I, P, I, P, I, P], ctypes.c_int),
 }
 
 
