@@ -94,23 +94,6 @@ def cpu_baseline(points, normals, depth: int, budget_s: float = 20.0) -> dict:
     }
 
 
-def uneven_batch(count: int, seed: int):
-    """64-instance workload: depths U{5..8}, points log-uniform [2k, 200k], shapes cycle
-    sphere -> torus -> superquadric, position noise U[0, 0.003] x diagonal."""
-    from mojo_poissonrecon import shapes
-
-    rng = np.random.default_rng(seed)
-    makers = (shapes.sphere, shapes.torus, shapes.superquadric)
-    clouds, depths = [], []
-    for i in range(count):
-        npts = int(round(math.exp(rng.uniform(math.log(2e3), math.log(2e5)))))
-        p, n = makers[i % 3](npts, seed=seed * 1000 + i)
-        p, n = shapes.perturb(p, n, seed=seed * 1000 + i, sigma=float(rng.uniform(0, 0.003)))
-        clouds.append((p, n))
-        depths.append(int(rng.integers(5, 9)))
-    return clouds, depths
-
-
 def run_batch_mode(clouds, depths, precision, reps, kernel_reps=0, kernels=None):
     """Records the batched run as one graph, replays it; returns (seconds per launch, latencies,
     meshes). With `kernels` [(name, id)] it also times those batched kernels afterwards on the
@@ -408,7 +391,7 @@ def main() -> int:
 
     extras = {}
     if not args.no_extras:
-        clouds, depths = uneven_batch(args.instances, seed=1)
+        clouds, depths = shapes.uneven_batch_workload(args.instances, seed=1)
         t32, _, m32 = run_batch_mode(clouds, depths, "mixed", 3)
         t64, _, m64 = run_batch_mode(clouds, depths, "fp64", 3)
         errs = []

@@ -73,11 +73,12 @@ class Batch:
 
 def reconstruct_batch(clouds, *, depths, point_weight: float = 4.0, confidence=False,
                       cycles: int = 4, scale: float = 1.1, samples_per_node: float = 1.5,
-                      precision: str = "mixed") -> list[Mesh]:
+                      precision: str = "mixed", graph: bool = False) -> list[Mesh]:
     """Reconstruct each ``(points, normals)`` cloud at its depth with one batched device pass.
 
     ``depths`` and ``confidence`` may be scalars or per-cloud sequences. Every cloud gets the
-    validation of reconstruct(); a cloud with no valid rows yields an empty Mesh.
+    validation of reconstruct(); a cloud with no valid rows yields an empty Mesh. With
+    ``graph`` the batched run is recorded as one HIP graph and replayed as a single launch.
     """
     clouds = list(clouds)
     count = len(clouds)
@@ -100,7 +101,12 @@ def reconstruct_batch(clouds, *, depths, point_weight: float = 4.0, confidence=F
             j.set_points(reqs[i].points, reqs[i].normals)
         batch = Batch(jobs)
         try:
-            batch.run([reqs[i].params for i in live])
+            params = [reqs[i].params for i in live]
+            if graph:
+                batch.capture(params)  # records only
+                batch.launch()
+            else:
+                batch.run(params)
             results = batch.wait()
         finally:
             batch.close()

@@ -12,6 +12,8 @@ random axis perpendicular to it; normal lengths are drawn from ``length_range``.
 
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 
@@ -100,6 +102,22 @@ def superquadric_workload(count: int = 2_000_000, seed: int = 0):
     """Mixed-precision depth-9 workload: 2M superquadric points, sigma 0.001, 3 deg."""
     p, n = superquadric(count, seed)
     return perturb(p, n, seed, sigma=0.001, degrees=3.0)
+
+
+def uneven_batch_workload(count: int = 64, seed: int = 1):
+    """Batch of independent clouds: depths U{5..8}, points log-uniform [2k, 200k], shapes
+    cycling sphere -> torus -> superquadric, position noise U[0, 0.003] x diagonal. Returns
+    (clouds, depths)."""
+    rng = np.random.default_rng(seed)
+    makers = (sphere, torus, superquadric)
+    clouds, depths = [], []
+    for i in range(count):
+        npts = int(round(math.exp(rng.uniform(math.log(2e3), math.log(2e5)))))
+        p, n = makers[i % 3](npts, seed=seed * 1000 + i)
+        p, n = perturb(p, n, seed=seed * 1000 + i, sigma=float(rng.uniform(0, 0.003)))
+        clouds.append((p, n))
+        depths.append(int(rng.integers(5, 9)))
+    return clouds, depths
 
 
 def random_rotation(rng) -> np.ndarray:
