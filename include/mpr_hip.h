@@ -161,6 +161,16 @@ int mpr_job_wait(mpr_job *job, mpr_result *result);
 /* Non-blocking completion test: 1 done, 0 running, <0 error. */
 int mpr_job_query(mpr_job *job);
 int mpr_job_fetch_mesh(mpr_job *job, double *vertices, double *vertex_density, int64_t *faces);
+/* Host mesh output (the mesh half of core.py:422-426 without a separate device-to-host
+ * copy): with enable != 0 every extraction of the job also copies its mesh (world space with
+ * MPR_STAGE_WORLD) into pinned host buffers owned by the job, on the job's stream (sized on
+ * the device from the counts, so it is part of a recorded graph too); mpr_job_fetch_mesh then
+ * copies from host memory, and mpr_job_host_mesh gives the pinned buffers themselves (valid
+ * until the next run, overflow growth or mpr_job_set_host_mesh). Default off. No reference
+ * counterpart. */
+int mpr_job_set_host_mesh(mpr_job *job, int32_t enable);
+int mpr_job_host_mesh(mpr_job *job, const double **vertices, const double **vertex_density,
+                      const int64_t **faces);
 /* Any pointer may be NULL. Each grid requires MPR_STAGE_GRIDS on the run (without it the
  * assembly writes only the solver-layout grids near the splat and leaves the density as
  * splatted). */
@@ -171,8 +181,10 @@ int mpr_job_fetch_grids(mpr_job *job, double *field, double *density, double *sc
 int mpr_job_build_octree(mpr_job *job, int32_t depth, int64_t *node_count);
 int mpr_job_fetch_octree(mpr_job *job, int64_t *depth, int64_t *morton, int64_t *parent,
                          int64_t *child_start, int64_t *child_count, int64_t *point_leaf);
-/* Records the job's run (for fixed params and point count) into a HIP graph and replays
- * it; mpr_job_launch replays without re-recording. */
+/* Records the job's run (for fixed params and point count) into a HIP graph (recording
+ * runs nothing); mpr_job_launch replays it without re-recording, except that a graph dropped
+ * because the job's mesh buffers moved (overflow growth, mpr_job_set_host_mesh) is first
+ * recorded again with the same parameters. */
 int mpr_job_capture(mpr_job *job, const mpr_params *params);
 int mpr_job_launch(mpr_job *job);
 /* Device pointers of the job's input buffers (for callers whose points are already in HBM). */

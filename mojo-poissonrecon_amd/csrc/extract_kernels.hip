@@ -646,6 +646,52 @@ void launch_bit_extract(const FieldView& f, const DensityView& dens, int n, cons
   MPR_LAUNCH_CHECK();
 }
 
+// Host copy of the mesh: the first counts[0] vertices (3 doubles each) and vertex densities
+// and 2 * counts[1] faces (3 int64 each), clamped to the capacities, copied word for word to
+// (pinned) host buffers of the same layout. The counts live on the device, so the copy is
+// sized there: a fixed grid strides over 16-byte word pairs of the three arrays in turn.
+__global__ __launch_bounds__(kThreads) void k_mesh_to_host(const uint64_t* v, const uint64_t* d,
+                                                           const uint64_t* f,
+                                                           const uint32_t* counts, uint64_t vcap,
+                                                           uint64_t fcap, uint64_t* hv,
+                                                           uint64_t* hd, uint64_t* hf) {
+  using U2 = unsigned long long __attribute__((ext_vector_type(2)));
+  const uint64_t nv = counts[0] < vcap ? counts[0] : vcap;
+  const uint64_t nf = 2ull * counts[1] < fcap ? 2ull * counts[1] : fcap;
+  const uint64_t w[3] = {3 * nv, nv, 3 * nf};
+  const uint64_t p0 = (w[0] + 1) / 2, p1 = (w[1] + 1) / 2, p2 = (w[2] + 1) / 2;
+  const uint64_t total = p0 + p1 + p2;
+  for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * kThreads) {
+    const uint64_t* src;
+    uint64_t* dst;
+    uint64_t j, words;
+    if (i < p0) {
+      src = v, dst = hv, j = i, words = w[0];
+    } else if (i < p0 + p1) {
+      src = d, dst = hd, j = i - p0, words = w[1];
+    } else {
+      src = f, dst = hf, j = i - p0 - p1, words = w[2];
+    }
+    if (2 * j + 1 < words)
+      *(U2*)(dst + 2 * j) = *(const U2*)(src + 2 * j);
+    else
+      dst[2 * j] = src[2 * j];
+  }
+}
+
+void launch_mesh_to_host(const double* verts, const double* vdens, const int64_t* faces,
+                         const uint32_t* counts, uint64_t vcap, uint64_t fcap, double* h_verts,
+                         double* h_vdens, int64_t* h_faces, hipStream_t s) {
+  const uint64_t pairs = (3 * vcap + 1) / 2 + (vcap + 1) / 2 + (3 * fcap + 1) / 2;
+  const unsigned blocks = (unsigned)std::min<uint64_t>(2048, std::max<uint64_t>(1, (pairs + kThreads - 1) / kThreads));
+  hipLaunchKernelGGL(k_mesh_to_host, dim3(blocks), dim3(kThreads), 0, s,
+                     (const uint64_t*)verts, (const uint64_t*)vdens, (const uint64_t*)faces,
+                     counts, vcap, fcap, (uint64_t*)h_verts, (uint64_t*)h_vdens,
+                     (uint64_t*)h_faces);
+  MPR_LAUNCH_CHECK();
+}
+
 template void launch_cells_emit<int32_t>(const double*, const double*, int, const double*, const uint32_t*,
                                          int32_t*, double*, double*, uint64_t, const double*,
                                          hipStream_t);

@@ -162,6 +162,12 @@ void launch_bit_extract(const FieldView& f, const DensityView& dens, int n, cons
                         uint64_t vcap, int64_t* faces, uint64_t fcap, const double* xform,
                         hipStream_t s);
 
+// copies the mesh (counts[0] vertices, 2 * counts[1] faces, clamped to the capacities, sizes
+// read on the device) to host buffers of the same layout, e.g. pinned host memory
+void launch_mesh_to_host(const double* verts, const double* vdens, const int64_t* faces,
+                         const uint32_t* counts, uint64_t vcap, uint64_t fcap, double* h_verts,
+                         double* h_vdens, int64_t* h_faces, hipStream_t s);
+
 void launch_cells_count(const double* f, int n, const double* iso, uint32_t* counts, hipStream_t s);
 void launch_cells_chunk_counts(const double* f, int n, const double* iso, int64_t* chunks,
                                hipStream_t s);

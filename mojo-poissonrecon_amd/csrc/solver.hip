@@ -143,10 +143,28 @@ struct Job {
   double *verts = nullptr, *vdens = nullptr;
   int64_t* faces = nullptr;
   uint64_t vcap = 0, fcap = 0;
-  std::vector<void*> mesh_bufs;  // grown extraction outputs (owned separately)
+  std::vector<void*> mesh_bufs;  // grown extraction outputs (owned separately, latest set only)
+  // host mesh: each extraction also copies the mesh into these pinned host buffers on the
+  // job's stream (capacities vcap / fcap), so fetching it is a host copy
+  bool host_mesh = false;
+  double *h_verts = nullptr, *h_vdens = nullptr;
+  int64_t* h_faces = nullptr;
+  void free_host_mesh() {
+    for (void* p : {(void*)h_verts, (void*)h_vdens, (void*)h_faces})
+      if (p) (void)hipHostFree(p);
+    h_verts = h_vdens = nullptr;
+    h_faces = nullptr;
+  }
+  void alloc_host_mesh() {
+    free_host_mesh();
+    MPR_HIP(hipHostMalloc((void**)&h_verts, 24 * std::max<uint64_t>(1, vcap), hipHostMallocDefault));
+    MPR_HIP(hipHostMalloc((void**)&h_vdens, 8 * std::max<uint64_t>(1, vcap), hipHostMallocDefault));
+    MPR_HIP(hipHostMalloc((void**)&h_faces, 24 * std::max<uint64_t>(1, fcap), hipHostMallocDefault));
+  }
   // host results
   HostScalars* host = nullptr;
   mpr_params last{}, captured{};
+  bool has_capture = false;  // `captured` was recorded (the graph may since have been dropped)
   bool ran = false, grids_kept = false;
   // graph
   hipGraph_t graph = nullptr;
@@ -220,6 +238,11 @@ void Job::resolve_overflow(hipStream_t s) {
   const uint64_t nv = host->counts[0], nf = 2ull * host->counts[1];
   if (nv <= vcap && nf <= fcap) return;
   const uint64_t nvc = std::max<uint64_t>(nv, vcap), nfc = std::max<uint64_t>(nf, fcap);
+  // the stream is idle (the caller waited for it) and the job's graph is dropped below, so
+  // the previously grown set is no longer referenced (a batch graph holding it records again
+  // when mesh_generation changes): free it, keeping only the latest set
+  for (void* p : mesh_bufs) (void)hipFree(p);
+  mesh_bufs.clear();
   void *v = nullptr, *d = nullptr, *f = nullptr, *c = nullptr;
   MPR_HIP(hipMalloc(&v, 24 * nvc));
   mesh_bufs.push_back(v);
@@ -236,6 +259,7 @@ void Job::resolve_overflow(hipStream_t s) {
   surf.vcell_cap = nvc;
   vcap = nvc;
   fcap = nfc;
+  if (host_mesh) alloc_host_mesh();
   ++mesh_generation;
   if (exec) {  // the captured graph points at the old buffers
     MPR_HIP(hipGraphExecDestroy(exec));
@@ -539,6 +563,7 @@ Job::~Job() {
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
   for (void* p : mesh_bufs) (void)hipFree(p);
+  free_host_mesh();
   oct.release();
   arena.release();
   if (host) (void)hipHostFree(host);
@@ -548,6 +573,8 @@ Job::~Job() {
 void Job::enqueue_extract(bool world, hipStream_t s) {
   launch_bit_extract(split_field(L64[0].x, geoms[0]), density_view(), n, d_iso, surf, d_counts,
                      verts, vdens, vcap, faces, fcap, world ? d_params : nullptr, s);
+  if (host_mesh)
+    launch_mesh_to_host(verts, vdens, faces, d_counts, vcap, fcap, h_verts, h_vdens, h_faces, s);
 }
 
 // normalise, splat, assemble, screening pyramid and initial guess (core.py:229-273)
@@ -843,15 +870,20 @@ int mpr_job_capture(mpr_job* job, const mpr_params* params) {
   MPR_HIP(hipGraphInstantiate(&job->exec, job->graph, nullptr, nullptr, 0));
   // recording ran nothing: the job's state is still that of its last executed work
   job->captured = p;
+  job->has_capture = true;
   job->restore(before);
   return 0;
   MPR_GUARD_END
 }
 
 int mpr_job_launch(mpr_job* job) {
-  if (!job || !job->exec) {
+  if (!job || (!job->exec && !job->has_capture)) {
     set_error("job has no captured graph");
     return -1;
+  }
+  if (!job->exec) {  // dropped when the mesh buffers moved: record the same run again
+    const mpr_params p = job->captured;
+    if (int rc = mpr_job_capture(job, &p)) return rc;
   }
   MPR_GUARD_BEGIN
   MPR_HIP(hipGraphLaunch(job->exec, job->stream));
@@ -888,6 +920,13 @@ int mpr_job_fetch_mesh(mpr_job* job, double* vertices, double* vertex_density, i
     set_error("mesh exceeds capacity; call mpr_job_wait first");
     return -1;
   }
+  if (job->host_mesh) {  // the run's extraction copied the mesh to pinned host memory
+    MPR_HIP(hipStreamSynchronize(job->stream));
+    if (vertices && nv) std::memcpy(vertices, job->h_verts, 24 * nv);
+    if (vertex_density && nv) std::memcpy(vertex_density, job->h_vdens, 8 * nv);
+    if (faces && nf) std::memcpy(faces, job->h_faces, 24 * nf);
+    return 0;
+  }
   if (vertices && nv)
     MPR_HIP(hipMemcpyAsync(vertices, job->verts, 24 * nv, hipMemcpyDeviceToHost, job->stream));
   if (vertex_density && nv)
@@ -897,6 +936,41 @@ int mpr_job_fetch_mesh(mpr_job* job, double* vertices, double* vertex_density, i
   MPR_HIP(hipStreamSynchronize(job->stream));
   return 0;
   MPR_GUARD_END
+}
+
+int mpr_job_set_host_mesh(mpr_job* job, int32_t enable) {
+  if (!job) return -1;
+  MPR_GUARD_BEGIN
+  const bool on = enable != 0;
+  if (on == job->host_mesh) return 0;
+  MPR_HIP(hipStreamSynchronize(job->stream));
+  if (on)
+    job->alloc_host_mesh();
+  else
+    job->free_host_mesh();
+  job->host_mesh = on;
+  // recorded graphs of this job or of batches holding it have the other extraction
+  if (job->exec) {
+    MPR_HIP(hipGraphExecDestroy(job->exec));
+    MPR_HIP(hipGraphDestroy(job->graph));
+    job->exec = nullptr;
+    job->graph = nullptr;
+  }
+  ++job->mesh_generation;
+  return 0;
+  MPR_GUARD_END
+}
+
+int mpr_job_host_mesh(mpr_job* job, const double** vertices, const double** vertex_density,
+                      const int64_t** faces) {
+  if (!job || !job->host_mesh) {
+    set_error("the job does not copy its mesh to host memory (mpr_job_set_host_mesh)");
+    return -1;
+  }
+  if (vertices) *vertices = job->h_verts;
+  if (vertex_density) *vertex_density = job->h_vdens;
+  if (faces) *faces = job->h_faces;
+  return 0;
 }
 
 int mpr_job_fetch_grids(mpr_job* job, double* field, double* density, double* screen, double* rhs) {

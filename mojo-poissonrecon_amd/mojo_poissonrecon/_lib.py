@@ -92,6 +92,8 @@ _PIPELINE = {
     "mpr_job_wait": ([P, ctypes.POINTER(mpr_result)], ctypes.c_int),
     "mpr_job_query": ([P], ctypes.c_int),
     "mpr_job_fetch_mesh": ([P, P, P, P], ctypes.c_int),
+    "mpr_job_set_host_mesh": ([P, S], ctypes.c_int),
+    "mpr_job_host_mesh": ([P, _PP, _PP, _PP], ctypes.c_int),
     "mpr_job_fetch_grids": ([P, P, P, P, P], ctypes.c_int),
     "mpr_job_build_octree": ([P, S, _IP], ctypes.c_int),
     "mpr_job_fetch_octree": ([P, P, P, P, P, P, P], ctypes.c_int),

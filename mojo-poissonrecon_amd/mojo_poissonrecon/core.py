@@ -122,9 +122,14 @@ def _filter(points: np.ndarray, normals: np.ndarray):
 # ---- device jobs ----------------------------------------------------------------------
 
 class Job:
-    """One device-resident reconstruction slot (include/mpr_hip.h ``mpr_job``)."""
+    """One device-resident reconstruction slot (include/mpr_hip.h ``mpr_job``).
 
-    def __init__(self, depth: int, max_points: int, precision: int = _lib.PRECISION_FP64):
+    ``host_mesh``: every extraction also copies the mesh into the job's pinned host buffers
+    on its stream (``mpr_job_set_host_mesh``), so fetching it is a host copy.
+    """
+
+    def __init__(self, depth: int, max_points: int, precision: int = _lib.PRECISION_FP64,
+                 host_mesh: bool = True):
         self._lib = lib()
         self.depth = depth
         self.max_points = max_points
@@ -134,6 +139,14 @@ class Job:
               "mpr_job_create")
         self.handle = handle
         self.count = 0
+        self.host_mesh = False
+        if host_mesh:
+            self.set_host_mesh(True)
+
+    def set_host_mesh(self, enable: bool) -> None:
+        check(self._lib.mpr_job_set_host_mesh(self.handle, int(bool(enable))),
+              "mpr_job_set_host_mesh")
+        self.host_mesh = bool(enable)
 
     @staticmethod
     def footprint(depth: int, max_points: int, precision: int = _lib.PRECISION_FP64) -> int:

@@ -205,3 +205,35 @@ def test_workgroup_size_table_without_a_device():
     for kernel, planes in (("relax", 3), ("relax", 8), ("relax", 0), ("defect", 2)):
         with pytest.raises(DeviceError):
             tune.set_tuned_planes(kernel, "fp64", 7, planes)
+
+
+def test_ply_round_trip(tmp_path):
+    from mojo_poissonrecon.core import Mesh
+    from mojo_poissonrecon.ply import read_ply, write_ply
+
+    rng = np.random.default_rng(4)
+    v = rng.normal(size=(50, 3)) * 1e3
+    f = rng.integers(0, 50, (80, 3))
+    d = rng.random(50)
+    mesh = Mesh(v, f, d, 0.5, 1.0)
+    for binary in (True, False):
+        path = tmp_path / f"m{int(binary)}.ply"
+        write_ply(path, mesh, binary=binary)
+        rv, rf, rd = read_ply(path)
+        assert np.array_equal(rv, v) and np.array_equal(rf, f) and np.array_equal(rd, d)
+        header = path.read_bytes().split(b"end_header")[0].decode()
+        assert "element vertex 50" in header and "element face 80" in header
+        assert "property list uchar int vertex_indices" in header
+    write_ply(tmp_path / "nd.ply", mesh, density=False)
+    rv, rf, rd = read_ply(tmp_path / "nd.ply")
+    assert rd is None and np.array_equal(rv, v)
+    # binary layout: 4 doubles per vertex, then 13 bytes per face
+    size = (tmp_path / "m1.ply").stat().st_size
+    head = len((tmp_path / "m1.ply").read_bytes().split(b"end_header\n")[0]) + len(b"end_header\n")
+    assert size == head + 50 * 32 + 80 * 13
+    empty = Mesh(np.empty((0, 3)), np.empty((0, 3), np.int64), np.empty(0))
+    write_ply(tmp_path / "e.ply", empty)
+    rv, rf, _ = read_ply(tmp_path / "e.ply")
+    assert rv.shape == (0, 3) and rf.shape == (0, 3)
+    with pytest.raises(ValueError, match="out of range"):
+        write_ply(tmp_path / "bad.ply", Mesh(v, f + 50, d))

@@ -472,3 +472,51 @@ def test_job_rejects_negative_or_non_finite_point_weight():
                 job.run(Job.params(_lib.STAGE_SOLVE, point_weight=pw))
     finally:
         job.close()
+
+
+def test_host_mesh_output_equals_device_fetch_and_round_trips_as_ply(tmp_path):
+    import ctypes
+
+    from mojo_poissonrecon import _lib
+    from mojo_poissonrecon.core import Job, mesh_from_job, prepare_reconstruct
+    from mojo_poissonrecon.ply import read_ply, write_ply
+
+    p, n = shapes.torus_workload(50_000, seed=5)
+    req = prepare_reconstruct(p, n, depth=7, confidence=True)
+    dev = Job(7, len(req.points), req.precision, host_mesh=False)
+    host = Job(7, len(req.points), req.precision, host_mesh=True)
+    meshes = []
+    for job in (dev, host):
+        job.set_points(req.points, req.normals)
+        job.run(req.params)
+        meshes.append(mesh_from_job(job, job.wait(), len(req.points)))
+    md, mh = meshes
+    assert len(mh.faces) > 10_000
+    assert np.array_equal(md.vertices, mh.vertices)
+    assert np.array_equal(md.faces, mh.faces)
+    assert np.array_equal(md.density, mh.density)
+    # the pinned buffers themselves hold the mesh
+    v, d, f = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    _lib.check(_lib.lib().mpr_job_host_mesh(host.handle, ctypes.byref(v), ctypes.byref(d),
+                                            ctypes.byref(f)), "mpr_job_host_mesh")
+    pv = np.ctypeslib.as_array(ctypes.cast(v, ctypes.POINTER(ctypes.c_double)),
+                               (len(mh.vertices), 3))
+    pf = np.ctypeslib.as_array(ctypes.cast(f, ctypes.POINTER(ctypes.c_int64)), (len(mh.faces), 3))
+    assert np.array_equal(pv, mh.vertices) and np.array_equal(pf, mh.faces)
+    # graph replay in host mode, then host mode off and on again (graphs re-record)
+    host.capture(req.params)
+    host.launch()
+    assert np.array_equal(mesh_from_job(host, host.wait(), len(req.points)).faces, mh.faces)
+    host.set_host_mesh(False)
+    host.launch()
+    assert np.array_equal(mesh_from_job(host, host.wait(), len(req.points)).vertices, mh.vertices)
+    assert _lib.lib().mpr_job_host_mesh(host.handle, None, None, None) == -1
+    host.set_host_mesh(True)
+    host.run(req.params)
+    assert np.array_equal(mesh_from_job(host, host.wait(), len(req.points)).faces, mh.faces)
+    write_ply(tmp_path / "torus.ply", mh)
+    rv, rf, rd = read_ply(tmp_path / "torus.ply")
+    assert np.array_equal(rv, mh.vertices) and np.array_equal(rf, mh.faces)
+    assert np.array_equal(rd, mh.density)
+    dev.close()
+    host.close()
