@@ -117,7 +117,10 @@ enum {
   MPR_STAGE_SOLVE = 2,      /* splat, assemble, multigrid, residual, iso value */
   MPR_STAGE_EXTRACT = 4,    /* dual-grid mesh of the solution at the iso value */
   MPR_STAGE_GRIDS = 8,      /* also write natural-layout field, density, screen, rhs for fetching */
-  MPR_STAGE_WORLD = 16      /* write mesh vertices in world space (core.py:425) */
+  MPR_STAGE_WORLD = 16,     /* write mesh vertices in world space (core.py:425) */
+  MPR_STAGE_FILTER = 32     /* drop rows with a non-finite value or a zero normal on device,
+                               keeping the order (core.py:73-83, 407-409), then normalise the
+                               kept rows as MPR_STAGE_NORMALIZE does */
 };
 
 typedef struct mpr_params {
@@ -140,6 +143,8 @@ typedef struct mpr_result {
   double residual_norm;   /* float64 ||b - A x|| on the finest grid */
   double origin[3];       /* normalisation transform (MPR_STAGE_NORMALIZE) */
   double width;
+  int64_t valid_rows;     /* rows kept by MPR_STAGE_FILTER (all P without it); the splat
+                             accepts exactly these */
 } mpr_result;
 
 typedef struct mpr_job mpr_job;
@@ -189,6 +194,11 @@ int mpr_job_capture(mpr_job *job, const mpr_params *params);
 int mpr_job_launch(mpr_job *job);
 /* Device pointers of the job's input buffers (for callers whose points are already in HBM). */
 int mpr_job_device_points(mpr_job *job, double **points, double **normals);
+/* After a run with MPR_STAGE_FILTER: the number of kept rows and (either may be NULL) their
+ * unit-cube points and normals in order ((count,3) float64 each; the caller sizes them for
+ * all P rows or first asks with NULL buffers). A check of the device filter and
+ * normalisation against core.py:73-83,377-385; no reference counterpart. */
+int mpr_job_fetch_filtered(mpr_job *job, int64_t *count, double *unit_points, double *normals);
 int mpr_job_set_count(mpr_job *job, int64_t P);
 
 /* ---- batches: many jobs, one launch per V-cycle step ----------------------------------

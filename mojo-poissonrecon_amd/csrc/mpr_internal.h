@@ -85,6 +85,19 @@ int bbox_partial_count(uint32_t P);
 // core.py:377-385 on the device: params = [origin x, y, z, width], unit = (p - origin)/width
 void launch_normalize(const double* pts, uint32_t P, double scale, double* part, double* params,
                       double* unit, hipStream_t s);
+// Row filter of core.py:73-83 fused with the normalisation of core.py:377-385: the rows with
+// six finite values and a non-zero normal, in order, go to unit[0..V) (normalised by their
+// bounding box) and nrm_out[0..V); the other P - V rows follow as zero points and normals.
+// *total (device) receives V. Scratch: filter_blocks(P) counts and offsets, 6 * that many
+// doubles of bbox partials, scan_scratch_words(filter_blocks(P)) scan words.
+struct FilterScratch {
+  uint32_t *counts = nullptr, *offsets = nullptr, *scan = nullptr;
+  double* part = nullptr;
+};
+uint32_t filter_blocks(uint32_t P);
+void launch_filter_normalize(const double* pts, const double* nrm, uint32_t P, double scale,
+                             const FilterScratch& f, uint32_t* total, double* params, double* unit,
+                             double* nrm_out, hipStream_t s);
 void launch_octree_codes(const double* pts, uint32_t P, int depth, int64_t* codes, hipStream_t s);
 void launch_scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t count, uint32_t* total,
                            hipStream_t s, uint32_t* scratch = nullptr);
@@ -137,8 +150,10 @@ void launch_splat(const double* pts, const double* nrm, uint32_t P, int n, int c
                   double spn, unsigned long long* accepted, const SplatScratch& b,
                   uint32_t* touched, hipStream_t s);
 int iso_partial_count(uint32_t P);
+// count (device, nullable): only rows below *count take part
 void launch_iso(const FieldView& field, const double* pts, const double* nrm, uint32_t P, int n,
-                int confidence, double* part, double* iso, hipStream_t s);
+                int confidence, double* part, double* iso, hipStream_t s,
+                const uint32_t* count = nullptr);
 
 // ---- extraction kernels (extract_kernels.hip) --------------------------------------------
 uint32_t cell_tiles(int n);

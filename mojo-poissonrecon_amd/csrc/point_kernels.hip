@@ -91,6 +91,11 @@ __global__ __launch_bounds__(kThreads) void k_bbox_finish(const double* part, in
     mn[a] = lds[a][0];
     mx[a] = lds[3 + a][0];
   }
+  if (!(mn[0] <= mx[0])) {  // no points (every row dropped): the identity transform
+    params[0] = params[1] = params[2] = 0.0;
+    params[3] = 1.0;
+    return;
+  }
   double ext = mx[0] - mn[0];
   for (int a = 1; a < 3; ++a) ext = (mx[a] - mn[a]) > ext ? (mx[a] - mn[a]) : ext;
   double width = ext * scale;
@@ -100,6 +105,70 @@ __global__ __launch_bounds__(kThreads) void k_bbox_finish(const double* part, in
     params[a] = center - 0.5 * width;
   }
   params[3] = width;
+}
+
+// ---- row validation (core.py:73-83) fused with the normalisation ------------------------
+// A row is kept when its six values are finite and its normal has a non-zero component.
+// The reference also asks that the normal scaled by its largest |component| s have a finite,
+// positive squared norm; for a finite normal with s > 0 that is a sum of three squares of
+// values in [-1, 1], one of them exactly 1 (s / s), so it lies in [1, 3] and both tests keep
+// the same rows on every input.
+__device__ __forceinline__ bool row_valid(const double* pts, const double* nrm, uint32_t i) {
+  const double p0 = pts[3ull * i], p1 = pts[3ull * i + 1], p2 = pts[3ull * i + 2];
+  const double n0 = nrm[3ull * i], n1 = nrm[3ull * i + 1], n2 = nrm[3ull * i + 2];
+  return isfinite(p0) && isfinite(p1) && isfinite(p2) && isfinite(n0) && isfinite(n1) &&
+         isfinite(n2) && (n0 != 0.0 || n1 != 0.0 || n2 != 0.0);
+}
+
+// per block of kThreads rows: the number of kept rows and their bounding box
+__global__ __launch_bounds__(kThreads) void k_filter_count(const double* pts, const double* nrm,
+                                                           uint32_t P, uint32_t* counts,
+                                                           double* part) {
+  __shared__ double lds[6][kThreads];
+  __shared__ uint32_t lds4[4];
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  const bool ok = i < P && row_valid(pts, nrm, i);
+  const uint32_t c = block_count256(ok ? 1u : 0u, lds4);
+  for (int a = 0; a < 3; ++a) {
+    const double v = ok ? pts[3ull * i + a] : 0.0;
+    lds[a][threadIdx.x] = ok ? v : INFINITY;
+    lds[3 + a][threadIdx.x] = ok ? v : -INFINITY;
+  }
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int a = 0; a < 3; ++a) {
+        double o = lds[a][threadIdx.x + w];
+        if (o < lds[a][threadIdx.x]) lds[a][threadIdx.x] = o;
+        o = lds[3 + a][threadIdx.x + w];
+        if (o > lds[3 + a][threadIdx.x]) lds[3 + a][threadIdx.x] = o;
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = lds[threadIdx.x][0];
+  if (threadIdx.x == 0) counts[blockIdx.x] = c;
+}
+
+// Kept rows in their order to the front (unit-cube points (p - origin) / width, normals as
+// given), dropped rows behind them as zero points with zero normals, which the splat rejects
+// and the iso value skips (rows past *total).
+__global__ __launch_bounds__(kThreads) void k_filter_compact(const double* pts, const double* nrm,
+                                                             uint32_t P, const uint32_t* offsets,
+                                                             const uint32_t* total,
+                                                             const double* params, double* unit,
+                                                             double* nrm_out) {
+  __shared__ uint32_t lds4[4];
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  const bool ok = i < P && row_valid(pts, nrm, i);
+  uint32_t block_total;
+  const uint32_t rank = block_prefix256(ok, lds4, block_total);
+  if (i >= P) return;
+  const uint32_t before = offsets[blockIdx.x] + rank;  // kept rows before row i
+  const uint32_t dst = ok ? before : *total + (i - before);
+  for (int a = 0; a < 3; ++a) {
+    unit[3ull * dst + a] = ok ? (pts[3ull * i + a] - params[a]) / params[3] : 0.0;
+    nrm_out[3ull * dst + a] = ok ? nrm[3ull * i + a] : 0.0;
+  }
 }
 
 __global__ __launch_bounds__(kThreads) void k_normalize(const double* pts, uint32_t P,
@@ -659,14 +728,15 @@ __global__ __launch_bounds__(kThreads) void k_splat_sum2(const uint32_t* keys,
 }
 
 // ---- iso value (capi.mojo:745-795) -----------------------------------------------------
+// rows at or past *count (when given) are absent
 __global__ __launch_bounds__(kThreads) void k_iso_partials(FieldView field,
                                                            const double* pts, const double* nrm,
                                                            uint32_t P, int n, int confidence,
-                                                           double* part) {
+                                                           double* part, const uint32_t* count) {
   __shared__ double lds4[4];
   const uint32_t p = blockIdx.x * kThreads + threadIdx.x;
   double vw = 0.0, wt = 0.0;
-  if (p < P) {
+  if (p < P && (!count || p < *count)) {
     const double weight =
         confidence ? norm3(nrm[3ull * p], nrm[3ull * p + 1], nrm[3ull * p + 2]) : 1.0;
     const int64_t res = n - 1;
@@ -741,6 +811,26 @@ void launch_normalize(const double* pts, uint32_t P, double scale, double* part,
   if (P) {
     hipLaunchKernelGGL(k_normalize, dim3(blocks_for(3ull * P, kThreads)), dim3(kThreads), 0, s,
                        pts, P, params, unit);
+    MPR_LAUNCH_CHECK();
+  }
+}
+
+uint32_t filter_blocks(uint32_t P) { return std::max<uint32_t>(1u, blocks_for(P, kThreads)); }
+
+void launch_filter_normalize(const double* pts, const double* nrm, uint32_t P, double scale,
+                             const FilterScratch& f, uint32_t* total, double* params, double* unit,
+                             double* nrm_out, hipStream_t s) {
+  const uint32_t nb = filter_blocks(P);
+  hipLaunchKernelGGL(k_filter_count, dim3(nb), dim3(kThreads), 0, s, pts, nrm, P, f.counts,
+                     f.part);
+  MPR_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_bbox_finish, dim3(1), dim3(kThreads), 0, s, f.part, (int)nb, scale,
+                     params);
+  MPR_LAUNCH_CHECK();
+  launch_scan_exclusive(f.counts, f.offsets, nb, total, s, f.scan);
+  if (P) {
+    hipLaunchKernelGGL(k_filter_compact, dim3(nb), dim3(kThreads), 0, s, pts, nrm, P, f.offsets,
+                       total, params, unit, nrm_out);
     MPR_LAUNCH_CHECK();
   }
 }
@@ -930,10 +1020,10 @@ void launch_splat(const double* pts, const double* nrm, uint32_t P, int n, int c
 int iso_partial_count(uint32_t P) { return (int)std::max<uint32_t>(1u, blocks_for(P, kThreads)); }
 
 void launch_iso(const FieldView& field, const double* pts, const double* nrm, uint32_t P, int n,
-                int confidence, double* part, double* iso, hipStream_t s) {
+                int confidence, double* part, double* iso, hipStream_t s, const uint32_t* count) {
   const int nb = iso_partial_count(P);
   hipLaunchKernelGGL(k_iso_partials, dim3(nb), dim3(kThreads), 0, s, field, pts, nrm, P, n,
-                     confidence, part);
+                     confidence, part, count);
   MPR_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_iso_finish, dim3(1), dim3(kThreads), 0, s, part, nb, iso);
   MPR_LAUNCH_CHECK();

@@ -98,6 +98,8 @@ struct HostScalars {
   double iso;
   double resnorm;
   double params[4];  // origin xyz, width
+  uint32_t valid;    // rows kept by MPR_STAGE_FILTER
+  uint32_t pad;
 };
 
 }  // namespace
@@ -108,8 +110,9 @@ struct Job {
   hipStream_t stream = nullptr;
   DevArena arena;
   std::vector<Geom> geoms;
-  // inputs
-  double *pts = nullptr, *nrm = nullptr, *unit = nullptr;
+  // inputs; with MPR_STAGE_FILTER the kept rows' normals are compacted into nrm_kept
+  double *pts = nullptr, *nrm = nullptr, *unit = nullptr, *nrm_kept = nullptr;
+  FilterScratch filt;
   // natural-layout grids
   double *density = nullptr, *screen = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr,
          *rhs = nullptr, *field = nullptr;
@@ -172,6 +175,7 @@ struct Job {
   // octree (allocated on demand)
   DevArena oct;
   int64_t oct_nodes = 0;
+  uint32_t oct_points = 0;  // points the tree was built over (point_leaf entries)
   int oct_depth = -1;
   int64_t *o_depth = nullptr, *o_morton = nullptr, *o_parent = nullptr, *o_cstart = nullptr,
           *o_ccount = nullptr, *o_leaf = nullptr;
@@ -189,7 +193,15 @@ struct Job {
   }
   uint64_t live_words() const { return (2ull * geoms[0].C + 31) / 32; }
   const double* unit_points(const mpr_params& p) const {
-    return (p.stages & MPR_STAGE_NORMALIZE) ? unit : pts;
+    return (p.stages & (MPR_STAGE_NORMALIZE | MPR_STAGE_FILTER)) ? unit : pts;
+  }
+  const double* normals(const mpr_params& p) const {
+    return (p.stages & MPR_STAGE_FILTER) ? nrm_kept : nrm;
+  }
+  // rows the splat and the iso value see: all P, of which the first host->valid are kept
+  // (the rest zero rows) with MPR_STAGE_FILTER
+  const uint32_t* kept_count(const mpr_params& p) const {
+    return (p.stages & MPR_STAGE_FILTER) ? &d_scalars->valid : nullptr;
   }
   // pipeline stages; all enqueue on `s` without host synchronisation
   void setup(const mpr_params& p, hipStream_t s);
@@ -281,6 +293,7 @@ void Job::result(mpr_result* r) const {
   r->residual_norm = host->resnorm;
   for (int a = 0; a < 3; ++a) r->origin[a] = host->params[a];
   r->width = host->params[3];
+  r->valid_rows = (last.stages & MPR_STAGE_FILTER) ? (int64_t)host->valid : (int64_t)P;
 }
 
 // A fixed set of jobs whose V-cycles run together: one launch per V-cycle step and grid
@@ -462,6 +475,14 @@ void Job::plan(DevArena& a) {
   pts = a.alloc<double>(P3);
   nrm = a.alloc<double>(P3);
   unit = a.alloc<double>(P3);
+  nrm_kept = a.alloc<double>(P3);
+  {
+    const uint32_t fb = filter_blocks(std::max<uint32_t>(1u, max_points));
+    filt.counts = a.alloc<uint32_t>(fb);
+    filt.offsets = a.alloc<uint32_t>(fb);
+    filt.scan = a.alloc<uint32_t>(scan_scratch_words(fb));
+    filt.part = a.alloc<double>(6ull * fb);
+  }
   density = a.alloc<double>(M);
   screen = a.alloc<double>(M);
   vx = a.alloc<double>(M);
@@ -580,15 +601,19 @@ void Job::enqueue_extract(bool world, hipStream_t s) {
 // normalise, splat, assemble, screening pyramid and initial guess (core.py:229-273)
 void Job::setup(const mpr_params& p, hipStream_t s) {
   const uint64_t M = (uint64_t)n * n * n;
-  if (p.stages & MPR_STAGE_NORMALIZE) launch_normalize(pts, P, p.scale, bbox_part, d_params, unit, s);
+  if (p.stages & MPR_STAGE_FILTER)
+    launch_filter_normalize(pts, nrm, P, p.scale, filt, &d_scalars->valid, d_params, unit,
+                            nrm_kept, s);
+  else if (p.stages & MPR_STAGE_NORMALIZE)
+    launch_normalize(pts, P, p.scale, bbox_part, d_params, unit, s);
   if (!(p.stages & MPR_STAGE_SOLVE)) return;
   const double* upts = unit_points(p);
   // the splat grids are not zero-filled: the splat marks the nodes it writes and the
   // assembly and the extraction read every other node as zero
   launch_fill<uint32_t>(touched, 0u, (M + 31) / 32, s);
   launch_fill<unsigned long long>(d_accepted, 0ull, 1, s);
-  launch_splat(upts, nrm, P, n, p.confidence, density, vx, vy, vz, screen, p.samples_per_node,
-               d_accepted, splat, touched, s);
+  launch_splat(upts, normals(p), P, n, p.confidence, density, vx, vy, vz, screen,
+               p.samples_per_node, d_accepted, splat, touched, s);
   const bool grids = (p.stages & MPR_STAGE_GRIDS) != 0;
   const bool mixed = precision == MPR_PRECISION_MIXED;
   // Without MPR_STAGE_GRIDS the assembly writes only the nodes near this run's and the
@@ -627,8 +652,8 @@ void Job::finish(const mpr_params& p, hipStream_t s) {
   launch_norm_finish(norm_part, norm_nparts, d_resnorm, s);
   if (p.stages & MPR_STAGE_GRIDS) launch_split_to_nat<double>(L64[0].x, field, geoms[0], s);
   if (P)
-    launch_iso(split_field(L64[0].x, geoms[0]), unit_points(p), nrm, P, n, p.confidence,
-               iso_part, d_iso, s);
+    launch_iso(split_field(L64[0].x, geoms[0]), unit_points(p), normals(p), P, n, p.confidence,
+               iso_part, d_iso, s, kept_count(p));
   else
     launch_fill<double>(d_iso, 0.0, 1, s);
 }
@@ -818,6 +843,24 @@ int mpr_job_device_points(mpr_job* job, double** points, double** normals) {
   return 0;
 }
 
+int mpr_job_fetch_filtered(mpr_job* job, int64_t* count, double* unit_points, double* normals) {
+  if (!job || !job->ran || !(job->last.stages & MPR_STAGE_FILTER)) {
+    set_error("the job's last run had no MPR_STAGE_FILTER");
+    return -1;
+  }
+  MPR_GUARD_BEGIN
+  MPR_HIP(hipStreamSynchronize(job->stream));
+  const uint64_t v = job->host->valid;
+  if (count) *count = (int64_t)v;
+  if (unit_points && v)
+    MPR_HIP(hipMemcpyAsync(unit_points, job->unit, 24 * v, hipMemcpyDeviceToHost, job->stream));
+  if (normals && v)
+    MPR_HIP(hipMemcpyAsync(normals, job->nrm_kept, 24 * v, hipMemcpyDeviceToHost, job->stream));
+  MPR_HIP(hipStreamSynchronize(job->stream));
+  return 0;
+  MPR_GUARD_END
+}
+
 int mpr_job_set_count(mpr_job* job, int64_t P) {
   if (!job || P < 0 || (uint64_t)P > job->max_points) return -1;
   job->P = (uint32_t)P;
@@ -997,8 +1040,15 @@ int mpr_job_build_octree(mpr_job* job, int32_t depth, int64_t* node_count) {
   }
   MPR_GUARD_BEGIN
   hipStream_t s = job->stream;
-  const uint32_t P = job->P;
-  const double* upts = (job->ran && (job->last.stages & MPR_STAGE_NORMALIZE)) ? job->unit : job->pts;
+  // after a filtered run the tree covers the kept rows (their count is read back with the
+  // run's scalars, so the run must have finished)
+  const bool filtered = job->ran && (job->last.stages & MPR_STAGE_FILTER);
+  if (filtered) MPR_HIP(hipStreamSynchronize(s));
+  const uint32_t P = filtered ? job->host->valid : job->P;
+  const double* upts =
+      (job->ran && (job->last.stages & (MPR_STAGE_NORMALIZE | MPR_STAGE_FILTER))) ? job->unit
+                                                                                   : job->pts;
+  job->oct_points = P;
   job->oct.release();
   DevArena& a = job->oct;
   if (P == 0) {
@@ -1060,7 +1110,7 @@ int mpr_job_fetch_octree(mpr_job* job, int64_t* depth, int64_t* morton, int64_t*
     return -1;
   }
   MPR_GUARD_BEGIN
-  if (job->P == 0) {  // core.py:94-97
+  if (job->oct_points == 0) {  // core.py:94-97
     if (depth) depth[0] = 0;
     if (morton) morton[0] = 0;
     if (parent) parent[0] = -1;
@@ -1076,7 +1126,8 @@ int mpr_job_fetch_octree(mpr_job* job, int64_t* depth, int64_t* morton, int64_t*
   if (child_start) MPR_HIP(hipMemcpyAsync(child_start, job->o_cstart, b, hipMemcpyDeviceToHost, s));
   if (child_count) MPR_HIP(hipMemcpyAsync(child_count, job->o_ccount, b, hipMemcpyDeviceToHost, s));
   if (point_leaf)
-    MPR_HIP(hipMemcpyAsync(point_leaf, job->o_leaf, 8ull * job->P, hipMemcpyDeviceToHost, s));
+    MPR_HIP(hipMemcpyAsync(point_leaf, job->o_leaf, 8ull * job->oct_points, hipMemcpyDeviceToHost,
+                           s));
   MPR_HIP(hipStreamSynchronize(s));
   return 0;
   MPR_GUARD_END

@@ -50,6 +50,7 @@ class mpr_result(ctypes.Structure):
         ("residual_norm", ctypes.c_double),
         ("origin", ctypes.c_double * 3),
         ("width", ctypes.c_double),
+        ("valid_rows", ctypes.c_int64),
     ]
 
 
@@ -60,6 +61,7 @@ STAGE_SOLVE = 2
 STAGE_EXTRACT = 4
 STAGE_GRIDS = 8
 STAGE_WORLD = 16
+STAGE_FILTER = 32
 
 # the reference's kernel table (_lib.py:20-32), unchanged
 _SIGNATURES = {
@@ -101,6 +103,7 @@ _PIPELINE = {
     "mpr_job_launch": ([P], ctypes.c_int),
     "mpr_job_device_points": ([P, _PP, _PP], ctypes.c_int),
     "mpr_job_set_count": ([P, I], ctypes.c_int),
+    "mpr_job_fetch_filtered": ([P, _IP, P, P], ctypes.c_int),
     "mpr_job_time_kernel": ([P, S, S, ctypes.POINTER(F), ctypes.POINTER(F)], ctypes.c_int),
     "mpr_autotune": ([S, S, P, P, P], ctypes.c_int),
     "mpr_tune_get": ([S, S, S], ctypes.c_int),

@@ -374,18 +374,6 @@ def extract_isosurface(field, *, iso_value: float = 0.0, density=None) -> Mesh:
     return Mesh(vertices, faces, vertex_density, iso_value)
 
 
-def _normalize(points: np.ndarray, scale: float) -> tuple[np.ndarray, np.ndarray, float]:
-    """core.py:377-385 (host restatement; the pipeline does the same on the device)."""
-    minimum = points.min(axis=0)
-    maximum = points.max(axis=0)
-    center = 0.5 * (minimum + maximum)
-    width = float(np.max(maximum - minimum)) * scale
-    if not np.isfinite(width) or width <= 0:
-        width = 1.0
-    origin = center - 0.5 * width
-    return np.ascontiguousarray((points - origin) / width), origin, width
-
-
 def reconstruct(
     points,
     normals,
@@ -403,7 +391,7 @@ def reconstruct(
                               confidence=confidence, cycles=cycles, scale=scale,
                               samples_per_node=samples_per_node, precision=precision)
     if req is None:
-        return Mesh(np.empty((0, 3), np.float64), np.empty((0, 3), np.int64), np.empty(0))
+        return _empty_mesh()
     with _jobs.lend(req.depth, len(req.points), req.precision) as job:
         job.set_points(req.points, req.normals)
         job.run(req.params)
@@ -412,7 +400,8 @@ def reconstruct(
 
 @dataclass(frozen=True)
 class ReconstructRequest:
-    """A validated reconstruct() call: filtered rows, depth, precision and device params."""
+    """A validated reconstruct() call: its rows (filtered on the device), depth, precision
+    and device params."""
     points: np.ndarray
     normals: np.ndarray
     depth: int
@@ -423,7 +412,10 @@ class ReconstructRequest:
 def prepare_reconstruct(points, normals, *, depth=6, point_weight=4.0, confidence=False,
                         cycles=4, scale=1.1, samples_per_node=1.5, precision="fp64"):
     """Validation of reconstruct() (core.py:400-411 and the solver checks of
-    core.py:211-220), in the reference's order. Returns None when no row survives."""
+    core.py:211-220), in the reference's order. The row filter of core.py:407-409 runs on the
+    device (MPR_STAGE_FILTER), fused with the normalisation. Returns None when there is no
+    row at all (or, as in the reference, when no row survives the filter and the solver
+    parameters were never looked at)."""
     points = _points(points)
     normals = _points(normals)
     if points.shape != normals.shape:
@@ -431,29 +423,47 @@ def prepare_reconstruct(points, normals, *, depth=6, point_weight=4.0, confidenc
     scale = _finite_parameter("scale", scale)
     if scale <= 1.0:
         raise ValueError("scale must be greater than 1")
-    points, normals = _filter(points, normals)
     if not len(points):
         return None
-    depth = _integer_parameter("depth", depth)
-    cycles = _integer_parameter("cycles", cycles)
-    point_weight = _finite_parameter("point_weight", point_weight)
-    samples_per_node = _finite_parameter("samples_per_node", samples_per_node)
-    mode = _precision(precision)
-    if not 2 <= depth <= MAX_DEPTH:
-        raise ValueError(f"depth must be between 2 and {MAX_DEPTH}")
-    if cycles < 1:
-        raise ValueError("cycles must be positive")
-    if point_weight < 0 or samples_per_node <= 0:
-        raise ValueError("point_weight must be non-negative and samples_per_node positive")
-    stages = (_lib.STAGE_NORMALIZE | _lib.STAGE_SOLVE | _lib.STAGE_EXTRACT | _lib.STAGE_WORLD)
+    try:
+        depth = _integer_parameter("depth", depth)
+        cycles = _integer_parameter("cycles", cycles)
+        point_weight = _finite_parameter("point_weight", point_weight)
+        samples_per_node = _finite_parameter("samples_per_node", samples_per_node)
+        mode = _precision(precision)
+        if not 2 <= depth <= MAX_DEPTH:
+            raise ValueError(f"depth must be between 2 and {MAX_DEPTH}")
+        if cycles < 1:
+            raise ValueError("cycles must be positive")
+        if point_weight < 0 or samples_per_node <= 0:
+            raise ValueError("point_weight must be non-negative and samples_per_node positive")
+    except (TypeError, ValueError):
+        # the reference validates these only once a row has survived its filter
+        # (core.py:407-421): with none left it returns an empty mesh instead of raising
+        if not _valid_oriented_rows(points, normals).any():
+            return None
+        raise
+    stages = (_lib.STAGE_FILTER | _lib.STAGE_NORMALIZE | _lib.STAGE_SOLVE | _lib.STAGE_EXTRACT
+              | _lib.STAGE_WORLD)
     params = Job.params(stages, point_weight, samples_per_node, scale, cycles, confidence)
     return ReconstructRequest(points, normals, depth, mode, params)
 
 
-def mesh_from_job(job: Job, result: _lib.mpr_result, count: int) -> Mesh:
-    """The world-space Mesh of a finished reconstruct run (core.py:422-426)."""
-    if result.accepted != count:
-        raise RuntimeError(f"GPU splat accepted {result.accepted} of {count} validated samples")
+def _empty_mesh() -> Mesh:
+    return Mesh(np.empty((0, 3), np.float64), np.empty((0, 3), np.int64), np.empty(0))
+
+
+def mesh_from_job(job: Job, result: _lib.mpr_result, count: int | None = None) -> Mesh:
+    """The world-space Mesh of a finished reconstruct run (core.py:410-411, 422-426). The
+    splat must have accepted every row the device filter kept (`count`, when given, is the
+    number of rows uploaded)."""
+    kept = int(result.valid_rows)
+    if count is not None and kept > count:
+        raise RuntimeError(f"GPU filter kept {kept} of {count} rows")
+    if result.accepted != kept:
+        raise RuntimeError(f"GPU splat accepted {result.accepted} of {kept} validated samples")
+    if kept == 0:  # core.py:410-411: no valid row
+        return _empty_mesh()
     vertices, faces, density = job.fetch_mesh(result)
     if not len(vertices):
         return Mesh(np.empty((0, 3), np.float64), np.empty((0, 3), np.int64), np.empty(0),

@@ -157,13 +157,6 @@ def test_row_filter_matches_reference_restatement_on_adversarial_rows():
     assert mpr.core._valid_oriented_rows(clean, clean).all()
 
 
-def test_host_normalisation_matches_reference_restatement():
-    p = np.random.default_rng(1).normal(size=(1000, 3)) * [3.0, 1.0, 0.5] + 7.0
-    a = mpr.core._normalize(p, 1.1)
-    b = oracle.normalize(p, 1.1)
-    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
-
-
 def test_server_job_capacity_rounds_to_powers_of_two():
     from mojo_poissonrecon.serve import job_capacity
     assert job_capacity(0) == 16384 and job_capacity(3000) == 16384

@@ -520,3 +520,90 @@ def test_host_mesh_output_equals_device_fetch_and_round_trips_as_ply(tmp_path):
     assert np.array_equal(rd, mh.density)
     dev.close()
     host.close()
+
+
+def adversarial_rows(count=20000, seed=0):
+    # core.py:73-83 with huge, tiny, zero, infinite and NaN components
+    rng = np.random.default_rng(seed)
+    P = rng.normal(size=(count, 3)) * 10.0 ** rng.integers(-300, 300, size=(count, 1))
+    N = rng.normal(size=(count, 3)) * 10.0 ** rng.integers(-320, 308, size=(count, 1))
+    N[::7] = 0
+    N[1::11, 0] = np.inf
+    P[2::13, 1] = np.nan
+    N[3::17] = [0, 0, 5e-324]
+    N[4::19] = [1.7e308, -1.7e308, 1e308]
+    N[5::23] = [-0.0, 0.0, -0.0]
+    P[6::29, 2] = -np.inf
+    return P, N
+
+
+def test_device_row_filter_and_normalisation_equal_the_reference_restatement():
+    import ctypes
+
+    from mojo_poissonrecon import _lib
+    from mojo_poissonrecon.core import Job
+
+    P, N = adversarial_rows()
+    with np.errstate(all="ignore"):
+        mask = oracle.valid_oriented_rows(P, N)
+        unit, origin, width = oracle.normalize(P[mask], 1.1)
+    job = Job(5, len(P), _lib.PRECISION_FP64)
+    job.set_points(P, N)
+    job.run(Job.params(_lib.STAGE_FILTER | _lib.STAGE_NORMALIZE))
+    r = job.wait()
+    count = ctypes.c_int64()
+    got_unit, got_n = np.zeros_like(P), np.zeros_like(N)
+    _lib.check(_lib.lib().mpr_job_fetch_filtered(job.handle, ctypes.byref(count),
+                                                 got_unit.ctypes.data, got_n.ctypes.data),
+               "mpr_job_fetch_filtered")
+    assert count.value == mask.sum() == r.valid_rows
+    assert np.array_equal(got_n[:count.value], N[mask])  # kept rows, in order, bit for bit
+    assert np.array_equal(got_unit[:count.value], unit)
+    assert list(r.origin) == list(origin) and r.width == width
+    # a cloud whose every row is dropped: the identity transform and no kept row
+    job.set_points(P[~mask][:500], N[~mask][:500])
+    job.run(Job.params(_lib.STAGE_FILTER | _lib.STAGE_NORMALIZE))
+    r = job.wait()
+    assert r.valid_rows == 0 and list(r.origin) == [0.0, 0.0, 0.0] and r.width == 1.0
+    job.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_reconstruct_with_invalid_rows_equals_reconstruct_of_the_kept_rows(precision):
+    from mojo_poissonrecon.batch import reconstruct_batch
+
+    p, n = shapes.torus_workload(40_000, seed=9)
+    rng = np.random.default_rng(2)
+    bad_p = rng.uniform(-1, 1, (6000, 3))
+    bad_n = rng.normal(size=(6000, 3))
+    bad_p[0::6, 1] = np.nan
+    bad_n[1::6, 0] = np.inf
+    bad_n[2::6] = 0.0
+    bad_n[3::6] = [-0.0, 0.0, -0.0]
+    bad_p[4::6, 2] = -np.inf
+    bad_n[5::6] = [0.0, 0.0, 5e-324]  # kept: a subnormal but non-zero normal
+    order = rng.permutation(len(p) + len(bad_p))
+    allp = np.concatenate([p, bad_p])[order]
+    alln = np.concatenate([n, bad_n])[order]
+    with np.errstate(all="ignore"):
+        keep = oracle.valid_oriented_rows(allp, alln)
+    assert keep.sum() == len(p) + 1000
+    want = mpr.reconstruct(np.ascontiguousarray(allp[keep]), np.ascontiguousarray(alln[keep]),
+                           depth=7, confidence=True, precision=precision)
+    got = mpr.reconstruct(allp, alln, depth=7, confidence=True, precision=precision)
+    for a, b in ((got, want),):
+        assert a.iso_value == b.iso_value and a.residual_norm == b.residual_norm
+        assert np.array_equal(a.vertices, b.vertices) and np.array_equal(a.faces, b.faces)
+        assert np.array_equal(a.density, b.density)
+    # the same inside a batch, beside a cloud with no valid row
+    none_p, none_n = rng.uniform(-1, 1, (300, 3)), np.zeros((300, 3))
+    batched = reconstruct_batch([(allp, alln), (none_p, none_n)], depths=[7, 6], confidence=True,
+                                precision=precision)
+    assert np.array_equal(batched[0].faces, want.faces)
+    assert np.array_equal(batched[0].vertices, want.vertices)
+    assert batched[1].is_empty and batched[1].iso_value == 0.0
+    # no valid row: an empty mesh, before the solver parameters are looked at (core.py:410-411)
+    assert mpr.reconstruct(none_p, none_n, depth=6).is_empty
+    assert mpr.reconstruct(none_p, none_n, depth=99).is_empty
+    with pytest.raises(ValueError, match="depth"):
+        mpr.reconstruct(p, n, depth=99)
