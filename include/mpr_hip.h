@@ -285,6 +285,16 @@ int mpr_iso_value_f64_noalloc(const double *field, const double *points, const d
                               int64_t point_count, int64_t n, int64_t confidence,
                               double *iso_value, void *stream);
 
+/* Referenced vertices and area-weighted unit normals of a triangle mesh (core.py:429-450,
+ * oriented_points_from_mesh), on the GPU with the arithmetic of the reference's per-face
+ * loop (results bit-identical): host vertices (V,3) float64 and faces (F,3) int64 in;
+ * points_out and normals_out ((V,3) float64 each, caller-sized for all V) receive the kept
+ * vertices in order and *count_out their number. Returns 0, or -1 (face index out of range,
+ * device error) with mpr_last_error() set. */
+int mpr_oriented_points_from_mesh(const double *vertices, int64_t vertex_count,
+                                  const int64_t *faces, int64_t face_count, double *points_out,
+                                  double *normals_out, int64_t *count_out);
+
 /* Distances between two triangle meshes (host buffers: vertices (V,3) float64, faces (F,3)
  * int64), the accuracy metric of the reference's end-to-end tests
  * (tests/test_reconstruction.py:50-62), computed exactly (point to triangle) on the GPU:

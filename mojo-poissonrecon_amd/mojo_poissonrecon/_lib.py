@@ -129,6 +129,7 @@ _PIPELINE = {
     "mpr_mesh_destroy": ([P], ctypes.c_int),
     "mpr_iso_value_f64_noalloc": ([P, P, P, I, I, I, P, P], ctypes.c_int),
     "mpr_mesh_distance": ([P, I, P, I, P, I, P, I, P], ctypes.c_int),
+    "mpr_oriented_points_from_mesh": ([P, I, P, I, P, P, _IP], ctypes.c_int),
 }
 
 

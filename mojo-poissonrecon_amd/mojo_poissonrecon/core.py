@@ -474,8 +474,9 @@ def mesh_from_job(job: Job, result: _lib.mpr_result, count: int | None = None) -
 def oriented_points_from_mesh(vertices, faces) -> tuple[np.ndarray, np.ndarray]:
     """Referenced vertices and area-weighted normals of a triangle mesh (core.py:429-450).
 
-    Same arithmetic as the reference's per-face loop: unnormalised cross products added to
-    each corner in face order (np.add.at is unbuffered and ordered), then normalised.
+    Computed on the GPU (mpr_oriented_points_from_mesh) with the arithmetic of the
+    reference's per-face loop: unnormalised cross products added to each corner in face
+    order, then normalised, so the result is the same bit for bit.
     """
     vertices = _points(vertices)
     faces = i64(faces)
@@ -483,20 +484,15 @@ def oriented_points_from_mesh(vertices, faces) -> tuple[np.ndarray, np.ndarray]:
         raise ValueError("faces must have shape (m, 3)")
     if len(faces) and (faces.min() < 0 or faces.max() >= len(vertices)):
         raise ValueError("face index out of range")
-    normals = np.zeros_like(vertices)
-    referenced = np.zeros(len(vertices), dtype=bool)
-    if len(faces):
-        a, b, c = faces[:, 0], faces[:, 1], faces[:, 2]
-        cross = np.cross(vertices[b] - vertices[a], vertices[c] - vertices[a])
-        ok = np.isfinite(cross).all(axis=1) & (np.einsum("ij,ij->i", cross, cross) > 0)
-        corners = np.stack([a[ok], b[ok], c[ok]], axis=1).reshape(-1)
-        contrib = np.repeat(cross[ok], 3, axis=0)
-        np.add.at(normals, corners, contrib)
-        referenced[corners] = True
-    lengths = np.linalg.norm(normals, axis=1)
-    keep = referenced & (lengths > 0) & np.isfinite(lengths)
-    normals[keep] /= lengths[keep, None]
-    return np.ascontiguousarray(vertices[keep]), np.ascontiguousarray(normals[keep])
+    points = np.empty_like(vertices)
+    normals = np.empty_like(vertices)
+    count = ctypes.c_int64()
+    check(lib().mpr_oriented_points_from_mesh(ptr(vertices), len(vertices), ptr(faces),
+                                              len(faces), ptr(points), ptr(normals),
+                                              ctypes.byref(count)),
+          "mpr_oriented_points_from_mesh")
+    k = int(count.value)
+    return np.ascontiguousarray(points[:k]), np.ascontiguousarray(normals[:k])
 
 
 def reconstruct_mesh(vertices, faces, **kwargs) -> Mesh:

@@ -43,6 +43,7 @@ _SIGNATURES = {
     "oracle_dual_vertices": ([_P, _P, _I, _F, _P, _P, _P, _I], _I),
     "oracle_dual_faces": ([_P, _I, _F, _P, _P, _I], _I),
     "oracle_solve": ([_P, _P, _I, _I, _F, _I, _I, _F, _P, _P, _P, _P, _P, _P], _I),
+    "oracle_oriented_points": ([_P, _I, _P, _I, _P, _P], _I),
 }
 
 
@@ -410,6 +411,16 @@ def reconstruct(points, normals, depth: int = 6, point_weight: float = 4.0,
     return dict(vertices=np.ascontiguousarray(v * width + origin), faces=f, density=d,
                 iso_value=sol["iso_value"], residual_norm=sol["residual_norm"], solution=sol,
                 origin=origin, width=width)
+
+
+def oriented_points_from_mesh_c(vertices, faces):
+    """core.py:429-450 as the C per-face loop (mpr_oracle.c), for large meshes."""
+    vertices = _f(vertices).reshape(-1, 3)
+    faces = np.ascontiguousarray(faces, dtype=np.int64).reshape(-1, 3)
+    p, n = np.empty_like(vertices), np.empty_like(vertices)
+    k = lib().oracle_oriented_points(_p(vertices), len(vertices), _p(faces), len(faces),
+                                     _p(p), _p(n))
+    return np.ascontiguousarray(p[:k]), np.ascontiguousarray(n[:k])
 
 
 def oriented_points_from_mesh(vertices, faces):

@@ -470,6 +470,50 @@ int64_t oracle_dual_faces(const double *field, int64_t n, double iso, const int6
     return fc;
 }
 
+/* core.py:429-450: per-face loop. cross = (b - a) x (c - a) as numpy.cross computes it;
+ * a face counts when its cross product is finite with a positive squared norm (np.dot);
+ * its cross product is added to its three corners in face order; a vertex is kept when it is
+ * referenced and the norm of its sum (numpy.linalg.norm: sqrt((x^2 + y^2) + z^2)) is finite
+ * and positive, and its normal is the sum divided by that norm. Kept vertices keep their
+ * order. Returns their count. */
+int64_t oracle_oriented_points(const double *vertices, int64_t vertex_count, const int64_t *faces,
+                               int64_t face_count, double *points_out, double *normals_out) {
+    double *sum = calloc((size_t)(3 * vertex_count + 1), sizeof(double));
+    unsigned char *referenced = calloc((size_t)(vertex_count + 1), 1);
+    for (int64_t f = 0; f < face_count; ++f) {
+        int64_t a = faces[3 * f], b = faces[3 * f + 1], c = faces[3 * f + 2];
+        double e1[3], e2[3], cr[3];
+        for (int k = 0; k < 3; ++k) {
+            e1[k] = vertices[3 * b + k] - vertices[3 * a + k];
+            e2[k] = vertices[3 * c + k] - vertices[3 * a + k];
+        }
+        cr[0] = e1[1] * e2[2] - e1[2] * e2[1];
+        cr[1] = e1[2] * e2[0] - e1[0] * e2[2];
+        cr[2] = e1[0] * e2[1] - e1[1] * e2[0];
+        if (!(isfinite(cr[0]) && isfinite(cr[1]) && isfinite(cr[2]))) continue;
+        if (!(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2] > 0.0)) continue;
+        int64_t corner[3] = {a, b, c};
+        for (int j = 0; j < 3; ++j) {
+            for (int k = 0; k < 3; ++k) sum[3 * corner[j] + k] += cr[k];
+            referenced[corner[j]] = 1;
+        }
+    }
+    int64_t kept = 0;
+    for (int64_t i = 0; i < vertex_count; ++i) {
+        double x = sum[3 * i], y = sum[3 * i + 1], z = sum[3 * i + 2];
+        double len = sqrt((x * x + y * y) + z * z);
+        if (!referenced[i] || !(len > 0.0) || !isfinite(len)) continue;
+        for (int k = 0; k < 3; ++k) {
+            points_out[3 * kept + k] = vertices[3 * i + k];
+            normals_out[3 * kept + k] = sum[3 * i + k] / len;
+        }
+        kept += 1;
+    }
+    free(sum);
+    free(referenced);
+    return kept;
+}
+
 /* core.py:141-193: correction-scheme V-cycle, coarsest (n <= 3) gets 40 iterations. */
 static void v_cycle(int level, int levels, double **sol, double **rhs, double **scr, double **res,
                     const int64_t *ns, double pw, int pre, int post) {

@@ -39,6 +39,10 @@ int64_t oracle_dual_vertices(const double *field, const double *density, int64_t
 int64_t oracle_dual_faces(const double *field, int64_t n, double iso, const int64_t *cell_map,
                           int64_t *faces, int64_t capacity);
 
+/* core.py:429-450: referenced vertices and area-weighted normals (the per-face loop). */
+int64_t oracle_oriented_points(const double *vertices, int64_t vertex_count, const int64_t *faces,
+                               int64_t face_count, double *points_out, double *normals_out);
+
 /* core.py:196-306 orchestration after validation and the octree: splat, density/spn,
  * assemble, screen pyramid, `cycles` V-cycles, final residual and iso value.
  * Outputs are (n,n,n) with n = 2^depth + 1; returns the splat's accepted count. */

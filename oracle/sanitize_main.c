@@ -80,6 +80,13 @@ int main(void) {
     free(x); free(b); free(s); free(r); free(c); free(vx); free(vy); free(vz);
   }
 
+  /* mesh normals of the extracted mesh (core.py:429-450) */
+  double *opts = malloc((size_t)(3 * (nv > 0 ? nv : 1)) * sizeof(double));
+  double *onrm = malloc((size_t)(3 * (nv > 0 ? nv : 1)) * sizeof(double));
+  const int64_t kept = oracle_oriented_points(verts, nv, faces, nf, opts, onrm);
+  checksum += (double)kept;
+  free(opts); free(onrm);
+
   printf("sanitize ok: accepted %lld, iso %.6f (%.6f), residual %.3f, vertices %lld/%lld, "
          "faces %lld/%lld, checksum %.6g\n",
          (long long)accepted, iso, iso2, res, (long long)nv, (long long)nv2, (long long)nf,

@@ -107,35 +107,49 @@ UNREFERENCED = (np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [9.
 @pytest.mark.parametrize("vertices,faces,expected",
                          [EMPTY, SINGLE_TRIANGLE, DUPLICATE_VERTICES, ZERO_AREA, NON_MANIFOLD,
                           UNREFERENCED])
-def test_degenerate_mesh_preprocessing(vertices, faces, expected):
-    # reference tests/test_reconstruction.py:86-144
-    points, normals = mpr.oriented_points_from_mesh(vertices, faces)
+def test_oracle_mesh_preprocessing_on_the_reference_fixtures(vertices, faces, expected):
+    # reference tests/test_reconstruction.py:86-144: the oracle's per-face loop, in Python and
+    # in C, keeps the expected vertices with unit normals (the GPU form is checked against
+    # both in tests/test_gpu_mesh_normals.py)
+    points, normals = oracle.oriented_points_from_mesh(vertices, faces)
     assert len(points) == expected
     assert points.shape == normals.shape
     assert np.isfinite(normals).all()
     if expected:
         assert np.allclose(np.linalg.norm(normals, axis=1), 1.0)
-    want_p, want_n = oracle.oriented_points_from_mesh(vertices, faces)
-    assert np.array_equal(points, want_p) and np.array_equal(normals, want_n)
+    cp, cn = oracle.oriented_points_from_mesh_c(vertices, faces)
+    assert np.array_equal(points, cp) and np.array_equal(normals, cn)
 
 
-def test_mesh_normals_are_area_weighted():
+def test_oracle_mesh_normals_are_area_weighted():
     # reference tests/test_reconstruction.py:171-179
     vertices = np.array([[0, 0, 0], [2, 0, 0], [0, 2, 0], [0, 0, 1]], dtype=np.float64)
     faces = np.array([[0, 1, 2], [0, 3, 1]], dtype=np.int64)
-    points, normals = mpr.oriented_points_from_mesh(vertices, faces)
-    origin = np.flatnonzero(np.all(points == vertices[0], axis=1))[0]
-    assert np.allclose(normals[origin], [0.0, 1 / np.sqrt(5), 2 / np.sqrt(5)])
+    for points, normals in (oracle.oriented_points_from_mesh(vertices, faces),
+                            oracle.oriented_points_from_mesh_c(vertices, faces)):
+        origin = np.flatnonzero(np.all(points == vertices[0], axis=1))[0]
+        assert np.allclose(normals[origin], [0.0, 1 / np.sqrt(5), 2 / np.sqrt(5)])
 
 
-def test_vectorised_mesh_normals_equal_the_per_face_loop_bitwise():
+def test_c_mesh_normals_loop_equals_the_python_loop_bitwise():
     rng = np.random.default_rng(5)
-    vertices = rng.normal(size=(300, 3))
-    faces = rng.integers(0, 300, size=(1000, 3))
+    vertices = rng.normal(size=(3000, 3)) * 10.0 ** rng.integers(-160, 160, size=(3000, 1))
+    faces = rng.integers(0, 3000, size=(10000, 3))
     faces[:10, 1] = faces[:10, 0]  # degenerate faces
-    p, n = mpr.oriented_points_from_mesh(vertices, faces)
-    wp, wn = oracle.oriented_points_from_mesh(vertices, faces)
+    with np.errstate(all="ignore"):
+        p, n = oracle.oriented_points_from_mesh_c(vertices, faces)
+        wp, wn = oracle.oriented_points_from_mesh(vertices, faces)
     assert np.array_equal(p, wp) and np.array_equal(n, wn)
+
+
+def test_mesh_validation_messages():
+    v = np.zeros((3, 3))
+    with pytest.raises(ValueError, match="shape"):
+        mpr.oriented_points_from_mesh(v, np.zeros((2, 4), np.int64))
+    with pytest.raises(ValueError, match="out of range"):
+        mpr.oriented_points_from_mesh(v, np.array([[0, 1, 3]]))
+    with pytest.raises(ValueError, match="finite integers"):
+        mpr.oriented_points_from_mesh(v, np.array([[0.5, 1, 2]]))
 
 
 def test_row_filter_matches_reference_restatement_on_adversarial_rows():
