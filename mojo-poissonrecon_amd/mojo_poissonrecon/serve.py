@@ -2,22 +2,28 @@
 
 Requests (oriented point clouds with a depth) are admitted while device memory and slots
 allow: an admitted request takes an idle device job of its depth from a pool (or a new
-one, if its HBM footprint fits the budget), uploads its points on that job's stream and
-enqueues the whole reconstruction without waiting. Requests of one depth admitted in the
-same tick run as one batch (one launch per V-cycle step for all of them). Work of
-different requests overlaps on the device; completed meshes are read back as soon as their
-stream finishes, which frees the slot and memory for the next arrivals. Latency is
-measured from a request's arrival to its mesh being on the host.
+one, if its HBM footprint fits the budget), uploads its raw rows on that job's stream (the
+row filter and normalisation run on the device) and enqueues the whole reconstruction
+without waiting. Requests of one depth admitted in the same tick run as one batch (one
+launch per V-cycle step for all of them); batches over the same pooled jobs are reused.
+Work of different requests overlaps on the device; each extraction copies its mesh into
+its job's pinned host buffers on the job's stream, so a finished request's mesh is read
+back with a host copy, which (like the uploads) runs on a thread pool beside the loop.
+Latency is measured from a request's arrival to its mesh being on the host.
 
-The FP64 single-request reconstruct() stays the reference path; each served result is
-bit-identical to that request's solo run (every stage is deterministic and an instance's
-arithmetic does not depend on its batch).
+Failures are isolated per flight (one batch, or one solo request): its jobs are discarded,
+the rest of the server keeps running, and each of its requests is run again alone as the
+plain FP64 reconstruct() (the reference path) or, if that fails too, completed with its
+error. Each served result is bit-identical to that request's solo run (every stage is
+deterministic and an instance's arithmetic does not depend on its batch).
 """
 
 from __future__ import annotations
 
 import itertools
 import time
+from collections import OrderedDict
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -36,9 +42,12 @@ class Request:
     params: _lib.mpr_params
     precision: int
     arrival: float
+    kwargs: dict = field(default_factory=dict)
     admitted: float = 0.0
     completed: float = 0.0
     mesh: Mesh | None = None
+    error: str | None = None   # set when the request failed, fallback included
+    fallback: bool = False     # served by the FP64 single-request path after a failure
 
     @property
     def latency(self) -> float:
@@ -55,7 +64,10 @@ class _Flight:
 @dataclass
 class ServerStats:
     completed: int = 0
+    failed: int = 0
+    fallbacks: int = 0
     batches: int = 0
+    batches_reused: int = 0
     peak_inflight: int = 0
     peak_bytes: int = 0
     jobs_created: int = 0
@@ -72,11 +84,26 @@ def job_capacity(points: int) -> int:
     return max(16384, 1 << max(0, int(points) - 1).bit_length())
 
 
+def _fp64_fallback(r: Request) -> Mesh:
+    from .core import reconstruct
+
+    kw = dict(r.kwargs)
+    kw["precision"] = "fp64"
+    return reconstruct(r.points, r.normals, depth=r.depth, **kw)
+
+
 class ReconstructionServer:
-    """Admits reconstruction requests by HBM footprint and overlaps them on the device."""
+    """Admits reconstruction requests by HBM footprint and overlaps them on the device.
+
+    ``job_factory(depth, capacity, precision)``, ``batch_factory(jobs)`` and
+    ``fallback(request) -> Mesh`` default to the device Job, Batch and the FP64 solo
+    reconstruct(); they can be replaced (tests use stand-ins to force failures).
+    """
 
     def __init__(self, *, hbm_budget: int | None = None, max_inflight: int = 12,
-                 max_batch: int = 8, precision: str = "mixed"):
+                 max_batch: int = 8, precision: str = "mixed", host_threads: int = 8,
+                 batch_cache: int = 8, job_factory=None, batch_factory=None,
+                 fallback=_fp64_fallback):
         self.precision = precision
         self.max_inflight = max_inflight
         self.max_batch = max_batch
@@ -86,11 +113,18 @@ class ReconstructionServer:
                        "mpr_device_info")
             hbm_budget = int(0.7 * total.value)
         self.hbm_budget = hbm_budget
-        self._pool: list[Job] = []        # idle jobs
+        self._job_factory = job_factory or (lambda d, c, p: Job(d, c, p))
+        self._batch_factory = batch_factory or Batch
+        self._fallback = fallback
+        self._pool: list = []             # jobs (idle or busy)
         self._busy: set[int] = set()      # ids of jobs in flight
         self._bytes = 0
         self._waiting: list[Request] = []
         self._flights: list[_Flight] = []
+        self._ready: list[Request] = []   # failed flights' requests, returned by the next step
+        self._batches: OrderedDict = OrderedDict()  # job ids -> Batch, least recent first
+        self._batch_cache = batch_cache
+        self._threads = ThreadPoolExecutor(max_workers=max(1, host_threads))
         self._ids = itertools.count()
         self.stats = ServerStats()
 
@@ -105,12 +139,13 @@ class ReconstructionServer:
         if req is None:
             return None
         r = Request(next(self._ids), req.points, req.normals, req.depth, req.params,
-                    req.precision, now)
+                    req.precision, now, kwargs={k: v for k, v in kwargs.items()
+                                                if k != "precision"})
         self._waiting.append(r)
         return r
 
     def step(self) -> list[Request]:
-        """Admit what fits, launch it, and collect finished requests (non-blocking)."""
+        """Collect finished requests, then admit and launch what fits (non-blocking)."""
         t0 = time.perf_counter()
         done = self._collect()
         t1 = time.perf_counter()
@@ -121,7 +156,7 @@ class ReconstructionServer:
 
     def drain(self) -> list[Request]:
         out = []
-        while self._waiting or self._flights:
+        while self._waiting or self._flights or self._ready:
             out += self.step()
             if self._flights and not out:
                 time.sleep(0)
@@ -129,25 +164,47 @@ class ReconstructionServer:
 
     def close(self) -> None:
         for f in self._flights:
-            if f.batch is not None:
-                f.batch.wait()
-            else:
-                for j in f.jobs:
-                    j.wait()
-        for f in self._flights:
-            if f.batch is not None:
-                f.batch.close()
+            try:
+                if f.batch is not None:
+                    f.batch.wait()
+                else:
+                    for j in f.jobs:
+                        j.wait()
+            except Exception:  # the jobs are closed below either way
+                pass
         self._flights.clear()
+        for b in self._batches.values():
+            b.close()
+        self._batches.clear()
         for j in self._pool:
             j.close()
         self._pool.clear()
+        self._busy.clear()
+        self._bytes = 0
+        self._threads.shutdown(wait=True)
 
     @property
     def inflight(self) -> int:
         return sum(len(f.requests) for f in self._flights)
 
     # ---- internals ------------------------------------------------------------------
-    def _take_job(self, r: Request) -> Job | None:
+    def _footprint(self, j) -> int:
+        return Job.footprint(j.depth, j.max_points, j.precision)
+
+    def _discard_job(self, job) -> None:
+        """Drop a job (evicted, or in an unknown state after a failure) and its batches."""
+        for key in [k for k in self._batches if id(job) in k]:
+            self._batches.pop(key).close()
+        if job in self._pool:
+            self._pool.remove(job)
+            self._bytes -= self._footprint(job)
+        self._busy.discard(id(job))
+        try:
+            job.close()
+        except Exception:
+            pass
+
+    def _take_job(self, r: Request):
         best = None
         for j in self._pool:
             if (id(j) not in self._busy and j.depth == r.depth and j.precision == r.precision
@@ -158,17 +215,14 @@ class ReconstructionServer:
             return best
         capacity = job_capacity(len(r.points))
         need = Job.footprint(r.depth, capacity, r.precision)
-        # evict idle jobs until the new one fits the budget
+        # evict idle jobs (largest first) until the new one fits the budget
         idle = [j for j in self._pool if id(j) not in self._busy]
-        idle.sort(key=lambda j: Job.footprint(j.depth, j.max_points, j.precision))
+        idle.sort(key=self._footprint)
         while self._bytes + need > self.hbm_budget and idle:
-            victim = idle.pop()
-            self._bytes -= Job.footprint(victim.depth, victim.max_points, victim.precision)
-            self._pool.remove(victim)
-            victim.close()
+            self._discard_job(idle.pop())
         if self._bytes + need > self.hbm_budget:
             return None
-        job = Job(r.depth, capacity, r.precision)
+        job = self._job_factory(r.depth, capacity, r.precision)
         self.stats.jobs_created += 1
         self._pool.append(job)
         self._bytes += need
@@ -178,28 +232,60 @@ class ReconstructionServer:
     def _admit(self) -> None:
         # one batch per depth: a batch completes as a whole, so shallow requests are not
         # held back by a deeper one admitted in the same tick
-        groups: dict[int, list[tuple[Request, Job]]] = {}
-        pending = 0
-        while self._waiting and self.inflight + pending < self.max_inflight:
+        admitted: list = []
+        while self._waiting and self.inflight + len(admitted) < self.max_inflight:
             r = self._waiting[0]
             job = self._take_job(r)
             if job is None:
                 break  # memory is full: wait for a completion (FIFO admission)
             self._waiting.pop(0)
             self._busy.add(id(job))
-            job.set_points(r.points, r.normals)
-            r.admitted = time.perf_counter()
+            admitted.append((r, job))
+        if not admitted:
+            return
+        # uploads on the thread pool (each one a host copy that releases the GIL)
+        uploads = list(self._threads.map(lambda rj: self._upload(*rj), admitted))
+        groups: dict[int, list] = {}
+        now = time.perf_counter()
+        for (r, job), err in zip(admitted, uploads):
+            r.admitted = now
+            if err is not None:
+                self._fail(_Flight([r], [job]), err)
+                continue
             group = groups.setdefault(r.depth, [])
             group.append((r, job))
-            pending += 1
             if len(group) == self.max_batch:
                 self._launch(group)
-                pending -= len(group)
                 groups[r.depth] = []
         for group in groups.values():
             if group:
                 self._launch(group)
         self.stats.peak_inflight = max(self.stats.peak_inflight, self.inflight)
+
+    @staticmethod
+    def _upload(r: Request, job) -> Exception | None:
+        try:
+            job.set_points(r.points, r.normals)
+            return None
+        except Exception as e:  # noqa: BLE001 - reported per request
+            return e
+
+    def _batch_for(self, jobs: list):
+        key = tuple(id(j) for j in jobs)
+        b = self._batches.get(key)
+        if b is not None:
+            self._batches.move_to_end(key)
+            self.stats.batches_reused += 1
+            return b
+        b = self._batch_factory(jobs)
+        self._batches[key] = b
+        if len(self._batches) > self._batch_cache:  # drop the least recent idle batches
+            flying = {id(f.batch) for f in self._flights}
+            for k in [k for k, v in self._batches.items() if id(v) not in flying and v is not b]:
+                if len(self._batches) <= self._batch_cache:
+                    break
+                self._batches.pop(k).close()
+        return b
 
     def _launch(self, group) -> None:
         reqs = [r for r, _ in group]
@@ -209,32 +295,83 @@ class ReconstructionServer:
             == (reqs[0].params.cycles, reqs[0].params.pre_smooth, reqs[0].params.post_smooth,
                 reqs[0].params.point_weight) for r in reqs)
         if len(group) > 1 and uniform:
-            batch = Batch(jobs)
-            batch.run([r.params for r in reqs])
-            self._flights.append(_Flight(reqs, jobs, batch))
+            flight = _Flight(reqs, jobs)
+            try:
+                flight.batch = self._batch_for(jobs)
+                flight.batch.run([r.params for r in reqs])
+            except Exception as e:  # noqa: BLE001
+                self._fail(flight, e)
+                return
+            self._flights.append(flight)
             self.stats.batches += 1
         else:
             for r, j in group:
-                j.run(r.params)
-                self._flights.append(_Flight([r], [j]))
+                flight = _Flight([r], [j])
+                try:
+                    j.run(r.params)
+                except Exception as e:  # noqa: BLE001
+                    self._fail(flight, e)
+                    continue
+                self._flights.append(flight)
+
+    def _fail(self, f: _Flight, error: Exception) -> None:
+        """A flight failed: discard its jobs (and batch) and serve its requests by the
+        fallback, one at a time; they are returned by the next step (completed, or with
+        `error` set)."""
+        if f.batch is not None:
+            for key in [k for k, b in self._batches.items() if b is f.batch]:
+                self._batches.pop(key)
+            try:
+                f.batch.close()
+            except Exception:
+                pass
+        for j in f.jobs:
+            self._discard_job(j)
+        for r in f.requests:
+            try:
+                if self._fallback is None:
+                    raise error
+                r.mesh = self._fallback(r)
+                r.fallback = True
+                self.stats.fallbacks += 1
+            except Exception as e:  # noqa: BLE001
+                r.error = f"{type(error).__name__}: {error}"
+                if e is not error:
+                    r.error += f"; fallback {type(e).__name__}: {e}"
+                r.mesh = None
+                self.stats.failed += 1
+            r.completed = time.perf_counter()
+            self.stats.completed += 1
+            self.stats.latencies.append(r.latency)
+            self._ready.append(r)
 
     def _collect(self) -> list[Request]:
-        done = []
-        still = []
+        still, finished = [], []
         for f in self._flights:
-            ready = f.batch.done() if f.batch is not None else f.jobs[0].done()
-            if not ready:
-                still.append(f)
+            try:
+                ready = f.batch.done() if f.batch is not None else f.jobs[0].done()
+            except Exception as e:  # noqa: BLE001
+                self._fail(f, e)
                 continue
-            results = f.batch.wait() if f.batch is not None else [f.jobs[0].wait()]
-            for r, j, res in zip(f.requests, f.jobs, results):
-                r.mesh = mesh_from_job(j, res, len(r.points))
-                r.completed = time.perf_counter()
+            (finished if ready else still).append(f)
+        self._flights = still
+        for f in finished:
+            try:
+                results = f.batch.wait() if f.batch is not None else [f.jobs[0].wait()]
+                # host copies of the meshes on the thread pool
+                meshes = list(self._threads.map(
+                    lambda t: mesh_from_job(t[1], t[2], len(t[0].points)),
+                    zip(f.requests, f.jobs, results)))
+            except Exception as e:  # noqa: BLE001
+                self._fail(f, e)
+                continue
+            now = time.perf_counter()
+            for r, j, m in zip(f.requests, f.jobs, meshes):
+                r.mesh = m
+                r.completed = now
                 self._busy.discard(id(j))
                 self.stats.completed += 1
                 self.stats.latencies.append(r.latency)
-                done.append(r)
-            if f.batch is not None:
-                f.batch.close()
-        self._flights = still
+                self._ready.append(r)
+        done, self._ready = self._ready, []
         return done

@@ -120,6 +120,32 @@ def uneven_batch_workload(count: int = 64, seed: int = 1):
     return clouds, depths
 
 
+def served_requests(count: int, qps: float, seed: int = 2, pool: int = 24):
+    """A served request stream: Poisson arrivals at `qps` (arrival times in seconds from 0),
+    each a cloud of 5k-500k points (log-uniform) on a random shape family, randomly rotated
+    and scaled into [-1, 1]^3, at a depth from {6, 7, 8, 9} weighted 4:3:2:1. The clouds are
+    drawn from `pool` pre-generated ones (rotated and scaled anew per request) to bound the
+    host work; yields (arrival, points, normals, depth) lazily."""
+    rng = np.random.default_rng(seed)
+    makers = (sphere, torus, superquadric)
+    base = []
+    for i in range(pool):
+        npts = int(round(math.exp(rng.uniform(math.log(5e3), math.log(5e5)))))
+        p, n = makers[i % 3](npts, seed=7000 + i)
+        base.append(perturb(p, n, seed=7000 + i, sigma=0.001, degrees=2.0))
+    arrivals = np.cumsum(rng.exponential(1.0 / qps, count))
+    depths = rng.choice([6, 7, 8, 9], size=count, p=[0.4, 0.3, 0.2, 0.1])
+    picks = rng.integers(0, pool, count)
+    for k in range(count):
+        p, n = base[picks[k]]
+        rot = random_rotation(rng)
+        scale = float(rng.uniform(0.5, 1.0))
+        pts = p @ rot.T
+        pts = pts / np.abs(pts).max() * scale
+        yield float(arrivals[k]), np.ascontiguousarray(pts), np.ascontiguousarray(n @ rot.T), \
+            int(depths[k])
+
+
 def random_rotation(rng) -> np.ndarray:
     q, r = np.linalg.qr(rng.normal(size=(3, 3)))
     q *= np.sign(np.diag(r))

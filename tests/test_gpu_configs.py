@@ -184,3 +184,40 @@ def test_uneven_64_batch_graph_equals_solo_runs_and_tracks_fp64():
         assert abs(len(m.vertices) - len(ref.vertices)) <= 2, i
     release_device_memory()
     print(f"worst mixed-vs-fp64 Hausdorff / diagonal: {worst:.3g}", flush=True)
+
+
+def test_served_stream_at_the_request_mix_equals_solo_runs():
+    # the served-stream workload's mix (5k-500k points, three shape families, random
+    # rotation and scale, depths 6:7:8:9 = 4:3:2:1, Poisson arrivals), shortened to 160
+    # requests; every eighth result is checked against its solo run (bit for bit) and its
+    # all-FP64 solve (within 1e-4 of the diagonal)
+    import time
+
+    from mojo_poissonrecon.serve import ReconstructionServer
+
+    stream = list(shapes.served_requests(160, qps=400.0, seed=3))
+    print("served stream, 160 requests", flush=True)
+    server = ReconstructionServer(max_inflight=16, max_batch=8, precision="mixed")
+    clouds = {}
+    done = []
+    start = time.perf_counter()
+    k = 0
+    while len(done) < len(stream):
+        now = time.perf_counter() - start
+        while k < len(stream) and stream[k][0] <= now:
+            arrival, p, n, d = stream[k]
+            r = server.submit(p, n, depth=d, arrival=start + arrival)
+            clouds[r.id] = (p, n, d)
+            k += 1
+        done += server.step()
+    server.close()
+    assert server.stats.batches >= 1 and server.stats.failed == 0
+    assert not any(r.fallback for r in done)
+    assert sorted(r.id for r in done) == sorted(clouds)
+    for r in sorted(done, key=lambda r: r.id)[::8]:
+        p, n, d = clouds[r.id]
+        identical_mesh(r.mesh, mpr.reconstruct(p, n, depth=d, precision="mixed"))
+        ref = mpr.reconstruct(p, n, depth=d, precision="fp64")
+        assert hausdorff(r.mesh, ref) <= 1e-4 * diag_of(p), (r.id, d, len(p))
+        assert r.latency > 0
+    release_device_memory()

@@ -1,0 +1,146 @@
+"""CPU tests of the serving loop's failure handling, with stand-in jobs and batches (no
+device): a flight whose batch fails at launch, at the completion query or at the wait is
+isolated - its jobs are discarded, its requests are served by the fallback (or completed
+with their error when the fallback fails too) - and the server keeps serving later
+requests, then closes cleanly."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from mojo_poissonrecon import _lib
+from mojo_poissonrecon.core import Mesh
+from mojo_poissonrecon.serve import ReconstructionServer
+
+
+class FakeJob:
+    def __init__(self, depth, capacity, precision):
+        self.depth, self.max_points, self.precision = depth, capacity, precision
+        self.count = 0
+        self.closed = False
+        self.runs = 0
+
+    def set_points(self, points, normals):
+        self.count = len(points)
+
+    def run(self, params):
+        self.runs += 1
+
+    def done(self):
+        return True
+
+    def wait(self):
+        r = _lib.mpr_result()
+        r.accepted = r.valid_rows = self.count
+        r.vertex_count, r.face_count = 3, 1
+        return r
+
+    def fetch_mesh(self, result):
+        return np.zeros((3, 3)), np.array([[0, 1, 2]]), np.zeros(3)
+
+    def close(self):
+        self.closed = True
+
+
+class FakeBatch:
+    fail_at = {}  # batch number -> "run" | "done" | "wait"
+    made = 0
+
+    def __init__(self, jobs):
+        self.jobs = jobs
+        self.number = FakeBatch.made
+        FakeBatch.made += 1
+        self.closed = False
+
+    def _maybe(self, where):
+        if FakeBatch.fail_at.get(self.number) == where:
+            raise _lib.DeviceError(f"batch {self.number} failed at {where}")
+
+    def run(self, params):
+        self._maybe("run")
+        for j in self.jobs:
+            j.runs += 1
+
+    def done(self):
+        self._maybe("done")
+        return True
+
+    def wait(self):
+        self._maybe("wait")
+        return [j.wait() for j in self.jobs]
+
+    def close(self):
+        self.closed = True
+
+
+def fallback_mesh(r):
+    return Mesh(np.ones((1, 3)), np.empty((0, 3), np.int64), np.ones(1), 0.25, 0.0)
+
+
+def make_server(fallback=fallback_mesh, jobs=None):
+    made = [] if jobs is None else jobs
+
+    def job_factory(d, c, p):
+        made.append(FakeJob(d, c, p))
+        return made[-1]
+
+    return ReconstructionServer(hbm_budget=1 << 50, max_inflight=6, max_batch=3,
+                                job_factory=job_factory, batch_factory=FakeBatch,
+                                fallback=fallback), made
+
+
+def cloud(n=100):
+    rng = np.random.default_rng(n)
+    return rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
+
+
+@pytest.mark.parametrize("where", ["run", "done", "wait"])
+def test_a_failed_batch_is_isolated_and_served_by_the_fallback(where):
+    FakeBatch.made = 0
+    FakeBatch.fail_at = {0: where}
+    server, jobs = make_server()
+    first = [server.submit(*cloud(), depth=6) for _ in range(3)]
+    done = server.step() + server.drain()
+    assert {r.id for r in done} == {r.id for r in first}
+    assert all(r.fallback and r.error is None and r.mesh.iso_value == 0.25 for r in done)
+    assert server.stats.fallbacks == 3 and server.stats.failed == 0
+    assert all(j.closed for j in jobs[:3])  # the failed flight's jobs are discarded
+    # the server keeps serving: a new batch on new jobs, not the fallback
+    later = [server.submit(*cloud(50 + k), depth=6) for k in range(3)]
+    done = server.drain()
+    assert {r.id for r in done} == {r.id for r in later}
+    assert not any(r.fallback for r in done) and all(r.mesh.faces.shape == (1, 3) for r in done)
+    assert not any(j.closed for j in jobs[3:])
+    server.close()
+    assert all(j.closed for j in jobs)
+
+
+def test_a_failed_fallback_completes_the_request_with_its_error():
+    FakeBatch.made = 0
+    FakeBatch.fail_at = {0: "wait"}
+
+    def broken(r):
+        raise RuntimeError("no device")
+
+    server, _ = make_server(fallback=broken)
+    for _ in range(3):
+        server.submit(*cloud(), depth=7)
+    done = server.drain()
+    assert len(done) == 3 and server.stats.failed == 3
+    assert all(r.mesh is None and "batch 0 failed at wait" in r.error and "no device" in r.error
+               for r in done)
+    server.close()
+
+
+def test_batches_over_the_same_jobs_are_reused():
+    FakeBatch.made = 0
+    FakeBatch.fail_at = {}
+    server, jobs = make_server()
+    for rnd in range(3):
+        for _ in range(3):
+            server.submit(*cloud(), depth=6)
+        assert len(server.drain()) == 3
+    assert len(jobs) == 3 and FakeBatch.made == 1
+    assert server.stats.batches == 3 and server.stats.batches_reused == 2
+    server.close()
