@@ -32,6 +32,7 @@ _F = ctypes.c_double
 
 _SIGNATURES = {
     "oracle_norm3": ([_F, _F, _F], _F),
+    "oracle_threads": ([], ctypes.c_int),
     "oracle_octree_codes": ([_P, _I, _I, _P], None),
     "oracle_splat": ([_P, _P, _I, _I, _I, _P, _P, _P, _P, _P, _F], _I),
     "oracle_assemble": ([_P, _P, _P, _P, _P, _I, _F, _F], None),
@@ -80,6 +81,25 @@ def lib() -> ctypes.CDLL:
     if _lib is None:
         _lib = _load(build())
     return _lib
+
+
+OMP_LIB = os.path.join(HERE, "liboracle_omp.so")
+_omp = None
+
+
+def openmp() -> ctypes.CDLL:
+    """The oracle built with -fopenmp (its plane loops on all host threads, results equal
+    to the serial build's), for timing the reference algorithm on every core."""
+    global _omp
+    if _omp is None:
+        if not os.path.exists(OMP_LIB) or os.path.getmtime(OMP_LIB) < os.path.getmtime(SRC):
+            tmp = OMP_LIB + ".tmp"
+            cmd = ["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-ffp-contract=off",
+                   "-fno-fast-math", "-fopenmp", SRC, "-o", tmp, "-lm"]
+            subprocess.run(cmd, check=True, capture_output=True, text=True)
+            os.replace(tmp, OMP_LIB)
+        _omp = _load(OMP_LIB)
+    return _omp
 
 
 def build_mutation(mutation: int, out_dir: str) -> ctypes.CDLL:
@@ -397,8 +417,8 @@ def normalize(points, scale: float):
 
 def reconstruct(points, normals, depth: int = 6, point_weight: float = 4.0,
                 confidence: bool = False, cycles: int = 4, scale: float = 1.1,
-                samples_per_node: float = 1.5) -> dict:
-    """core.py:388-426"""
+                samples_per_node: float = 1.5, handle=None) -> dict:
+    """core.py:388-426 (`handle`: the library whose solve runs, e.g. openmp())"""
     points, normals = _f(points).reshape(-1, 3), _f(normals).reshape(-1, 3)
     valid = valid_oriented_rows(points, normals)
     points, normals = points[valid], normals[valid]
@@ -406,7 +426,8 @@ def reconstruct(points, normals, depth: int = 6, point_weight: float = 4.0,
         return dict(vertices=np.empty((0, 3)), faces=np.empty((0, 3), np.int64),
                     density=np.empty(0), iso_value=0.0, residual_norm=0.0)
     unit, origin, width = normalize(points, scale)
-    sol = solve(unit, normals, depth, point_weight, confidence, cycles, samples_per_node)
+    sol = solve(unit, normals, depth, point_weight, confidence, cycles, samples_per_node,
+                handle=handle)
     v, f, d = extract_isosurface(sol["field"], sol["iso_value"], sol["density"])
     return dict(vertices=np.ascontiguousarray(v * width + origin), faces=f, density=d,
                 iso_value=sol["iso_value"], residual_norm=sol["residual_norm"], solution=sol,

@@ -23,6 +23,26 @@
 #endif
 #define MUTATED(k) (ORACLE_MUTATION == (k))
 
+/* Built with -fopenmp (oracle.build_openmp), the plane loops of the grid kernels run on
+ * several threads: within one colour sweep, restriction, prolongation, assembly or residual
+ * every output depends only on inputs no other iteration writes, so the results equal the
+ * serial build's bit for bit (the residual norm, the splat and the extraction stay serial).
+ * It exists to time the reference algorithm on all host cores. */
+#ifdef _OPENMP
+#include <omp.h>
+#define ORACLE_PARALLEL_FOR _Pragma("omp parallel for schedule(static)")
+#else
+#define ORACLE_PARALLEL_FOR
+#endif
+
+int oracle_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
 static inline int64_t gidx(int64_t x, int64_t y, int64_t z, int64_t n) {
     return (z * n + y) * n + x; /* capi.mojo:41-43 */
 }
@@ -156,6 +176,7 @@ void oracle_assemble(const double *vx, const double *vy, const double *vz,
                      double target_value) {
     double h_inv = (double)(n - 1);
     double h_inv2 = h_inv * h_inv;
+    ORACLE_PARALLEL_FOR
     for (int64_t z = 0; z < n; ++z)
         for (int64_t y = 0; y < n; ++y)
             for (int64_t x = 0; x < n; ++x) {
@@ -199,7 +220,8 @@ void oracle_relax(double *solution, const double *rhs, const double *screen, int
     double h_inv = (double)(n - 1);
     double h_inv2 = h_inv * h_inv;
     for (int64_t it = 0; it < iterations; ++it)
-        for (int64_t color = 0; color < 2; ++color)
+        for (int64_t color = 0; color < 2; ++color) {
+            ORACLE_PARALLEL_FOR
             for (int64_t z = 0; z < n; ++z)
                 for (int64_t y = 0; y < n; ++y)
                     for (int64_t x = 0; x < n; ++x) {
@@ -211,6 +233,7 @@ void oracle_relax(double *solution, const double *rhs, const double *screen, int
                         double diag = (double)deg * h_inv2 + screening;
                         if (diag > 1.0e-30) solution[idx] = (rhs[idx] + nsum * h_inv2) / diag;
                     }
+        }
 }
 
 /* capi.mojo:413-553. The norm sums squares in index order (the reference uses
@@ -219,6 +242,7 @@ double oracle_residual(const double *solution, const double *rhs, const double *
                        double *residual, int64_t n, double point_weight) {
     double h_inv = (double)(n - 1);
     double h_inv2 = h_inv * h_inv;
+    ORACLE_PARALLEL_FOR
     for (int64_t z = 0; z < n; ++z)
         for (int64_t y = 0; y < n; ++y)
             for (int64_t x = 0; x < n; ++x) {
@@ -246,6 +270,7 @@ double oracle_residual(const double *solution, const double *rhs, const double *
  * the in-grid weight sum; terms accumulate in (oz, oy, ox) order. */
 void oracle_restrict(const double *fine, double *coarse, int64_t fine_n) {
     int64_t cn = (fine_n + 1) / 2;
+    ORACLE_PARALLEL_FOR
     for (int64_t z = 0; z < cn; ++z)
         for (int64_t y = 0; y < cn; ++y)
             for (int64_t x = 0; x < cn; ++x) {
@@ -275,6 +300,7 @@ void oracle_restrict(const double *fine, double *coarse, int64_t fine_n) {
 /* capi.mojo:676-742 */
 void oracle_prolong_add(const double *coarse, double *fine, int64_t fine_n) {
     int64_t cn = (fine_n + 1) / 2;
+    ORACLE_PARALLEL_FOR
     for (int64_t z = 0; z < fine_n; ++z) {
         int64_t z0 = z / 2, z1 = z0 + 1 < cn - 1 ? z0 + 1 : cn - 1;
         double tz = (z & 1) ? 0.5 : 0.0;

@@ -17,6 +17,8 @@ extern "C" {
 #endif
 
 double oracle_norm3(double x, double y, double z);
+/* threads the grid kernels use (1 unless built with OpenMP) */
+int oracle_threads(void);
 void oracle_octree_codes(const double *points, int64_t point_count, int64_t depth,
                          int64_t *codes);
 int64_t oracle_splat(const double *points, const double *normals, int64_t point_count,

@@ -189,3 +189,16 @@ def test_every_orchestration_slip_fails_an_anchor(mutation, tmp_path):
         except AssertionError:
             failed.append(anchor.__name__)
     assert failed, f"no anchor rejects: {MUTATIONS[mutation]}"
+
+
+def test_openmp_oracle_equals_the_serial_oracle():
+    # the all-cores build (timed as the CPU baseline) computes the same grids bit for bit
+    points, normals = _cloud(3000, 13)
+    omp = oracle.openmp()
+    assert omp.oracle_threads() >= 1
+    for depth in (4, 6):
+        a = oracle.solve(points, normals, depth, confidence=True)
+        b = oracle.solve(points, normals, depth, confidence=True, handle=omp)
+        for name in ("density", "screen", "rhs", "field"):
+            assert np.array_equal(a[name], b[name]), (name, depth)
+        assert a["residual_norm"] == b["residual_norm"] and a["iso_value"] == b["iso_value"]
