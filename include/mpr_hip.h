@@ -70,6 +70,14 @@ int mpr_residual_f32(const float *solution, const float *rhs, const float *scree
 int mpr_restrict_f32(const float *fine, float *coarse, int64_t fine_n);
 int mpr_prolong_add_f32(const float *coarse, float *fine, int64_t fine_n);
 
+/* The mixed V-cycle's step after a level's pre-smoothing, on host float32 grids (n = 2^s + 1
+ * in [65, 1025]): the batched colour-0-only residual (colour-1 residual words left at zero;
+ * the sweep that ends pre-smoothing leaves them at rounding level) and the restriction that
+ * reads the fine colour 1 as zero, into coarse_rhs ((n+1)/2)^3. No reference counterpart: a
+ * check of those two kernels against a float32 restatement. 0, or -1 with mpr_last_error(). */
+int mpr_vcycle_transfer_f32(const float *solution, const float *rhs, const float *screen,
+                            float *residual, float *coarse_rhs, int64_t n, double point_weight);
+
 /* Replaces mpr_restrict_f64 (src/capi.mojo:556-673; _lib.py:27). */
 void mpr_restrict_f64(const double *fine, double *coarse, int64_t fine_n);
 

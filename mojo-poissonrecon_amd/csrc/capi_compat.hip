@@ -9,6 +9,7 @@
 #include <limits>
 #include <vector>
 
+#include "mpr_batch.h"
 #include "mpr_hip.h"
 #include "mpr_internal.h"
 
@@ -290,6 +291,44 @@ int mpr_prolong_add_f32(const float* coarse, float* fine, int64_t fine_n) {
     upload_f32_as_split(fine, f.p, gf, nat, Mf);
     launch_prolong_add<float>(c.p, gc, f.p, gf, kNull);
     download_split_f32(f.p, fine, gf, nat, Mf);
+    return 0;
+  } catch (const HipError& e) {
+    report(__func__, e.msg);
+    return -1;
+  }
+}
+
+int mpr_vcycle_transfer_f32(const float* solution, const float* rhs, const float* screen,
+                            float* residual, float* coarse_rhs, int64_t n, double point_weight) {
+  if (n < 65 || n > 1025 || ((n - 1) & (n - 2)) != 0 || !(point_weight >= 0.0)) {
+    set_error("mpr_vcycle_transfer_f32: n must be 2^s + 1 in [65, 1025], point_weight >= 0");
+    return -1;
+  }
+  if (ensure_device()) return -1;
+  try {
+    const int64_t cn = (n + 1) / 2;
+    const Geom g = make_geom((int32_t)n), gc = make_geom((int32_t)cn);
+    const size_t M = (size_t)n * n * n, Mc = (size_t)cn * cn * cn, W = 2ull * g.C;
+    DevBuf<double> nat(M);
+    DevBuf<float> x(W), r(W), sc(W), res(W), rc(2ull * gc.C);
+    upload_f32_as_split(solution, x.p, g, nat, M);
+    upload_f32_as_split(rhs, r.p, g, nat, M);
+    upload_f32_as_split(screen, sc.p, g, nat, M);
+    launch_fill<float>(res.p, 0.0f, W, kNull);
+    launch_fill<float>(rc.p, 0.0f, 2ull * gc.C, kNull);
+    StepPtrs<float> step{};
+    step.x = x.p;
+    step.rhs = r.p;
+    step.screen = sc.p;
+    step.res = res.p;
+    step.rhsc = rc.p;
+    step.top = 1;
+    DevBuf<StepPtrs<float>> tab(1);
+    h2d(tab.p, &step, 1);
+    launch_b_residual<float>(tab.p, 1, g, point_weight, kNull, true);
+    launch_b_restrict<float>(tab.p, 1, g, gc, false, kNull, true);
+    download_split_f32(res.p, residual, g, nat, M);
+    download_split_f32(rc.p, coarse_rhs, gc, nat, Mc);
     return 0;
   } catch (const HipError& e) {
     report(__func__, e.msg);

@@ -158,8 +158,11 @@ class BatchedVcycle {
     return sizeof(T) == 4 && pre > 0 && pw >= 0.0 && c > sc && group_layout(geom_[c]) &&
            group_layout(geom_[c - 1]);
   }
-  void restrict_finest(hipStream_t s) const {
-    launch_b_restrict<T>(step_[smax], step_count_[smax], geom_[smax], geom_[smax - 1], false, s);
+  // as the V-cycle launches it after pre-smoothing (colour 1 read as zero where the
+  // residual computed colour 0 only)
+  void restrict_finest(double pw, hipStream_t s) const {
+    launch_b_restrict<T>(step_[smax], step_count_[smax], geom_[smax], geom_[smax - 1], false, s,
+                         colour0_residual(smax, pw, 1));
   }
   void prolong_finest(bool colour1_only, hipStream_t s) const {
     launch_b_prolong<T>(step_[smax], step_count_[smax], geom_[smax - 1], geom_[smax],

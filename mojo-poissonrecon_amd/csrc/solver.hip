@@ -1406,16 +1406,17 @@ int mpr_job_time_kernel(mpr_job* job, int32_t kernel, int32_t reps, double* ms_p
     case 9:
       for (int r = 0; r < reps; ++r) job->vc32.residual_finest(pw, s);
       break;
-    case 10:
-      for (int r = 0; r < reps; ++r) job->vc32.restrict_finest(s);
-      bytes = 4.0 * (M0 + M1);
+    case 10:  // fine colour 0 (colour 1 read as zero after pre-smoothing), coarse written
+      for (int r = 0; r < reps; ++r) job->vc32.restrict_finest(pw, s);
+      bytes = job->vc32.colour0_residual(job->vc32.smax, pw, 1) ? 4.0 * (M0 / 2.0 + M1)
+                                                                 : 4.0 * (M0 + M1);
       break;
     case 11:  // colour 1 only: read coarse and fine colour 1, write fine colour 1
       for (int r = 0; r < reps; ++r) job->vc32.prolong_finest(true, s);
       bytes = 4.0 * (M1 + M0);
       break;
     case 12:
-      for (int r = 0; r < reps; ++r) job->vc64.restrict_finest(s);
+      for (int r = 0; r < reps; ++r) job->vc64.restrict_finest(pw, s);
       bytes = 8.0 * (M0 + M1);
       break;
     case 13:

@@ -266,3 +266,20 @@ def test_dual_capacity_semantics_match_oracle():
         assert got_f == ofc
         written = 2 * min(ofc // 2, capf // 2)
         assert np.array_equal(faces[:written], of[:written])
+
+
+@pytest.mark.parametrize("n", [65, 129])
+def test_colour0_residual_and_colour1_zero_restriction_bit_exact(n):
+    # the mixed V-cycle's batched step after pre-smoothing: residual of colour 0 only
+    # (colour-1 words stay zero), then the full-weighting restriction with the fine colour 1
+    # read as zero; against the float32 restatements
+    x, rhs, screen = f32_grids(n, 700 + n)
+    res = np.full_like(x, np.nan)
+    coarse = np.full(((n + 1) // 2,) * 3, np.nan, dtype=np.float32)
+    assert lib().mpr_vcycle_transfer_f32(x.ctypes.data, rhs.ctypes.data, screen.ctypes.data,
+                                         res.ctypes.data, coarse.ctypes.data, n, 4.0) == 0
+    z, y, xx = np.indices(x.shape)
+    colour0 = ((xx + y + z) & 1) == 0
+    want = np.where(colour0, oracle.residual_np(x, rhs, screen, 4.0), np.float32(0))
+    assert np.array_equal(res, want)
+    assert np.array_equal(coarse, oracle.restrict_np(want))
