@@ -425,7 +425,8 @@ def main() -> int:
 
     points, normals = shapes.torus_workload(args.points, seed=0)
     bclouds = [shapes.torus_workload(args.points, seed=s) for s in range(args.batch)]
-    sweep_kernels = [("rb_sweep_f32", 7), ("residual_f32", 9), ("defect_f64_f32", 13)]
+    sweep_kernels = [("rb_sweep_f32", 7), ("residual_f32", 9), ("restrict_f32", 10),
+                     ("prolong_f32_c1", 11), ("defect_f64_f32", 13)]
     if args.only == "mixed_batch":
         b, jobs, _ = make_batch(bclouds, [args.depth] * args.batch, "mixed")
         per, blat = run_batch_launches(b, args.steps, args.warmup)

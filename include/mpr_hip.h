@@ -230,8 +230,9 @@ int mpr_batch_query(mpr_batch *batch);
 int mpr_batch_wait(mpr_batch *batch, mpr_result *results);
 /* Kernel timing of a batch on its own stream with HIP events (as mpr_job_time_kernel, after
  * a run): one launch covers every job of the batch's finest depth. Kernels 7 (FP32 colour
- * sweep), 9 (FP32 residual) and 13 (FP64->FP32 defect; jobs of one depth) for a mixed
- * batch, 8 (FP64 colour sweep) for an FP64 one. bytes_per_launch sums the jobs' algorithmic
+ * sweep), 9 (FP32 residual), 10 (FP32 restriction of the finest residual), 11 (FP32
+ * colour-1 prolongation into the finest level) and 13 (FP64->FP32 defect; jobs of one
+ * depth) for a mixed batch, 8 (FP64 colour sweep) for an FP64 one. bytes_per_launch sums the jobs' algorithmic
  * bytes. No reference counterpart (measurement hook). */
 int mpr_batch_time_kernel(mpr_batch *batch, int32_t kernel, int32_t reps, double *ms_per_launch,
                           double *bytes_per_launch);

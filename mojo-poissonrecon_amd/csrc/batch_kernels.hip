@@ -352,9 +352,9 @@ __global__ __launch_bounds__(kThreads) void kb_restrict_group(const StepPtrs<T>*
   if (slot >= count) return;
   const StepPtrs<T> p = tab[slot];
   const uint32_t lane = (lb - slot * nbi) * kThreads + threadIdx.x;
-  restrict_group<T, kGroup<T>>(as_global((const T*)p.res), gf, as_global(p.rhsc),
-                               zero_coarse ? as_global(p.xc) : gptr<T>(nullptr), gc, lane, lanes,
-                               T(1), C1Z);
+  restrict_group<T, kGroup<T>, C1Z>(as_global((const T*)p.res), gf, as_global(p.rhsc),
+                                    zero_coarse ? as_global(p.xc) : gptr<T>(nullptr), gc, lane,
+                                    lanes, T(1));
 }
 
 template <class T>

@@ -300,9 +300,28 @@ struct AssembleArgs {
   double spn, pw, target;
 };
 
-// assembly of node t = (x, y, z)
-__device__ __forceinline__ void assemble_node(const AssembleArgs& a, const Geom& g, uint32_t t,
-                                              int x, int y, int z) {
+// Sets the live bits of a wavefront's lanes with one atomicOr per distinct word (the lanes
+// of a row chunk cover at most a few words; one atomic per lane would serialise up to 32
+// per word in the L2). Every lane of the wavefront calls it.
+__device__ __forceinline__ void wave_set_bits(uint32_t* words, bool set, uint32_t index) {
+  const uint32_t word = index >> 5, bit = 1u << (index & 31u);
+  uint64_t pending = __ballot(set);
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const uint32_t w = (uint32_t)__shfl((int)word, leader);
+    const bool mine = set && word == w;
+    uint32_t v = mine ? bit : 0u;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v |= (uint32_t)__shfl_xor((int)v, o);
+    if ((int)(threadIdx.x & 63u) == leader) atomicOr(&words[w], v);
+    pending &= ~__ballot(mine);
+  }
+}
+
+// assembly of node t = (x, y, z); returns whether its split slot (si) is live (rhs or
+// screen not +0.0, by bit pattern, so -0.0 counts as live); the callers set the live bits
+__device__ __forceinline__ bool assemble_node(const AssembleArgs& a, const Geom& g, uint32_t t,
+                                              int x, int y, int z, uint32_t& si_out) {
   const int n = g.n;
   const double h_inv = (double)(n - 1);
   const double h_inv2 = h_inv * h_inv;
@@ -323,16 +342,18 @@ __device__ __forceinline__ void assemble_node(const AssembleArgs& a, const Geom&
   const double r = -div + screening * a.target;
   if (a.rhs_nat) a.rhs_nat[t] = r;
   if (a.scr_nat_out) a.scr_nat_out[t] = s;
+  bool live = false;
+  si_out = 0u;
   if (a.rhs_split) {
     const uint32_t si = split_index(g, x, y, z);
     a.rhs_split[si] = r;
     a.scr_split[si] = s;
     if (a.scr_split32) a.scr_split32[si] = (float)s;
-    // live slots: rhs or screen not +0.0 (bit patterns, so -0.0 counts as live)
-    if (a.live && (__double_as_longlong(r) | __double_as_longlong(s)) != 0)
-      atomicOr(&a.live[si >> 5], 1u << (si & 31u));
+    live = a.live && (__double_as_longlong(r) | __double_as_longlong(s)) != 0;
+    si_out = si;
   }
   if (a.density) a.density[t] = splat_value(a.density, tb, t) / a.spn;
+  return live;
 }
 
 // dense assembly: every node; with `prev`, also records the touched bits for the next run
@@ -340,13 +361,18 @@ __global__ __launch_bounds__(kThreads) void k_assemble(AssembleArgs a, Geom g, F
                                                        uint32_t* prev) {
   const uint32_t t = logical_block() * kThreads + threadIdx.x;
   const int n = g.n;
-  if (t >= (uint32_t)n * n * n) return;
-  if (prev && (t & 31u) == 0) prev[t >> 5] = a.touched[t >> 5];
-  const uint32_t zy = div_n.div(t);
-  const int x = (int)(t - zy * (uint32_t)n);
-  const int z = (int)div_n.div(zy);
-  const int y = (int)(zy - (uint32_t)z * (uint32_t)n);
-  assemble_node(a, g, t, x, y, z);
+  const bool act = t < (uint32_t)n * n * n;
+  bool live = false;
+  uint32_t si = 0u;
+  if (act) {
+    if (prev && (t & 31u) == 0) prev[t >> 5] = a.touched[t >> 5];
+    const uint32_t zy = div_n.div(t);
+    const int x = (int)(t - zy * (uint32_t)n);
+    const int z = (int)div_n.div(zy);
+    const int y = (int)(zy - (uint32_t)z * (uint32_t)n);
+    live = assemble_node(a, g, t, x, y, z, si);
+  }
+  if (a.live) wave_set_bits(a.live, live, si);
 }
 
 // Sparse assembly, step 1: lists the 64-node row chunks (chunk k of row zy: x in
@@ -381,10 +407,14 @@ __global__ __launch_bounds__(kThreads) void k_assemble_chunks(AssembleArgs a, Ge
     const uint32_t chunk = list[i];
     const uint32_t zy = chunk / per_row, k = chunk - zy * per_row;
     const int x = (int)(64u * k + lane);
-    if (x >= n) continue;
-    const int z = (int)(zy / (uint32_t)n);
-    const int y = (int)(zy - (uint32_t)z * (uint32_t)n);
-    assemble_node(a, g, zy * (uint32_t)n + (uint32_t)x, x, y, z);
+    bool live = false;
+    uint32_t si = 0u;
+    if (x < n) {
+      const int z = (int)(zy / (uint32_t)n);
+      const int y = (int)(zy - (uint32_t)z * (uint32_t)n);
+      live = assemble_node(a, g, zy * (uint32_t)n + (uint32_t)x, x, y, z, si);
+    }
+    if (a.live) wave_set_bits(a.live, live, si);  // i is uniform over the wavefront
   }
 }
 

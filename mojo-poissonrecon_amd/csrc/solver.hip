@@ -1325,6 +1325,17 @@ static double pipeline_kernel_bytes(const mpr_job* job, int kernel, double pw) {
       return (3.0 + live) * 4.0 * M0 + mask;  // x, rhs, residual written; live screen
     case 13:  // x64 read, r32 written; live rhs64 and screen64
       return (8.0 + 4.0 + 16.0 * live) * M0 + mask;
+    case 10: {  // fine residual read (colour 0 only in the C1Z form), coarse rhs written
+      const Geom& g1 = job->geoms[1];
+      const double M1 = (double)g1.n * g1.n * g1.n;
+      return job->vc32.colour0_residual(job->vc32.smax, pw, 1) ? 4.0 * (M0 / 2.0 + M1)
+                                                               : 4.0 * (M0 + M1);
+    }
+    case 11: {  // colour 1 only: fine colour-1 x read and written, coarse x read
+      const Geom& g1 = job->geoms[1];
+      const double M1 = (double)g1.n * g1.n * g1.n;
+      return 4.0 * (M0 + M1);
+    }
   }
   return 0.0;
 }
@@ -1437,13 +1448,15 @@ int mpr_job_time_kernel(mpr_job* job, int32_t kernel, int32_t reps, double* ms_p
 
 int mpr_batch_time_kernel(mpr_batch* b, int32_t kernel, int32_t reps, double* ms_per_launch,
                           double* bytes_per_launch) {
-  if (!b || reps < 1 || (kernel != 7 && kernel != 8 && kernel != 9 && kernel != 13)) {
-    set_error("batch kernel timing: kernel 7, 8, 9 or 13 and reps >= 1");
+  if (!b || reps < 1 ||
+      (kernel != 7 && kernel != 8 && kernel != 9 && kernel != 10 && kernel != 11 &&
+       kernel != 13)) {
+    set_error("batch kernel timing: kernel 7, 8, 9, 10, 11 or 13 and reps >= 1");
     return -1;
   }
   const bool mixed = b->precision == MPR_PRECISION_MIXED;
   if ((kernel == 8) == mixed) {
-    set_error("batch kernel timing: kernel 8 needs an FP64 batch, 7, 9 and 13 a mixed one");
+    set_error("batch kernel timing: kernel 8 needs an FP64 batch, 7, 9, 10, 11 and 13 a mixed one");
     return -1;
   }
   for (Job* j : b->jobs)
@@ -1477,6 +1490,12 @@ int mpr_batch_time_kernel(mpr_batch* b, int32_t kernel, int32_t reps, double* ms
       break;
     case 9:
       for (int r = 0; r < reps; ++r) b->vc32.residual_finest(pw, s);
+      break;
+    case 10:
+      for (int r = 0; r < reps; ++r) b->vc32.restrict_finest(pw, s);
+      break;
+    case 11:
+      for (int r = 0; r < reps; ++r) b->vc32.prolong_finest(true, s);
       break;
     case 13:
       for (int r = 0; r < reps; ++r) b->dtab.defect(pw, s);

@@ -524,10 +524,16 @@ __device__ __forceinline__ void prolong_group(gptr<const T> XC, const Geom& gc, 
 // (oz, oy, ox) order with its weights, so the result is restrict_node's. Needs 16-byte
 // aligned rows on both levels. Both coarse colours' groups at slots K0.. of row (Y,Z) read
 // the same fine chunks, so one lane restricts both from one set of loads.
-template <class T, int V>
+//
+// The nine rows are taken one z-plane (three rows) at a time: the per-node sums still run
+// over r = 0..8 in order, and a lane holds one plane's chunks instead of all nine rows'
+// (about a third of the registers, so more waves hide the load latency). C1Z: the fine
+// colour-1 values are known to be zero (the residual computed colour 0 only) and only the
+// colour-0 chunks are loaded.
+template <class T, int V, bool C1Z = false>
 __device__ __forceinline__ void restrict_group(gptr<const T> F, const Geom& gf, gptr<T> RC,
                                                gptr<T> XC, const Geom& gc, uint32_t lane,
-                                               uint32_t lanes, T scale, bool c1_zero = false) {
+                                               uint32_t lanes, T scale) {
   using CT = typename SlotGroup<T, V>::CT;
   constexpr int W = SlotGroup<T, V>::W;
   constexpr int NR = 2 * V / W;  // chunks per fine row and colour
@@ -540,77 +546,160 @@ __device__ __forceinline__ void restrict_group(gptr<const T> F, const Geom& gf, 
   const uint32_t K0 = rr - Y * (uint32_t)gc.H;
   const int nf = gf.n;
   const uint32_t base = 2u * K0;
-  // every load first (9 rows x 2 colours x NR chunks + 9 edge slots), then the sums
-  CT aq[9][NR], ao[9][NR];
-  T before[9];
+  // chunks past the row end only hold pad slots (never used): read the row start
+  uint32_t slot[NR];
 #pragma unroll
-  for (int r = 0; r < 9; ++r) {
-    const int oz = r / 3 - 1, oy = r % 3 - 1;
-    const int zz = 2 * (int)Z + oz, yy = 2 * (int)Y + oy;
-    const int zc = zz < 0 ? 0 : (zz >= nf ? nf - 1 : zz);
-    const int yc = yy < 0 ? 0 : (yy >= nf ? nf - 1 : yy);
-    const uint32_t q = (uint32_t)((yc + zc) & 1);
-    const uint32_t row = ((uint32_t)zc * (uint32_t)nf + (uint32_t)yc) * (uint32_t)gf.H;
-    const uint32_t rq = q * gf.C + row, ro = (1u - q) * gf.C + row;
-    // c1_zero: the fine colour-1 values are known to be zero and are not loaded
-    const bool zq = c1_zero && q == 1u, zo = c1_zero && q == 0u;
+  for (int i = 0; i < NR; ++i)
+    slot[i] = base + (uint32_t)(i * W) + W <= (uint32_t)gf.H ? base + i * W : 0u;
+  const uint32_t bslot = base > 0 ? base - 1 : 0;
+  T sum[2][V], wsum[2][V];
 #pragma unroll
-    for (int i = 0; i < NR; ++i) {
-      // chunks past the row end only hold pad slots (never used): read the row start
-      const uint32_t slot = base + (uint32_t)(i * W) + W <= (uint32_t)gf.H ? base + i * W : 0u;
-      aq[r][i] = zq ? CT(T(0)) : *(const __attribute__((address_space(1))) CT*)(F + rq + slot);
-      ao[r][i] = zo ? CT(T(0)) : *(const __attribute__((address_space(1))) CT*)(F + ro + slot);
+  for (int cc = 0; cc < 2; ++cc)
+#pragma unroll
+    for (int j = 0; j < V; ++j) sum[cc][j] = wsum[cc][j] = T(0);
+#pragma unroll
+  for (int oz = -1; oz <= 1; ++oz) {
+    // this plane's three rows, every load first; the scheduling barrier keeps the machine
+    // scheduler from hoisting the next plane's loads above this plane's sums
+    __builtin_amdgcn_sched_barrier(0);
+    if (C1Z) {
+      // colour 0 only. In a fine row of parity q the colour-0 nodes sit at x = q mod 2, so
+      // of coarse node j's three terms (x-1, x, x+1) the centre is colour 0 when q = 0 and
+      // the two sides when q = 1; the others are the +0.0 the full form would load. The
+      // window a[] holds colour-0 slots 2 K0 - 1 .. 2 K0 + 2V - 1 of the row.
+      T a[3][2 * V + 1];
+      uint32_t qk[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int zz = 2 * (int)Z + oz, yy = 2 * (int)Y + k - 1;
+        const int zc = zz < 0 ? 0 : (zz >= nf ? nf - 1 : zz);
+        const int yc = yy < 0 ? 0 : (yy >= nf ? nf - 1 : yy);
+        qk[k] = (uint32_t)((yc + zc) & 1);
+        const uint32_t row = ((uint32_t)zc * (uint32_t)nf + (uint32_t)yc) * (uint32_t)gf.H;
+        a[k][0] = F[row + bslot];
+#pragma unroll
+        for (int i = 0; i < NR; ++i) {
+          const CT c = *(const __attribute__((address_space(1))) CT*)(F + row + slot[i]);
+#pragma unroll
+          for (int w = 0; w < W; ++w) a[k][1 + i * W + w] = c[w];
+        }
+      }
+#pragma unroll
+      for (int cc = 0; cc < 2; ++cc) {
+        const int P = (cc ^ (int)Y ^ (int)Z) & 1;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const int oy = k - 1;
+          const int zz = 2 * (int)Z + oz, yy = 2 * (int)Y + oy;
+          const bool iny = zz >= 0 && zz < nf && yy >= 0 && yy < nf;
+          const T wz = oz == 0 ? T(2) : T(1), wy = oy == 0 ? T(2) : T(1);
+          const bool q = qk[k] != 0u;
+#pragma unroll
+          for (int j = 0; j < V; ++j) {
+            // slot 2(K0+j)+P is window index 2j+P+1, slot 2(K0+j)+P-1 index 2j+P
+            const T hi = P ? a[k][2 * j + 2] : a[k][2 * j + 1];
+            const T lo = P ? a[k][2 * j + 1] : a[k][2 * j];  // unused at x = 0 (inm false)
+            const T vm = q ? lo : T(0);  // x - 1
+            const T v0 = q ? T(0) : hi;  // x
+            const T vp = q ? hi : T(0);  // x + 1
+            const int xc = 2 * (2 * ((int)K0 + j) + P);
+            const bool inm = iny && xc - 1 >= 0, in0 = iny && xc < nf, inp = iny && xc + 1 < nf;
+            const T wm = T(1) * wy * wz, w0 = T(2) * wy * wz, wp = T(1) * wy * wz;
+            if (inm) {
+              sum[cc][j] += vm * wm;
+              wsum[cc][j] += wm;
+            }
+            if (in0) {
+              sum[cc][j] += v0 * w0;
+              wsum[cc][j] += w0;
+            }
+            if (inp) {
+              sum[cc][j] += vp * wp;
+              wsum[cc][j] += wp;
+            }
+          }
+        }
+      }
+      // pin this plane's partial sums here, so the compiler cannot sink a colour's whole
+      // chain of additions below the last plane (which would keep every plane's words live)
+#pragma unroll
+      for (int cc = 0; cc < 2; ++cc)
+#pragma unroll
+        for (int j = 0; j < V; ++j) asm volatile("" : "+v"(sum[cc][j]), "+v"(wsum[cc][j]));
+      continue;
     }
-    before[r] = zo ? T(0) : F[ro + (base > 0 ? base - 1 : 0)];
+    CT aq[3][NR], ao[3][NR];
+    T before[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int oy = k - 1;
+      const int zz = 2 * (int)Z + oz, yy = 2 * (int)Y + oy;
+      const int zc = zz < 0 ? 0 : (zz >= nf ? nf - 1 : zz);
+      const int yc = yy < 0 ? 0 : (yy >= nf ? nf - 1 : yy);
+      const uint32_t q = (uint32_t)((yc + zc) & 1);
+      const uint32_t row = ((uint32_t)zc * (uint32_t)nf + (uint32_t)yc) * (uint32_t)gf.H;
+      const uint32_t rq = q * gf.C + row, ro = (1u - q) * gf.C + row;
+#pragma unroll
+      for (int i = 0; i < NR; ++i) {
+        aq[k][i] = *(const __attribute__((address_space(1))) CT*)(F + rq + slot[i]);
+        ao[k][i] = *(const __attribute__((address_space(1))) CT*)(F + ro + slot[i]);
+      }
+      before[k] = F[ro + bslot];
+    }
+#pragma unroll
+    for (int cc = 0; cc < 2; ++cc) {
+      const int P = (cc ^ (int)Y ^ (int)Z) & 1;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int oy = k - 1;
+        const int zz = 2 * (int)Z + oz, yy = 2 * (int)Y + oy;
+        const bool iny = zz >= 0 && zz < nf && yy >= 0 && yy < nf;
+        const T wz = oz == 0 ? T(2) : T(1), wy = oy == 0 ? T(2) : T(1);
+#pragma unroll
+        for (int j = 0; j < V; ++j) {
+          const T q0 = aq[k][(2 * j) / W][(2 * j) % W];
+          const T q1 = aq[k][(2 * j + 1) / W][(2 * j + 1) % W];
+          const T o0 = ao[k][(2 * j) / W][(2 * j) % W];
+          const T o1 = ao[k][(2 * j + 1) / W][(2 * j + 1) % W];
+          T om;
+          if (j == 0)
+            om = before[k];
+          else
+            om = ao[k][(2 * j - 1) / W][(2 * j - 1) % W];
+          const T vm = P ? o0 : om;  // x - 1
+          const T v0 = P ? q1 : q0;  // x
+          const T vp = P ? o1 : o0;  // x + 1
+          const int xc = 2 * (2 * ((int)K0 + j) + P);
+          const bool inm = iny && xc - 1 >= 0, in0 = iny && xc < nf, inp = iny && xc + 1 < nf;
+          const T wm = T(1) * wy * wz, w0 = T(2) * wy * wz, wp = T(1) * wy * wz;
+          if (inm) {
+            sum[cc][j] += vm * wm;
+            wsum[cc][j] += wm;
+          }
+          if (in0) {
+            sum[cc][j] += v0 * w0;
+            wsum[cc][j] += w0;
+          }
+          if (inp) {
+            sum[cc][j] += vp * wp;
+            wsum[cc][j] += wp;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int cc = 0; cc < 2; ++cc)
+#pragma unroll
+      for (int j = 0; j < V; ++j) asm volatile("" : "+v"(sum[cc][j]), "+v"(wsum[cc][j]));
   }
 #pragma unroll
   for (int cc = 0; cc < 2; ++cc) {
     const int P = (cc ^ (int)Y ^ (int)Z) & 1;
-    T sum[V], wsum[V];
-#pragma unroll
-    for (int j = 0; j < V; ++j) sum[j] = wsum[j] = T(0);
-#pragma unroll
-    for (int r = 0; r < 9; ++r) {
-      const int oz = r / 3 - 1, oy = r % 3 - 1;
-      const int zz = 2 * (int)Z + oz, yy = 2 * (int)Y + oy;
-      const bool iny = zz >= 0 && zz < nf && yy >= 0 && yy < nf;
-      const T wz = oz == 0 ? T(2) : T(1), wy = oy == 0 ? T(2) : T(1);
-#pragma unroll
-      for (int j = 0; j < V; ++j) {
-        const T q0 = aq[r][(2 * j) / W][(2 * j) % W];
-        const T q1 = aq[r][(2 * j + 1) / W][(2 * j + 1) % W];
-        const T o0 = ao[r][(2 * j) / W][(2 * j) % W];
-        const T o1 = ao[r][(2 * j + 1) / W][(2 * j + 1) % W];
-        T om;
-        if (j == 0)
-          om = before[r];
-        else
-          om = ao[r][(2 * j - 1) / W][(2 * j - 1) % W];
-        const T vm = P ? o0 : om;  // x - 1
-        const T v0 = P ? q1 : q0;  // x
-        const T vp = P ? o1 : o0;  // x + 1
-        const int xc = 2 * (2 * ((int)K0 + j) + P);
-        const bool inm = iny && xc - 1 >= 0, in0 = iny && xc < nf, inp = iny && xc + 1 < nf;
-        const T wm = T(1) * wy * wz, w0 = T(2) * wy * wz, wp = T(1) * wy * wz;
-        if (inm) {
-          sum[j] += vm * wm;
-          wsum[j] += wm;
-        }
-        if (in0) {
-          sum[j] += v0 * w0;
-          wsum[j] += w0;
-        }
-        if (inp) {
-          sum[j] += vp * wp;
-          wsum[j] += wp;
-        }
-      }
-    }
     T out[V];
     uint32_t mask = 0;
 #pragma unroll
     for (int j = 0; j < V; ++j) {
-      out[j] = sum[j] / wsum[j] * scale;
+      out[j] = sum[cc][j] / wsum[cc][j] * scale;
       if (in && 2 * ((int)K0 + j) + P < gc.n) mask |= 1u << j;
     }
     const uint32_t own = (uint32_t)cc * gc.C + s0;
