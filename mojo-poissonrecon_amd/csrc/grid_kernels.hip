@@ -10,6 +10,7 @@
 #include <algorithm>
 
 #include "mpr_internal.h"
+#include "multi.h"
 #include "stencil.h"
 
 namespace mpr {
@@ -27,9 +28,9 @@ inline unsigned xcd_grid(uint64_t work, unsigned threads) {
 }
 
 template <class T>
-__global__ __launch_bounds__(kThreads) void k_fill(T* p, T v, uint64_t count) {
-  uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
-  uint64_t stride = (uint64_t)gridDim.x * kThreads;
+__device__ __forceinline__ void fill_body(Blk blk_, T* p, T v, uint64_t count) {
+  uint64_t i = (uint64_t)blk_.id * kThreads + threadIdx.x;
+  uint64_t stride = (uint64_t)blk_.count * kThreads;
   for (; i < count; i += stride) p[i] = v;
 }
 
@@ -158,9 +159,9 @@ __global__ __launch_bounds__(1024) void k_norm_finish(const double* partials, in
 }
 
 template <class T>
-__global__ __launch_bounds__(kThreads) void k_restrict(const T* F, Geom gf, T* Cc, Geom gc,
+__device__ __forceinline__ void restrict_nodes_body(Blk blk_, const T* F, Geom gf, T* Cc, Geom gc,
                                                        T scale, float* copy32) {
-  const uint32_t t = logical_block() * kThreads + threadIdx.x;
+  const uint32_t t = xcd_block(blk_) * kThreads + threadIdx.x;
   if (t >= 2u * gc.C) return;
   // colours interleaved (neighbouring threads restrict neighbouring coarse nodes, which
   // read overlapping fine nodes of both colours)
@@ -278,10 +279,10 @@ __device__ __forceinline__ bool any_bit(const uint32_t* w, uint32_t b, uint32_t 
 // need = nodes whose 7-point stencil (linear offsets 0, +-1, +-n, +-n^2) meets a node of
 // cur or prev. Linear offsets also reach across row and plane ends, which only adds nodes
 // (a superset is harmless: an added node is written with the value the assembly computes).
-__global__ __launch_bounds__(kThreads) void k_need_mask(const uint32_t* cur, const uint32_t* prev,
+__device__ __forceinline__ void need_mask_body(Blk blk_, const uint32_t* cur, const uint32_t* prev,
                                                         int64_t words, int64_t sy, int64_t sz,
                                                         uint32_t* need) {
-  const int64_t w = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  const int64_t w = (int64_t)blk_.id * kThreads + threadIdx.x;
   if (w >= words) return;
   const int64_t b = 32 * w;
   need[w] = bits_at(cur, prev, b, words) | bits_at(cur, prev, b + 1, words) |
@@ -357,9 +358,9 @@ __device__ __forceinline__ bool assemble_node(const AssembleArgs& a, const Geom&
 }
 
 // dense assembly: every node; with `prev`, also records the touched bits for the next run
-__global__ __launch_bounds__(kThreads) void k_assemble(AssembleArgs a, Geom g, FastDiv div_n,
+__device__ __forceinline__ void assemble_dense_body(Blk blk_, AssembleArgs a, Geom g, FastDiv div_n,
                                                        uint32_t* prev) {
-  const uint32_t t = logical_block() * kThreads + threadIdx.x;
+  const uint32_t t = xcd_block(blk_) * kThreads + threadIdx.x;
   const int n = g.n;
   const bool act = t < (uint32_t)n * n * n;
   bool live = false;
@@ -379,13 +380,13 @@ __global__ __launch_bounds__(kThreads) void k_assemble(AssembleArgs a, Geom g, F
 // [64k, 64k+64)) that hold a node of the need mask (list order is free: chunks are
 // independent), and records this run's touched bits for the next run's need mask (the need
 // mask was computed from the old ones by the previous kernel).
-__global__ __launch_bounds__(kThreads) void k_need_chunks(const uint32_t* need,
+__device__ __forceinline__ void need_chunks_body(Blk blk_, const uint32_t* need,
                                                           const uint32_t* touched,
                                                           uint32_t* prev, uint32_t words, int n,
                                                           uint32_t per_row, uint32_t chunks,
                                                           uint32_t* list, uint32_t* count) {
-  const uint32_t tid = blockIdx.x * kThreads + threadIdx.x;
-  for (uint32_t w = tid; w < words; w += gridDim.x * kThreads) prev[w] = touched[w];
+  const uint32_t tid = blk_.id * kThreads + threadIdx.x;
+  for (uint32_t w = tid; w < words; w += blk_.count * kThreads) prev[w] = touched[w];
   if (tid >= chunks) return;
   const uint32_t zy = tid / per_row, k = tid - zy * per_row;
   const uint32_t x0 = 64u * k;
@@ -395,15 +396,15 @@ __global__ __launch_bounds__(kThreads) void k_need_chunks(const uint32_t* need,
 
 // Sparse assembly, step 2: one wavefront per listed chunk (a lane per node), for every
 // listed chunk: nodes of a chunk outside the need mask get the assembly's own value, +0.0
-__global__ __launch_bounds__(kThreads) void k_assemble_chunks(AssembleArgs a, Geom g,
+__device__ __forceinline__ void assemble_chunks_body(Blk blk_, AssembleArgs a, Geom g,
                                                               uint32_t per_row,
                                                               const uint32_t* list,
                                                               const uint32_t* count) {
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t waves = gridDim.x * (kThreads / 64);
+  const uint32_t waves = blk_.count * (kThreads / 64);
   const uint32_t total = *count;
   const int n = g.n;
-  for (uint32_t i = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); i < total; i += waves) {
+  for (uint32_t i = blk_.id * (kThreads / 64) + (threadIdx.x >> 6); i < total; i += waves) {
     const uint32_t chunk = list[i];
     const uint32_t zy = chunk / per_row, k = chunk - zy * per_row;
     const int x = (int)(64u * k + lane);
@@ -418,14 +419,109 @@ __global__ __launch_bounds__(kThreads) void k_assemble_chunks(AssembleArgs a, Ge
   }
 }
 
+// ---- multi-instance kernel bodies (multi.h) ---------------------------------------------
+template <class T>
+struct Fill {
+  struct Args {
+    T* p;
+    T v;
+    uint64_t count;
+  };
+  static constexpr int kThreads = mpr::kThreads;
+  static __device__ void run(const Args& a, Blk b) { fill_body<T>(b, a.p, a.v, a.count); }
+};
+
+template <class T>
+struct RestrictNodes {
+  struct Args {
+    const T* F;
+    Geom gf;
+    T* Cc;
+    Geom gc;
+    T scale;
+    float* copy32;
+  };
+  static constexpr int kThreads = mpr::kThreads;
+  static __device__ void run(const Args& a, Blk b) {
+    restrict_nodes_body<T>(b, a.F, a.gf, a.Cc, a.gc, a.scale, a.copy32);
+  }
+};
+
+struct NeedMask {
+  struct Args {
+    const uint32_t *cur, *prev;
+    int64_t words, sy, sz;
+    uint32_t* need;
+  };
+  static constexpr int kThreads = mpr::kThreads;
+  static __device__ void run(const Args& a, Blk b) {
+    need_mask_body(b, a.cur, a.prev, a.words, a.sy, a.sz, a.need);
+  }
+};
+
+struct NeedChunks {
+  struct Args {
+    const uint32_t *need, *touched;
+    uint32_t* prev;
+    uint32_t words;
+    int n;
+    uint32_t per_row, chunks;
+    uint32_t *list, *count;
+  };
+  static constexpr int kThreads = mpr::kThreads;
+  static __device__ void run(const Args& a, Blk b) {
+    need_chunks_body(b, a.need, a.touched, a.prev, a.words, a.n, a.per_row, a.chunks, a.list,
+                     a.count);
+  }
+};
+
+struct AssembleChunks {
+  struct Args {
+    AssembleArgs a;
+    Geom g;
+    uint32_t per_row;
+    const uint32_t *list, *count;
+  };
+  static constexpr int kThreads = mpr::kThreads;
+  static __device__ void run(const Args& a, Blk b) {
+    assemble_chunks_body(b, a.a, a.g, a.per_row, a.list, a.count);
+  }
+};
+
+struct AssembleDense {
+  struct Args {
+    AssembleArgs a;
+    Geom g;
+    FastDiv div_n;
+    uint32_t* prev;
+  };
+  static constexpr int kThreads = mpr::kThreads;
+  static __device__ void run(const Args& a, Blk b) {
+    assemble_dense_body(b, a.a, a.g, a.div_n, a.prev);
+  }
+};
+
 }  // namespace
+
+inline uint32_t fill_blocks(uint64_t count) {
+  return (uint32_t)std::min<uint64_t>(blocks_for(count, kThreads), 8192);
+}
+
+template <class T>
+void launch_fill_multi(const FillJob<T>* jobs, size_t count, hipStream_t s) {
+  std::vector<typename Fill<T>::Args> a(count);
+  std::vector<uint32_t> nb(count);
+  for (size_t i = 0; i < count; ++i) {
+    a[i] = {jobs[i].p, jobs[i].value, jobs[i].count};
+    nb[i] = jobs[i].count ? fill_blocks(jobs[i].count) : 0u;
+  }
+  launch_multi<Fill<T>>(a, nb, s);
+}
 
 template <class T>
 void launch_fill(T* p, T value, uint64_t count, hipStream_t s) {
   if (!count) return;
-  unsigned nb = (unsigned)std::min<uint64_t>(blocks_for(count, kThreads), 8192);
-  hipLaunchKernelGGL(k_fill<T>, dim3(nb), dim3(kThreads), 0, s, p, value, count);
-  MPR_LAUNCH_CHECK();
+  launch_one<Fill<T>>(typename Fill<T>::Args{p, value, count}, fill_blocks(count), s);
 }
 
 template <class T>
@@ -491,10 +587,20 @@ void launch_restrict(const T* F, const Geom& gf, T* Cc, const Geom& gc, double s
     MPR_LAUNCH_CHECK();
     return;
   }
-  const unsigned nb = xcd_grid(2ull * gc.C, kThreads);
-  hipLaunchKernelGGL(k_restrict<T>, dim3(nb), dim3(kThreads), 0, s, F, gf, Cc, gc, (T)scale,
-                     copy32);
-  MPR_LAUNCH_CHECK();
+  const RestrictJob<T> j{F, gf, Cc, gc, scale, copy32};
+  launch_restrict_nodes_multi(&j, 1, s);
+}
+
+template <class T>
+void launch_restrict_nodes_multi(const RestrictJob<T>* jobs, size_t count, hipStream_t s) {
+  std::vector<typename RestrictNodes<T>::Args> a(count);
+  std::vector<uint32_t> nb(count);
+  for (size_t i = 0; i < count; ++i) {
+    const RestrictJob<T>& j = jobs[i];
+    a[i] = {j.F, j.gf, j.Cc, j.gc, (T)j.scale, j.copy32};
+    nb[i] = j.F ? xcd_grid(2ull * j.gc.C, kThreads) : 0u;
+  }
+  launch_multi<RestrictNodes<T>>(a, nb, s);
 }
 
 template <class T>
@@ -541,43 +647,67 @@ void launch_add_correction(double* X, const float* E, uint64_t count, hipStream_
   MPR_LAUNCH_CHECK();
 }
 
+void launch_assemble_multi(const AssembleJob* jobs, size_t count, hipStream_t s) {
+  std::vector<AssembleDense::Args> dense(count);
+  std::vector<NeedMask::Args> nm(count);
+  std::vector<FillJob<uint32_t>> zero(count);
+  std::vector<NeedChunks::Args> nc(count);
+  std::vector<AssembleChunks::Args> ac(count);
+  std::vector<uint32_t> nb_dense(count, 0u), nb_mask(count, 0u), nb_need(count, 0u),
+      nb_asm(count, 0u);
+  bool any_dense = false, any_sparse = false;
+  for (size_t i = 0; i < count; ++i) {
+    const AssembleJob& j = jobs[i];
+    const Geom& g = j.g;
+    const uint64_t m = (uint64_t)g.n * g.n * g.n;
+    const AssembleArgs a{j.vx, j.vy, j.vz, j.screen_nat, j.rhs_nat, j.rhs_split, j.screen_split,
+                         j.density, j.screen_nat_out, j.screen_split32, j.touched, j.live, j.spn,
+                         j.pw, j.target};
+    const TouchedHistory* hist = j.hist;
+    if (hist && (!j.touched || !j.rhs_split))
+      throw HipError{"assembly history needs the touched bits"};
+    zero[i] = {nullptr, 0u, 0};
+    if (!hist || hist->write_all) {
+      dense[i] = {a, g, FastDiv((uint32_t)g.n), hist ? hist->prev : nullptr};
+      nb_dense[i] = xcd_grid(m, kThreads);
+      any_dense = true;
+      continue;
+    }
+    // only the split outputs keep the zero-input value at untouched nodes; the
+    // natural-layout outputs are written by the dense assembly
+    if (j.rhs_nat || j.density || j.screen_nat_out)
+      throw HipError{"sparse assembly writes the split outputs only"};
+    const uint32_t words = (uint32_t)((m + 31) / 32);
+    nm[i] = {j.touched, hist->prev, (int64_t)words, (int64_t)g.n, (int64_t)g.n * g.n, hist->need};
+    nb_mask[i] = blocks_for(words, kThreads);
+    const uint32_t per_row = assemble_chunks_per_row(g.n);
+    const uint32_t chunks = (uint32_t)g.n * (uint32_t)g.n * per_row;
+    zero[i] = {hist->count, 0u, 1};
+    nc[i] = {hist->need, j.touched, hist->prev, words, g.n, per_row, chunks, hist->list,
+             hist->count};
+    nb_need[i] = blocks_for(chunks, kThreads);
+    ac[i] = {a, g, per_row, hist->list, hist->count};
+    // a fixed grid (graph-capturable) whose wavefronts walk the chunk list
+    nb_asm[i] = std::min<unsigned>(blocks_for(chunks, kThreads / 64), 4096u);
+    any_sparse = true;
+  }
+  if (any_dense) launch_multi<AssembleDense>(dense, nb_dense, s);
+  if (!any_sparse) return;
+  launch_multi<NeedMask>(nm, nb_mask, s);
+  launch_fill_multi(zero.data(), count, s);
+  launch_multi<NeedChunks>(nc, nb_need, s);
+  launch_multi<AssembleChunks>(ac, nb_asm, s);
+}
+
 void launch_assemble(const double* vx, const double* vy, const double* vz,
                      const double* screen_nat, double* rhs_nat, double* rhs_split,
                      double* screen_split, double* density, double spn, const Geom& g,
                      double pw, double target, const uint32_t* touched, double* screen_nat_out,
                      float* screen_split32, const TouchedHistory* hist, uint32_t* live,
                      hipStream_t s) {
-  const uint64_t m = (uint64_t)g.n * g.n * g.n;
-  const AssembleArgs a{vx, vy, vz, screen_nat, rhs_nat, rhs_split, screen_split, density,
-                       screen_nat_out, screen_split32, touched, live, spn, pw, target};
-  if (hist && (!touched || !rhs_split)) throw HipError{"assembly history needs the touched bits"};
-  if (!hist || hist->write_all) {
-    hipLaunchKernelGGL(k_assemble, dim3(xcd_grid(m, kThreads)), dim3(kThreads), 0, s, a, g,
-                       FastDiv((uint32_t)g.n), hist ? hist->prev : nullptr);
-    MPR_LAUNCH_CHECK();
-    return;
-  }
-  // only the split outputs keep the zero-input value at untouched nodes; the natural-layout
-  // outputs are written by the dense assembly
-  if (rhs_nat || density || screen_nat_out)
-    throw HipError{"sparse assembly writes the split outputs only"};
-  const uint32_t words = (uint32_t)((m + 31) / 32);
-  hipLaunchKernelGGL(k_need_mask, dim3(blocks_for(words, kThreads)), dim3(kThreads), 0, s,
-                     touched, hist->prev, (int64_t)words, (int64_t)g.n, (int64_t)g.n * g.n,
-                     hist->need);
-  MPR_LAUNCH_CHECK();
-  const uint32_t per_row = assemble_chunks_per_row(g.n);
-  const uint32_t chunks = (uint32_t)g.n * (uint32_t)g.n * per_row;
-  launch_fill<uint32_t>(hist->count, 0u, 1, s);
-  hipLaunchKernelGGL(k_need_chunks, dim3(blocks_for(chunks, kThreads)), dim3(kThreads), 0, s,
-                     hist->need, touched, hist->prev, words, g.n, per_row, chunks, hist->list,
-                     hist->count);
-  MPR_LAUNCH_CHECK();
-  // a fixed grid (graph-capturable) whose wavefronts walk the chunk list
-  const unsigned nb = std::min<unsigned>(blocks_for(chunks, kThreads / 64), 4096u);
-  hipLaunchKernelGGL(k_assemble_chunks, dim3(nb), dim3(kThreads), 0, s, a, g, per_row,
-                     hist->list, hist->count);
-  MPR_LAUNCH_CHECK();
+  const AssembleJob j{vx, vy, vz, screen_nat, rhs_nat, rhs_split, screen_split, density, spn, g,
+                      pw, target, touched, screen_nat_out, screen_split32, hist, live};
+  launch_assemble_multi(&j, 1, s);
 }
 
 #define MPR_INST(T)                                                                          \
@@ -593,6 +723,13 @@ MPR_INST(double)
 MPR_INST(float)
 #undef MPR_INST
 template void launch_fill<int32_t>(int32_t*, int32_t, uint64_t, hipStream_t);
+template void launch_fill_multi<int32_t>(const FillJob<int32_t>*, size_t, hipStream_t);
+template void launch_fill_multi<uint32_t>(const FillJob<uint32_t>*, size_t, hipStream_t);
+template void launch_fill_multi<double>(const FillJob<double>*, size_t, hipStream_t);
+template void launch_fill_multi<unsigned long long>(const FillJob<unsigned long long>*, size_t,
+                                                    hipStream_t);
+template void launch_restrict_nodes_multi<double>(const RestrictJob<double>*, size_t,
+                                                   hipStream_t);
 template void launch_fill<uint32_t>(uint32_t*, uint32_t, uint64_t, hipStream_t);
 template void launch_fill<int64_t>(int64_t*, int64_t, uint64_t, hipStream_t);
 template void launch_fill<unsigned long long>(unsigned long long*, unsigned long long, uint64_t,

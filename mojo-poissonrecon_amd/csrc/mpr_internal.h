@@ -19,6 +19,27 @@ struct LevelPtrs {
 
 // ---- grid kernels (grid_kernels.hip) ----------------------------------------------------
 template <class T> void launch_fill(T* p, T value, uint64_t count, hipStream_t s);
+// fills of several buffers in one launch (multi.h); count 0 skips a job
+template <class T>
+struct FillJob {
+  T* p;
+  T value;
+  uint64_t count;
+};
+template <class T> void launch_fill_multi(const FillJob<T>* jobs, size_t count, hipStream_t s);
+// node-by-node restriction of several grids in one launch (launch_restrict's scalar form);
+// a job with F null is skipped
+template <class T>
+struct RestrictJob {
+  const T* F;
+  Geom gf;
+  T* Cc;
+  Geom gc;
+  double scale;
+  float* copy32;
+};
+template <class T>
+void launch_restrict_nodes_multi(const RestrictJob<T>* jobs, size_t count, hipStream_t s);
 template <class T>
 void launch_relax(T* X, const T* R, const T* S, const Geom& g, double pw, int iterations,
                   hipStream_t s);
@@ -80,6 +101,20 @@ void launch_assemble(const double* vx, const double* vy, const double* vz,
                      double pw, double target, const uint32_t* touched, double* screen_nat_out,
                      float* screen_split32, const TouchedHistory* hist, uint32_t* live,
                      hipStream_t s);
+// one instance of launch_assemble_multi (launch_assemble's arguments)
+struct AssembleJob {
+  const double *vx, *vy, *vz, *screen_nat;
+  double *rhs_nat, *rhs_split, *screen_split, *density;
+  double spn;
+  Geom g;
+  double pw, target;
+  const uint32_t* touched;
+  double* screen_nat_out;
+  float* screen_split32;
+  const TouchedHistory* hist;
+  uint32_t* live;
+};
+void launch_assemble_multi(const AssembleJob* jobs, size_t count, hipStream_t s);
 // ---- point kernels (point_kernels.hip) --------------------------------------------------
 int bbox_partial_count(uint32_t P);
 // core.py:377-385 on the device: params = [origin x, y, z, width], unit = (p - origin)/width
@@ -95,6 +130,25 @@ struct FilterScratch {
   double* part = nullptr;
 };
 uint32_t filter_blocks(uint32_t P);
+// one instance of the row filter + normalisation (launch_filter_normalize's arguments)
+struct FilterJob {
+  const double *pts, *nrm;
+  uint32_t P;
+  double scale;
+  FilterScratch f;
+  uint32_t* total;
+  double *params, *unit, *nrm_out;
+};
+// every job's filter in one launch per step (multi.h)
+void launch_filter_normalize_multi(const FilterJob* jobs, size_t count, hipStream_t s);
+struct ScanJob {
+  const uint32_t* in;
+  uint32_t* out;
+  uint32_t count;
+  uint32_t* total;
+  uint32_t* scratch;
+};
+void launch_scan_exclusive_multi(const ScanJob* jobs, size_t count, hipStream_t s);
 void launch_filter_normalize(const double* pts, const double* nrm, uint32_t P, double scale,
                              const FilterScratch& f, uint32_t* total, double* params, double* unit,
                              double* nrm_out, hipStream_t s);
@@ -112,6 +166,25 @@ void launch_radix_sort(K* keys, uint32_t* vals, K* k_tmp, uint32_t* v_tmp, uint3
                        uint32_t* hist, uint32_t* offs, hipStream_t s,
                        const double* gather_src = nullptr, double* gather_dst = nullptr,
                        K** result_keys = nullptr, uint32_t** result_vals = nullptr);
+// one sort of launch_radix_sort_multi (launch_radix_sort's arguments)
+template <class K>
+struct SortJob {
+  K* keys;
+  uint32_t* vals;
+  K* k_tmp;
+  uint32_t* v_tmp;
+  uint32_t n;
+  int bits;
+  uint32_t *hist, *offs;
+  const double* gather_src;
+  double* gather_dst;
+  K** result_keys;
+  uint32_t** result_vals;
+};
+// every job's sort in one launch per step (multi.h); the jobs share the digit schedule of
+// the widest key, which leaves each job's sorted order what its own sort gives
+template <class K>
+void launch_radix_sort_multi(const SortJob<K>* jobs, size_t count, hipStream_t s);
 void launch_leaf_keys(const double* pts, uint32_t P, int depth, uint64_t* keys, uint32_t* perm,
                       hipStream_t s);
 uint32_t compaction_tiles(uint64_t n);
@@ -149,6 +222,19 @@ void launch_splat(const double* pts, const double* nrm, uint32_t P, int n, int c
                   double* density, double* vx, double* vy, double* vz, double* screen,
                   double spn, unsigned long long* accepted, const SplatScratch& b,
                   uint32_t* touched, hipStream_t s);
+// one instance of launch_splat_multi (launch_splat's arguments)
+struct SplatJob {
+  const double *pts, *nrm;
+  uint32_t P;
+  int n, confidence;
+  double *density, *vx, *vy, *vz, *screen;
+  double spn;
+  unsigned long long* accepted;
+  SplatScratch b;
+  uint32_t* touched;
+};
+// every job's splat in one launch per step (multi.h), each job's results those of its own
+void launch_splat_multi(const SplatJob* jobs, size_t count, hipStream_t s);
 int iso_partial_count(uint32_t P);
 // count (device, nullable): only rows below *count take part
 void launch_iso(const FieldView& field, const double* pts, const double* nrm, uint32_t P, int n,

@@ -6,6 +6,7 @@
 
 #include "block_scan.h"
 #include "mpr_internal.h"
+#include "multi.h"
 
 namespace mpr {
 
@@ -61,7 +62,7 @@ __global__ __launch_bounds__(kThreads) void k_bbox_partials(const double* pts, u
 }
 
 // params: [origin x, y, z, width]
-__global__ __launch_bounds__(kThreads) void k_bbox_finish(const double* part, int nparts,
+__device__ __forceinline__ void bbox_finish_body(Blk blk_, const double* part, int nparts,
                                                           double scale, double* params) {
   __shared__ double lds[6][kThreads];
   double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -121,12 +122,12 @@ __device__ __forceinline__ bool row_valid(const double* pts, const double* nrm, 
 }
 
 // per block of kThreads rows: the number of kept rows and their bounding box
-__global__ __launch_bounds__(kThreads) void k_filter_count(const double* pts, const double* nrm,
+__device__ __forceinline__ void filter_count_body(Blk blk_, const double* pts, const double* nrm,
                                                            uint32_t P, uint32_t* counts,
                                                            double* part) {
   __shared__ double lds[6][kThreads];
   __shared__ uint32_t lds4[4];
-  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t i = blk_.id * kThreads + threadIdx.x;
   const bool ok = i < P && row_valid(pts, nrm, i);
   const uint32_t c = block_count256(ok ? 1u : 0u, lds4);
   for (int a = 0; a < 3; ++a) {
@@ -145,25 +146,25 @@ __global__ __launch_bounds__(kThreads) void k_filter_count(const double* pts, co
       }
     __syncthreads();
   }
-  if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = lds[threadIdx.x][0];
-  if (threadIdx.x == 0) counts[blockIdx.x] = c;
+  if (threadIdx.x < 6) part[6 * blk_.id + threadIdx.x] = lds[threadIdx.x][0];
+  if (threadIdx.x == 0) counts[blk_.id] = c;
 }
 
 // Kept rows in their order to the front (unit-cube points (p - origin) / width, normals as
 // given), dropped rows behind them as zero points with zero normals, which the splat rejects
 // and the iso value skips (rows past *total).
-__global__ __launch_bounds__(kThreads) void k_filter_compact(const double* pts, const double* nrm,
+__device__ __forceinline__ void filter_compact_body(Blk blk_, const double* pts, const double* nrm,
                                                              uint32_t P, const uint32_t* offsets,
                                                              const uint32_t* total,
                                                              const double* params, double* unit,
                                                              double* nrm_out) {
   __shared__ uint32_t lds4[4];
-  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t i = blk_.id * kThreads + threadIdx.x;
   const bool ok = i < P && row_valid(pts, nrm, i);
   uint32_t block_total;
   const uint32_t rank = block_prefix256(ok, lds4, block_total);
   if (i >= P) return;
-  const uint32_t before = offsets[blockIdx.x] + rank;  // kept rows before row i
+  const uint32_t before = offsets[blk_.id] + rank;  // kept rows before row i
   const uint32_t dst = ok ? before : *total + (i - before);
   for (int a = 0; a < 3; ++a) {
     unit[3ull * dst + a] = ok ? (pts[3ull * i + a] - params[a]) / params[3] : 0.0;
@@ -235,13 +236,13 @@ __device__ __forceinline__ uint32_t radix_tile() {
 
 // per-tile digit counts, digit-major: hist[d * nblocks + tile]
 template <class K, int DB>
-__global__ __launch_bounds__(kThreads) void k_radix_hist(const K* keys, uint32_t n, int shift,
+__device__ __forceinline__ void radix_hist_body(Blk blk_, const K* keys, uint32_t n, int shift,
                                                          uint32_t* hist, uint32_t nblocks) {
   constexpr uint32_t kBins = 1u << DB;
   __shared__ uint32_t h[kBins];
   for (uint32_t b = threadIdx.x; b < kBins; b += kThreads) h[b] = 0;
   __syncthreads();
-  const uint32_t tile = radix_tile();
+  const uint32_t tile = xcd_block(blk_);
   const uint32_t base = tile * kRadixTile;
   for (int r = 0; r < kRadixItems; ++r) {
     const uint32_t i = base + r * kThreads + threadIdx.x;
@@ -258,7 +259,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_hist(const K* keys, uint32_t
 // in input order (round-major, wavefront-minor); (3) every key is written. gdst (optional):
 // gdst[pos] = gsrc[val] beside each moved pair, a payload gathered into the sorted order.
 template <class K, int DB>
-__global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* kin, const uint32_t* vin,
+__device__ __forceinline__ void radix_scatter_body(Blk blk_, const K* kin, const uint32_t* vin,
                                                             K* kout, uint32_t* vout, uint32_t n,
                                                             int shift, const uint32_t* offsets,
                                                             uint32_t nblocks, const double* gsrc,
@@ -270,7 +271,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* kin, const 
   uint32_t* flat = &cnt[0][0][0];
   for (uint32_t i = threadIdx.x; i < (uint32_t)(kRadixItems * kWaves) * kBins; i += kThreads)
     flat[i] = 0;
-  const uint32_t tile = radix_tile();
+  const uint32_t tile = xcd_block(blk_);
   const uint32_t base = tile * kRadixTile;
   K keys[kRadixItems];
   uint32_t vals[kRadixItems], digit[kRadixItems], rank[kRadixItems];
@@ -396,7 +397,7 @@ __global__ __launch_bounds__(kThreads) void k_point_leaf(const uint32_t* perm,
 }
 
 // ---- device-wide exclusive scan of tile counts (one workgroup) -------------------------
-__global__ __launch_bounds__(1024) void k_scan_exclusive(const uint32_t* in, uint32_t* out,
+__device__ __forceinline__ void scan_exclusive_body(Blk blk_, const uint32_t* in, uint32_t* out,
                                                          uint32_t count, uint32_t* total) {
   __shared__ uint32_t part[1024];
   const uint32_t per = (count + 1023u) / 1024u;
@@ -448,23 +449,23 @@ __device__ __forceinline__ uint32_t block_exclusive1024(uint32_t v, uint32_t* ld
   return base + inc - v;
 }
 
-__global__ __launch_bounds__(1024) void k_scan_chunk_sums(const uint32_t* in, uint32_t count,
+__device__ __forceinline__ void scan_chunk_sums_body(Blk blk_, const uint32_t* in, uint32_t count,
                                                           uint32_t* sums) {
   __shared__ uint32_t lds16[16];
-  const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x * 4u;
+  const uint32_t base = blk_.id * kScanChunk + threadIdx.x * 4u;
   uint32_t v = 0;
 #pragma unroll
   for (uint32_t k = 0; k < 4; ++k)
     if (base + k < count) v += in[base + k];
   uint32_t total;
   block_exclusive1024(v, lds16, total);
-  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+  if (threadIdx.x == 0) sums[blk_.id] = total;
 }
 
-__global__ __launch_bounds__(1024) void k_scan_chunks(const uint32_t* in, uint32_t* out,
+__device__ __forceinline__ void scan_chunks_body(Blk blk_, const uint32_t* in, uint32_t* out,
                                                       uint32_t count, const uint32_t* offsets) {
   __shared__ uint32_t lds16[16];
-  const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x * 4u;
+  const uint32_t base = blk_.id * kScanChunk + threadIdx.x * 4u;
   uint32_t v[4], sum = 0;
 #pragma unroll
   for (uint32_t k = 0; k < 4; ++k) {
@@ -472,7 +473,7 @@ __global__ __launch_bounds__(1024) void k_scan_chunks(const uint32_t* in, uint32
     sum += v[k];
   }
   uint32_t total;
-  uint32_t run = offsets[blockIdx.x] + block_exclusive1024(sum, lds16, total);
+  uint32_t run = offsets[blk_.id] + block_exclusive1024(sum, lds16, total);
 #pragma unroll
   for (uint32_t k = 0; k < 4; ++k) {
     if (base + k < count) out[base + k] = run;
@@ -491,13 +492,13 @@ __global__ __launch_bounds__(1024) void k_scan_chunks(const uint32_t* in, uint32
 // and summed along the same sorted pairs.
 
 // per point: accepted? and the trilinear cell of pass 1 (capi.mojo:127-173)
-__global__ __launch_bounds__(kThreads) void k_splat_pairs(const double* pts, const double* nrm,
+__device__ __forceinline__ void splat_pairs_body(Blk blk_, const double* pts, const double* nrm,
                                                           uint32_t P, int n, int confidence,
                                                           uint32_t* keys, uint32_t* ids,
                                                           double* w1, uint32_t sentinel,
                                                           unsigned long long* accepted) {
   __shared__ uint32_t lds4[4];
-  const uint32_t p = blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t p = blk_.id * kThreads + threadIdx.x;
   uint32_t ok = 0;
   if (p < P) {
     const uint64_t base = 8ull * p;
@@ -575,11 +576,11 @@ __device__ __forceinline__ RunStart run_start(const uint32_t* keys, uint32_t j, 
 // (ws: pass 1's weights in the sorted order). With `touched`, the grids count as
 // zero-filled (the sums start at +0.0, as on the reference's fresh grids) and the node's bit
 // is set instead of reading the grid.
-__global__ __launch_bounds__(kThreads) void k_splat_sum1(const uint32_t* keys, const double* ws,
+__device__ __forceinline__ void splat_sum1_body(Blk blk_, const uint32_t* keys, const double* ws,
                                                          uint32_t count, uint32_t M,
                                                          double* density, double* screen,
                                                          uint32_t* touched) {
-  const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t j = blk_.id * kThreads + threadIdx.x;
   const RunStart r = run_start(keys, j, count, M);
   if (!r.head) return;
   const uint32_t key = r.key;
@@ -618,12 +619,12 @@ __global__ __launch_bounds__(kThreads) void k_splat_sum1(const uint32_t* keys, c
 }
 
 // pass 2 contributions (capi.mojo:176-223): n^ * (w (n-1) / factor) for each corner
-__global__ __launch_bounds__(kThreads) void k_splat_pass2_pairs(const double* pts,
+__device__ __forceinline__ void splat_pass2_body(Blk blk_, const double* pts,
                                                                 const double* nrm, uint32_t P,
                                                                 int n, int confidence,
                                                                 const double* density,
                                                                 double* c2, double spn) {
-  const uint32_t p = blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t p = blk_.id * kThreads + threadIdx.x;
   if (p >= P) return;
   double nx = nrm[3ull * p], ny = nrm[3ull * p + 1], nz = nrm[3ull * p + 2];
   const double length = norm3(nx, ny, nz);
@@ -676,11 +677,11 @@ __global__ __launch_bounds__(kThreads) void k_splat_pass2_pairs(const double* pt
 
 // one lane per node: v -= n^ vw over its contributions in point order (from +0.0 when the
 // grids count as zero-filled, as k_splat_sum1)
-__global__ __launch_bounds__(kThreads) void k_splat_sum2(const uint32_t* keys,
+__device__ __forceinline__ void splat_sum2_body(Blk blk_, const uint32_t* keys,
                                                          const uint32_t* ids, const double* c2,
                                                          uint32_t count, uint32_t M, double* vx,
                                                          double* vy, double* vz, int from_zero) {
-  const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t j = blk_.id * kThreads + threadIdx.x;
   const RunStart r = run_start(keys, j, count, M);
   if (!r.head) return;
   const uint32_t key = r.key;
@@ -794,6 +795,183 @@ __global__ __launch_bounds__(kThreads) void k_iso_finish(const double* part, int
   if (threadIdx.x == 0) iso[0] = lb[0] > 0.0 ? la[0] / lb[0] : 0.0;
 }
 
+// ---- multi-instance kernel bodies (multi.h) ---------------------------------------------
+struct BboxFinish {
+  struct Args {
+    const double* part;
+    int nparts;
+    double scale;
+    double* params;
+  };
+  static constexpr int kThreads = mpr::kThreads;
+  static __device__ void run(const Args& a, Blk b) {
+    bbox_finish_body(b, a.part, a.nparts, a.scale, a.params);
+  }
+};
+
+struct FilterCount {
+  struct Args {
+    const double *pts, *nrm;
+    uint32_t P;
+    uint32_t* counts;
+    double* part;
+  };
+  static constexpr int kThreads = mpr::kThreads;
+  static __device__ void run(const Args& a, Blk b) {
+    filter_count_body(b, a.pts, a.nrm, a.P, a.counts, a.part);
+  }
+};
+
+struct FilterCompact {
+  struct Args {
+    const double *pts, *nrm;
+    uint32_t P;
+    const uint32_t *offsets, *total;
+    const double* params;
+    double *unit, *nrm_out;
+  };
+  static constexpr int kThreads = mpr::kThreads;
+  static __device__ void run(const Args& a, Blk b) {
+    filter_compact_body(b, a.pts, a.nrm, a.P, a.offsets, a.total, a.params, a.unit, a.nrm_out);
+  }
+};
+
+struct ScanExclusive {
+  struct Args {
+    const uint32_t* in;
+    uint32_t* out;
+    uint32_t count;
+    uint32_t* total;
+  };
+  static constexpr int kThreads = 1024;
+  static __device__ void run(const Args& a, Blk b) {
+    scan_exclusive_body(b, a.in, a.out, a.count, a.total);
+  }
+};
+
+struct ScanChunkSums {
+  struct Args {
+    const uint32_t* in;
+    uint32_t count;
+    uint32_t* sums;
+  };
+  static constexpr int kThreads = 1024;
+  static __device__ void run(const Args& a, Blk b) { scan_chunk_sums_body(b, a.in, a.count, a.sums); }
+};
+
+struct ScanChunks {
+  struct Args {
+    const uint32_t* in;
+    uint32_t* out;
+    uint32_t count;
+    const uint32_t* offsets;
+  };
+  static constexpr int kThreads = 1024;
+  static __device__ void run(const Args& a, Blk b) {
+    scan_chunks_body(b, a.in, a.out, a.count, a.offsets);
+  }
+};
+
+template <class K>
+struct RadixHistArgs {
+  const K* keys;
+  uint32_t n;
+  int shift;
+  uint32_t* hist;
+};
+
+template <class K>
+struct RadixScatterArgs {
+  const K* kin;
+  const uint32_t* vin;
+  K* kout;
+  uint32_t* vout;
+  uint32_t n;
+  int shift;
+  const uint32_t* offsets;
+  const double* gsrc;
+  double* gdst;
+};
+
+template <class K, int DB>
+struct RadixHist {
+  using Args = RadixHistArgs<K>;
+  static constexpr int kThreads = mpr::kThreads;
+  static __device__ void run(const Args& a, Blk b) {
+    radix_hist_body<K, DB>(b, a.keys, a.n, a.shift, a.hist, b.count);
+  }
+};
+
+template <class K, int DB>
+struct RadixScatter {
+  using Args = RadixScatterArgs<K>;
+  static constexpr int kThreads = mpr::kThreads;
+  static __device__ void run(const Args& a, Blk b) {
+    radix_scatter_body<K, DB>(b, a.kin, a.vin, a.kout, a.vout, a.n, a.shift, a.offsets, b.count,
+                              a.gsrc, a.gdst);
+  }
+};
+
+struct SplatPairs {
+  struct Args {
+    const double *pts, *nrm;
+    uint32_t P;
+    int n, confidence;
+    uint32_t *keys, *ids;
+    double* w1;
+    uint32_t sentinel;
+    unsigned long long* accepted;
+  };
+  static constexpr int kThreads = mpr::kThreads;
+  static __device__ void run(const Args& a, Blk b) {
+    splat_pairs_body(b, a.pts, a.nrm, a.P, a.n, a.confidence, a.keys, a.ids, a.w1, a.sentinel,
+                     a.accepted);
+  }
+};
+
+struct SplatSum1 {
+  struct Args {
+    const uint32_t* keys;
+    const double* ws;
+    uint32_t count, M;
+    double *density, *screen;
+    uint32_t* touched;
+  };
+  static constexpr int kThreads = mpr::kThreads;
+  static __device__ void run(const Args& a, Blk b) {
+    splat_sum1_body(b, a.keys, a.ws, a.count, a.M, a.density, a.screen, a.touched);
+  }
+};
+
+struct SplatPass2 {
+  struct Args {
+    const double *pts, *nrm;
+    uint32_t P;
+    int n, confidence;
+    const double* density;
+    double* c2;
+    double spn;
+  };
+  static constexpr int kThreads = mpr::kThreads;
+  static __device__ void run(const Args& a, Blk b) {
+    splat_pass2_body(b, a.pts, a.nrm, a.P, a.n, a.confidence, a.density, a.c2, a.spn);
+  }
+};
+
+struct SplatSum2 {
+  struct Args {
+    const uint32_t *keys, *ids;
+    const double* c2;
+    uint32_t count, M;
+    double *vx, *vy, *vz;
+    int from_zero;
+  };
+  static constexpr int kThreads = mpr::kThreads;
+  static __device__ void run(const Args& a, Blk b) {
+    splat_sum2_body(b, a.keys, a.ids, a.c2, a.count, a.M, a.vx, a.vy, a.vz, a.from_zero);
+  }
+};
+
 }  // namespace
 
 // ------------------------------------------------------------------------------------------
@@ -806,8 +984,7 @@ void launch_normalize(const double* pts, uint32_t P, double scale, double* part,
   const int nb = bbox_partial_count(P);
   hipLaunchKernelGGL(k_bbox_partials, dim3(nb), dim3(kThreads), 0, s, pts, P, part);
   MPR_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_bbox_finish, dim3(1), dim3(kThreads), 0, s, part, nb, scale, params);
-  MPR_LAUNCH_CHECK();
+  launch_one<BboxFinish>(BboxFinish::Args{part, nb, scale, params}, 1, s);
   if (P) {
     hipLaunchKernelGGL(k_normalize, dim3(blocks_for(3ull * P, kThreads)), dim3(kThreads), 0, s,
                        pts, P, params, unit);
@@ -817,22 +994,32 @@ void launch_normalize(const double* pts, uint32_t P, double scale, double* part,
 
 uint32_t filter_blocks(uint32_t P) { return std::max<uint32_t>(1u, blocks_for(P, kThreads)); }
 
+void launch_filter_normalize_multi(const FilterJob* jobs, size_t count, hipStream_t s) {
+  std::vector<FilterCount::Args> fc(count);
+  std::vector<BboxFinish::Args> bf(count);
+  std::vector<FilterCompact::Args> cp(count);
+  std::vector<ScanJob> sc(count);
+  std::vector<uint32_t> nb(count), one(count, 1u), ncp(count);
+  for (size_t i = 0; i < count; ++i) {
+    const FilterJob& j = jobs[i];
+    nb[i] = filter_blocks(j.P);
+    fc[i] = {j.pts, j.nrm, j.P, j.f.counts, j.f.part};
+    bf[i] = {j.f.part, (int)nb[i], j.scale, j.params};
+    sc[i] = {j.f.counts, j.f.offsets, nb[i], j.total, j.f.scan};
+    cp[i] = {j.pts, j.nrm, j.P, j.f.offsets, j.total, j.params, j.unit, j.nrm_out};
+    ncp[i] = j.P ? nb[i] : 0u;
+  }
+  launch_multi<FilterCount>(fc, nb, s);
+  launch_multi<BboxFinish>(bf, one, s);
+  launch_scan_exclusive_multi(sc.data(), count, s);
+  launch_multi<FilterCompact>(cp, ncp, s);
+}
+
 void launch_filter_normalize(const double* pts, const double* nrm, uint32_t P, double scale,
                              const FilterScratch& f, uint32_t* total, double* params, double* unit,
                              double* nrm_out, hipStream_t s) {
-  const uint32_t nb = filter_blocks(P);
-  hipLaunchKernelGGL(k_filter_count, dim3(nb), dim3(kThreads), 0, s, pts, nrm, P, f.counts,
-                     f.part);
-  MPR_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_bbox_finish, dim3(1), dim3(kThreads), 0, s, f.part, (int)nb, scale,
-                     params);
-  MPR_LAUNCH_CHECK();
-  launch_scan_exclusive(f.counts, f.offsets, nb, total, s, f.scan);
-  if (P) {
-    hipLaunchKernelGGL(k_filter_compact, dim3(nb), dim3(kThreads), 0, s, pts, nrm, P, f.offsets,
-                       total, params, unit, nrm_out);
-    MPR_LAUNCH_CHECK();
-  }
+  const FilterJob j{pts, nrm, P, scale, f, total, params, unit, nrm_out};
+  launch_filter_normalize_multi(&j, 1, s);
 }
 
 void launch_octree_codes(const double* pts, uint32_t P, int depth, int64_t* codes,
@@ -843,21 +1030,35 @@ void launch_octree_codes(const double* pts, uint32_t P, int depth, int64_t* code
   MPR_LAUNCH_CHECK();
 }
 
+void launch_scan_exclusive_multi(const ScanJob* jobs, size_t count, hipStream_t s) {
+  // jobs of more than one chunk (with scratch) scan their chunk sums first
+  std::vector<ScanChunkSums::Args> cs(count);
+  std::vector<ScanExclusive::Args> ex(count);
+  std::vector<ScanChunks::Args> ch(count);
+  std::vector<uint32_t> nchunk(count, 0u), one(count, 1u);
+  bool chunked = false;
+  for (size_t i = 0; i < count; ++i) {
+    const ScanJob& j = jobs[i];
+    if (j.count > kScanChunk && j.scratch) {
+      const uint32_t chunks = (j.count + kScanChunk - 1) / kScanChunk;
+      nchunk[i] = chunks;
+      chunked = true;
+      cs[i] = {j.in, j.count, j.scratch};
+      ex[i] = {j.scratch, j.scratch, chunks, j.total};
+      ch[i] = {j.in, j.out, j.count, j.scratch};
+    } else {
+      ex[i] = {j.in, j.out, j.count, j.total};
+    }
+  }
+  if (chunked) launch_multi<ScanChunkSums>(cs, nchunk, s);
+  launch_multi<ScanExclusive>(ex, one, s);
+  if (chunked) launch_multi<ScanChunks>(ch, nchunk, s);
+}
+
 void launch_scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t count, uint32_t* total,
                            hipStream_t s, uint32_t* scratch) {
-  if (count > kScanChunk && scratch) {
-    const uint32_t chunks = (count + kScanChunk - 1) / kScanChunk;
-    hipLaunchKernelGGL(k_scan_chunk_sums, dim3(chunks), dim3(1024), 0, s, in, count, scratch);
-    MPR_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, scratch, scratch, chunks,
-                       total);
-    MPR_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_scan_chunks, dim3(chunks), dim3(1024), 0, s, in, out, count, scratch);
-    MPR_LAUNCH_CHECK();
-    return;
-  }
-  hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, in, out, count, total);
-  MPR_LAUNCH_CHECK();
+  const ScanJob j{in, out, count, total, scratch};
+  launch_scan_exclusive_multi(&j, 1, s);
 }
 
 uint32_t scan_scratch_words(uint32_t count) { return (count + kScanChunk - 1) / kScanChunk + 1; }
@@ -873,58 +1074,94 @@ size_t radix_scratch_words(uint32_t n) {
   return (size_t)kRadixMaxBins * nb + scan_scratch_words((uint32_t)kRadixMaxBins * nb);
 }
 
-// Sorts (keys, vals) by the low `bits` key bits, stably, with 9-bit digits where that saves
-// a pass (25..27 bits: depth-8 node keys, depth-9 Morton codes) and 8-bit digits otherwise.
-// (k_tmp, v_tmp) are ping-pong buffers; hist and offs hold radix_scratch_words(n) words.
-// The result ends in (keys, vals), or, with result_keys / result_vals, in whichever pair of
-// buffers the last pass wrote (no copy back), which they then point at.
-// gather_dst (optional, n words): gather_dst[i] = gather_src[vals[i]] in the sorted order,
-// written by the last pass.
+// Sorts each job's (keys, vals) by the low `bits` key bits, stably, with 9-bit digits where
+// that saves a pass (25..27 bits: depth-8 node keys, depth-9 Morton codes) and 8-bit digits
+// otherwise; the jobs of one call share the digit width and pass count of the widest key
+// (a pass over all-zero digits moves nothing: a stable sort's order is the same for any
+// digit schedule). (k_tmp, v_tmp) are ping-pong buffers; hist and offs hold
+// radix_scratch_words(n) words. The result ends in (keys, vals), or, with result_keys /
+// result_vals, in whichever pair of buffers the last pass wrote (no copy back), which they
+// then point at. gather_dst (optional, n words): gather_dst[i] = gather_src[vals[i]] in the
+// sorted order, written by the last pass.
 template <class K>
-void launch_radix_sort(K* keys, uint32_t* vals, K* k_tmp, uint32_t* v_tmp, uint32_t n, int bits,
-                       uint32_t* hist, uint32_t* offs, hipStream_t s, const double* gather_src,
-                       double* gather_dst, K** result_keys, uint32_t** result_vals) {
-  if (result_keys) *result_keys = keys;
-  if (result_vals) *result_vals = vals;
-  if (n == 0 || (n == 1 && !gather_dst)) return;
-  if (bits > 8 * (int)sizeof(K)) throw HipError{"radix sort: more key bits than the key holds"};
-  const uint32_t nb = radix_tiles(n);
+void launch_radix_sort_multi(const SortJob<K>* jobs, size_t count, hipStream_t s) {
+  int bits = 0;
+  std::vector<char> active(count, 0);
+  for (size_t i = 0; i < count; ++i) {
+    const SortJob<K>& j = jobs[i];
+    if (j.result_keys) *j.result_keys = j.keys;
+    if (j.result_vals) *j.result_vals = j.vals;
+    if (j.n == 0 || (j.n == 1 && !j.gather_dst)) continue;
+    if (j.bits > 8 * (int)sizeof(K)) throw HipError{"radix sort: more key bits than the key holds"};
+    active[i] = 1;
+    bits = std::max(bits, j.bits);
+  }
+  if (!bits) return;
   const int wide = (bits + 8) / 9 < (bits + 7) / 8;
   const int db = wide ? 9 : 8;
   const int passes = (bits + db - 1) / db;
   const uint32_t bins = 1u << db;
-  K *ka = keys, *kb = k_tmp;
-  uint32_t *va = vals, *vb = v_tmp;
+  std::vector<K*> ka(count), kb(count);
+  std::vector<uint32_t*> va(count), vb(count);
+  std::vector<uint32_t> nb(count, 0u);
+  for (size_t i = 0; i < count; ++i) {
+    ka[i] = jobs[i].keys;
+    kb[i] = jobs[i].k_tmp;
+    va[i] = jobs[i].vals;
+    vb[i] = jobs[i].v_tmp;
+    if (active[i]) nb[i] = radix_tiles(jobs[i].n);
+  }
+  std::vector<RadixHistArgs<K>> h(count);
+  std::vector<RadixScatterArgs<K>> sc(count);
+  std::vector<ScanJob> scan(count);
   for (int p = 0; p < passes; ++p) {
     const int shift = db * p;
     const bool last = p == passes - 1;
-    const double* gs = last ? gather_src : nullptr;
-    double* gd = last ? gather_dst : nullptr;
+    size_t n_active = 0;
+    for (size_t i = 0; i < count; ++i) {
+      const SortJob<K>& j = jobs[i];
+      h[i] = {ka[i], j.n, shift, j.hist};
+      sc[i] = {ka[i], va[i], kb[i], vb[i], j.n, shift, j.offs, last ? j.gather_src : nullptr,
+               last ? j.gather_dst : nullptr};
+      // inactive jobs scan nothing
+      scan[n_active] = {j.hist, j.offs, bins * nb[i], nullptr, j.hist + (size_t)bins * nb[i]};
+      n_active += active[i] ? 1 : 0;
+    }
     if (wide)
-      hipLaunchKernelGGL((k_radix_hist<K, 9>), dim3(nb), dim3(kThreads), 0, s, ka, n, shift, hist,
-                         nb);
+      launch_multi<RadixHist<K, 9>>(h, nb, s);
     else
-      hipLaunchKernelGGL((k_radix_hist<K, 8>), dim3(nb), dim3(kThreads), 0, s, ka, n, shift, hist,
-                         nb);
-    MPR_LAUNCH_CHECK();
-    launch_scan_exclusive(hist, offs, bins * nb, nullptr, s, hist + (size_t)bins * nb);
+      launch_multi<RadixHist<K, 8>>(h, nb, s);
+    launch_scan_exclusive_multi(scan.data(), n_active, s);
     if (wide)
-      hipLaunchKernelGGL((k_radix_scatter<K, 9>), dim3(nb), dim3(kThreads), 0, s, ka, va, kb, vb,
-                         n, shift, offs, nb, gs, gd);
+      launch_multi<RadixScatter<K, 9>>(sc, nb, s);
     else
-      hipLaunchKernelGGL((k_radix_scatter<K, 8>), dim3(nb), dim3(kThreads), 0, s, ka, va, kb, vb,
-                         n, shift, offs, nb, gs, gd);
-    MPR_LAUNCH_CHECK();
-    std::swap(ka, kb);
-    std::swap(va, vb);
+      launch_multi<RadixScatter<K, 8>>(sc, nb, s);
+    for (size_t i = 0; i < count; ++i)
+      if (active[i]) {
+        std::swap(ka[i], kb[i]);
+        std::swap(va[i], vb[i]);
+      }
   }
-  if (result_keys && result_vals) {
-    *result_keys = ka;
-    *result_vals = va;
-  } else if (ka != keys) {
-    MPR_HIP(hipMemcpyAsync(keys, ka, sizeof(K) * n, hipMemcpyDeviceToDevice, s));
-    MPR_HIP(hipMemcpyAsync(vals, va, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
+  for (size_t i = 0; i < count; ++i) {
+    const SortJob<K>& j = jobs[i];
+    if (!active[i]) continue;
+    if (j.result_keys && j.result_vals) {
+      *j.result_keys = ka[i];
+      *j.result_vals = va[i];
+    } else if (ka[i] != j.keys) {
+      MPR_HIP(hipMemcpyAsync(j.keys, ka[i], sizeof(K) * j.n, hipMemcpyDeviceToDevice, s));
+      MPR_HIP(hipMemcpyAsync(j.vals, va[i], sizeof(uint32_t) * j.n, hipMemcpyDeviceToDevice, s));
+    }
   }
+}
+
+template <class K>
+void launch_radix_sort(K* keys, uint32_t* vals, K* k_tmp, uint32_t* v_tmp, uint32_t n, int bits,
+                       uint32_t* hist, uint32_t* offs, hipStream_t s, const double* gather_src,
+                       double* gather_dst, K** result_keys, uint32_t** result_vals) {
+  const SortJob<K> j{keys, vals, k_tmp, v_tmp, n, bits, hist, offs, gather_src, gather_dst,
+                     result_keys, result_vals};
+  launch_radix_sort_multi(&j, 1, s);
 }
 template void launch_radix_sort<uint32_t>(uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t,
                                           int, uint32_t*, uint32_t*, hipStream_t, const double*,
@@ -984,37 +1221,54 @@ SplatScratch splat_scratch_sizes(uint32_t P) {
   return b;
 }
 
+void launch_splat_multi(const SplatJob* jobs, size_t count, hipStream_t s) {
+  std::vector<SplatPairs::Args> pa(count);
+  std::vector<SortJob<uint32_t>> so(count);
+  std::vector<SplatSum1::Args> s1(count);
+  std::vector<SplatPass2::Args> p2(count);
+  std::vector<SplatSum2::Args> s2(count);
+  std::vector<uint32_t> nb(count, 0u), np(count, 0u);
+  std::vector<uint32_t*> keys(count, nullptr), ids(count, nullptr);
+  for (size_t i = 0; i < count; ++i) {
+    const SplatJob& j = jobs[i];
+    const uint32_t pairs = 8u * j.P;
+    if (pairs > j.b.pairs) throw HipError{"splat scratch smaller than the point count"};
+    // node keys and the sentinel M (sorts after every node) fit 32 bits: M <= 1025^3 < 2^31
+    const uint32_t M = (uint32_t)((uint64_t)j.n * j.n * j.n);
+    int bits = 1;
+    while ((1ull << bits) <= M) ++bits;
+    if (j.P) {
+      nb[i] = blocks_for(j.P, kThreads);
+      np[i] = blocks_for(pairs, kThreads);
+    }
+    pa[i] = {j.pts, j.nrm, j.P, j.n, j.confidence, j.b.keys, j.b.ids, j.b.w1, M, j.accepted};
+    // the sort's last pass also lays pass 1's weights out in the sorted order, in the space
+    // of pass 2's contributions (written after k_splat_sum1 has read them); the sorted
+    // pairs stay in whichever buffers the last pass wrote
+    so[i] = {j.b.keys, j.b.ids, j.b.ktmp, j.b.itmp, j.P ? pairs : 0u, bits, j.b.hist, j.b.offs,
+             j.b.w1, j.b.c2, &keys[i], &ids[i]};
+  }
+  launch_multi<SplatPairs>(pa, nb, s);
+  launch_radix_sort_multi(so.data(), count, s);
+  for (size_t i = 0; i < count; ++i) {
+    const SplatJob& j = jobs[i];
+    const uint32_t pairs = 8u * j.P, M = (uint32_t)((uint64_t)j.n * j.n * j.n);
+    s1[i] = {keys[i], j.b.c2, pairs, M, j.density, j.screen, j.touched};
+    p2[i] = {j.pts, j.nrm, j.P, j.n, j.confidence, j.density, j.b.c2, j.spn};
+    s2[i] = {keys[i], ids[i], j.b.c2, pairs, M, j.vx, j.vy, j.vz, j.touched ? 1 : 0};
+  }
+  launch_multi<SplatSum1>(s1, np, s);
+  launch_multi<SplatPass2>(p2, nb, s);
+  launch_multi<SplatSum2>(s2, np, s);
+}
+
 void launch_splat(const double* pts, const double* nrm, uint32_t P, int n, int confidence,
                   double* density, double* vx, double* vy, double* vz, double* screen,
                   double spn, unsigned long long* accepted, const SplatScratch& b,
                   uint32_t* touched, hipStream_t s) {
-  if (!P) return;
-  const uint32_t pairs = 8u * P;
-  if (pairs > b.pairs) throw HipError{"splat scratch smaller than the point count"};
-  // node keys and the sentinel M (sorts after every node) fit 32 bits: M <= 1025^3 < 2^31
-  const uint32_t M = (uint32_t)((uint64_t)n * n * n);
-  int bits = 1;
-  while ((1ull << bits) <= M) ++bits;
-  const unsigned nb = blocks_for(P, kThreads), np = blocks_for(pairs, kThreads);
-  hipLaunchKernelGGL(k_splat_pairs, dim3(nb), dim3(kThreads), 0, s, pts, nrm, P, n, confidence,
-                     b.keys, b.ids, b.w1, M, accepted);
-  MPR_LAUNCH_CHECK();
-  // the sort's last pass also lays pass 1's weights out in the sorted order, in the space
-  // of pass 2's contributions (written after k_splat_sum1 has read them); the sorted pairs
-  // stay in whichever buffers the last pass wrote
-  double* ws = b.c2;
-  uint32_t *keys = nullptr, *ids = nullptr;
-  launch_radix_sort(b.keys, b.ids, b.ktmp, b.itmp, pairs, bits, b.hist, b.offs, s, b.w1, ws,
-                    &keys, &ids);
-  hipLaunchKernelGGL(k_splat_sum1, dim3(np), dim3(kThreads), 0, s, keys, ws, pairs, M,
-                     density, screen, touched);
-  MPR_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_splat_pass2_pairs, dim3(nb), dim3(kThreads), 0, s, pts, nrm, P, n,
-                     confidence, density, b.c2, spn);
-  MPR_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_splat_sum2, dim3(np), dim3(kThreads), 0, s, keys, ids, b.c2, pairs, M,
-                     vx, vy, vz, touched ? 1 : 0);
-  MPR_LAUNCH_CHECK();
+  const SplatJob j{pts, nrm, P, n, confidence, density, vx, vy, vz, screen, spn, accepted, b,
+                   touched};
+  launch_splat_multi(&j, 1, s);
 }
 
 int iso_partial_count(uint32_t P) { return (int)std::max<uint32_t>(1u, blocks_for(P, kThreads)); }

@@ -205,6 +205,10 @@ struct Job {
   }
   // pipeline stages; all enqueue on `s` without host synchronisation
   void setup(const mpr_params& p, hipStream_t s);
+  // the setups of several jobs, one launch per step for all of them (multi.h); each job's
+  // results are those of its own setup
+  static void setup_all(Job* const* jobs, const mpr_params* const* params, size_t count,
+                        hipStream_t s);
   void finish(const mpr_params& p, hipStream_t s);
   void enqueue_extract(bool world, hipStream_t s);
   // the parameters of the work last enqueued or launched on this job (eagerly or as a graph)
@@ -439,7 +443,11 @@ void Batch::build() {
 }
 
 void Batch::enqueue(const mpr_params* p) {
-  fork_join(0, [&](size_t i, hipStream_t q) { jobs[i]->setup(p[i], q); });
+  {
+    std::vector<const mpr_params*> pp(jobs.size());
+    for (size_t i = 0; i < jobs.size(); ++i) pp[i] = &p[i];
+    Job::setup_all(jobs.data(), pp.data(), jobs.size(), stream);
+  }
   if (p[0].stages & MPR_STAGE_SOLVE)
     run_cycles(vc64, precision == MPR_PRECISION_MIXED ? &vc32 : nullptr, &dtab, p[0], stream);
   fork_join(1, [&](size_t i, hipStream_t q) {
@@ -600,46 +608,102 @@ void Job::enqueue_extract(bool world, hipStream_t s) {
 
 // normalise, splat, assemble, screening pyramid and initial guess (core.py:229-273)
 void Job::setup(const mpr_params& p, hipStream_t s) {
-  const uint64_t M = (uint64_t)n * n * n;
-  if (p.stages & MPR_STAGE_FILTER)
-    launch_filter_normalize(pts, nrm, P, p.scale, filt, &d_scalars->valid, d_params, unit,
-                            nrm_kept, s);
-  else if (p.stages & MPR_STAGE_NORMALIZE)
-    launch_normalize(pts, P, p.scale, bbox_part, d_params, unit, s);
-  if (!(p.stages & MPR_STAGE_SOLVE)) return;
-  const double* upts = unit_points(p);
+  Job* self = this;
+  const mpr_params* pp = &p;
+  setup_all(&self, &pp, 1, s);
+}
+
+void Job::setup_all(Job* const* jobs, const mpr_params* const* params, size_t count,
+                    hipStream_t s) {
+  std::vector<FilterJob> filter;
+  std::vector<Job*> solve;
+  std::vector<const mpr_params*> sp;
+  for (size_t i = 0; i < count; ++i) {
+    Job& j = *jobs[i];
+    const mpr_params& p = *params[i];
+    if (p.stages & MPR_STAGE_FILTER)
+      filter.push_back({j.pts, j.nrm, j.P, p.scale, j.filt, &j.d_scalars->valid, j.d_params,
+                        j.unit, j.nrm_kept});
+    else if (p.stages & MPR_STAGE_NORMALIZE)
+      launch_normalize(j.pts, j.P, p.scale, j.bbox_part, j.d_params, j.unit, s);
+    if (p.stages & MPR_STAGE_SOLVE) {
+      solve.push_back(&j);
+      sp.push_back(&p);
+    }
+  }
+  if (!filter.empty()) launch_filter_normalize_multi(filter.data(), filter.size(), s);
+  const size_t m = solve.size();
+  if (!m) return;
   // the splat grids are not zero-filled: the splat marks the nodes it writes and the
   // assembly and the extraction read every other node as zero
-  launch_fill<uint32_t>(touched, 0u, (M + 31) / 32, s);
-  launch_fill<unsigned long long>(d_accepted, 0ull, 1, s);
-  launch_splat(upts, normals(p), P, n, p.confidence, density, vx, vy, vz, screen,
-               p.samples_per_node, d_accepted, splat, touched, s);
-  const bool grids = (p.stages & MPR_STAGE_GRIDS) != 0;
-  const bool mixed = precision == MPR_PRECISION_MIXED;
-  // Without MPR_STAGE_GRIDS the assembly writes only the nodes near this run's and the
-  // previous run's splat (the rest of the right-hand side and screening weights stay +0.0)
-  // and leaves the density raw; with it, every grid is written in natural layout too. The
-  // FP32 screening weights of the mixed V-cycle are written beside the FP64 ones.
-  // The live-slot bits let the finest level's kernels skip loading the +0.0 rhs and screen
-  // words away from the splat.
-  const TouchedHistory hist{prev_touched, need_bits, grids, chunk_list, chunk_count};
-  launch_fill<uint32_t>(live_bits, 0u, live_words(), s);
-  launch_assemble(vx, vy, vz, screen, grids ? rhs : nullptr, L64[0].rhs, L64[0].screen,
-                  grids ? density : nullptr, p.samples_per_node, geoms[0], p.point_weight, 0.5,
-                  touched, grids ? screen : nullptr, mixed ? L32[0].screen : nullptr, &hist,
-                  live_bits, s);
-  adopt(p);  // the extraction enqueued next reads the density as this assembly leaves it
-  for (size_t l = 1; l < L64.size(); ++l)
-    launch_restrict<double>(L64[l - 1].screen, geoms[l - 1], L64[l].screen, geoms[l], 4.0, s,
-                            mixed ? L32[l].screen : nullptr);
+  std::vector<FillJob<uint32_t>> touched(m), live(m);
+  std::vector<FillJob<unsigned long long>> accepted(m);
+  std::vector<SplatJob> splat(m);
+  std::vector<AssembleJob> assemble(m);
+  std::vector<TouchedHistory> hist(m);
+  size_t levels = 0;
+  for (size_t i = 0; i < m; ++i) {
+    Job& j = *solve[i];
+    const mpr_params& p = *sp[i];
+    const uint64_t M = (uint64_t)j.n * j.n * j.n;
+    const bool grids = (p.stages & MPR_STAGE_GRIDS) != 0;
+    const bool mixed = j.precision == MPR_PRECISION_MIXED;
+    touched[i] = {j.touched, 0u, (M + 31) / 32};
+    accepted[i] = {j.d_accepted, 0ull, 1};
+    splat[i] = {j.unit_points(p), j.normals(p), j.P, j.n, p.confidence, j.density, j.vx, j.vy,
+                j.vz, j.screen, p.samples_per_node, j.d_accepted, j.splat, j.touched};
+    // Without MPR_STAGE_GRIDS the assembly writes only the nodes near this run's and the
+    // previous run's splat (the rest of the right-hand side and screening weights stay
+    // +0.0) and leaves the density raw; with it, every grid is written in natural layout
+    // too. The FP32 screening weights of the mixed V-cycle are written beside the FP64
+    // ones. The live-slot bits let the finest level's kernels skip loading the +0.0 rhs and
+    // screen words away from the splat.
+    hist[i] = {j.prev_touched, j.need_bits, grids, j.chunk_list, j.chunk_count};
+    live[i] = {j.live_bits, 0u, j.live_words()};
+    assemble[i] = {j.vx, j.vy, j.vz, j.screen, grids ? j.rhs : nullptr, j.L64[0].rhs,
+                   j.L64[0].screen, grids ? j.density : nullptr, p.samples_per_node, j.geoms[0],
+                   p.point_weight, 0.5, j.touched, grids ? j.screen : nullptr,
+                   mixed ? j.L32[0].screen : nullptr, &hist[i], j.live_bits};
+    levels = std::max(levels, j.L64.size());
+  }
+  launch_fill_multi(touched.data(), m, s);
+  launch_fill_multi(accepted.data(), m, s);
+  launch_splat_multi(splat.data(), m, s);
+  launch_fill_multi(live.data(), m, s);
+  launch_assemble_multi(assemble.data(), m, s);
+  for (size_t i = 0; i < m; ++i)
+    solve[i]->adopt(*sp[i]);  // the extraction enqueued next reads the density as assembled
+  // screening pyramid: level l of every job that has one, in one launch
+  std::vector<RestrictJob<double>> pyr(m);
+  for (size_t l = 1; l < levels; ++l) {
+    for (size_t i = 0; i < m; ++i) {
+      Job& j = *solve[i];
+      const bool mixed = j.precision == MPR_PRECISION_MIXED;
+      if (l < j.L64.size())
+        pyr[i] = {j.L64[l - 1].screen, j.geoms[l - 1], j.L64[l].screen, j.geoms[l], 4.0,
+                  mixed ? j.L32[l].screen : nullptr};
+      else
+        pyr[i] = {nullptr, Geom{}, nullptr, Geom{}, 4.0, nullptr};
+    }
+    launch_restrict_nodes_multi(pyr.data(), m, s);
+  }
   // initial guess 0.5 (core.py:270); no samples: the solution is 0 (core.py:286-287), which
   // the V-cycles then leave at 0 (zero right-hand side and screening). When the first
   // mixed-precision cycle's defect and fused correction cover this job, they read the
   // guess as the constant and write every node of x, so x is not filled.
-  const bool implicit = mixed && P && p.cycles > 0 && p.post_smooth > 0 &&
-                        depth > mixed_unfused_class();
-  launch_fill<int>(x_initial, implicit ? 1 : 0, 1, s);
-  if (!implicit) launch_fill<double>(L64[0].x, P ? kInitialX : 0.0, 2ull * geoms[0].C, s);
+  std::vector<FillJob<int32_t>> flag(m);
+  std::vector<FillJob<double>> x(m);
+  for (size_t i = 0; i < m; ++i) {
+    Job& j = *solve[i];
+    const mpr_params& p = *sp[i];
+    const bool mixed = j.precision == MPR_PRECISION_MIXED;
+    const bool implicit = mixed && j.P && p.cycles > 0 && p.post_smooth > 0 &&
+                          j.depth > mixed_unfused_class();
+    flag[i] = {j.x_initial, implicit ? 1 : 0, 1};
+    x[i] = {j.L64[0].x, j.P ? kInitialX : 0.0, implicit ? 0 : 2ull * j.geoms[0].C};
+  }
+  launch_fill_multi(flag.data(), m, s);
+  launch_fill_multi(x.data(), m, s);
 }
 
 // final FP64 residual norm, natural-layout field and iso value (core.py:289-303)
