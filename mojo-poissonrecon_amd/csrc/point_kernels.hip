@@ -618,7 +618,8 @@ __device__ __forceinline__ void splat_sum1_body(Blk blk_, const uint32_t* keys, 
   screen[key] = sc;
 }
 
-// pass 2 contributions (capi.mojo:176-223): n^ * (w (n-1) / factor) for each corner
+// pass 2 (capi.mojo:176-223), per point: the density factor and what each corner's
+// contribution n^ * (w (n-1) / factor) is made of
 __device__ __forceinline__ void splat_pass2_body(Blk blk_, const double* pts,
                                                                 const double* nrm, uint32_t P,
                                                                 int n, int confidence,
@@ -657,72 +658,72 @@ __device__ __forceinline__ void splat_pass2_body(Blk blk_, const double* pts,
   }
   const double q = local / spn;
   const double factor = q > 1.0 ? q : 1.0;
-  double* out = c2 + 24ull * p;
-  int k = 0;
-  for (int dz = 0; dz < 2; ++dz) {
-    const double wz = dz ? fz : 1.0 - fz;
-    for (int dy = 0; dy < 2; ++dy) {
-      const double wy = dy ? fy : 1.0 - fy;
-      for (int dx = 0; dx < 2; ++dx, ++k) {
-        const double wx = dx ? fx : 1.0 - fx;
-        const double w = wx * wy * wz * sw;
-        const double vw = w * (double)res / factor;
-        out[3 * k] = nx * vw;
-        out[3 * k + 1] = ny * vw;
-        out[3 * k + 2] = nz * vw;
-      }
-    }
-  }
+  // the point's record: pass 2's clamped fractions, weight, unit normal and density factor,
+  // from which each of its 8 contributions is formed where it is summed (splat_sum2_body)
+  double* rec = c2 + 8ull * p;
+  rec[0] = fx;
+  rec[1] = fy;
+  rec[2] = fz;
+  rec[3] = sw;
+  rec[4] = nx;
+  rec[5] = ny;
+  rec[6] = nz;
+  rec[7] = factor;
 }
 
 // one lane per node: v -= n^ vw over its contributions in point order (from +0.0 when the
 // grids count as zero-filled, as k_splat_sum1)
 __device__ __forceinline__ void splat_sum2_body(Blk blk_, const uint32_t* keys,
-                                                         const uint32_t* ids, const double* c2,
-                                                         uint32_t count, uint32_t M, double* vx,
-                                                         double* vy, double* vz, int from_zero) {
+                                                const uint32_t* ids, const double* rec,
+                                                uint32_t count, uint32_t M, int n, double* vx,
+                                                double* vy, double* vz, int from_zero) {
   const uint32_t j = blk_.id * kThreads + threadIdx.x;
   const RunStart r = run_start(keys, j, count, M);
   if (!r.head) return;
   const uint32_t key = r.key;
+  const double res = (double)(n - 1);
   double a = 0.0, b = 0.0, c = 0.0;
   if (!from_zero) {
     a = vx[key];
     b = vy[key];
     c = vz[key];
   }
+  // pair id 8 p + k, k = 4 dz + 2 dy + dx (splat_pairs_body's corner order); the
+  // contribution is pass 2's w = wx wy wz sw, vw = w (n-1) / factor, n^ vw
+  using D2 = double __attribute__((ext_vector_type(2)));
+  struct Rec {
+    D2 v[4];  // fx fy, fz sw, nx ny, nz factor
+  };
+  auto load = [&](uint32_t id) {
+    const D2* q = reinterpret_cast<const D2*>(rec + 8ull * (id >> 3));
+    return Rec{{q[0], q[1], q[2], q[3]}};
+  };
+  auto sub = [&](const Rec& q, uint32_t id) {
+    const uint32_t k = id & 7u;
+    const double fx = q.v[0][0], fy = q.v[0][1], fz = q.v[1][0], sw = q.v[1][1];
+    const double wz = (k & 4u) ? fz : 1.0 - fz;
+    const double wy = (k & 2u) ? fy : 1.0 - fy;
+    const double wx = (k & 1u) ? fx : 1.0 - fx;
+    const double w = wx * wy * wz * sw;
+    const double vw = w * res / q.v[3][1];
+    a -= q.v[2][0] * vw;
+    b -= q.v[2][1] * vw;
+    c -= q.v[3][0] * vw;
+  };
   uint32_t i = j;
-  for (; i + 4 <= r.end; i += 4) {
-    const double* q[4];
+  for (; i + 4 <= r.end; i += 4) {  // four records in flight, summed in order
+    uint32_t id[4];
+    Rec q[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) q[k] = c2 + 3ull * ids[i + k];
-    double v[4][3];
+    for (int t = 0; t < 4; ++t) id[t] = ids[i + t];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      v[k][0] = q[k][0];
-      v[k][1] = q[k][1];
-      v[k][2] = q[k][2];
-    }
+    for (int t = 0; t < 4; ++t) q[t] = load(id[t]);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      a -= v[k][0];
-      b -= v[k][1];
-      c -= v[k][2];
-    }
+    for (int t = 0; t < 4; ++t) sub(q[t], id[t]);
   }
-  for (; i < r.end; ++i) {
-    const double* q = c2 + 3ull * ids[i];
-    a -= q[0];
-    b -= q[1];
-    c -= q[2];
-  }
+  for (; i < r.end; ++i) sub(load(ids[i]), ids[i]);
   if (r.open)
-    for (; i < count && keys[i] == key; ++i) {
-      const double* q = c2 + 3ull * ids[i];
-      a -= q[0];
-      b -= q[1];
-      c -= q[2];
-    }
+    for (; i < count && keys[i] == key; ++i) sub(load(ids[i]), ids[i]);
   vx[key] = a;
   vy[key] = b;
   vz[key] = c;
@@ -961,14 +962,15 @@ struct SplatPass2 {
 struct SplatSum2 {
   struct Args {
     const uint32_t *keys, *ids;
-    const double* c2;
+    const double* rec;
     uint32_t count, M;
+    int n;
     double *vx, *vy, *vz;
     int from_zero;
   };
   static constexpr int kThreads = mpr::kThreads;
   static __device__ void run(const Args& a, Blk b) {
-    splat_sum2_body(b, a.keys, a.ids, a.c2, a.count, a.M, a.vx, a.vy, a.vz, a.from_zero);
+    splat_sum2_body(b, a.keys, a.ids, a.rec, a.count, a.M, a.n, a.vx, a.vy, a.vz, a.from_zero);
   }
 };
 
@@ -1254,8 +1256,11 @@ void launch_splat_multi(const SplatJob* jobs, size_t count, hipStream_t s) {
     const SplatJob& j = jobs[i];
     const uint32_t pairs = 8u * j.P, M = (uint32_t)((uint64_t)j.n * j.n * j.n);
     s1[i] = {keys[i], j.b.c2, pairs, M, j.density, j.screen, j.touched};
-    p2[i] = {j.pts, j.nrm, j.P, j.n, j.confidence, j.density, j.b.c2, j.spn};
-    s2[i] = {keys[i], ids[i], j.b.c2, pairs, M, j.vx, j.vy, j.vz, j.touched ? 1 : 0};
+    // per-point records of pass 2 behind the sorted pass-1 weights (8 P <= 2 pairs words of
+    // the 3 pairs c2 holds)
+    double* rec = j.b.c2 + pairs;
+    p2[i] = {j.pts, j.nrm, j.P, j.n, j.confidence, j.density, rec, j.spn};
+    s2[i] = {keys[i], ids[i], rec, pairs, M, j.n, j.vx, j.vy, j.vz, j.touched ? 1 : 0};
   }
   launch_multi<SplatSum1>(s1, np, s);
   launch_multi<SplatPass2>(p2, nb, s);
