@@ -5,6 +5,24 @@
 
 namespace mpr {
 
+// Sets bit index & 31 of words[index >> 5] for the lanes with `set`, one atomicOr per
+// distinct word of the wavefront (one atomic per lane would serialise the lanes that share
+// a word in the L2). Every lane of the wavefront calls it.
+__device__ __forceinline__ void wave_set_bits(uint32_t* words, bool set, uint32_t index) {
+  const uint32_t word = index >> 5, bit = 1u << (index & 31u);
+  uint64_t pending = __ballot(set);
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const uint32_t w = (uint32_t)__shfl((int)word, leader);
+    const bool mine = set && word == w;
+    uint32_t v = mine ? bit : 0u;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v |= (uint32_t)__shfl_xor((int)v, o);
+    if ((int)(threadIdx.x & 63u) == leader) atomicOr(&words[w], v);
+    pending &= ~__ballot(mine);
+  }
+}
+
 // Exclusive prefix of `flag` over the 256 threads of a block in thread order, using one
 // ballot per wavefront and the 4 wave totals in LDS. `total` receives the block's count.
 // Every thread of the block must call it (it synchronises twice).

@@ -9,6 +9,7 @@
 // contiguous slab of planes and keep the z-1/z+1 neighbour planes in that XCD's L2).
 #include <algorithm>
 
+#include "block_scan.h"
 #include "mpr_internal.h"
 #include "multi.h"
 #include "stencil.h"
@@ -244,11 +245,14 @@ __global__ __launch_bounds__(kThreads) void k_add_correction(double* X, const fl
   for (; i < count; i += stride) X[i] += (double)E[i];
 }
 
-// value of a splat grid at node i: unset `touched` bits mean the node received nothing
+// value of a splat grid at node i: unset `touched` bits mean the node received nothing.
+// The word is loaded whatever the bit (a node that received nothing holds a stale value,
+// which the bit then replaces by +0.0), so the two loads are in flight together.
 __device__ __forceinline__ double splat_value(const double* v, const uint32_t* touched,
                                               uint64_t i) {
-  if (touched && !((touched[i >> 5] >> (i & 31u)) & 1u)) return 0.0;
-  return v[i];
+  const double val = v[i];
+  if (!touched) return val;
+  return ((touched[i >> 5] >> (i & 31u)) & 1u) ? val : 0.0;
 }
 
 // 32 node bits starting at node b of the union of two bit sets (nodes outside [0, 32 words)
@@ -300,24 +304,6 @@ struct AssembleArgs {
   uint32_t* live;
   double spn, pw, target;
 };
-
-// Sets the live bits of a wavefront's lanes with one atomicOr per distinct word (the lanes
-// of a row chunk cover at most a few words; one atomic per lane would serialise up to 32
-// per word in the L2). Every lane of the wavefront calls it.
-__device__ __forceinline__ void wave_set_bits(uint32_t* words, bool set, uint32_t index) {
-  const uint32_t word = index >> 5, bit = 1u << (index & 31u);
-  uint64_t pending = __ballot(set);
-  while (pending) {
-    const int leader = __ffsll((unsigned long long)pending) - 1;
-    const uint32_t w = (uint32_t)__shfl((int)word, leader);
-    const bool mine = set && word == w;
-    uint32_t v = mine ? bit : 0u;
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v |= (uint32_t)__shfl_xor((int)v, o);
-    if ((int)(threadIdx.x & 63u) == leader) atomicOr(&words[w], v);
-    pending &= ~__ballot(mine);
-  }
-}
 
 // assembly of node t = (x, y, z); returns whether its split slot (si) is live (rhs or
 // screen not +0.0, by bit pattern, so -0.0 counts as live); the callers set the live bits
@@ -397,9 +383,9 @@ __device__ __forceinline__ void need_chunks_body(Blk blk_, const uint32_t* need,
 // Sparse assembly, step 2: one wavefront per listed chunk (a lane per node), for every
 // listed chunk: nodes of a chunk outside the need mask get the assembly's own value, +0.0
 __device__ __forceinline__ void assemble_chunks_body(Blk blk_, AssembleArgs a, Geom g,
-                                                              uint32_t per_row,
-                                                              const uint32_t* list,
-                                                              const uint32_t* count) {
+                                                     uint32_t per_row, const uint32_t* list,
+                                                     const uint32_t* count,
+                                                     const uint32_t* need) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t waves = blk_.count * (kThreads / 64);
   const uint32_t total = *count;
@@ -410,13 +396,95 @@ __device__ __forceinline__ void assemble_chunks_body(Blk blk_, AssembleArgs a, G
     const int x = (int)(64u * k + lane);
     bool live = false;
     uint32_t si = 0u;
-    if (x < n) {
+    // a node outside the need mask has no touched neighbour now or in the previous run, so
+    // its assembled words are the +0.0 they already hold: it is skipped
+    const uint32_t t = zy * (uint32_t)n + (uint32_t)x;
+    if (x < n && ((need[t >> 5] >> (t & 31u)) & 1u)) {
       const int z = (int)(zy / (uint32_t)n);
       const int y = (int)(zy - (uint32_t)z * (uint32_t)n);
-      live = assemble_node(a, g, zy * (uint32_t)n + (uint32_t)x, x, y, z, si);
+      live = assemble_node(a, g, t, x, y, z, si);
     }
     if (a.live) wave_set_bits(a.live, live, si);  // i is uniform over the wavefront
   }
+}
+
+// 64 bits of a bit set starting at bit `pos`, those at or past `end` cleared
+__device__ __forceinline__ uint64_t bits64_at(const uint32_t* b, uint64_t pos, uint64_t end,
+                                              uint64_t words) {
+  const uint64_t q = pos >> 5;
+  const uint32_t sh = (uint32_t)(pos & 31u);
+  const uint64_t w0 = b[q], w1 = q + 1 < words ? b[q + 1] : 0u, w2 = q + 2 < words ? b[q + 2] : 0u;
+  uint64_t v = (w0 | (w1 << 32)) >> sh;
+  if (sh) v |= w2 << (64u - sh);
+  if (end <= pos) return 0u;
+  if (end - pos < 64u) v &= (1ull << (end - pos)) - 1ull;
+  return v;
+}
+
+// the even-position bits of x, packed into the low 32 bits
+__device__ __forceinline__ uint32_t even_bits(uint64_t x) {
+  x &= 0x5555555555555555ull;
+  x = (x | (x >> 1)) & 0x3333333333333333ull;
+  x = (x | (x >> 2)) & 0x0f0f0f0f0f0f0f0full;
+  x = (x | (x >> 4)) & 0x00ff00ff00ff00ffull;
+  x = (x | (x >> 8)) & 0x0000ffff0000ffffull;
+  x = (x | (x >> 16)) & 0x00000000ffffffffull;
+  return (uint32_t)x;
+}
+
+// Coarse "support" bits: coarse node (X,Y,Z) is set when a fine node of its 27-node
+// restriction support (2X+ox, 2Y+oy, 2Z+oz, in the grid) is set in `fine` (natural layout,
+// nf^3 bits). One lane per word of the coarse bit set, which holds cw = ceil(nc / 32) words
+// per coarse row: coarse x 32 w .. 32 w + 31 of row (Z, Y) take fine x 64 w - 1 .. 64 w + 63.
+__device__ __forceinline__ void coarse_support_body(Blk blk_, const uint32_t* fine, int nf,
+                                                    int nc, uint32_t cw, uint32_t* out) {
+  const uint64_t q = (uint64_t)blk_.id * kThreads + threadIdx.x;
+  const uint64_t total = (uint64_t)nc * nc * cw;
+  if (q >= total) return;
+  const uint32_t w = (uint32_t)(q % cw);
+  const uint64_t zy = q / cw;
+  const int Z = (int)(zy / (uint64_t)nc), Y = (int)(zy % (uint64_t)nc);
+  const uint64_t words = ((uint64_t)nf * nf * nf + 31) / 32;
+  uint32_t acc = 0;
+  for (int oz = -1; oz <= 1; ++oz) {
+    const int zz = 2 * Z + oz;
+    if (zz < 0 || zz >= nf) continue;
+    for (int oy = -1; oy <= 1; ++oy) {
+      const int yy = 2 * Y + oy;
+      if (yy < 0 || yy >= nf) continue;
+      const uint64_t base = ((uint64_t)zz * nf + yy) * nf, x0 = 64ull * w;
+      if (x0 >= (uint64_t)nf) continue;
+      const uint64_t v = bits64_at(fine, base + x0, base + nf, words);
+      uint32_t before = 0;  // fine x = 64 w - 1, the x - 1 neighbour of coarse x = 32 w
+      if (w > 0) before = (fine[(base + x0 - 1) >> 5] >> ((base + x0 - 1) & 31u)) & 1u;
+      const uint32_t e = even_bits(v), o = even_bits(v >> 1);
+      acc |= e | o | (o << 1) | before;
+    }
+  }
+  const uint32_t x_first = 32u * w;
+  if ((uint32_t)nc - x_first < 32u) acc &= (1u << ((uint32_t)nc - x_first)) - 1u;
+  out[q] = acc;
+}
+
+// restrict_nodes_body over the coarse nodes set in `support` (coarse_support_body of the
+// fine nodes that this or the previous run may have written); every other coarse node keeps
+// the +0.0 it holds: the restriction of +0.0 words
+template <class T>
+__device__ __forceinline__ void restrict_sparse_body(Blk blk_, const T* F, Geom gf, T* Cc, Geom gc,
+                                                     T scale, float* copy32,
+                                                     const uint32_t* support) {
+  const uint32_t t = xcd_block(blk_) * kThreads + threadIdx.x;
+  if (t >= 2u * gc.C) return;
+  const int c = (int)(t & 1u);
+  const uint32_t i = (uint32_t)c * gc.C + (t >> 1);
+  int X, Y, Z;
+  if (!split_decode(gc, c, t >> 1, X, Y, Z)) return;
+  const uint32_t cw = ((uint32_t)gc.n + 31u) / 32u;
+  const uint32_t word = support[((uint32_t)Z * (uint32_t)gc.n + (uint32_t)Y) * cw + ((uint32_t)X >> 5)];
+  if (!((word >> ((uint32_t)X & 31u)) & 1u)) return;
+  const T v = restrict_node(F, gf, X, Y, Z, scale);
+  Cc[i] = v;
+  if (copy32) copy32[i] = (float)v;
 }
 
 // ---- multi-instance kernel bodies (multi.h) ---------------------------------------------
@@ -444,6 +512,36 @@ struct RestrictNodes {
   static constexpr int kThreads = mpr::kThreads;
   static __device__ void run(const Args& a, Blk b) {
     restrict_nodes_body<T>(b, a.F, a.gf, a.Cc, a.gc, a.scale, a.copy32);
+  }
+};
+
+template <class T>
+struct RestrictSparse {
+  struct Args {
+    const T* F;
+    Geom gf;
+    T* Cc;
+    Geom gc;
+    T scale;
+    float* copy32;
+    const uint32_t* support;
+  };
+  static constexpr int kThreads = mpr::kThreads;
+  static __device__ void run(const Args& a, Blk b) {
+    restrict_sparse_body<T>(b, a.F, a.gf, a.Cc, a.gc, a.scale, a.copy32, a.support);
+  }
+};
+
+struct CoarseSupport {
+  struct Args {
+    const uint32_t* fine;
+    int nf, nc;
+    uint32_t cw;
+    uint32_t* out;
+  };
+  static constexpr int kThreads = mpr::kThreads;
+  static __device__ void run(const Args& a, Blk b) {
+    coarse_support_body(b, a.fine, a.nf, a.nc, a.cw, a.out);
   }
 };
 
@@ -480,11 +578,11 @@ struct AssembleChunks {
     AssembleArgs a;
     Geom g;
     uint32_t per_row;
-    const uint32_t *list, *count;
+    const uint32_t *list, *count, *need;
   };
   static constexpr int kThreads = mpr::kThreads;
   static __device__ void run(const Args& a, Blk b) {
-    assemble_chunks_body(b, a.a, a.g, a.per_row, a.list, a.count);
+    assemble_chunks_body(b, a.a, a.g, a.per_row, a.list, a.count, a.need);
   }
 };
 
@@ -587,20 +685,34 @@ void launch_restrict(const T* F, const Geom& gf, T* Cc, const Geom& gc, double s
     MPR_LAUNCH_CHECK();
     return;
   }
-  const RestrictJob<T> j{F, gf, Cc, gc, scale, copy32};
+  const RestrictJob<T> j{F, gf, Cc, gc, scale, copy32, nullptr, nullptr};
   launch_restrict_nodes_multi(&j, 1, s);
 }
 
 template <class T>
 void launch_restrict_nodes_multi(const RestrictJob<T>* jobs, size_t count, hipStream_t s) {
   std::vector<typename RestrictNodes<T>::Args> a(count);
-  std::vector<uint32_t> nb(count);
+  std::vector<typename RestrictSparse<T>::Args> sp(count);
+  std::vector<CoarseSupport::Args> cs(count);
+  std::vector<uint32_t> nb(count, 0u), nbs(count, 0u), nbc(count, 0u);
   for (size_t i = 0; i < count; ++i) {
     const RestrictJob<T>& j = jobs[i];
-    a[i] = {j.F, j.gf, j.Cc, j.gc, (T)j.scale, j.copy32};
-    nb[i] = j.F ? xcd_grid(2ull * j.gc.C, kThreads) : 0u;
+    if (!j.F) continue;
+    const uint32_t blocks = xcd_grid(2ull * j.gc.C, kThreads);
+    if (j.need) {
+      const uint32_t cw = ((uint32_t)j.gc.n + 31u) / 32u;
+      cs[i] = {j.need, j.gf.n, j.gc.n, cw, j.support};
+      nbc[i] = blocks_for((uint64_t)j.gc.n * j.gc.n * cw, kThreads);
+      sp[i] = {j.F, j.gf, j.Cc, j.gc, (T)j.scale, j.copy32, j.support};
+      nbs[i] = blocks;
+    } else {
+      a[i] = {j.F, j.gf, j.Cc, j.gc, (T)j.scale, j.copy32};
+      nb[i] = blocks;
+    }
   }
   launch_multi<RestrictNodes<T>>(a, nb, s);
+  launch_multi<CoarseSupport>(cs, nbc, s);
+  launch_multi<RestrictSparse<T>>(sp, nbs, s);
 }
 
 template <class T>
@@ -686,7 +798,7 @@ void launch_assemble_multi(const AssembleJob* jobs, size_t count, hipStream_t s)
     nc[i] = {hist->need, j.touched, hist->prev, words, g.n, per_row, chunks, hist->list,
              hist->count};
     nb_need[i] = blocks_for(chunks, kThreads);
-    ac[i] = {a, g, per_row, hist->list, hist->count};
+    ac[i] = {a, g, per_row, hist->list, hist->count, hist->need};
     // a fixed grid (graph-capturable) whose wavefronts walk the chunk list
     nb_asm[i] = std::min<unsigned>(blocks_for(chunks, kThreads / 64), 4096u);
     any_sparse = true;

@@ -28,7 +28,11 @@ struct FillJob {
 };
 template <class T> void launch_fill_multi(const FillJob<T>* jobs, size_t count, hipStream_t s);
 // node-by-node restriction of several grids in one launch (launch_restrict's scalar form);
-// a job with F null is skipped
+// a job with F null is skipped. With `need` (a natural-layout bit set of the fine level)
+// only the coarse nodes whose restriction support holds one of its nodes are written: the
+// caller guarantees that the fine words outside it are +0.0 and that the coarse words
+// outside its support already hold +0.0 (the restriction of +0.0 words). `support` then
+// holds coarse_support_words(gc.n) words of scratch.
 template <class T>
 struct RestrictJob {
   const T* F;
@@ -37,7 +41,12 @@ struct RestrictJob {
   Geom gc;
   double scale;
   float* copy32;
+  const uint32_t* need;
+  uint32_t* support;
 };
+inline uint64_t coarse_support_words(int nc) {
+  return (uint64_t)nc * nc * (((uint64_t)nc + 31) / 32);
+}
 template <class T>
 void launch_restrict_nodes_multi(const RestrictJob<T>* jobs, size_t count, hipStream_t s);
 template <class T>

@@ -572,39 +572,31 @@ __device__ __forceinline__ RunStart run_start(const uint32_t* keys, uint32_t j, 
   return r;
 }
 
-// one lane per node: density += 2 w and screen += w over its contributions, in point order
-// (ws: pass 1's weights in the sorted order). With `touched`, the grids count as
-// zero-filled (the sums start at +0.0, as on the reference's fresh grids) and the node's bit
-// is set instead of reading the grid.
+// density += 2 w and screen += w over each node's contributions, in point order (ws: pass
+// 1's weights in the sorted order). Every lane loads its own position's weight; the lane
+// that starts a run then adds the run's weights from LDS in order (a run that continues
+// past the wavefront goes on with loads). With `touched`, the grids count as zero-filled
+// (the sums start at +0.0, as on the reference's fresh grids) and the node's bit is set
+// instead of reading the grid.
 __device__ __forceinline__ void splat_sum1_body(Blk blk_, const uint32_t* keys, const double* ws,
-                                                         uint32_t count, uint32_t M,
-                                                         double* density, double* screen,
-                                                         uint32_t* touched) {
+                                                uint32_t count, uint32_t M, double* density,
+                                                double* screen, uint32_t* touched) {
+  __shared__ double lw[kThreads];
   const uint32_t j = blk_.id * kThreads + threadIdx.x;
   const RunStart r = run_start(keys, j, count, M);
+  lw[threadIdx.x] = j < count ? ws[j] : 0.0;
+  if (touched) wave_set_bits(touched, r.head, r.key);
+  __syncthreads();
   if (!r.head) return;
   const uint32_t key = r.key;
   double d = 0.0, sc = 0.0;
-  if (touched) {
-    atomicOr(&touched[key >> 5], 1u << (key & 31u));
-  } else {
+  if (!touched) {
     d = density[key];
     sc = screen[key];
   }
   uint32_t i = j;
-  for (; i + 4 <= r.end; i += 4) {
-    const double w0 = ws[i], w1 = ws[i + 1], w2 = ws[i + 2], w3 = ws[i + 3];
-    d += 2.0 * w0;
-    sc += w0;
-    d += 2.0 * w1;
-    sc += w1;
-    d += 2.0 * w2;
-    sc += w2;
-    d += 2.0 * w3;
-    sc += w3;
-  }
   for (; i < r.end; ++i) {
-    const double w = ws[i];
+    const double w = lw[threadIdx.x + (i - j)];
     d += 2.0 * w;
     sc += w;
   }
@@ -671,62 +663,66 @@ __device__ __forceinline__ void splat_pass2_body(Blk blk_, const double* pts,
   rec[7] = factor;
 }
 
-// one lane per node: v -= n^ vw over its contributions in point order (from +0.0 when the
-// grids count as zero-filled, as k_splat_sum1)
+// v -= n^ vw over each node's contributions in point order (from +0.0 when the grids count
+// as zero-filled, as splat_sum1_body). Every lane forms its own position's contribution
+// from its point's record (pair id 8 p + k, k = 4 dz + 2 dy + dx, splat_pairs_body's corner
+// order; pass 2's w = wx wy wz sw, vw = w (n-1) / factor); the lane that starts a run then
+// subtracts the run's contributions from LDS in order.
+__device__ __forceinline__ void splat_contribution(const double* rec, uint32_t id, double res,
+                                                   double (&c)[3]) {
+  using D2 = double __attribute__((ext_vector_type(2)));
+  const D2* q = reinterpret_cast<const D2*>(rec + 8ull * (id >> 3));
+  const D2 f01 = q[0], f2s = q[1], n01 = q[2], n2f = q[3];  // fx fy, fz sw, nx ny, nz factor
+  const uint32_t k = id & 7u;
+  const double wz = (k & 4u) ? f2s[0] : 1.0 - f2s[0];
+  const double wy = (k & 2u) ? f01[1] : 1.0 - f01[1];
+  const double wx = (k & 1u) ? f01[0] : 1.0 - f01[0];
+  const double w = wx * wy * wz * f2s[1];
+  const double vw = w * res / n2f[1];
+  c[0] = n01[0] * vw;
+  c[1] = n01[1] * vw;
+  c[2] = n2f[0] * vw;
+}
+
 __device__ __forceinline__ void splat_sum2_body(Blk blk_, const uint32_t* keys,
                                                 const uint32_t* ids, const double* rec,
                                                 uint32_t count, uint32_t M, int n, double* vx,
                                                 double* vy, double* vz, int from_zero) {
+  __shared__ double lc[3][kThreads];
   const uint32_t j = blk_.id * kThreads + threadIdx.x;
   const RunStart r = run_start(keys, j, count, M);
+  const double res = (double)(n - 1);
+  double c[3] = {0.0, 0.0, 0.0};
+  if (j < count && r.key < M) splat_contribution(rec, ids[j], res, c);
+  lc[0][threadIdx.x] = c[0];
+  lc[1][threadIdx.x] = c[1];
+  lc[2][threadIdx.x] = c[2];
+  __syncthreads();
   if (!r.head) return;
   const uint32_t key = r.key;
-  const double res = (double)(n - 1);
-  double a = 0.0, b = 0.0, c = 0.0;
+  double a = 0.0, b = 0.0, cz = 0.0;
   if (!from_zero) {
     a = vx[key];
     b = vy[key];
-    c = vz[key];
+    cz = vz[key];
   }
-  // pair id 8 p + k, k = 4 dz + 2 dy + dx (splat_pairs_body's corner order); the
-  // contribution is pass 2's w = wx wy wz sw, vw = w (n-1) / factor, n^ vw
-  using D2 = double __attribute__((ext_vector_type(2)));
-  struct Rec {
-    D2 v[4];  // fx fy, fz sw, nx ny, nz factor
-  };
-  auto load = [&](uint32_t id) {
-    const D2* q = reinterpret_cast<const D2*>(rec + 8ull * (id >> 3));
-    return Rec{{q[0], q[1], q[2], q[3]}};
-  };
-  auto sub = [&](const Rec& q, uint32_t id) {
-    const uint32_t k = id & 7u;
-    const double fx = q.v[0][0], fy = q.v[0][1], fz = q.v[1][0], sw = q.v[1][1];
-    const double wz = (k & 4u) ? fz : 1.0 - fz;
-    const double wy = (k & 2u) ? fy : 1.0 - fy;
-    const double wx = (k & 1u) ? fx : 1.0 - fx;
-    const double w = wx * wy * wz * sw;
-    const double vw = w * res / q.v[3][1];
-    a -= q.v[2][0] * vw;
-    b -= q.v[2][1] * vw;
-    c -= q.v[3][0] * vw;
-  };
   uint32_t i = j;
-  for (; i + 4 <= r.end; i += 4) {  // four records in flight, summed in order
-    uint32_t id[4];
-    Rec q[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) id[t] = ids[i + t];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) q[t] = load(id[t]);
-#pragma unroll
-    for (int t = 0; t < 4; ++t) sub(q[t], id[t]);
+  for (; i < r.end; ++i) {
+    const uint32_t t = threadIdx.x + (i - j);
+    a -= lc[0][t];
+    b -= lc[1][t];
+    cz -= lc[2][t];
   }
-  for (; i < r.end; ++i) sub(load(ids[i]), ids[i]);
   if (r.open)
-    for (; i < count && keys[i] == key; ++i) sub(load(ids[i]), ids[i]);
+    for (; i < count && keys[i] == key; ++i) {
+      splat_contribution(rec, ids[i], res, c);
+      a -= c[0];
+      b -= c[1];
+      cz -= c[2];
+    }
   vx[key] = a;
   vy[key] = b;
-  vz[key] = c;
+  vz[key] = cz;
 }
 
 // ---- iso value (capi.mojo:745-795) -----------------------------------------------------

@@ -135,6 +135,7 @@ struct Job {
   uint32_t* touched = nullptr;  // splat node bits, ceil(n^3 / 32) words
   // the previous assembly's touched bits and the sparse assembly's need mask (same size)
   uint32_t *prev_touched = nullptr, *need_bits = nullptr;
+  uint32_t* support_bits = nullptr;  // coarse support of need_bits (the first pyramid step)
   uint32_t *chunk_list = nullptr, *chunk_count = nullptr;  // the sparse assembly's chunks
   // live slots of the finest level (LiveMask, stencil.h): rhs or screen non-zero
   uint32_t* live_bits = nullptr;
@@ -539,6 +540,7 @@ void Job::plan(DevArena& a) {
   touched = a.alloc<uint32_t>((M + 31) / 32);
   prev_touched = a.alloc<uint32_t>((M + 31) / 32);
   need_bits = a.alloc<uint32_t>((M + 31) / 32);
+  support_bits = geoms.size() > 1 ? a.alloc<uint32_t>(coarse_support_words(geoms[1].n)) : nullptr;
   chunk_list = a.alloc<uint32_t>((uint64_t)n * n * assemble_chunks_per_row(n));
   chunk_count = a.alloc<uint32_t>(1);
   live_bits = a.alloc<uint32_t>(live_words());
@@ -679,11 +681,17 @@ void Job::setup_all(Job* const* jobs, const mpr_params* const* params, size_t co
     for (size_t i = 0; i < m; ++i) {
       Job& j = *solve[i];
       const bool mixed = j.precision == MPR_PRECISION_MIXED;
+      // The first restriction reads the finest screen, which is +0.0 outside the need mask
+      // of a sparse assembly (the nodes with a touched neighbour now or in the previous run,
+      // all nodes on a job's first run); a coarse node whose support misses the mask is
+      // +0.0 and was already +0.0 after the previous run, so only the others are written.
+      const bool sparse = l == 1 && !(sp[i]->stages & MPR_STAGE_GRIDS);
       if (l < j.L64.size())
         pyr[i] = {j.L64[l - 1].screen, j.geoms[l - 1], j.L64[l].screen, j.geoms[l], 4.0,
-                  mixed ? j.L32[l].screen : nullptr};
+                  mixed ? j.L32[l].screen : nullptr, sparse ? j.need_bits : nullptr,
+                  j.support_bits};
       else
-        pyr[i] = {nullptr, Geom{}, nullptr, Geom{}, 4.0, nullptr};
+        pyr[i] = {nullptr, Geom{}, nullptr, Geom{}, 4.0, nullptr, nullptr, nullptr};
     }
     launch_restrict_nodes_multi(pyr.data(), m, s);
   }
