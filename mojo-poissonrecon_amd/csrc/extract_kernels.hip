@@ -10,6 +10,7 @@
 
 #include "block_scan.h"
 #include "mpr_internal.h"
+#include "multi.h"
 
 namespace mpr {
 
@@ -309,9 +310,9 @@ __device__ __forceinline__ uint32_t low_bits(uint32_t count) {
   return count >= 32u ? 0xffffffffu : ((1u << count) - 1u);
 }
 
-__global__ __launch_bounds__(kThreads) void k_sign_bits(FieldView f, const double* isop,
+__device__ __forceinline__ void sign_bits_body(Blk blk_, FieldView f, const double* isop,
                                                         BitGeom G, uint32_t* bits) {
-  const uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  const uint64_t i = (uint64_t)blk_.id * kThreads + threadIdx.x;
   const uint32_t wi = (uint32_t)(i >> 5), k = (uint32_t)(i & 31u);
   const uint32_t words = G.n * G.n * G.bw;
   const uint32_t row = G.d_bw.div(wi < words ? wi : 0u);
@@ -341,10 +342,10 @@ __device__ __forceinline__ uint32_t cell_mask(const uint32_t* bits, const BitGeo
   return (any & ~all) & low_bits(G.cells - 32u * w);
 }
 
-__global__ __launch_bounds__(kThreads) void k_bit_cells_count(const uint32_t* bits, BitGeom G,
+__device__ __forceinline__ void bit_cells_count_body(Blk blk_, const uint32_t* bits, BitGeom G,
                                                               uint32_t* cmask, uint32_t* counts) {
   __shared__ uint32_t lds4[4];
-  const uint32_t r = blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t r = blk_.id * kThreads + threadIdx.x;
   uint32_t m = 0;
   if (r < G.cells * G.cells * G.cw) {
     uint32_t y, z, w;
@@ -352,23 +353,23 @@ __global__ __launch_bounds__(kThreads) void k_bit_cells_count(const uint32_t* bi
     cmask[r] = m;
   }
   const uint32_t c = block_count256((uint32_t)__popc(m), lds4);
-  if (threadIdx.x == 0) counts[blockIdx.x] = c;
+  if (threadIdx.x == 0) counts[blk_.id] = c;
 }
 
 // Raster-order indices of the active cells: coff[r] = index of run r's first vertex, and
 // vcell[idx] = the cell (x, run) of vertex idx for the vertices below the capacity.
-__global__ __launch_bounds__(kThreads) void k_bit_cells_index(BitGeom G, const uint32_t* cmask,
+__device__ __forceinline__ void bit_cells_index_body(Blk blk_, BitGeom G, const uint32_t* cmask,
                                                               const uint32_t* tile_offsets,
                                                               uint32_t* coff, uint32_t* vcell,
                                                               uint64_t capacity) {
   __shared__ uint32_t lds4[4];
-  const uint32_t r = blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t r = blk_.id * kThreads + threadIdx.x;
   const bool live = r < G.cells * G.cells * G.cw;
   uint32_t m = live ? cmask[r] : 0u;
   uint32_t total;
   const uint32_t pre = block_exclusive256((uint32_t)__popc(m), lds4, total);
   if (!live) return;
-  const uint32_t base = tile_offsets[blockIdx.x] + pre;
+  const uint32_t base = tile_offsets[blk_.id] + pre;
   coff[r] = base;
   uint64_t idx = base;
   while (m && idx < capacity) {
@@ -379,13 +380,13 @@ __global__ __launch_bounds__(kThreads) void k_bit_cells_index(BitGeom G, const u
 }
 
 // one lane per vertex: the dual vertex of its cell (capi.mojo:958-1038)
-__global__ __launch_bounds__(kThreads) void k_bit_vertices(FieldView f, DensityView dens,
+__device__ __forceinline__ void bit_vertices_body(Blk blk_, FieldView f, DensityView dens,
                                                            const double* isop, BitGeom G,
                                                            const uint32_t* vcell,
                                                            const uint32_t* count, double* verts,
                                                            double* vdens, uint64_t capacity,
                                                            const double* xform) {
-  const uint64_t idx = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  const uint64_t idx = (uint64_t)blk_.id * kThreads + threadIdx.x;
   if (idx >= capacity || idx >= *count) return;
   const uint32_t c = vcell[idx];
   const uint32_t r = c >> 5, k = c & 31u;
@@ -455,10 +456,10 @@ __device__ __forceinline__ uint32_t edge_runs(const BitGeom& G) {
   return (G.r0 + G.r1 + G.r2) * G.cw;
 }
 
-__global__ __launch_bounds__(kThreads) void k_bit_edges_count(const uint32_t* bits, BitGeom G,
+__device__ __forceinline__ void bit_edges_count_body(Blk blk_, const uint32_t* bits, BitGeom G,
                                                               uint32_t* counts) {
   __shared__ uint32_t lds4[4];
-  const uint32_t e = blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t e = blk_.id * kThreads + threadIdx.x;
   uint32_t m = 0;
   if (e < edge_runs(G)) {
     int axis;
@@ -466,7 +467,7 @@ __global__ __launch_bounds__(kThreads) void k_bit_edges_count(const uint32_t* bi
     m = edge_mask(bits, G, e, axis, y, z, w, below);
   }
   const uint32_t c = block_count256((uint32_t)__popc(m), lds4);
-  if (threadIdx.x == 0) counts[blockIdx.x] = c;
+  if (threadIdx.x == 0) counts[blk_.id] = c;
 }
 
 __device__ __forceinline__ int64_t vertex_of(const BitGeom& G, const uint32_t* cmask,
@@ -476,19 +477,19 @@ __device__ __forceinline__ int64_t vertex_of(const BitGeom& G, const uint32_t* c
   return (int64_t)coff[r] + __popc(cmask[r] & low_bits(x & 31u));
 }
 
-__global__ __launch_bounds__(kThreads) void k_bit_edges_emit(const uint32_t* bits, BitGeom G,
+__device__ __forceinline__ void bit_edges_emit_body(Blk blk_, const uint32_t* bits, BitGeom G,
                                                              const uint32_t* cmask,
                                                              const uint32_t* coff,
                                                              const uint32_t* tile_offsets,
                                                              int64_t* faces, uint64_t capacity) {
   __shared__ uint32_t lds4[4];
-  const uint32_t e = blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t e = blk_.id * kThreads + threadIdx.x;
   uint32_t m = 0, y = 0, z = 0, w = 0, below = 0;
   int axis = 0;
   if (e < edge_runs(G)) m = edge_mask(bits, G, e, axis, y, z, w, below);
   uint32_t total;
   const uint32_t pre = block_exclusive256((uint32_t)__popc(m), lds4, total);
-  uint64_t quad = (uint64_t)tile_offsets[blockIdx.x] + pre;
+  uint64_t quad = (uint64_t)tile_offsets[blk_.id] + pre;
   while (m) {
     const uint32_t k = (uint32_t)__ffs(m) - 1u;
     m &= m - 1u;
@@ -544,6 +545,87 @@ EdgeGeom make_edge_geom(int n) {
   G.d_cm1 = FastDiv(std::max<uint32_t>(1u, G.cells - 1));
   return G;
 }
+
+// ---- multi-instance kernel bodies (multi.h) ---------------------------------------------
+struct SignBits {
+  struct Args {
+    FieldView f;
+    const double* iso;
+    BitGeom G;
+    uint32_t* bits;
+  };
+  static constexpr int kThreads = mpr::kThreads;
+  static __device__ void run(const Args& a, Blk b) { sign_bits_body(b, a.f, a.iso, a.G, a.bits); }
+};
+
+struct BitCellsCount {
+  struct Args {
+    const uint32_t* bits;
+    BitGeom G;
+    uint32_t *cmask, *counts;
+  };
+  static constexpr int kThreads = mpr::kThreads;
+  static __device__ void run(const Args& a, Blk b) {
+    bit_cells_count_body(b, a.bits, a.G, a.cmask, a.counts);
+  }
+};
+
+struct BitCellsIndex {
+  struct Args {
+    BitGeom G;
+    const uint32_t *cmask, *tile_offsets;
+    uint32_t *coff, *vcell;
+    uint64_t capacity;
+  };
+  static constexpr int kThreads = mpr::kThreads;
+  static __device__ void run(const Args& a, Blk b) {
+    bit_cells_index_body(b, a.G, a.cmask, a.tile_offsets, a.coff, a.vcell, a.capacity);
+  }
+};
+
+struct BitVertices {
+  struct Args {
+    FieldView f;
+    DensityView dens;
+    const double* iso;
+    BitGeom G;
+    const uint32_t *vcell, *count;
+    double *verts, *vdens;
+    uint64_t capacity;
+    const double* xform;
+  };
+  static constexpr int kThreads = mpr::kThreads;
+  static __device__ void run(const Args& a, Blk b) {
+    bit_vertices_body(b, a.f, a.dens, a.iso, a.G, a.vcell, a.count, a.verts, a.vdens, a.capacity,
+                      a.xform);
+  }
+};
+
+struct BitEdgesCount {
+  struct Args {
+    const uint32_t* bits;
+    BitGeom G;
+    uint32_t* counts;
+  };
+  static constexpr int kThreads = mpr::kThreads;
+  static __device__ void run(const Args& a, Blk b) {
+    bit_edges_count_body(b, a.bits, a.G, a.counts);
+  }
+};
+
+struct BitEdgesEmit {
+  struct Args {
+    const uint32_t* bits;
+    BitGeom G;
+    const uint32_t *cmask, *coff, *tile_offsets;
+    int64_t* faces;
+    uint64_t capacity;
+  };
+  static constexpr int kThreads = mpr::kThreads;
+  static __device__ void run(const Args& a, Blk b) {
+    bit_edges_emit_body(b, a.bits, a.G, a.cmask, a.coff, a.tile_offsets, a.faces, a.capacity);
+  }
+};
 
 }  // namespace
 
@@ -614,43 +696,57 @@ BitSurface bit_surface_sizes(int n) {
   return b;
 }
 
+void launch_bit_extract_multi(const ExtractJob* jobs, size_t count, hipStream_t s) {
+  std::vector<SignBits::Args> sb(count);
+  std::vector<BitCellsCount::Args> cc(count);
+  std::vector<BitCellsIndex::Args> ci(count);
+  std::vector<BitVertices::Args> bv(count);
+  std::vector<BitEdgesCount::Args> ec(count);
+  std::vector<BitEdgesEmit::Args> ee(count);
+  std::vector<ScanJob> scan_cells(count), scan_edges(count);
+  std::vector<uint32_t> nb_sign(count), nb_cells(count), nb_vert(count), nb_edges(count);
+  for (size_t i = 0; i < count; ++i) {
+    const ExtractJob& j = jobs[i];
+    const BitSurface& b = *j.b;
+    const BitGeom G = make_bit_geom(j.n);
+    const uint64_t lanes = b.bit_words * 32ull;
+    const uint64_t vlanes = std::min<uint64_t>(j.vcap, b.vcell_cap);
+    sb[i] = {j.f, j.iso, G, b.bits};
+    nb_sign[i] = (uint32_t)((lanes + kThreads - 1) / kThreads);
+    cc[i] = {b.bits, G, b.cmask, b.cell_tc};
+    nb_cells[i] = b.cell_tiles;
+    scan_cells[i] = {b.cell_tc, b.cell_to, b.cell_tiles, &j.counts[0], nullptr};
+    ci[i] = {G, b.cmask, b.cell_to, b.coff, b.vcell, vlanes};
+    bv[i] = {j.f, j.dens, j.iso, G, b.vcell, &j.counts[0], j.verts, j.vdens, vlanes, j.xform};
+    nb_vert[i] = (uint32_t)((vlanes + kThreads - 1) / kThreads);
+    ec[i] = {b.bits, G, b.edge_tc};
+    nb_edges[i] = b.edge_tiles;
+    scan_edges[i] = {b.edge_tc, b.edge_to, b.edge_tiles, &j.counts[1], nullptr};
+    ee[i] = {b.bits, G, b.cmask, b.coff, b.edge_to, j.faces, j.fcap};
+  }
+  launch_multi<SignBits>(sb, nb_sign, s);
+  launch_multi<BitCellsCount>(cc, nb_cells, s);
+  launch_scan_exclusive_multi(scan_cells.data(), count, s);
+  launch_multi<BitCellsIndex>(ci, nb_cells, s);
+  launch_multi<BitVertices>(bv, nb_vert, s);
+  launch_multi<BitEdgesCount>(ec, nb_edges, s);
+  launch_scan_exclusive_multi(scan_edges.data(), count, s);
+  launch_multi<BitEdgesEmit>(ee, nb_edges, s);
+}
+
 void launch_bit_extract(const FieldView& f, const DensityView& dens, int n, const double* iso,
                         const BitSurface& b, uint32_t* counts, double* verts, double* vdens,
                         uint64_t vcap, int64_t* faces, uint64_t fcap, const double* xform,
                         hipStream_t s) {
-  const BitGeom G = make_bit_geom(n);
-  const uint64_t lanes = b.bit_words * 32ull;
-  hipLaunchKernelGGL(k_sign_bits, dim3((unsigned)((lanes + kThreads - 1) / kThreads)),
-                     dim3(kThreads), 0, s, f, iso, G, b.bits);
-  MPR_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_bit_cells_count, dim3(b.cell_tiles), dim3(kThreads), 0, s, b.bits, G,
-                     b.cmask, b.cell_tc);
-  MPR_LAUNCH_CHECK();
-  launch_scan_exclusive(b.cell_tc, b.cell_to, b.cell_tiles, &counts[0], s);
-  hipLaunchKernelGGL(k_bit_cells_index, dim3(b.cell_tiles), dim3(kThreads), 0, s, G, b.cmask,
-                     b.cell_to, b.coff, b.vcell, std::min<uint64_t>(vcap, b.vcell_cap));
-  MPR_LAUNCH_CHECK();
-  const uint64_t vlanes = std::min<uint64_t>(vcap, b.vcell_cap);
-  if (vlanes) {
-    hipLaunchKernelGGL(k_bit_vertices, dim3((unsigned)((vlanes + kThreads - 1) / kThreads)),
-                       dim3(kThreads), 0, s, f, dens, iso, G, b.vcell, &counts[0], verts, vdens,
-                       vlanes, xform);
-    MPR_LAUNCH_CHECK();
-  }
-  hipLaunchKernelGGL(k_bit_edges_count, dim3(b.edge_tiles), dim3(kThreads), 0, s, b.bits, G,
-                     b.edge_tc);
-  MPR_LAUNCH_CHECK();
-  launch_scan_exclusive(b.edge_tc, b.edge_to, b.edge_tiles, &counts[1], s);
-  hipLaunchKernelGGL(k_bit_edges_emit, dim3(b.edge_tiles), dim3(kThreads), 0, s, b.bits, G,
-                     b.cmask, b.coff, b.edge_to, faces, fcap);
-  MPR_LAUNCH_CHECK();
+  const ExtractJob j{f, dens, n, iso, &b, counts, verts, vdens, vcap, faces, fcap, xform};
+  launch_bit_extract_multi(&j, 1, s);
 }
 
 // Host copy of the mesh: the first counts[0] vertices (3 doubles each) and vertex densities
 // and 2 * counts[1] faces (3 int64 each), clamped to the capacities, copied word for word to
 // (pinned) host buffers of the same layout. The counts live on the device, so the copy is
 // sized there: a fixed grid strides over 16-byte word pairs of the three arrays in turn.
-__global__ __launch_bounds__(kThreads) void k_mesh_to_host(const uint64_t* v, const uint64_t* d,
+__device__ __forceinline__ void mesh_to_host_body(Blk blk_, const uint64_t* v, const uint64_t* d,
                                                            const uint64_t* f,
                                                            const uint32_t* counts, uint64_t vcap,
                                                            uint64_t fcap, uint64_t* hv,
@@ -661,8 +757,8 @@ __global__ __launch_bounds__(kThreads) void k_mesh_to_host(const uint64_t* v, co
   const uint64_t w[3] = {3 * nv, nv, 3 * nf};
   const uint64_t p0 = (w[0] + 1) / 2, p1 = (w[1] + 1) / 2, p2 = (w[2] + 1) / 2;
   const uint64_t total = p0 + p1 + p2;
-  for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < total;
-       i += (uint64_t)gridDim.x * kThreads) {
+  for (uint64_t i = (uint64_t)blk_.id * kThreads + threadIdx.x; i < total;
+       i += (uint64_t)blk_.count * kThreads) {
     const uint64_t* src;
     uint64_t* dst;
     uint64_t j, words;
@@ -680,16 +776,39 @@ __global__ __launch_bounds__(kThreads) void k_mesh_to_host(const uint64_t* v, co
   }
 }
 
+namespace {
+struct MeshToHost {
+  struct Args {
+    const uint64_t *v, *d, *f;
+    const uint32_t* counts;
+    uint64_t vcap, fcap;
+    uint64_t *hv, *hd, *hf;
+  };
+  static constexpr int kThreads = mpr::kThreads;
+  static __device__ void run(const Args& a, Blk b) {
+    mesh_to_host_body(b, a.v, a.d, a.f, a.counts, a.vcap, a.fcap, a.hv, a.hd, a.hf);
+  }
+};
+}  // namespace
+
+void launch_mesh_to_host_multi(const MeshCopyJob* jobs, size_t count, hipStream_t s) {
+  std::vector<MeshToHost::Args> a(count);
+  std::vector<uint32_t> nb(count);
+  for (size_t i = 0; i < count; ++i) {
+    const MeshCopyJob& j = jobs[i];
+    const uint64_t pairs = (3 * j.vcap + 1) / 2 + (j.vcap + 1) / 2 + (3 * j.fcap + 1) / 2;
+    nb[i] = (uint32_t)std::min<uint64_t>(2048, std::max<uint64_t>(1, (pairs + kThreads - 1) / kThreads));
+    a[i] = {(const uint64_t*)j.verts, (const uint64_t*)j.vdens, (const uint64_t*)j.faces, j.counts,
+            j.vcap, j.fcap, (uint64_t*)j.h_verts, (uint64_t*)j.h_vdens, (uint64_t*)j.h_faces};
+  }
+  launch_multi<MeshToHost>(a, nb, s);
+}
+
 void launch_mesh_to_host(const double* verts, const double* vdens, const int64_t* faces,
                          const uint32_t* counts, uint64_t vcap, uint64_t fcap, double* h_verts,
                          double* h_vdens, int64_t* h_faces, hipStream_t s) {
-  const uint64_t pairs = (3 * vcap + 1) / 2 + (vcap + 1) / 2 + (3 * fcap + 1) / 2;
-  const unsigned blocks = (unsigned)std::min<uint64_t>(2048, std::max<uint64_t>(1, (pairs + kThreads - 1) / kThreads));
-  hipLaunchKernelGGL(k_mesh_to_host, dim3(blocks), dim3(kThreads), 0, s,
-                     (const uint64_t*)verts, (const uint64_t*)vdens, (const uint64_t*)faces,
-                     counts, vcap, fcap, (uint64_t*)h_verts, (uint64_t*)h_vdens,
-                     (uint64_t*)h_faces);
-  MPR_LAUNCH_CHECK();
+  const MeshCopyJob j{verts, vdens, faces, counts, vcap, fcap, h_verts, h_vdens, h_faces};
+  launch_mesh_to_host_multi(&j, 1, s);
 }
 
 template void launch_cells_emit<int32_t>(const double*, const double*, int, const double*, const uint32_t*,

@@ -113,13 +113,13 @@ __global__ __launch_bounds__(kThreads) void k_residual(const T* X, const T* R, c
 // colour blocks, so x streams from memory once); block partials of the squared norm as
 // k_residual (colour 0 then colour 1 per lane)
 template <class T, class TR>
-__global__ __launch_bounds__(kThreads) void k_residual_group(const T* X, const T* R, const T* S,
+__device__ __forceinline__ void residual_group_body(Blk blk_, const T* X, const T* R, const T* S,
                                                              TR* out, Geom g, uint32_t lanes,
                                                              T pw, T h2, double* partials,
                                                              const uint32_t* live) {
   constexpr int V = 16 / (int)sizeof(T);
   __shared__ double lds[kThreads / 64];
-  const uint32_t lb = logical_block();
+  const uint32_t lb = xcd_block(blk_);
   const uint32_t lane = lb * kThreads + threadIdx.x;
   double sq = 0.0;
   SlotGroup<T, V> G[2][1];
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(kThreads) void k_residual_group(const T* X, const T
   }
 }
 
-__global__ __launch_bounds__(1024) void k_norm_finish(const double* partials, int count,
+__device__ __forceinline__ void norm_finish_body(Blk blk_, const double* partials, int count,
                                                       double* out) {
   __shared__ double lds[1024];
   double v = 0.0;
@@ -488,6 +488,36 @@ __device__ __forceinline__ void restrict_sparse_body(Blk blk_, const T* F, Geom 
 }
 
 // ---- multi-instance kernel bodies (multi.h) ---------------------------------------------
+template <class T, class TR>
+struct ResidualGroup {
+  struct Args {
+    const T *X, *R, *S;
+    TR* out;
+    Geom g;
+    uint32_t lanes;
+    T pw, h2;
+    double* partials;
+    const uint32_t* live;
+  };
+  static constexpr int kThreads = mpr::kThreads;
+  static __device__ void run(const Args& a, Blk b) {
+    residual_group_body<T, TR>(b, a.X, a.R, a.S, a.out, a.g, a.lanes, a.pw, a.h2, a.partials,
+                               a.live);
+  }
+};
+
+struct NormFinish {
+  struct Args {
+    const double* partials;
+    int count;
+    double* out;
+  };
+  static constexpr int kThreads = 1024;
+  static __device__ void run(const Args& a, Blk b) {
+    norm_finish_body(b, a.partials, a.count, a.out);
+  }
+};
+
 template <class T>
 struct Fill {
   struct Args {
@@ -651,9 +681,8 @@ int launch_residual(const T* X, const T* R, const T* S, TR* out, const Geom& g, 
   const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
   if (group_layout(g)) {
     const uint32_t lanes = g.C / (uint32_t)(16 / sizeof(T));
-    hipLaunchKernelGGL((k_residual_group<T, TR>), dim3(xcd_grid(lanes, kThreads)),
-                       dim3(kThreads), 0, s, X, R, S, out, g, lanes, (T)pw, h2, partials, live);
-    MPR_LAUNCH_CHECK();
+    launch_one<ResidualGroup<T, TR>>({X, R, S, out, g, lanes, (T)pw, h2, partials, live},
+                                     xcd_grid(lanes, kThreads), s);
     return (int)blocks_for(lanes, kThreads);
   }
   const unsigned nb = xcd_grid(2ull * g.C, kThreads * kNodesPerThread);
@@ -669,8 +698,28 @@ int residual_partial_count(const Geom& g, size_t word_bytes) {
 }
 
 void launch_norm_finish(const double* partials, int count, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_norm_finish, dim3(1), dim3(1024), 0, s, partials, count, out);
-  MPR_LAUNCH_CHECK();
+  launch_one<NormFinish>({partials, count, out}, 1, s);
+}
+
+void launch_residual_norm_multi(const NormJob* jobs, size_t count, hipStream_t s) {
+  std::vector<ResidualGroup<double, double>::Args> rg(count);
+  std::vector<NormFinish::Args> nf(count);
+  std::vector<uint32_t> nb(count, 0u), one(count, 1u);
+  for (size_t i = 0; i < count; ++i) {
+    const NormJob& j = jobs[i];
+    nf[i] = {j.partials, j.nparts, j.out};
+    if (!group_layout(j.g)) {  // small grids: the node-by-node kernel, launched alone
+      launch_residual<double, double>(j.X, j.R, j.S, (double*)nullptr, j.g, j.pw, j.partials, s,
+                                      j.live);
+      continue;
+    }
+    const uint32_t lanes = j.g.C / 2u;
+    const double h2 = (double)(j.g.n - 1) * (double)(j.g.n - 1);
+    rg[i] = {j.X, j.R, j.S, nullptr, j.g, lanes, j.pw, h2, j.partials, j.live};
+    nb[i] = xcd_grid(lanes, kThreads);
+  }
+  launch_multi<ResidualGroup<double, double>>(rg, nb, s);
+  launch_multi<NormFinish>(nf, one, s);
 }
 
 template <class T>

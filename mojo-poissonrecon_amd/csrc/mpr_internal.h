@@ -66,6 +66,18 @@ int launch_residual(const T* X, const T* R, const T* S, TR* out, const Geom& g, 
 int residual_partial_count(const Geom& g, size_t word_bytes);
 // out = sqrt(sum partials) with a fixed summation order
 void launch_norm_finish(const double* partials, int count, double* out, hipStream_t s);
+// the final FP64 residual norm of several solutions in one launch per step: out = sqrt of
+// the sum of squared residuals of (X, R, S), nparts = residual_partial_count(g, 8)
+struct NormJob {
+  const double *X, *R, *S;
+  Geom g;
+  double pw;
+  double* partials;
+  int nparts;
+  double* out;
+  const uint32_t* live;
+};
+void launch_residual_norm_multi(const NormJob* jobs, size_t count, hipStream_t s);
 template <class T>
 void launch_restrict(const T* F, const Geom& gf, T* Cc, const Geom& gc, double scale,
                      hipStream_t s, float* copy32 = nullptr);  // copy32: also store as float
@@ -245,6 +257,16 @@ struct SplatJob {
 // every job's splat in one launch per step (multi.h), each job's results those of its own
 void launch_splat_multi(const SplatJob* jobs, size_t count, hipStream_t s);
 int iso_partial_count(uint32_t P);
+// one instance of launch_iso_multi (launch_iso's arguments)
+struct IsoJob {
+  FieldView field;
+  const double *pts, *nrm;
+  uint32_t P;
+  int n, confidence;
+  double *part, *iso;
+  const uint32_t* count;
+};
+void launch_iso_multi(const IsoJob* jobs, size_t count, hipStream_t s);
 // count (device, nullable): only rows below *count take part
 void launch_iso(const FieldView& field, const double* pts, const double* nrm, uint32_t P, int n,
                 int confidence, double* part, double* iso, hipStream_t s,
@@ -272,11 +294,36 @@ void launch_bit_extract(const FieldView& f, const DensityView& dens, int n, cons
                         uint64_t vcap, int64_t* faces, uint64_t fcap, const double* xform,
                         hipStream_t s);
 
+// one instance of launch_bit_extract_multi (launch_bit_extract's arguments)
+struct ExtractJob {
+  FieldView f;
+  DensityView dens;
+  int n;
+  const double* iso;
+  const BitSurface* b;
+  uint32_t* counts;
+  double *verts, *vdens;
+  uint64_t vcap;
+  int64_t* faces;
+  uint64_t fcap;
+  const double* xform;
+};
+void launch_bit_extract_multi(const ExtractJob* jobs, size_t count, hipStream_t s);
+
 // copies the mesh (counts[0] vertices, 2 * counts[1] faces, clamped to the capacities, sizes
 // read on the device) to host buffers of the same layout, e.g. pinned host memory
 void launch_mesh_to_host(const double* verts, const double* vdens, const int64_t* faces,
                          const uint32_t* counts, uint64_t vcap, uint64_t fcap, double* h_verts,
                          double* h_vdens, int64_t* h_faces, hipStream_t s);
+struct MeshCopyJob {
+  const double *verts, *vdens;
+  const int64_t* faces;
+  const uint32_t* counts;
+  uint64_t vcap, fcap;
+  double *h_verts, *h_vdens;
+  int64_t* h_faces;
+};
+void launch_mesh_to_host_multi(const MeshCopyJob* jobs, size_t count, hipStream_t s);
 
 void launch_cells_count(const double* f, int n, const double* iso, uint32_t* counts, hipStream_t s);
 void launch_cells_chunk_counts(const double* f, int n, const double* iso, int64_t* chunks,

@@ -727,12 +727,12 @@ __device__ __forceinline__ void splat_sum2_body(Blk blk_, const uint32_t* keys,
 
 // ---- iso value (capi.mojo:745-795) -----------------------------------------------------
 // rows at or past *count (when given) are absent
-__global__ __launch_bounds__(kThreads) void k_iso_partials(FieldView field,
+__device__ __forceinline__ void iso_partials_body(Blk blk_, FieldView field,
                                                            const double* pts, const double* nrm,
                                                            uint32_t P, int n, int confidence,
                                                            double* part, const uint32_t* count) {
   __shared__ double lds4[4];
-  const uint32_t p = blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t p = blk_.id * kThreads + threadIdx.x;
   double vw = 0.0, wt = 0.0;
   if (p < P && (!count || p < *count)) {
     const double weight =
@@ -765,13 +765,13 @@ __global__ __launch_bounds__(kThreads) void k_iso_partials(FieldView field,
   const double a = block_sum_d(vw, lds4);
   const double b = block_sum_d(wt, lds4);
   if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = a;
-    part[2 * blockIdx.x + 1] = b;
+    part[2 * blk_.id] = a;
+    part[2 * blk_.id + 1] = b;
   }
 }
 
 // fixed summation order: thread t sums partials t, t+256, ...; then a fixed tree
-__global__ __launch_bounds__(kThreads) void k_iso_finish(const double* part, int nparts,
+__device__ __forceinline__ void iso_finish_body(Blk blk_, const double* part, int nparts,
                                                          double* iso) {
   __shared__ double la[kThreads], lb[kThreads];
   double a = 0.0, b = 0.0;
@@ -907,6 +907,31 @@ struct RadixScatter {
     radix_scatter_body<K, DB>(b, a.kin, a.vin, a.kout, a.vout, a.n, a.shift, a.offsets, b.count,
                               a.gsrc, a.gdst);
   }
+};
+
+struct IsoPartials {
+  struct Args {
+    FieldView field;
+    const double *pts, *nrm;
+    uint32_t P;
+    int n, confidence;
+    double* part;
+    const uint32_t* count;
+  };
+  static constexpr int kThreads = mpr::kThreads;
+  static __device__ void run(const Args& a, Blk b) {
+    iso_partials_body(b, a.field, a.pts, a.nrm, a.P, a.n, a.confidence, a.part, a.count);
+  }
+};
+
+struct IsoFinish {
+  struct Args {
+    const double* part;
+    int nparts;
+    double* iso;
+  };
+  static constexpr int kThreads = mpr::kThreads;
+  static __device__ void run(const Args& a, Blk b) { iso_finish_body(b, a.part, a.nparts, a.iso); }
 };
 
 struct SplatPairs {
@@ -1274,14 +1299,24 @@ void launch_splat(const double* pts, const double* nrm, uint32_t P, int n, int c
 
 int iso_partial_count(uint32_t P) { return (int)std::max<uint32_t>(1u, blocks_for(P, kThreads)); }
 
+void launch_iso_multi(const IsoJob* jobs, size_t count, hipStream_t s) {
+  std::vector<IsoPartials::Args> pa(count);
+  std::vector<IsoFinish::Args> fi(count);
+  std::vector<uint32_t> nb(count), one(count, 1u);
+  for (size_t i = 0; i < count; ++i) {
+    const IsoJob& j = jobs[i];
+    nb[i] = (uint32_t)iso_partial_count(j.P);
+    pa[i] = {j.field, j.pts, j.nrm, j.P, j.n, j.confidence, j.part, j.count};
+    fi[i] = {j.part, (int)nb[i], j.iso};
+  }
+  launch_multi<IsoPartials>(pa, nb, s);
+  launch_multi<IsoFinish>(fi, one, s);
+}
+
 void launch_iso(const FieldView& field, const double* pts, const double* nrm, uint32_t P, int n,
                 int confidence, double* part, double* iso, hipStream_t s, const uint32_t* count) {
-  const int nb = iso_partial_count(P);
-  hipLaunchKernelGGL(k_iso_partials, dim3(nb), dim3(kThreads), 0, s, field, pts, nrm, P, n,
-                     confidence, part, count);
-  MPR_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_iso_finish, dim3(1), dim3(kThreads), 0, s, part, nb, iso);
-  MPR_LAUNCH_CHECK();
+  const IsoJob j{field, pts, nrm, P, n, confidence, part, iso, count};
+  launch_iso_multi(&j, 1, s);
 }
 
 }  // namespace mpr

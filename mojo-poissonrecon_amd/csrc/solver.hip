@@ -210,6 +210,9 @@ struct Job {
   // results are those of its own setup
   static void setup_all(Job* const* jobs, const mpr_params* const* params, size_t count,
                         hipStream_t s);
+  // finish, extraction and readback of several jobs, one launch per step for all of them
+  static void finish_all(Job* const* jobs, const mpr_params* const* params, size_t count,
+                         hipStream_t s);
   void finish(const mpr_params& p, hipStream_t s);
   void enqueue_extract(bool world, hipStream_t s);
   // the parameters of the work last enqueued or launched on this job (eagerly or as a graph)
@@ -357,18 +360,14 @@ class BatchResources {
   std::multimap<size_t, void*> buffers_;
 };
 
-// Jobs' own stages (normalise .. pyramid, and residual norm .. readback) are chains of
-// mostly small kernels; a batch runs them on up to kBatchSideStreams side streams forked
-// from and joined back into the batch stream (parallel branches of a captured graph), so the
-// jobs' chains overlap. The V-cycles run on the batch stream, one launch per step for all.
-constexpr size_t kBatchSideStreams = 8;
-
+// A batch runs everything on its own stream: the jobs' own stages (normalise .. pyramid,
+// and residual norm .. readback, chains of mostly small kernels) as multi-instance launches
+// that take up to 8 jobs each (Job::setup_all, Job::finish_all), the V-cycles one launch per
+// step for all jobs.
 struct Batch {
   std::vector<Job*> jobs;
   int precision = 0;
   hipStream_t stream = nullptr;
-  std::vector<hipStream_t> side;  // pooled (BatchResources)
-  std::vector<hipEvent_t> events;  // per phase: fork, then one join per side stream
   void* tables = nullptr;  // pooled (BatchResources)
   size_t tables_cap = 0;
   BatchedVcycle<double> vc64;
@@ -381,10 +380,6 @@ struct Batch {
 
   void build();
   void enqueue(const mpr_params* p);
-  // runs work(i, stream) for every job on the side streams between a fork from and a join
-  // into the batch stream; phase selects the events (0 before the V-cycles, 1 after)
-  template <class F>
-  void fork_join(int phase, F&& work);
   void drop_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -394,27 +389,9 @@ struct Batch {
   ~Batch() {
     drop_graph();
     if (tables) BatchResources::get().give_buffer(tables, tables_cap);
-    for (hipStream_t q : side) {
-      (void)hipStreamSynchronize(q);
-      BatchResources::get().give_stream(q);
-    }
-    for (hipEvent_t e : events) (void)hipEventDestroy(e);
     if (stream) BatchResources::get().give_stream(stream);
   }
 };
-
-template <class F>
-void Batch::fork_join(int phase, F&& work) {
-  const size_t k = side.size();
-  hipEvent_t* ev = events.data() + (size_t)phase * (k + 1);
-  MPR_HIP(hipEventRecord(ev[0], stream));
-  for (size_t j = 0; j < k; ++j) MPR_HIP(hipStreamWaitEvent(side[j], ev[0], 0));
-  for (size_t i = 0; i < jobs.size(); ++i) work(i, side[i % k]);
-  for (size_t j = 0; j < k; ++j) {
-    MPR_HIP(hipEventRecord(ev[1 + j], side[j]));
-    MPR_HIP(hipStreamWaitEvent(stream, ev[1 + j], 0));
-  }
-}
 
 void Batch::build() {
   std::vector<InstanceLevels<double>> i64;
@@ -451,12 +428,11 @@ void Batch::enqueue(const mpr_params* p) {
   }
   if (p[0].stages & MPR_STAGE_SOLVE)
     run_cycles(vc64, precision == MPR_PRECISION_MIXED ? &vc32 : nullptr, &dtab, p[0], stream);
-  fork_join(1, [&](size_t i, hipStream_t q) {
-    jobs[i]->finish(p[i], q);
-    if (p[i].stages & MPR_STAGE_EXTRACT)
-      jobs[i]->enqueue_extract((p[i].stages & MPR_STAGE_WORLD) != 0, q);
-    jobs[i]->readback(q);
-  });
+  {
+    std::vector<const mpr_params*> pp(jobs.size());
+    for (size_t i = 0; i < jobs.size(); ++i) pp[i] = &p[i];
+    Job::finish_all(jobs.data(), pp.data(), jobs.size(), stream);
+  }
 }
 
 // V-cycles of a set of jobs (core.py:274-285): FP64 as the reference, or FP32 V-cycles on
@@ -719,15 +695,53 @@ void Job::finish(const mpr_params& p, hipStream_t s) {
   if (!(p.stages & MPR_STAGE_SOLVE)) return;
   // only the norm of the final residual is reported; the iso value and the extraction read
   // the solution in the solver's layout, so the natural field is written for fetching only
-  launch_residual<double, double>(L64[0].x, L64[0].rhs, L64[0].screen, (double*)nullptr,
-                                  geoms[0], p.point_weight, norm_part, s, live_bits);
-  launch_norm_finish(norm_part, norm_nparts, d_resnorm, s);
+  const NormJob nj{L64[0].x, L64[0].rhs, L64[0].screen, geoms[0], p.point_weight, norm_part,
+                   norm_nparts, d_resnorm, live_bits};
+  launch_residual_norm_multi(&nj, 1, s);
   if (p.stages & MPR_STAGE_GRIDS) launch_split_to_nat<double>(L64[0].x, field, geoms[0], s);
   if (P)
     launch_iso(split_field(L64[0].x, geoms[0]), unit_points(p), normals(p), P, n, p.confidence,
                iso_part, d_iso, s, kept_count(p));
   else
     launch_fill<double>(d_iso, 0.0, 1, s);
+}
+
+void Job::finish_all(Job* const* jobs, const mpr_params* const* params, size_t count,
+                     hipStream_t s) {
+  std::vector<NormJob> norm;
+  std::vector<IsoJob> iso;
+  std::vector<FillJob<double>> iso_zero;
+  std::vector<ExtractJob> ext;
+  std::vector<MeshCopyJob> copy;
+  for (size_t i = 0; i < count; ++i) {
+    Job& j = *jobs[i];
+    const mpr_params& p = *params[i];
+    if (p.stages & MPR_STAGE_SOLVE) {
+      norm.push_back({j.L64[0].x, j.L64[0].rhs, j.L64[0].screen, j.geoms[0], p.point_weight,
+                      j.norm_part, j.norm_nparts, j.d_resnorm, j.live_bits});
+      if (p.stages & MPR_STAGE_GRIDS)
+        launch_split_to_nat<double>(j.L64[0].x, j.field, j.geoms[0], s);
+      if (j.P)
+        iso.push_back({split_field(j.L64[0].x, j.geoms[0]), j.unit_points(p), j.normals(p), j.P,
+                       j.n, p.confidence, j.iso_part, j.d_iso, j.kept_count(p)});
+      else
+        iso_zero.push_back({j.d_iso, 0.0, 1});
+    }
+    if (p.stages & MPR_STAGE_EXTRACT) {
+      ext.push_back({split_field(j.L64[0].x, j.geoms[0]), j.density_view(), j.n, j.d_iso,
+                     &j.surf, j.d_counts, j.verts, j.vdens, j.vcap, j.faces, j.fcap,
+                     (p.stages & MPR_STAGE_WORLD) ? j.d_params : nullptr});
+      if (j.host_mesh)
+        copy.push_back({j.verts, j.vdens, j.faces, j.d_counts, j.vcap, j.fcap, j.h_verts,
+                        j.h_vdens, j.h_faces});
+    }
+  }
+  if (!norm.empty()) launch_residual_norm_multi(norm.data(), norm.size(), s);
+  if (!iso.empty()) launch_iso_multi(iso.data(), iso.size(), s);
+  if (!iso_zero.empty()) launch_fill_multi(iso_zero.data(), iso_zero.size(), s);
+  if (!ext.empty()) launch_bit_extract_multi(ext.data(), ext.size(), s);
+  if (!copy.empty()) launch_mesh_to_host_multi(copy.data(), copy.size(), s);
+  for (size_t i = 0; i < count; ++i) jobs[i]->readback(s);
 }
 
 void Job::readback(hipStream_t s) {
@@ -1232,13 +1246,6 @@ int mpr_batch_create(mpr_job* const* jobs, int32_t count, mpr_batch** out) {
     b->precision = jobs[0]->precision;
     for (int i = 0; i < count; ++i) b->jobs.push_back(jobs[i]);
     b->stream = BatchResources::get().take_stream();
-    const size_t k = std::min<size_t>(kBatchSideStreams, b->jobs.size());
-    for (size_t j = 0; j < k; ++j) b->side.push_back(BatchResources::get().take_stream());
-    for (size_t e = 0; e < 2 * (k + 1); ++e) {
-      hipEvent_t ev = nullptr;
-      MPR_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-      b->events.push_back(ev);
-    }
     for (Job* j : b->jobs) MPR_HIP(hipStreamSynchronize(j->stream));
     b->build();
     *out = b;
