@@ -712,7 +712,7 @@ void launch_bit_extract_multi(const ExtractJob* jobs, size_t count, hipStream_t 
     const uint64_t lanes = b.bit_words * 32ull;
     const uint64_t vlanes = std::min<uint64_t>(j.vcap, b.vcell_cap);
     sb[i] = {j.f, j.iso, G, b.bits};
-    nb_sign[i] = (uint32_t)((lanes + kThreads - 1) / kThreads);
+    nb_sign[i] = j.bits_ready ? 0u : (uint32_t)((lanes + kThreads - 1) / kThreads);
     cc[i] = {b.bits, G, b.cmask, b.cell_tc};
     nb_cells[i] = b.cell_tiles;
     scan_cells[i] = {b.cell_tc, b.cell_to, b.cell_tiles, &j.counts[0], nullptr};
@@ -738,7 +738,7 @@ void launch_bit_extract(const FieldView& f, const DensityView& dens, int n, cons
                         const BitSurface& b, uint32_t* counts, double* verts, double* vdens,
                         uint64_t vcap, int64_t* faces, uint64_t fcap, const double* xform,
                         hipStream_t s) {
-  const ExtractJob j{f, dens, n, iso, &b, counts, verts, vdens, vcap, faces, fcap, xform};
+  const ExtractJob j{f, dens, n, iso, &b, counts, verts, vdens, vcap, faces, fcap, xform, false};
   launch_bit_extract_multi(&j, 1, s);
 }
 

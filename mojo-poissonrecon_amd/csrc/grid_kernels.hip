@@ -112,11 +112,17 @@ __global__ __launch_bounds__(kThreads) void k_residual(const T* X, const T* R, c
 // residual of slot group `lane` in both colours (one lane takes the same slots of both
 // colour blocks, so x streams from memory once); block partials of the squared norm as
 // k_residual (colour 0 then colour 1 per lane)
-template <class T, class TR>
+// With SIGN (FP64 solutions): also the sign bits of the extraction (node x of row (z, y) is
+// set when its value is below *iso, natural layout, bw words per row, launch_bit_extract's
+// k_sign_bits), from the x words the residual loads anyway. A lane's slots cover 4 nodes
+// x = 2 k0 .. 2 k0 + 3 of both colours; the 8 lanes of a word OR their nibbles with shuffles
+// and the first lane of each run adds them to the (zeroed) word.
+template <class T, class TR, bool SIGN = false>
 __device__ __forceinline__ void residual_group_body(Blk blk_, const T* X, const T* R, const T* S,
-                                                             TR* out, Geom g, uint32_t lanes,
-                                                             T pw, T h2, double* partials,
-                                                             const uint32_t* live) {
+                                                    TR* out, Geom g, uint32_t lanes, T pw, T h2,
+                                                    double* partials, const uint32_t* live,
+                                                    const double* iso = nullptr,
+                                                    uint32_t* bits = nullptr, uint32_t bw = 0) {
   constexpr int V = 16 / (int)sizeof(T);
   __shared__ double lds[kThreads / 64];
   const uint32_t lb = xcd_block(blk_);
@@ -138,6 +144,33 @@ __device__ __forceinline__ void residual_group_body(Blk blk_, const T* X, const 
     if (G[c][0].in)
 #pragma unroll
       for (int j = 0; j < V; ++j) sq += (double)v[j] * (double)v[j];
+  }
+  if constexpr (SIGN) {
+    const double level = *iso;
+    const int x_first = G[0][0].x0 - G[0][0].p;  // 2 k0, a multiple of 4
+    uint32_t nib = 0;
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int J = 0; J < V; ++J) {
+        const int x = G[c][0].x0 + 2 * J;
+        const T val = G[c][0].self[J / SlotGroup<T, V>::W][J % SlotGroup<T, V>::W];
+        if (G[c][0].in && x < g.n && (double)val < level) nib |= 1u << (x - x_first);
+      }
+    const uint32_t wid = G[0][0].in ? ((uint32_t)G[0][0].z * (uint32_t)g.n + (uint32_t)G[0][0].y) *
+                                              bw + ((uint32_t)x_first >> 5)
+                                        : 0xffffffffu;
+    uint32_t v = nib << ((uint32_t)x_first & 31u);
+    const uint32_t wl = threadIdx.x & 63u;
+#pragma unroll
+    for (int off = 1; off < 8; off <<= 1) {
+      const uint32_t ov = (uint32_t)__shfl_down((int)v, off, 64);
+      const uint32_t ow = (uint32_t)__shfl_down((int)wid, off, 64);
+      if (wl + (uint32_t)off < 64u && ow == wid) v |= ov;
+    }
+    const uint32_t pw_id = (uint32_t)__shfl_up((int)wid, 1, 64);
+    const bool head = wl == 0u || pw_id != wid;
+    if (head && v && wid != 0xffffffffu) atomicOr(&bits[wid], v);
   }
   if (partials) {
     const double total = block_sum(sq, lds);
@@ -506,6 +539,21 @@ struct ResidualGroup {
   }
 };
 
+struct ResidualSignBits {
+  struct Args {
+    ResidualGroup<double, double>::Args r;
+    const double* iso;
+    uint32_t* bits;
+    uint32_t bw;
+  };
+  static constexpr int kThreads = mpr::kThreads;
+  static __device__ void run(const Args& a, Blk b) {
+    residual_group_body<double, double, true>(b, a.r.X, a.r.R, a.r.S, a.r.out, a.r.g, a.r.lanes,
+                                              a.r.pw, a.r.h2, a.r.partials, a.r.live, a.iso,
+                                              a.bits, a.bw);
+  }
+};
+
 struct NormFinish {
   struct Args {
     const double* partials;
@@ -701,13 +749,19 @@ void launch_norm_finish(const double* partials, int count, double* out, hipStrea
   launch_one<NormFinish>({partials, count, out}, 1, s);
 }
 
+bool residual_norm_sets_sign_bits(const Geom& g) { return group_layout(g); }
+
 void launch_residual_norm_multi(const NormJob* jobs, size_t count, hipStream_t s) {
   std::vector<ResidualGroup<double, double>::Args> rg(count);
+  std::vector<ResidualSignBits::Args> rs(count);
+  std::vector<FillJob<uint32_t>> zero(count);
   std::vector<NormFinish::Args> nf(count);
-  std::vector<uint32_t> nb(count, 0u), one(count, 1u);
+  std::vector<uint32_t> nb(count, 0u), nbs(count, 0u), one(count, 1u);
+  bool any_sign = false;
   for (size_t i = 0; i < count; ++i) {
     const NormJob& j = jobs[i];
     nf[i] = {j.partials, j.nparts, j.out};
+    zero[i] = {nullptr, 0u, 0};
     if (!group_layout(j.g)) {  // small grids: the node-by-node kernel, launched alone
       launch_residual<double, double>(j.X, j.R, j.S, (double*)nullptr, j.g, j.pw, j.partials, s,
                                       j.live);
@@ -715,10 +769,22 @@ void launch_residual_norm_multi(const NormJob* jobs, size_t count, hipStream_t s
     }
     const uint32_t lanes = j.g.C / 2u;
     const double h2 = (double)(j.g.n - 1) * (double)(j.g.n - 1);
-    rg[i] = {j.X, j.R, j.S, nullptr, j.g, lanes, j.pw, h2, j.partials, j.live};
-    nb[i] = xcd_grid(lanes, kThreads);
+    const ResidualGroup<double, double>::Args r{j.X, j.R, j.S, nullptr, j.g, lanes, j.pw, h2,
+                                                j.partials, j.live};
+    if (j.sign_bits) {
+      const uint32_t bw = ((uint32_t)j.g.n + 31u) / 32u;
+      zero[i] = {j.sign_bits, 0u, (uint64_t)j.g.n * j.g.n * bw};
+      rs[i] = {r, j.iso, j.sign_bits, bw};
+      nbs[i] = xcd_grid(lanes, kThreads);
+      any_sign = true;
+    } else {
+      rg[i] = r;
+      nb[i] = xcd_grid(lanes, kThreads);
+    }
   }
+  if (any_sign) launch_fill_multi(zero.data(), count, s);
   launch_multi<ResidualGroup<double, double>>(rg, nb, s);
+  launch_multi<ResidualSignBits>(rs, nbs, s);
   launch_multi<NormFinish>(nf, one, s);
 }
 

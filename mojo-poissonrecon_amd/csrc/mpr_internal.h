@@ -76,7 +76,12 @@ struct NormJob {
   int nparts;
   double* out;
   const uint32_t* live;
+  // optional, where residual_norm_sets_sign_bits(g): the extraction's sign bits of X against
+  // *iso (n * n * ceil(n / 32) words, launch_bit_extract's layout), set in the same pass
+  const double* iso;
+  uint32_t* sign_bits;
 };
+bool residual_norm_sets_sign_bits(const Geom& g);
 void launch_residual_norm_multi(const NormJob* jobs, size_t count, hipStream_t s);
 template <class T>
 void launch_restrict(const T* F, const Geom& gf, T* Cc, const Geom& gc, double scale,
@@ -307,6 +312,7 @@ struct ExtractJob {
   int64_t* faces;
   uint64_t fcap;
   const double* xform;
+  bool bits_ready;  // b->bits already hold the sign bits (launch_residual_norm_multi)
 };
 void launch_bit_extract_multi(const ExtractJob* jobs, size_t count, hipStream_t s);
 

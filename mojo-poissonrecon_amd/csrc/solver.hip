@@ -213,7 +213,6 @@ struct Job {
   // finish, extraction and readback of several jobs, one launch per step for all of them
   static void finish_all(Job* const* jobs, const mpr_params* const* params, size_t count,
                          hipStream_t s);
-  void finish(const mpr_params& p, hipStream_t s);
   void enqueue_extract(bool world, hipStream_t s);
   // the parameters of the work last enqueued or launched on this job (eagerly or as a graph)
   void adopt(const mpr_params& p) {
@@ -690,22 +689,10 @@ void Job::setup_all(Job* const* jobs, const mpr_params* const* params, size_t co
   launch_fill_multi(x.data(), m, s);
 }
 
-// final FP64 residual norm, natural-layout field and iso value (core.py:289-303)
-void Job::finish(const mpr_params& p, hipStream_t s) {
-  if (!(p.stages & MPR_STAGE_SOLVE)) return;
-  // only the norm of the final residual is reported; the iso value and the extraction read
-  // the solution in the solver's layout, so the natural field is written for fetching only
-  const NormJob nj{L64[0].x, L64[0].rhs, L64[0].screen, geoms[0], p.point_weight, norm_part,
-                   norm_nparts, d_resnorm, live_bits};
-  launch_residual_norm_multi(&nj, 1, s);
-  if (p.stages & MPR_STAGE_GRIDS) launch_split_to_nat<double>(L64[0].x, field, geoms[0], s);
-  if (P)
-    launch_iso(split_field(L64[0].x, geoms[0]), unit_points(p), normals(p), P, n, p.confidence,
-               iso_part, d_iso, s, kept_count(p));
-  else
-    launch_fill<double>(d_iso, 0.0, 1, s);
-}
-
+// Final FP64 residual norm, natural-layout field and iso value (core.py:289-303), then the
+// extraction and the readback, of several jobs. Only the norm of the final residual is
+// reported; the iso value and the extraction read the solution in the solver's layout, so
+// the natural field is written for fetching only.
 void Job::finish_all(Job* const* jobs, const mpr_params* const* params, size_t count,
                      hipStream_t s) {
   std::vector<NormJob> norm;
@@ -716,9 +703,14 @@ void Job::finish_all(Job* const* jobs, const mpr_params* const* params, size_t c
   for (size_t i = 0; i < count; ++i) {
     Job& j = *jobs[i];
     const mpr_params& p = *params[i];
+    const bool extract = (p.stages & MPR_STAGE_EXTRACT) != 0;
+    bool fused = false;
     if (p.stages & MPR_STAGE_SOLVE) {
+      // the iso value first: the residual pass then also sets the extraction's sign bits
+      fused = extract && residual_norm_sets_sign_bits(j.geoms[0]);
       norm.push_back({j.L64[0].x, j.L64[0].rhs, j.L64[0].screen, j.geoms[0], p.point_weight,
-                      j.norm_part, j.norm_nparts, j.d_resnorm, j.live_bits});
+                      j.norm_part, j.norm_nparts, j.d_resnorm, j.live_bits,
+                      fused ? j.d_iso : nullptr, fused ? j.surf.bits : nullptr});
       if (p.stages & MPR_STAGE_GRIDS)
         launch_split_to_nat<double>(j.L64[0].x, j.field, j.geoms[0], s);
       if (j.P)
@@ -727,18 +719,18 @@ void Job::finish_all(Job* const* jobs, const mpr_params* const* params, size_t c
       else
         iso_zero.push_back({j.d_iso, 0.0, 1});
     }
-    if (p.stages & MPR_STAGE_EXTRACT) {
+    if (extract) {
       ext.push_back({split_field(j.L64[0].x, j.geoms[0]), j.density_view(), j.n, j.d_iso,
                      &j.surf, j.d_counts, j.verts, j.vdens, j.vcap, j.faces, j.fcap,
-                     (p.stages & MPR_STAGE_WORLD) ? j.d_params : nullptr});
+                     (p.stages & MPR_STAGE_WORLD) ? j.d_params : nullptr, fused});
       if (j.host_mesh)
         copy.push_back({j.verts, j.vdens, j.faces, j.d_counts, j.vcap, j.fcap, j.h_verts,
                         j.h_vdens, j.h_faces});
     }
   }
-  if (!norm.empty()) launch_residual_norm_multi(norm.data(), norm.size(), s);
   if (!iso.empty()) launch_iso_multi(iso.data(), iso.size(), s);
   if (!iso_zero.empty()) launch_fill_multi(iso_zero.data(), iso_zero.size(), s);
+  if (!norm.empty()) launch_residual_norm_multi(norm.data(), norm.size(), s);
   if (!ext.empty()) launch_bit_extract_multi(ext.data(), ext.size(), s);
   if (!copy.empty()) launch_mesh_to_host_multi(copy.data(), copy.size(), s);
   for (size_t i = 0; i < count; ++i) jobs[i]->readback(s);
@@ -753,9 +745,9 @@ void Job::enqueue(const mpr_params& p) {
   setup(p, s);
   if ((p.stages & MPR_STAGE_SOLVE) && P)
     run_cycles(vc64, precision == MPR_PRECISION_MIXED ? &vc32 : nullptr, &dtab, p, s);
-  finish(p, s);
-  if (p.stages & MPR_STAGE_EXTRACT) enqueue_extract((p.stages & MPR_STAGE_WORLD) != 0, s);
-  readback(s);
+  Job* self = this;
+  const mpr_params* pp = &p;
+  finish_all(&self, &pp, 1, s);
 }
 
 }  // namespace mpr
