@@ -37,7 +37,8 @@ constexpr uint32_t kSpan = kThreads * kNodesPerThread;
 // whether an instance's x counts as zero in this launch (ZeroStart)
 template <class T>
 __device__ __forceinline__ bool start_zero(const StepPtrs<T>& p, int zero_start) {
-  return zero_start == kZeroAll || (zero_start == kZeroBelowTop && !p.top);
+  return zero_start == kZeroAll || zero_start == kZeroAllDefectTop ||
+         (zero_start == kZeroBelowTop && !p.top);
 }
 
 template <class T>
@@ -168,6 +169,7 @@ __global__ __launch_bounds__(B) void kb_relax_group(const StepPtrs<T>* tab, uint
   const uint32_t slot = lb / nbi;
   if (slot >= count) return;
   const StepPtrs<T> p = tab[slot];
+  if (zero_start == kZeroAllDefectTop && p.x64) return;  // written by the defect kernel
   const uint32_t l0 = ((lb - slot * nbi) * B + threadIdx.x) * NG;
   const LiveMask live{p.live, p.live_rhs != 0};
   // the edge slot is loaded, not exchanged between lanes (group_edges): the shuffle waits
@@ -288,11 +290,25 @@ __global__ __launch_bounds__(B) void kb_relax_column(const StepPtrs<T>* tab, uin
 // C0: colour 0 only; the colour-1 residual words are left as they are (for a grid whose
 // last sweep was a colour-1 sweep, where the colour-1 residual is zero up to rounding and
 // the restriction reads it as zero).
-template <class T, class TO, bool KNOWN = false, bool C0 = false>
+// The FP32 V-cycle's first colour-0 sweep of a top level from x = 0, for colour-0 slot j of
+// a group at row (y, z) with first node x0: e = (r + 0 h2) / (deg h2 + (pw s) h2) with the
+// sweep's FP32 operands (r the FP32 defect, s the FP32 screening weight, h2 = (float) h^-2),
+// the values group_relax_one computes when every x word is +0.0
+__device__ __forceinline__ float first_sweep_value(float r, float s, int x, int y, int z, int n,
+                                                   float pw, float h2) {
+  const int deg = (int)(x > 0) + (int)(x + 1 < n) + (int)(y > 0) + (int)(y + 1 < n) +
+                  (int)(z > 0) + (int)(z + 1 < n);
+  const float screening = pw * s * h2;
+  const float diag = (float)deg * h2 + screening;
+  return (r + 0.0f * h2) / diag;
+}
+
+template <class T, class TO, bool KNOWN = false, bool C0 = false, bool E0 = false>
 __device__ __forceinline__ void residual_group(gptr<const T> X, gptr<const T> R, gptr<const T> S,
                                                gptr<TO> out, const Geom& g, uint32_t lane,
                                                uint32_t lanes, T pw, T h2, LiveMask live,
-                                               T xv = T(0)) {
+                                               T xv = T(0), gptr<float> e0 = nullptr,
+                                               float pwf = 0.f, float h2f = 0.f) {
   constexpr int V = kGroup<T>;
   SlotGroup<T, V> G0[1], G1[1];
   load_group<T, V>(X, R, S, g, 0, lane, lanes, true, G0[0], KNOWN, live, false, nullptr, xv);
@@ -303,6 +319,23 @@ __device__ __forceinline__ void residual_group(gptr<const T> X, gptr<const T> R,
   T v[V];
   group_residual<T, V>(G0[0], g.n, pw, h2, v);
   store_group<T, TO, V>(out + G0[0].own, v, G0[0].in ? (1u << V) - 1 : 0u);
+  if constexpr (E0) {
+    float e[V];
+    uint32_t mask = 0;
+#pragma unroll
+    for (int j = 0; j < V; ++j) {
+      const int x = G0[0].x0 + 2 * j;
+      const float s32 = (float)G0[0].s[j / SlotGroup<T, V>::W][j % SlotGroup<T, V>::W];
+      e[j] = first_sweep_value((float)v[j], s32, x, G0[0].y, G0[0].z, g.n, pwf, h2f);
+      if (G0[0].in && x < g.n) mask |= 1u << j;
+    }
+    static_assert(V == 2, "FP64 slot groups: two slots, one 8-byte FP32 store");
+    using F2 = float __attribute__((ext_vector_type(2)));
+    if (mask == 3u)
+      *(__attribute__((address_space(1))) F2*)(e0 + G0[0].own) = F2{e[0], e[1]};
+    else if (mask == 1u)
+      e0[G0[0].own] = e[0];
+  }
   if (C0) return;
   group_residual<T, V>(G1[0], g.n, pw, h2, v);
   store_group<T, TO, V>(out + G1[0].own, v, G1[0].in ? (1u << V) - 1 : 0u);
@@ -426,7 +459,8 @@ __global__ __launch_bounds__(kThreads) void kb_defect(const TopPtrs* tab, uint32
 __device__ __forceinline__ void initial_defect_group(gptr<const double> R, gptr<const double> S,
                                                      gptr<float> out, const Geom& g,
                                                      uint32_t lane, uint32_t lanes, double pw,
-                                                     double h2, const uint32_t* live) {
+                                                     double h2, const uint32_t* live,
+                                                     gptr<float> e0, float pwf, float h2f) {
   constexpr int V = kGroup<double>;
   static_assert(V == 2, "one double2 per colour and lane");
   if (lane >= lanes) return;
@@ -461,6 +495,15 @@ __device__ __forceinline__ void initial_defect_group(gptr<const double> R, gptr<
       v[j] = x < n ? (float)((j ? rv.y : rv.x) - applied) : 0.f;
     }
     *(__attribute__((address_space(1))) F2*)(out + own) = F2{v[0], v[1]};
+    if (e0 && c == 0) {  // the first colour-0 sweep (residual_group's E0)
+#pragma unroll
+      for (int j = 0; j < V; ++j) {
+        const int x = 2 * ((int)k0 + j) + p;
+        if (x < n)
+          e0[own + j] = first_sweep_value(v[j], (float)(j ? sv.y : sv.x), x, (int)y, (int)z, n,
+                                          pwf, h2f);
+      }
+    }
   }
 }
 
@@ -469,7 +512,8 @@ __device__ __forceinline__ void initial_defect_group(gptr<const double> R, gptr<
 // occupancy that hides its two dependent loads
 __global__ __launch_bounds__(kThreads) void kb_defect_initial(const TopPtrs* tab, uint32_t count,
                                                               uint32_t nbi, Geom g, uint32_t lanes,
-                                                              double pw, double h2) {
+                                                              double pw, double h2, int e0,
+                                                              float pwf, float h2f) {
   const uint32_t per = gridDim.x >> 3;
   const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
   const uint32_t slot = lb / nbi;
@@ -477,15 +521,17 @@ __global__ __launch_bounds__(kThreads) void kb_defect_initial(const TopPtrs* tab
   const TopPtrs p = tab[slot];
   if (!p.x_initial || *p.x_initial == 0) return;
   initial_defect_group(as_global(p.rhs64), as_global(p.screen64), as_global(p.r32), g,
-                       (lb - slot * nbi) * kThreads + threadIdx.x, lanes, pw, h2, p.live);
+                       (lb - slot * nbi) * kThreads + threadIdx.x, lanes, pw, h2, p.live,
+                       e0 ? as_global(p.e32) : gptr<float>(nullptr), pwf, h2f);
 }
 
 // FIRST: the first cycle's defect; entries whose x_initial flag is set are left to
 // kb_defect_initial
-template <int B, bool FIRST>
+template <int B, bool FIRST, bool E0>
 __global__ __launch_bounds__(B) void kb_defect_group(const TopPtrs* tab, uint32_t count,
                                                      uint32_t nbi, Geom g, uint32_t lanes,
-                                                     double pw, double h2) {
+                                                     double pw, double h2, float pwf,
+                                                     float h2f) {
   const uint32_t per = gridDim.x >> 3;
   const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
   const uint32_t slot = lb / nbi;
@@ -493,9 +539,10 @@ __global__ __launch_bounds__(B) void kb_defect_group(const TopPtrs* tab, uint32_
   const TopPtrs p = tab[slot];
   const uint32_t lane = (lb - slot * nbi) * B + threadIdx.x;
   if (FIRST && p.x_initial && *p.x_initial != 0) return;
-  residual_group<double, float>(as_global((const double*)p.x64), as_global(p.rhs64),
-                                as_global(p.screen64), as_global(p.r32), g, lane, lanes, pw, h2,
-                                LiveMask{p.live, true});
+  residual_group<double, float, false, false, E0>(
+      as_global((const double*)p.x64), as_global(p.rhs64), as_global(p.screen64),
+      as_global(p.r32), g, lane, lanes, pw, h2, LiveMask{p.live, true}, 0.0,
+      as_global(p.e32), pwf, h2f);
 }
 
 __global__ __launch_bounds__(kThreads) void kb_correct(const TopPtrs* tab, uint32_t count,
@@ -861,30 +908,44 @@ void launch_b_fill_x(const StepPtrs<T>* tab, uint32_t count, const Geom& g, hipS
 }
 
 void launch_b_defect(const TopPtrs* tab, uint32_t count, const Geom& g, double pw, bool first,
-                     hipStream_t s) {
+                     hipStream_t s, bool e0) {
   if (!count) return;
   const double h2 = (double)(g.n - 1) * (double)(g.n - 1);
+  // the FP32 sweep's own operands (launch_b_relax<float>)
+  const float pwf = (float)pw, h2f = (float)h2;
   if (group_layout(g)) {
     const uint32_t lanes = g.C / (uint32_t)kGroup<double>;
     const int block = tuned_block(kTuneDefect, true, size_class(g));
     const uint32_t nbi = blocks_for(lanes, (unsigned)block);  // both colours per lane
-#define MPR_LAUNCH_DEFECT_F(B, F)                                                           \
-  hipLaunchKernelGGL((kb_defect_group<B, F>), dim3(padded8((uint64_t)nbi * count)), dim3(B), \
-                     0, s, tab, count, nbi, g, lanes, pw, h2)
-#define MPR_LAUNCH_DEFECT1(B) MPR_LAUNCH_DEFECT_F(B, true)
-#define MPR_LAUNCH_DEFECT0(B) MPR_LAUNCH_DEFECT_F(B, false)
+#define MPR_LAUNCH_DEFECT_F(B, F, E)                                                            \
+  hipLaunchKernelGGL((kb_defect_group<B, F, E>), dim3(padded8((uint64_t)nbi * count)), dim3(B), \
+                     0, s, tab, count, nbi, g, lanes, pw, h2, pwf, h2f)
+#define MPR_LAUNCH_DEFECT1E(B) MPR_LAUNCH_DEFECT_F(B, true, true)
+#define MPR_LAUNCH_DEFECT0E(B) MPR_LAUNCH_DEFECT_F(B, false, true)
+#define MPR_LAUNCH_DEFECT1(B) MPR_LAUNCH_DEFECT_F(B, true, false)
+#define MPR_LAUNCH_DEFECT0(B) MPR_LAUNCH_DEFECT_F(B, false, false)
     if (first) {
       hipLaunchKernelGGL(kb_defect_initial, dim3(padded8((uint64_t)blocks_for(lanes, kThreads) *
                                                          count)),
                          dim3(kThreads), 0, s, tab, count, blocks_for(lanes, kThreads), g, lanes,
-                         pw, h2);
+                         pw, h2, e0 ? 1 : 0, pwf, h2f);
       MPR_LAUNCH_CHECK();
-      MPR_WITH_BLOCK(block, MPR_LAUNCH_DEFECT1)
+      if (e0) {
+        MPR_WITH_BLOCK(block, MPR_LAUNCH_DEFECT1E)
+      } else {
+        MPR_WITH_BLOCK(block, MPR_LAUNCH_DEFECT1)
+      }
     } else {
-      MPR_WITH_BLOCK(block, MPR_LAUNCH_DEFECT0)
+      if (e0) {
+        MPR_WITH_BLOCK(block, MPR_LAUNCH_DEFECT0E)
+      } else {
+        MPR_WITH_BLOCK(block, MPR_LAUNCH_DEFECT0)
+      }
     }
 #undef MPR_LAUNCH_DEFECT0
 #undef MPR_LAUNCH_DEFECT1
+#undef MPR_LAUNCH_DEFECT0E
+#undef MPR_LAUNCH_DEFECT1E
 #undef MPR_LAUNCH_DEFECT_F
     MPR_LAUNCH_CHECK();
     return;

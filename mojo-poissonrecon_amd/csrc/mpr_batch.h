@@ -34,7 +34,9 @@ struct StepPtrs {
 
 // First sweep of a V-cycle level from a known-zero x: 0 x as stored, 1 zero for instances
 // below their finest level, 2 zero for every instance
-enum ZeroStart { kZeroNone = 0, kZeroBelowTop = 1, kZeroAll = 2 };
+// 3 zero for every instance, and an instance with an x64 pointer (a mixed-precision top
+// level) skips the sweep: the defect kernel wrote the sweep's colour-0 values
+enum ZeroStart { kZeroNone = 0, kZeroBelowTop = 1, kZeroAll = 2, kZeroAllDefectTop = 3 };
 
 // Initial guess of the solve (core.py:270)
 constexpr double kInitialX = 0.5;
@@ -45,7 +47,7 @@ struct TopPtrs {
   const double* rhs64;
   const double* screen64;
   float* r32;
-  const float* e32;
+  float* e32;  // the FP32 correction (the top level's FP32 x)
   const uint32_t* live;  // live slots of rhs64 / screen64 (LiveMask, stencil.h), or null
   // device flag, set for a run that did not fill x64 with kInitialX: the first cycle's
   // defect reads x64 as kInitialX and its fused correction stores kInitialX + e (only for
@@ -92,7 +94,7 @@ template <class T>
 void launch_b_fill_x(const StepPtrs<T>* tab, uint32_t count, const Geom& g, hipStream_t s);
 // first: the first cycle's defect (TopPtrs::x_initial)
 void launch_b_defect(const TopPtrs* tab, uint32_t count, const Geom& g, double pw, bool first,
-                     hipStream_t s);
+                     hipStream_t s, bool e0 = false);
 void launch_b_correct(const TopPtrs* tab, uint32_t count, const Geom& g, hipStream_t s);
 template <class T>
 void launch_b_coarse(const CoarseEntry<T>* tab, uint32_t count, int top_class, double pw, int pre,
@@ -135,8 +137,10 @@ class BatchedVcycle {
   // the instance's x64 for every instance whose finest class is above unfused_class(); the
   // others need DefectTables::correct(s, unfused_class()). first: this is the first cycle
   // of the defect-correction loop (TopPtrs::x_initial).
+  // defect_e0 (with zero_tops, pre > 0, pw >= 0): the defect kernel already wrote each top
+  // level's first colour-0 sweep (DefectTables::defect's e0), which is not launched again.
   void run(double pw, int pre, int post, bool zero_tops, hipStream_t s,
-           bool fuse_correct = false, bool first = false) const;
+           bool fuse_correct = false, bool first = false, bool defect_e0 = false) const;
   // highest finest class without the fused correction: the LDS coarse classes and the
   // levels below 33^3, which the slot-group sweep does not cover
   int unfused_class() const { return sc > 4 ? sc : 4; }
@@ -187,8 +191,10 @@ class DefectTables {
  public:
   void build(const std::vector<int>& depths, const std::vector<TopPtrs>& tops, void* table_mem);
   static size_t table_bytes(size_t count) { return sizeof(TopPtrs) * count + 64 * 16; }
-  // r32 = b - A x64; first: the first cycle of the loop (TopPtrs::x_initial)
-  void defect(double pw, hipStream_t s, bool first = false) const;
+  // r32 = b - A x64; first: the first cycle of the loop (TopPtrs::x_initial). e0: also the
+  // FP32 V-cycle's first colour-0 sweep of the top level from x = 0, e32 = r32 / diagonal
+  // on colour 0, in that sweep's FP32 arithmetic (group layouts)
+  void defect(double pw, hipStream_t s, bool first = false, bool e0 = false) const;
   // x64 += e32 for the instances of finest class <= max_class (all by default)
   void correct(hipStream_t s, int max_class = 1 << 30) const;
 

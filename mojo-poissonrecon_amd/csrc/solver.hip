@@ -440,11 +440,14 @@ void run_cycles(const BatchedVcycle<double>& v64, const BatchedVcycle<float>* v3
                 const DefectTables* dt, const mpr_params& p, hipStream_t s) {
   for (int c = 0; c < p.cycles; ++c) {
     if (v32) {
-      dt->defect(p.point_weight, s, c == 0);  // r = b - A x (FP64), stored as FP32
+      // r = b - A x (FP64), stored as FP32; with pre-smoothing from e = 0 (point_weight >= 0)
+      // the same pass writes the top level's first colour-0 sweep, e = r / diagonal
+      const bool e0 = p.pre_smooth > 0 && p.point_weight >= 0.0;
+      dt->defect(p.point_weight, s, c == 0, e0);
       // e = V32(r) from e = 0, and x += e: with post-smoothing the top level's last sweep
       // adds e into x for instances deeper than the coarse classes, the rest use kb_correct
       const bool fuse = p.post_smooth > 0;
-      v32->run(p.point_weight, p.pre_smooth, p.post_smooth, true, s, fuse, c == 0);
+      v32->run(p.point_weight, p.pre_smooth, p.post_smooth, true, s, fuse, c == 0, e0);
       dt->correct(s, fuse ? v32->unfused_class() : 1 << 30);
     } else {
       v64.run(p.point_weight, p.pre_smooth, p.post_smooth, false, s);

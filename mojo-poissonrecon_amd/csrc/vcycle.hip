@@ -95,7 +95,7 @@ void BatchedVcycle<T>::build(const std::vector<InstanceLevels<T>>& inst, void* t
 
 template <class T>
 void BatchedVcycle<T>::run(double pw, int pre, int post, bool zero_tops, hipStream_t s,
-                           bool fuse_correct, bool first) const {
+                           bool fuse_correct, bool first, bool defect_e0) const {
   // A level that starts at zero (core.py:174 solution[l+1].fill(0); every level of the
   // correction equation when zero_tops) needs no fill when pre-smoothing runs: the first
   // colour-0 sweep reads the other colour as zero, and every node of that colour has a
@@ -105,7 +105,9 @@ void BatchedVcycle<T>::run(double pw, int pre, int post, bool zero_tops, hipStre
   const bool fill = pre <= 0 || pw < 0.0;
   if (zero_tops && fill)
     for (int c = sc + 1; c <= smax; ++c) launch_b_fill_x<T>(top_[c], top_count_[c], geom_[c], s);
-  const int zero_start = fill ? kZeroNone : (zero_tops ? kZeroAll : kZeroBelowTop);
+  const int zero_start =
+      fill ? kZeroNone
+           : (zero_tops ? (defect_e0 ? kZeroAllDefectTop : kZeroAll) : kZeroBelowTop);
   for (int c = smax; c > sc; --c) {
     if (!step_count_[c]) continue;
     launch_b_relax<T>(step_[c], step_count_[c], geom_[c], pw, pre, s, zero_start);
@@ -151,9 +153,9 @@ void DefectTables::build(const std::vector<int>& depths, const std::vector<TopPt
     MPR_HIP(hipMemcpy(table_mem, w.host.data(), w.host.size(), hipMemcpyHostToDevice));
 }
 
-void DefectTables::defect(double pw, hipStream_t s, bool first) const {
+void DefectTables::defect(double pw, hipStream_t s, bool first, bool e0) const {
   for (size_t d = 1; d < count_.size(); ++d)
-    launch_b_defect(tab_[d], count_[d], geom_[d], pw, first, s);
+    launch_b_defect(tab_[d], count_[d], geom_[d], pw, first, s, e0);
 }
 
 void DefectTables::correct(hipStream_t s, int max_class) const {
