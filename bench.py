@@ -202,7 +202,7 @@ def end_to_end(points, normals, clouds, depth, steps, warmup) -> dict:
     download) per request, and the batched form with persistent jobs (upload every cloud,
     one launch, download every mesh) per batch."""
     import mojo_poissonrecon as mpr
-    from mojo_poissonrecon.core import mesh_from_job
+    from mojo_poissonrecon.batch import meshes_from_jobs, upload_all
 
     out = {}
     for prec in ("fp64", "mixed"):
@@ -214,10 +214,9 @@ def end_to_end(points, normals, clouds, depth, steps, warmup) -> dict:
                                host_mesh=True)
 
     def step():
-        for j, (p, n) in zip(jobs, clouds):
-            j.set_points(p, n)
+        upload_all(jobs, clouds)
         b.run([r.params for r in reqs])
-        return [mesh_from_job(j, res) for j, res in zip(jobs, b.wait())]
+        return meshes_from_jobs(jobs, b.wait())
 
     total, lat = timed(step, steps, warmup)
     out["batch_mixed"] = {"recon_per_s": round(len(clouds) * steps / total, 3),

@@ -11,12 +11,40 @@ runs.
 from __future__ import annotations
 
 import ctypes
+import threading
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
 from . import _lib
 from ._lib import check, lib
 from .core import Job, Mesh, mesh_from_job, prepare_reconstruct
+
+
+_pool = None
+_pool_lock = threading.Lock()
+
+
+def host_pool() -> ThreadPoolExecutor:
+    """Threads for the host side of a batch: uploads and mesh copies release the GIL in the
+    library (hipMemcpy staging, memcpy out of pinned memory and the page faults of fresh
+    NumPy arrays), so several run at once."""
+    global _pool
+    with _pool_lock:
+        if _pool is None:
+            _pool = ThreadPoolExecutor(max_workers=8, thread_name_prefix="mpr-host")
+        return _pool
+
+
+def upload_all(jobs, clouds) -> None:
+    """set_points of every (job, (points, normals)) pair, on the host pool."""
+    list(host_pool().map(lambda jc: jc[0].set_points(*jc[1]), zip(jobs, clouds)))
+
+
+def meshes_from_jobs(jobs, results, counts=None) -> list[Mesh]:
+    """mesh_from_job of every job of a finished batch, on the host pool."""
+    counts = [None] * len(jobs) if counts is None else counts
+    return list(host_pool().map(lambda a: mesh_from_job(*a), zip(jobs, results, counts)))
 
 
 class Batch:
@@ -97,8 +125,7 @@ def reconstruct_batch(clouds, *, depths, point_weight: float = 4.0, confidence=F
         return meshes
     jobs = [Job(reqs[i].depth, len(reqs[i].points), reqs[i].precision) for i in live]
     try:
-        for j, i in zip(jobs, live):
-            j.set_points(reqs[i].points, reqs[i].normals)
+        upload_all(jobs, [(reqs[i].points, reqs[i].normals) for i in live])
         batch = Batch(jobs)
         try:
             params = [reqs[i].params for i in live]
@@ -110,8 +137,9 @@ def reconstruct_batch(clouds, *, depths, point_weight: float = 4.0, confidence=F
             results = batch.wait()
         finally:
             batch.close()
-        for j, i, r in zip(jobs, live, results):
-            meshes[i] = mesh_from_job(j, r, len(reqs[i].points))
+        for i, m in zip(live, meshes_from_jobs(jobs, results,
+                                               [len(reqs[i].points) for i in live])):
+            meshes[i] = m
     finally:
         for j in jobs:
             j.close()
