@@ -767,7 +767,8 @@ inline unsigned padded8(uint64_t blocks) { return (unsigned)((blocks + 7u) & ~7u
 
 template <class T>
 void launch_b_relax(const StepPtrs<T>* tab, uint32_t count, const Geom& g, double pw,
-                    int iterations, hipStream_t s, int zero_start, bool correct, bool first) {
+                    int iterations, hipStream_t s, int zero_start, bool correct, bool first,
+                    bool skip_first) {
   if (!count || !iterations) return;
   const T h2 = (T)((double)(g.n - 1) * (double)(g.n - 1));
   if (group_layout(g)) {
@@ -781,6 +782,7 @@ void launch_b_relax(const StepPtrs<T>* tab, uint32_t count, const Geom& g, doubl
     const uint32_t col_nbi = blocks_for(col_lanes, (unsigned)block);
     for (int it = 0; it < iterations; ++it)
       for (int c = 0; c < 2; ++c) {
+        if (skip_first && it == 0 && c == 0) continue;
         const int zs = it == 0 && c == 0 ? zero_start : (int)kZeroNone;
         const int corr = correct && sizeof(T) == 4 && it == iterations - 1 && c == 1 ? 1 : 0;
         if (nz > 1 && zs == kZeroNone && !corr) {
@@ -821,6 +823,7 @@ void launch_b_relax(const StepPtrs<T>* tab, uint32_t count, const Geom& g, doubl
   const uint32_t nbi = blocks_for(g.C, kSpan);
   for (int it = 0; it < iterations; ++it)
     for (int c = 0; c < 2; ++c) {
+      if (skip_first && it == 0 && c == 0) continue;
       hipLaunchKernelGGL(kb_relax_color<T>, dim3(padded8((uint64_t)nbi * count)), dim3(kThreads),
                          0, s, tab, count, nbi, g, c, (T)pw, h2,
                          it == 0 && c == 0 ? zero_start : (int)kZeroNone);
@@ -1018,7 +1021,7 @@ void init_coarse_kernels() {
 
 #define MPR_BINST(T)                                                                           \
   template void launch_b_relax<T>(const StepPtrs<T>*, uint32_t, const Geom&, double, int,      \
-                                  hipStream_t, int, bool, bool);                               \
+                                  hipStream_t, int, bool, bool, bool);                               \
   template void launch_b_residual<T>(const StepPtrs<T>*, uint32_t, const Geom&, double,        \
                                      hipStream_t, bool);                                       \
   template void launch_b_restrict<T>(const StepPtrs<T>*, uint32_t, const Geom&, const Geom&,   \

@@ -72,9 +72,11 @@ struct CoarseEntry {
 // an x64 pointer into it (x64 += e for both colours) and does not store e there (FP32 only);
 // first: that is the first cycle's correction (TopPtrs::x_initial)
 template <class T>
+// skip_first: the first colour-0 sweep is not launched (every instance of the table takes
+// kZeroAllDefectTop's skip)
 void launch_b_relax(const StepPtrs<T>* tab, uint32_t count, const Geom& g, double pw,
                     int iterations, hipStream_t s, int zero_start = kZeroNone,
-                    bool correct = false, bool first = false);
+                    bool correct = false, bool first = false, bool skip_first = false);
 // colour0_only: the residual of colour 0 alone (slot-group layout only); colour1_zero: the
 // restriction reads the fine colour 1 as zero (FP32 with the slot-group layout on both grids
 // only). Together they serve a grid whose last sweep was a colour-1 sweep, after which the

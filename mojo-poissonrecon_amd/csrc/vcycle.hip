@@ -110,7 +110,10 @@ void BatchedVcycle<T>::run(double pw, int pre, int post, bool zero_tops, hipStre
            : (zero_tops ? (defect_e0 ? kZeroAllDefectTop : kZeroAll) : kZeroBelowTop);
   for (int c = smax; c > sc; --c) {
     if (!step_count_[c]) continue;
-    launch_b_relax<T>(step_[c], step_count_[c], geom_[c], pw, pre, s, zero_start);
+    // every instance of the class at its top: the defect pass wrote the first sweep
+    const bool skip = zero_start == kZeroAllDefectTop && top_count_[c] == step_count_[c];
+    launch_b_relax<T>(step_[c], step_count_[c], geom_[c], pw, pre, s, zero_start, false, false,
+                      skip);
     // FP32 after pre-smoothing: the colour-1 residual is zero up to rounding, and only
     // colour 0 is computed and restricted
     const bool c0 = colour0_residual(c, pw, pre);
