@@ -470,7 +470,8 @@ __device__ __forceinline__ uint32_t even_bits(uint64_t x) {
 // nf^3 bits). One lane per word of the coarse bit set, which holds cw = ceil(nc / 32) words
 // per coarse row: coarse x 32 w .. 32 w + 31 of row (Z, Y) take fine x 64 w - 1 .. 64 w + 63.
 __device__ __forceinline__ void coarse_support_body(Blk blk_, const uint32_t* fine, int nf,
-                                                    int nc, uint32_t cw, uint32_t* out) {
+                                                    int nc, uint32_t cw, uint32_t* out,
+                                                    uint32_t* list, uint32_t* count) {
   const uint64_t q = (uint64_t)blk_.id * kThreads + threadIdx.x;
   const uint64_t total = (uint64_t)nc * nc * cw;
   if (q >= total) return;
@@ -497,27 +498,34 @@ __device__ __forceinline__ void coarse_support_body(Blk blk_, const uint32_t* fi
   const uint32_t x_first = 32u * w;
   if ((uint32_t)nc - x_first < 32u) acc &= (1u << ((uint32_t)nc - x_first)) - 1u;
   out[q] = acc;
+  if (acc) list[atomicAdd(count, 1u)] = (uint32_t)q;  // list order is free: nodes are independent
 }
 
 // restrict_nodes_body over the coarse nodes set in `support` (coarse_support_body of the
 // fine nodes that this or the previous run may have written); every other coarse node keeps
-// the +0.0 it holds: the restriction of +0.0 words
+// the +0.0 it holds: the restriction of +0.0 words. A fixed grid walks the list of non-zero
+// support words, 32 lanes per word (coarse x = 32 w + lane of row (Z, Y)).
 template <class T>
 __device__ __forceinline__ void restrict_sparse_body(Blk blk_, const T* F, Geom gf, T* Cc, Geom gc,
                                                      T scale, float* copy32,
-                                                     const uint32_t* support) {
-  const uint32_t t = xcd_block(blk_) * kThreads + threadIdx.x;
-  if (t >= 2u * gc.C) return;
-  const int c = (int)(t & 1u);
-  const uint32_t i = (uint32_t)c * gc.C + (t >> 1);
-  int X, Y, Z;
-  if (!split_decode(gc, c, t >> 1, X, Y, Z)) return;
+                                                     const uint32_t* support,
+                                                     const uint32_t* list, const uint32_t* count) {
   const uint32_t cw = ((uint32_t)gc.n + 31u) / 32u;
-  const uint32_t word = support[((uint32_t)Z * (uint32_t)gc.n + (uint32_t)Y) * cw + ((uint32_t)X >> 5)];
-  if (!((word >> ((uint32_t)X & 31u)) & 1u)) return;
-  const T v = restrict_node(F, gf, X, Y, Z, scale);
-  Cc[i] = v;
-  if (copy32) copy32[i] = (float)v;
+  const uint32_t total = *count;
+  const uint32_t sub = threadIdx.x & 31u;
+  const uint32_t slots = blk_.count * (kThreads / 32);
+  for (uint32_t k = blk_.id * (kThreads / 32) + (threadIdx.x >> 5); k < total; k += slots) {
+    const uint32_t q = list[k];
+    const uint32_t word = support[q];
+    if (!((word >> sub) & 1u)) continue;
+    const uint32_t zy = q / cw, w = q - zy * cw;
+    const int Z = (int)(zy / (uint32_t)gc.n), Y = (int)(zy - (uint32_t)Z * (uint32_t)gc.n);
+    const int X = (int)(32u * w + sub);
+    const uint32_t i = split_index(gc, X, Y, Z);
+    const T v = restrict_node(F, gf, X, Y, Z, scale);
+    Cc[i] = v;
+    if (copy32) copy32[i] = (float)v;
+  }
 }
 
 // ---- multi-instance kernel bodies (multi.h) ---------------------------------------------
@@ -602,11 +610,12 @@ struct RestrictSparse {
     Geom gc;
     T scale;
     float* copy32;
-    const uint32_t* support;
+    const uint32_t *support, *list, *count;
   };
   static constexpr int kThreads = mpr::kThreads;
   static __device__ void run(const Args& a, Blk b) {
-    restrict_sparse_body<T>(b, a.F, a.gf, a.Cc, a.gc, a.scale, a.copy32, a.support);
+    restrict_sparse_body<T>(b, a.F, a.gf, a.Cc, a.gc, a.scale, a.copy32, a.support, a.list,
+                            a.count);
   }
 };
 
@@ -615,11 +624,11 @@ struct CoarseSupport {
     const uint32_t* fine;
     int nf, nc;
     uint32_t cw;
-    uint32_t* out;
+    uint32_t *out, *list, *count;
   };
   static constexpr int kThreads = mpr::kThreads;
   static __device__ void run(const Args& a, Blk b) {
-    coarse_support_body(b, a.fine, a.nf, a.nc, a.cw, a.out);
+    coarse_support_body(b, a.fine, a.nf, a.nc, a.cw, a.out, a.list, a.count);
   }
 };
 
@@ -809,23 +818,35 @@ void launch_restrict_nodes_multi(const RestrictJob<T>* jobs, size_t count, hipSt
   std::vector<typename RestrictNodes<T>::Args> a(count);
   std::vector<typename RestrictSparse<T>::Args> sp(count);
   std::vector<CoarseSupport::Args> cs(count);
+  std::vector<FillJob<uint32_t>> zero(count);
   std::vector<uint32_t> nb(count, 0u), nbs(count, 0u), nbc(count, 0u);
+  bool any_sparse = false;
   for (size_t i = 0; i < count; ++i) {
     const RestrictJob<T>& j = jobs[i];
+    zero[i] = {nullptr, 0u, 0};
     if (!j.F) continue;
     const uint32_t blocks = xcd_grid(2ull * j.gc.C, kThreads);
     if (j.need) {
       const uint32_t cw = ((uint32_t)j.gc.n + 31u) / 32u;
-      cs[i] = {j.need, j.gf.n, j.gc.n, cw, j.support};
-      nbc[i] = blocks_for((uint64_t)j.gc.n * j.gc.n * cw, kThreads);
-      sp[i] = {j.F, j.gf, j.Cc, j.gc, (T)j.scale, j.copy32, j.support};
-      nbs[i] = blocks;
+      const uint64_t words = (uint64_t)j.gc.n * j.gc.n * cw;
+      // the support words, then the list of the non-zero ones and its length
+      uint32_t* list = j.support + words;
+      uint32_t* len = list + words;
+      zero[i] = {len, 0u, 1};
+      cs[i] = {j.need, j.gf.n, j.gc.n, cw, j.support, list, len};
+      nbc[i] = blocks_for(words, kThreads);
+      sp[i] = {j.F, j.gf, j.Cc, j.gc, (T)j.scale, j.copy32, j.support, list, len};
+      // a fixed grid (graph-capturable) walking the list, 8 words per block and round
+      nbs[i] = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(1, (words + 7) / 8), 2048);
+      any_sparse = true;
     } else {
       a[i] = {j.F, j.gf, j.Cc, j.gc, (T)j.scale, j.copy32};
       nb[i] = blocks;
     }
   }
   launch_multi<RestrictNodes<T>>(a, nb, s);
+  if (!any_sparse) return;
+  launch_fill_multi(zero.data(), count, s);
   launch_multi<CoarseSupport>(cs, nbc, s);
   launch_multi<RestrictSparse<T>>(sp, nbs, s);
 }

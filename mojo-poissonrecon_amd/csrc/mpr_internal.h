@@ -44,8 +44,9 @@ struct RestrictJob {
   const uint32_t* need;
   uint32_t* support;
 };
+// the support words, a list of as many word indices and the list's length
 inline uint64_t coarse_support_words(int nc) {
-  return (uint64_t)nc * nc * (((uint64_t)nc + 31) / 32);
+  return 2 * (uint64_t)nc * nc * (((uint64_t)nc + 31) / 32) + 1;
 }
 template <class T>
 void launch_restrict_nodes_multi(const RestrictJob<T>* jobs, size_t count, hipStream_t s);
