@@ -395,49 +395,50 @@ __device__ __forceinline__ void assemble_dense_body(Blk blk_, AssembleArgs a, Ge
   if (a.live) wave_set_bits(a.live, live, si);
 }
 
-// Sparse assembly, step 1: lists the 64-node row chunks (chunk k of row zy: x in
-// [64k, 64k+64)) that hold a node of the need mask (list order is free: chunks are
-// independent), and records this run's touched bits for the next run's need mask (the need
-// mask was computed from the old ones by the previous kernel).
-__device__ __forceinline__ void need_chunks_body(Blk blk_, const uint32_t* need,
-                                                          const uint32_t* touched,
-                                                          uint32_t* prev, uint32_t words, int n,
-                                                          uint32_t per_row, uint32_t chunks,
-                                                          uint32_t* list, uint32_t* count) {
-  const uint32_t tid = blk_.id * kThreads + threadIdx.x;
-  for (uint32_t w = tid; w < words; w += blk_.count * kThreads) prev[w] = touched[w];
-  if (tid >= chunks) return;
-  const uint32_t zy = tid / per_row, k = tid - zy * per_row;
-  const uint32_t x0 = 64u * k;
-  const uint32_t len = min(64u, (uint32_t)n - x0);
-  if (any_bit(need, zy * (uint32_t)n + x0, len)) list[atomicAdd(count, 1u)] = tid;
+// Sparse assembly, step 1: lists the words of the need mask that hold a node (list order is
+// free: nodes are independent), and records this run's touched bits for the next run's need
+// mask (the need mask was computed from the old ones by the previous kernel).
+__device__ __forceinline__ void need_words_body(Blk blk_, const uint32_t* need,
+                                                const uint32_t* touched, uint32_t* prev,
+                                                uint32_t words, uint32_t* list,
+                                                uint32_t* count) {
+  const uint32_t w = blk_.id * kThreads + threadIdx.x;
+  if (w >= words) return;
+  prev[w] = touched[w];
+  if (need[w]) list[atomicAdd(count, 1u)] = w;
 }
 
-// Sparse assembly, step 2: one wavefront per listed chunk (a lane per node), for every
-// listed chunk: nodes of a chunk outside the need mask get the assembly's own value, +0.0
-__device__ __forceinline__ void assemble_chunks_body(Blk blk_, AssembleArgs a, Geom g,
-                                                     uint32_t per_row, const uint32_t* list,
-                                                     const uint32_t* count,
-                                                     const uint32_t* need) {
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t waves = blk_.count * (kThreads / 64);
+// Sparse assembly, step 2: a fixed grid walks the listed need words, 32 lanes per word
+// (node t = 32 w + lane, natural layout); a node outside the need mask has no touched
+// neighbour now or in the previous run, so its assembled words are the +0.0 they already
+// hold and it is skipped.
+__device__ __forceinline__ void assemble_words_body(Blk blk_, AssembleArgs a, Geom g,
+                                                    FastDiv div_n, const uint32_t* list,
+                                                    const uint32_t* count,
+                                                    const uint32_t* need) {
+  const uint32_t sub = threadIdx.x & 31u;
+  const uint32_t slots = blk_.count * (kThreads / 32);
   const uint32_t total = *count;
-  const int n = g.n;
-  for (uint32_t i = blk_.id * (kThreads / 64) + (threadIdx.x >> 6); i < total; i += waves) {
-    const uint32_t chunk = list[i];
-    const uint32_t zy = chunk / per_row, k = chunk - zy * per_row;
-    const int x = (int)(64u * k + lane);
+  const uint64_t m = (uint64_t)g.n * g.n * g.n;
+  // k is uniform over each half-wavefront, and wave_set_bits needs every lane of the
+  // wavefront: the loop runs to the bound of the wavefront's first half-wave slot
+  const uint32_t first = blk_.id * (kThreads / 32) + ((threadIdx.x >> 6) << 1);
+  for (uint32_t base = first; base < total; base += slots) {
+    const uint32_t k = base + ((threadIdx.x >> 5) & 1u);
     bool live = false;
     uint32_t si = 0u;
-    // a node outside the need mask has no touched neighbour now or in the previous run, so
-    // its assembled words are the +0.0 they already hold: it is skipped
-    const uint32_t t = zy * (uint32_t)n + (uint32_t)x;
-    if (x < n && ((need[t >> 5] >> (t & 31u)) & 1u)) {
-      const int z = (int)(zy / (uint32_t)n);
-      const int y = (int)(zy - (uint32_t)z * (uint32_t)n);
-      live = assemble_node(a, g, t, x, y, z, si);
+    if (k < total) {
+      const uint32_t w = list[k];
+      const uint32_t t = 32u * w + sub;
+      if (t < m && ((need[w] >> sub) & 1u)) {
+        const uint32_t zy = div_n.div(t);
+        const int x = (int)(t - zy * (uint32_t)g.n);
+        const int z = (int)div_n.div(zy);
+        const int y = (int)(zy - (uint32_t)z * (uint32_t)g.n);
+        live = assemble_node(a, g, t, x, y, z, si);
+      }
     }
-    if (a.live) wave_set_bits(a.live, live, si);  // i is uniform over the wavefront
+    if (a.live) wave_set_bits(a.live, live, si);
   }
 }
 
@@ -644,32 +645,29 @@ struct NeedMask {
   }
 };
 
-struct NeedChunks {
+struct NeedWords {
   struct Args {
     const uint32_t *need, *touched;
     uint32_t* prev;
     uint32_t words;
-    int n;
-    uint32_t per_row, chunks;
     uint32_t *list, *count;
   };
   static constexpr int kThreads = mpr::kThreads;
   static __device__ void run(const Args& a, Blk b) {
-    need_chunks_body(b, a.need, a.touched, a.prev, a.words, a.n, a.per_row, a.chunks, a.list,
-                     a.count);
+    need_words_body(b, a.need, a.touched, a.prev, a.words, a.list, a.count);
   }
 };
 
-struct AssembleChunks {
+struct AssembleWords {
   struct Args {
     AssembleArgs a;
     Geom g;
-    uint32_t per_row;
+    FastDiv div_n;
     const uint32_t *list, *count, *need;
   };
   static constexpr int kThreads = mpr::kThreads;
   static __device__ void run(const Args& a, Blk b) {
-    assemble_chunks_body(b, a.a, a.g, a.per_row, a.list, a.count, a.need);
+    assemble_words_body(b, a.a, a.g, a.div_n, a.list, a.count, a.need);
   }
 };
 
@@ -899,8 +897,8 @@ void launch_assemble_multi(const AssembleJob* jobs, size_t count, hipStream_t s)
   std::vector<AssembleDense::Args> dense(count);
   std::vector<NeedMask::Args> nm(count);
   std::vector<FillJob<uint32_t>> zero(count);
-  std::vector<NeedChunks::Args> nc(count);
-  std::vector<AssembleChunks::Args> ac(count);
+  std::vector<NeedWords::Args> nc(count);
+  std::vector<AssembleWords::Args> ac(count);
   std::vector<uint32_t> nb_dense(count, 0u), nb_mask(count, 0u), nb_need(count, 0u),
       nb_asm(count, 0u);
   bool any_dense = false, any_sparse = false;
@@ -928,23 +926,20 @@ void launch_assemble_multi(const AssembleJob* jobs, size_t count, hipStream_t s)
     const uint32_t words = (uint32_t)((m + 31) / 32);
     nm[i] = {j.touched, hist->prev, (int64_t)words, (int64_t)g.n, (int64_t)g.n * g.n, hist->need};
     nb_mask[i] = blocks_for(words, kThreads);
-    const uint32_t per_row = assemble_chunks_per_row(g.n);
-    const uint32_t chunks = (uint32_t)g.n * (uint32_t)g.n * per_row;
     zero[i] = {hist->count, 0u, 1};
-    nc[i] = {hist->need, j.touched, hist->prev, words, g.n, per_row, chunks, hist->list,
-             hist->count};
-    nb_need[i] = blocks_for(chunks, kThreads);
-    ac[i] = {a, g, per_row, hist->list, hist->count, hist->need};
-    // a fixed grid (graph-capturable) whose wavefronts walk the chunk list
-    nb_asm[i] = std::min<unsigned>(blocks_for(chunks, kThreads / 64), 4096u);
+    nc[i] = {hist->need, j.touched, hist->prev, words, hist->list, hist->count};
+    nb_need[i] = blocks_for(words, kThreads);
+    ac[i] = {a, g, FastDiv((uint32_t)g.n), hist->list, hist->count, hist->need};
+    // a fixed grid (graph-capturable) walking the word list, 8 words per block and round
+    nb_asm[i] = std::min<unsigned>(blocks_for(words, 8), 4096u);
     any_sparse = true;
   }
   if (any_dense) launch_multi<AssembleDense>(dense, nb_dense, s);
   if (!any_sparse) return;
   launch_multi<NeedMask>(nm, nb_mask, s);
   launch_fill_multi(zero.data(), count, s);
-  launch_multi<NeedChunks>(nc, nb_need, s);
-  launch_multi<AssembleChunks>(ac, nb_asm, s);
+  launch_multi<NeedWords>(nc, nb_need, s);
+  launch_multi<AssembleWords>(ac, nb_asm, s);
 }
 
 void launch_assemble(const double* vx, const double* vy, const double* vz,

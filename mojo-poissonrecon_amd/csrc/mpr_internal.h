@@ -118,7 +118,7 @@ struct TouchedHistory {
   uint32_t* prev;
   uint32_t* need;
   bool write_all;
-  uint32_t* list;   // n * n * assemble_chunks_per_row(n) words: the 64-node row chunks written
+  uint32_t* list;   // ceil(n^3 / 32) words: the need-mask words holding a node
   uint32_t* count;  // one word: their number
 };
 inline uint32_t assemble_chunks_per_row(int n) { return (uint32_t)(n + 63) / 64u; }

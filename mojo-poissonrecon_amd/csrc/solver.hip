@@ -136,7 +136,7 @@ struct Job {
   // the previous assembly's touched bits and the sparse assembly's need mask (same size)
   uint32_t *prev_touched = nullptr, *need_bits = nullptr;
   uint32_t* support_bits = nullptr;  // coarse support of need_bits (the first pyramid step)
-  uint32_t *chunk_list = nullptr, *chunk_count = nullptr;  // the sparse assembly's chunks
+  uint32_t *chunk_list = nullptr, *chunk_count = nullptr;  // the sparse assembly's words
   // live slots of the finest level (LiveMask, stencil.h): rhs or screen non-zero
   uint32_t* live_bits = nullptr;
   int* x_initial = nullptr;  // the FP64 solution was not filled with kInitialX (TopPtrs)
@@ -519,7 +519,7 @@ void Job::plan(DevArena& a) {
   prev_touched = a.alloc<uint32_t>((M + 31) / 32);
   need_bits = a.alloc<uint32_t>((M + 31) / 32);
   support_bits = geoms.size() > 1 ? a.alloc<uint32_t>(coarse_support_words(geoms[1].n)) : nullptr;
-  chunk_list = a.alloc<uint32_t>((uint64_t)n * n * assemble_chunks_per_row(n));
+  chunk_list = a.alloc<uint32_t>((M + 31) / 32);
   chunk_count = a.alloc<uint32_t>(1);
   live_bits = a.alloc<uint32_t>(live_words());
   x_initial = a.alloc<int>(1);
