@@ -231,8 +231,9 @@ int mpr_batch_wait(mpr_batch *batch, mpr_result *results);
 /* Kernel timing of a batch on its own stream with HIP events (as mpr_job_time_kernel, after
  * a run): one launch covers every job of the batch's finest depth. Kernels 7 (FP32 colour
  * sweep), 9 (FP32 residual), 10 (FP32 restriction of the finest residual), 11 (FP32
- * colour-1 prolongation into the finest level) and 13 (FP64->FP32 defect; jobs of one
- * depth) for a mixed batch, 8 (FP64 colour sweep) for an FP64 one. bytes_per_launch sums the jobs' algorithmic
+ * colour-1 prolongation into the finest level), 13 (FP64->FP32 defect; jobs of one
+ * depth) and 14 (fused smoothing, two red-black iterations in one pass) for a mixed batch,
+ * 8 (FP64 colour sweep) for an FP64 one. bytes_per_launch sums the jobs' algorithmic
  * bytes. No reference counterpart (measurement hook). */
 int mpr_batch_time_kernel(mpr_batch *batch, int32_t kernel, int32_t reps, double *ms_per_launch,
                           double *bytes_per_launch);
@@ -244,9 +245,10 @@ int mpr_batch_time_kernel(mpr_batch *batch, int32_t kernel, int32_t reps, double
  * 4 FP64 prolongation, 5 FP32 residual, 6 FP64->FP32 defect residual; the batched V-cycle
  * kernels the pipeline itself launches: 7 FP32 colour sweep, 8 FP64 colour sweep, 9 FP32
  * residual, 10 FP32 restriction, 11 FP32 prolongation of colour 1, 12 FP64 restriction,
- * 13 FP64->FP32 defect residual. Kernels 7, 8, 9 and 13 skip the right-hand-side and
- * screening words that are +0.0 away from the splat (live-slot bits), and their byte
- * counts include only the words they load. No reference counterpart (measurement hook). */
+ * 13 FP64->FP32 defect residual, 14 FP32 fused smoothing (two red-black iterations of the
+ * finest level in one pass, mpr_tune_set_march). Kernels 7, 8, 9, 13 and 14 skip the
+ * right-hand-side and screening words that are +0.0 away from the splat (live-slot bits),
+ * and their byte counts include only the words they load. No reference counterpart (measurement hook). */
 int mpr_job_time_kernel(mpr_job *job, int32_t kernel, int32_t reps, double *ms_per_launch,
                         double *bytes_per_launch);
 
@@ -278,6 +280,23 @@ int mpr_tune_set_groups(int32_t kernel, int32_t fp64, int32_t size_class, int32_
  * groups setting), or -1; and its setter (0 or -1) */
 int mpr_tune_get_planes(int32_t kernel, int32_t fp64, int32_t size_class);
 int mpr_tune_set_planes(int32_t kernel, int32_t fp64, int32_t size_class, int32_t planes);
+/* Fused smoothing of the mixed-precision V-cycle: on FP32 levels with 16-byte aligned rows
+ * (n >= 33), `iterations` red-black iterations of each smoothing phase run as one pass over
+ * the level (a z-marching kernel holding a ring of planes on chip) instead of 2 x iterations
+ * single colour sweeps: 0 (every sweep on its own), 1, or 2 (the default). Results are
+ * bit-identical for every setting. It applies to jobs and batches created afterwards. No
+ * reference counterpart (a schedule of relax_f64's arithmetic, capi.mojo:309-410). get
+ * returns the setting, set 0 or -1. */
+int mpr_tune_get_march(void);
+int mpr_tune_set_march(int32_t iterations);
+/* Test hook of the fused smoothing's division: on a node without screening weight the
+ * diagonal is d = deg (n-1)^2 (deg = 3..6) and the pass divides by it with its correctly
+ * rounded reciprocal and one FMA correction. Tries all 2^32 float bit patterns as the
+ * numerator for each of the four d of level n (skipping inf and NaN) and stores in
+ * *mismatches how many quotients differ in any bit from the IEEE quotient (0 means the fast
+ * division is exact for every finite numerator). Takes about a second per level. No
+ * reference counterpart. 0 or a negative status. */
+int mpr_fast_division_check(int32_t n, int64_t *mismatches);
 
 /* Allocation-free variant of mpr_iso_value_f64. No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under HIP graph capture, serving loops

@@ -30,6 +30,9 @@ struct StepPtrs {
   // elsewhere): the top level's last post-smoothing sweep adds the correction into it
   double* x64;
   const int* x_initial;  // with x64: the instance's initial-guess flag (TopPtrs)
+  // the level's second x buffer (LevelPtrs::xalt): the fused smoothing reads x and writes
+  // xalt (null where the level has none)
+  T* xalt;
 };
 
 // First sweep of a V-cycle level from a known-zero x: 0 x as stored, 1 zero for instances
@@ -103,6 +106,24 @@ void launch_b_coarse(const CoarseEntry<T>* tab, uint32_t count, int top_class, d
                      int post, hipStream_t s);
 // uploads the coarse class table; call outside stream capture before the first launch
 void init_coarse_kernels();
+// Fused smoothing (march.hip): `sweeps` red-black sweeps (2, 4 or 6) of every entry in one
+// pass, colours alternating from first_colour, reading x and writing xalt (or, with correct,
+// adding the result of both colours into x64 for entries that have one; first: x64 holds
+// kInitialX when the entry's x_initial flag is set). zero_start as launch_b_relax's: the
+// colour not swept first reads +0.0. Returns false (nothing launched) where the level's shape
+// or point_weight < 0 rules it out. Every entry needs an xalt buffer.
+bool launch_b_march(const StepPtrs<float>* tab, uint32_t count, const Geom& g, double pw,
+                    int sweeps, int first_colour, int zero_start, bool correct, bool first,
+                    hipStream_t s);
+bool march_supported(const Geom& g, uint32_t count, int sweeps);
+// number of finite float numerators a for which the fused smoothing's division by a
+// diagonal deg (n-1)^2 (deg = 3..6, a node without screening weight) differs from the IEEE
+// quotient; every one of the 2^32 bit patterns is tried for each diagonal
+int64_t fast_division_mismatches(int n, hipStream_t s);
+// red-black iterations of each smoothing phase of the V-cycle's FP32 levels fused into one
+// pass: 0 (none), 1 or 2 (process-wide; tables and graphs built later follow it)
+int march_iterations();
+void set_march_iterations(int iterations);
 // largest coarse class whose LDS-resident hierarchy fits one workgroup
 template <class T> int coarse_b_top_class();
 
@@ -174,6 +195,10 @@ class BatchedVcycle {
     launch_b_prolong<T>(step_[smax], step_count_[smax], geom_[smax - 1], geom_[smax],
                         colour1_only, s);
   }
+  // the fused smoothing of the finest class as the V-cycle's post-smoothing launches it
+  // (march_iterations() red-black iterations in one pass, x -> xalt); false where the class
+  // does not march
+  bool march_finest(double pw, hipStream_t s) const;
   int smax = 0, sc = 0;
 
  private:
@@ -182,6 +207,11 @@ class BatchedVcycle {
   std::vector<uint32_t> step_count_;
   std::vector<StepPtrs<T>*> top_;           // by class: instances whose finest class it is
   std::vector<uint32_t> top_count_;
+  // by class: step_ with x and xalt exchanged (the level after a fused smoothing phase), and
+  // the red-black iterations its smoothing phases fuse (0: none; FP32, every entry with an
+  // xalt buffer)
+  std::vector<StepPtrs<T>*> swap_;
+  std::vector<int> march_;
   CoarseEntry<T>* coarse_ = nullptr;        // all instances
   CoarseEntry<T>* coarse_zero_ = nullptr;   // same, every top zeroed
   uint32_t coarse_count_ = 0;

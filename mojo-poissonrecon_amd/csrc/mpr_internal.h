@@ -15,6 +15,9 @@ struct LevelPtrs {
   T* screen;
   T* res;
   Geom g;
+  // second x buffer of an FP32 level that the fused smoothing (march.hip) writes out of
+  // place, or null
+  T* xalt = nullptr;
 };
 
 // ---- grid kernels (grid_kernels.hip) ----------------------------------------------------

@@ -500,6 +500,8 @@ void Job::plan(DevArena& a) {
       f.rhs = a.alloc<float>(w);
       f.screen = a.alloc<float>(w);
       f.res = a.alloc<float>(w);
+      // the fused smoothing's second x buffer (march.hip) on the levels it can run on
+      f.xalt = march_supported(g, 1, 4) ? a.alloc<float>(w) : nullptr;
       L32.push_back(f);
     }
   }
@@ -1399,6 +1401,9 @@ static double pipeline_kernel_bytes(const mpr_job* job, int kernel, double pw) {
       return (3.0 + live) * 4.0 * M0 + mask;  // x, rhs, residual written; live screen
     case 13:  // x64 read, r32 written; live rhs64 and screen64
       return (8.0 + 4.0 + 16.0 * live) * M0 + mask;
+    case 14:  // fused smoothing, two iterations: the colour read first, the rhs (the dense
+              // defect), both colours written once; live screen words and the live bits
+      return (2.5 + live) * 4.0 * M0 + mask;
     case 10: {  // fine residual read (colour 0 only in the C1Z form), coarse rhs written
       const Geom& g1 = job->geoms[1];
       const double M1 = (double)g1.n * g1.n * g1.n;
@@ -1416,12 +1421,12 @@ static double pipeline_kernel_bytes(const mpr_job* job, int kernel, double pw) {
 
 int mpr_job_time_kernel(mpr_job* job, int32_t kernel, int32_t reps, double* ms_per_launch,
                         double* bytes_per_launch) {
-  if (!job || reps < 1 || kernel < 0 || kernel > 13) {
+  if (!job || reps < 1 || kernel < 0 || kernel > 14) {
     set_error("invalid kernel timing request");
     return -1;
   }
   if ((kernel == 1 || kernel == 5 || kernel == 6 || kernel == 7 || kernel == 9 ||
-       kernel == 10 || kernel == 11 || kernel == 13) &&
+       kernel == 10 || kernel == 11 || kernel == 13 || kernel == 14) &&
       job->L32.empty()) {
     set_error("FP32 kernels need a mixed-precision job");
     return -1;
@@ -1434,7 +1439,7 @@ int mpr_job_time_kernel(mpr_job* job, int32_t kernel, int32_t reps, double* ms_p
   const double pw = job->ran ? job->last.point_weight : 4.0;
   auto& a = job->L64[0];
   // the pipeline kernels' byte counts read the live-slot bits back: before the timed region
-  double bytes = (kernel == 7 || kernel == 8 || kernel == 9 || kernel == 13)
+  double bytes = (kernel == 7 || kernel == 8 || kernel == 9 || kernel == 13 || kernel == 14)
                      ? pipeline_kernel_bytes(job, kernel, pw)
                      : 0.0;
   int launches = reps;
@@ -1507,6 +1512,10 @@ int mpr_job_time_kernel(mpr_job* job, int32_t kernel, int32_t reps, double* ms_p
     case 13:
       for (int r = 0; r < reps; ++r) job->dtab.defect(pw, s);
       break;
+    case 14:
+      for (int r = 0; r < reps; ++r)
+        if (!job->vc32.march_finest(pw, s)) throw HipError{"the finest level does not march"};
+      break;
   }
   MPR_HIP(hipEventRecord(e1, s));
   MPR_HIP(hipEventSynchronize(e1));
@@ -1524,13 +1533,13 @@ int mpr_batch_time_kernel(mpr_batch* b, int32_t kernel, int32_t reps, double* ms
                           double* bytes_per_launch) {
   if (!b || reps < 1 ||
       (kernel != 7 && kernel != 8 && kernel != 9 && kernel != 10 && kernel != 11 &&
-       kernel != 13)) {
-    set_error("batch kernel timing: kernel 7, 8, 9, 10, 11 or 13 and reps >= 1");
+       kernel != 13 && kernel != 14)) {
+    set_error("batch kernel timing: kernel 7, 8, 9, 10, 11, 13 or 14 and reps >= 1");
     return -1;
   }
   const bool mixed = b->precision == MPR_PRECISION_MIXED;
   if ((kernel == 8) == mixed) {
-    set_error("batch kernel timing: kernel 8 needs an FP64 batch, 7, 9, 10, 11 and 13 a mixed one");
+    set_error("batch kernel timing: kernel 8 needs an FP64 batch, 7, 9, 10, 11, 13 and 14 a mixed one");
     return -1;
   }
   for (Job* j : b->jobs)
@@ -1573,6 +1582,10 @@ int mpr_batch_time_kernel(mpr_batch* b, int32_t kernel, int32_t reps, double* ms
       break;
     case 13:
       for (int r = 0; r < reps; ++r) b->dtab.defect(pw, s);
+      break;
+    case 14:
+      for (int r = 0; r < reps; ++r)
+        if (!b->vc32.march_finest(pw, s)) throw HipError{"the finest class does not march"};
       break;
   }
   MPR_HIP(hipEventRecord(e1, s));

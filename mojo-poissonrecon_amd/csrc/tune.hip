@@ -206,6 +206,15 @@ void set_tuned_planes(int kernel, bool fp64, int size_class, int planes) {
   g_planes[kernel][fp64 ? 1 : 0][size_class].store(planes, std::memory_order_relaxed);
 }
 
+namespace {
+std::atomic<int> g_march{2};
+}  // namespace
+
+int march_iterations() { return g_march.load(std::memory_order_relaxed); }
+void set_march_iterations(int iterations) {
+  g_march.store(iterations, std::memory_order_relaxed);
+}
+
 }  // namespace mpr
 
 using namespace mpr;
@@ -285,6 +294,36 @@ int mpr_tune_get_planes(int32_t kernel, int32_t fp64, int32_t size_class) {
     return -1;
   }
   return tuned_planes(kernel, fp64 != 0, size_class);
+}
+
+int mpr_tune_get_march(void) { return march_iterations(); }
+
+int mpr_fast_division_check(int32_t n, int64_t* mismatches) {
+  if (!mismatches || n < 3 || n > 1025) {
+    set_error("mpr_fast_division_check: n in [3, 1025] and a result pointer");
+    return -1;
+  }
+  if (ensure_device()) return -1;
+  hipStream_t s = nullptr;
+  try {
+    MPR_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *mismatches = fast_division_mismatches(n, s);
+    (void)hipStreamDestroy(s);
+  } catch (const HipError& e) {
+    if (s) (void)hipStreamDestroy(s);
+    set_error(e.msg);
+    return -2;
+  }
+  return 0;
+}
+
+int mpr_tune_set_march(int32_t iterations) {
+  if (iterations < 0 || iterations > 2) {
+    set_error("mpr_tune_set_march: iterations must be 0, 1 or 2");
+    return -1;
+  }
+  set_march_iterations(iterations);
+  return 0;
 }
 
 int mpr_tune_set_planes(int32_t kernel, int32_t fp64, int32_t size_class, int32_t planes) {

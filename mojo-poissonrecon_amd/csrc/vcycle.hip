@@ -34,7 +34,7 @@ size_t table_words(const std::vector<InstanceLevels<T>>& inst) {
 template <class T>
 size_t BatchedVcycle<T>::table_bytes(const std::vector<InstanceLevels<T>>& inst) {
   const size_t n = inst.size();
-  return sizeof(StepPtrs<T>) * 2 * table_words(inst) + 2 * sizeof(CoarseEntry<T>) * n +
+  return sizeof(StepPtrs<T>) * 3 * table_words(inst) + 2 * sizeof(CoarseEntry<T>) * n +
          64 * 64;
 }
 
@@ -51,8 +51,11 @@ void BatchedVcycle<T>::build(const std::vector<InstanceLevels<T>>& inst, void* t
   step_count_.assign(smax + 1, 0);
   top_.assign(smax + 1, nullptr);
   top_count_.assign(smax + 1, 0);
+  swap_.assign(smax + 1, nullptr);
+  march_.assign(smax + 1, 0);
   for (int s = sc + 1; s <= smax; ++s) {
-    std::vector<StepPtrs<T>> steps, tops;
+    std::vector<StepPtrs<T>> steps, tops, swaps;
+    bool alt = true;
     for (const auto& i : inst) {
       if (i.depth < s) continue;
       const int k = i.depth - s;
@@ -61,14 +64,24 @@ void BatchedVcycle<T>::build(const std::vector<InstanceLevels<T>>& inst, void* t
       const bool top = i.depth == s;
       StepPtrs<T> e{a.x, a.rhs, a.screen, a.res, b.x, b.rhs, top ? 1 : 0,
                     top && i.live_rhs ? 1 : 0, top ? i.live : nullptr, top ? i.x64 : nullptr,
-                    top ? i.x_initial : nullptr};
+                    top ? i.x_initial : nullptr, a.xalt};
       steps.push_back(e);
       if (i.depth == s) tops.push_back(e);
+      alt = alt && a.xalt != nullptr;
+      std::swap(e.x, e.xalt);
+      swaps.push_back(e);
     }
     step_[s] = w.put(steps);
     step_count_[s] = (uint32_t)steps.size();
     top_[s] = w.put(tops);
     top_count_[s] = (uint32_t)tops.size();
+    if constexpr (sizeof(T) == 4) {
+      const int it = march_iterations();
+      if (alt && it > 0 && march_supported(geom_[s], step_count_[s], 2 * it)) {
+        swap_[s] = w.put(swaps);
+        march_[s] = it;
+      }
+    }
   }
   std::vector<CoarseEntry<T>> ce, cz;
   for (const auto& i : inst) {
@@ -108,18 +121,35 @@ void BatchedVcycle<T>::run(double pw, int pre, int post, bool zero_tops, hipStre
   const int zero_start =
       fill ? kZeroNone
            : (zero_tops ? (defect_e0 ? kZeroAllDefectTop : kZeroAll) : kZeroBelowTop);
+  // classes whose smoothing phases march: march_[c] red-black iterations of the pre-
+  // smoothing in one pass from x into xalt, then the level lives in xalt (swap_) until the
+  // post-smoothing's pass writes it back; the other iterations are single sweeps in place
+  std::vector<char> marched(smax + 1, 0);
   for (int c = smax; c > sc; --c) {
     if (!step_count_[c]) continue;
     // every instance of the class at its top: the defect pass wrote the first sweep
     const bool skip = zero_start == kZeroAllDefectTop && top_count_[c] == step_count_[c];
-    launch_b_relax<T>(step_[c], step_count_[c], geom_[c], pw, pre, s, zero_start, false, false,
-                      skip);
+    if constexpr (sizeof(T) == 4) {
+      const int it = march_[c];
+      if (it && !fill && post >= it && pre >= (skip ? it + 1 : it) &&
+          (skip || zero_start != kZeroAllDefectTop) &&
+          launch_b_march(step_[c], step_count_[c], geom_[c], pw, 2 * it, skip ? 1 : 0,
+                         skip ? (int)kZeroNone : zero_start, false, false, s)) {
+        marched[c] = 1;
+        // the rest: colour 1 first when the defect pass wrote the first colour-0 sweep
+        launch_b_relax<T>(swap_[c], step_count_[c], geom_[c], pw, pre - it, s, kZeroNone, false,
+                          false, skip);
+      }
+    }
+    if (!marched[c])
+      launch_b_relax<T>(step_[c], step_count_[c], geom_[c], pw, pre, s, zero_start, false, false,
+                        skip);
+    const StepPtrs<T>* tab = marched[c] ? swap_[c] : step_[c];
     // FP32 after pre-smoothing: the colour-1 residual is zero up to rounding, and only
     // colour 0 is computed and restricted
     const bool c0 = colour0_residual(c, pw, pre);
-    launch_b_residual<T>(step_[c], step_count_[c], geom_[c], pw, s, c0);
-    launch_b_restrict<T>(step_[c], step_count_[c], geom_[c], geom_[c - 1], fill && c - 1 > sc,
-                         s, c0);
+    launch_b_residual<T>(tab, step_count_[c], geom_[c], pw, s, c0);
+    launch_b_restrict<T>(tab, step_count_[c], geom_[c], geom_[c - 1], fill && c - 1 > sc, s, c0);
   }
   launch_b_coarse<T>(zero_tops ? coarse_zero_ : coarse_, coarse_count_, coarse_class_, pw, pre,
                      post, s);
@@ -128,11 +158,32 @@ void BatchedVcycle<T>::run(double pw, int pre, int post, bool zero_tops, hipStre
     // The post-smoothing's first colour-0 sweep overwrites every colour-0 node without
     // reading it (its diagonal is >= 3 h^-2 > 0 for point_weight >= 0), so only colour 1
     // needs the correction.
-    launch_b_prolong<T>(step_[c], step_count_[c], geom_[c - 1], geom_[c], post > 0 && pw >= 0.0,
-                        s);
-    launch_b_relax<T>(step_[c], step_count_[c], geom_[c], pw, post, s, kZeroNone,
+    const StepPtrs<T>* tab = marched[c] ? swap_[c] : step_[c];
+    launch_b_prolong<T>(tab, step_count_[c], geom_[c - 1], geom_[c], post > 0 && pw >= 0.0, s);
+    int rest = post;
+    if constexpr (sizeof(T) == 4) {
+      if (marched[c]) {
+        // back into x: the colour-0 sweep reads only the prolonged colour 1. With no sweep
+        // after it, the pass itself adds the correction into x64.
+        rest = post - march_[c];
+        if (!launch_b_march(swap_[c], step_count_[c], geom_[c], pw, 2 * march_[c], 0, kZeroNone,
+                            fuse_correct && rest == 0, first, s))
+          throw HipError{"fused smoothing: post-smoothing launch refused"};
+      }
+    }
+    launch_b_relax<T>(step_[c], step_count_[c], geom_[c], pw, rest, s, kZeroNone,
                       fuse_correct && post > 0, first);
   }
+}
+
+template <class T>
+bool BatchedVcycle<T>::march_finest(double pw, hipStream_t s) const {
+  if constexpr (sizeof(T) == 4) {
+    if (march_[smax])
+      return launch_b_march(step_[smax], step_count_[smax], geom_[smax], pw, 2 * march_[smax], 0,
+                            kZeroNone, false, false, s);
+  }
+  return false;
 }
 
 void DefectTables::build(const std::vector<int>& depths, const std::vector<TopPtrs>& tops,

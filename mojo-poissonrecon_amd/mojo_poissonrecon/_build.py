@@ -23,7 +23,7 @@ ARCH = os.environ.get("MPR_OFFLOAD_ARCH", "gfx950")
 SOURCES = ["grid_kernels.hip", "batch_kernels.hip", "vcycle.hip", "point_kernels.hip",
            "extract_kernels.hip", "solver.hip",
            "capi_compat.hip", "tune.hip", "noalloc.hip",
-           "metric_kernels.hip", "mesh_kernels.hip"]
+           "metric_kernels.hip", "mesh_kernels.hip", "march.hip"]
 
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

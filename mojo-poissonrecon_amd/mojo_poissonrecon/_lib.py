@@ -112,6 +112,9 @@ _PIPELINE = {
     "mpr_tune_set_groups": ([S, S, S, S], ctypes.c_int),
     "mpr_tune_get_planes": ([S, S, S], ctypes.c_int),
     "mpr_tune_set_planes": ([S, S, S, S], ctypes.c_int),
+    "mpr_tune_get_march": ([], ctypes.c_int),
+    "mpr_tune_set_march": ([S], ctypes.c_int),
+    "mpr_fast_division_check": ([S, _IP], ctypes.c_int),
     "mpr_batch_create": ([P, S, _PP], ctypes.c_int),
     "mpr_batch_destroy": ([P], ctypes.c_int),
     "mpr_batch_run": ([P, ctypes.POINTER(mpr_params)], ctypes.c_int),

@@ -78,3 +78,27 @@ def tuned_planes(kernel: str, precision: str, size_class: int) -> int:
 def set_tuned_planes(kernel: str, precision: str, size_class: int, planes: int) -> None:
     check(load_library().mpr_tune_set_planes(*_key(kernel, precision, size_class), int(planes)),
           "mpr_tune_set_planes")
+
+
+def march_iterations() -> int:
+    """Red-black iterations of each smoothing phase of the mixed-precision V-cycle's FP32
+    levels fused into one pass over the level: 0, 1 or 2 (the default). Results are
+    bit-identical for every setting."""
+    v = load_library().mpr_tune_get_march()
+    check(-1 if v < 0 else 0, "mpr_tune_get_march")
+    return v
+
+
+def set_march_iterations(iterations: int) -> None:
+    """Sets the fused iterations (0, 1 or 2) for jobs and batches created afterwards."""
+    check(load_library().mpr_tune_set_march(int(iterations)), "mpr_tune_set_march")
+
+
+def fast_division_mismatches(n: int) -> int:
+    """Finite float numerators (all 2^32 bit patterns) for which the fused smoothing's
+    division by a node's unscreened diagonal deg (n-1)^2, deg = 3..6, differs from the IEEE
+    quotient (0: exact everywhere)."""
+    out = ctypes.c_int64()
+    check(load_library().mpr_fast_division_check(int(n), ctypes.byref(out)),
+          "mpr_fast_division_check")
+    return out.value
