@@ -43,6 +43,7 @@ constexpr int march_threads() {
   return 1024;
 }
 constexpr size_t kMarchLds = 160 * 1024;
+constexpr int kMarchMinN = 257;
 
 __device__ __forceinline__ F4 ld4(gptr<const float> p) {
   return *(const __attribute__((address_space(1))) F4*)p;
@@ -493,7 +494,11 @@ MarchPlan march_plan(const Geom& g, uint32_t count) {
   if (!group_layout(g) || count == 0) return pl;
   constexpr int W = ring_planes<S>();
   const int n = g.n, lr = g.H / 4;
-  if (lr != 5 && lr != 9 && lr != 17 && lr != 33) return pl;  // the kernel's row widths
+  // the kernel's row widths, and levels of 257^3: on smaller ones single sweeps are
+  // latency-bound and the march's pipeline fill and halos cost as much or more (measured,
+  // 8 instances per launch: 129^3 98 us per fused pass against 4 x 24 us, 65^3 42.5 us
+  // against 4 x 7 us, 33^3 29.5 us against 4 x 5 us)
+  if (n < kMarchMinN || (lr != 5 && lr != 9 && lr != 17 && lr != 33)) return pl;
   const int rows_lds = (int)(kMarchLds / (2ull * W * (size_t)(lr + 1) * 16ull)) - 2;
   const int nr_max = std::min({march_threads<S>() / lr, rows_lds, n});
   if (nr_max < 2 * S + 1 && nr_max < n) return pl;

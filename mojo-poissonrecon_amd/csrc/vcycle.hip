@@ -131,10 +131,13 @@ void BatchedVcycle<T>::run(double pw, int pre, int post, bool zero_tops, hipStre
     const bool skip = zero_start == kZeroAllDefectTop && top_count_[c] == step_count_[c];
     if constexpr (sizeof(T) == 4) {
       const int it = march_[c];
+      // with the defect pass writing the top levels' first sweep, a class with no top level
+      // starts from zero like any other (kZeroAll); one that mixes both does not march
+      const int zs = zero_start == kZeroAllDefectTop && top_count_[c] == 0 ? kZeroAll : zero_start;
       if (it && !fill && post >= it && pre >= (skip ? it + 1 : it) &&
-          (skip || zero_start != kZeroAllDefectTop) &&
+          (skip || zs != kZeroAllDefectTop) &&
           launch_b_march(step_[c], step_count_[c], geom_[c], pw, 2 * it, skip ? 1 : 0,
-                         skip ? (int)kZeroNone : zero_start, false, false, s)) {
+                         skip ? (int)kZeroNone : zs, false, false, s)) {
         marched[c] = 1;
         // the rest: colour 1 first when the defect pass wrote the first colour-0 sweep
         launch_b_relax<T>(swap_[c], step_count_[c], geom_[c], pw, pre - it, s, kZeroNone, false,

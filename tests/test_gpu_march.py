@@ -4,13 +4,14 @@ level in one z-marching pass must give the single sweeps' results bit for bit.
 The fused pass repeats relax_node's arithmetic per node (capi.mojo:309-410) in the same
 order, and every value it reads was produced by the same sweep sequence, so the mixed
 solve's fields, iso value, residual norm and mesh are identical for every setting of
-mpr_tune_set_march (0: single sweeps, 1 or 2 iterations fused): one request at depths 5-8
-(levels of 33^3 .. 257^3 march), the parameter corners the schedule has its own code for (no
-screening, one cycle, six cycles, heavy screening) and an uneven batch whose size classes
-mix finest and coarser levels, and the smoothing counts that reach every branch of the
-schedule. Its division by an unscreened diagonal (a correctly rounded reciprocal and two FMA
-corrections) must equal the IEEE quotient for every finite float numerator. Tolerance: none
-(array_equal, bit-pattern equality).
+mpr_tune_set_march (0: single sweeps, 1 or 2 iterations fused; levels of 257^3 march): one
+request at depths 5-9 (depth 8's finest level marches from the defect pass's first sweep,
+depth 9's second level from zero; depths 5-7 have no such level and must be untouched), the
+parameter corners the schedule has its own code for (no screening, one cycle, six cycles,
+heavy screening), an uneven batch of depths 5-8, and the smoothing counts that reach every
+branch of the schedule. Its division by an unscreened diagonal (a correctly rounded
+reciprocal and two FMA corrections) must equal the IEEE quotient for every finite float
+numerator. Tolerance: none (array_equal, bit-pattern equality).
 """
 
 from __future__ import annotations
@@ -63,7 +64,8 @@ def assert_identical(a, b):
     assert np.array_equal(ma.density, mb.density)
 
 
-@pytest.mark.parametrize("depth,count", [(5, 3000), (6, 12000), (7, 40000), (8, 200000)])
+@pytest.mark.parametrize("depth,count",
+                         [(5, 3000), (6, 12000), (7, 40000), (8, 200000), (9, 300000)])
 def test_fused_smoothing_bit_identical(march_default, depth, count):
     if depth == 8:
         p, n = shapes.torus_workload(count, seed=depth)
@@ -79,8 +81,8 @@ def test_fused_smoothing_bit_identical(march_default, depth, count):
 @pytest.mark.parametrize("kw", [dict(point_weight=0.0), dict(cycles=1), dict(cycles=6),
                                 dict(point_weight=40.0, samples_per_node=3.0)])
 def test_fused_smoothing_parameters(march_default, kw):
-    p, n = shapes.superquadric(20000, seed=3)
-    kw = dict(depth=7, **kw)
+    p, n = shapes.superquadric(60000, seed=3)
+    kw = dict(depth=8, **kw)
     base = run(p, n, 0, **kw)
     for it in SETTINGS[1:]:
         assert_identical(base, run(p, n, it, **kw))
@@ -107,12 +109,12 @@ def test_fused_smoothing_smoothing_counts(march_default, pre, post):
     # every branch of the schedule: the pass's own correction output (post = fused
     # iterations), a pre-smoothing too short to fuse after the defect pass's first sweep,
     # one-iteration phases, longer ones
-    p, n = shapes.torus(30000, seed=5)
+    p, n = shapes.torus(60000, seed=5)
     p, n = shapes.perturb(p, n, seed=5, sigma=0.002, degrees=5.0)
     out = []
     for it in SETTINGS:
         tune.set_march_iterations(it)
-        job = Job(7, len(p), _lib.PRECISION_MIXED)
+        job = Job(8, len(p), _lib.PRECISION_MIXED)
         try:
             job.set_points(p, n)
             job.run(Job.params(_lib.STAGE_NORMALIZE | _lib.STAGE_SOLVE | _lib.STAGE_GRIDS, 4.0,
