@@ -20,7 +20,8 @@ Also reported:
                transfers included: meshes come back through pinned host buffers), and the
                batched form with its host uploads and downloads
   accuracy     mixed vs FP64 mesh distance (exact point-to-triangle, GPU) over the diagonal
-  roofline     the dominant kernel (the FP32 colour sweep of the batch's finest level)
+  roofline     the dominant kernel: the fused smoothing of the batch's finest level (two
+               red-black iterations of all instances in one z-marching pass)
   batch64      64 clouds of uneven depth and size, mixed and FP64, per-instance error vs FP64
   depth9       2M-point superquadric at depth 9, mixed and FP64
   serve        a QPS sweep of the served stream, then --serve-requests requests at a rate below
@@ -45,12 +46,12 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the roofline kernel (kb_relax_group<float>, one colour of the
-# 257^3 finest level of all instances of the batch), keyed by (depth, batch):
-# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from rocprofv3 counter runs (the sweep timing's plain
+# HBM bytes per launch of the roofline kernel (kb_march<4, 33>, two red-black iterations of
+# the 257^3 finest level of all instances of the batch), keyed by (depth, batch):
+# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from rocprofv3 counter runs (the timing repetitions'
 # launches), the factors calibrated on a plain elementwise copy
-# (profiles/kb_relax_group_f32_depth8_pmc.txt).
-ROOFLINE_TRAFFIC_BYTES = {(8, 8): 894_785_904}
+# (profiles/kb_march_f32_depth8_pmc.txt).
+ROOFLINE_TRAFFIC_BYTES = {(8, 8): 2_233_881_515}
 
 
 def log(*args):
@@ -396,7 +397,15 @@ def main() -> int:
     ap.add_argument("--kernels-only", action="store_true",
                     help="one reconstruction per precision, then only the kernel timings "
                          "(a short program for hardware-counter runs)")
+    ap.add_argument("--march", type=int, choices=(0, 1, 2), default=None,
+                    help="red-black iterations per fused smoothing pass of the 257^3 and larger "
+                         "FP32 levels (0: single colour sweeps; default: the library's, 2); "
+                         "0 only with --only, since the roofline kernel is the fused pass")
     args = ap.parse_args()
+    if args.march == 0 and not args.only:
+        print("--march 0 leaves no fused pass to report as the roofline kernel; use it with "
+              "--only", file=sys.stderr)
+        return 2
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(independent reconstructions shard one per GPU with no exchange)", file=sys.stderr)
@@ -408,6 +417,9 @@ def main() -> int:
 
     if not os.path.exists(_build.LIB):
         mpr.build()
+    if args.march is not None:
+        from mojo_poissonrecon import tune as march_tune
+        march_tune.set_march_iterations(args.march)
     if not args.no_autotune:
         from mojo_poissonrecon import tune
         t0 = time.perf_counter()
@@ -424,8 +436,10 @@ def main() -> int:
 
     points, normals = shapes.torus_workload(args.points, seed=0)
     bclouds = [shapes.torus_workload(args.points, seed=s) for s in range(args.batch)]
-    sweep_kernels = [("rb_sweep_f32", 7), ("residual_f32", 9), ("restrict_f32", 10),
-                     ("prolong_f32_c1", 11), ("defect_f64_f32", 13)]
+    sweep_kernels = [("march_f32", 14), ("rb_sweep_f32", 7), ("residual_f32", 9),
+                     ("restrict_f32", 10), ("prolong_f32_c1", 11), ("defect_f64_f32", 13)]
+    if args.march == 0:
+        sweep_kernels = sweep_kernels[1:]
     if args.only == "mixed_batch":
         b, jobs, _ = make_batch(bclouds, [args.depth] * args.batch, "mixed")
         per, blat = run_batch_launches(b, args.steps, args.warmup)
@@ -503,13 +517,17 @@ def main() -> int:
     e2e = end_to_end(points, normals, bclouds, args.depth, max(5, args.steps // 2), 2)
     log("end_to_end:", e2e)
 
-    # the dominant kernel: one colour of the FP32 sweep over all instances' finest levels.
-    # Algorithmic bytes as counted for a red-black iteration (4 words per node: x read, rhs
-    # and screen read, x written), half of them per colour sweep: 2 n^3 words per instance.
-    dom = bkernels["rb_sweep_f32"]
+    # the dominant kernel: the fused smoothing of all instances' finest levels, two red-black
+    # iterations per launch. Algorithmic bytes by the rule for a red-black iteration (4 words
+    # per node: x read, rhs and screen read, x written, whether or not iterations are fused):
+    # 8 n^3 words per instance. The pass itself loads each plane's input colour, rhs and live
+    # screening words once and writes both colours once (loaded_bytes), so its achieved rate
+    # by the rule can exceed what a single sweep reaches; traffic is the counters' bytes.
+    dom = bkernels["march_f32"]
     n = (1 << args.depth) + 1
-    alg_bytes = float(args.batch) * 2.0 * n ** 3 * 4.0
+    alg_bytes = float(args.batch) * 8.0 * n ** 3 * 4.0
     achieved = alg_bytes / (dom["ms"] * 1e-3) / 1e9
+    traffic = ROOFLINE_TRAFFIC_BYTES.get((args.depth, args.batch))
 
     extras = {}
     if not args.no_extras:
@@ -555,21 +573,33 @@ def main() -> int:
         "accuracy_mixed_vs_fp64_over_diag": accuracy,
         "mesh": {"vertices": modes["mixed"]["nv"], "fp64_vertices": fp64["nv"]},
         "roofline": {
-            "kernel": f"kb_relax_group<float>: one colour of the finest level of all {args.batch} "
-                      "instances of the batch (one launch)",
+            "kernel": f"kb_march<4, 33>: two red-black iterations (four colour sweeps) of the "
+                      f"finest level of all {args.batch} instances of the batch in one "
+                      "z-marching pass (one launch)",
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": ROOFLINE_TRAFFIC_BYTES.get((args.depth, args.batch)),
+            "traffic": traffic,
             "algorithmic_bytes": round(alg_bytes),
-            "algorithmic_bytes_rule": "2 n^3 words of 4 B per instance per colour sweep (half "
-                                      "of a red-black iteration's read x, rhs, screen, write x)",
+            "algorithmic_bytes_rule": "8 n^3 words of 4 B per instance: two red-black "
+                                      "iterations of 4 words per node (read x, rhs, screen, "
+                                      "write x), fused or not",
             "us_per_launch": round(dom["ms"] * 1e3, 2),
             "loaded_bytes": round(dom["bytes"]),
             "loaded_frac": round(dom["gbs"] / HBM_PEAK_GBS, 4),
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)",
+            "traffic_GBps": (round(traffic / (dom["ms"] * 1e-3) / 1e9, 1)
+                             if traffic is not None else None),
+            "limiter": "VALU: the pass recomputes halo rows and moves about half of the "
+                       "algorithmic bytes, so frac above 1 reads as the fusion's saving and "
+                       "traffic_GBps as the HBM rate it draws",
+            "single_sweep": {"kernel": "kb_relax_group<float>, one colour (the unfused sweep)",
+                             "us_per_launch": round(bkernels["rb_sweep_f32"]["ms"] * 1e3, 2),
+                             "algorithmic_bytes": round(float(args.batch) * 2.0 * n ** 3 * 4.0),
+                             "achieved": round(float(args.batch) * 2.0 * n ** 3 * 4.0 /
+                                               (bkernels["rb_sweep_f32"]["ms"] * 1e-3) / 1e9, 1)},
         },
         "kernels": {k: {"us": round(v["ms"] * 1e3, 2), "GBps": round(v["gbs"], 1)}
                     for k, v in kernels.items()},
