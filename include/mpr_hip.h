@@ -290,8 +290,9 @@ int mpr_tune_set_planes(int32_t kernel, int32_t fp64, int32_t size_class, int32_
 int mpr_tune_get_march(void);
 int mpr_tune_set_march(int32_t iterations);
 /* Test hook of the fused smoothing's division: on a node without screening weight the
- * diagonal is d = deg (n-1)^2 (deg = 3..6) and the pass divides by it with its correctly
- * rounded reciprocal and one FMA correction. Tries all 2^32 float bit patterns as the
+ * diagonal is d = deg (n-1)^2 (deg = 3..6) and the pass divides by it with one
+ * multiplication and one FMA (the reciprocal split into two floats; numerators of magnitude
+ * below 2^-60 divide the IEEE way). Tries all 2^32 float bit patterns as the
  * numerator for each of the four d of level n (skipping inf and NaN) and stores in
  * *mismatches how many quotients differ in any bit from the IEEE quotient (0 means the fast
  * division is exact for every finite numerator). Takes about a second per level. No

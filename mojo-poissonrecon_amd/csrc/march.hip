@@ -70,26 +70,25 @@ struct MarchArgs {
   int32_t zero_start;          // ZeroStart: the input colour reads +0.0 (not loaded)
   int32_t correct, first;      // last phase of the mixed loop (x64 += e), first cycle
   float pw, h2;
-  // deg h^-2 for deg = 0..6 (exact: small integers times a power of two) and the correctly
-  // rounded reciprocals of deg = 3..6 (mpr_fast_division_check)
-  float dh[7], rd[7];
+  // the reciprocal of deg h^-2 for deg = 0..6 split into a float and the float nearest its
+  // remainder (recip_split; deg = 3..6 are used)
+  float zh[7], zl[7];
 };
 
-// Division by a diagonal known to be deg h^-2 (a node without screening weight): q0 = a r
-// with r = RN(1 / d), then two corrections with the exact remainder (Markstein):
-// q_{i+1} = fma(a - d q_i, r, q_i), and the sign of a zero quotient taken from a (d > 0). The
-// first correction brings q within one ulp of a / d, and from a faithful q the second rounds
-// correctly as long as the quotient is a normal number, so the sweep divides nonzero
-// |a| < kTinyNumerator the IEEE way: fast_quotient. It equals the IEEE quotient a / d bit for
-// bit for every finite float a and the level's four diagonals (checked exhaustively on the
-// device, mpr_fast_division_check; without the guard the corrections round wrongly only at
-// |a| < 2^-109 for the diagonals of levels up to 257^3).
-constexpr float kTinyNumerator = 0x1p-90f;
+// Division by a diagonal known in advance, d = deg h^-2 (a node without screening weight),
+// with one multiplication and one FMA: q = RN(a zh + RN(a zl)), where zh = RN(1 / d) and
+// zl = RN(1 / d - zh) (Brisebarre, Muller and Raina, "Accelerating correctly rounded
+// floating-point division when the divisor is known in advance", IEEE Trans. Computers
+// 53(8), 2004). Whether it rounds correctly for every numerator depends on the divisor's
+// significand only, here that of 3, 5 (4: zl = 0, an exact scaling) or 6: checked
+// exhaustively on the device for every finite float numerator and the four diagonals of a
+// level (mpr_fast_division_check). A zero quotient takes the sign of a (with zl < 0 the FMA
+// would add a zero of the other sign). Once |a zl| is subnormal the product loses bits, so
+// the sweep divides nonzero |a| < kTinyNumerator the IEEE way: fast_quotient.
+constexpr float kTinyNumerator = 0x1p-60f;
 
-__device__ __forceinline__ float div_known(float a, float d, float r) {
-  const float q0 = a * r;
-  const float q1 = __builtin_fmaf(__builtin_fmaf(-d, q0, a), r, q0);
-  const float q = __builtin_fmaf(__builtin_fmaf(-d, q1, a), r, q1);
+__device__ __forceinline__ float div_known(float a, float zh, float zl) {
+  const float q = __builtin_fmaf(a, zh, a * zl);
   return __builtin_bit_cast(float, (__builtin_bit_cast(uint32_t, q) & 0x7fffffffu) |
                                        (__builtin_bit_cast(uint32_t, a) & 0x80000000u));
 }
@@ -99,9 +98,22 @@ __device__ __forceinline__ bool tiny_numerator(float a) {
   return __builtin_fabsf(a) < kTinyNumerator && a != 0.0f;
 }
 
-// the sweep's division of a by an unscreened diagonal d, as a function of one value
-__device__ __forceinline__ float fast_quotient(float a, float d, float r) {
-  return tiny_numerator(a) ? a / d : div_known(a, d, r);
+// the sweep's division of a by an unscreened diagonal d = 1 / (zh + zl), as a function of one
+// value
+__device__ __forceinline__ float fast_quotient(float a, float d, float zh, float zl) {
+  return tiny_numerator(a) ? a / d : div_known(a, zh, zl);
+}
+
+// zh = RN(1 / d) and zl = RN(1 / d - zh) for d > 0 (0 and 0 for d = 0). d zh is exact in
+// double (24 x 24 bits), so 1 - d zh is the exact remainder and zl its quotient by d, within
+// one rounding of the double quotient.
+void recip_split(float d, float& zh, float& zl) {
+  if (!(d > 0.0f)) {
+    zh = zl = 0.0f;
+    return;
+  }
+  zh = (float)(1.0 / (double)d);
+  zl = (float)((1.0 - (double)d * (double)zh) / (double)d);
 }
 
 // Per-lane state of the march. Sweep j of step P reads the rhs of plane P - j + 1 in its
@@ -138,7 +150,7 @@ struct March {
   int n, LR, r, k, y, z0, z1, yo0, yo1, Pb, Pe, cB, by;
   uint32_t lane_off;  // (y H + 4 k): the lane's chunk inside a plane of one colour
   bool act, zin, corr, init64, live_rhs, xedge, klast;
-  float dhb, rdb, dhm, rdm;  // diagonal and reciprocal without / with an x boundary
+  float zhb, zlb, zhm, zlm;  // the diagonal's reciprocal pair without / with an x boundary
 
   // real slots of the lane's chunk in colour c's row of plane q
   __device__ uint32_t vmask(int c, int q) const {
@@ -266,9 +278,9 @@ struct March {
     bool tiny = false;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const float d = i == 0 && xb ? dhm : dhb;
-      const float rr = i == 0 && xb ? rdm : rdb;
-      out[i] = div_known(num[i], d, rr);
+      const float zh = i == 0 && xb ? zhm : zhb;
+      const float zl = i == 0 && xb ? zlm : zlb;
+      out[i] = div_known(num[i], zh, zl);
       tiny = tiny || tiny_numerator(num[i]);
     }
     // screened nodes, z-boundary planes and tiny numerators (fast_quotient's IEEE branch)
@@ -422,10 +434,10 @@ __global__ __launch_bounds__(march_threads<S>()) void kb_march(const StepPtrs<fl
   m.klast = m.k == LR - 1;
   m.xedge = m.k == 0 || m.klast;
   m.by = (int)(m.y == 0) + (int)(m.y == n - 1);
-  m.dhb = args.dh[6 - m.by];
-  m.rdb = args.rd[6 - m.by];
-  m.dhm = args.dh[5 - m.by];
-  m.rdm = args.rd[5 - m.by];
+  m.zhb = args.zh[6 - m.by];
+  m.zlb = args.zl[6 - m.by];
+  m.zhm = args.zh[5 - m.by];
+  m.zlm = args.zl[5 - m.by];
   // pads: chunk 0 of every segment and the whole of rows 0 and NR + 1
   F4* ring = m.ring;
   const uint32_t segs = (uint32_t)(NR + 2) * 2u * (uint32_t)W;
@@ -565,10 +577,7 @@ bool launch_march_s(const StepPtrs<float>* tab, uint32_t count, const Geom& g, d
   a.first = first ? 1 : 0;
   a.pw = (float)pw;
   a.h2 = (float)((double)(g.n - 1) * (double)(g.n - 1));
-  for (int d = 0; d <= 6; ++d) {
-    a.dh[d] = (float)d * a.h2;
-    a.rd[d] = d ? 1.0f / a.dh[d] : 0.0f;
-  }
+  for (int d = 0; d <= 6; ++d) recip_split((float)d * a.h2, a.zh[d], a.zl[d]);
   switch (pl.a.lr) {
     case 5: launch_march_kernel<S, 5>(tab, count, pl, g, a, s); break;
     case 9: launch_march_kernel<S, 9>(tab, count, pl, g, a, s); break;
@@ -580,8 +589,8 @@ bool launch_march_s(const StepPtrs<float>* tab, uint32_t count, const Geom& g, d
 
 // counts the finite floats a (bit patterns start .. start + count - 1) whose fast_quotient
 // differs from the IEEE quotient a / d in any bit
-__global__ __launch_bounds__(256) void k_fast_div_check(float d, float r, uint64_t start,
-                                                         uint64_t count,
+__global__ __launch_bounds__(256) void k_fast_div_check(float d, float zh, float zl,
+                                                         uint64_t start, uint64_t count,
                                                          unsigned long long* bad) {
   unsigned long long local = 0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -590,7 +599,8 @@ __global__ __launch_bounds__(256) void k_fast_div_check(float d, float r, uint64
     if ((bits & 0x7f800000u) == 0x7f800000u) continue;  // inf and NaN
     const float a = __builtin_bit_cast(float, bits);
     const float want = a / d;
-    if (__builtin_bit_cast(uint32_t, fast_quotient(a, d, r)) != __builtin_bit_cast(uint32_t, want))
+    if (__builtin_bit_cast(uint32_t, fast_quotient(a, d, zh, zl)) !=
+        __builtin_bit_cast(uint32_t, want))
       ++local;
   }
   if (local) atomicAdd(bad, local);
@@ -607,8 +617,10 @@ int64_t fast_division_mismatches(int n, hipStream_t s) {
     const float h2 = (float)((double)(n - 1) * (double)(n - 1));
     for (int deg = 3; deg <= 6; ++deg) {
       const float d = (float)deg * h2;
-      hipLaunchKernelGGL(k_fast_div_check, dim3(8192), dim3(256), 0, s, d, 1.0f / d,
-                         (uint64_t)0, (uint64_t)1 << 32, bad);
+      float zh, zl;
+      recip_split(d, zh, zl);
+      hipLaunchKernelGGL(k_fast_div_check, dim3(8192), dim3(256), 0, s, d, zh, zl, (uint64_t)0,
+                         (uint64_t)1 << 32, bad);
       MPR_LAUNCH_CHECK();
     }
     MPR_HIP(hipMemcpyAsync(&total, bad, sizeof(total), hipMemcpyDeviceToHost, s));

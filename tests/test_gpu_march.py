@@ -9,9 +9,9 @@ request at depths 5-9 (depth 8's finest level marches from the defect pass's fir
 depth 9's second level from zero; depths 5-7 have no such level and must be untouched), the
 parameter corners the schedule has its own code for (no screening, one cycle, six cycles,
 heavy screening), an uneven batch of depths 5-8, and the smoothing counts that reach every
-branch of the schedule. Its division by an unscreened diagonal (a correctly rounded
-reciprocal and two FMA corrections) must equal the IEEE quotient for every finite float
-numerator. Tolerance: none (array_equal, bit-pattern equality).
+branch of the schedule. Its division by an unscreened diagonal (one multiplication and one
+FMA with the reciprocal split into two floats) must equal the IEEE quotient for every finite
+float numerator, on the levels of 65^3 to 1025^3. Tolerance: none (array_equal, bit-pattern equality).
 """
 
 from __future__ import annotations
@@ -128,7 +128,7 @@ def test_fused_smoothing_smoothing_counts(march_default, pre, post):
         assert np.array_equal(field, out[0][2])
 
 
-@pytest.mark.parametrize("n", [65, 257])
+@pytest.mark.parametrize("n", [65, 257, 513, 1025])
 def test_fast_division_exact_for_every_float(n):
     # 4 diagonals x 2^32 numerators each, on the device
     assert tune.fast_division_mismatches(n) == 0
