@@ -136,13 +136,15 @@ def cpu_baseline(points, normals, depth: int) -> tuple[dict, dict]:
 
 # ---- device modes -----------------------------------------------------------------------
 
-def make_batch(clouds, depths, precision, graph=True, host_mesh=False):
+def make_batch(clouds, depths, precision, graph=True, host_mesh=False, inner_cycles=0):
     """Jobs and a batch over the clouds. host_mesh: the extraction also copies each mesh to
-    the job's pinned host buffers (the end-to-end legs); without, meshes stay in HBM."""
+    the job's pinned host buffers (the end-to-end legs); without, meshes stay in HBM.
+    inner_cycles: FP32 V-cycles per FP64 defect (mixed; 0 the library's default, one)."""
     from mojo_poissonrecon.batch import Batch
     from mojo_poissonrecon.core import Job, prepare_reconstruct
 
-    reqs = [prepare_reconstruct(p, n, depth=d, precision=precision, confidence=True)
+    reqs = [prepare_reconstruct(p, n, depth=d, precision=precision, confidence=True,
+                                inner_cycles=inner_cycles)
             for (p, n), d in zip(clouds, depths)]
     jobs = [Job(r.depth, len(r.points), r.precision, host_mesh=host_mesh) for r in reqs]
     for j, r in zip(jobs, reqs):
@@ -514,6 +516,18 @@ def main() -> int:
     modes["mixed_batch"] = dict(value=args.batch / per_launch, p50=pct(blat, 50),
                                 p99=pct(blat, 99))
     log(f"mixed_batch x{args.batch}: {modes['mixed_batch']['value']:.2f} recon/s")
+    # the same batch with two FP32 V-cycles per FP64 defect (inner_cycles = 2): two FP64
+    # defects and corrections fewer, and the first cloud's mesh error against its FP64 mesh
+    b, bjobs, _ = make_batch(bclouds, [args.depth] * args.batch, "mixed", inner_cycles=2)
+    per2, blat2 = run_batch_launches(b, args.steps, args.warmup)
+    mesh2 = mesh_from_job(bjobs[0], b.wait()[0])
+    b.close()
+    for j in bjobs:
+        j.close()
+    inner2 = {"value": round(args.batch / per2, 3), "p50_ms": pct(blat2, 50),
+              "p99_ms": pct(blat2, 99),
+              "accuracy_vs_fp64_over_diag": mesh_error(mesh2, meshes["fp64"], diag_of(points))}
+    log(f"mixed_batch x{args.batch}, inner_cycles 2: {inner2['value']:.2f} recon/s")
     e2e = end_to_end(points, normals, bclouds, args.depth, max(5, args.steps // 2), 2)
     log("end_to_end:", e2e)
 
@@ -551,6 +565,8 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 V-cycles in f64 defect correction (f64 splat, residual, iso)",
+        "inner_cycles": "library default: one f32 V-cycle per f64 defect (4 outer steps); "
+                        "mixed_batch_inner_cycles_2 is the same batch with two",
         "data": "synthetic",
         "config": {
             "workload": f"torus {len(points)} points, depth {args.depth} ({n}^3), sigma 0.002 "
@@ -571,6 +587,7 @@ def main() -> int:
         "end_to_end": e2e,
         "speedup_vs_plain_fp64": round(mixed_b["value"] / fp64["value"], 3),
         "accuracy_mixed_vs_fp64_over_diag": accuracy,
+        "mixed_batch_inner_cycles_2": inner2,
         "mesh": {"vertices": modes["mixed"]["nv"], "fp64_vertices": fp64["nv"]},
         "roofline": {
             "kernel": f"kb_march<4, 33>: two red-black iterations (four colour sweeps) of the "

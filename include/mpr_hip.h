@@ -140,7 +140,16 @@ typedef struct mpr_params {
   int32_t pre_smooth;      /* default 3 */
   int32_t post_smooth;     /* default 3 */
   int32_t stages;          /* MPR_STAGE_* bits */
-  int32_t reserved;
+  int32_t inner_cycles;    /* MPR_PRECISION_MIXED: FP32 V-cycles per FP64 defect (>= 0; 0 and 1
+                              give one, the default). The cycles V-cycles run in outer steps
+                              of this many: one FP64 defect r = b - A x, that many V-cycles on
+                              A e = r (the first from e = 0, the rest from the current e), one
+                              x += e. One per defect makes the mixed iterate track the FP64
+                              iterate cycle for cycle; two save two FP64 defects and
+                              corrections of a 4-cycle solve at a larger distance to the FP64
+                              mesh (depth-8 batch of 8: 17.1 -> 15.4 ms; mesh Hausdorff 2e-11
+                              -> 2e-10 of the diagonal on the torus, worst of 112 uneven
+                              requests 2e-8 -> 5e-6). Ignored in FP64. */
 } mpr_params;
 
 typedef struct mpr_result {

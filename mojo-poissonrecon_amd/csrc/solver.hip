@@ -435,23 +435,30 @@ void Batch::enqueue(const mpr_params* p) {
 }
 
 // V-cycles of a set of jobs (core.py:274-285): FP64 as the reference, or FP32 V-cycles on
-// the correction equation inside an FP64 defect-correction loop.
+// the correction equation inside an FP64 defect-correction loop, inner_cycles FP32 V-cycles
+// per FP64 defect.
 void run_cycles(const BatchedVcycle<double>& v64, const BatchedVcycle<float>* v32,
                 const DefectTables* dt, const mpr_params& p, hipStream_t s) {
-  for (int c = 0; c < p.cycles; ++c) {
-    if (v32) {
-      // r = b - A x (FP64), stored as FP32; with pre-smoothing from e = 0 (point_weight >= 0)
-      // the same pass writes the top level's first colour-0 sweep, e = r / diagonal
-      const bool e0 = p.pre_smooth > 0 && p.point_weight >= 0.0;
-      dt->defect(p.point_weight, s, c == 0, e0);
-      // e = V32(r) from e = 0, and x += e: with post-smoothing the top level's last sweep
-      // adds e into x for instances deeper than the coarse classes, the rest use kb_correct
-      const bool fuse = p.post_smooth > 0;
-      v32->run(p.point_weight, p.pre_smooth, p.post_smooth, true, s, fuse, c == 0, e0);
-      dt->correct(s, fuse ? v32->unfused_class() : 1 << 30);
-    } else {
+  if (!v32) {
+    for (int c = 0; c < p.cycles; ++c)
       v64.run(p.point_weight, p.pre_smooth, p.post_smooth, false, s);
-    }
+    return;
+  }
+  const int inner = p.inner_cycles > 0 ? p.inner_cycles : 1;
+  for (int c = 0; c < p.cycles; c += inner) {
+    const int k = std::min(inner, p.cycles - c);
+    // r = b - A x (FP64), stored as FP32; with pre-smoothing from e = 0 (point_weight >= 0)
+    // the same pass writes the top level's first colour-0 sweep, e = r / diagonal
+    const bool e0 = p.pre_smooth > 0 && p.point_weight >= 0.0;
+    dt->defect(p.point_weight, s, c == 0, e0);
+    // e = V32(r) from e = 0 (k V-cycles, the later ones from the current e), and x += e: with
+    // post-smoothing the top level's last sweep adds e into x for instances deeper than the
+    // coarse classes, the rest use kb_correct
+    const bool fuse = p.post_smooth > 0;
+    for (int i = 0; i < k; ++i)
+      v32->run(p.point_weight, p.pre_smooth, p.post_smooth, i == 0, s, fuse && i == k - 1,
+               c == 0, e0 && i == 0);
+    dt->correct(s, fuse ? v32->unfused_class() : 1 << 30);
   }
 }
 
@@ -790,7 +797,7 @@ using namespace mpr;
 static bool valid_params(const mpr_params* p) {
   if (!p) return false;
   // core.py:213-220: finite, point_weight >= 0, samples_per_node > 0
-  return p->cycles >= 0 && p->pre_smooth >= 0 && p->post_smooth >= 0 &&
+  return p->cycles >= 0 && p->pre_smooth >= 0 && p->post_smooth >= 0 && p->inner_cycles >= 0 &&
          std::isfinite(p->samples_per_node) && p->samples_per_node > 0 &&
          std::isfinite(p->point_weight) && p->point_weight >= 0.0;
 }
@@ -1221,6 +1228,7 @@ static bool batch_params_ok(const mpr_batch* b, const mpr_params* p) {
     if (!valid_params(&p[i])) return false;
     if (p[i].cycles != p[0].cycles || p[i].pre_smooth != p[0].pre_smooth ||
         p[i].post_smooth != p[0].post_smooth || p[i].point_weight != p[0].point_weight ||
+        std::max(p[i].inner_cycles, 1) != std::max(p[0].inner_cycles, 1) ||
         (p[i].stages & MPR_STAGE_SOLVE) != (p[0].stages & MPR_STAGE_SOLVE))
       return false;
   }
@@ -1263,7 +1271,8 @@ int mpr_batch_destroy(mpr_batch* b) {
 
 int mpr_batch_run(mpr_batch* b, const mpr_params* params) {
   if (!b || !params || !batch_params_ok(b, params)) {
-    set_error("batch parameters must be valid and share cycles, smoothing and point weight");
+    set_error("batch parameters must be valid and share cycles, inner cycles, smoothing and "
+              "point weight");
     return -1;
   }
   MPR_GUARD_BEGIN
@@ -1281,7 +1290,8 @@ int mpr_batch_run(mpr_batch* b, const mpr_params* params) {
 
 int mpr_batch_capture(mpr_batch* b, const mpr_params* params) {
   if (!b || !params || !batch_params_ok(b, params)) {
-    set_error("batch parameters must be valid and share cycles, smoothing and point weight");
+    set_error("batch parameters must be valid and share cycles, inner cycles, smoothing and "
+              "point weight");
     return -1;
   }
   MPR_GUARD_BEGIN

@@ -37,7 +37,7 @@ class mpr_params(ctypes.Structure):
         ("pre_smooth", ctypes.c_int32),
         ("post_smooth", ctypes.c_int32),
         ("stages", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("inner_cycles", ctypes.c_int32),
     ]
 
 

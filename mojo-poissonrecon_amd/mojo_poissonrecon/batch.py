@@ -101,7 +101,8 @@ class Batch:
 
 def reconstruct_batch(clouds, *, depths, point_weight: float = 4.0, confidence=False,
                       cycles: int = 4, scale: float = 1.1, samples_per_node: float = 1.5,
-                      precision: str = "mixed", graph: bool = False) -> list[Mesh]:
+                      precision: str = "mixed", graph: bool = False,
+                      inner_cycles: int = 0) -> list[Mesh]:
     """Reconstruct each ``(points, normals)`` cloud at its depth with one batched device pass.
 
     ``depths`` and ``confidence`` may be scalars or per-cloud sequences. Every cloud gets the
@@ -116,7 +117,7 @@ def reconstruct_batch(clouds, *, depths, point_weight: float = 4.0, confidence=F
         raise ValueError("depths and confidence must match the number of clouds")
     reqs = [prepare_reconstruct(p, n, depth=d, point_weight=point_weight, confidence=c,
                                 cycles=cycles, scale=scale, samples_per_node=samples_per_node,
-                                precision=precision)
+                                precision=precision, inner_cycles=inner_cycles)
             for (p, n), d, c in zip(clouds, depths, conf)]
     live = [i for i, r in enumerate(reqs) if r is not None]
     meshes: list[Mesh] = [Mesh(np.empty((0, 3)), np.empty((0, 3), np.int64), np.empty(0))

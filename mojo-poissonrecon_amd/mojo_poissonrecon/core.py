@@ -94,6 +94,13 @@ def _precision(value: str) -> int:
     return _PRECISIONS[value]
 
 
+def _inner_cycles(value) -> int:
+    value = _integer_parameter("inner_cycles", value)
+    if value < 0:
+        raise ValueError("inner_cycles must be non-negative (0: the library's default)")
+    return value
+
+
 def _valid_oriented_rows(points: np.ndarray, normals: np.ndarray) -> np.ndarray:
     """core.py:73-83: finite rows with a non-zero normal.
 
@@ -170,10 +177,13 @@ class Job:
 
     @staticmethod
     def params(stages: int, point_weight=4.0, samples_per_node=1.5, scale=1.1, cycles=4,
-               confidence=False, pre_smooth=3, post_smooth=3) -> _lib.mpr_params:
+               confidence=False, pre_smooth=3, post_smooth=3,
+               inner_cycles=0) -> _lib.mpr_params:
+        """Run parameters; inner_cycles: FP32 V-cycles per FP64 defect in mixed precision
+        (0: the library's default, one)."""
         return _lib.mpr_params(float(point_weight), float(samples_per_node), float(scale),
                                int(cycles), int(bool(confidence)), int(pre_smooth),
-                               int(post_smooth), int(stages), 0)
+                               int(post_smooth), int(stages), int(inner_cycles))
 
     def run(self, params: _lib.mpr_params) -> None:
         check(self._lib.mpr_job_run(self.handle, ctypes.byref(params)), "mpr_job_run")
@@ -301,8 +311,11 @@ def solve_screened_poisson(
     cycles: int = 4,
     samples_per_node: float = 1.5,
     precision: str = "fp64",
+    inner_cycles: int = 0,
 ) -> GridSolution:
-    """Build the octree, splat samples, and solve the screened Poisson system (core.py:196-306)."""
+    """Build the octree, splat samples, and solve the screened Poisson system (core.py:196-306).
+    precision and inner_cycles (mixed precision: FP32 V-cycles per FP64 defect, 0 for the
+    library's default) have no counterpart in the reference."""
     points = _points(points)
     normals = _points(normals)
     if points.shape != normals.shape:
@@ -312,6 +325,7 @@ def solve_screened_poisson(
     point_weight = _finite_parameter("point_weight", point_weight)
     samples_per_node = _finite_parameter("samples_per_node", samples_per_node)
     mode = _precision(precision)
+    inner_cycles = _inner_cycles(inner_cycles)
     if not 2 <= depth <= MAX_DEPTH:
         raise ValueError(f"depth must be between 2 and {MAX_DEPTH}")
     if cycles < 1:
@@ -326,7 +340,7 @@ def solve_screened_poisson(
     with _jobs.lend(depth, len(points), mode) as job:
         job.set_points(points, normals)
         job.run(Job.params(_lib.STAGE_SOLVE | _lib.STAGE_GRIDS, point_weight, samples_per_node,
-                           cycles=cycles, confidence=confidence))
+                           cycles=cycles, confidence=confidence, inner_cycles=inner_cycles))
         result = job.wait()
         if result.accepted != len(points):
             raise RuntimeError(
@@ -385,11 +399,14 @@ def reconstruct(
     scale: float = 1.1,
     samples_per_node: float = 1.5,
     precision: str = "fp64",
+    inner_cycles: int = 0,
 ) -> Mesh:
-    """Reconstruct a watertight surface from oriented points (core.py:388-426)."""
+    """Reconstruct a watertight surface from oriented points (core.py:388-426). precision and
+    inner_cycles as solve_screened_poisson's."""
     req = prepare_reconstruct(points, normals, depth=depth, point_weight=point_weight,
                               confidence=confidence, cycles=cycles, scale=scale,
-                              samples_per_node=samples_per_node, precision=precision)
+                              samples_per_node=samples_per_node, precision=precision,
+                              inner_cycles=inner_cycles)
     if req is None:
         return _empty_mesh()
     with _jobs.lend(req.depth, len(req.points), req.precision) as job:
@@ -410,7 +427,8 @@ class ReconstructRequest:
 
 
 def prepare_reconstruct(points, normals, *, depth=6, point_weight=4.0, confidence=False,
-                        cycles=4, scale=1.1, samples_per_node=1.5, precision="fp64"):
+                        cycles=4, scale=1.1, samples_per_node=1.5, precision="fp64",
+                        inner_cycles=0):
     """Validation of reconstruct() (core.py:400-411 and the solver checks of
     core.py:211-220), in the reference's order. The row filter of core.py:407-409 runs on the
     device (MPR_STAGE_FILTER), fused with the normalisation. Returns None when there is no
@@ -431,6 +449,7 @@ def prepare_reconstruct(points, normals, *, depth=6, point_weight=4.0, confidenc
         point_weight = _finite_parameter("point_weight", point_weight)
         samples_per_node = _finite_parameter("samples_per_node", samples_per_node)
         mode = _precision(precision)
+        inner_cycles = _inner_cycles(inner_cycles)
         if not 2 <= depth <= MAX_DEPTH:
             raise ValueError(f"depth must be between 2 and {MAX_DEPTH}")
         if cycles < 1:
@@ -445,7 +464,8 @@ def prepare_reconstruct(points, normals, *, depth=6, point_weight=4.0, confidenc
         raise
     stages = (_lib.STAGE_FILTER | _lib.STAGE_NORMALIZE | _lib.STAGE_SOLVE | _lib.STAGE_EXTRACT
               | _lib.STAGE_WORLD)
-    params = Job.params(stages, point_weight, samples_per_node, scale, cycles, confidence)
+    params = Job.params(stages, point_weight, samples_per_node, scale, cycles, confidence,
+                        inner_cycles=inner_cycles)
     return ReconstructRequest(points, normals, depth, mode, params)
 
 

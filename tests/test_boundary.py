@@ -83,6 +83,10 @@ def test_public_validation_rejects_unsafe_values():
         mpr.solve_screened_poisson(points, normals, depth=11)
     with pytest.raises(ValueError, match="precision"):
         mpr.solve_screened_poisson(points, normals, precision="fp16")
+    with pytest.raises(ValueError, match="inner_cycles"):
+        mpr.solve_screened_poisson(points, normals, precision="mixed", inner_cycles=-1)
+    with pytest.raises(TypeError, match="inner_cycles"):
+        mpr.reconstruct(points, normals, precision="mixed", inner_cycles=True)
     with pytest.raises(ValueError, match="inside"):
         mpr.solve_screened_poisson(points + 1.0, normals)
     with pytest.raises(ValueError, match="cubic"):
