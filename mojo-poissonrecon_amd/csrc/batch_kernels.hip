@@ -421,6 +421,90 @@ __global__ __launch_bounds__(kThreads) void kb_prolong_group(const StepPtrs<T>* 
                               l, lanes, vec_coarse != 0);
 }
 
+// FP32 prolong-add of colour 1 (the form before a post-smoothing) by blocks of four fine rows:
+// a lane takes the 4-slot chunk k of fine rows (2Y + a, 2Z + b), a, b in {0, 1}. All four
+// rows interpolate from the same coarse rows (Y | min(Y+1, nc-1), Z | min(Z+1, nc-1)) over
+// the same coarse x = 4k .. 4k + 4 (only tx differs, by the row's x parity), so the coarse
+// words are loaded once for 16 fine nodes instead of once per 4. Per node the arithmetic and
+// the clamping are prolong_group's (vec_coarse form), in the same order.
+__global__ __launch_bounds__(kThreads) void kb_prolong_c1_block(const StepPtrs<float>* tab,
+                                                                uint32_t count, uint32_t nbi,
+                                                                Geom gc, Geom gf, uint32_t nk,
+                                                                uint32_t nb2, uint32_t lanes) {
+  using F2 = float __attribute__((ext_vector_type(2)));
+  using F4 = float __attribute__((ext_vector_type(4)));
+  const uint32_t per = gridDim.x >> 3;
+  const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  const uint32_t slot = lb / nbi;
+  if (slot >= count) return;
+  const StepPtrs<float> p = tab[slot];
+  const uint32_t lane = (lb - slot * nbi) * kThreads + threadIdx.x;
+  if (lane >= lanes) return;
+  const uint32_t k = lane % nk, rest = lane / nk;
+  const int Y = (int)(rest % nb2), Z = (int)(rest / nb2);
+  const int n = gf.n, nc = gc.n;
+  const uint32_t k0 = 4u * k;
+  const gptr<const float> XC = as_global((const float*)p.xc);
+  const gptr<float> F = as_global(p.x);
+  const int y1 = Y + 1 < nc - 1 ? Y + 1 : nc - 1;
+  const int z1 = Z + 1 < nc - 1 ? Z + 1 : nc - 1;
+  // coarse rows (Y,Z), (y1,Z), (Y,z1), (y1,z1) at coarse x = k0 .. k0 + 4
+  float cv[4][5];
+  const uint32_t h = k0 >> 1;
+  const int m = nc - 1 - (int)k0;  // x = k0 + m is the last coarse node
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int yr = (r & 1) ? y1 : Y, zr = (r & 2) ? z1 : Z;
+    const uint32_t row = ((uint32_t)zr * (uint32_t)nc + (uint32_t)yr) * (uint32_t)gc.H;
+    const uint32_t ca = (uint32_t)((yr + zr) & 1);  // colour of the even x = k0
+    const uint32_t oa = ca * gc.C + row + h, ob = (1u - ca) * gc.C + row + h;
+    const F2 a = *(const __attribute__((address_space(1))) F2*)(XC + oa);
+    const F2 b = *(const __attribute__((address_space(1))) F2*)(XC + ob);
+    const float a2 = XC[oa + 2];
+    const float raw[5] = {a[0], b[0], a[1], b[1], a2};
+#pragma unroll
+    for (int i = 0; i <= 4; ++i) {
+      float v = raw[i];
+#pragma unroll
+      for (int q = 0; q < i; ++q)
+        if (i > m && q == m) v = raw[q];
+      cv[r][i] = v;
+    }
+  }
+  F4 xf[4];
+  uint32_t own[4];
+  bool in[4];
+#pragma unroll
+  for (int ab = 0; ab < 4; ++ab) {
+    const int y = 2 * Y + (ab & 1), z = 2 * Z + (ab >> 1);
+    in[ab] = y < n && z < n;
+    own[ab] = gf.C + ((uint32_t)(in[ab] ? z : 0) * (uint32_t)n + (uint32_t)(in[ab] ? y : 0)) *
+                         (uint32_t)gf.H + k0;
+    xf[ab] = *(const __attribute__((address_space(1))) F4*)(F + own[ab]);
+  }
+#pragma unroll
+  for (int ab = 0; ab < 4; ++ab) {
+    if (!in[ab]) continue;
+    const int y = 2 * Y + (ab & 1), z = 2 * Z + (ab >> 1);
+    const int px = (1 ^ y ^ z) & 1;
+    const float tx = px ? 0.5f : 0.0f, ty = (y & 1) ? 0.5f : 0.0f, tz = (z & 1) ? 0.5f : 0.0f;
+    float out[4];
+    uint32_t mask = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const float c00 = cv[0][j] * (1.0f - tx) + cv[0][j + 1] * tx;
+      const float c10 = cv[1][j] * (1.0f - tx) + cv[1][j + 1] * tx;
+      const float c01 = cv[2][j] * (1.0f - tx) + cv[2][j + 1] * tx;
+      const float c11 = cv[3][j] * (1.0f - tx) + cv[3][j + 1] * tx;
+      const float c0 = c00 * (1.0f - ty) + c10 * ty;
+      const float c1 = c01 * (1.0f - ty) + c11 * ty;
+      out[j] = xf[ab][j] + (c0 * (1.0f - tz) + c1 * tz);
+      if (2 * ((int)k0 + j) + px < n) mask |= 1u << j;
+    }
+    store_group<float, float, 4>(F + own[ab], out, mask);
+  }
+}
+
 template <class T>
 __global__ __launch_bounds__(kThreads) void kb_fill_x(const StepPtrs<T>* tab, uint32_t count,
                                                       uint32_t nbi, uint32_t words) {
@@ -885,6 +969,17 @@ template <class T>
 void launch_b_prolong(const StepPtrs<T>* tab, uint32_t count, const Geom& gc, const Geom& gf,
                       bool colour1_only, hipStream_t s) {
   if (!count) return;
+  if constexpr (sizeof(T) == 4) {
+    if (colour1_only && group_layout(gf) && group_layout(gc)) {
+      const uint32_t nk = (uint32_t)gf.H / 4u, nb2 = (uint32_t)gc.n;
+      const uint32_t lanes = nk * nb2 * nb2;
+      const uint32_t nbi = blocks_for(lanes, kThreads);
+      hipLaunchKernelGGL(kb_prolong_c1_block, dim3(padded8((uint64_t)nbi * count)),
+                         dim3(kThreads), 0, s, tab, count, nbi, gc, gf, nk, nb2, lanes);
+      MPR_LAUNCH_CHECK();
+      return;
+    }
+  }
   if (group_layout(gf)) {
     const uint32_t lanes = gf.C / (uint32_t)kGroup<T>;
     const int ncol = colour1_only ? 1 : 2;
