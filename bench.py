@@ -295,7 +295,7 @@ def depth9_run(steps: int) -> dict:
 
 def serve_stream(requests: int, qps: float, seed: int, warmup: int = 0,
                  sample_every: int = 0) -> dict:
-    """The served stream (shapes.served_requests: Poisson arrivals at `qps`, 5k-500k points,
+    """The served stream (shapes.served_schedule: Poisson arrivals at `qps`, 5k-500k points,
     three shape families randomly rotated and scaled, depths 6:7:8:9 = 4:3:2:1), mixed
     precision. The first `warmup` requests fill the server's job pool and are excluded from
     the latency percentiles. With sample_every, every such request's mesh is compared with
@@ -304,28 +304,46 @@ def serve_stream(requests: int, qps: float, seed: int, warmup: int = 0,
     from mojo_poissonrecon import shapes
     from mojo_poissonrecon.serve import ReconstructionServer
 
+    from collections import deque
+    from concurrent.futures import ThreadPoolExecutor
+
     total = requests + warmup
-    stream = shapes.served_requests(total, qps, seed=seed)
+    base, items = shapes.served_schedule(total, qps, seed=seed)
     server = ReconstructionServer(max_inflight=16, max_batch=8, precision="mixed")
     done, clouds = [], {}
+    # the client side: 4 threads form each request's rows (rotation and scaling of a pool
+    # cloud) up to 32 requests ahead of their arrival, off the server's thread; a request
+    # whose rows are late is submitted when they are ready, its latency still counted from
+    # its scheduled arrival
+    client = ThreadPoolExecutor(4)
+    ahead: deque = deque()
+    formed = 0
+
+    def refill():
+        nonlocal formed
+        while formed < total and len(ahead) < 32:
+            ahead.append(client.submit(shapes.served_cloud, base, items[formed]))
+            formed += 1
+
+    refill()
     start = time.perf_counter()
-    pending = next(stream, None)
     submitted = 0
     last_log = start
     while len(done) < total:
         now = time.perf_counter() - start
-        while pending is not None and pending[0] <= now:
-            arrival, p, n, d = pending
+        while ahead and items[submitted][0] <= now and ahead[0].done():
+            arrival, p, n, d = ahead.popleft().result()
             r = server.submit(p, n, depth=d, arrival=start + arrival)
             if sample_every and submitted >= warmup and (submitted - warmup) % sample_every == 0:
                 clouds[r.id] = (p, n, d)
             submitted += 1
-            pending = next(stream, None)
+            refill()
         done += server.step()
         if time.perf_counter() - last_log > 20.0:
             last_log = time.perf_counter()
             log(f"serve {qps:g} QPS: {len(done)}/{total} done")
     wall = time.perf_counter() - start
+    client.shutdown()
     stats = server.stats
     server.close()
     measured = [r for r in done if r.id >= warmup]

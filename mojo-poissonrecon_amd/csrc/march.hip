@@ -23,7 +23,9 @@
 // screening weights, so every diagonal is >= 3 h^-2 > 0 and relax_node's diagonal guard
 // never skips a node (the first sweep's colour is not read, only overwritten).
 #include <algorithm>
+#include <map>
 #include <mutex>
+#include <tuple>
 
 #include "mpr_batch.h"
 #include "stencil.h"
@@ -543,6 +545,25 @@ MarchPlan march_plan(const Geom& g, uint32_t count) {
   return pl;
 }
 
+// march_plan's search over tilings (a few hundred thousand steps on 257^3 and larger levels)
+// is done once per level shape, instance count and sweep count, then looked up: the
+// V-cycle asks for a plan at every launch
+template <int S>
+MarchPlan cached_march_plan(const Geom& g, uint32_t count) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, uint32_t>, MarchPlan> plans;
+  const auto key = std::make_tuple(g.n, g.H, count);
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    const auto it = plans.find(key);
+    if (it != plans.end()) return it->second;
+  }
+  const MarchPlan pl = march_plan<S>(g, count);
+  std::lock_guard<std::mutex> lock(mu);
+  plans.emplace(key, pl);
+  return pl;
+}
+
 template <int S, int LRT>
 void set_march_lds() {
   static std::once_flag once;
@@ -568,7 +589,7 @@ void launch_march_kernel(const StepPtrs<float>* tab, uint32_t count, const March
 template <int S>
 bool launch_march_s(const StepPtrs<float>* tab, uint32_t count, const Geom& g, double pw,
                     int first_colour, int zero_start, bool correct, bool first, hipStream_t s) {
-  const MarchPlan pl = march_plan<S>(g, count);
+  const MarchPlan pl = cached_march_plan<S>(g, count);
   if (!pl.ok) return false;
   MarchArgs a = pl.a;
   a.cA = first_colour;
@@ -635,8 +656,8 @@ int64_t fast_division_mismatches(int n, hipStream_t s) {
 
 bool march_supported(const Geom& g, uint32_t count, int sweeps) {
   switch (sweeps) {
-    case 2: return march_plan<2>(g, count).ok;
-    case 4: return march_plan<4>(g, count).ok;
+    case 2: return cached_march_plan<2>(g, count).ok;
+    case 4: return cached_march_plan<4>(g, count).ok;
     default: return false;
   }
 }

@@ -152,7 +152,10 @@ class BatchedVcycle {
  public:
   // Builds and uploads the tables (synchronously) into `table_mem` (device memory of at
   // least table_bytes(instances) bytes).
-  void build(const std::vector<InstanceLevels<T>>& inst, void* table_mem);
+  // stage: a pinned host buffer laid out like table_mem, which receives the tables for the
+  // caller's asynchronous upload; null: the tables are copied to table_mem synchronously
+  void build(const std::vector<InstanceLevels<T>>& inst, void* table_mem,
+             unsigned char* stage = nullptr);
   static size_t table_bytes(const std::vector<InstanceLevels<T>>& inst);
   // One V-cycle of every instance (core.py:141-193). zero_tops: every instance's top x
   // starts at zero (the correction equation of the defect-correction loop). fuse_correct
@@ -221,7 +224,8 @@ class BatchedVcycle {
 // FP64 defect-correction tables: instances grouped by finest class.
 class DefectTables {
  public:
-  void build(const std::vector<int>& depths, const std::vector<TopPtrs>& tops, void* table_mem);
+  void build(const std::vector<int>& depths, const std::vector<TopPtrs>& tops, void* table_mem,
+             unsigned char* stage = nullptr);
   static size_t table_bytes(size_t count) { return sizeof(TopPtrs) * count + 64 * 16; }
   // r32 = b - A x64; first: the first cycle of the loop (TopPtrs::x_initial). e0: also the
   // FP32 V-cycle's first colour-0 sweep of the top level from x = 0, e32 = r32 / diagonal

@@ -108,6 +108,7 @@ struct Job {
   int depth = 0, n = 0, precision = 0;
   uint32_t max_points = 0, P = 0;
   hipStream_t stream = nullptr;
+  hipEvent_t queued = nullptr;  // recorded on stream when a batch takes the job's work over
   DevArena arena;
   std::vector<Geom> geoms;
   // inputs; with MPR_STAGE_FILTER the kept rows' normals are compacted into nrm_kept
@@ -352,11 +353,31 @@ class BatchResources {
     std::lock_guard<std::mutex> lock(mu_);
     buffers_.emplace(cap, p);
   }
+  // pinned host staging of `cap` bytes (a capacity take_buffer returned)
+  unsigned char* take_stage(size_t cap) {
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      auto it = stages_.find(cap);
+      if (it != stages_.end()) {
+        unsigned char* p = it->second;
+        stages_.erase(it);
+        return p;
+      }
+    }
+    void* p = nullptr;
+    MPR_HIP(hipHostMalloc(&p, cap, hipHostMallocDefault));
+    return static_cast<unsigned char*>(p);
+  }
+  void give_stage(unsigned char* p, size_t cap) {
+    std::lock_guard<std::mutex> lock(mu_);
+    stages_.emplace(cap, p);
+  }
 
  private:
   std::mutex mu_;
   std::vector<hipStream_t> streams_;
   std::multimap<size_t, void*> buffers_;
+  std::multimap<size_t, unsigned char*> stages_;
 };
 
 // A batch runs everything on its own stream: the jobs' own stages (normalise .. pyramid,
@@ -368,6 +389,7 @@ struct Batch {
   int precision = 0;
   hipStream_t stream = nullptr;
   void* tables = nullptr;  // pooled (BatchResources)
+  unsigned char* stage = nullptr;  // its pinned host staging, same capacity
   size_t tables_cap = 0;
   BatchedVcycle<double> vc64;
   BatchedVcycle<float> vc32;
@@ -379,6 +401,14 @@ struct Batch {
 
   void build();
   void enqueue(const mpr_params* p);
+  // the batch's stream waits (on the device) for the work queued so far on its jobs' own
+  // streams, such as their point uploads
+  void after_job_streams() {
+    for (Job* j : jobs) {
+      MPR_HIP(hipEventRecord(j->queued, j->stream));
+      MPR_HIP(hipStreamWaitEvent(stream, j->queued, 0));
+    }
+  }
   void drop_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -388,6 +418,7 @@ struct Batch {
   ~Batch() {
     drop_graph();
     if (tables) BatchResources::get().give_buffer(tables, tables_cap);
+    if (stage) BatchResources::get().give_stage(stage, tables_cap);
     if (stream) BatchResources::get().give_stream(stream);
   }
 };
@@ -411,12 +442,16 @@ void Batch::build() {
   const size_t b32 = mixed ? round(BatchedVcycle<float>::table_bytes(i32)) : 0;
   const size_t bd = mixed ? round(DefectTables::table_bytes(tops.size())) : 0;
   tables = BatchResources::get().take_buffer(b64 + b32 + bd, tables_cap);
+  stage = BatchResources::get().take_stage(tables_cap);
   unsigned char* t = static_cast<unsigned char*>(tables);
-  vc64.build(i64, t);
+  vc64.build(i64, t, stage);
   if (mixed) {
-    vc32.build(i32, t + b64);
-    dtab.build(depths, tops, t + b64 + b32);
+    vc32.build(i32, t + b64, stage + b64);
+    dtab.build(depths, tops, t + b64 + b32, stage + b64 + b32);
   }
+  // one asynchronous upload on the batch's stream (a synchronous copy would wait behind the
+  // work of other batches in flight); the staging stays the batch's until it is destroyed
+  MPR_HIP(hipMemcpyAsync(tables, stage, b64 + b32 + bd, hipMemcpyHostToDevice, stream));
 }
 
 void Batch::enqueue(const mpr_params* p) {
@@ -585,6 +620,7 @@ Job::~Job() {
   oct.release();
   arena.release();
   if (host) (void)hipHostFree(host);
+  if (queued) (void)hipEventDestroy(queued);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -878,6 +914,7 @@ int mpr_job_create(int32_t depth, int64_t max_points, int32_t precision, mpr_job
   j->max_points = (uint32_t)max_points;
   j->precision = precision;
   MPR_HIP(hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking));
+  MPR_HIP(hipEventCreateWithFlags(&j->queued, hipEventDisableTiming));
   MPR_HIP(hipHostMalloc((void**)&j->host, sizeof(HostScalars), hipHostMallocDefault));
   std::memset(j->host, 0, sizeof(HostScalars));
   // one allocation for every buffer of the job: its HBM footprint is mpr_job_footprint
@@ -1251,8 +1288,7 @@ int mpr_batch_create(mpr_job* const* jobs, int32_t count, mpr_batch** out) {
     b->precision = jobs[0]->precision;
     for (int i = 0; i < count; ++i) b->jobs.push_back(jobs[i]);
     b->stream = BatchResources::get().take_stream();
-    for (Job* j : b->jobs) MPR_HIP(hipStreamSynchronize(j->stream));
-    b->build();
+    b->build();  // host work and a table upload into an idle buffer: no wait on the jobs
     *out = b;
     return 0;
   } catch (const HipError& e) {
@@ -1276,7 +1312,7 @@ int mpr_batch_run(mpr_batch* b, const mpr_params* params) {
     return -1;
   }
   MPR_GUARD_BEGIN
-  for (Job* j : b->jobs) MPR_HIP(hipStreamSynchronize(j->stream));  // order after uploads
+  b->after_job_streams();  // after the jobs' uploads, without waiting on the host
   b->last.clear();
   for (size_t i = 0; i < b->jobs.size(); ++i) b->last.push_back(canonical(params[i]));
   for (size_t i = 0; i < b->jobs.size(); ++i) {
@@ -1340,7 +1376,7 @@ int mpr_batch_launch(mpr_batch* b) {
     std::vector<mpr_params> p = b->last;
     if (int rc = mpr_batch_capture(b, p.data())) return rc;
   }
-  for (Job* j : b->jobs) MPR_HIP(hipStreamSynchronize(j->stream));
+  b->after_job_streams();
   for (size_t i = 0; i < b->jobs.size(); ++i) {
     b->jobs[i]->adopt(b->last[i]);
     b->jobs[i]->ran = true;

@@ -38,8 +38,18 @@ size_t BatchedVcycle<T>::table_bytes(const std::vector<InstanceLevels<T>>& inst)
          64 * 64;
 }
 
+// the tables into pinned staging memory (the caller uploads it) or straight to the device
+void put_tables(void* table_mem, unsigned char* stage, const std::vector<unsigned char>& host) {
+  if (host.empty()) return;
+  if (stage)
+    std::memcpy(stage, host.data(), host.size());
+  else
+    MPR_HIP(hipMemcpy(table_mem, host.data(), host.size(), hipMemcpyHostToDevice));
+}
+
 template <class T>
-void BatchedVcycle<T>::build(const std::vector<InstanceLevels<T>>& inst, void* table_mem) {
+void BatchedVcycle<T>::build(const std::vector<InstanceLevels<T>>& inst, void* table_mem,
+                             unsigned char* stage) {
   init_coarse_kernels();
   smax = 1;
   for (const auto& i : inst) smax = std::max(smax, i.depth);
@@ -103,7 +113,7 @@ void BatchedVcycle<T>::build(const std::vector<InstanceLevels<T>>& inst, void* t
   coarse_class_ = 1;
   for (const auto& e : ce) coarse_class_ = std::max(coarse_class_, e.top);
   if (w.host.size() > table_bytes(inst)) throw HipError{"batched V-cycle table overflow"};
-  MPR_HIP(hipMemcpy(table_mem, w.host.data(), w.host.size(), hipMemcpyHostToDevice));
+  put_tables(table_mem, stage, w.host);
 }
 
 template <class T>
@@ -190,7 +200,7 @@ bool BatchedVcycle<T>::march_finest(double pw, hipStream_t s) const {
 }
 
 void DefectTables::build(const std::vector<int>& depths, const std::vector<TopPtrs>& tops,
-                         void* table_mem) {
+                         void* table_mem, unsigned char* stage) {
   int dmax = 1;
   for (int d : depths) dmax = std::max(dmax, d);
   geom_.assign(dmax + 1, Geom());
@@ -206,8 +216,7 @@ void DefectTables::build(const std::vector<int>& depths, const std::vector<TopPt
     count_[d] = (uint32_t)e.size();
   }
   if (w.host.size() > table_bytes(depths.size())) throw HipError{"defect table overflow"};
-  if (!w.host.empty())
-    MPR_HIP(hipMemcpy(table_mem, w.host.data(), w.host.size(), hipMemcpyHostToDevice));
+  put_tables(table_mem, stage, w.host);
 }
 
 void DefectTables::defect(double pw, hipStream_t s, bool first, bool e0) const {

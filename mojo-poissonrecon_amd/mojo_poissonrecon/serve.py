@@ -8,8 +8,10 @@ without waiting. Requests of one depth admitted in the same tick run as one batc
 launch per V-cycle step for all of them); batches over the same pooled jobs are reused.
 Work of different requests overlaps on the device; each extraction copies its mesh into
 its job's pinned host buffers on the job's stream, so a finished request's mesh is read
-back with a host copy, which (like the uploads) runs on a thread pool beside the loop.
-Latency is measured from a request's arrival to its mesh being on the host.
+back with a host copy. Uploads and mesh copies run on a thread pool and the loop never
+waits for them: the requests admitted in one tick are launched once all their uploads are
+done, and a finished flight's requests complete once their copies are. Latency is
+measured from a request's arrival to its mesh being on the host.
 
 Failures are isolated per flight (one batch, or one solo request): its jobs are discarded,
 the rest of the server keeps running, and each of its requests is run again alone as the
@@ -59,6 +61,7 @@ class _Flight:
     requests: list
     jobs: list
     batch: Batch | None = None
+    copies: list | None = None  # the mesh copies' futures, once the device work is done
 
 
 @dataclass
@@ -103,10 +106,15 @@ class ReconstructionServer:
     def __init__(self, *, hbm_budget: int | None = None, max_inflight: int = 12,
                  max_batch: int = 8, precision: str = "mixed", host_threads: int = 8,
                  batch_cache: int = 8, job_factory=None, batch_factory=None,
-                 fallback=_fp64_fallback):
+                 fallback=_fp64_fallback, linger: float = 0.004, min_active: int = 2):
         self.precision = precision
         self.max_inflight = max_inflight
         self.max_batch = max_batch
+        # uploaded requests of one depth wait to be launched together: until max_batch of
+        # them are ready, the device has fewer than min_active flights running, or the
+        # oldest has waited `linger` seconds since its admission
+        self.linger = linger
+        self.min_active = min_active
         if hbm_budget is None:
             total = _lib.ctypes.c_int64()
             _lib.check(_lib.lib().mpr_device_info(None, 0, _lib.ctypes.byref(total), None),
@@ -121,6 +129,8 @@ class ReconstructionServer:
         self._bytes = 0
         self._waiting: list[Request] = []
         self._flights: list[_Flight] = []
+        self._cohorts: list[list] = []    # (request, job, upload future) admitted in one tick
+        self._staged: dict[int, list] = {}  # depth -> (request, job) uploaded, not launched
         self._ready: list[Request] = []   # failed flights' requests, returned by the next step
         self._batches: OrderedDict = OrderedDict()  # job ids -> Batch, least recent first
         self._batch_cache = batch_cache
@@ -156,14 +166,24 @@ class ReconstructionServer:
 
     def drain(self) -> list[Request]:
         out = []
-        while self._waiting or self._flights or self._ready:
+        while self._waiting or self._flights or self._ready or self._cohorts or self._staged:
             out += self.step()
-            if self._flights and not out:
+            if (self._flights or self._cohorts or self._staged) and not out:
                 time.sleep(0)
         return out
 
     def close(self) -> None:
+        for cohort in self._cohorts:
+            for _, _, fut in cohort:
+                fut.result()  # an upload still copying from the caller's rows
+        self._cohorts.clear()
+        self._staged.clear()
         for f in self._flights:
+            for c in f.copies or ():
+                try:
+                    c.result()
+                except Exception:
+                    pass
             try:
                 if f.batch is not None:
                     f.batch.wait()
@@ -185,7 +205,8 @@ class ReconstructionServer:
 
     @property
     def inflight(self) -> int:
-        return sum(len(f.requests) for f in self._flights)
+        return (sum(len(f.requests) for f in self._flights) + sum(len(c) for c in self._cohorts)
+                + sum(len(g) for g in self._staged.values()))
 
     # ---- internals ------------------------------------------------------------------
     def _footprint(self, j) -> int:
@@ -241,25 +262,41 @@ class ReconstructionServer:
             self._waiting.pop(0)
             self._busy.add(id(job))
             admitted.append((r, job))
-        if not admitted:
-            return
-        # uploads on the thread pool (each one a host copy that releases the GIL)
-        uploads = list(self._threads.map(lambda rj: self._upload(*rj), admitted))
-        groups: dict[int, list] = {}
+        if admitted:
+            # uploads on the thread pool (each one a host copy that releases the GIL)
+            now = time.perf_counter()
+            cohort = []
+            for r, job in admitted:
+                r.admitted = now
+                cohort.append((r, job, self._threads.submit(self._upload, r, job)))
+            self._cohorts.append(cohort)
+        # stage the ticks whose uploads are all done (a tick's requests stay together), in
+        # admission order
+        while self._cohorts and all(fut.done() for _, _, fut in self._cohorts[0]):
+            for r, job, fut in self._cohorts.pop(0):
+                err = fut.result()
+                if err is not None:
+                    self._fail(_Flight([r], [job]), err)
+                else:
+                    self._staged.setdefault(r.depth, []).append((r, job))
+        # launch full batches, and the rest once the device runs short of work or they
+        # have waited long enough
+        active = sum(1 for f in self._flights if f.copies is None)
         now = time.perf_counter()
-        for (r, job), err in zip(admitted, uploads):
-            r.admitted = now
-            if err is not None:
-                self._fail(_Flight([r], [job]), err)
-                continue
-            group = groups.setdefault(r.depth, [])
-            group.append((r, job))
-            if len(group) == self.max_batch:
+        for depth in list(self._staged):
+            group = self._staged[depth]
+            while len(group) >= self.max_batch:
+                self._launch(group[:self.max_batch])
+                group = group[self.max_batch:]
+                active += 1
+            if group and (active < self.min_active or now - group[0][0].admitted >= self.linger):
                 self._launch(group)
-                groups[r.depth] = []
-        for group in groups.values():
+                group = []
+                active += 1
             if group:
-                self._launch(group)
+                self._staged[depth] = group
+            else:
+                del self._staged[depth]
         self.stats.peak_inflight = max(self.stats.peak_inflight, self.inflight)
 
     @staticmethod
@@ -348,20 +385,27 @@ class ReconstructionServer:
     def _collect(self) -> list[Request]:
         still, finished = [], []
         for f in self._flights:
-            try:
-                ready = f.batch.done() if f.batch is not None else f.jobs[0].done()
-            except Exception as e:  # noqa: BLE001
-                self._fail(f, e)
-                continue
-            (finished if ready else still).append(f)
+            if f.copies is None:
+                try:
+                    ready = f.batch.done() if f.batch is not None else f.jobs[0].done()
+                    if ready:
+                        results = f.batch.wait() if f.batch is not None else [f.jobs[0].wait()]
+                        # host copies of the meshes on the thread pool; the flight keeps its
+                        # jobs (their pinned meshes are being read) until every copy is done
+                        f.copies = [self._threads.submit(mesh_from_job, j, res, len(r.points))
+                                    for r, j, res in zip(f.requests, f.jobs, results)]
+                except Exception as e:  # noqa: BLE001
+                    self._fail(f, e)
+                    continue
+                still.append(f)
+            elif all(c.done() for c in f.copies):
+                finished.append(f)
+            else:
+                still.append(f)
         self._flights = still
         for f in finished:
             try:
-                results = f.batch.wait() if f.batch is not None else [f.jobs[0].wait()]
-                # host copies of the meshes on the thread pool
-                meshes = list(self._threads.map(
-                    lambda t: mesh_from_job(t[1], t[2], len(t[0].points)),
-                    zip(f.requests, f.jobs, results)))
+                meshes = [c.result() for c in f.copies]
             except Exception as e:  # noqa: BLE001
                 self._fail(f, e)
                 continue

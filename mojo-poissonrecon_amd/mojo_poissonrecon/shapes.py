@@ -120,12 +120,10 @@ def uneven_batch_workload(count: int = 64, seed: int = 1):
     return clouds, depths
 
 
-def served_requests(count: int, qps: float, seed: int = 2, pool: int = 24):
-    """A served request stream: Poisson arrivals at `qps` (arrival times in seconds from 0),
-    each a cloud of 5k-500k points (log-uniform) on a random shape family, randomly rotated
-    and scaled into [-1, 1]^3, at a depth from {6, 7, 8, 9} weighted 4:3:2:1. The clouds are
-    drawn from `pool` pre-generated ones (rotated and scaled anew per request) to bound the
-    host work; yields (arrival, points, normals, depth) lazily."""
+def served_schedule(count: int, qps: float, seed: int = 2, pool: int = 24):
+    """The served stream of served_requests without its point arrays: the `pool` base clouds
+    and, per request, (arrival, base index, rotation, scale, depth). served_cloud(base, item)
+    forms a request's rows, so a client can form them ahead of time on other threads."""
     rng = np.random.default_rng(seed)
     makers = (sphere, torus, superquadric)
     base = []
@@ -136,14 +134,32 @@ def served_requests(count: int, qps: float, seed: int = 2, pool: int = 24):
     arrivals = np.cumsum(rng.exponential(1.0 / qps, count))
     depths = rng.choice([6, 7, 8, 9], size=count, p=[0.4, 0.3, 0.2, 0.1])
     picks = rng.integers(0, pool, count)
+    items = []
     for k in range(count):
-        p, n = base[picks[k]]
         rot = random_rotation(rng)
         scale = float(rng.uniform(0.5, 1.0))
-        pts = p @ rot.T
-        pts = pts / np.abs(pts).max() * scale
-        yield float(arrivals[k]), np.ascontiguousarray(pts), np.ascontiguousarray(n @ rot.T), \
-            int(depths[k])
+        items.append((float(arrivals[k]), int(picks[k]), rot, scale, int(depths[k])))
+    return base, items
+
+
+def served_cloud(base, item):
+    """(arrival, points, normals, depth) of one served_schedule item."""
+    arrival, pick, rot, scale, depth = item
+    p, n = base[pick]
+    pts = p @ rot.T
+    pts = pts / np.abs(pts).max() * scale
+    return arrival, np.ascontiguousarray(pts), np.ascontiguousarray(n @ rot.T), depth
+
+
+def served_requests(count: int, qps: float, seed: int = 2, pool: int = 24):
+    """A served request stream: Poisson arrivals at `qps` (arrival times in seconds from 0),
+    each a cloud of 5k-500k points (log-uniform) on a random shape family, randomly rotated
+    and scaled into [-1, 1]^3, at a depth from {6, 7, 8, 9} weighted 4:3:2:1. The clouds are
+    drawn from `pool` pre-generated ones (rotated and scaled anew per request) to bound the
+    host work; yields (arrival, points, normals, depth) lazily."""
+    base, items = served_schedule(count, qps, seed, pool)
+    for item in items:
+        yield served_cloud(base, item)
 
 
 def random_rotation(rng) -> np.ndarray:

@@ -144,3 +144,31 @@ def test_batches_over_the_same_jobs_are_reused():
     assert len(jobs) == 3 and FakeBatch.made == 1
     assert server.stats.batches == 3 and server.stats.batches_reused == 2
     server.close()
+
+
+def test_a_failed_upload_is_isolated_and_the_rest_of_its_tick_runs():
+    # uploads run on the thread pool without blocking the loop; the requests admitted in
+    # one tick launch together once all their uploads are done, minus those that failed
+    FakeBatch.made = 0
+    FakeBatch.fail_at = {}
+    made = []
+
+    def job_factory(d, c, p):
+        j = FakeJob(d, c, p)
+        if not made:
+            def broken(points, normals):
+                raise RuntimeError("upload failed")
+            j.set_points = broken
+        made.append(j)
+        return j
+
+    server = ReconstructionServer(hbm_budget=1 << 50, max_inflight=6, max_batch=3,
+                                  job_factory=job_factory, batch_factory=FakeBatch,
+                                  fallback=fallback_mesh)
+    reqs = [server.submit(*cloud(), depth=6) for _ in range(3)]
+    done = server.drain()
+    assert {r.id for r in done} == {r.id for r in reqs}
+    assert [r.id for r in done if r.fallback] == [reqs[0].id]
+    assert made[0].closed and not any(j.closed for j in made[1:])
+    assert FakeBatch.made == 1 and server.stats.batches == 1  # the other two, as one batch
+    server.close()
