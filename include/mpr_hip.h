@@ -78,6 +78,13 @@ int mpr_prolong_add_f32(const float *coarse, float *fine, int64_t fine_n);
 int mpr_vcycle_transfer_f32(const float *solution, const float *rhs, const float *screen,
                             float *residual, float *coarse_rhs, int64_t n, double point_weight);
 
+/* The mixed V-cycle's prolongation before a post-smoothing, on host float32 grids
+ * (fine_n = 2^s + 1 in [65, 1025]): fine += trilinear(coarse) on the fine colour-1 nodes
+ * ((x + y + z) odd) only, by the batched kernel the V-cycle launches (colour 0 is
+ * overwritten by the next sweep without being read). No reference counterpart: a check of
+ * that kernel against a float32 restatement. 0, or -1 with mpr_last_error(). */
+int mpr_prolong_c1_f32(const float *coarse, float *fine, int64_t fine_n);
+
 /* Replaces mpr_restrict_f64 (src/capi.mojo:556-673; _lib.py:27). */
 void mpr_restrict_f64(const double *fine, double *coarse, int64_t fine_n);
 

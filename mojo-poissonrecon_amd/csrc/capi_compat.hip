@@ -336,6 +336,34 @@ int mpr_vcycle_transfer_f32(const float* solution, const float* rhs, const float
   }
 }
 
+int mpr_prolong_c1_f32(const float* coarse, float* fine, int64_t fine_n) {
+  if (fine_n < 65 || fine_n > 1025 || ((fine_n - 1) & (fine_n - 2)) != 0) {
+    set_error("mpr_prolong_c1_f32: fine_n must be 2^s + 1 in [65, 1025]");
+    return -1;
+  }
+  if (ensure_device()) return -1;
+  try {
+    const int64_t cn = (fine_n + 1) / 2;
+    const Geom gf = make_geom((int32_t)fine_n), gc = make_geom((int32_t)cn);
+    const size_t Mf = (size_t)fine_n * fine_n * fine_n, Mc = (size_t)cn * cn * cn;
+    DevBuf<double> nat(Mf);
+    DevBuf<float> f(2ull * gf.C), c(2ull * gc.C);
+    upload_f32_as_split(coarse, c.p, gc, nat, Mc);
+    upload_f32_as_split(fine, f.p, gf, nat, Mf);
+    StepPtrs<float> step{};
+    step.x = f.p;
+    step.xc = c.p;
+    DevBuf<StepPtrs<float>> tab(1);
+    h2d(tab.p, &step, 1);
+    launch_b_prolong<float>(tab.p, 1, gc, gf, true, kNull);
+    download_split_f32(f.p, fine, gf, nat, Mf);
+    return 0;
+  } catch (const HipError& e) {
+    report(__func__, e.msg);
+    return -1;
+  }
+}
+
 void mpr_restrict_f64(const double* fine, double* coarse, int64_t fine_n) {
   if (fine_n < 1 || ensure_device()) return;
   try {

@@ -128,6 +128,7 @@ _PIPELINE = {
     "mpr_restrict_f32": ([P, P, I], ctypes.c_int),
     "mpr_prolong_add_f32": ([P, P, I], ctypes.c_int),
     "mpr_vcycle_transfer_f32": ([P, P, P, P, P, I, F], ctypes.c_int),
+    "mpr_prolong_c1_f32": ([P, P, I], ctypes.c_int),
     "mpr_extract": ([P, P, I, F, _PP, _IP, _IP], ctypes.c_int),
     "mpr_mesh_fetch": ([P, P, P, P], ctypes.c_int),
     "mpr_mesh_destroy": ([P], ctypes.c_int),

@@ -94,6 +94,24 @@ def test_prolong_f32_bit_exact_against_float32_restatement(n):
     assert np.array_equal(fine, expected)
 
 
+@pytest.mark.parametrize("n", [65, 129, 257])
+def test_colour1_prolongation_bit_exact(n):
+    # the mixed V-cycle's prolongation before a post-smoothing (a lane updates four fine
+    # rows from one load of their shared coarse rows): colour 1 gets the float32
+    # restatement's sum, colour 0 keeps its values
+    rng = np.random.default_rng(900 + n)
+    cn = (n + 1) // 2
+    coarse = rng.normal(size=(cn, cn, cn)).astype(np.float32)
+    coarse[0, 0, 0] = -0.0  # signed zeros and exact halves take the same roundings
+    coarse[cn - 1, cn - 1, cn - 1] = 0.5
+    fine = rng.normal(size=(n, n, n)).astype(np.float32)
+    z, y, x = np.indices(fine.shape)
+    want = np.where(((x + y + z) & 1) == 1, oracle.prolong_np(coarse, fine), fine)
+    got = fine.copy()
+    assert lib().mpr_prolong_c1_f32(coarse.ctypes.data, got.ctypes.data, n) == 0
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
 @pytest.mark.parametrize("n", [3, 9, 10, 17, 65, 66, 129, 130])
 def test_restrict_bit_exact(n):
     fine = np.random.default_rng(11).normal(size=(n, n, n))
