@@ -51,7 +51,7 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from rocprofv3 counter runs (the timing repetitions'
 # launches), the factors calibrated on a plain elementwise copy
 # (profiles/kb_march_f32_depth8_pmc.txt).
-ROOFLINE_TRAFFIC_BYTES = {(8, 8): 2_233_881_515}
+ROOFLINE_TRAFFIC_BYTES = {(8, 8): 2_228_616_096}
 
 
 def log(*args):
