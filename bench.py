@@ -293,24 +293,35 @@ def depth9_run(steps: int) -> dict:
             / out["fp64"]["residual_norm"]}
 
 
+def new_server():
+    from mojo_poissonrecon.serve import ReconstructionServer
+
+    return ReconstructionServer(max_inflight=16, max_batch=8, precision="mixed")
+
+
 def serve_stream(requests: int, qps: float, seed: int, warmup: int = 0,
-                 sample_every: int = 0) -> dict:
+                 sample_every: int = 0, server=None) -> dict:
     """The served stream (shapes.served_schedule: Poisson arrivals at `qps`, 5k-500k points,
     three shape families randomly rotated and scaled, depths 6:7:8:9 = 4:3:2:1), mixed
     precision. The first `warmup` requests fill the server's job pool and are excluded from
     the latency percentiles. With sample_every, every such request's mesh is compared with
-    its all-FP64 solo solve."""
+    its all-FP64 solo solve. `server`: a running server to use (its job pool stays warm
+    from earlier streams; counts below are this stream's), else a new one, closed after."""
     import mojo_poissonrecon as mpr
     from mojo_poissonrecon import shapes
-    from mojo_poissonrecon.serve import ReconstructionServer
 
     from collections import deque
     from concurrent.futures import ThreadPoolExecutor
 
     total = requests + warmup
     base, items = shapes.served_schedule(total, qps, seed=seed)
-    server = ReconstructionServer(max_inflight=16, max_batch=8, precision="mixed")
+    own = server is None
+    if own:
+        server = new_server()
+    st0 = dict(vars(server.stats))
+    st0.pop("latencies")
     done, clouds = [], {}
+    id0 = None
     # the client side: 4 threads form each request's rows (rotation and scaling of a pool
     # cloud) up to 32 requests ahead of their arrival, off the server's thread; a request
     # whose rows are late is submitted when they are ready, its latency still counted from
@@ -334,6 +345,7 @@ def serve_stream(requests: int, qps: float, seed: int, warmup: int = 0,
         while ahead and items[submitted][0] <= now and ahead[0].done():
             arrival, p, n, d = ahead.popleft().result()
             r = server.submit(p, n, depth=d, arrival=start + arrival)
+            id0 = r.id if id0 is None else id0
             if sample_every and submitted >= warmup and (submitted - warmup) % sample_every == 0:
                 clouds[r.id] = (p, n, d)
             submitted += 1
@@ -345,20 +357,23 @@ def serve_stream(requests: int, qps: float, seed: int, warmup: int = 0,
     wall = time.perf_counter() - start
     client.shutdown()
     stats = server.stats
-    server.close()
-    measured = [r for r in done if r.id >= warmup]
+    delta = {k: getattr(stats, k) - v for k, v in st0.items()
+             if not k.startswith("peak")}
+    if own:
+        server.close()
+    measured = [r for r in done if r.id >= id0 + warmup]
     lat = np.array([r.latency for r in measured])
     last = max(r.completed for r in measured) - start
     first = min(r.arrival for r in measured) - start
     out = {"requests": requests, "warmup_requests": warmup, "qps_offered": qps,
            "recon_per_s": round(len(measured) / (last - first), 2),
-           "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "batches": stats.batches,
-           "batches_reused": stats.batches_reused, "failed": stats.failed,
-           "fallbacks": stats.fallbacks, "peak_inflight": stats.peak_inflight,
-           "peak_hbm_gb": round(stats.peak_bytes / 1e9, 2), "jobs_created": stats.jobs_created,
-           "host_s": {"prepare": round(stats.host_prepare, 3),
-                      "launch": round(stats.host_launch, 3),
-                      "collect": round(stats.host_collect, 3), "wall": round(wall, 3)}}
+           "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "batches": delta["batches"],
+           "batches_reused": delta["batches_reused"], "failed": delta["failed"],
+           "fallbacks": delta["fallbacks"], "peak_inflight": stats.peak_inflight,
+           "peak_hbm_gb": round(stats.peak_bytes / 1e9, 2), "jobs_created": delta["jobs_created"],
+           "host_s": {"prepare": round(delta["host_prepare"], 3),
+                      "launch": round(delta["host_launch"], 3),
+                      "collect": round(delta["host_collect"], 3), "wall": round(wall, 3)}}
     if clouds:
         errs = []
         for r in measured:
@@ -374,15 +389,21 @@ def serve_stream(requests: int, qps: float, seed: int, warmup: int = 0,
 
 def serve_runs(requests: int, sweep: list[float], sweep_requests: int, qps: float | None):
     """A QPS sweep of short streams (capacity = the best completed rate), then the long
-    stream at `qps`, or at 80 % of that capacity."""
+    stream at `qps`, or at 80 % of that capacity, all on one server: its job pool warms up
+    in the first stream and stays warm, as in a long-running server (a new server's first
+    few hundred requests create most of its jobs, each a device allocation of up to tens
+    of GB, and measure that warm-up instead)."""
+    server = new_server()
     points = []
     for q in sweep:
-        r = serve_stream(sweep_requests, q, seed=5, warmup=50)
+        r = serve_stream(sweep_requests, q, seed=5, warmup=50, server=server)
         points.append({k: r[k] for k in ("qps_offered", "recon_per_s", "p50_ms", "p99_ms")})
         log("serve sweep:", points[-1])
     capacity = max(p["recon_per_s"] for p in points) if points else None
     rate = qps or (round(0.8 * capacity, 1) if capacity else 100.0)
-    main = serve_stream(requests, rate, seed=2, warmup=100, sample_every=max(1, requests // 20))
+    main = serve_stream(requests, rate, seed=2, warmup=100, sample_every=max(1, requests // 20),
+                        server=server)
+    server.close()
     log("serve:", main)
     return {"sweep": points, "capacity_recon_per_s": capacity, "stream": main,
             "depth_mix": "6:7:8:9 = 4:3:2:1", "points": "5k-500k log-uniform"}
