@@ -296,7 +296,7 @@ def depth9_run(steps: int) -> dict:
 def new_server():
     from mojo_poissonrecon.serve import ReconstructionServer
 
-    return ReconstructionServer(max_inflight=16, max_batch=8, precision="mixed")
+    return ReconstructionServer(max_inflight=32, max_batch=8, precision="mixed")
 
 
 def serve_stream(requests: int, qps: float, seed: int, warmup: int = 0,
@@ -424,7 +424,7 @@ def main() -> int:
                          "capacity the sweep measures)")
     ap.add_argument("--serve-sweep", default="100,200,400,800",
                     help="comma-separated offered rates of the short sweep streams ('' for none)")
-    ap.add_argument("--serve-sweep-requests", type=int, default=500)
+    ap.add_argument("--serve-sweep-requests", type=int, default=1000)
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the served streams and print their line")
     ap.add_argument("--no-cpu-baseline", action="store_true")

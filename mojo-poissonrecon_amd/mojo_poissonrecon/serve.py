@@ -8,10 +8,12 @@ without waiting. Requests of one depth admitted in the same tick run as one batc
 launch per V-cycle step for all of them); batches over the same pooled jobs are reused.
 Work of different requests overlaps on the device; each extraction copies its mesh into
 its job's pinned host buffers on the job's stream, so a finished request's mesh is read
-back with a host copy. Uploads and mesh copies run on a thread pool and the loop never
-waits for them: the requests admitted in one tick are launched once all their uploads are
-done, and a finished flight's requests complete once their copies are. Latency is
-measured from a request's arrival to its mesh being on the host.
+back with a host copy. Uploads, mesh copies, the creation of new pool jobs and the release
+of evicted ones (device allocations and frees, which can wait behind the work in flight)
+run on a thread pool and the loop never waits for them: the requests admitted in one tick
+are launched once all their jobs are made and uploaded, and a finished flight's requests
+complete once their copies are. Latency is measured from a request's arrival to its mesh
+being on the host.
 
 Failures are isolated per flight (one batch, or one solo request): its jobs are discarded,
 the rest of the server keeps running, and each of its requests is run again alone as the
@@ -129,7 +131,9 @@ class ReconstructionServer:
         self._bytes = 0
         self._waiting: list[Request] = []
         self._flights: list[_Flight] = []
-        self._cohorts: list[list] = []    # (request, job, upload future) admitted in one tick
+        # (request, upload future) admitted in one tick; the future makes the request's job
+        # first when no idle one fitted, and returns (job, error)
+        self._cohorts: list[list] = []
         self._staged: dict[int, list] = {}  # depth -> (request, job) uploaded, not launched
         self._ready: list[Request] = []   # failed flights' requests, returned by the next step
         self._batches: OrderedDict = OrderedDict()  # job ids -> Batch, least recent first
@@ -174,8 +178,10 @@ class ReconstructionServer:
 
     def close(self) -> None:
         for cohort in self._cohorts:
-            for _, _, fut in cohort:
-                fut.result()  # an upload still copying from the caller's rows
+            for _, fut in cohort:
+                job, _ = fut.result()  # an upload still copying from the caller's rows
+                if job is not None and job not in self._pool:
+                    self._pool.append(job)
         self._cohorts.clear()
         self._staged.clear()
         for f in self._flights:
@@ -212,20 +218,30 @@ class ReconstructionServer:
     def _footprint(self, j) -> int:
         return Job.footprint(j.depth, j.max_points, j.precision)
 
-    def _discard_job(self, job) -> None:
-        """Drop a job (evicted, or in an unknown state after a failure) and its batches."""
-        for key in [k for k in self._batches if id(job) in k]:
-            self._batches.pop(key).close()
+    def _detach_job(self, job) -> list:
+        """Take a job and its batches out of the server; returns them, to be closed."""
+        gone = [self._batches.pop(k) for k in [k for k in self._batches if id(job) in k]]
         if job in self._pool:
             self._pool.remove(job)
             self._bytes -= self._footprint(job)
         self._busy.discard(id(job))
-        try:
-            job.close()
-        except Exception:
-            pass
+        return gone + [job]
+
+    @staticmethod
+    def _close_all(objs) -> None:
+        for o in objs:
+            try:
+                o.close()
+            except Exception:
+                pass
+
+    def _discard_job(self, job) -> None:
+        """Drop a job in an unknown state after a failure, and its batches."""
+        self._close_all(self._detach_job(job))
 
     def _take_job(self, r: Request):
+        """An idle pooled job for `r`, or (capacity, footprint, evicted) for a new one that
+        fits the budget once the idle jobs in `evicted` are gone, or None."""
         best = None
         for j in self._pool:
             if (id(j) not in self._busy and j.depth == r.depth and j.precision == r.precision
@@ -239,16 +255,16 @@ class ReconstructionServer:
         # evict idle jobs (largest first) until the new one fits the budget
         idle = [j for j in self._pool if id(j) not in self._busy]
         idle.sort(key=self._footprint)
+        evicted = []
         while self._bytes + need > self.hbm_budget and idle:
-            self._discard_job(idle.pop())
+            evicted += self._detach_job(idle.pop())
         if self._bytes + need > self.hbm_budget:
+            self._threads.submit(self._close_all, evicted)
             return None
-        job = self._job_factory(r.depth, capacity, r.precision)
         self.stats.jobs_created += 1
-        self._pool.append(job)
-        self._bytes += need
+        self._bytes += need  # held from now; the job is made on the thread pool
         self.stats.peak_bytes = max(self.stats.peak_bytes, self._bytes)
-        return job
+        return capacity, need, evicted
 
     def _admit(self) -> None:
         # one batch per depth: a batch completes as a whole, so shallow requests are not
@@ -260,23 +276,31 @@ class ReconstructionServer:
             if job is None:
                 break  # memory is full: wait for a completion (FIFO admission)
             self._waiting.pop(0)
-            self._busy.add(id(job))
+            if not isinstance(job, tuple):
+                self._busy.add(id(job))
             admitted.append((r, job))
         if admitted:
-            # uploads on the thread pool (each one a host copy that releases the GIL)
+            # job creation and uploads on the thread pool (an upload is a host copy that
+            # releases the GIL)
             now = time.perf_counter()
             cohort = []
             for r, job in admitted:
                 r.admitted = now
-                cohort.append((r, job, self._threads.submit(self._upload, r, job)))
+                cohort.append((r, self._threads.submit(self._upload, r, job)))
             self._cohorts.append(cohort)
         # stage the ticks whose uploads are all done (a tick's requests stay together), in
         # admission order
-        while self._cohorts and all(fut.done() for _, _, fut in self._cohorts[0]):
-            for r, job, fut in self._cohorts.pop(0):
-                err = fut.result()
+        while self._cohorts and all(fut.done() for _, fut in self._cohorts[0]):
+            for r, fut in self._cohorts.pop(0):
+                job, err = fut.result()
+                if job is not None and job not in self._pool:
+                    self._pool.append(job)
+                    self._busy.add(id(job))
                 if err is not None:
-                    self._fail(_Flight([r], [job]), err)
+                    if job is None:  # its creation failed: give back the bytes it held
+                        self._bytes -= Job.footprint(r.depth, job_capacity(len(r.points)),
+                                                     r.precision)
+                    self._fail(_Flight([r], [job] if job is not None else []), err)
                 else:
                     self._staged.setdefault(r.depth, []).append((r, job))
         # launch full batches, and the rest once the device runs short of work or they
@@ -299,13 +323,21 @@ class ReconstructionServer:
                 del self._staged[depth]
         self.stats.peak_inflight = max(self.stats.peak_inflight, self.inflight)
 
-    @staticmethod
-    def _upload(r: Request, job) -> Exception | None:
+    def _upload(self, r: Request, job):
+        """Runs on the thread pool: makes the job first when `job` is (capacity, footprint,
+        evicted), after closing the evicted ones, then uploads the request's rows."""
+        if isinstance(job, tuple):
+            capacity, _, evicted = job
+            self._close_all(evicted)
+            try:
+                job = self._job_factory(r.depth, capacity, r.precision)
+            except Exception as e:  # noqa: BLE001 - reported per request
+                return None, e
         try:
             job.set_points(r.points, r.normals)
-            return None
+            return job, None
         except Exception as e:  # noqa: BLE001 - reported per request
-            return e
+            return job, e
 
     def _batch_for(self, jobs: list):
         key = tuple(id(j) for j in jobs)

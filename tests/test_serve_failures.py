@@ -172,3 +172,61 @@ def test_a_failed_upload_is_isolated_and_the_rest_of_its_tick_runs():
     assert made[0].closed and not any(j.closed for j in made[1:])
     assert FakeBatch.made == 1 and server.stats.batches == 1  # the other two, as one batch
     server.close()
+
+
+def test_a_failed_job_creation_is_isolated_and_its_bytes_are_returned():
+    # new pool jobs are made on the thread pool; one whose creation fails is served by the
+    # fallback, the bytes it held are given back, and the rest of its tick runs
+    FakeBatch.made = 0
+    FakeBatch.fail_at = {}
+    made = []
+
+    def job_factory(d, c, p):
+        if not made:
+            made.append(None)
+            raise _lib.DeviceError("out of memory")
+        made.append(FakeJob(d, c, p))
+        return made[-1]
+
+    server = ReconstructionServer(hbm_budget=1 << 50, max_inflight=6, max_batch=3,
+                                  job_factory=job_factory, batch_factory=FakeBatch,
+                                  fallback=fallback_mesh)
+    reqs = [server.submit(*cloud(), depth=6) for _ in range(3)]
+    done = server.drain()
+    assert {r.id for r in done} == {r.id for r in reqs}
+    assert sum(r.fallback for r in done) == 1 and server.stats.fallbacks == 1
+    assert server.stats.batches == 1 and len(made) == 3
+    pooled = sum(server._footprint(j) for j in made[1:])
+    assert server._bytes == pooled  # the failed creation's reservation was returned
+    server.close()
+    assert all(j.closed for j in made[1:])
+
+
+def test_evicted_jobs_are_closed_off_the_loop_and_replaced():
+    # a budget of two depth-6 jobs: depth-7 requests evict the idle depth-6 jobs, whose
+    # close runs on the thread pool before the new jobs are made
+    FakeBatch.made = 0
+    FakeBatch.fail_at = {}
+    made = []
+
+    def job_factory(d, c, p):
+        made.append(FakeJob(d, c, p))
+        return made[-1]
+
+    from mojo_poissonrecon.core import Job
+    from mojo_poissonrecon.serve import job_capacity
+    fp7 = Job.footprint(7, job_capacity(100), _lib.PRECISION_MIXED)
+    server = ReconstructionServer(hbm_budget=fp7, max_inflight=6, max_batch=3,
+                                  job_factory=job_factory, batch_factory=FakeBatch,
+                                  fallback=fallback_mesh)
+    for _ in range(2):
+        server.submit(*cloud(), depth=6)
+    assert len(server.drain()) == 2
+    six = list(made)
+    assert len(six) == 2 and not any(j.closed for j in six)
+    r = server.submit(*cloud(), depth=7)
+    done = server.drain()
+    assert [x.id for x in done] == [r.id] and not done[0].fallback
+    assert all(j.closed for j in six) and len(made) == 3 and made[2].depth == 7
+    assert server._bytes == fp7
+    server.close()
