@@ -494,33 +494,42 @@ __device__ __forceinline__ uint32_t even_bits(uint64_t x) {
 __device__ __forceinline__ void coarse_support_body(Blk blk_, const uint32_t* fine, int nf,
                                                     int nc, uint32_t cw, uint32_t* out,
                                                     uint32_t* list, uint32_t* count) {
+  __shared__ uint32_t lds4[4];
+  __shared__ uint32_t block_base;
   const uint64_t q = (uint64_t)blk_.id * kThreads + threadIdx.x;
   const uint64_t total = (uint64_t)nc * nc * cw;
-  if (q >= total) return;
-  const uint32_t w = (uint32_t)(q % cw);
-  const uint64_t zy = q / cw;
-  const int Z = (int)(zy / (uint64_t)nc), Y = (int)(zy % (uint64_t)nc);
-  const uint64_t words = ((uint64_t)nf * nf * nf + 31) / 32;
   uint32_t acc = 0;
-  for (int oz = -1; oz <= 1; ++oz) {
-    const int zz = 2 * Z + oz;
-    if (zz < 0 || zz >= nf) continue;
-    for (int oy = -1; oy <= 1; ++oy) {
-      const int yy = 2 * Y + oy;
-      if (yy < 0 || yy >= nf) continue;
-      const uint64_t base = ((uint64_t)zz * nf + yy) * nf, x0 = 64ull * w;
-      if (x0 >= (uint64_t)nf) continue;
-      const uint64_t v = bits64_at(fine, base + x0, base + nf, words);
-      uint32_t before = 0;  // fine x = 64 w - 1, the x - 1 neighbour of coarse x = 32 w
-      if (w > 0) before = (fine[(base + x0 - 1) >> 5] >> ((base + x0 - 1) & 31u)) & 1u;
-      const uint32_t e = even_bits(v), o = even_bits(v >> 1);
-      acc |= e | o | (o << 1) | before;
+  if (q < total) {
+    const uint32_t w = (uint32_t)(q % cw);
+    const uint64_t zy = q / cw;
+    const int Z = (int)(zy / (uint64_t)nc), Y = (int)(zy % (uint64_t)nc);
+    const uint64_t words = ((uint64_t)nf * nf * nf + 31) / 32;
+    for (int oz = -1; oz <= 1; ++oz) {
+      const int zz = 2 * Z + oz;
+      if (zz < 0 || zz >= nf) continue;
+      for (int oy = -1; oy <= 1; ++oy) {
+        const int yy = 2 * Y + oy;
+        if (yy < 0 || yy >= nf) continue;
+        const uint64_t base = ((uint64_t)zz * nf + yy) * nf, x0 = 64ull * w;
+        if (x0 >= (uint64_t)nf) continue;
+        const uint64_t v = bits64_at(fine, base + x0, base + nf, words);
+        uint32_t before = 0;  // fine x = 64 w - 1, the x - 1 neighbour of coarse x = 32 w
+        if (w > 0) before = (fine[(base + x0 - 1) >> 5] >> ((base + x0 - 1) & 31u)) & 1u;
+        const uint32_t e = even_bits(v), o = even_bits(v >> 1);
+        acc |= e | o | (o << 1) | before;
+      }
     }
+    const uint32_t x_first = 32u * w;
+    if ((uint32_t)nc - x_first < 32u) acc &= (1u << ((uint32_t)nc - x_first)) - 1u;
+    out[q] = acc;
   }
-  const uint32_t x_first = 32u * w;
-  if ((uint32_t)nc - x_first < 32u) acc &= (1u << ((uint32_t)nc - x_first)) - 1u;
-  out[q] = acc;
-  if (acc) list[atomicAdd(count, 1u)] = (uint32_t)q;  // list order is free: nodes are independent
+  // list order is free (nodes are independent): one reservation per block, since every
+  // reservation of a job goes to one counter and its atomics serialise in the L2
+  uint32_t listed;
+  const uint32_t at = block_prefix256(acc != 0u, lds4, listed);
+  if (threadIdx.x == 0 && listed) block_base = atomicAdd(count, listed);
+  __syncthreads();
+  if (acc) list[block_base + at] = (uint32_t)q;
 }
 
 // restrict_nodes_body over the coarse nodes set in `support` (coarse_support_body of the
