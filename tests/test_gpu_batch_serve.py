@@ -192,6 +192,23 @@ def test_server_results_equal_solo_runs():
         assert r.latency > 0
 
 
+def test_server_judges_sampled_results_against_fp64():
+    # verify_every: the judge (plain FP64 reconstruct) re-solves every 2nd result on the
+    # host threads; the mixed meshes lie within 1e-4 of the diagonal of the FP64 ones
+    rng = np.random.default_rng(3)
+    server = ReconstructionServer(max_inflight=6, max_batch=3, precision="mixed",
+                                  verify_every=2)
+    cs = clouds(6)
+    reqs = [server.submit(p, n, depth=int(rng.choice([5, 6, 7]))) for p, n in cs]
+    done = server.drain()
+    errors = {r.id: r.fp64_error.result() for r in reqs if r.fp64_error is not None}
+    server.close()
+    assert len(done) == 6 and server.stats.verified == 3
+    assert sorted(errors) == [r.id for r in reqs if r.id % 2 == 0]
+    for e in errors.values():
+        assert 0.0 <= e["chamfer"] <= e["hausdorff"] <= 1e-4
+
+
 def test_server_admission_respects_memory_budget():
     p, n = shapes.sphere(3000, 1)
     fp = Job.footprint(6, job_capacity(len(p)), 1)
