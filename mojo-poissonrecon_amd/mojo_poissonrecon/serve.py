@@ -24,6 +24,7 @@ deterministic and an instance's arithmetic does not depend on its batch).
 
 from __future__ import annotations
 
+import functools
 import itertools
 import time
 from collections import OrderedDict
@@ -87,10 +88,24 @@ class ServerStats:
     host_collect: float = 0.0
 
 
-def job_capacity(points: int) -> int:
-    """Point capacity of a pooled job: the next power of two, at least 16384, so that a
-    pool job serves later clouds of similar size instead of only the one it was made for."""
-    return max(16384, 1 << max(0, int(points) - 1).bit_length())
+def job_capacity(points: int, depth: int | None = None,
+                 precision: int = _lib.PRECISION_MIXED) -> int:
+    """Point capacity of a pooled job: the next power of two, at least 16384, so that a pool
+    job serves later clouds of similar size instead of only the one it was made for. With
+    `depth`, raised further while the job's HBM footprint stays within 3 % of the depth's
+    smallest job: on deep grids the point buffers are a small part of it (a depth-9 job for
+    512k points is 1.6 % larger than one for 16k), and one capacity class lets every pooled
+    job of that depth serve every such request instead of a new job being made."""
+    c = max(16384, 1 << max(0, int(points) - 1).bit_length())
+    return c if depth is None else _grown_capacity(c, int(depth), int(precision))
+
+
+@functools.lru_cache(maxsize=None)
+def _grown_capacity(c: int, depth: int, precision: int) -> int:
+    limit = 1.03 * Job.footprint(depth, 16384, precision)
+    while c < (1 << 24) and Job.footprint(depth, 2 * c, precision) <= limit:
+        c *= 2
+    return c
 
 
 def _fp64_fallback(r: Request) -> Mesh:
@@ -268,7 +283,7 @@ class ReconstructionServer:
                     best = j
         if best is not None:
             return best
-        capacity = job_capacity(len(r.points))
+        capacity = job_capacity(len(r.points), r.depth, r.precision)
         need = Job.footprint(r.depth, capacity, r.precision)
         # evict idle jobs (largest first) until the new one fits the budget
         idle = [j for j in self._pool if id(j) not in self._busy]
@@ -316,8 +331,9 @@ class ReconstructionServer:
                     self._busy.add(id(job))
                 if err is not None:
                     if job is None:  # its creation failed: give back the bytes it held
-                        self._bytes -= Job.footprint(r.depth, job_capacity(len(r.points)),
-                                                     r.precision)
+                        self._bytes -= Job.footprint(
+                            r.depth, job_capacity(len(r.points), r.depth, r.precision),
+                            r.precision)
                     self._fail(_Flight([r], [job] if job is not None else []), err)
                 else:
                     self._staged.setdefault(r.depth, []).append((r, job))

@@ -215,7 +215,7 @@ def test_evicted_jobs_are_closed_off_the_loop_and_replaced():
 
     from mojo_poissonrecon.core import Job
     from mojo_poissonrecon.serve import job_capacity
-    fp7 = Job.footprint(7, job_capacity(100), _lib.PRECISION_MIXED)
+    fp7 = Job.footprint(7, job_capacity(100, 7, _lib.PRECISION_MIXED), _lib.PRECISION_MIXED)
     server = ReconstructionServer(hbm_budget=fp7, max_inflight=6, max_batch=3,
                                   job_factory=job_factory, batch_factory=FakeBatch,
                                   fallback=fallback_mesh)
@@ -266,3 +266,22 @@ def test_verify_every_sends_every_kth_result_to_the_judge():
     assert sorted(judged) == [r.id for r in reqs if r.id % 2 == 0]
     with pytest.raises(ValueError):
         ReconstructionServer(hbm_budget=1 << 50, verify_every=-1)
+
+
+def test_pooled_capacity_grows_while_the_footprint_barely_does():
+    from mojo_poissonrecon.core import Job
+    from mojo_poissonrecon.serve import job_capacity
+    mixed = _lib.PRECISION_MIXED
+    assert job_capacity(5000) == 16384 and job_capacity(300_000) == 1 << 19
+    for depth in (5, 6, 7, 8, 9):
+        floor = Job.footprint(depth, 16384, mixed)
+        for points in (100, 20_000, 300_000, 2_000_000):
+            c = job_capacity(points, depth, mixed)
+            assert c >= job_capacity(points) and c >= points and c & (c - 1) == 0
+            grown = c > job_capacity(points)
+            assert not grown or Job.footprint(depth, c, mixed) <= 1.03 * floor
+            if c < 1 << 24:  # the next class would pass the 3 % bound
+                assert Job.footprint(depth, 2 * c, mixed) > 1.03 * floor
+    # a depth-9 job serves every cloud of up to 512k points; shallow jobs stay small
+    assert job_capacity(5000, 9, mixed) >= 1 << 19
+    assert job_capacity(5000, 6, mixed) == 16384
