@@ -54,7 +54,7 @@ class Request:
     error: str | None = None   # set when the request failed, fallback included
     fallback: bool = False     # served by the FP64 single-request path after a failure
     # with the server's verify_every: resolves to {"hausdorff", "chamfer"}, the exact
-    # distances between this mesh and the judge's (plain FP64) mesh over the bbox diagonal
+    # distances between this mesh and the verifier's (plain FP64) mesh over the bbox diagonal
     fp64_error: Future | None = None
 
     @property
@@ -80,7 +80,7 @@ class ServerStats:
     peak_inflight: int = 0
     peak_bytes: int = 0
     jobs_created: int = 0
-    verified: int = 0          # results sent to the judge (verify_every)
+    verified: int = 0          # results sent to the verifier (verify_every)
     latencies: list = field(default_factory=list)
     # host seconds spent in each part of the loop (validation, upload + launch, readback)
     host_prepare: float = 0.0
@@ -123,12 +123,12 @@ class ReconstructionServer:
     ``fallback(request) -> Mesh`` default to the device Job, Batch and the FP64 solo
     reconstruct(); they can be replaced (tests use stand-ins to force failures).
 
-    ``verify_every=k`` (0 = off) sends every k-th completed request (by id) to the judge,
-    ``judge(request) -> Mesh`` (default: the same plain FP64 reconstruct), on the host
+    ``verify_every=k`` (0 = off) sends every k-th completed request (by id) to the verifier,
+    ``verifier(request) -> Mesh`` (default: the same plain FP64 reconstruct), on the host
     thread pool, and sets ``request.fp64_error`` to a future of the exact Hausdorff and
     Chamfer distances between the two meshes over the bbox diagonal of the request's
     finite rows (``distance(a, b)``, default ``metrics.mesh_distance``). The request
-    completes without waiting for it; the judge's solves share the device with the
+    completes without waiting for it; the verifier's solves share the device with the
     stream.
     """
 
@@ -136,12 +136,12 @@ class ReconstructionServer:
                  max_batch: int = 8, precision: str = "mixed", host_threads: int = 8,
                  batch_cache: int = 8, job_factory=None, batch_factory=None,
                  fallback=_fp64_fallback, linger: float = 0.004, min_active: int = 2,
-                 verify_every: int = 0, judge=None, distance=None):
+                 verify_every: int = 0, verifier=None, distance=None):
         if not isinstance(verify_every, int) or verify_every < 0:
             raise ValueError("verify_every must be a non-negative int")
         self.precision = precision
         self.verify_every = verify_every
-        self._judge = judge or _fp64_fallback
+        self._verifier = verifier or _fp64_fallback
         self._distance = distance
         self.max_inflight = max_inflight
         self.max_batch = max_batch
@@ -418,11 +418,11 @@ class ReconstructionServer:
                 self._flights.append(flight)
 
     def _verify(self, r: Request) -> dict:
-        """Runs on the thread pool: the judge's mesh for `r` and the distance to r's mesh."""
+        """Runs on the thread pool: the verifier's mesh for `r` and the distance to r's mesh."""
         distance = self._distance
         if distance is None:
             from .metrics import mesh_distance as distance
-        ref = self._judge(r)
+        ref = self._verifier(r)
         p = r.points[np.isfinite(r.points).all(axis=1)]
         diag = float(np.linalg.norm(p.max(axis=0) - p.min(axis=0))) if len(p) else 0.0
         d = distance(r.mesh, ref)

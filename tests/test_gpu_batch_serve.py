@@ -192,8 +192,8 @@ def test_server_results_equal_solo_runs():
         assert r.latency > 0
 
 
-def test_server_judges_sampled_results_against_fp64():
-    # verify_every: the judge (plain FP64 reconstruct) re-solves every 2nd result on the
+def test_server_verifies_sampled_results_against_fp64():
+    # verify_every: the verifier (plain FP64 reconstruct) re-solves every 2nd result on the
     # host threads; the mixed meshes lie within 1e-4 of the diagonal of the FP64 ones
     rng = np.random.default_rng(3)
     server = ReconstructionServer(max_inflight=6, max_batch=3, precision="mixed",

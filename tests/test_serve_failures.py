@@ -232,16 +232,16 @@ def test_evicted_jobs_are_closed_off_the_loop_and_replaced():
     server.close()
 
 
-def test_verify_every_sends_every_kth_result_to_the_judge():
-    # the judge (the FP64 solo path by default) re-solves every k-th completed request on
+def test_verify_every_sends_every_kth_result_to_the_verifier():
+    # the verifier (the FP64 solo path by default) re-solves every k-th completed request on
     # the thread pool; its future gives the distances over the bbox diagonal
     from types import SimpleNamespace
     FakeBatch.made = 0
     FakeBatch.fail_at = {}
-    judged = []
+    verified = []
 
-    def judge(r):
-        judged.append(r.id)
+    def verifier(r):
+        verified.append(r.id)
         return Mesh(np.zeros((3, 3)), np.array([[0, 1, 2]]), np.zeros(3), 0.5, 0.0)
 
     def distance(a, b):
@@ -250,7 +250,7 @@ def test_verify_every_sends_every_kth_result_to_the_judge():
     server = ReconstructionServer(hbm_budget=1 << 50, max_inflight=6, max_batch=3,
                                   job_factory=lambda d, c, p: FakeJob(d, c, p),
                                   batch_factory=FakeBatch, fallback=fallback_mesh,
-                                  verify_every=2, judge=judge, distance=distance)
+                                  verify_every=2, verifier=verifier, distance=distance)
     points = np.array([[0.0, 0.0, 0.0], [3.0, 4.0, 0.0], [np.nan, 0.0, 0.0]] * 10)
     normals = np.ones_like(points)
     reqs = [server.submit(points, normals, depth=6) for _ in range(5)]
@@ -263,7 +263,7 @@ def test_verify_every_sends_every_kth_result_to_the_judge():
             assert err == {"hausdorff": 2.0 / 5.0, "chamfer": 1.0 / 5.0}  # finite rows: diag 5
         else:
             assert r.fp64_error is None
-    assert sorted(judged) == [r.id for r in reqs if r.id % 2 == 0]
+    assert sorted(verified) == [r.id for r in reqs if r.id % 2 == 0]
     with pytest.raises(ValueError):
         ReconstructionServer(hbm_budget=1 << 50, verify_every=-1)
 
